@@ -1,0 +1,179 @@
+"""Python view of the atlas_rt C ABI (include/atlas_rt.h).
+
+This module is plumbing for tests and benchmarks: it loads the in-tree
+libatlas_rt.so, mirrors the engine-facing objects (a BLAS is the engine's
+Volume::BVH + packed MeshData arrays, src/engine/volume/BVH.h:114-138,
+src/engine/mesh/MeshData.cpp:172-269) and passes torch device buffers to the
+HIP kernels.  All compute happens in the library; there is no Python or CPU
+fallback, and every call raises AtlasRTError when the library reports a
+non-OK status or is missing.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _abi
+from ._abi import (AABB_DTYPE, BVH_TRI_DTYPE, GPU_TRI_DTYPE, HIT_DTYPE, INSTANCE_DTYPE, NODE_DTYPE,
+                   PACKED_RAY_DTYPE, RAY_DTYPE)
+
+__all__ = [
+    "AtlasRTError", "lib", "lib_path", "Blas", "NODE_DTYPE", "GPU_TRI_DTYPE", "INSTANCE_DTYPE",
+    "PACKED_RAY_DTYPE", "BVH_TRI_DTYPE", "RAY_DTYPE", "HIT_DTYPE", "AABB_DTYPE",
+]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def lib_path():
+    return os.path.join(os.path.dirname(_HERE), "lib", "libatlas_rt.so")
+
+
+class AtlasRTError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"atlas_rt status {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    """Load libatlas_rt.so and bind every prototype; raises if it is not built."""
+    global _LIB
+    if _LIB is None:
+        path = lib_path()
+        if not os.path.exists(path):
+            raise AtlasRTError(-1, f"{path} is not built (run __graft_entry__.build())")
+        handle = C.CDLL(path)
+        for name, (res, args) in _abi.PROTOTYPES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = handle
+    return _LIB
+
+
+def check(st):
+    if st != _abi.OK:
+        raise AtlasRTError(st, lib().atlas_rt_last_error().decode())
+
+
+def _ptr(a):
+    """Device pointer of a torch tensor, host pointer of a numpy array, or None."""
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        assert a.flags["C_CONTIGUOUS"]
+        return a.ctypes.data
+    if isinstance(a, int):
+        return a
+    assert a.is_contiguous()
+    return a.data_ptr()
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        import torch
+        return torch.cuda.current_stream().cuda_stream
+    if isinstance(stream, int):
+        return stream
+    return stream.cuda_stream
+
+
+def as_bvh_triangles(verts):
+    """(n, 9) float32 vertices -> BVH_TRI_DTYPE records with idx = row."""
+    verts = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1, 9)
+    out = np.zeros(len(verts), dtype=BVH_TRI_DTYPE)
+    out["v0"] = verts[:, 0:3]
+    out["v1"] = verts[:, 3:6]
+    out["v2"] = verts[:, 6:9]
+    out["idx"] = np.arange(len(verts), dtype=np.uint32)
+    return out
+
+
+class Blas:
+    """A bottom-level BVH living on the current HIP device."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle)
+
+    @classmethod
+    def build(cls, verts, aabbs=None, parallel=True, host_only=False):
+        tris = verts if (isinstance(verts, np.ndarray) and verts.dtype == BVH_TRI_DTYPE) else as_bvh_triangles(verts)
+        tris = np.ascontiguousarray(tris)
+        if aabbs is not None:
+            aabbs = np.ascontiguousarray(aabbs, dtype=np.float32).reshape(-1, 6)
+        flags = (_abi.BUILD_PARALLEL if parallel else 0) | (_abi.BUILD_HOST_ONLY if host_only else 0)
+        out = C.c_void_p()
+        check(lib().atlas_rt_blas_build(_ptr(aabbs), _ptr(tris), len(tris), flags, C.byref(out)))
+        return cls(out.value)
+
+    @classmethod
+    def create(cls, nodes, tris, orig_idx=None):
+        nodes = np.ascontiguousarray(nodes, dtype=NODE_DTYPE)
+        tris = np.ascontiguousarray(tris, dtype=GPU_TRI_DTYPE)
+        if orig_idx is not None:
+            orig_idx = np.ascontiguousarray(orig_idx, dtype=np.uint32)
+        out = C.c_void_p()
+        check(lib().atlas_rt_blas_create(_ptr(nodes) if len(nodes) else None, len(nodes), _ptr(tris),
+                                         _ptr(orig_idx), len(tris), C.byref(out)))
+        return cls(out.value)
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            check(lib().atlas_rt_blas_destroy(self._h))
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def info(self):
+        n, t, d = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        check(lib().atlas_rt_blas_info(self._h, C.byref(n), C.byref(t), C.byref(d)))
+        return n.value, t.value, d.value
+
+    def export(self):
+        n, t, _ = self.info()
+        nodes = np.zeros(n, dtype=NODE_DTYPE)
+        tris = np.zeros(t, dtype=GPU_TRI_DTYPE)
+        orig = np.zeros(t, dtype=np.uint32)
+        check(lib().atlas_rt_blas_export(self._h, _ptr(nodes) if n else None, _ptr(tris), _ptr(orig)))
+        return nodes, tris, orig
+
+    def trace_closest(self, d_rays, d_hits, d_slots=None, count=None, kernel=_abi.KERNEL_AUTO, stream=None):
+        """d_rays: device buffer of RAY_DTYPE records (e.g. float32 (n, 8)); d_hits: (n, 4) 32-bit."""
+        n = count if count is not None else d_rays.numel() // 8
+        check(lib().atlas_rt_trace_closest_rays(self._h, _ptr(d_rays), _ptr(d_hits), _ptr(d_slots), n, kernel,
+                                                _stream_ptr(stream)))
+
+    def trace_closest_stats(self, d_rays, d_hits, d_visits, d_slots=None, count=None, stream=None):
+        n = count if count is not None else d_rays.numel() // 8
+        check(lib().atlas_rt_trace_closest_stats(self._h, _ptr(d_rays), _ptr(d_hits), _ptr(d_slots),
+                                                 _ptr(d_visits), n, _stream_ptr(stream)))
+
+    def trace_any(self, d_rays, d_hit, count=None, kernel=_abi.KERNEL_AUTO, stream=None):
+        n = count if count is not None else d_rays.numel() // 8
+        check(lib().atlas_rt_trace_any_rays(self._h, _ptr(d_rays), _ptr(d_hit), n, kernel, _stream_ptr(stream)))
+
+    def trace_closest_host(self, rays):
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        hits = np.zeros(len(rays), dtype=HIT_DTYPE)
+        check(lib().atlas_rt_trace_closest_host(self._h, _ptr(rays), _ptr(hits), len(rays)))
+        return hits
+
+    def trace_any_host(self, rays):
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        hit = np.zeros(len(rays), dtype=np.uint32)
+        check(lib().atlas_rt_trace_any_host(self._h, _ptr(rays), _ptr(hit), len(rays)))
+        return hit
+
+    def last_kernel_ms(self):
+        ms = C.c_float()
+        check(lib().atlas_rt_last_kernel_ms(self._h, C.byref(ms)))
+        return ms.value

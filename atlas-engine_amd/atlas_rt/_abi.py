@@ -1,0 +1,81 @@
+"""ctypes view of include/atlas_rt.h (struct layouts, status codes, prototypes).
+
+numpy structured dtypes mirror the C structs byte for byte so host arrays can
+be passed straight through the C ABI.
+"""
+import ctypes as C
+
+import numpy as np
+
+OK = 0
+E_INVALID_ARG = 1
+E_SIZE_MISMATCH = 2
+E_EMPTY = 3
+E_HIP = 4
+E_STACK_OVERFLOW = 5
+E_OUT_OF_MEMORY = 6
+E_UNSUPPORTED = 7
+E_NO_DEVICE = 8
+
+BUILD_SERIAL = 0
+BUILD_PARALLEL = 1
+BUILD_HOST_ONLY = 2
+
+KERNEL_AUTO = 0
+KERNEL_BVH2_EXACT = 1
+KERNEL_BVH2_PERSISTENT = 2
+KERNEL_BVH4 = 3
+KERNEL_BVH8 = 4
+
+MASK_ALL = 1 << 7
+MASK_SHADOW = 1 << 6
+
+# GPUBVHNode (RTStructures.h:95-103), 64 B.
+NODE_DTYPE = np.dtype([
+    ("left_min", "<f4", 3), ("left_max", "<f4", 3),
+    ("right_min", "<f4", 3), ("right_max", "<f4", 3),
+    ("left_ptr", "<i4"), ("right_ptr", "<i4"), ("padding", "<i4", 2),
+])
+# GPUBVHTriangle (RTStructures.h:23-27), 48 B.
+GPU_TRI_DTYPE = np.dtype([("v0", "<f4", 4), ("v1", "<f4", 4), ("v2", "<f4", 4)])
+# GPUBVHInstance (RTStructures.h:85-93), 64 B.
+INSTANCE_DTYPE = np.dtype([
+    ("inverse_matrix", "<f4", (3, 4)), ("mesh_offset", "<i4"), ("material_offset", "<i4"),
+    ("next_instance", "<i4"), ("mask", "<u4"),
+])
+# PackedRay (structures.hsh:9-13), 48 B.
+PACKED_RAY_DTYPE = np.dtype([("origin", "<f4", 4), ("direction", "<f4", 4), ("hit", "<f4", 4)])
+# Volume::BVHTriangle input, 40 B.
+BVH_TRI_DTYPE = np.dtype([("v0", "<f4", 3), ("v1", "<f4", 3), ("v2", "<f4", 3), ("idx", "<u4")])
+# Compact ray (32 B) and hit (16 B).
+RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("tmin", "<f4"), ("direction", "<f4", 3), ("tmax", "<f4")])
+HIT_DTYPE = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("prim_id", "<i4")])
+AABB_DTYPE = np.dtype([("min", "<f4", 3), ("max", "<f4", 3)])
+
+assert NODE_DTYPE.itemsize == 64 and GPU_TRI_DTYPE.itemsize == 48
+assert INSTANCE_DTYPE.itemsize == 64 and PACKED_RAY_DTYPE.itemsize == 48
+assert BVH_TRI_DTYPE.itemsize == 40 and RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 16
+
+vp = C.c_void_p
+u32 = C.c_uint32
+i32 = C.c_int32
+f32 = C.c_float
+status = C.c_int32
+
+# name -> (restype, argtypes).  Every symbol include/atlas_rt.h declares.
+PROTOTYPES = {
+    "atlas_rt_last_error": (C.c_char_p, []),
+    "atlas_rt_version": (C.c_char_p, []),
+    "atlas_rt_device_count": (i32, []),
+    "atlas_rt_blas_build": (status, [vp, vp, u32, u32, C.POINTER(vp)]),
+    "atlas_rt_blas_create": (status, [vp, u32, vp, vp, u32, C.POINTER(vp)]),
+    "atlas_rt_blas_destroy": (status, [vp]),
+    "atlas_rt_blas_info": (status, [vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32)]),
+    "atlas_rt_blas_export": (status, [vp, vp, vp, vp]),
+    "atlas_rt_trace_closest_rays": (status, [vp, vp, vp, vp, u32, i32, vp]),
+    "atlas_rt_trace_closest_stats": (status, [vp, vp, vp, vp, vp, u32, vp]),
+    "atlas_rt_trace_any_rays": (status, [vp, vp, vp, u32, i32, vp]),
+    "atlas_rt_trace_closest_host": (status, [vp, vp, vp, u32]),
+    "atlas_rt_trace_any_host": (status, [vp, vp, vp, u32]),
+    "atlas_rt_last_kernel_ms": (status, [vp, C.POINTER(f32)]),
+}

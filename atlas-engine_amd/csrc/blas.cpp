@@ -1,0 +1,267 @@
+// BLAS objects: host build (builder.cpp), packing into the engine layout
+// (MeshData::BuildBVH, src/engine/mesh/MeshData.cpp:172-269) and device upload
+// (Mesh::BuildBVH, src/engine/mesh/Mesh.cpp:97-108), plus the small runtime
+// helpers shared by every handle (errors, scratch, timing events).
+#include <cmath>
+#include <cstring>
+#include <thread>
+
+#include "builder.h"
+#include "rt_hostmath.h"
+#include "rt_internal.h"
+
+namespace atlas_rt {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+const char* last_error() { return g_last_error.c_str(); }
+
+atlas_rt_status fail(atlas_rt_status code, const std::string& msg) {
+    set_error(msg);
+    return code;
+}
+
+atlas_rt_status hip_fail(hipError_t e, const char* what) {
+    set_error(std::string(what) + ": " + hipGetErrorString(e));
+    if (e == hipErrorOutOfMemory) return ATLAS_RT_E_OUT_OF_MEMORY;
+    if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return ATLAS_RT_E_NO_DEVICE;
+    return ATLAS_RT_E_HIP;
+}
+
+atlas_rt_status Scratch::ensure(size_t need) {
+    if (need <= bytes && ptr) return ATLAS_RT_OK;
+    release();
+    size_t sz = need < 256 ? 256 : need;
+    hipError_t e = hipMalloc(&ptr, sz);
+    if (e != hipSuccess) {
+        ptr = nullptr;
+        return hip_fail(e, "hipMalloc(scratch)");
+    }
+    bytes = sz;
+    hipGetDevice(&device);
+    return ATLAS_RT_OK;
+}
+
+void Scratch::release() {
+    if (ptr) hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
+atlas_rt_status LaunchTiming::begin(hipStream_t s) {
+    if (!start) {
+        ATLAS_HIP_TRY(hipEventCreate(&start));
+        ATLAS_HIP_TRY(hipEventCreate(&stop));
+    }
+    ATLAS_HIP_TRY(hipEventRecord(start, s));
+    return ATLAS_RT_OK;
+}
+
+atlas_rt_status LaunchTiming::end(hipStream_t s) {
+    ATLAS_HIP_TRY(hipEventRecord(stop, s));
+    pending = true;
+    return ATLAS_RT_OK;
+}
+
+atlas_rt_status LaunchTiming::read(float* ms) {
+    if (!pending) {
+        *ms = last_ms;
+        return ATLAS_RT_OK;
+    }
+    ATLAS_HIP_TRY(hipEventSynchronize(stop));
+    ATLAS_HIP_TRY(hipEventElapsedTime(&last_ms, start, stop));
+    pending = false;
+    *ms = last_ms;
+    return ATLAS_RT_OK;
+}
+
+void LaunchTiming::release() {
+    if (start) hipEventDestroy(start);
+    if (stop) hipEventDestroy(stop);
+    start = stop = nullptr;
+}
+
+atlas_rt_status upload_blas(atlas_rt_blas& b) {
+    int dev = 0;
+    ATLAS_HIP_TRY(hipGetDevice(&dev));
+    const size_t nn = b.nodes.size(), nt = b.tris.size();
+    // Kernel triangle layout: (v0, index bits), (v1 - v0, end flag), (v2 - v0, 0).
+    std::vector<float> packed(12 * nt);
+    for (size_t i = 0; i < nt; i++) {
+        const atlas_rt_gpu_bvh_triangle& t = b.tris[i];
+        float* o = &packed[12 * i];
+        const uint32_t idx = b.orig_idx.empty() ? uint32_t(i) : b.orig_idx[i];
+        o[0] = t.v0[0];
+        o[1] = t.v0[1];
+        o[2] = t.v0[2];
+        std::memcpy(&o[3], &idx, 4);
+        o[4] = t.v1[0] - t.v0[0];
+        o[5] = t.v1[1] - t.v0[1];
+        o[6] = t.v1[2] - t.v0[2];
+        o[7] = t.v0[3] > 0.0f ? 1.0f : -1.0f;
+        o[8] = t.v2[0] - t.v0[0];
+        o[9] = t.v2[1] - t.v0[1];
+        o[10] = t.v2[2] - t.v0[2];
+        o[11] = 0.0f;
+    }
+    void* dn = nullptr;
+    void* dt = nullptr;
+    if (nn) ATLAS_HIP_TRY(hipMalloc(&dn, nn * sizeof(atlas_rt_gpu_bvh_node)));
+    hipError_t e = hipMalloc(&dt, packed.size() * sizeof(float));
+    if (e != hipSuccess) {
+        if (dn) hipFree(dn);
+        return hip_fail(e, "hipMalloc(tris)");
+    }
+    if (nn) ATLAS_HIP_TRY(hipMemcpy(dn, b.nodes.data(), nn * sizeof(atlas_rt_gpu_bvh_node), hipMemcpyHostToDevice));
+    ATLAS_HIP_TRY(hipMemcpy(dt, packed.data(), packed.size() * sizeof(float), hipMemcpyHostToDevice));
+    b.d_nodes = reinterpret_cast<float4*>(dn);
+    b.d_tris = reinterpret_cast<float4*>(dt);
+    b.device = dev;
+    b.on_device = true;
+    return ATLAS_RT_OK;
+}
+
+int default_threads() {
+    unsigned h = std::thread::hardware_concurrency();
+    if (h == 0) h = 1;
+    if (h > 16) h = 16;
+    return int(h);
+}
+
+}  // namespace atlas_rt
+
+using namespace atlas_rt;
+
+static void pack_triangles(atlas_rt_blas& b, const atlas_rt_bvh_triangle* tris, const BuildOutput& out) {
+    const size_t n = out.ref_idx.size();
+    b.tris.resize(n);
+    b.orig_idx.resize(n);
+    for (size_t i = 0; i < n; i++) {
+        const atlas_rt_bvh_triangle& s = tris[out.ref_idx[i]];
+        atlas_rt_gpu_bvh_triangle& g = b.tris[i];
+        std::memcpy(g.v0, s.v0, 12);
+        std::memcpy(g.v1, s.v1, 12);
+        std::memcpy(g.v2, s.v2, 12);
+        g.v0[3] = out.end_of_node[i] ? 1.0f : -1.0f;  // MeshData.cpp:243
+        const int32_t material = 0;
+        std::memcpy(&g.v1[3], &material, 4);          // MeshData.cpp:244
+        g.v2[3] = 1.0f;                                // opacity, MeshData.cpp:245
+        b.orig_idx[i] = s.idx;
+    }
+}
+
+extern "C" {
+
+atlas_rt_status atlas_rt_blas_build(const atlas_rt_aabb* aabbs, const atlas_rt_bvh_triangle* tris, uint32_t count,
+                                    uint32_t flags, atlas_rt_blas** out) {
+    if (!out) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_blas_build: out is NULL");
+    *out = nullptr;
+    if (!tris) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_blas_build: tris is NULL");
+    if (count == 0) return fail(ATLAS_RT_E_EMPTY, "atlas_rt_blas_build: no triangles");
+    if (flags & ~(ATLAS_RT_BUILD_PARALLEL | ATLAS_RT_BUILD_HOST_ONLY))
+        return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_blas_build: unknown flag bits");
+    std::vector<float> verts(9 * size_t(count));
+    std::vector<float> boxes(6 * size_t(count));
+    for (uint32_t i = 0; i < count; i++) {
+        float* v = &verts[9 * size_t(i)];
+        std::memcpy(v, tris[i].v0, 12);
+        std::memcpy(v + 3, tris[i].v1, 12);
+        std::memcpy(v + 6, tris[i].v2, 12);
+        float* bx = &boxes[6 * size_t(i)];
+        if (aabbs) {
+            std::memcpy(bx, aabbs[i].min, 12);
+            std::memcpy(bx + 3, aabbs[i].max, 12);
+        } else {
+            // glm::min(glm::min(v0, v1), v2) / max (MeshData.cpp:143-146).
+            for (int k = 0; k < 3; k++) {
+                bx[k] = gl_min(gl_min(v[k], v[3 + k]), v[6 + k]);
+                bx[3 + k] = gl_max(gl_max(v[k], v[3 + k]), v[6 + k]);
+            }
+        }
+    }
+    BuildOutput bo;
+    const int threads = (flags & ATLAS_RT_BUILD_PARALLEL) ? default_threads() : 1;
+    if (!build_triangle_bvh(boxes.data(), verts.data(), count, threads, bo))
+        return fail(ATLAS_RT_E_EMPTY, "atlas_rt_blas_build: build failed");
+    auto* b = new atlas_rt_blas();
+    b->nodes = std::move(bo.nodes);
+    pack_triangles(*b, tris, bo);
+    b->max_depth = bo.max_depth;
+    if (!(flags & ATLAS_RT_BUILD_HOST_ONLY)) {
+        atlas_rt_status st = upload_blas(*b);
+        if (st != ATLAS_RT_OK) {
+            delete b;
+            return st;
+        }
+    }
+    *out = b;
+    return ATLAS_RT_OK;
+}
+
+atlas_rt_status atlas_rt_blas_create(const atlas_rt_gpu_bvh_node* nodes, uint32_t node_count,
+                                     const atlas_rt_gpu_bvh_triangle* tris, const uint32_t* orig_idx,
+                                     uint32_t tri_count, atlas_rt_blas** out) {
+    if (!out) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_blas_create: out is NULL");
+    *out = nullptr;
+    if (!tris || (node_count && !nodes)) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_blas_create: NULL array");
+    if (tri_count == 0) return fail(ATLAS_RT_E_EMPTY, "atlas_rt_blas_create: no triangles");
+    std::vector<uint8_t> eon(tri_count);
+    for (uint32_t i = 0; i < tri_count; i++) eon[i] = tris[i].v0[3] > 0.0f ? 1 : 0;
+    uint32_t depth = 0;
+    if (!validate_tree(nodes, node_count, tri_count, eon.data(), &depth))
+        return fail(ATLAS_RT_E_INVALID_ARG,
+                    "atlas_rt_blas_create: malformed tree (pointer out of range, not pre-order, or leaf without end flag)");
+    if (node_count == 0) {
+        uint32_t j = 0;
+        while (j < tri_count && !eon[j]) j++;
+        if (j >= tri_count) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_blas_create: root leaf without end flag");
+    }
+    auto* b = new atlas_rt_blas();
+    b->nodes.assign(nodes, nodes + node_count);
+    b->tris.assign(tris, tris + tri_count);
+    if (orig_idx) b->orig_idx.assign(orig_idx, orig_idx + tri_count);
+    else {
+        b->orig_idx.resize(tri_count);
+        for (uint32_t i = 0; i < tri_count; i++) b->orig_idx[i] = i;
+    }
+    b->max_depth = depth;
+    atlas_rt_status st = upload_blas(*b);
+    if (st != ATLAS_RT_OK) {
+        delete b;
+        return st;
+    }
+    *out = b;
+    return ATLAS_RT_OK;
+}
+
+atlas_rt_status atlas_rt_blas_destroy(atlas_rt_blas* b) {
+    if (!b) return ATLAS_RT_OK;
+    if (b->d_nodes) hipFree(b->d_nodes);
+    if (b->d_tris) hipFree(b->d_tris);
+    b->spill.release();
+    b->flags.release();
+    b->timing.release();
+    delete b;
+    return ATLAS_RT_OK;
+}
+
+atlas_rt_status atlas_rt_blas_info(const atlas_rt_blas* b, uint32_t* node_count, uint32_t* tri_count,
+                                   uint32_t* max_depth) {
+    if (!b) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_blas_info: NULL handle");
+    if (node_count) *node_count = uint32_t(b->nodes.size());
+    if (tri_count) *tri_count = uint32_t(b->tris.size());
+    if (max_depth) *max_depth = b->max_depth;
+    return ATLAS_RT_OK;
+}
+
+atlas_rt_status atlas_rt_blas_export(const atlas_rt_blas* b, atlas_rt_gpu_bvh_node* nodes,
+                                     atlas_rt_gpu_bvh_triangle* tris, uint32_t* orig_idx) {
+    if (!b) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_blas_export: NULL handle");
+    if (nodes && !b->nodes.empty()) std::memcpy(nodes, b->nodes.data(), b->nodes.size() * sizeof(*nodes));
+    if (tris) std::memcpy(tris, b->tris.data(), b->tris.size() * sizeof(*tris));
+    if (orig_idx) std::memcpy(orig_idx, b->orig_idx.data(), b->orig_idx.size() * sizeof(uint32_t));
+    return ATLAS_RT_OK;
+}
+
+}  // extern "C"

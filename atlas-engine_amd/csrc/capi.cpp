@@ -1,0 +1,151 @@
+// extern "C" trace entry points over one BLAS, and library-level queries.
+#include <cstring>
+
+#include "rt_internal.h"
+
+namespace atlas_rt {
+const char* last_error();
+}
+
+using namespace atlas_rt;
+
+static atlas_rt_status check_blas_ready(const atlas_rt_blas* b, const char* who) {
+    if (!b) return fail(ATLAS_RT_E_INVALID_ARG, std::string(who) + ": NULL BLAS handle");
+    if (!b->on_device) return fail(ATLAS_RT_E_INVALID_ARG, std::string(who) + ": BLAS was built host-only");
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev != b->device)
+        return fail(ATLAS_RT_E_INVALID_ARG, std::string(who) + ": BLAS lives on another device");
+    return ATLAS_RT_OK;
+}
+
+static atlas_rt_status check_kernel(int32_t kernel, const char* who) {
+    if (kernel == ATLAS_RT_KERNEL_AUTO || kernel == ATLAS_RT_KERNEL_BVH2_EXACT) return ATLAS_RT_OK;
+    return fail(ATLAS_RT_E_UNSUPPORTED, std::string(who) + ": kernel variant not available");
+}
+
+extern "C" {
+
+const char* atlas_rt_last_error(void) { return last_error(); }
+
+const char* atlas_rt_version(void) { return "atlas_rt 0.1.0 (gfx950)"; }
+
+int32_t atlas_rt_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+atlas_rt_status atlas_rt_trace_closest_rays(const atlas_rt_blas* blas, const atlas_rt_ray* d_rays,
+                                            atlas_rt_hit* d_hits, int32_t* d_slot, uint32_t count, int32_t kernel,
+                                            void* stream) {
+    atlas_rt_status st = check_blas_ready(blas, "atlas_rt_trace_closest_rays");
+    if (st != ATLAS_RT_OK) return st;
+    if (count && (!d_rays || !d_hits)) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_trace_closest_rays: NULL buffer");
+    st = check_kernel(kernel, "atlas_rt_trace_closest_rays");
+    if (st != ATLAS_RT_OK) return st;
+    TraceLaunch L;
+    L.rays = d_rays;
+    L.hits = d_hits;
+    L.slots = d_slot;
+    L.count = count;
+    L.kernel = kernel;
+    L.stream = reinterpret_cast<hipStream_t>(stream);
+    return launch_blas_closest(*const_cast<atlas_rt_blas*>(blas), L);
+}
+
+atlas_rt_status atlas_rt_trace_closest_stats(const atlas_rt_blas* blas, const atlas_rt_ray* d_rays,
+                                             atlas_rt_hit* d_hits, int32_t* d_slot, uint32_t* d_visits,
+                                             uint32_t count, void* stream) {
+    atlas_rt_status st = check_blas_ready(blas, "atlas_rt_trace_closest_stats");
+    if (st != ATLAS_RT_OK) return st;
+    if (count && (!d_rays || !d_hits || !d_visits))
+        return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_trace_closest_stats: NULL buffer");
+    TraceLaunch L;
+    L.rays = d_rays;
+    L.hits = d_hits;
+    L.slots = d_slot;
+    L.visits = d_visits;
+    L.count = count;
+    L.kernel = ATLAS_RT_KERNEL_BVH2_EXACT;
+    L.stream = reinterpret_cast<hipStream_t>(stream);
+    return launch_blas_closest(*const_cast<atlas_rt_blas*>(blas), L);
+}
+
+atlas_rt_status atlas_rt_trace_any_rays(const atlas_rt_blas* blas, const atlas_rt_ray* d_rays, uint32_t* d_hit,
+                                        uint32_t count, int32_t kernel, void* stream) {
+    atlas_rt_status st = check_blas_ready(blas, "atlas_rt_trace_any_rays");
+    if (st != ATLAS_RT_OK) return st;
+    if (count && (!d_rays || !d_hit)) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_trace_any_rays: NULL buffer");
+    st = check_kernel(kernel, "atlas_rt_trace_any_rays");
+    if (st != ATLAS_RT_OK) return st;
+    TraceLaunch L;
+    L.rays = d_rays;
+    L.hits = d_hit;
+    L.count = count;
+    L.kernel = kernel;
+    L.stream = reinterpret_cast<hipStream_t>(stream);
+    return launch_blas_any(*const_cast<atlas_rt_blas*>(blas), L);
+}
+
+atlas_rt_status atlas_rt_trace_closest_host(const atlas_rt_blas* blas, const atlas_rt_ray* rays, atlas_rt_hit* hits,
+                                            uint32_t count) {
+    atlas_rt_status st = check_blas_ready(blas, "atlas_rt_trace_closest_host");
+    if (st != ATLAS_RT_OK) return st;
+    if (count == 0) return ATLAS_RT_OK;
+    if (!rays || !hits) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_trace_closest_host: NULL buffer");
+    void *dr = nullptr, *dh = nullptr;
+    ATLAS_HIP_TRY(hipMalloc(&dr, size_t(count) * sizeof(atlas_rt_ray)));
+    hipError_t e = hipMalloc(&dh, size_t(count) * sizeof(atlas_rt_hit));
+    if (e != hipSuccess) {
+        hipFree(dr);
+        return hip_fail(e, "hipMalloc(hits)");
+    }
+    st = ATLAS_RT_OK;
+    e = hipMemcpy(dr, rays, size_t(count) * sizeof(atlas_rt_ray), hipMemcpyHostToDevice);
+    if (e != hipSuccess) st = hip_fail(e, "hipMemcpy(rays)");
+    if (st == ATLAS_RT_OK)
+        st = atlas_rt_trace_closest_rays(blas, reinterpret_cast<atlas_rt_ray*>(dr), reinterpret_cast<atlas_rt_hit*>(dh),
+                                         nullptr, count, ATLAS_RT_KERNEL_AUTO, nullptr);
+    if (st == ATLAS_RT_OK) {
+        e = hipMemcpy(hits, dh, size_t(count) * sizeof(atlas_rt_hit), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) st = hip_fail(e, "hipMemcpy(hits)");
+    }
+    hipFree(dr);
+    hipFree(dh);
+    return st;
+}
+
+atlas_rt_status atlas_rt_trace_any_host(const atlas_rt_blas* blas, const atlas_rt_ray* rays, uint32_t* hit,
+                                        uint32_t count) {
+    atlas_rt_status st = check_blas_ready(blas, "atlas_rt_trace_any_host");
+    if (st != ATLAS_RT_OK) return st;
+    if (count == 0) return ATLAS_RT_OK;
+    if (!rays || !hit) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_trace_any_host: NULL buffer");
+    void *dr = nullptr, *dh = nullptr;
+    ATLAS_HIP_TRY(hipMalloc(&dr, size_t(count) * sizeof(atlas_rt_ray)));
+    hipError_t e = hipMalloc(&dh, size_t(count) * sizeof(uint32_t));
+    if (e != hipSuccess) {
+        hipFree(dr);
+        return hip_fail(e, "hipMalloc(hit)");
+    }
+    st = ATLAS_RT_OK;
+    e = hipMemcpy(dr, rays, size_t(count) * sizeof(atlas_rt_ray), hipMemcpyHostToDevice);
+    if (e != hipSuccess) st = hip_fail(e, "hipMemcpy(rays)");
+    if (st == ATLAS_RT_OK)
+        st = atlas_rt_trace_any_rays(blas, reinterpret_cast<atlas_rt_ray*>(dr), reinterpret_cast<uint32_t*>(dh), count,
+                                     ATLAS_RT_KERNEL_AUTO, nullptr);
+    if (st == ATLAS_RT_OK) {
+        e = hipMemcpy(hit, dh, size_t(count) * sizeof(uint32_t), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) st = hip_fail(e, "hipMemcpy(hit)");
+    }
+    hipFree(dr);
+    hipFree(dh);
+    return st;
+}
+
+atlas_rt_status atlas_rt_last_kernel_ms(const atlas_rt_blas* blas, float* ms) {
+    if (!blas || !ms) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_last_kernel_ms: NULL argument");
+    return const_cast<atlas_rt_blas*>(blas)->timing.read(ms);
+}
+
+}  // extern "C"

@@ -1,0 +1,107 @@
+// Device-side intersection math for gfx950, bit-exact with the engine's shaders.
+//
+// Slab test: IntersectAABB (data/shader/raytracer/intersections.hsh:18-34)
+//   t0 = (bmin - o) / d, t1 = (bmax - o) / d, tsmall = min, tbig = max,
+//   tminf = max(max(tmin, ts.x), max(ts.y, ts.z)),
+//   tmaxf = min(min(tmax, tb.x), min(tb.y, tb.z)), hit iff tminf <= tmaxf,
+//   dist = hit ? tminf : tmax.
+// Triangle: IntersectTriangle (intersections.hsh:36-58)
+//   e0 = v1-v0, e1 = v2-v0, s = o-v0, p = s x e0, q = d x e1,
+//   (t,u,v) = (p.e1, q.s, p.d) / (q.e0); accept t,u,v >= 0 and u+v <= 1.
+//
+// Exactness:
+//  * Every float operation is written out in a fixed order, and FMA
+//    contraction is off (#pragma below + -ffp-contract=off), so products and
+//    sums round exactly as in the CPU oracle.  dot = (x*x' + y*y') + z*z'.
+//  * Division: a correctly rounded float quotient a/b equals
+//    (float)((double)a * (1.0 / (double)b)).  The double reciprocal and
+//    product each carry at most 2^-53 relative error, so the double quotient is
+//    within 2^-52 of a/b, while a/b of two floats lies at least 2^-50 (relative)
+//    from any float rounding midpoint; the final conversion therefore rounds
+//    to the same float as IEEE division, including infinities, NaN and
+//    subnormal results.  It costs cvt + v_mul_f64 + cvt (3 VALU ops) per
+//    quotient, against 11 for the IEEE f32 division sequence, and the
+//    reciprocals of the ray direction are computed once per ray.
+//  * min/max: GLSL leaves NaN behaviour undefined; this code uses IEEE
+//    minNum/maxNum (fminf/fmaxf -> v_min_f32/v_max_f32), the same rule the
+//    CPU oracle uses.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#pragma clang fp contract(off)
+
+namespace atlas_rt {
+
+struct DevRay {
+    float ox, oy, oz;
+    float dx, dy, dz;
+    double rx, ry, rz;  // 1 / d, correctly rounded double
+    float tmin;
+};
+
+__device__ __forceinline__ float qdiv(float a, double r) { return (float)((double)a * r); }
+
+__device__ __forceinline__ void make_dev_ray(DevRay& R, float ox, float oy, float oz, float dx, float dy, float dz,
+                                             float tmin) {
+    R.ox = ox;
+    R.oy = oy;
+    R.oz = oz;
+    R.dx = dx;
+    R.dy = dy;
+    R.dz = dz;
+    R.rx = 1.0 / (double)dx;
+    R.ry = 1.0 / (double)dy;
+    R.rz = 1.0 / (double)dz;
+    R.tmin = tmin;
+}
+
+// Slab test against one box; returns hit and writes the engine's `dist`.
+__device__ __forceinline__ bool slab_dist(const DevRay& R, float lx, float ly, float lz, float hx, float hy, float hz,
+                                          float tmax, float& dist) {
+    const float t0x = qdiv(lx - R.ox, R.rx);
+    const float t0y = qdiv(ly - R.oy, R.ry);
+    const float t0z = qdiv(lz - R.oz, R.rz);
+    const float t1x = qdiv(hx - R.ox, R.rx);
+    const float t1y = qdiv(hy - R.oy, R.ry);
+    const float t1z = qdiv(hz - R.oz, R.rz);
+    const float sx = fminf(t0x, t1x), sy = fminf(t0y, t1y), sz = fminf(t0z, t1z);
+    const float bx = fmaxf(t0x, t1x), by = fmaxf(t0y, t1y), bz = fmaxf(t0z, t1z);
+    const float tminf = fmaxf(fmaxf(R.tmin, sx), fmaxf(sy, sz));
+    const float tmaxf = fminf(fminf(tmax, bx), fminf(by, bz));
+    const bool hit = tminf <= tmaxf;
+    dist = hit ? tminf : tmax;
+    return hit;
+}
+
+// Triangle test on the precomputed layout (v0, e0 = v1 - v0, e1 = v2 - v0; the
+// subtractions are the shader's own first two operations, done once on the
+// host with the same rounding).  Writes (t,u,v); returns the engine's
+// geometric accept (t,u,v >= 0, u+v <= 1).
+__device__ __forceinline__ bool tri_uvt(const DevRay& R, float v0x, float v0y, float v0z, float e0x, float e0y,
+                                        float e0z, float e1x, float e1y, float e1z, float& t, float& u, float& v) {
+    const float sx = R.ox - v0x, sy = R.oy - v0y, sz = R.oz - v0z;
+    // p = cross(s, e0), q = cross(d, e1) in glm's component order.
+    const float px = sy * e0z - e0y * sz;
+    const float py = sz * e0x - e0z * sx;
+    const float pz = sx * e0y - e0x * sy;
+    const float qx = R.dy * e1z - e1y * R.dz;
+    const float qy = R.dz * e1x - e1z * R.dx;
+    const float qz = R.dx * e1y - e1x * R.dy;
+    const float den = (qx * e0x + qy * e0y) + qz * e0z;
+    const float a = (px * e1x + py * e1y) + pz * e1z;
+    const float b = (qx * sx + qy * sy) + qz * sz;
+    const float c = (px * R.dx + py * R.dy) + pz * R.dz;
+    const double rd = 1.0 / (double)den;
+    t = qdiv(a, rd);
+    u = qdiv(b, rd);
+    v = qdiv(c, rd);
+    return t >= 0.0f && u >= 0.0f && v >= 0.0f && u + v <= 1.0f;
+}
+
+__device__ __forceinline__ bool any_nan3(float x, float y, float z) {
+    return __builtin_isnan(x) || __builtin_isnan(y) || __builtin_isnan(z);
+}
+
+}  // namespace atlas_rt

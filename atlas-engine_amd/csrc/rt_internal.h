@@ -1,0 +1,106 @@
+// Internal host-side declarations shared by the C ABI, the BLAS/world objects
+// and the kernel launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/atlas_rt.h"
+
+namespace atlas_rt {
+
+// Thread-local last error, set by fail().
+void set_error(const std::string& msg);
+atlas_rt_status fail(atlas_rt_status code, const std::string& msg);
+atlas_rt_status hip_fail(hipError_t e, const char* what);
+
+#define ATLAS_HIP_TRY(expr)                                  \
+    do {                                                     \
+        hipError_t _e = (expr);                              \
+        if (_e != hipSuccess) return ::atlas_rt::hip_fail(_e, #expr); \
+    } while (0)
+
+// Device-side BLAS arrays.
+//  nodes: engine GPUBVHNode layout, 4 x float4 per node (data0..data3,
+//         structures.hsh:179-184).
+//  tris : 3 x float4 per BVH slot: (v0.xyz, caller index bits),
+//         (e0 = v1 - v0, end-of-leaf +1/-1), (e1 = v2 - v0, 0).
+struct DeviceBlas {
+    const float4* nodes = nullptr;
+    const float4* tris = nullptr;
+    uint32_t node_count = 0;
+    uint32_t tri_count = 0;
+    uint32_t max_depth = 0;
+};
+
+// Stack spill scratch, grown on demand and owned by a handle.
+struct Scratch {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    int device = -1;
+    atlas_rt_status ensure(size_t need);
+    void release();
+};
+
+struct LaunchTiming {
+    hipEvent_t start = nullptr, stop = nullptr;
+    bool pending = false;
+    float last_ms = 0.0f;
+    atlas_rt_status begin(hipStream_t s);
+    atlas_rt_status end(hipStream_t s);
+    atlas_rt_status read(float* ms);
+    void release();
+};
+
+}  // namespace atlas_rt
+
+struct atlas_rt_blas {
+    // Host copies in the engine's layout (what BVH::GetTree() and the packed
+    // MeshData arrays hold).
+    std::vector<atlas_rt_gpu_bvh_node> nodes;
+    std::vector<atlas_rt_gpu_bvh_triangle> tris;
+    std::vector<uint32_t> orig_idx;
+    uint32_t max_depth = 0;
+
+    int device = -1;
+    bool on_device = false;
+    float4* d_nodes = nullptr;
+    float4* d_tris = nullptr;
+    atlas_rt::Scratch spill;
+    atlas_rt::Scratch flags;
+    atlas_rt::LaunchTiming timing;
+
+    atlas_rt::DeviceBlas view() const {
+        atlas_rt::DeviceBlas v;
+        v.nodes = d_nodes;
+        v.tris = d_tris;
+        v.node_count = uint32_t(nodes.size());
+        v.tri_count = uint32_t(tris.size());
+        v.max_depth = max_depth;
+        return v;
+    }
+};
+
+namespace atlas_rt {
+
+// Kernel launchers (trace_blas.hip).
+struct TraceLaunch {
+    const void* rays = nullptr;  // atlas_rt_ray[count]
+    void* hits = nullptr;        // atlas_rt_hit[count] (closest) or uint32_t[count] (any)
+    int32_t* slots = nullptr;
+    uint32_t* visits = nullptr;  // optional uint2 per ray: (inner nodes fetched, triangles tested)
+    uint32_t count = 0;
+    int32_t kernel = ATLAS_RT_KERNEL_AUTO;
+    hipStream_t stream = nullptr;
+};
+
+atlas_rt_status launch_blas_closest(atlas_rt_blas& blas, const TraceLaunch& L);
+atlas_rt_status launch_blas_any(atlas_rt_blas& blas, const TraceLaunch& L);
+
+// Upload the engine-layout arrays of `blas` to the current device.
+atlas_rt_status upload_blas(atlas_rt_blas& blas);
+
+}  // namespace atlas_rt

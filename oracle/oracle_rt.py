@@ -1,0 +1,218 @@
+"""TEST INFRASTRUCTURE - the oracle (ctypes bindings of liboracle_rt.so).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+import this module, and only as the checker.  It restates the reference
+algorithms (src/engine/volume/BVH.cpp, Ray.cpp, AABB.cpp,
+data/shader/raytracer/bvh.hsh, intersections.hsh) on the CPU; see the C++
+sources in this directory for file:line citations.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+NODE_DTYPE = np.dtype([
+    ("left_min", "<f4", 3), ("left_max", "<f4", 3),
+    ("right_min", "<f4", 3), ("right_max", "<f4", 3),
+    ("left_ptr", "<i4"), ("right_ptr", "<i4"), ("padding", "<i4", 2),
+])
+GPU_TRI_DTYPE = np.dtype([("v0", "<f4", 4), ("v1", "<f4", 4), ("v2", "<f4", 4)])
+INSTANCE_DTYPE = np.dtype([
+    ("inverse_matrix", "<f4", (3, 4)), ("mesh_offset", "<i4"), ("material_offset", "<i4"),
+    ("next_instance", "<i4"), ("mask", "<u4"),
+])
+RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("tmin", "<f4"), ("direction", "<f4", 3), ("tmax", "<f4")])
+HIT_DTYPE = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("prim_id", "<i4")])
+
+MODE_RGPU, MODE_RCPU, MODE_BRUTE = 0, 1, 2
+
+
+def lib_path():
+    return os.path.join(_HERE, "liboracle_rt.so")
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(lib_path()):
+            raise RuntimeError(f"{lib_path()} is not built (make -C oracle)")
+        L = C.CDLL(lib_path())
+        vp, u32, i32, f32 = C.c_void_p, C.c_uint32, C.c_int32, C.c_float
+        L.oracle_bvh_build_tris.restype = vp
+        L.oracle_bvh_build_tris.argtypes = [vp, vp, u32]
+        L.oracle_bvh_build_aabbs.restype = vp
+        L.oracle_bvh_build_aabbs.argtypes = [vp, u32]
+        L.oracle_bvh_node_count.restype = u32
+        L.oracle_bvh_node_count.argtypes = [vp]
+        L.oracle_bvh_ref_count.restype = u32
+        L.oracle_bvh_ref_count.argtypes = [vp]
+        L.oracle_bvh_export.restype = None
+        L.oracle_bvh_export.argtypes = [vp, vp, vp, vp, vp, vp, i32, f32]
+        L.oracle_bvh_free.restype = None
+        L.oracle_bvh_free.argtypes = [vp]
+        L.oracle_bvh_depth.restype = u32
+        L.oracle_bvh_depth.argtypes = [vp, u32]
+        L.oracle_trace_closest.restype = None
+        L.oracle_trace_closest.argtypes = [vp, u32, vp, vp, u32, vp, u32, vp, vp, vp, vp, i32, i32]
+        L.oracle_trace_any.restype = None
+        L.oracle_trace_any.argtypes = [vp, u32, vp, u32, vp, u32, vp, vp, vp, i32, i32]
+        L.oracle_trace_world.restype = None
+        L.oracle_trace_world.argtypes = [vp, u32, vp, u32, vp, vp, vp, vp, vp, u32, vp, u32, u32, i32,
+                                         vp, vp, vp, vp, vp, vp, i32]
+        L.oracle_triangle_test.restype = i32
+        L.oracle_triangle_test.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.oracle_slab_gpu.restype = i32
+        L.oracle_slab_gpu.argtypes = [vp, vp, vp, vp, f32, f32, vp]
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def threads_default():
+    return os.cpu_count() or 1
+
+
+class OracleBVH:
+    """Result of the restated reference builder, packed like MeshData::BuildBVH."""
+
+    def __init__(self, nodes, tris, ref_idx, end_of_node):
+        self.nodes = nodes
+        self.tris = tris
+        self.ref_idx = ref_idx
+        self.end_of_node = end_of_node
+
+    @property
+    def depth(self):
+        return int(lib().oracle_bvh_depth(_p(self.nodes), len(self.nodes)))
+
+
+def build_triangles(verts, aabbs=None, material_idx=0, opacity=1.0):
+    verts = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1, 9)
+    if aabbs is not None:
+        aabbs = np.ascontiguousarray(aabbs, dtype=np.float32).reshape(-1, 6)
+    h = lib().oracle_bvh_build_tris(_p(aabbs), _p(verts), len(verts))
+    if not h:
+        raise ValueError("oracle build failed")
+    try:
+        n = lib().oracle_bvh_node_count(h)
+        r = lib().oracle_bvh_ref_count(h)
+        nodes = np.zeros(n, dtype=NODE_DTYPE)
+        ref = np.zeros(r, dtype=np.uint32)
+        eon = np.zeros(r, dtype=np.uint8)
+        tris = np.zeros(r, dtype=GPU_TRI_DTYPE)
+        lib().oracle_bvh_export(h, _p(nodes) if n else None, _p(ref), _p(eon), _p(verts), _p(tris),
+                                material_idx, opacity)
+    finally:
+        lib().oracle_bvh_free(h)
+    return OracleBVH(nodes, tris, ref, eon)
+
+
+def build_aabbs(aabbs):
+    aabbs = np.ascontiguousarray(aabbs, dtype=np.float32).reshape(-1, 6)
+    h = lib().oracle_bvh_build_aabbs(_p(aabbs), len(aabbs))
+    if not h:
+        raise ValueError("oracle build failed")
+    try:
+        n = lib().oracle_bvh_node_count(h)
+        r = lib().oracle_bvh_ref_count(h)
+        nodes = np.zeros(n, dtype=NODE_DTYPE)
+        ref = np.zeros(r, dtype=np.uint32)
+        eon = np.zeros(r, dtype=np.uint8)
+        lib().oracle_bvh_export(h, _p(nodes) if n else None, _p(ref), _p(eon), None, None, 0, 0.0)
+    finally:
+        lib().oracle_bvh_free(h)
+    return OracleBVH(nodes, None, ref, eon)
+
+
+def trace_closest(bvh_nodes, bvh_tris, orig_idx, rays, mode=MODE_RGPU, threads=None, stats=False):
+    """Closest hit on the packed arrays.  Returns (hits, slots[, node_visits, tri_tests])."""
+    rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+    n = len(rays)
+    hits = np.zeros(n, dtype=HIT_DTYPE)
+    slots = np.zeros(n, dtype=np.int32)
+    nv = np.zeros(n, dtype=np.uint32) if stats else None
+    tt = np.zeros(n, dtype=np.uint32) if stats else None
+    orig = None if orig_idx is None else np.ascontiguousarray(orig_idx, dtype=np.uint32)
+    lib().oracle_trace_closest(_p(bvh_nodes) if len(bvh_nodes) else None, len(bvh_nodes), _p(bvh_tris), _p(orig),
+                               len(bvh_tris), _p(rays), n, _p(hits), _p(slots), _p(nv), _p(tt), mode,
+                               threads or threads_default())
+    if stats:
+        return hits, slots, nv, tt
+    return hits, slots
+
+
+def trace_any(bvh_nodes, bvh_tris, rays, mode=MODE_RGPU, threads=None, stats=False):
+    rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+    n = len(rays)
+    hit = np.zeros(n, dtype=np.uint32)
+    nv = np.zeros(n, dtype=np.uint32) if stats else None
+    tt = np.zeros(n, dtype=np.uint32) if stats else None
+    lib().oracle_trace_any(_p(bvh_nodes) if len(bvh_nodes) else None, len(bvh_nodes), _p(bvh_tris), len(bvh_tris),
+                           _p(rays), n, _p(hit), _p(nv), _p(tt), mode, threads or threads_default())
+    if stats:
+        return hit, nv, tt
+    return hit
+
+
+def trace_world(tlas_nodes, instances, blases, rays, mask=1 << 7, any_hit=False, threads=None, stats=False):
+    """Two-level R-GPU trace.  blases: list of (nodes, tris, orig_idx) packed arrays."""
+    rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+    n = len(rays)
+    k = len(blases)
+    keep = []
+    node_ptrs = (C.c_void_p * k)()
+    tri_ptrs = (C.c_void_p * k)()
+    orig_ptrs = (C.c_void_p * k)()
+    node_counts = np.zeros(k, dtype=np.uint32)
+    tri_counts = np.zeros(k, dtype=np.uint32)
+    for i, (bn, bt, bo) in enumerate(blases):
+        bn = np.ascontiguousarray(bn, dtype=NODE_DTYPE)
+        bt = np.ascontiguousarray(bt, dtype=GPU_TRI_DTYPE)
+        bo = np.ascontiguousarray(bo, dtype=np.uint32)
+        keep += [bn, bt, bo]
+        node_ptrs[i] = bn.ctypes.data if len(bn) else None
+        tri_ptrs[i] = bt.ctypes.data
+        orig_ptrs[i] = bo.ctypes.data
+        node_counts[i] = len(bn)
+        tri_counts[i] = len(bt)
+    tlas_nodes = np.ascontiguousarray(tlas_nodes, dtype=NODE_DTYPE)
+    instances = np.ascontiguousarray(instances, dtype=INSTANCE_DTYPE)
+    hits = np.zeros(n, dtype=HIT_DTYPE)
+    slots = np.zeros(n, dtype=np.int32)
+    inst = np.zeros(n, dtype=np.int32)
+    nv = np.zeros(n, dtype=np.uint32) if stats else None
+    tt = np.zeros(n, dtype=np.uint32) if stats else None
+    iv = np.zeros(n, dtype=np.uint32) if stats else None
+    lib().oracle_trace_world(_p(tlas_nodes), len(tlas_nodes), _p(instances), len(instances),
+                             C.cast(node_ptrs, C.c_void_p), _p(node_counts), C.cast(tri_ptrs, C.c_void_p),
+                             C.cast(orig_ptrs, C.c_void_p), _p(tri_counts), k, _p(rays), n, mask,
+                             1 if any_hit else 0, _p(hits), _p(slots), _p(inst), _p(nv), _p(tt), _p(iv),
+                             threads or threads_default())
+    if stats:
+        return hits, slots, inst, nv, tt, iv
+    return hits, slots, inst
+
+
+def triangle_test(o, d, v0, v1, v2):
+    f = lambda x: np.ascontiguousarray(x, dtype=np.float32)
+    sol = np.zeros(3, dtype=np.float32)
+    h = lib().oracle_triangle_test(_p(f(o)), _p(f(d)), _p(f(v0)), _p(f(v1)), _p(f(v2)), _p(sol))
+    return bool(h), sol
+
+
+def slab_test(o, d, bmin, bmax, tmin, tmax):
+    f = lambda x: np.ascontiguousarray(x, dtype=np.float32)
+    dist = np.zeros(1, dtype=np.float32)
+    h = lib().oracle_slab_gpu(_p(f(o)), _p(f(d)), _p(f(bmin)), _p(f(bmax)), tmin, tmax, _p(dist))
+    return bool(h), float(dist[0])

@@ -1,0 +1,160 @@
+"""GPU parity of the single-BLAS traversal kernels against the oracle.
+
+The oracle restates HitClosest / HitAny (data/shader/raytracer/bvh.hsh:191-273,
+359-441) on the same packed arrays the kernels read, so the bar is bit
+equality: hit slot, caller index, t, u and v must match exactly.
+"""
+import numpy as np
+import pytest
+
+import atlas_rt
+import oracle_rt
+import rt_cases
+from atlas_rt import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _to_dev(torch, rays):
+    return torch.from_numpy(rays.view(np.float32).reshape(-1, 8).copy()).cuda()
+
+
+def _gpu_closest(torch, blas, rays, stats=False):
+    d_rays = _to_dev(torch, rays)
+    n = len(rays)
+    d_hits = torch.empty((n, 4), dtype=torch.float32, device="cuda")
+    d_slots = torch.empty(n, dtype=torch.int32, device="cuda")
+    if stats:
+        d_vis = torch.empty((n, 2), dtype=torch.int32, device="cuda")
+        blas.trace_closest_stats(d_rays, d_hits, d_vis, d_slots)
+    else:
+        blas.trace_closest(d_rays, d_hits, d_slots)
+    torch.cuda.synchronize()
+    hits = d_hits.cpu().numpy().view(atlas_rt.HIT_DTYPE).reshape(-1)
+    slots = d_slots.cpu().numpy()
+    if stats:
+        return hits, slots, d_vis.cpu().numpy().astype(np.uint32)
+    return hits, slots
+
+
+def _gpu_any(torch, blas, rays):
+    d_rays = _to_dev(torch, rays)
+    d_hit = torch.empty(len(rays), dtype=torch.int32, device="cuda")
+    blas.trace_any(d_rays, d_hit)
+    torch.cuda.synchronize()
+    return d_hit.cpu().numpy().astype(np.uint32)
+
+
+def _assert_same(h_gpu, s_gpu, h_ref, s_ref, what):
+    bad = np.nonzero(np.any(rt_cases.hit_bits(h_gpu) != rt_cases.hit_bits(h_ref), axis=1) | (s_gpu != s_ref))[0]
+    assert len(bad) == 0, (f"{what}: {len(bad)} rays differ, first {bad[:5]}: gpu {h_gpu[bad[:3]]} "
+                           f"slots {s_gpu[bad[:3]]} vs oracle {h_ref[bad[:3]]} slots {s_ref[bad[:3]]}")
+
+
+@pytest.mark.parametrize("n_tris,n_rays,salt", [(1, 4096, 1), (2, 4096, 2), (3, 4096, 3), (7, 8192, 4),
+                                                (256, 65536, 5), (4096, 65536, 6), (65536, 131072, 7)])
+def test_closest_bit_exact_random(torch_cuda, n_tris, n_rays, salt):
+    print(f"case closest n_tris={n_tris} n_rays={n_rays} salt={salt}", flush=True)
+    verts, rays = rt_cases.soup(n_tris, n_rays, salt)
+    blas = atlas_rt.Blas.build(verts)
+    nodes, tris, orig = blas.export()
+    h, s = _gpu_closest(torch_cuda, blas, rays)
+    hr, sr = oracle_rt.trace_closest(nodes, tris, orig, rays)
+    _assert_same(h, s, hr, sr, f"closest {n_tris}x{n_rays}")
+
+
+@pytest.mark.parametrize("n_tris,salt", [(4096, 11), (65536, 12)])
+def test_closest_edge_rays(torch_cuda, n_tris, salt):
+    print(f"case closest-edge n_tris={n_tris} salt={salt}", flush=True)
+    verts, _ = rt_cases.soup(n_tris, 16, salt)
+    rays = rt_cases.edge_rays(verts, salt=salt)
+    blas = atlas_rt.Blas.build(verts)
+    nodes, tris, orig = blas.export()
+    h, s = _gpu_closest(torch_cuda, blas, rays)
+    hr, sr = oracle_rt.trace_closest(nodes, tris, orig, rays)
+    _assert_same(h, s, hr, sr, "closest edge rays")
+    assert (h["prim_id"] >= 0).sum() > len(rays) // 4
+
+
+def test_closest_c0_full_scale(torch_cuda):
+    """Config 0 at full size (65,536 triangles, 1,048,576 rays): bit-exact."""
+    print("case closest c0 full", flush=True)
+    verts, rays = W.c0()
+    blas = atlas_rt.Blas.build(verts)
+    nodes, tris, orig = blas.export()
+    h, s, vis = _gpu_closest(torch_cuda, blas, rays, stats=True)
+    hr, sr, nv, tt = oracle_rt.trace_closest(nodes, tris, orig, rays, stats=True)
+    _assert_same(h, s, hr, sr, "c0 full")
+    # the kernel's own visit counters equal the oracle's
+    assert np.array_equal(vis[:, 0], nv) and np.array_equal(vis[:, 1], tt)
+
+
+def test_any_bit_exact(torch_cuda):
+    print("case any-hit", flush=True)
+    verts, rays = rt_cases.soup(16384, 65536, 21)
+    rays = np.concatenate([rays, rt_cases.edge_rays(verts, salt=21)])
+    blas = atlas_rt.Blas.build(verts)
+    nodes, tris, _ = blas.export()
+    got = _gpu_any(torch_cuda, blas, rays)
+    ref = oracle_rt.trace_any(nodes, tris, rays)
+    assert np.array_equal(got, ref)
+    # limited tmax: any-hit must agree with "closest t < tmax"
+    rays2 = rays.copy()
+    rays2["tmax"] = 2.0
+    got2 = _gpu_any(torch_cuda, blas, rays2)
+    assert np.array_equal(got2, oracle_rt.trace_any(nodes, tris, rays2))
+
+
+def test_stack_spill_deep_chain(torch_cuda):
+    """A tree 80 levels deep: the stack spills past the 32 LDS entries."""
+    print("case deep chain", flush=True)
+    nodes, tris, rays = rt_cases.chain_tree(80)
+    blas = atlas_rt.Blas.create(nodes, tris)
+    assert blas.info()[2] >= 80
+    h, s = _gpu_closest(torch_cuda, blas, rays)
+    hr, sr = oracle_rt.trace_closest(nodes, tris, np.arange(len(tris), dtype=np.uint32), rays)
+    _assert_same(h, s, hr, sr, "deep chain")
+    assert np.all(h["prim_id"] == 0)
+    np.testing.assert_array_equal(h["t"], np.float32(105.0))
+    assert np.array_equal(_gpu_any(torch_cuda, blas, rays), oracle_rt.trace_any(nodes, tris, rays))
+
+
+def test_imported_oracle_blas(torch_cuda):
+    """A BLAS built by the oracle and imported in the engine layout traces identically."""
+    print("case imported", flush=True)
+    verts, rays = rt_cases.soup(8192, 32768, 31)
+    ob = oracle_rt.build_triangles(verts)
+    blas = atlas_rt.Blas.create(ob.nodes, ob.tris, ob.ref_idx)
+    h, s = _gpu_closest(torch_cuda, blas, rays)
+    hr, sr = oracle_rt.trace_closest(ob.nodes, ob.tris, ob.ref_idx, rays)
+    _assert_same(h, s, hr, sr, "imported")
+
+
+def test_host_buffer_api(torch_cuda):
+    verts, rays = rt_cases.soup(2048, 8192, 41)
+    blas = atlas_rt.Blas.build(verts)
+    nodes, tris, orig = blas.export()
+    h = blas.trace_closest_host(rays)
+    hr, _ = oracle_rt.trace_closest(nodes, tris, orig, rays)
+    assert np.array_equal(rt_cases.hit_bits(h), rt_cases.hit_bits(hr))
+    assert np.array_equal(blas.trace_any_host(rays), oracle_rt.trace_any(nodes, tris, rays))
+
+
+def test_hits_match_brute_force_and_fp64(torch_cuda):
+    """Independent of the tree: the GPU closest t equals brute force over all
+    triangles, and (t, u, v) are within 1e-5 relative of an FP64 recompute
+    except for near-degenerate determinants, which are counted."""
+    verts, rays = rt_cases.soup(4096, 65536, 51)
+    blas = atlas_rt.Blas.build(verts)
+    nodes, tris, orig = blas.export()
+    h, s = _gpu_closest(torch_cuda, blas, rays)
+    hb, _ = oracle_rt.trace_closest(nodes, tris, orig, rays, mode=oracle_rt.MODE_BRUTE)
+    # equal t everywhere, so any id difference is an equal-t tie resolved by
+    # traversal order (brute force tests in slot order)
+    assert np.array_equal(h["t"].view(np.int32), hb["t"].view(np.int32))
+    import fp64
+    err = fp64.tuv_errors(verts, rays, h)
+    print("fp64 errors", err)
+    assert err["n_hits"] > 1000
+    # near-degenerate (grazing) hits may exceed 1e-5; they must stay rare
+    assert err["n_over"] <= max(2, err["n_hits"] * 1e-3), err
