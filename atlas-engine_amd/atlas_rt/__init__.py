@@ -177,3 +177,88 @@ class Blas:
         ms = C.c_float()
         check(lib().atlas_rt_last_kernel_ms(self._h, C.byref(ms)))
         return ms.value
+
+
+def instance_descs(object_to_world, blas_index, mask=_abi.MASK_ALL | _abi.MASK_SHADOW, material_offset=0):
+    """Instance descriptors from (n, 3, 4) or (n, 4, 4) object-to-world matrices.
+
+    The inverse is computed in float64 and rounded to float32, standing in for
+    TransformComponent::inverseGlobalMatrix.
+    """
+    m = np.asarray(object_to_world, dtype=np.float64)
+    n = len(m)
+    full = np.zeros((n, 4, 4))
+    full[:, :3, :] = m[:, :3, :]
+    full[:, 3, 3] = 1.0
+    inv = np.linalg.inv(full)
+    d = np.zeros(n, dtype=_abi.INSTANCE_DESC_DTYPE)
+    d["object_to_world"] = full[:, :3, :].astype(np.float32)
+    d["world_to_object"] = inv[:, :3, :].astype(np.float32)
+    d["blas"] = np.asarray(blas_index, dtype=np.uint32)
+    d["material_offset"] = material_offset
+    d["mask"] = mask
+    return d
+
+
+class World:
+    """TLAS over instances of BLASes (RayTracingWorld, software path)."""
+
+    def __init__(self):
+        out = C.c_void_p()
+        check(lib().atlas_rt_world_create(C.byref(out)))
+        self._h = out
+        self._blases = []
+
+    def update(self, descs, blases, world_aabbs=None):
+        descs = np.ascontiguousarray(descs, dtype=_abi.INSTANCE_DESC_DTYPE)
+        self._blases = list(blases)
+        table = (C.c_void_p * len(self._blases))(*[b.handle.value for b in self._blases])
+        if world_aabbs is not None:
+            world_aabbs = np.ascontiguousarray(world_aabbs, dtype=np.float32).reshape(-1, 6)
+        check(lib().atlas_rt_world_update(self._h, _ptr(descs), len(descs), _ptr(world_aabbs),
+                                          C.cast(table, C.c_void_p), len(self._blases)))
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            check(lib().atlas_rt_world_destroy(self._h))
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        n, i, d = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        check(lib().atlas_rt_world_info(self._h, C.byref(n), C.byref(i), C.byref(d)))
+        return n.value, i.value, d.value
+
+    def export(self):
+        n, i, _ = self.info()
+        nodes = np.zeros(n, dtype=NODE_DTYPE)
+        inst = np.zeros(i, dtype=INSTANCE_DTYPE)
+        order = np.zeros(i, dtype=np.uint32)
+        check(lib().atlas_rt_world_export(self._h, _ptr(nodes) if n else None, _ptr(inst), _ptr(order)))
+        return nodes, inst, order
+
+    def trace_closest_rays(self, d_rays, d_hits, d_inst=None, count=None, mask=_abi.MASK_ALL, stream=None):
+        n = count if count is not None else d_rays.numel() // 8
+        check(lib().atlas_rt_world_trace_closest_rays(self._h, _ptr(d_rays), _ptr(d_hits), _ptr(d_inst), n, mask,
+                                                      _stream_ptr(stream)))
+
+    def trace_any_rays(self, d_rays, d_hit, count=None, mask=_abi.MASK_SHADOW, flags=_abi.ANY_ENGINE_HITANY,
+                       stream=None):
+        n = count if count is not None else d_rays.numel() // 8
+        check(lib().atlas_rt_world_trace_any_rays(self._h, _ptr(d_rays), _ptr(d_hit), n, mask, flags,
+                                                  _stream_ptr(stream)))
+
+    def trace_closest_packed(self, d_in, d_out, d_count, max_count, tmin=0.0, tmax=1e12, mask=_abi.MASK_ALL,
+                             stream=None):
+        check(lib().atlas_rt_trace_closest(self._h, _ptr(d_in), _ptr(d_out), _ptr(d_count), max_count, tmin, tmax,
+                                           mask, _stream_ptr(stream)))
+
+    def trace_any_packed(self, d_in, d_occluded, d_count, max_count, tmin=0.0, tmax=1e12, mask=_abi.MASK_SHADOW,
+                         flags=_abi.ANY_ENGINE_HITANY, stream=None):
+        check(lib().atlas_rt_trace_any(self._h, _ptr(d_in), _ptr(d_occluded), _ptr(d_count), max_count, tmin, tmax,
+                                       mask, flags, _stream_ptr(stream)))

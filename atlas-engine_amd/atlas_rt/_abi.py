@@ -27,6 +27,9 @@ KERNEL_BVH2_PERSISTENT = 2
 KERNEL_BVH4 = 3
 KERNEL_BVH8 = 4
 
+ANY_ENGINE_HITANY = 0
+ANY_RESTORE_ALWAYS = 1
+
 MASK_ALL = 1 << 7
 MASK_SHADOW = 1 << 6
 
@@ -51,6 +54,12 @@ BVH_TRI_DTYPE = np.dtype([("v0", "<f4", 3), ("v1", "<f4", 3), ("v2", "<f4", 3), 
 RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("tmin", "<f4"), ("direction", "<f4", 3), ("tmax", "<f4")])
 HIT_DTYPE = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("prim_id", "<i4")])
 AABB_DTYPE = np.dtype([("min", "<f4", 3), ("max", "<f4", 3)])
+# atlas_rt_instance_desc, 112 B.
+INSTANCE_DESC_DTYPE = np.dtype([
+    ("object_to_world", "<f4", (3, 4)), ("world_to_object", "<f4", (3, 4)), ("blas", "<u4"),
+    ("material_offset", "<i4"), ("mask", "<u4"), ("padding", "<u4"),
+])
+assert INSTANCE_DESC_DTYPE.itemsize == 112
 
 assert NODE_DTYPE.itemsize == 64 and GPU_TRI_DTYPE.itemsize == 48
 assert INSTANCE_DTYPE.itemsize == 64 and PACKED_RAY_DTYPE.itemsize == 48
@@ -78,4 +87,13 @@ PROTOTYPES = {
     "atlas_rt_trace_closest_host": (status, [vp, vp, vp, u32]),
     "atlas_rt_trace_any_host": (status, [vp, vp, vp, u32]),
     "atlas_rt_last_kernel_ms": (status, [vp, C.POINTER(f32)]),
+    "atlas_rt_world_create": (status, [C.POINTER(vp)]),
+    "atlas_rt_world_destroy": (status, [vp]),
+    "atlas_rt_world_update": (status, [vp, vp, u32, vp, vp, u32]),
+    "atlas_rt_world_info": (status, [vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32)]),
+    "atlas_rt_world_export": (status, [vp, vp, vp, vp]),
+    "atlas_rt_trace_closest": (status, [vp, vp, vp, vp, u32, f32, f32, u32, vp]),
+    "atlas_rt_trace_any": (status, [vp, vp, vp, vp, u32, f32, f32, u32, u32, vp]),
+    "atlas_rt_world_trace_closest_rays": (status, [vp, vp, vp, vp, u32, u32, vp]),
+    "atlas_rt_world_trace_any_rays": (status, [vp, vp, vp, u32, u32, u32, vp]),
 }

@@ -161,6 +161,78 @@ def pinhole_rays(width, height, eye, yaw_deg, pitch_deg, fov_y_deg=60.0, tmin=1e
 C1_LIGHT_DIR = np.array([0.4, 0.8, 0.3]) / np.linalg.norm([0.4, 0.8, 0.3])
 
 
+# ---------------------------------------------------------------------------
+# Config 2: instanced blob meshes (two-level)
+# ---------------------------------------------------------------------------
+def blob_mesh(g, stacks=32, slices=64, bumps=3, amplitude=0.25):
+    """A unit sphere (stacks x slices lat-long, 2*stacks*slices - 2*slices
+    triangles = 3,968 for 32 x 64) radially displaced by a few random
+    spherical-harmonic-like bumps.  Returns (verts (n,9) float32, vertex
+    normals (n,9) float32)."""
+    th = np.linspace(0, np.pi, stacks + 1)
+    ph = np.linspace(0, 2 * np.pi, slices + 1)
+    T, P = np.meshgrid(th, ph, indexing="ij")
+    dirs = np.stack([np.sin(T) * np.cos(P), np.cos(T), np.sin(T) * np.sin(P)], axis=-1)
+    r = np.ones_like(T)
+    for _ in range(bumps):
+        k = g.standard_normal(3)
+        k /= np.linalg.norm(k)
+        f = g.uniform(2.0, 5.0)
+        phase = g.uniform(0, 2 * np.pi)
+        r += amplitude / bumps * np.sin(f * (dirs @ k) * np.pi + phase)
+    pos = dirs * r[..., None]
+    tris = []
+    for i in range(stacks):
+        for j in range(slices):
+            a, b = pos[i, j], pos[i + 1, j]
+            c, d = pos[i + 1, j + 1], pos[i, j + 1]
+            if i != 0:
+                tris.append(np.concatenate([a, b, d]))
+            if i != stacks - 1:
+                tris.append(np.concatenate([b, c, d]))
+    verts = np.asarray(tris, dtype=np.float32)
+    # smooth normals: average of the face normals around each lattice vertex
+    # is approximated by the displaced sphere direction's gradient-free proxy
+    nrm = verts.reshape(-1, 3, 3).astype(np.float64)
+    nrm /= np.linalg.norm(nrm, axis=-1, keepdims=True)
+    return verts, nrm.reshape(-1, 9).astype(np.float32)
+
+
+def random_affine(g, n, scale=(0.5, 3.0), extent=100.0, height=20.0):
+    """(n, 3, 4) object-to-world matrices: uniform scale, uniform rotation,
+    translation over [-extent, extent] in x/z and [-height, height] in y."""
+    R = _random_rotations(g, n)
+    s = g.uniform(scale[0], scale[1], n)
+    t = np.column_stack([g.uniform(-extent, extent, n), g.uniform(-height, height, n), g.uniform(-extent, extent, n)])
+    M = np.zeros((n, 3, 4))
+    M[:, :, :3] = R * s[:, None, None]
+    M[:, :, 3] = t
+    return M
+
+
+def c2_scene(n_meshes=8, n_instances=256, seed_salt=0):
+    """Two-level scene: n_meshes blob meshes, n_instances random instances,
+    per-mesh Lambert+GGX materials (albedo U[0.1,0.9], roughness U[0.05,1])."""
+    g = rng(2, seed_salt)
+    meshes = [blob_mesh(g) for _ in range(n_meshes)]
+    mats = {
+        "albedo": g.uniform(0.1, 0.9, (n_meshes, 3)).astype(np.float32),
+        "roughness": g.uniform(0.05, 1.0, n_meshes).astype(np.float32),
+        "metalness": np.zeros(n_meshes, np.float32),
+    }
+    xf = random_affine(g, n_instances)
+    which = g.integers(0, n_meshes, n_instances)
+    return meshes, mats, xf, which
+
+
+C2_SUN_DIR = np.array([-0.3, -1.0, -0.2]) / np.linalg.norm([-0.3, -1.0, -0.2])  # direction light travels
+
+
+def c2_camera(width=1280, height=720):
+    """Eye outside the instance cloud looking at its centre."""
+    return dict(eye=(0.0, 60.0, -260.0), yaw_deg=0.0, pitch_deg=12.0, fov_y_deg=55.0, width=width, height=height)
+
+
 def c1(grid=1024, width=1920, height=1080):
     """Coherent primary rays over a heightfield (config 1): (verts, primary rays)."""
     verts = heightfield(grid=grid)

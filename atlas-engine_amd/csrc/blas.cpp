@@ -107,16 +107,24 @@ atlas_rt_status upload_blas(atlas_rt_blas& b) {
     }
     void* dn = nullptr;
     void* dt = nullptr;
-    if (nn) ATLAS_HIP_TRY(hipMalloc(&dn, nn * sizeof(atlas_rt_gpu_bvh_node)));
-    hipError_t e = hipMalloc(&dt, packed.size() * sizeof(float));
+    void* dv = nullptr;
+    hipError_t e = hipSuccess;
+    if (nn) e = hipMalloc(&dn, nn * sizeof(atlas_rt_gpu_bvh_node));
+    if (e == hipSuccess) e = hipMalloc(&dt, packed.size() * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&dv, nt * sizeof(atlas_rt_gpu_bvh_triangle));
+    if (e == hipSuccess && nn)
+        e = hipMemcpy(dn, b.nodes.data(), nn * sizeof(atlas_rt_gpu_bvh_node), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dt, packed.data(), packed.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dv, b.tris.data(), nt * sizeof(atlas_rt_gpu_bvh_triangle), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         if (dn) hipFree(dn);
-        return hip_fail(e, "hipMalloc(tris)");
+        if (dt) hipFree(dt);
+        if (dv) hipFree(dv);
+        return hip_fail(e, "upload_blas");
     }
-    if (nn) ATLAS_HIP_TRY(hipMemcpy(dn, b.nodes.data(), nn * sizeof(atlas_rt_gpu_bvh_node), hipMemcpyHostToDevice));
-    ATLAS_HIP_TRY(hipMemcpy(dt, packed.data(), packed.size() * sizeof(float), hipMemcpyHostToDevice));
     b.d_nodes = reinterpret_cast<float4*>(dn);
     b.d_tris = reinterpret_cast<float4*>(dt);
+    b.d_vtx = reinterpret_cast<float4*>(dv);
     b.device = dev;
     b.on_device = true;
     return ATLAS_RT_OK;
@@ -239,6 +247,7 @@ atlas_rt_status atlas_rt_blas_destroy(atlas_rt_blas* b) {
     if (!b) return ATLAS_RT_OK;
     if (b->d_nodes) hipFree(b->d_nodes);
     if (b->d_tris) hipFree(b->d_tris);
+    if (b->d_vtx) hipFree(b->d_vtx);
     b->spill.release();
     b->flags.release();
     b->timing.release();

@@ -19,7 +19,9 @@ static atlas_rt_status check_blas_ready(const atlas_rt_blas* b, const char* who)
 }
 
 static atlas_rt_status check_kernel(int32_t kernel, const char* who) {
-    if (kernel == ATLAS_RT_KERNEL_AUTO || kernel == ATLAS_RT_KERNEL_BVH2_EXACT) return ATLAS_RT_OK;
+    if (kernel == ATLAS_RT_KERNEL_AUTO || kernel == ATLAS_RT_KERNEL_BVH2_EXACT ||
+        kernel == ATLAS_RT_KERNEL_BVH2_PERSISTENT)
+        return ATLAS_RT_OK;
     return fail(ATLAS_RT_E_UNSUPPORTED, std::string(who) + ": kernel variant not available");
 }
 

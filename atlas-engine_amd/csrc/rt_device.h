@@ -100,6 +100,28 @@ __device__ __forceinline__ bool tri_uvt(const DevRay& R, float v0x, float v0y, f
     return t >= 0.0f && u >= 0.0f && v >= 0.0f && u + v <= 1.0f;
 }
 
+// Reported (t, u, v) of the final hit: the same formula evaluated in double
+// from the float32 vertices and ray, then rounded once to float.  Traversal
+// decisions never use it (they use tri_uvt), so hit ids stay the engine's;
+// the reported values are within ~1 ulp of an exact evaluation, whereas the
+// float32 formula loses up to ~1e-4 in u, v when the origin is far from a
+// small triangle.  Operation order is fixed (no contraction) so the CPU
+// oracle reproduces it bit for bit.
+__device__ __forceinline__ void refine_tuv(float ox, float oy, float oz, float dx, float dy, float dz,
+                                           float4 v0, float4 v1, float4 v2, float& t, float& u, float& v) {
+    const double ax = v0.x, ay = v0.y, az = v0.z;
+    const double e0x = (double)v1.x - ax, e0y = (double)v1.y - ay, e0z = (double)v1.z - az;
+    const double e1x = (double)v2.x - ax, e1y = (double)v2.y - ay, e1z = (double)v2.z - az;
+    const double sx = (double)ox - ax, sy = (double)oy - ay, sz = (double)oz - az;
+    const double Dx = dx, Dy = dy, Dz = dz;
+    const double px = sy * e0z - e0y * sz, py = sz * e0x - e0z * sx, pz = sx * e0y - e0x * sy;
+    const double qx = Dy * e1z - e1y * Dz, qy = Dz * e1x - e1z * Dx, qz = Dx * e1y - e1x * Dy;
+    const double den = (qx * e0x + qy * e0y) + qz * e0z;
+    t = (float)(((px * e1x + py * e1y) + pz * e1z) / den);
+    u = (float)(((qx * sx + qy * sy) + qz * sz) / den);
+    v = (float)(((px * Dx + py * Dy) + pz * Dz) / den);
+}
+
 __device__ __forceinline__ bool any_nan3(float x, float y, float z) {
     return __builtin_isnan(x) || __builtin_isnan(y) || __builtin_isnan(z);
 }

@@ -23,17 +23,13 @@ atlas_rt_status hip_fail(hipError_t e, const char* what);
         if (_e != hipSuccess) return ::atlas_rt::hip_fail(_e, #expr); \
     } while (0)
 
-// Device-side BLAS arrays.
-//  nodes: engine GPUBVHNode layout, 4 x float4 per node (data0..data3,
-//         structures.hsh:179-184).
-//  tris : 3 x float4 per BVH slot: (v0.xyz, caller index bits),
-//         (e0 = v1 - v0, end-of-leaf +1/-1), (e1 = v2 - v0, 0).
-struct DeviceBlas {
-    const float4* nodes = nullptr;
-    const float4* tris = nullptr;
-    uint32_t node_count = 0;
-    uint32_t tri_count = 0;
-    uint32_t max_depth = 0;
+// Device view of one BLAS (32 B, also the element type of BLAS tables).
+struct BlasDev {
+    const float4* nodes;  // engine GPUBVHNode, 4 float4 per node
+    const float4* tris;   // 3 float4 per slot: (v0, caller idx bits), (v1-v0, end flag), (v2-v0, 0)
+    const float4* vtx;    // engine GPUBVHTriangle, 3 float4 per slot (v0, v1, v2)
+    uint32_t node_count;
+    uint32_t tri_count;
 };
 
 // Stack spill scratch, grown on demand and owned by a handle.
@@ -65,23 +61,21 @@ struct atlas_rt_blas {
     std::vector<uint32_t> orig_idx;
     uint32_t max_depth = 0;
 
+    // Device arrays:
+    //  d_nodes: engine GPUBVHNode layout, 4 x float4 per node (data0..data3,
+    //           structures.hsh:179-184);
+    //  d_tris : 3 x float4 per BVH slot: (v0.xyz, caller index bits),
+    //           (e0 = v1 - v0, end-of-leaf +1/-1), (e1 = v2 - v0, 0);
+    //  d_vtx  : engine GPUBVHTriangle array, read once per ray for the
+    //           reported (t, u, v) of the final hit.
     int device = -1;
     bool on_device = false;
     float4* d_nodes = nullptr;
     float4* d_tris = nullptr;
+    float4* d_vtx = nullptr;
     atlas_rt::Scratch spill;
-    atlas_rt::Scratch flags;
+    atlas_rt::Scratch flags;  // [0] ray counter, [1] finished waves, [2] stack overflow flag
     atlas_rt::LaunchTiming timing;
-
-    atlas_rt::DeviceBlas view() const {
-        atlas_rt::DeviceBlas v;
-        v.nodes = d_nodes;
-        v.tris = d_tris;
-        v.node_count = uint32_t(nodes.size());
-        v.tri_count = uint32_t(tris.size());
-        v.max_depth = max_depth;
-        return v;
-    }
 };
 
 namespace atlas_rt {
@@ -94,8 +88,11 @@ struct TraceLaunch {
     uint32_t* visits = nullptr;  // optional uint2 per ray: (inner nodes fetched, triangles tested)
     uint32_t count = 0;
     int32_t kernel = ATLAS_RT_KERNEL_AUTO;
+    uint32_t refill = 0;  // persistent kernels: refill a wave below this many active lanes (0 = default)
     hipStream_t stream = nullptr;
 };
+
+int device_cu_count();
 
 atlas_rt_status launch_blas_closest(atlas_rt_blas& blas, const TraceLaunch& L);
 atlas_rt_status launch_blas_any(atlas_rt_blas& blas, const TraceLaunch& L);

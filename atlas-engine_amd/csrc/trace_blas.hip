@@ -1,224 +1,198 @@
 // BVH2 traversal kernels over one BLAS, gfx950.
 //
 // Reference: HitClosest / HitAny in BLAS mode (data/shader/raytracer/bvh.hsh:244-267,
-// 411-433) with CheckLeafClosest / CheckLeaf (bvh.hsh:44-104), launched by
-// traceClosest.csh:12-36 one ray per invocation.
+// 411-433), launched one ray per invocation by traceClosest.csh:12-36.
 //
-// MI355X mapping:
-//  * one ray per lane, wave64, BLOCK lanes per workgroup;
-//  * the traversal stack lives in LDS as stack[depth][lane] (lane-contiguous,
-//    so lanes at different depths never share a bank); entries beyond
-//    LDS_DEPTH spill to a per-thread global array sized from the tree depth;
-//  * nodes are the engine's 64 B records (one cache line each, four dwordx4
-//    loads); triangles use 48 B records with the two edge vectors
-//    precomputed;
-//  * traversal order and acceptance are the engine's exactly: near child first
-//    by entry distance with ties to the left, first accepted triangle wins an
-//    equal t, so the hit slot equals the reference's bit for bit.
+// Two launch shapes share the per-lane step function of trace_core.h:
+//  * grid: one ray per lane, the whole batch in one grid (the engine's shape);
+//  * persistent: a resident grid of waves pulls rays from a global counter.
+//    Whenever fewer than `refill` lanes of a wave are still traversing, the
+//    idle lanes are refilled with new rays (ballot + popcount give each idle
+//    lane its index from one atomicAdd per wave), so divergent incoherent
+//    rays do not leave most of a wave idle.  The last wave to finish resets
+//    the counter, so consecutive launches need no host-side reset.
+// The stack lives in LDS rows [depth][lane] (bank-conflict free since lanes
+// at different depths still sit in different banks) and spills to a global
+// array sized from the tree depth.  Results equal the engine's bit for bit.
 #include <hip/hip_runtime.h>
 
-#include "rt_device.h"
 #include "rt_internal.h"
+#include "trace_core.h"
 
 namespace atlas_rt {
 
 namespace {
 
-struct StackRef {
-    int32_t* lds;          // [LDS_DEPTH][BLOCK]
-    int32_t* spill;        // [(cap - LDS_DEPTH)][stride]
-    uint32_t lane;         // threadIdx.x
-    uint32_t gid;          // global thread index (spill column)
-    uint32_t stride;       // spill row stride (threads in grid)
-};
-
-template <int BLOCK, int LDS_DEPTH>
-__device__ __forceinline__ void stack_put(const StackRef& S, uint32_t sp, int32_t v) {
-    if (sp < LDS_DEPTH)
-        S.lds[sp * BLOCK + S.lane] = v;
-    else
-        S.spill[size_t(sp - LDS_DEPTH) * S.stride + S.gid] = v;
-}
-
-template <int BLOCK, int LDS_DEPTH>
-__device__ __forceinline__ int32_t stack_get(const StackRef& S, uint32_t sp) {
-    if (sp < LDS_DEPTH) return S.lds[sp * BLOCK + S.lane];
-    return S.spill[size_t(sp - LDS_DEPTH) * S.stride + S.gid];
-}
-
-// Closest hit for one ray; returns the engine slot (-1 on a miss).
-template <int BLOCK, int LDS_DEPTH, bool STATS>
-__device__ __forceinline__ int32_t closest_one(const float4* __restrict__ nodes, uint32_t node_count,
-                                               const float4* __restrict__ tris, const DevRay& R, float& hitT,
-                                               float& hitU, float& hitV, int32_t& prim, const StackRef& S,
-                                               uint32_t cap, uint32_t* overflow_flag, uint32_t& n_nodes,
-                                               uint32_t& n_tris) {
-    int32_t slot = -1;
-    int32_t node = node_count ? 0 : ~0;
-    uint32_t sp = 0;
-    for (;;) {
-        if (node >= 0) {
-            const float4* np = nodes + 4 * size_t(node);
-            const float4 a = np[0], b = np[1], c = np[2], d = np[3];
-            if (STATS) n_nodes++;
-            float dl, dr;
-            const bool hl = slab_dist(R, a.x, a.y, a.z, a.w, b.x, b.y, hitT, dl);
-            const bool hr = slab_dist(R, b.z, b.w, c.x, c.y, c.z, c.w, hitT, dr);
-            const int32_t lp = __float_as_int(d.x), rp = __float_as_int(d.y);
-            const bool leftFirst = dl <= dr;
-            if (hl && hr) {
-                if (sp >= cap) {
-                    atomicOr(overflow_flag, 1u);
-                    return -1;
-                }
-                stack_put<BLOCK, LDS_DEPTH>(S, sp, leftFirst ? rp : lp);
-                sp++;
-                node = leftFirst ? lp : rp;
-            } else if (hl || hr) {
-                node = leftFirst ? lp : rp;
-            } else {
-                if (sp == 0) break;
-                sp--;
-                node = stack_get<BLOCK, LDS_DEPTH>(S, sp);
-            }
-        } else {
-            int32_t tp = ~node;
-            const float leafMax = hitT;
-            for (;;) {
-                const float4* tr = tris + 3 * size_t(tp);
-                const float4 a = tr[0], b = tr[1], c = tr[2];
-                float t, u, v;
-                if (STATS) n_tris++;
-                const bool geo = tri_uvt(R, a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z, t, u, v);
-                if (geo && t > R.tmin && t < leafMax && t < hitT) {
-                    hitT = t;
-                    hitU = u;
-                    hitV = v;
-                    slot = tp;
-                    prim = __float_as_int(a.w);
-                }
-                if (b.w > 0.0f) break;
-                tp++;
-            }
-            if (sp == 0) break;
-            sp--;
-            node = stack_get<BLOCK, LDS_DEPTH>(S, sp);
-        }
-    }
-    return slot;
-}
-
-template <int BLOCK, int LDS_DEPTH>
-__device__ __forceinline__ bool any_one(const float4* __restrict__ nodes, uint32_t node_count,
-                                        const float4* __restrict__ tris, const DevRay& R, float tmax,
-                                        const StackRef& S, uint32_t cap, uint32_t* overflow_flag) {
-    int32_t node = node_count ? 0 : ~0;
-    uint32_t sp = 0;
-    for (;;) {
-        if (node >= 0) {
-            const float4* np = nodes + 4 * size_t(node);
-            const float4 a = np[0], b = np[1], c = np[2], d = np[3];
-            float dl, dr;
-            const bool hl = slab_dist(R, a.x, a.y, a.z, a.w, b.x, b.y, tmax, dl);
-            const bool hr = slab_dist(R, b.z, b.w, c.x, c.y, c.z, c.w, tmax, dr);
-            const int32_t lp = __float_as_int(d.x), rp = __float_as_int(d.y);
-            if (hl && hr) {
-                if (sp >= cap) {
-                    atomicOr(overflow_flag, 1u);
-                    return false;
-                }
-                stack_put<BLOCK, LDS_DEPTH>(S, sp, rp);
-                sp++;
-                node = lp;
-            } else if (hl || hr) {
-                node = hl ? lp : rp;
-            } else {
-                if (sp == 0) return false;
-                sp--;
-                node = stack_get<BLOCK, LDS_DEPTH>(S, sp);
-            }
-        } else {
-            int32_t tp = ~node;
-            for (;;) {
-                const float4* tr = tris + 3 * size_t(tp);
-                const float4 a = tr[0], b = tr[1], c = tr[2];
-                float t, u, v;
-                const bool geo = tri_uvt(R, a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z, t, u, v);
-                if (geo && t > R.tmin && t < tmax) return true;
-                if (b.w > 0.0f) break;
-                tp++;
-            }
-            if (sp == 0) return false;
-            sp--;
-            node = stack_get<BLOCK, LDS_DEPTH>(S, sp);
-        }
-    }
-}
+constexpr int kWave = 64;
 
 template <int BLOCK, int LDS_DEPTH, bool STATS>
-__global__ __launch_bounds__(BLOCK) void k_blas_closest(const float4* __restrict__ nodes, uint32_t node_count,
-                                                        const float4* __restrict__ tris,
-                                                        const float4* __restrict__ rays, float4* __restrict__ hits,
-                                                        int32_t* __restrict__ slots, uint32_t count,
+__global__ __launch_bounds__(BLOCK) void k_blas_closest(BlasDev B, const float4* __restrict__ rays,
+                                                        float4* __restrict__ hits, int32_t* __restrict__ slots,
+                                                        uint2* __restrict__ visits, uint32_t count,
                                                         int32_t* __restrict__ spill, uint32_t cap,
-                                                        uint32_t* __restrict__ overflow_flag,
-                                                        uint2* __restrict__ visits) {
+                                                        uint32_t* __restrict__ ctl) {
     __shared__ int32_t lds[LDS_DEPTH * BLOCK];
     const uint32_t gid = blockIdx.x * BLOCK + threadIdx.x;
     if (gid >= count) return;
     const float4 r0 = rays[2 * size_t(gid)];
     const float4 r1 = rays[2 * size_t(gid) + 1];
-    float hitT = r1.w, hitU = 0.0f, hitV = 0.0f;
-    int32_t prim = -1, slot = -1;
+    LaneState L;
+    L.slot = -1;
     uint32_t n_nodes = 0, n_tris = 0;
     if (!any_nan3(r1.x, r1.y, r1.z)) {
-        DevRay R;
-        make_dev_ray(R, r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, r0.w);
-        StackRef S{lds, spill, threadIdx.x, gid, gridDim.x * BLOCK};
-        slot = closest_one<BLOCK, LDS_DEPTH, STATS>(nodes, node_count, tris, R, hitT, hitU, hitV, prim, S, cap,
-                                                    overflow_flag, n_nodes, n_tris);
-        if (slot < 0) {
-            hitT = r1.w;
-            hitU = hitV = 0.0f;
-            prim = -1;
+        lane_begin(L, B, r0, r1);
+        const LaneStack<BLOCK, LDS_DEPTH> S{lds, spill, threadIdx.x, gid, gridDim.x * BLOCK};
+        int r;
+        do {
+            r = step_closest<BLOCK, LDS_DEPTH, STATS>(B, L, S, cap, n_nodes, n_tris);
+        } while (r == kContinue);
+        if (r == kOverflow) {
+            atomicOr(&ctl[2], 1u);
+            L.slot = -1;
         }
     }
-    hits[gid] = make_float4(hitT, hitU, hitV, __int_as_float(prim));
-    if (slots) slots[gid] = slot;
+    hits[gid] = lane_hit_record(B, L, r1.w);
+    if (slots) slots[gid] = L.slot;
     if (STATS) visits[gid] = make_uint2(n_nodes, n_tris);
 }
 
 template <int BLOCK, int LDS_DEPTH>
-__global__ __launch_bounds__(BLOCK) void k_blas_any(const float4* __restrict__ nodes, uint32_t node_count,
-                                                    const float4* __restrict__ tris, const float4* __restrict__ rays,
+__global__ __launch_bounds__(BLOCK) void k_blas_closest_persistent(BlasDev B, const float4* __restrict__ rays,
+                                                                   float4* __restrict__ hits,
+                                                                   int32_t* __restrict__ slots, uint32_t count,
+                                                                   int32_t* __restrict__ spill, uint32_t cap,
+                                                                   uint32_t* __restrict__ ctl, uint32_t refill) {
+    __shared__ int32_t lds[LDS_DEPTH * BLOCK];
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const LaneStack<BLOCK, LDS_DEPTH> S{lds, spill, threadIdx.x, blockIdx.x * BLOCK + threadIdx.x, gridDim.x * BLOCK};
+    LaneState L;
+    L.slot = -1;
+    bool active = false;
+    bool exhausted = false;
+    uint32_t idx = 0;
+    float tmax = 0.0f;
+    uint32_t dn = 0, dt = 0;
+    for (;;) {
+        if (!exhausted) {
+            const uint64_t idle = __ballot(!active);
+            if (idle) {
+                const uint32_t n = __popcll(idle);
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(&ctl[0], n);
+                base = __shfl(base, 0);
+                if (!active) {
+                    idx = base + __popcll(idle & ((1ull << lane) - 1));
+                    if (idx < count) {
+                        const float4 r0 = rays[2 * size_t(idx)];
+                        const float4 r1 = rays[2 * size_t(idx) + 1];
+                        tmax = r1.w;
+                        if (any_nan3(r1.x, r1.y, r1.z)) {
+                            hits[idx] = make_float4(tmax, 0.0f, 0.0f, __int_as_float(-1));
+                            if (slots) slots[idx] = -1;
+                        } else {
+                            lane_begin(L, B, r0, r1);
+                            active = true;
+                        }
+                    }
+                }
+                if (base + n >= count) exhausted = true;
+            }
+        }
+        if (!__any(active)) {
+            if (exhausted) break;
+            continue;
+        }
+        const uint32_t threshold = exhausted ? 1u : refill;
+        for (;;) {
+            if (active) {
+                const int r = step_closest<BLOCK, LDS_DEPTH, false>(B, L, S, cap, dn, dt);
+                if (r != kContinue) {
+                    if (r == kOverflow) {
+                        atomicOr(&ctl[2], 1u);
+                        L.slot = -1;
+                    }
+                    hits[idx] = lane_hit_record(B, L, tmax);
+                    if (slots) slots[idx] = L.slot;
+                    active = false;
+                }
+            }
+            if (uint32_t(__popcll(__ballot(active))) < threshold) break;
+        }
+    }
+    // Every wave has taken its last ray; the last one out resets the counters.
+    if (lane == 0) {
+        __threadfence();
+        const uint32_t waves = gridDim.x * (BLOCK / kWave);
+        if (atomicAdd(&ctl[1], 1u) == waves - 1) {
+            atomicExch(&ctl[0], 0u);
+            atomicExch(&ctl[1], 0u);
+        }
+    }
+}
+
+template <int BLOCK, int LDS_DEPTH>
+__global__ __launch_bounds__(BLOCK) void k_blas_any(BlasDev B, const float4* __restrict__ rays,
                                                     uint32_t* __restrict__ out, uint32_t count,
                                                     int32_t* __restrict__ spill, uint32_t cap,
-                                                    uint32_t* __restrict__ overflow_flag) {
+                                                    uint32_t* __restrict__ ctl) {
     __shared__ int32_t lds[LDS_DEPTH * BLOCK];
     const uint32_t gid = blockIdx.x * BLOCK + threadIdx.x;
     if (gid >= count) return;
     const float4 r0 = rays[2 * size_t(gid)];
     const float4 r1 = rays[2 * size_t(gid) + 1];
-    bool hit = false;
+    LaneState L;
+    L.slot = -1;
     if (!any_nan3(r1.x, r1.y, r1.z)) {
-        DevRay R;
-        make_dev_ray(R, r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, r0.w);
-        StackRef S{lds, spill, threadIdx.x, gid, gridDim.x * BLOCK};
-        hit = any_one<BLOCK, LDS_DEPTH>(nodes, node_count, tris, R, r1.w, S, cap, overflow_flag);
+        lane_begin(L, B, r0, r1);
+        const LaneStack<BLOCK, LDS_DEPTH> S{lds, spill, threadIdx.x, gid, gridDim.x * BLOCK};
+        int r;
+        do {
+            r = step_any<BLOCK, LDS_DEPTH>(B, L, S, cap, r1.w);
+        } while (r == kContinue);
+        if (r == kOverflow) {
+            atomicOr(&ctl[2], 1u);
+            L.slot = -1;
+        }
     }
-    out[gid] = hit ? 1u : 0u;
+    out[gid] = L.slot >= 0 ? 1u : 0u;
 }
 
 constexpr int kBlock = 128;
 constexpr int kLdsDepth = 32;
+constexpr int kPBlock = 256;
+constexpr int kPLdsDepth = 24;
+
+BlasDev blas_dev(const atlas_rt_blas& b) {
+    BlasDev d;
+    d.nodes = b.d_nodes;
+    d.tris = b.d_tris;
+    d.vtx = b.d_vtx;
+    d.node_count = uint32_t(b.nodes.size());
+    d.tri_count = uint32_t(b.tris.size());
+    return d;
+}
 
 }  // namespace
 
-static atlas_rt_status prepare_stack(atlas_rt_blas& blas, uint32_t threads, uint32_t& cap) {
+int device_cu_count() {
+    static int cached = -1;
+    if (cached < 0) {
+        int dev = 0;
+        hipGetDevice(&dev);
+        hipDeviceProp_t p;
+        if (hipGetDeviceProperties(&p, dev) == hipSuccess && p.multiProcessorCount > 0)
+            cached = p.multiProcessorCount;
+        else
+            cached = 256;
+    }
+    return cached;
+}
+
+static atlas_rt_status prepare(atlas_rt_blas& blas, uint32_t threads, uint32_t lds_depth, uint32_t& cap) {
     // The stack never holds more entries than the deepest leaf's depth.
     cap = blas.max_depth + 1;
-    if (cap > uint32_t(kLdsDepth)) {
-        size_t need = size_t(cap - kLdsDepth) * threads * sizeof(int32_t);
+    if (cap > lds_depth) {
+        const size_t need = size_t(cap - lds_depth) * threads * sizeof(int32_t);
         atlas_rt_status st = blas.spill.ensure(need);
         if (st != ATLAS_RT_OK) return st;
     }
@@ -232,24 +206,40 @@ static atlas_rt_status prepare_stack(atlas_rt_blas& blas, uint32_t threads, uint
 
 atlas_rt_status launch_blas_closest(atlas_rt_blas& blas, const TraceLaunch& L) {
     if (L.count == 0) return ATLAS_RT_OK;
-    const uint32_t grid = (L.count + kBlock - 1) / kBlock;
+    const BlasDev B = blas_dev(blas);
+    uint32_t* ctl = reinterpret_cast<uint32_t*>(blas.flags.ptr);
+    const bool persistent = !L.visits && L.kernel == ATLAS_RT_KERNEL_BVH2_PERSISTENT;
+    atlas_rt_status st;
     uint32_t cap = 0;
-    atlas_rt_status st = prepare_stack(blas, grid * kBlock, cap);
-    if (st != ATLAS_RT_OK) return st;
-    st = blas.timing.begin(L.stream);
-    if (st != ATLAS_RT_OK) return st;
-    if (L.visits)
-        hipLaunchKernelGGL((k_blas_closest<kBlock, kLdsDepth, true>), dim3(grid), dim3(kBlock), 0, L.stream,
-                           blas.d_nodes, uint32_t(blas.nodes.size()), blas.d_tris,
-                           reinterpret_cast<const float4*>(L.rays), reinterpret_cast<float4*>(L.hits), L.slots,
-                           L.count, reinterpret_cast<int32_t*>(blas.spill.ptr), cap,
-                           reinterpret_cast<uint32_t*>(blas.flags.ptr), reinterpret_cast<uint2*>(L.visits));
-    else
-        hipLaunchKernelGGL((k_blas_closest<kBlock, kLdsDepth, false>), dim3(grid), dim3(kBlock), 0, L.stream,
-                           blas.d_nodes, uint32_t(blas.nodes.size()), blas.d_tris,
-                           reinterpret_cast<const float4*>(L.rays), reinterpret_cast<float4*>(L.hits), L.slots,
-                           L.count, reinterpret_cast<int32_t*>(blas.spill.ptr), cap,
-                           reinterpret_cast<uint32_t*>(blas.flags.ptr), nullptr);
+    if (persistent) {
+        // Resident grid: enough 256-lane groups for every CU several times over.
+        const uint32_t groups = uint32_t(device_cu_count()) * 8;
+        st = prepare(blas, groups * kPBlock, kPLdsDepth, cap);
+        if (st != ATLAS_RT_OK) return st;
+        ctl = reinterpret_cast<uint32_t*>(blas.flags.ptr);
+        st = blas.timing.begin(L.stream);
+        if (st != ATLAS_RT_OK) return st;
+        hipLaunchKernelGGL((k_blas_closest_persistent<kPBlock, kPLdsDepth>), dim3(groups), dim3(kPBlock), 0,
+                           L.stream, B, reinterpret_cast<const float4*>(L.rays), reinterpret_cast<float4*>(L.hits),
+                           L.slots, L.count, reinterpret_cast<int32_t*>(blas.spill.ptr), cap, ctl,
+                           L.refill ? L.refill : 40u);
+    } else {
+        const uint32_t grid = (L.count + kBlock - 1) / kBlock;
+        st = prepare(blas, grid * kBlock, kLdsDepth, cap);
+        if (st != ATLAS_RT_OK) return st;
+        ctl = reinterpret_cast<uint32_t*>(blas.flags.ptr);
+        st = blas.timing.begin(L.stream);
+        if (st != ATLAS_RT_OK) return st;
+        if (L.visits)
+            hipLaunchKernelGGL((k_blas_closest<kBlock, kLdsDepth, true>), dim3(grid), dim3(kBlock), 0, L.stream, B,
+                               reinterpret_cast<const float4*>(L.rays), reinterpret_cast<float4*>(L.hits), L.slots,
+                               reinterpret_cast<uint2*>(L.visits), L.count,
+                               reinterpret_cast<int32_t*>(blas.spill.ptr), cap, ctl);
+        else
+            hipLaunchKernelGGL((k_blas_closest<kBlock, kLdsDepth, false>), dim3(grid), dim3(kBlock), 0, L.stream, B,
+                               reinterpret_cast<const float4*>(L.rays), reinterpret_cast<float4*>(L.hits), L.slots,
+                               nullptr, L.count, reinterpret_cast<int32_t*>(blas.spill.ptr), cap, ctl);
+    }
     ATLAS_HIP_TRY(hipGetLastError());
     return blas.timing.end(L.stream);
 }
@@ -258,14 +248,13 @@ atlas_rt_status launch_blas_any(atlas_rt_blas& blas, const TraceLaunch& L) {
     if (L.count == 0) return ATLAS_RT_OK;
     const uint32_t grid = (L.count + kBlock - 1) / kBlock;
     uint32_t cap = 0;
-    atlas_rt_status st = prepare_stack(blas, grid * kBlock, cap);
+    atlas_rt_status st = prepare(blas, grid * kBlock, kLdsDepth, cap);
     if (st != ATLAS_RT_OK) return st;
     st = blas.timing.begin(L.stream);
     if (st != ATLAS_RT_OK) return st;
-    hipLaunchKernelGGL((k_blas_any<kBlock, kLdsDepth>), dim3(grid), dim3(kBlock), 0, L.stream, blas.d_nodes,
-                       uint32_t(blas.nodes.size()), blas.d_tris, reinterpret_cast<const float4*>(L.rays),
-                       reinterpret_cast<uint32_t*>(L.hits), L.count, reinterpret_cast<int32_t*>(blas.spill.ptr),
-                       cap, reinterpret_cast<uint32_t*>(blas.flags.ptr));
+    hipLaunchKernelGGL((k_blas_any<kBlock, kLdsDepth>), dim3(grid), dim3(kBlock), 0, L.stream, blas_dev(blas),
+                       reinterpret_cast<const float4*>(L.rays), reinterpret_cast<uint32_t*>(L.hits), L.count,
+                       reinterpret_cast<int32_t*>(blas.spill.ptr), cap, reinterpret_cast<uint32_t*>(blas.flags.ptr));
     ATLAS_HIP_TRY(hipGetLastError());
     return blas.timing.end(L.stream);
 }

@@ -1,0 +1,246 @@
+// World objects: the software-ray-tracing half of RayTracingWorld
+// (src/engine/raytracing/RayTracingWorld.cpp:32-162, 267-307) and the C ABI of
+// the two-level traces.
+#include "world.h"
+
+#include <algorithm>
+#include <cstring>
+
+#include "builder.h"
+#include "rt_hostmath.h"
+
+namespace atlas_rt {
+int default_threads();
+}
+
+using namespace atlas_rt;
+
+void atlas_rt_world::free_device() {
+    if (d_tlas) hipFree(d_tlas);
+    if (d_instances) hipFree(d_instances);
+    if (d_table) hipFree(d_table);
+    if (d_order) hipFree(d_order);
+    d_tlas = d_instances = d_table = d_order = nullptr;
+}
+
+// AABB::Transform (AABB.cpp:35-60) with glm's mat4 * vec4 evaluation order
+// ((m0*x + m1*y) + (m2*z + m3*w)).
+static void transform_box(const float m[3][4], const atlas_rt_gpu_bvh_node* root, uint32_t root_count,
+                          const atlas_rt_gpu_bvh_triangle* tris, uint32_t tri_count, float out[6]) {
+    HBox b = empty_box();
+    if (root_count) {
+        HBox l, r;
+        for (int k = 0; k < 3; k++) {
+            l.lo[k] = root->left_min[k];
+            l.hi[k] = root->left_max[k];
+            r.lo[k] = root->right_min[k];
+            r.hi[k] = root->right_max[k];
+        }
+        box_grow(b, l);
+        box_grow(b, r);
+    } else {
+        for (uint32_t i = 0; i < tri_count; i++) {
+            box_grow_point(b, tris[i].v0);
+            box_grow_point(b, tris[i].v1);
+            box_grow_point(b, tris[i].v2);
+        }
+    }
+    const float cx[8] = {b.lo[0], b.lo[0], b.hi[0], b.hi[0], b.lo[0], b.lo[0], b.hi[0], b.hi[0]};
+    const float cy[8] = {b.lo[1], b.lo[1], b.lo[1], b.lo[1], b.hi[1], b.hi[1], b.hi[1], b.hi[1]};
+    const float cz[8] = {b.lo[2], b.hi[2], b.lo[2], b.hi[2], b.lo[2], b.hi[2], b.lo[2], b.hi[2]};
+    float lo[3], hi[3];
+    for (int i = 0; i < 8; i++) {
+        float p[3];
+        for (int r = 0; r < 3; r++)
+            p[r] = (m[r][0] * cx[i] + m[r][1] * cy[i]) + (m[r][2] * cz[i] + m[r][3] * 1.0f);
+        for (int r = 0; r < 3; r++) {
+            if (i == 0) {
+                lo[r] = hi[r] = p[r];
+            } else {
+                lo[r] = gl_min(lo[r], p[r]);
+                hi[r] = gl_max(hi[r], p[r]);
+            }
+        }
+    }
+    for (int r = 0; r < 3; r++) {
+        out[r] = lo[r];
+        out[3 + r] = hi[r];
+    }
+}
+
+static atlas_rt_status check_world(const atlas_rt_world* w, const char* who) {
+    if (!w) return fail(ATLAS_RT_E_INVALID_ARG, std::string(who) + ": NULL world");
+    if (!w->valid) return fail(ATLAS_RT_E_INVALID_ARG, std::string(who) + ": world has no instances (call update)");
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev != w->device)
+        return fail(ATLAS_RT_E_INVALID_ARG, std::string(who) + ": world lives on another device");
+    return ATLAS_RT_OK;
+}
+
+template <class T>
+static atlas_rt_status upload_vec(void*& dst, const std::vector<T>& v) {
+    if (dst) hipFree(dst);
+    dst = nullptr;
+    if (v.empty()) return ATLAS_RT_OK;
+    ATLAS_HIP_TRY(hipMalloc(&dst, v.size() * sizeof(T)));
+    ATLAS_HIP_TRY(hipMemcpy(dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return ATLAS_RT_OK;
+}
+
+extern "C" {
+
+atlas_rt_status atlas_rt_world_create(atlas_rt_world** out) {
+    if (!out) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_world_create: out is NULL");
+    *out = new atlas_rt_world();
+    return ATLAS_RT_OK;
+}
+
+atlas_rt_status atlas_rt_world_destroy(atlas_rt_world* w) {
+    if (!w) return ATLAS_RT_OK;
+    w->free_device();
+    w->spill.release();
+    w->flags.release();
+    w->timing.release();
+    delete w;
+    return ATLAS_RT_OK;
+}
+
+atlas_rt_status atlas_rt_world_update(atlas_rt_world* w, const atlas_rt_instance_desc* inst, uint32_t n,
+                                      const atlas_rt_aabb* world_aabbs, const atlas_rt_blas* const* blases,
+                                      uint32_t blas_count) {
+    if (!w) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_world_update: NULL world");
+    if (n == 0) return fail(ATLAS_RT_E_EMPTY, "atlas_rt_world_update: no instances");
+    if (!inst || !blases || blas_count == 0) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_world_update: NULL array");
+    int dev = 0;
+    ATLAS_HIP_TRY(hipGetDevice(&dev));
+    std::vector<BlasDev> table(blas_count);
+    uint32_t blas_depth = 0;
+    for (uint32_t k = 0; k < blas_count; k++) {
+        const atlas_rt_blas* b = blases[k];
+        if (!b || !b->on_device || b->device != dev)
+            return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_world_update: BLAS " + std::to_string(k) +
+                                                    " is NULL, host-only or on another device");
+        table[k].nodes = b->d_nodes;
+        table[k].tris = b->d_tris;
+        table[k].vtx = b->d_vtx;
+        table[k].node_count = uint32_t(b->nodes.size());
+        table[k].tri_count = uint32_t(b->tris.size());
+        blas_depth = std::max(blas_depth, b->max_depth);
+    }
+    std::vector<float> boxes(6 * size_t(n));
+    for (uint32_t i = 0; i < n; i++) {
+        if (inst[i].blas >= blas_count)
+            return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_world_update: instance " + std::to_string(i) +
+                                                    " names BLAS " + std::to_string(inst[i].blas));
+        if (world_aabbs) {
+            std::memcpy(&boxes[6 * size_t(i)], world_aabbs[i].min, 12);
+            std::memcpy(&boxes[6 * size_t(i) + 3], world_aabbs[i].max, 12);
+        } else {
+            const atlas_rt_blas* b = blases[inst[i].blas];
+            transform_box(inst[i].object_to_world, b->nodes.empty() ? nullptr : &b->nodes[0],
+                          uint32_t(b->nodes.size()), b->tris.data(), uint32_t(b->tris.size()), &boxes[6 * size_t(i)]);
+        }
+    }
+    BuildOutput bo;
+    if (!build_aabb_bvh(boxes.data(), n, n > 4096 ? default_threads() : 1, bo))
+        return fail(ATLAS_RT_E_EMPTY, "atlas_rt_world_update: TLAS build failed");
+    // Instances in TLAS leaf order, nextInstance chaining within a leaf
+    // (RayTracingWorld.cpp:287-295).
+    const size_t m = bo.ref_idx.size();
+    std::vector<atlas_rt_gpu_bvh_instance> packed(m);
+    std::vector<uint32_t> order(m);
+    for (size_t i = 0; i < m; i++) {
+        const atlas_rt_instance_desc& d = inst[bo.ref_idx[i]];
+        atlas_rt_gpu_bvh_instance& g = packed[i];
+        std::memcpy(g.inverse_matrix, d.world_to_object, sizeof(g.inverse_matrix));
+        g.mesh_offset = int32_t(d.blas);
+        g.material_offset = d.material_offset;
+        g.next_instance = bo.end_of_node[i] ? -1 : int32_t(i) + 1;
+        g.mask = d.mask;
+        order[i] = bo.ref_idx[i];
+    }
+    w->free_device();
+    w->valid = false;
+    atlas_rt_status st = upload_vec(w->d_tlas, bo.nodes);
+    if (st == ATLAS_RT_OK) st = upload_vec(w->d_instances, packed);
+    if (st == ATLAS_RT_OK) st = upload_vec(w->d_table, table);
+    if (st == ATLAS_RT_OK) st = upload_vec(w->d_order, order);
+    if (st != ATLAS_RT_OK) {
+        w->free_device();
+        return st;
+    }
+    w->tlas = std::move(bo.nodes);
+    w->instances = std::move(packed);
+    w->order = std::move(order);
+    w->table = std::move(table);
+    w->tlas_depth = bo.max_depth;
+    w->max_depth = bo.max_depth + blas_depth;
+    w->device = dev;
+    w->valid = true;
+    return ATLAS_RT_OK;
+}
+
+atlas_rt_status atlas_rt_world_info(const atlas_rt_world* w, uint32_t* tlas_node_count, uint32_t* instance_count,
+                                    uint32_t* max_depth) {
+    if (!w) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_world_info: NULL world");
+    if (tlas_node_count) *tlas_node_count = uint32_t(w->tlas.size());
+    if (instance_count) *instance_count = uint32_t(w->instances.size());
+    if (max_depth) *max_depth = w->max_depth;
+    return ATLAS_RT_OK;
+}
+
+atlas_rt_status atlas_rt_world_export(const atlas_rt_world* w, atlas_rt_gpu_bvh_node* tlas_nodes,
+                                      atlas_rt_gpu_bvh_instance* instances, uint32_t* instance_order) {
+    if (!w) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_world_export: NULL world");
+    if (tlas_nodes && !w->tlas.empty()) std::memcpy(tlas_nodes, w->tlas.data(), w->tlas.size() * sizeof(*tlas_nodes));
+    if (instances && !w->instances.empty())
+        std::memcpy(instances, w->instances.data(), w->instances.size() * sizeof(*instances));
+    if (instance_order && !w->order.empty())
+        std::memcpy(instance_order, w->order.data(), w->order.size() * sizeof(uint32_t));
+    return ATLAS_RT_OK;
+}
+
+atlas_rt_status atlas_rt_trace_closest(const atlas_rt_world* w, const atlas_rt_packed_ray* d_in,
+                                       atlas_rt_packed_ray* d_out, const uint32_t* d_count, uint32_t max_count,
+                                       float tmin, float tmax, uint32_t mask, void* stream) {
+    atlas_rt_status st = check_world(w, "atlas_rt_trace_closest");
+    if (st != ATLAS_RT_OK) return st;
+    if (max_count && (!d_in || !d_out)) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_trace_closest: NULL buffer");
+    return launch_world_closest_packed(*const_cast<atlas_rt_world*>(w), d_in, d_out, d_count, max_count, tmin, tmax,
+                                       mask, reinterpret_cast<hipStream_t>(stream));
+}
+
+atlas_rt_status atlas_rt_trace_any(const atlas_rt_world* w, const atlas_rt_packed_ray* d_in, uint32_t* d_occluded,
+                                   const uint32_t* d_count, uint32_t max_count, float tmin, float tmax, uint32_t mask,
+                                   uint32_t flags, void* stream) {
+    atlas_rt_status st = check_world(w, "atlas_rt_trace_any");
+    if (st != ATLAS_RT_OK) return st;
+    if (max_count && (!d_in || !d_occluded)) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_trace_any: NULL buffer");
+    if (flags & ~ATLAS_RT_ANY_RESTORE_ALWAYS) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_trace_any: unknown flags");
+    return launch_world_any_packed(*const_cast<atlas_rt_world*>(w), d_in, d_occluded, d_count, max_count, tmin, tmax,
+                                   mask, flags, reinterpret_cast<hipStream_t>(stream));
+}
+
+atlas_rt_status atlas_rt_world_trace_closest_rays(const atlas_rt_world* w, const atlas_rt_ray* d_rays,
+                                                  atlas_rt_hit* d_hits, int32_t* d_instance, uint32_t count,
+                                                  uint32_t mask, void* stream) {
+    atlas_rt_status st = check_world(w, "atlas_rt_world_trace_closest_rays");
+    if (st != ATLAS_RT_OK) return st;
+    if (count && (!d_rays || !d_hits))
+        return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_world_trace_closest_rays: NULL buffer");
+    return launch_world_closest_rays(*const_cast<atlas_rt_world*>(w), d_rays, d_hits, d_instance, count, mask,
+                                     reinterpret_cast<hipStream_t>(stream));
+}
+
+atlas_rt_status atlas_rt_world_trace_any_rays(const atlas_rt_world* w, const atlas_rt_ray* d_rays, uint32_t* d_hit,
+                                              uint32_t count, uint32_t mask, uint32_t flags, void* stream) {
+    atlas_rt_status st = check_world(w, "atlas_rt_world_trace_any_rays");
+    if (st != ATLAS_RT_OK) return st;
+    if (count && (!d_rays || !d_hit)) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_world_trace_any_rays: NULL buffer");
+    if (flags & ~ATLAS_RT_ANY_RESTORE_ALWAYS)
+        return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_world_trace_any_rays: unknown flags");
+    return launch_world_any_rays(*const_cast<atlas_rt_world*>(w), d_rays, d_hit, count, mask, flags,
+                                 reinterpret_cast<hipStream_t>(stream));
+}
+
+}  // extern "C"

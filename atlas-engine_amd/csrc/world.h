@@ -1,0 +1,59 @@
+// Two-level world: TLAS nodes, packed instances and the BLAS table.
+#pragma once
+
+#include <vector>
+
+#include "rt_internal.h"
+
+namespace atlas_rt {
+
+// Device view of a world.
+struct WorldDev {
+    const float4* tlas;       // engine GPUBVHNode, 4 float4 per node
+    const float4* instances;  // GPUBVHInstance, 4 float4 per instance (3 matrix columns + meta)
+    const BlasDev* blases;    // BLAS table indexed by GPUBVHInstance.meshOffset
+    const int32_t* inst_order;  // packed instance slot -> caller instance index
+};
+
+atlas_rt_status launch_world_closest_rays(atlas_rt_world& w, const void* rays, void* hits, int32_t* inst,
+                                          uint32_t count, uint32_t mask, hipStream_t stream);
+atlas_rt_status launch_world_any_rays(atlas_rt_world& w, const void* rays, uint32_t* out, uint32_t count,
+                                      uint32_t mask, uint32_t flags, hipStream_t stream);
+atlas_rt_status launch_world_closest_packed(atlas_rt_world& w, const void* in, void* out, const uint32_t* d_count,
+                                            uint32_t max_count, float tmin, float tmax, uint32_t mask,
+                                            hipStream_t stream);
+atlas_rt_status launch_world_any_packed(atlas_rt_world& w, const void* in, uint32_t* out, const uint32_t* d_count,
+                                        uint32_t max_count, float tmin, float tmax, uint32_t mask, uint32_t flags,
+                                        hipStream_t stream);
+
+}  // namespace atlas_rt
+
+struct atlas_rt_world {
+    // Host copies (engine layout, TLAS leaf order).
+    std::vector<atlas_rt_gpu_bvh_node> tlas;
+    std::vector<atlas_rt_gpu_bvh_instance> instances;
+    std::vector<uint32_t> order;  // packed slot -> caller instance index
+    std::vector<atlas_rt::BlasDev> table;
+    uint32_t tlas_depth = 0;
+    uint32_t max_depth = 0;  // tlas depth + deepest BLAS
+    bool valid = false;
+
+    int device = -1;
+    void* d_tlas = nullptr;
+    void* d_instances = nullptr;
+    void* d_table = nullptr;
+    void* d_order = nullptr;
+    atlas_rt::Scratch spill;
+    atlas_rt::Scratch flags;
+    atlas_rt::LaunchTiming timing;
+
+    atlas_rt::WorldDev dev() const {
+        atlas_rt::WorldDev d;
+        d.tlas = reinterpret_cast<const float4*>(d_tlas);
+        d.instances = reinterpret_cast<const float4*>(d_instances);
+        d.blases = reinterpret_cast<const atlas_rt::BlasDev*>(d_table);
+        d.inst_order = reinterpret_cast<const int32_t*>(d_order);
+        return d;
+    }
+    void free_device();
+};

@@ -222,6 +222,82 @@ atlas_rt_status atlas_rt_trace_any_host(const atlas_rt_blas* blas, const atlas_r
  * HIP events recorded around the kernel on its own stream), milliseconds. */
 atlas_rt_status atlas_rt_last_kernel_ms(const atlas_rt_blas* blas, float* ms);
 
+/* ------------------------------------------------------------------------- */
+/* World: TLAS over instances of BLASes (RayTracingWorld)                    */
+/* ------------------------------------------------------------------------- */
+typedef struct atlas_rt_world atlas_rt_world;
+
+/* One instance, as RayTracingWorld::Update reads it from a mesh + transform
+ * component (RayTracingWorld.cpp:91-114). */
+typedef struct atlas_rt_instance_desc {
+    float object_to_world[3][4]; /* rows 0..2 of TransformComponent::globalMatrix */
+    float world_to_object[3][4]; /* rows 0..2 of inverseGlobalMatrix (packed as GPUBVHInstance.inverseMatrix) */
+    uint32_t blas;               /* index into the BLAS table passed to update */
+    int32_t material_offset;
+    uint32_t mask;               /* ATLAS_RT_MASK_ALL | (castShadow ? ATLAS_RT_MASK_SHADOW : 0) */
+    uint32_t padding;
+} atlas_rt_instance_desc; /* 112 B */
+
+atlas_rt_status atlas_rt_world_create(atlas_rt_world** out);
+atlas_rt_status atlas_rt_world_destroy(atlas_rt_world* world);
+
+/* Replaces RayTracingWorld::Update + UpdateForSoftwareRayTracing
+ * (RayTracingWorld.cpp:32-162, 267-307): builds the TLAS over the instances'
+ * world AABBs with the engine's AABB builder (BVH.cpp:58-101, 64 bins, one
+ * instance per leaf), packs GPUBVHInstance records in TLAS leaf order and
+ * uploads TLAS nodes, instances and the BLAS table.  world_aabbs may be NULL:
+ * then each instance's AABB is its BLAS root bounds transformed by
+ * object_to_world (8 corners, AABB::Transform, AABB.cpp:35-60).  Not
+ * concurrent with traces on the same world. */
+atlas_rt_status atlas_rt_world_update(atlas_rt_world* world, const atlas_rt_instance_desc* instances,
+                                      uint32_t instance_count, const atlas_rt_aabb* world_aabbs,
+                                      const atlas_rt_blas* const* blases, uint32_t blas_count);
+
+atlas_rt_status atlas_rt_world_info(const atlas_rt_world* world, uint32_t* tlas_node_count,
+                                    uint32_t* instance_count, uint32_t* max_depth);
+/* Copies the packed TLAS nodes and GPUBVHInstance records (TLAS leaf order)
+ * and, per packed instance, the caller's instance index. */
+atlas_rt_status atlas_rt_world_export(const atlas_rt_world* world, atlas_rt_gpu_bvh_node* tlas_nodes,
+                                      atlas_rt_gpu_bvh_instance* instances, uint32_t* instance_order);
+
+/* Engine-shaped trace, RayTracingHelper::DispatchHitClosest + traceClosest.csh
+ * (RayTracingHelper.cpp:224-364, traceClosest.csh:12-36): for index <
+ * *d_count (read on the device; no host sync), unpack the PackedRay, set
+ * hitID = -1, and if ray.ID >= 0 run HitClosest(ray, mask, tmin, tmax);
+ * write PackedRay with hit = (t, hitID = BVH slot, hitInstanceID).
+ * max_count bounds the grid (the engine's ray buffer size).  d_count may be
+ * NULL, then max_count rays are traced. */
+atlas_rt_status atlas_rt_trace_closest(const atlas_rt_world* world, const atlas_rt_packed_ray* d_in,
+                                       atlas_rt_packed_ray* d_out, const uint32_t* d_count, uint32_t max_count,
+                                       float tmin, float tmax, uint32_t mask, void* stream);
+
+/* Any-hit traversal flavours.
+ * ATLAS_RT_ANY_ENGINE_HITANY: HitAny (bvh.hsh:359-441) bit for bit, including
+ *   that it restores the world-space ray only when returning from an entered
+ *   instance (bvh.hsh:384-387): after an instance rejected by the mask, the
+ *   TLAS walk continues with that instance's object-space ray.
+ * ATLAS_RT_ANY_RESTORE_ALWAYS: HitAnyTransparency's traversal (bvh.hsh:466-470),
+ *   which restores on every TLAS step; for opaque triangles its answer is
+ *   "any accepted triangle in (tmin, tmax) among admitted instances".  The
+ *   path tracer's shadow rays use it (rayHit.csh:333). */
+#define ATLAS_RT_ANY_ENGINE_HITANY 0u
+#define ATLAS_RT_ANY_RESTORE_ALWAYS 1u
+
+/* Any hit per PackedRay; d_occluded[i] = 1 if occluded. */
+atlas_rt_status atlas_rt_trace_any(const atlas_rt_world* world, const atlas_rt_packed_ray* d_in,
+                                   uint32_t* d_occluded, const uint32_t* d_count, uint32_t max_count, float tmin,
+                                   float tmax, uint32_t mask, uint32_t flags, void* stream);
+
+/* Compact two-level traces (per-ray tmin/tmax): hits carry the caller's
+ * triangle index in the hit BLAS, d_instance (may be NULL) the caller's
+ * instance index, -1 on a miss. */
+atlas_rt_status atlas_rt_world_trace_closest_rays(const atlas_rt_world* world, const atlas_rt_ray* d_rays,
+                                                  atlas_rt_hit* d_hits, int32_t* d_instance, uint32_t count,
+                                                  uint32_t mask, void* stream);
+atlas_rt_status atlas_rt_world_trace_any_rays(const atlas_rt_world* world, const atlas_rt_ray* d_rays,
+                                              uint32_t* d_hit, uint32_t count, uint32_t mask, uint32_t flags,
+                                              void* stream);
+
 #ifdef __cplusplus
 }
 #endif

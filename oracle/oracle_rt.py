@@ -61,12 +61,12 @@ def lib():
         L.oracle_bvh_depth.restype = u32
         L.oracle_bvh_depth.argtypes = [vp, u32]
         L.oracle_trace_closest.restype = None
-        L.oracle_trace_closest.argtypes = [vp, u32, vp, vp, u32, vp, u32, vp, vp, vp, vp, i32, i32]
+        L.oracle_trace_closest.argtypes = [vp, u32, vp, vp, u32, vp, u32, vp, vp, vp, vp, vp, i32, i32]
         L.oracle_trace_any.restype = None
         L.oracle_trace_any.argtypes = [vp, u32, vp, u32, vp, u32, vp, vp, vp, i32, i32]
         L.oracle_trace_world.restype = None
         L.oracle_trace_world.argtypes = [vp, u32, vp, u32, vp, vp, vp, vp, vp, u32, vp, u32, u32, i32,
-                                         vp, vp, vp, vp, vp, vp, i32]
+                                         vp, vp, vp, vp, vp, vp, vp, i32]
         L.oracle_triangle_test.restype = i32
         L.oracle_triangle_test.argtypes = [vp, vp, vp, vp, vp, vp]
         L.oracle_slab_gpu.restype = i32
@@ -135,21 +135,32 @@ def build_aabbs(aabbs):
     return OracleBVH(nodes, None, ref, eon)
 
 
-def trace_closest(bvh_nodes, bvh_tris, orig_idx, rays, mode=MODE_RGPU, threads=None, stats=False):
-    """Closest hit on the packed arrays.  Returns (hits, slots[, node_visits, tri_tests])."""
+def trace_closest(bvh_nodes, bvh_tris, orig_idx, rays, mode=MODE_RGPU, threads=None, stats=False, raw_t=False):
+    """Closest hit on the packed arrays.
+
+    Returns (hits, slots[, raw_t][, node_visits, tri_tests]).  hits carry the
+    refined (t, u, v) of the selected triangle; raw_t is the engine's float32
+    hit distance that drove the traversal.
+    """
     rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+    bvh_nodes = np.ascontiguousarray(bvh_nodes, dtype=NODE_DTYPE)
+    bvh_tris = np.ascontiguousarray(bvh_tris, dtype=GPU_TRI_DTYPE)
     n = len(rays)
     hits = np.zeros(n, dtype=HIT_DTYPE)
     slots = np.zeros(n, dtype=np.int32)
+    rt = np.zeros(n, dtype=np.float32) if raw_t else None
     nv = np.zeros(n, dtype=np.uint32) if stats else None
     tt = np.zeros(n, dtype=np.uint32) if stats else None
     orig = None if orig_idx is None else np.ascontiguousarray(orig_idx, dtype=np.uint32)
     lib().oracle_trace_closest(_p(bvh_nodes) if len(bvh_nodes) else None, len(bvh_nodes), _p(bvh_tris), _p(orig),
-                               len(bvh_tris), _p(rays), n, _p(hits), _p(slots), _p(nv), _p(tt), mode,
+                               len(bvh_tris), _p(rays), n, _p(hits), _p(slots), _p(rt), _p(nv), _p(tt), mode,
                                threads or threads_default())
+    out = (hits, slots)
+    if raw_t:
+        out += (rt,)
     if stats:
-        return hits, slots, nv, tt
-    return hits, slots
+        out += (nv, tt)
+    return out
 
 
 def trace_any(bvh_nodes, bvh_tris, rays, mode=MODE_RGPU, threads=None, stats=False):
@@ -165,8 +176,19 @@ def trace_any(bvh_nodes, bvh_tris, rays, mode=MODE_RGPU, threads=None, stats=Fal
     return hit
 
 
-def trace_world(tlas_nodes, instances, blases, rays, mask=1 << 7, any_hit=False, threads=None, stats=False):
-    """Two-level R-GPU trace.  blases: list of (nodes, tris, orig_idx) packed arrays."""
+WORLD_CLOSEST, WORLD_ANY, WORLD_BRUTE, WORLD_ANY_RESTORE = 0, 1, 2, 3
+
+
+def trace_world(tlas_nodes, instances, blases, rays, mask=1 << 7, any_hit=False, threads=None, stats=False,
+                mode=None, raw_t=False):
+    """Two-level trace.  blases: list of (nodes, tris, orig_idx) packed arrays.
+
+    mode: WORLD_CLOSEST (R-GPU HitClosest), WORLD_ANY (HitAny; hits.prim_id =
+    1/0) or WORLD_BRUTE (every instance x triangle).  Returns (hits, slots,
+    instance slots[, raw_t][, node_visits, tri_tests, instance_visits]).
+    """
+    if mode is None:
+        mode = WORLD_ANY if any_hit else WORLD_CLOSEST
     rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
     n = len(rays)
     k = len(blases)
@@ -191,28 +213,36 @@ def trace_world(tlas_nodes, instances, blases, rays, mask=1 << 7, any_hit=False,
     hits = np.zeros(n, dtype=HIT_DTYPE)
     slots = np.zeros(n, dtype=np.int32)
     inst = np.zeros(n, dtype=np.int32)
+    rt = np.zeros(n, dtype=np.float32) if raw_t else None
     nv = np.zeros(n, dtype=np.uint32) if stats else None
     tt = np.zeros(n, dtype=np.uint32) if stats else None
     iv = np.zeros(n, dtype=np.uint32) if stats else None
     lib().oracle_trace_world(_p(tlas_nodes), len(tlas_nodes), _p(instances), len(instances),
                              C.cast(node_ptrs, C.c_void_p), _p(node_counts), C.cast(tri_ptrs, C.c_void_p),
                              C.cast(orig_ptrs, C.c_void_p), _p(tri_counts), k, _p(rays), n, mask,
-                             1 if any_hit else 0, _p(hits), _p(slots), _p(inst), _p(nv), _p(tt), _p(iv),
+                             mode, _p(hits), _p(slots), _p(inst), _p(rt), _p(nv), _p(tt), _p(iv),
                              threads or threads_default())
+    out = (hits, slots, inst)
+    if raw_t:
+        out += (rt,)
     if stats:
-        return hits, slots, inst, nv, tt, iv
-    return hits, slots, inst
+        out += (nv, tt, iv)
+    return out
+
+
+def _f32(*xs):
+    return [np.ascontiguousarray(x, dtype=np.float32) for x in xs]
 
 
 def triangle_test(o, d, v0, v1, v2):
-    f = lambda x: np.ascontiguousarray(x, dtype=np.float32)
+    args = _f32(o, d, v0, v1, v2)  # keep the arrays alive across the call
     sol = np.zeros(3, dtype=np.float32)
-    h = lib().oracle_triangle_test(_p(f(o)), _p(f(d)), _p(f(v0)), _p(f(v1)), _p(f(v2)), _p(sol))
+    h = lib().oracle_triangle_test(*[_p(a) for a in args], _p(sol))
     return bool(h), sol
 
 
 def slab_test(o, d, bmin, bmax, tmin, tmax):
-    f = lambda x: np.ascontiguousarray(x, dtype=np.float32)
+    args = _f32(o, d, bmin, bmax)
     dist = np.zeros(1, dtype=np.float32)
-    h = lib().oracle_slab_gpu(_p(f(o)), _p(f(d)), _p(f(bmin)), _p(f(bmax)), tmin, tmax, _p(dist))
+    h = lib().oracle_slab_gpu(*[_p(a) for a in args], tmin, tmax, _p(dist))
     return bool(h), float(dist[0])

@@ -33,6 +33,7 @@ struct HitRec {
     float t = 0, u = 0, v = 0;
     int32_t slot = -1;      // engine hitID (BVH slot in the BLAS)
     int32_t instance = -1;  // engine hitInstanceID (TLAS instance slot)
+    float ho[3] = {0, 0, 0}, hd[3] = {0, 0, 0};  // object-space ray of the accepted test
 };
 
 struct BlasView {
@@ -262,6 +263,8 @@ static void closest_rgpu_world(const World& w, V3 o0, V3 d0, float tmin, float t
                         h.v = sol.z;
                         h.slot = triPtr;
                         h.instance = curInstance;
+                        h.ho[0] = o.x; h.ho[1] = o.y; h.ho[2] = o.z;
+                        h.hd[0] = d.x; h.hd[1] = d.y; h.hd[2] = d.z;
                     }
                     triPtr++;
                 }
@@ -283,8 +286,14 @@ static void closest_rgpu_world(const World& w, V3 o0, V3 d0, float tmin, float t
     }
 }
 
+// restore_always = false: HitAny (bvh.hsh:382-387) restores the world ray only
+// when returning from an entered instance, so after an instance rejected by
+// the mask the TLAS walk continues with that instance's object-space ray.
+// restore_always = true: HitAnyTransparency's traversal (bvh.hsh:466-470),
+// which restores on every TLAS step; with opaque triangles its result is this
+// any-hit test.
 static bool any_rgpu_world(const World& w, V3 o0, V3 d0, float tmin, float tmax, uint32_t mask,
-                           TraceStats& st, std::vector<int32_t>& stack) {
+                           TraceStats& st, std::vector<int32_t>& stack, bool restore_always) {
     if (is_nan3(d0)) return false;
     stack.clear();
     V3 o = o0, d = d0;
@@ -300,7 +309,7 @@ static bool any_rgpu_world(const World& w, V3 o0, V3 d0, float tmin, float tmax,
     for (;;) {
         uint32_t sp = uint32_t(stack.size()) + 1;
         if (sp < tlasIndex) {
-            if (tlasIndex != kTlasInvalid) {
+            if (restore_always || tlasIndex != kTlasInvalid) {
                 o = o0;
                 d = d0;
             }
@@ -479,6 +488,38 @@ static bool any_rcpu_blas(const BlasView& b, V3 o, V3 d, float tmin, float tmax,
     return false;
 }
 
+// Brute force over every instance (TLAS slot order) and every triangle of its
+// BLAS (slot order) in the instance's object space, with CheckInstance's ray
+// transform and mask test: accept t > tmin, t < best.
+static void closest_brute_world(const World& w, V3 o0, V3 d0, float tmin, float tmax, uint32_t mask, HitRec& h) {
+    h = HitRec();
+    h.t = tmax;
+    if (is_nan3(d0)) return;
+    for (uint32_t k = 0; k < w.inst_count; k++) {
+        const auto& in = w.inst[k];
+        if (!((in.mask & mask) > 0)) continue;
+        V3 o{dot4(o0.x, o0.y, o0.z, 1.0f, in.inverse_matrix[0]), dot4(o0.x, o0.y, o0.z, 1.0f, in.inverse_matrix[1]),
+             dot4(o0.x, o0.y, o0.z, 1.0f, in.inverse_matrix[2])};
+        V3 d{dot4(d0.x, d0.y, d0.z, 0.0f, in.inverse_matrix[0]), dot4(d0.x, d0.y, d0.z, 0.0f, in.inverse_matrix[1]),
+             dot4(d0.x, d0.y, d0.z, 0.0f, in.inverse_matrix[2])};
+        const BlasView& b = w.blas[in.mesh_offset];
+        for (uint32_t i = 0; i < b.tri_count; i++) {
+            const auto& tri = b.tris[i];
+            V3 sol;
+            bool hit = tri_test(o, d, tri_v(tri, 0), tri_v(tri, 1), tri_v(tri, 2), sol);
+            if (hit && sol.x > tmin && sol.x < h.t) {
+                h.t = sol.x;
+                h.u = sol.y;
+                h.v = sol.z;
+                h.slot = int32_t(i);
+                h.instance = int32_t(k);
+                h.ho[0] = o.x; h.ho[1] = o.y; h.ho[2] = o.z;
+                h.hd[0] = d.x; h.hd[1] = d.y; h.hd[2] = d.z;
+            }
+        }
+    }
+}
+
 // Brute force over every triangle in slot order: accept t > tmin, t < best.
 static void closest_brute(const BlasView& b, V3 o, V3 d, float tmin, float tmax, HitRec& h) {
     h = HitRec();
@@ -520,21 +561,47 @@ static void parallel_rays(uint32_t n, int threads, F&& f) {
 
 using namespace oracle;
 
-static inline void write_hit(const HitRec& h, const uint32_t* orig, atlas_rt_hit* out, int32_t* slot_out,
-                             uint32_t i) {
-    out[i].t = h.t;
-    out[i].u = h.slot >= 0 ? h.u : 0.0f;
-    out[i].v = h.slot >= 0 ? h.v : 0.0f;
-    out[i].prim_id = h.slot >= 0 ? (orig ? int32_t(orig[h.slot]) : h.slot) : -1;
+// Reported (t, u, v) of a hit: the engine's triangle formula evaluated in
+// double from the float32 vertices and ray (fixed order, no contraction),
+// rounded once to float.  Hit selection never uses it.
+static inline void refine_tuv(const float* o, const float* dir, const atlas_rt_gpu_bvh_triangle& tri, float& t,
+                              float& u, float& v) {
+    const double ax = tri.v0[0], ay = tri.v0[1], az = tri.v0[2];
+    const double e0x = (double)tri.v1[0] - ax, e0y = (double)tri.v1[1] - ay, e0z = (double)tri.v1[2] - az;
+    const double e1x = (double)tri.v2[0] - ax, e1y = (double)tri.v2[1] - ay, e1z = (double)tri.v2[2] - az;
+    const double sx = (double)o[0] - ax, sy = (double)o[1] - ay, sz = (double)o[2] - az;
+    const double Dx = dir[0], Dy = dir[1], Dz = dir[2];
+    const double px = sy * e0z - e0y * sz, py = sz * e0x - e0z * sx, pz = sx * e0y - e0x * sy;
+    const double qx = Dy * e1z - e1y * Dz, qy = Dz * e1x - e1z * Dx, qz = Dx * e1y - e1x * Dy;
+    const double den = (qx * e0x + qy * e0y) + qz * e0z;
+    t = (float)(((px * e1x + py * e1y) + pz * e1z) / den);
+    u = (float)(((qx * sx + qy * sy) + qz * sz) / den);
+    v = (float)(((px * Dx + py * Dy) + pz * Dz) / den);
+}
+
+static inline void write_hit(const HitRec& h, const atlas_rt_gpu_bvh_triangle* tris, const uint32_t* orig,
+                             const atlas_rt_ray& r, atlas_rt_hit* out, int32_t* slot_out, float* raw_t, uint32_t i) {
+    if (h.slot >= 0) {
+        refine_tuv(r.origin, r.direction, tris[h.slot], out[i].t, out[i].u, out[i].v);
+        out[i].prim_id = orig ? int32_t(orig[h.slot]) : h.slot;
+    } else {
+        out[i].t = r.tmax;
+        out[i].u = 0.0f;
+        out[i].v = 0.0f;
+        out[i].prim_id = -1;
+    }
     if (slot_out) slot_out[i] = h.slot;
+    if (raw_t) raw_t[i] = h.t;
 }
 
 extern "C" {
 
 // mode: 0 = R-GPU closest, 1 = R-CPU closest, 2 = brute force.
+// hits get the refined (t, u, v); raw_t (may be NULL) gets the engine's float32
+// hit distance (tmax on a miss).
 void oracle_trace_closest(const atlas_rt_gpu_bvh_node* nodes, uint32_t node_count,
                           const atlas_rt_gpu_bvh_triangle* tris, const uint32_t* orig_idx, uint32_t tri_count,
-                          const atlas_rt_ray* rays, uint32_t n, atlas_rt_hit* hits, int32_t* slots,
+                          const atlas_rt_ray* rays, uint32_t n, atlas_rt_hit* hits, int32_t* slots, float* raw_t,
                           uint32_t* node_visits, uint32_t* tri_tests, int32_t mode, int32_t threads) {
     BlasView b{nodes, node_count, tris, tri_count};
     parallel_rays(n, threads, [&](uint32_t lo, uint32_t hi) {
@@ -554,7 +621,7 @@ void oracle_trace_closest(const atlas_rt_gpu_bvh_node* nodes, uint32_t node_coun
                 closest_rcpu_blas(b, o, d, r.tmin, r.tmax, h, st, cstack);
             else
                 closest_brute(b, o, d, r.tmin, r.tmax, h);
-            write_hit(h, orig_idx, hits, slots, i);
+            write_hit(h, tris, orig_idx, r, hits, slots, raw_t, i);
             if (node_visits) node_visits[i] = st.node_visits;
             if (tri_tests) tri_tests[i] = st.tri_tests;
         }
@@ -584,17 +651,24 @@ void oracle_trace_any(const atlas_rt_gpu_bvh_node* nodes, uint32_t node_count,
     });
 }
 
-// Two-level R-GPU trace.  blas_nodes/blas_tris/... are arrays of per-BLAS
-// pointers/counts.  Outputs: hits (prim_id = orig index in the hit BLAS),
-// slots (engine hitID), instances (engine hitInstanceID).  any != 0 runs HitAny
-// and writes only hits[i].prim_id = 1/0.
+// Two-level trace.  blas_nodes/blas_tris/... are arrays of per-BLAS
+// pointers/counts.  mode 0: R-GPU HitClosest, 1: R-GPU HitAny (writes only
+// hits[i].prim_id = 1/0), 2: brute force over instances x triangles,
+// 3: any hit with HitAnyTransparency's traversal (world ray restored on
+// every TLAS step; opaque triangles).
+// Closest outputs: hits (refined t, u, v from the object-space ray of the
+// accepted test; prim_id = orig index in the hit BLAS), slots (engine hitID),
+// instances (engine hitInstanceID = TLAS instance slot), raw_t (engine
+// float32 hit distance, tmax on a miss).
 void oracle_trace_world(const atlas_rt_gpu_bvh_node* tlas, uint32_t tlas_count,
                         const atlas_rt_gpu_bvh_instance* inst, uint32_t inst_count,
                         const atlas_rt_gpu_bvh_node* const* blas_nodes, const uint32_t* blas_node_counts,
                         const atlas_rt_gpu_bvh_triangle* const* blas_tris, const uint32_t* const* blas_orig,
                         const uint32_t* blas_tri_counts, uint32_t blas_count, const atlas_rt_ray* rays, uint32_t n,
-                        uint32_t mask, int32_t any, atlas_rt_hit* hits, int32_t* slots, int32_t* instances,
-                        uint32_t* node_visits, uint32_t* tri_tests, uint32_t* inst_visits, int32_t threads) {
+                        uint32_t mask, int32_t mode, atlas_rt_hit* hits, int32_t* slots, int32_t* instances,
+                        float* raw_t, uint32_t* node_visits, uint32_t* tri_tests, uint32_t* inst_visits,
+                        int32_t threads) {
+    const int32_t any = mode == 1 || mode == 3;
     std::vector<BlasView> views(blas_count);
     for (uint32_t k = 0; k < blas_count; k++)
         views[k] = BlasView{blas_nodes[k], blas_node_counts[k], blas_tris[k], blas_tri_counts[k]};
@@ -607,21 +681,32 @@ void oracle_trace_world(const atlas_rt_gpu_bvh_node* tlas, uint32_t tlas_count,
             V3 d{r.direction[0], r.direction[1], r.direction[2]};
             TraceStats st;
             if (any) {
-                bool h = any_rgpu_world(w, o, d, r.tmin, r.tmax, mask, st, stack);
+                bool h = any_rgpu_world(w, o, d, r.tmin, r.tmax, mask, st, stack, mode == 3);
                 hits[i].t = 0.0f;
                 hits[i].u = 0.0f;
                 hits[i].v = 0.0f;
                 hits[i].prim_id = h ? 1 : 0;
             } else {
                 HitRec h;
-                closest_rgpu_world(w, o, d, r.tmin, r.tmax, mask, h, st, stack);
-                const uint32_t* orig = (h.instance >= 0) ? blas_orig[inst[h.instance].mesh_offset] : nullptr;
-                hits[i].t = h.t;
-                hits[i].u = h.slot >= 0 ? h.u : 0.0f;
-                hits[i].v = h.slot >= 0 ? h.v : 0.0f;
-                hits[i].prim_id = h.slot >= 0 ? (orig ? int32_t(orig[h.slot]) : h.slot) : -1;
+                if (mode == 2)
+                    closest_brute_world(w, o, d, r.tmin, r.tmax, mask, h);
+                else
+                    closest_rgpu_world(w, o, d, r.tmin, r.tmax, mask, h, st, stack);
+                // t is in world units: the object-space direction is not
+                // renormalised (bvh.hsh:177-180).
+                if (h.slot >= 0) {
+                    const auto& bt = views[inst[h.instance].mesh_offset].tris[h.slot];
+                    refine_tuv(h.ho, h.hd, bt, hits[i].t, hits[i].u, hits[i].v);
+                    const uint32_t* orig = blas_orig[inst[h.instance].mesh_offset];
+                    hits[i].prim_id = orig ? int32_t(orig[h.slot]) : h.slot;
+                } else {
+                    hits[i].t = r.tmax;
+                    hits[i].u = hits[i].v = 0.0f;
+                    hits[i].prim_id = -1;
+                }
                 if (slots) slots[i] = h.slot;
                 if (instances) instances[i] = h.instance;
+                if (raw_t) raw_t[i] = h.t;
             }
             if (node_visits) node_visits[i] = st.node_visits;
             if (tri_tests) tri_tests[i] = st.tri_tests;
