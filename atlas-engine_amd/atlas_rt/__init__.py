@@ -262,3 +262,151 @@ class World:
                          flags=_abi.ANY_ENGINE_HITANY, stream=None):
         check(lib().atlas_rt_trace_any(self._h, _ptr(d_in), _ptr(d_occluded), _ptr(d_count), max_count, tmin, tmax,
                                        mask, flags, _stream_ptr(stream)))
+
+
+def build_many(meshes, host_only=False):
+    """Build one BLAS per (n, 9) vertex array, one host thread per mesh."""
+    recs = [as_bvh_triangles(v) for v in meshes]
+    k = len(recs)
+    ptrs = (C.c_void_p * k)(*[r.ctypes.data for r in recs])
+    counts = np.array([len(r) for r in recs], dtype=np.uint32)
+    out = (C.c_void_p * k)()
+    flags = _abi.BUILD_HOST_ONLY if host_only else 0
+    check(lib().atlas_rt_blas_build_many(C.cast(ptrs, C.c_void_p), _ptr(counts), k, flags, C.cast(out, C.c_void_p)))
+    return [Blas(out[i]) for i in range(k)]
+
+
+class BatchSet:
+    """Many (BLAS, ray range) pairs traced by one persistent launch."""
+
+    def __init__(self, blases, batches):
+        self._blases = list(blases)
+        self.batches = np.ascontiguousarray(batches, dtype=_abi.BATCH_DTYPE)
+        table = (C.c_void_p * len(self._blases))(*[b.handle.value for b in self._blases])
+        out = C.c_void_p()
+        check(lib().atlas_rt_batch_create(C.cast(table, C.c_void_p), len(self._blases), _ptr(self.batches),
+                                          len(self.batches), C.byref(out)))
+        self._h = out
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            check(lib().atlas_rt_batch_destroy(self._h))
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _cfg(kernel, block, refill):
+        if kernel is None:
+            return None
+        return C.byref(_abi.LaunchConfig(kernel, block or 0, refill or 0, 0))
+
+    def trace_closest(self, d_rays, d_hits, d_count=None, kernel=None, block=0, refill=0, stream=None):
+        check(lib().atlas_rt_batch_trace_closest(self._h, _ptr(d_rays), _ptr(d_hits), _ptr(d_count),
+                                                 self._cfg(kernel, block, refill), _stream_ptr(stream)))
+
+    def tuning(self):
+        best = _abi.LaunchConfig()
+        n = C.c_uint32(64)
+        trials = (_abi.LaunchConfig * 64)()
+        ms = np.zeros(64, dtype=np.float32)
+        check(lib().atlas_rt_batch_tuning(self._h, C.byref(best), C.byref(n), C.cast(trials, C.c_void_p), _ptr(ms)))
+        as_dict = lambda c: {"kernel": c.kernel, "block": c.block, "refill": c.refill}
+        return as_dict(best), [(as_dict(trials[i]), float(ms[i])) for i in range(min(n.value, 64))]
+
+    def last_kernel_ms(self):
+        ms = C.c_float()
+        check(lib().atlas_rt_batch_last_kernel_ms(self._h, C.byref(ms)))
+        return ms.value
+
+    def graph_capture(self, d_rays, d_hits, d_count=None, kernel=None, block=0, refill=0):
+        check(lib().atlas_rt_batch_graph_capture(self._h, _ptr(d_rays), _ptr(d_hits), _ptr(d_count),
+                                                 self._cfg(kernel, block, refill)))
+
+    def graph_launch(self, stream=None):
+        check(lib().atlas_rt_batch_graph_launch(self._h, _stream_ptr(stream)))
+
+
+def _blas_set_shading(self, normals, material=None):
+    """Per-triangle vertex normals ((n, 9), caller order) and material indices."""
+    normals = np.ascontiguousarray(normals, dtype=np.float32).reshape(-1, 9)
+    if material is not None:
+        material = np.ascontiguousarray(material, dtype=np.int32)
+    check(lib().atlas_rt_blas_set_shading(self._h, _ptr(normals), _ptr(material)))
+
+
+Blas.set_shading = _blas_set_shading
+
+
+def camera(eye, yaw_deg, pitch_deg, fov_y_deg, aspect, near=0.1):
+    """atlas_rt_camera from an eye point and yaw/pitch (y up), built like
+    CameraComponent::GetFrustumCorners' near-plane corners 4, 5, 6."""
+    import math
+    pitch, yaw = math.radians(pitch_deg), math.radians(yaw_deg)
+    fwd = np.array([math.cos(pitch) * math.sin(yaw), -math.sin(pitch), math.cos(pitch) * math.cos(yaw)])
+    right = np.cross(fwd, [0.0, 1.0, 0.0])
+    right /= np.linalg.norm(right)
+    up = np.cross(right, fwd)
+    h = near * math.tan(math.radians(fov_y_deg) / 2.0)
+    w = aspect * h
+    eye = np.asarray(eye, dtype=np.float64)
+    near_pt = eye + fwd * near
+    c4 = near_pt + h * up - w * right
+    c5 = near_pt + h * up + w * right
+    c6 = near_pt - h * up - w * right
+    cam = _abi.Camera()
+    cam.location[:] = eye.astype(np.float32)
+    cam.origin[:] = c4.astype(np.float32)
+    cam.right[:] = (c5 - c4).astype(np.float32)
+    cam.bottom[:] = (c6 - c4).astype(np.float32)
+    return cam
+
+
+class PathTracer:
+    """Progressive wavefront path tracer over a World (PathTracingRenderer)."""
+
+    def __init__(self, world, width, height, bounces, materials, lights, env=(0.0, 0.0, 0.0), exposure=1.0):
+        p = _abi.PtParams(width, height, bounces, exposure, (C.c_float * 3)(*env), 0)
+        self._mats = np.ascontiguousarray(materials, dtype=_abi.MATERIAL_DTYPE)
+        self._lights = np.ascontiguousarray(lights, dtype=_abi.LIGHT_DTYPE)
+        self._world = world
+        self.width, self.height = width, height
+        out = C.c_void_p()
+        check(lib().atlas_rt_pathtracer_create(world._h, C.byref(p), _ptr(self._mats), len(self._mats),
+                                               _ptr(self._lights) if len(self._lights) else None,
+                                               len(self._lights), C.byref(out)))
+        self._h = out
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            check(lib().atlas_rt_pathtracer_destroy(self._h))
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def render(self, cam, frames=1, stream=None):
+        check(lib().atlas_rt_pathtracer_render(self._h, C.byref(cam), frames, _stream_ptr(stream)))
+
+    def reset(self):
+        check(lib().atlas_rt_pathtracer_reset(self._h))
+
+    def read(self, out=None, stream=None):
+        """(height, width, 4) float32 torch tensor: rgb radiance sum, sample count."""
+        import torch
+        if out is None:
+            out = torch.empty((self.height, self.width, 4), dtype=torch.float32, device="cuda")
+        check(lib().atlas_rt_pathtracer_read(self._h, _ptr(out), _stream_ptr(stream)))
+        return out
+
+    def sample_count(self):
+        n = C.c_int32()
+        check(lib().atlas_rt_pathtracer_sample_count(self._h, C.byref(n)))
+        return n.value

@@ -60,6 +60,35 @@ INSTANCE_DESC_DTYPE = np.dtype([
     ("material_offset", "<i4"), ("mask", "<u4"), ("padding", "<u4"),
 ])
 assert INSTANCE_DESC_DTYPE.itemsize == 112
+# atlas_rt_batch, 16 B.
+BATCH_DTYPE = np.dtype([("blas", "<u4"), ("ray_offset", "<u4"), ("ray_count", "<u4"), ("reserved", "<u4")])
+
+
+class LaunchConfig(C.Structure):
+    """atlas_rt_launch_config."""
+    _fields_ = [("kernel", C.c_int32), ("block", C.c_uint32), ("refill", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+# atlas_rt_material (48 B), atlas_rt_light (32 B).
+MATERIAL_DTYPE = np.dtype([
+    ("base_color", "<f4", 3), ("opacity", "<f4"), ("emissive", "<f4", 3), ("roughness", "<f4"),
+    ("metalness", "<f4"), ("ao", "<f4"), ("reflectance", "<f4"), ("two_sided", "<i4"),
+])
+LIGHT_DTYPE = np.dtype([("direction", "<f4", 3), ("type", "<u4"), ("radiance", "<f4", 3), ("reserved", "<f4")])
+assert MATERIAL_DTYPE.itemsize == 48 and LIGHT_DTYPE.itemsize == 32
+LIGHT_DIRECTIONAL = 0
+
+
+class Camera(C.Structure):
+    """atlas_rt_camera."""
+    _fields_ = [("location", C.c_float * 3), ("origin", C.c_float * 3), ("right", C.c_float * 3),
+                ("bottom", C.c_float * 3)]
+
+
+class PtParams(C.Structure):
+    """atlas_rt_pt_params."""
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("bounces", C.c_int32), ("exposure", C.c_float),
+                ("env_radiance", C.c_float * 3), ("flags", C.c_uint32)]
 
 assert NODE_DTYPE.itemsize == 64 and GPU_TRI_DTYPE.itemsize == 48
 assert INSTANCE_DTYPE.itemsize == 64 and PACKED_RAY_DTYPE.itemsize == 48
@@ -96,4 +125,20 @@ PROTOTYPES = {
     "atlas_rt_trace_any": (status, [vp, vp, vp, vp, u32, f32, f32, u32, u32, vp]),
     "atlas_rt_world_trace_closest_rays": (status, [vp, vp, vp, vp, u32, u32, vp]),
     "atlas_rt_world_trace_any_rays": (status, [vp, vp, vp, u32, u32, u32, vp]),
+    "atlas_rt_blas_build_many": (status, [vp, vp, u32, u32, vp]),
+    "atlas_rt_batch_create": (status, [vp, u32, vp, u32, C.POINTER(vp)]),
+    "atlas_rt_batch_destroy": (status, [vp]),
+    "atlas_rt_batch_trace_closest": (status, [vp, vp, vp, vp, vp, vp]),
+    "atlas_rt_batch_tuning": (status, [vp, vp, vp, vp, vp]),
+    "atlas_rt_batch_last_kernel_ms": (status, [vp, C.POINTER(f32)]),
+    "atlas_rt_batch_graph_capture": (status, [vp, vp, vp, vp, vp]),
+    "atlas_rt_batch_graph_launch": (status, [vp, vp]),
+    "atlas_rt_aabb_tree_build": (status, [vp, u32, u32, vp, C.POINTER(u32), vp, vp, C.POINTER(u32)]),
+    "atlas_rt_blas_set_shading": (status, [vp, vp, vp]),
+    "atlas_rt_pathtracer_create": (status, [vp, vp, vp, u32, vp, u32, C.POINTER(vp)]),
+    "atlas_rt_pathtracer_destroy": (status, [vp]),
+    "atlas_rt_pathtracer_render": (status, [vp, vp, u32, vp]),
+    "atlas_rt_pathtracer_reset": (status, [vp]),
+    "atlas_rt_pathtracer_read": (status, [vp, vp, vp]),
+    "atlas_rt_pathtracer_sample_count": (status, [vp, C.POINTER(i32)]),
 }

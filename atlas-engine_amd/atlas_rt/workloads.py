@@ -186,10 +186,12 @@ def blob_mesh(g, stacks=32, slices=64, bumps=3, amplitude=0.25):
         for j in range(slices):
             a, b = pos[i, j], pos[i + 1, j]
             c, d = pos[i + 1, j + 1], pos[i, j + 1]
+            # winding chosen so the engine's geometric normal
+            # cross(v0 - v1, v0 - v2) points outwards
             if i != 0:
-                tris.append(np.concatenate([a, b, d]))
+                tris.append(np.concatenate([a, d, b]))
             if i != stacks - 1:
-                tris.append(np.concatenate([b, c, d]))
+                tris.append(np.concatenate([b, d, c]))
     verts = np.asarray(tris, dtype=np.float32)
     # smooth normals: average of the face normals around each lattice vertex
     # is approximated by the displaced sphere direction's gradient-free proxy
@@ -270,3 +272,49 @@ def shadow_rays(primary, hits, verts, light_dir=C1_LIGHT_DIR, offset=1e-3, tmin=
     out["tmin"] = np.float32(tmin)
     out["tmax"] = np.float32(tmax)
     return out
+
+
+# ---------------------------------------------------------------------------
+# Config 3: batched uneven workloads
+# ---------------------------------------------------------------------------
+def c3(n_batches=512, tri_range=(1024, 65536), ray_range=(4096, 262144), seed_salt=0):
+    """512 independent triangle soups (sizes log-uniform in tri_range, built
+    as config 0) each with its own random ray batch (log-uniform in
+    ray_range).  Returns (list of (n,9) meshes, concatenated rays, batch
+    table as (blas, ray_offset, ray_count) rows)."""
+    g = rng(3, seed_salt)
+    lt = np.log2(tri_range)
+    lr = np.log2(ray_range)
+    sizes = np.round(2.0 ** g.uniform(lt[0], lt[1], n_batches)).astype(np.int64)
+    counts = np.round(2.0 ** g.uniform(lr[0], lr[1], n_batches)).astype(np.int64)
+    meshes = [triangle_soup(int(s), g) for s in sizes]
+    rays = random_rays(int(counts.sum()), g)
+    offs = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    table = np.stack([np.arange(n_batches), offs, counts], axis=1).astype(np.uint32)
+    return meshes, rays, table
+
+
+def c2_materials(mats):
+    """atlas_rt_material records for the config-2 blob materials (Lambert +
+    GGX: metalness 0, reflectance 0.5)."""
+    from ._abi import MATERIAL_DTYPE
+    n = len(mats["roughness"])
+    out = np.zeros(n, dtype=MATERIAL_DTYPE)
+    out["base_color"] = mats["albedo"]
+    out["opacity"] = 1.0
+    out["roughness"] = mats["roughness"]
+    out["metalness"] = mats.get("metalness", np.zeros(n, np.float32))
+    out["ao"] = 1.0
+    out["reflectance"] = 0.5
+    return out
+
+
+def sun_light(direction=C2_SUN_DIR, radiance=(3.0, 2.9, 2.7)):
+    from ._abi import LIGHT_DTYPE
+    L = np.zeros(1, dtype=LIGHT_DTYPE)
+    L["direction"] = np.asarray(direction, dtype=np.float32)
+    L["radiance"] = np.asarray(radiance, dtype=np.float32)
+    return L
+
+
+C2_ENV = (0.25, 0.3, 0.4)

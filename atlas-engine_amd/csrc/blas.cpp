@@ -248,10 +248,47 @@ atlas_rt_status atlas_rt_blas_destroy(atlas_rt_blas* b) {
     if (b->d_nodes) hipFree(b->d_nodes);
     if (b->d_tris) hipFree(b->d_tris);
     if (b->d_vtx) hipFree(b->d_vtx);
+    if (b->d_shade) hipFree(b->d_shade);
     b->spill.release();
     b->flags.release();
     b->timing.release();
     delete b;
+    return ATLAS_RT_OK;
+}
+
+atlas_rt_status atlas_rt_blas_set_shading(atlas_rt_blas* b, const float* normals, const int32_t* material) {
+    if (!b || !normals) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_blas_set_shading: NULL argument");
+    if (!b->on_device) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_blas_set_shading: BLAS was built host-only");
+    // Reorder to BVH slot order (slot i holds caller triangle orig_idx[i]).
+    const size_t n = b->tris.size();
+    std::vector<float> packed(12 * n);
+    for (size_t i = 0; i < n; i++) {
+        const uint32_t src = b->orig_idx[i];
+        const float* nv = normals + 9 * size_t(src);
+        const int32_t mat = material ? material[src] : 0;
+        float* o = &packed[12 * i];
+        o[0] = nv[0];
+        o[1] = nv[1];
+        o[2] = nv[2];
+        std::memcpy(&o[3], &mat, 4);
+        o[4] = nv[3];
+        o[5] = nv[4];
+        o[6] = nv[5];
+        o[7] = 0.0f;
+        o[8] = nv[6];
+        o[9] = nv[7];
+        o[10] = nv[8];
+        o[11] = 0.0f;
+    }
+    void* d = nullptr;
+    ATLAS_HIP_TRY(hipMalloc(&d, packed.size() * sizeof(float)));
+    hipError_t e = hipMemcpy(d, packed.data(), packed.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        hipFree(d);
+        return hip_fail(e, "atlas_rt_blas_set_shading");
+    }
+    if (b->d_shade) hipFree(b->d_shade);
+    b->d_shade = reinterpret_cast<float4*>(d);
     return ATLAS_RT_OK;
 }
 

@@ -1,0 +1,61 @@
+// Host-side path tracer object and the argument block of the shade kernel.
+#pragma once
+
+#include <vector>
+
+#include "pt_types.h"
+#include "rt_internal.h"
+#include "world.h"
+
+namespace atlas_rt {
+namespace pt {
+
+struct ShadeArgs {
+    WorldDev world;
+    const float4* xform;          // object-to-world rows, 3 float4 per packed instance
+    const float4* const* shade;   // per BLAS: 3 float4 per slot (n0 + material, n1, n2)
+    const Material* materials;
+    const Light* lights;
+    int32_t light_count;
+    int32_t bounces;
+    float env[3];
+    FrameCtl* ctl;
+    const float4* paths_in;
+    float4* paths_out;
+    const int4* hits;
+    float4* shadow;
+    float4* Lpix;
+};
+
+struct PathTracer {
+    uint32_t width = 0, height = 0;
+    int32_t bounces = 0;
+    int32_t light_count = 0;
+    const atlas_rt_world* world = nullptr;
+    WorldDev world_dev{};
+    uint32_t cap = 0;
+    // device buffers
+    FrameCtl* ctl = nullptr;
+    float4* paths[2] = {nullptr, nullptr};
+    int4* hits = nullptr;
+    float4* shadow = nullptr;
+    float4* Lpix = nullptr;
+    float4* accum = nullptr;
+    float4* xform = nullptr;
+    const float4** shade_table = nullptr;
+    Material* materials = nullptr;
+    Light* lights = nullptr;
+    Scratch spill;
+    Scratch flags;
+    ShadeArgs shade_args{};
+    int32_t host_samples = 0;
+};
+
+atlas_rt_status launch_frame(PathTracer& P, const Camera& cam, hipStream_t s);
+
+}  // namespace pt
+}  // namespace atlas_rt
+
+struct atlas_rt_pathtracer {
+    atlas_rt::pt::PathTracer P;
+};

@@ -1,0 +1,358 @@
+// Wavefront path tracer kernels, gfx950.
+//
+// Reference: PathTracingRenderer::Render (src/engine/renderer/PathTracingRenderer.cpp:29-292)
+// drives rayGen.csh, then bounces+1 x (traceClosest.csh, rayHit.csh).  The
+// engine traces each shadow ray inline in the hit shader (rayHit.csh:327-339,
+// HitAnyTransparency).  Here the bounce is split MI355X-style into separate
+// wavefront kernels so that traversal and shading do not share registers:
+//   frame_begin  per-bounce seeds from the device-side PCG32 state, counters
+//   raygen       camera rays (tile-ordered, rayGen.csh:25-90)
+//   trace        closest hit of every live path (HitClosest, engine order)
+//   shade        EvaluateBounce (rayHit.csh:160-325): env on miss, surface
+//                reconstruction, emission, the sun's contribution (queued as
+//                a shadow ray with its already-clamped radiance), BSDF
+//                sample, Russian roulette, stream compaction of survivors
+//   shadow       HitAnyTransparency-style any hit; visible -> add radiance
+//   accumulate   per-pixel running sum (rayHit.csh:119-132, linear HDR)
+// With opaque materials and visibility in {0, 1} the deferred shadow test
+// adds exactly what the inline one would: the bounce clamp scales the
+// unoccluded radiance, which is then multiplied by visibility.
+// Every count lives on the device, so a whole frame can be graph-captured.
+#include <hip/hip_runtime.h>
+
+#include "pathtracer.h"
+#include "pt_device.h"
+#include "trace_world_core.h"
+
+namespace atlas_rt {
+namespace pt {
+
+namespace {
+
+constexpr int kBlock = 128;
+constexpr int kLdsDepth = 32;
+
+__device__ __forceinline__ uint32_t pcg32_next(uint64_t& state) {
+    uint64_t x = state;
+    const uint32_t count = uint32_t(x >> 59u);
+    state = x * 6364136223846793005ull + 1442695040888963407ull;
+    x ^= x >> 18;
+    const uint32_t v = uint32_t(x >> 27u);
+    return (v >> count) | (v << ((32u - count) & 31u));
+}
+
+__global__ void k_frame_begin(FrameCtl* ctl, int32_t bounces, uint32_t pixels) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int32_t b = 0; b <= bounces && b < 16; b++)
+        ctl->seeds[b] = float(pcg32_next(ctl->pcg_state)) / float(0xFFFFFFFFu) * 0.999999f;
+    ctl->paths[0] = pixels;
+    for (int i = 1; i < 17; i++) ctl->paths[i] = 0;
+    for (int i = 0; i < 16; i++) ctl->shadows[i] = 0;
+}
+
+// rayGen.csh:25-90 with one sample per frame (z = 0): jittered pinhole ray
+// through the near-plane corner frame; the write index groups 8x8 tiles.
+__global__ __launch_bounds__(kBlock) void k_raygen(Camera cam, uint32_t width, uint32_t height,
+                                                   const FrameCtl* __restrict__ ctl, float4* __restrict__ paths,
+                                                   float4* __restrict__ Lpix) {
+    const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
+    if (g >= width * height) return;
+    const uint32_t px = g % width, py = g / width;
+    const float sc = float(ctl->sample_count);
+    const float jx = rnd_pair(sc, 0.0f);
+    const float jy = rnd_pair(sc, 1.0f);
+    const float cx = (float(px) + jx) / float(width);
+    const float cy = (float(py) + jy) / float(height);
+    const V o = mk(cam.origin[0], cam.origin[1], cam.origin[2]);
+    const V r = mk(cam.right[0], cam.right[1], cam.right[2]);
+    const V b = mk(cam.bottom[0], cam.bottom[1], cam.bottom[2]);
+    const V loc = mk(cam.loc[0], cam.loc[1], cam.loc[2]);
+    const V dir = normalize(((o + r * cx) + b * cy) - loc);
+    // 8x8 tiles, row-major tiles then row-major inside a tile
+    const uint32_t tiles_x = (width + 7) / 8;
+    const uint32_t tile = (py / 8) * tiles_x + (px / 8);
+    const uint32_t slot_in = (py % 8) * 8 + (px % 8);
+    uint32_t idx = tile * 64 + slot_in;
+    // ragged right / bottom tiles are packed behind the full ones
+    if ((width % 8) || (height % 8)) idx = g;
+    const uint32_t pixel = py * width + px;
+    paths[3 * size_t(idx)] = make_float4(loc.x, loc.y, loc.z, __uint_as_float(pixel));
+    paths[3 * size_t(idx) + 1] = make_float4(dir.x, dir.y, dir.z, 0.0f);
+    paths[3 * size_t(idx) + 2] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    Lpix[pixel] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// Closest hit of live paths: hit = (engine float32 t, slot, instance slot).
+template <int BLOCK, int LDS_DEPTH>
+__global__ __launch_bounds__(BLOCK) void k_trace(WorldDev W, const uint32_t* __restrict__ count_ptr,
+                                                 const float4* __restrict__ paths, int4* __restrict__ hits,
+                                                 int32_t* __restrict__ spill, uint32_t cap,
+                                                 uint32_t* __restrict__ flags) {
+    __shared__ int32_t lds[LDS_DEPTH * BLOCK];
+    const uint32_t gid = blockIdx.x * BLOCK + threadIdx.x;
+    if (gid >= *count_ptr) return;
+    const float4 p0 = paths[3 * size_t(gid)];
+    const float4 p1 = paths[3 * size_t(gid) + 1];
+    WorldLane L;
+    world_begin(L, make_float4(p0.x, p0.y, p0.z, 0.0f), make_float4(p1.x, p1.y, p1.z, kInf));
+    int32_t slot = -1, inst = -1;
+    if (!any_nan3(p1.x, p1.y, p1.z)) {
+        const LaneStack<BLOCK, LDS_DEPTH> S{lds, spill, threadIdx.x, gid, gridDim.x * BLOCK};
+        int r;
+        do {
+            r = world_step_closest<BLOCK, LDS_DEPTH>(L, W, ATLAS_RT_MASK_ALL, S, cap);
+        } while (r == kContinue);
+        if (r == kOverflow)
+            atomicOr(&flags[2], 1u);
+        else if (L.slot >= 0) {
+            slot = L.slot;
+            inst = L.inst;
+        }
+    }
+    hits[gid] = make_int4(__float_as_int(L.hitT), slot, inst, 0);
+}
+
+__device__ __forceinline__ V row_xform(const float4* m, V p, float w) {
+    // (m0 x + m1 y) + (m2 z + m3 w) per row
+    return mk((m[0].x * p.x + m[0].y * p.y) + (m[0].z * p.z + m[0].w * w),
+              (m[1].x * p.x + m[1].y * p.y) + (m[1].z * p.z + m[1].w * w),
+              (m[2].x * p.x + m[2].y * p.y) + (m[2].z * p.z + m[2].w * w));
+}
+
+__global__ __launch_bounds__(kBlock) void k_shade(ShadeArgs A, int32_t bounce) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    FrameCtl* ctl = A.ctl;
+    if (i >= ctl->paths[bounce]) return;
+    const float4 p0 = A.paths_in[3 * size_t(i)];
+    const float4 p1 = A.paths_in[3 * size_t(i) + 1];
+    const float4 p2 = A.paths_in[3 * size_t(i) + 2];
+    const int4 h = A.hits[i];
+    const uint32_t pixel = __float_as_uint(p0.w);
+    V thr = mk(p2.x, p2.y, p2.z);
+    const V ro = mk(p0.x, p0.y, p0.z), rd = mk(p1.x, p1.y, p1.z);
+    float4 acc = A.Lpix[pixel];
+    V add = mk(0.0f, 0.0f, 0.0f);
+    V no = ro, nd = rd;
+    if (h.y < 0) {
+        // miss: min(env * throughput, 10) (rayHit.csh:166-172)
+        const V e = mk(A.env[0], A.env[1], A.env[2]) * thr;
+        add = mk(fminf(e.x, 10.0f), fminf(e.y, 10.0f), fminf(e.z, 10.0f));
+        thr = mk(0.0f, 0.0f, 0.0f);
+    } else {
+        const int32_t slot = h.y, inst = h.z;
+        const float4 meta = A.world.instances[4 * size_t(inst) + 3];
+        const int32_t mesh = __float_as_int(meta.x);
+        const int32_t mat_off = __float_as_int(meta.y);
+        const float4* M = A.xform + 3 * size_t(inst);
+        const float4* vp = A.world.blases[mesh].vtx + 3 * size_t(slot);
+        const float4* sp = A.shade[mesh] + 3 * size_t(slot);
+        const float4 a0 = vp[0], a1 = vp[1], a2 = vp[2];
+        const float4 s0 = sp[0], s1 = sp[1], s2 = sp[2];
+        // TransformTriangle (raytracer/surface.hsh:46-61)
+        const V v0 = row_xform(M, mk(a0.x, a0.y, a0.z), 1.0f);
+        const V v1 = row_xform(M, mk(a1.x, a1.y, a1.z), 1.0f);
+        const V v2 = row_xform(M, mk(a2.x, a2.y, a2.z), 1.0f);
+        const V n0 = normalize(row_xform(M, mk(s0.x, s0.y, s0.z), 0.0f));
+        const V n1 = normalize(row_xform(M, mk(s1.x, s1.y, s1.z), 0.0f));
+        const V n2 = normalize(row_xform(M, mk(s2.x, s2.y, s2.z), 0.0f));
+        const Material m = A.materials[__float_as_int(s0.w) + mat_off];
+        // re-intersect for the barycentrics (surface.hsh:72-78)
+        const V e0 = v1 - v0, e1 = v2 - v0, s = ro - v0;
+        const V p = cross(s, e0), q = cross(rd, e1);
+        const float den = dot(q, e0);
+        const float dist = dot(p, e1) / den;
+        const float bs = dot(q, s) / den;
+        const float bt = dot(p, rd) / den;
+        const float br = 1.0f - bs - bt;
+        Surface S;
+        S.m = m;
+        S.P = ro + rd * dist;
+        V normal = normalize((n0 * br + n1 * bs) + n2 * bt);
+        V Ng = normalize(cross(v0 - v1, v0 - v2));
+        const bool flip = dot(Ng, rd) > 0.0f;
+        Ng = Ng * ((flip && m.two_sided > 0) ? -1.0f : 1.0f);
+        if (dot(normal, Ng) < 0.0f && m.two_sided > 0) normal = -normal;
+        S.V_ = -rd;
+        S.N = normalize(normal);
+        S.Ng = Ng;
+        const V base = mk(m.base[0], m.base[1], m.base[2]);
+        S.F0 = mixv(mk(0.04f, 0.04f, 0.04f), base, m.metalness);
+        S.F90 = 1.0f;
+        S.L = mk(0.0f, 0.0f, 1.0f);
+        S.NdotL = 0.0f;
+        if ((m.emissive[0] + m.emissive[1]) + m.emissive[2] > 0.0f && bounce == 0)
+            add = add + mk(m.emissive[0], m.emissive[1], m.emissive[2]);
+        // EvaluateDirectLight (rayHit.csh:209-235): one light per bounce
+        if (A.light_count > 0) {
+            float seed = ctl->seeds[bounce];
+            const float ray_seed = float(i);
+            float total = 0.0f, lpdf = 1.0f;
+            int32_t sel = 0;
+            for (int32_t l = 0; l < A.light_count; l++) {
+                const Light& Lt = A.lights[l];
+                const float brightness = dot(mk(Lt.radiance[0], Lt.radiance[1], Lt.radiance[2]),
+                                             mk(0.33333f, 0.33333f, 0.33333f));
+                float w = clampf(brightness, 0.0000001f, 1.0f);
+                total += w;
+                if (rnd2(ray_seed, seed) < w / total) {
+                    lpdf = w;
+                    sel = l;
+                }
+            }
+            const Light& Lt = A.lights[sel];
+            lpdf = lpdf / total * Lt.pdf;
+            S.L = normalize(-mk(Lt.N[0], Lt.N[1], Lt.N[2]));
+            update_surface(S);
+            V refl = (eval_diffuse(S) + eval_specular(S)) * m.opacity;
+            const V rad = mk(Lt.radiance[0], Lt.radiance[1], Lt.radiance[2]);
+            if (S.NdotL > 0.0f) {
+                V direct = (refl * rad) * S.NdotL / lpdf;
+                V c = thr * m.opacity * direct;
+                if (bounce > 0) {
+                    const float rm = fmaxf(fmaxf(c.x, fmaxf(c.y, c.z)), 10.0f);
+                    c = c * (10.0f / rm);
+                }
+                const uint32_t k = atomicAdd(&ctl->shadows[bounce], 1u);
+                const V so = S.P + S.N * kEpsilon;
+                A.shadow[3 * size_t(k)] = make_float4(so.x, so.y, so.z, __uint_as_float(pixel));
+                A.shadow[3 * size_t(k) + 1] = make_float4(S.L.x, S.L.y, S.L.z, kInf - 2.0f * kEpsilon);
+                A.shadow[3 * size_t(k) + 2] = make_float4(c.x, c.y, c.z, 0.0f);
+            }
+        }
+        // EvaluateIndirectLight (rayHit.csh:237-325)
+        no = S.P;
+        float seed = ctl->seeds[bounce];
+        const float ray_seed = float(pixel);
+        float refract = clampf(1.0f - m.opacity, 0.1f, 0.9f);
+        refract = m.opacity == 1.0f ? 0.0f : refract;
+        float rnd = saturate(rnd2(ray_seed, seed));
+        BrdfSample B;
+        bool refracted = false;
+        if (rnd >= refract) {
+            rnd = rnd2(ray_seed, seed);
+            const V F = fresnel_schlick(S.F0, S.F90, S.NdotV);
+            const float spec = clampf(dot(F, mk(0.33333f, 0.33333f, 0.33333f)), 0.1f, 0.9f);
+            const float u0 = rnd2(ray_seed, seed);
+            const float u1 = rnd2(ray_seed, seed);
+            if (rnd < spec) {
+                B = sample_specular(S, u0, u1);
+                B.reflectance = B.reflectance * m.opacity;
+                B.pdf *= spec;
+            } else {
+                B = sample_diffuse(S, u0, u1);
+                B.reflectance = B.reflectance * ((1.0f - m.metalness) * m.opacity);
+                B.pdf *= (1.0f - spec);
+            }
+            B.pdf *= (1.0f - refract);
+            no = no + S.V_ * kEpsilon;
+        } else {
+            B.L = rd;
+            B.pdf = refract;
+            B.reflectance = mk(1.0f - m.opacity, 1.0f - m.opacity, 1.0f - m.opacity);
+            S.NdotL = 1.0f;
+            no = no - S.N * kEpsilon;
+            refracted = true;
+        }
+        if (B.pdf > 0.0f && (B.reflectance.x + B.reflectance.y) + B.reflectance.z > 0.0f)
+            thr = thr * ((B.reflectance * S.NdotL) / B.pdf);
+        else
+            thr = mk(0.0f, 0.0f, 0.0f);
+        nd = normalize(B.L);
+        thr = thr * m.ao;
+        float prob = clampf(fmaxf(thr.x, fmaxf(thr.y, thr.z)), 0.01f, 0.99f);
+        prob = bounce < 3 ? fminf(3.0f * prob, 1.0f) : prob;
+        if (rnd2(ray_seed, seed) > prob)
+            thr = mk(0.0f, 0.0f, 0.0f);
+        else if (dot(nd, S.Ng) <= 0.0f && !refracted)
+            thr = mk(0.0f, 0.0f, 0.0f);
+        else
+            thr = thr / prob;
+    }
+    acc.x += add.x;
+    acc.y += add.y;
+    acc.z += add.z;
+    A.Lpix[pixel] = acc;
+    const float energy = (thr.x + thr.y) + thr.z;
+    if (energy == 0.0f || bounce == A.bounces) return;
+    const uint32_t k = atomicAdd(&ctl->paths[bounce + 1], 1u);
+    A.paths_out[3 * size_t(k)] = make_float4(no.x, no.y, no.z, p0.w);
+    A.paths_out[3 * size_t(k) + 1] = make_float4(nd.x, nd.y, nd.z, 0.0f);
+    A.paths_out[3 * size_t(k) + 2] = make_float4(thr.x, thr.y, thr.z, 0.0f);
+}
+
+// Shadow rays: HitAnyTransparency's traversal (restore every TLAS step),
+// INSTANCE_MASK_SHADOW, tmin 0 (rayHit.csh:327-339).
+template <int BLOCK, int LDS_DEPTH>
+__global__ __launch_bounds__(BLOCK) void k_shadow(WorldDev W, const FrameCtl* __restrict__ ctl, int32_t bounce,
+                                                  const float4* __restrict__ shadow, float4* __restrict__ Lpix,
+                                                  int32_t* __restrict__ spill, uint32_t cap,
+                                                  uint32_t* __restrict__ flags) {
+    __shared__ int32_t lds[LDS_DEPTH * BLOCK];
+    const uint32_t gid = blockIdx.x * BLOCK + threadIdx.x;
+    if (gid >= ctl->shadows[bounce]) return;
+    const float4 s0 = shadow[3 * size_t(gid)];
+    const float4 s1 = shadow[3 * size_t(gid) + 1];
+    const float4 c = shadow[3 * size_t(gid) + 2];
+    WorldLane L;
+    world_begin(L, make_float4(s0.x, s0.y, s0.z, 0.0f), make_float4(s1.x, s1.y, s1.z, s1.w));
+    const LaneStack<BLOCK, LDS_DEPTH> S{lds, spill, threadIdx.x, gid, gridDim.x * BLOCK};
+    int r;
+    do {
+        r = world_step_any<BLOCK, LDS_DEPTH, true>(L, W, ATLAS_RT_MASK_SHADOW, s1.w, S, cap);
+    } while (r == kContinue);
+    if (r == kOverflow) {
+        atomicOr(&flags[2], 1u);
+        return;
+    }
+    if (L.slot >= 0) return;  // occluded
+    const uint32_t pixel = __float_as_uint(s0.w);
+    float4 acc = Lpix[pixel];
+    acc.x += c.x;
+    acc.y += c.y;
+    acc.z += c.z;
+    Lpix[pixel] = acc;
+}
+
+__global__ __launch_bounds__(kBlock) void k_accumulate(FrameCtl* ctl, const float4* __restrict__ Lpix,
+                                                       float4* __restrict__ accum, uint32_t pixels) {
+    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p < pixels) {
+        const float4 l = Lpix[p];
+        float4 a = accum[p];
+        a.x += l.x;
+        a.y += l.y;
+        a.z += l.z;
+        a.w += 1.0f;
+        accum[p] = a;
+    }
+    if (p == 0) ctl->sample_count += 1;
+}
+
+}  // namespace
+
+atlas_rt_status launch_frame(PathTracer& P, const Camera& cam, hipStream_t s) {
+    const uint32_t pixels = P.width * P.height;
+    const uint32_t grid = (pixels + kBlock - 1) / kBlock;
+    uint32_t* flags = reinterpret_cast<uint32_t*>(P.flags.ptr);
+    hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(64), 0, s, P.ctl, P.bounces, pixels);
+    hipLaunchKernelGGL(k_raygen, dim3(grid), dim3(kBlock), 0, s, cam, P.width, P.height, P.ctl, P.paths[0], P.Lpix);
+    for (int32_t b = 0; b <= P.bounces; b++) {
+        float4* in = P.paths[b & 1];
+        float4* out = P.paths[(b + 1) & 1];
+        hipLaunchKernelGGL((k_trace<kBlock, kLdsDepth>), dim3(grid), dim3(kBlock), 0, s, P.world_dev,
+                           &P.ctl->paths[b], in, P.hits, reinterpret_cast<int32_t*>(P.spill.ptr), P.cap, flags);
+        ShadeArgs A = P.shade_args;
+        A.paths_in = in;
+        A.paths_out = out;
+        hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kBlock), 0, s, A, b);
+        if (P.light_count > 0)
+            hipLaunchKernelGGL((k_shadow<kBlock, kLdsDepth>), dim3(grid), dim3(kBlock), 0, s, P.world_dev, P.ctl, b,
+                               P.shadow, P.Lpix, reinterpret_cast<int32_t*>(P.spill.ptr), P.cap, flags);
+    }
+    hipLaunchKernelGGL(k_accumulate, dim3(grid), dim3(kBlock), 0, s, P.ctl, P.Lpix, P.accum, pixels);
+    ATLAS_HIP_TRY(hipGetLastError());
+    return ATLAS_RT_OK;
+}
+
+}  // namespace pt
+}  // namespace atlas_rt

@@ -73,6 +73,9 @@ struct atlas_rt_blas {
     float4* d_nodes = nullptr;
     float4* d_tris = nullptr;
     float4* d_vtx = nullptr;
+    //  d_shade: optional, 3 x float4 per slot: (n0, material index bits),
+    //           (n1, 0), (n2, 0) (the path tracer's shading data).
+    float4* d_shade = nullptr;
     atlas_rt::Scratch spill;
     atlas_rt::Scratch flags;  // [0] ray counter, [1] finished waves, [2] stack overflow flag
     atlas_rt::LaunchTiming timing;

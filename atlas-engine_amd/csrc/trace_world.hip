@@ -13,255 +13,11 @@
 #include <hip/hip_runtime.h>
 
 #include "rt_internal.h"
-#include "trace_core.h"
-#include "world.h"
+#include "trace_world_core.h"
 
 namespace atlas_rt {
 
 namespace {
-
-constexpr uint32_t kTlasInvalid = 0xFFFFFFFFu;
-
-struct WorldLane {
-    float o0x, o0y, o0z, d0x, d0y, d0z;  // world ray
-    double wrx, wry, wrz;                // world reciprocals
-    DevRay R;                            // ray of the current level
-    float hitT;
-    int32_t node;
-    int32_t slot;
-    int32_t inst;      // instance slot of the best hit
-    int32_t cur_inst;  // instance being traversed
-    uint32_t sp;
-    uint32_t tlas_index;
-    BlasDev B;
-    // object-space ray of the best hit (for the reported t, u, v)
-    float hox, hoy, hoz, hdx, hdy, hdz;
-};
-
-__device__ __forceinline__ float dot4(float x, float y, float z, float w, float4 m) {
-    return ((x * m.x + y * m.y) + z * m.z) + w * m.w;
-}
-
-__device__ __forceinline__ void restore_world(WorldLane& L) {
-    L.R.ox = L.o0x;
-    L.R.oy = L.o0y;
-    L.R.oz = L.o0z;
-    L.R.dx = L.d0x;
-    L.R.dy = L.d0y;
-    L.R.dz = L.d0z;
-    L.R.rx = L.wrx;
-    L.R.ry = L.wry;
-    L.R.rz = L.wrz;
-}
-
-__device__ __forceinline__ void world_begin(WorldLane& L, float4 r0, float4 r1) {
-    make_dev_ray(L.R, r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, r0.w);
-    L.o0x = r0.x;
-    L.o0y = r0.y;
-    L.o0z = r0.z;
-    L.d0x = r1.x;
-    L.d0y = r1.y;
-    L.d0z = r1.z;
-    L.wrx = L.R.rx;
-    L.wry = L.R.ry;
-    L.wrz = L.R.rz;
-    L.hitT = r1.w;
-    L.node = 0;
-    L.slot = -1;
-    L.inst = -1;
-    L.cur_inst = -1;
-    L.sp = 0;
-    L.tlas_index = kTlasInvalid;
-}
-
-// CheckInstance (bvh.hsh:172-189); returns whether the mask admits it.
-__device__ __forceinline__ bool enter_instance(WorldLane& L, const WorldDev& W, uint32_t mask) {
-    const int32_t ip = ~L.node;
-    const float4* m = W.instances + 4 * size_t(ip);
-    const float4 c0 = m[0], c1 = m[1], c2 = m[2], meta = m[3];
-    const float ox = dot4(L.o0x, L.o0y, L.o0z, 1.0f, c0);
-    const float oy = dot4(L.o0x, L.o0y, L.o0z, 1.0f, c1);
-    const float oz = dot4(L.o0x, L.o0y, L.o0z, 1.0f, c2);
-    const float dx = dot4(L.d0x, L.d0y, L.d0z, 0.0f, c0);
-    const float dy = dot4(L.d0x, L.d0y, L.d0z, 0.0f, c1);
-    const float dz = dot4(L.d0x, L.d0y, L.d0z, 0.0f, c2);
-    make_dev_ray(L.R, ox, oy, oz, dx, dy, dz, L.R.tmin);
-    L.cur_inst = ip;
-    const int32_t mesh = __float_as_int(meta.x);
-    L.B = W.blases[mesh];
-    L.node = L.B.node_count ? 0 : ~0;
-    return (uint32_t(__float_as_int(meta.w)) & mask) > 0;
-}
-
-template <int BLOCK, int LDS_DEPTH>
-__device__ __forceinline__ int world_step_closest(WorldLane& L, const WorldDev& W, uint32_t mask,
-                                                  const LaneStack<BLOCK, LDS_DEPTH>& S, uint32_t cap) {
-    const uint32_t spE = L.sp + 1;  // the engine's stackPtr
-    if (spE < L.tlas_index) {
-        restore_world(L);
-        L.tlas_index = kTlasInvalid;
-        if (L.node < 0) {
-            if (enter_instance(L, W, mask)) {
-                L.tlas_index = spE;
-            } else {
-                if (L.sp == 0) return kDone;
-                L.sp--;
-                L.node = S.get(L.sp);
-            }
-            return kContinue;
-        }
-        const float4* np = W.tlas + 4 * size_t(L.node);
-        const float4 a = np[0], b = np[1], c = np[2], d = np[3];
-        float dl, dr;
-        const bool hl = slab_dist(L.R, a.x, a.y, a.z, a.w, b.x, b.y, L.hitT, dl);
-        const bool hr = slab_dist(L.R, b.z, b.w, c.x, c.y, c.z, c.w, L.hitT, dr);
-        const int32_t lp = __float_as_int(d.x), rp = __float_as_int(d.y);
-        const bool leftFirst = dl <= dr;
-        if (hl && hr) {
-            if (L.sp >= cap) return kOverflow;
-            S.put(L.sp, leftFirst ? rp : lp);
-            L.sp++;
-            L.node = leftFirst ? lp : rp;
-        } else if (hl || hr) {
-            L.node = leftFirst ? lp : rp;
-        } else {
-            if (L.sp == 0) return kDone;
-            L.sp--;
-            L.node = S.get(L.sp);
-        }
-        return kContinue;
-    }
-    if (L.node >= 0) {
-        const float4* np = L.B.nodes + 4 * size_t(L.node);
-        const float4 a = np[0], b = np[1], c = np[2], d = np[3];
-        float dl, dr;
-        const bool hl = slab_dist(L.R, a.x, a.y, a.z, a.w, b.x, b.y, L.hitT, dl);
-        const bool hr = slab_dist(L.R, b.z, b.w, c.x, c.y, c.z, c.w, L.hitT, dr);
-        const int32_t lp = __float_as_int(d.x), rp = __float_as_int(d.y);
-        const bool leftFirst = dl <= dr;
-        if (hl && hr) {
-            if (L.sp >= cap) return kOverflow;
-            S.put(L.sp, leftFirst ? rp : lp);
-            L.sp++;
-            L.node = leftFirst ? lp : rp;
-        } else if (hl || hr) {
-            L.node = leftFirst ? lp : rp;
-        } else {
-            if (L.sp == 0) return kDone;
-            L.sp--;
-            L.node = S.get(L.sp);
-        }
-        return kContinue;
-    }
-    const int32_t tp = ~L.node;
-    const float4* tr = L.B.tris + 3 * size_t(tp);
-    const float4 a = tr[0], b = tr[1], c = tr[2];
-    float t, u, v;
-    const bool geo = tri_uvt(L.R, a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z, t, u, v);
-    if (geo && t > L.R.tmin && t < L.hitT) {
-        L.hitT = t;
-        L.slot = tp;
-        L.inst = L.cur_inst;
-        L.hox = L.R.ox;
-        L.hoy = L.R.oy;
-        L.hoz = L.R.oz;
-        L.hdx = L.R.dx;
-        L.hdy = L.R.dy;
-        L.hdz = L.R.dz;
-    }
-    if (b.w > 0.0f) {
-        if (L.sp == 0) return kDone;
-        L.sp--;
-        L.node = S.get(L.sp);
-    } else {
-        L.node = ~(tp + 1);
-    }
-    return kContinue;
-}
-
-// RESTORE_ALWAYS = false is HitAny (bvh.hsh:382-387): the world ray is restored
-// only when returning from an entered instance, so after an instance the mask
-// rejects, the TLAS walk continues with that instance's object-space ray.
-// RESTORE_ALWAYS = true is HitAnyTransparency's traversal (bvh.hsh:466-470),
-// used for the path tracer's shadow rays.
-template <int BLOCK, int LDS_DEPTH, bool RESTORE_ALWAYS>
-__device__ __forceinline__ int world_step_any(WorldLane& L, const WorldDev& W, uint32_t mask, float tmax,
-                                              const LaneStack<BLOCK, LDS_DEPTH>& S, uint32_t cap) {
-    const uint32_t spE = L.sp + 1;
-    if (spE < L.tlas_index) {
-        if (RESTORE_ALWAYS || L.tlas_index != kTlasInvalid) restore_world(L);
-        L.tlas_index = kTlasInvalid;
-        if (L.node < 0) {
-            if (enter_instance(L, W, mask)) {
-                L.tlas_index = spE;
-            } else {
-                if (L.sp == 0) return kDone;
-                L.sp--;
-                L.node = S.get(L.sp);
-            }
-            return kContinue;
-        }
-        const float4* np = W.tlas + 4 * size_t(L.node);
-        const float4 a = np[0], b = np[1], c = np[2], d = np[3];
-        float dl, dr;
-        const bool hl = slab_dist(L.R, a.x, a.y, a.z, a.w, b.x, b.y, tmax, dl);
-        const bool hr = slab_dist(L.R, b.z, b.w, c.x, c.y, c.z, c.w, tmax, dr);
-        const int32_t lp = __float_as_int(d.x), rp = __float_as_int(d.y);
-        if (hl && hr) {
-            if (L.sp >= cap) return kOverflow;
-            S.put(L.sp, rp);
-            L.sp++;
-            L.node = lp;
-        } else if (hl || hr) {
-            L.node = hl ? lp : rp;
-        } else {
-            if (L.sp == 0) return kDone;
-            L.sp--;
-            L.node = S.get(L.sp);
-        }
-        return kContinue;
-    }
-    if (L.node >= 0) {
-        const float4* np = L.B.nodes + 4 * size_t(L.node);
-        const float4 a = np[0], b = np[1], c = np[2], d = np[3];
-        float dl, dr;
-        const bool hl = slab_dist(L.R, a.x, a.y, a.z, a.w, b.x, b.y, tmax, dl);
-        const bool hr = slab_dist(L.R, b.z, b.w, c.x, c.y, c.z, c.w, tmax, dr);
-        const int32_t lp = __float_as_int(d.x), rp = __float_as_int(d.y);
-        if (hl && hr) {
-            if (L.sp >= cap) return kOverflow;
-            S.put(L.sp, rp);
-            L.sp++;
-            L.node = lp;
-        } else if (hl || hr) {
-            L.node = hl ? lp : rp;
-        } else {
-            if (L.sp == 0) return kDone;
-            L.sp--;
-            L.node = S.get(L.sp);
-        }
-        return kContinue;
-    }
-    const int32_t tp = ~L.node;
-    const float4* tr = L.B.tris + 3 * size_t(tp);
-    const float4 a = tr[0], b = tr[1], c = tr[2];
-    float t, u, v;
-    const bool geo = tri_uvt(L.R, a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z, t, u, v);
-    if (geo && t > L.R.tmin && t < tmax) {
-        L.slot = tp;
-        L.inst = L.cur_inst;
-        L.hitT = t;
-        return kDone;
-    }
-    if (b.w > 0.0f) {
-        if (L.sp == 0) return kDone;
-        L.sp--;
-        L.node = S.get(L.sp);
-    } else {
-        L.node = ~(tp + 1);
-    }
-    return kContinue;
-}
 
 // Compact ray in (32 B) -> refined hit (16 B) + caller instance index.
 template <int BLOCK, int LDS_DEPTH>

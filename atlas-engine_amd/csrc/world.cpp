@@ -89,6 +89,30 @@ static atlas_rt_status upload_vec(void*& dst, const std::vector<T>& v) {
 
 extern "C" {
 
+atlas_rt_status atlas_rt_aabb_tree_build(const atlas_rt_aabb* aabbs, uint32_t count, uint32_t flags,
+                                         atlas_rt_gpu_bvh_node* nodes, uint32_t* node_count, uint32_t* ref_idx,
+                                         uint8_t* end_of_node, uint32_t* ref_count) {
+    if (!aabbs || !node_count || !ref_count) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_aabb_tree_build: NULL argument");
+    if (count == 0) return fail(ATLAS_RT_E_EMPTY, "atlas_rt_aabb_tree_build: no items");
+    std::vector<float> boxes(6 * size_t(count));
+    for (uint32_t i = 0; i < count; i++) {
+        std::memcpy(&boxes[6 * size_t(i)], aabbs[i].min, 12);
+        std::memcpy(&boxes[6 * size_t(i) + 3], aabbs[i].max, 12);
+    }
+    BuildOutput bo;
+    const int threads = (flags & ATLAS_RT_BUILD_PARALLEL) ? default_threads() : 1;
+    if (!build_aabb_bvh(boxes.data(), count, threads, bo)) return fail(ATLAS_RT_E_EMPTY, "atlas_rt_aabb_tree_build");
+    const uint32_t nn = uint32_t(bo.nodes.size()), nr = uint32_t(bo.ref_idx.size());
+    const bool fits = *node_count >= nn && *ref_count >= nr;
+    *node_count = nn;
+    *ref_count = nr;
+    if (!fits) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_aabb_tree_build: output capacity too small");
+    if (nodes) std::memcpy(nodes, bo.nodes.data(), nn * sizeof(atlas_rt_gpu_bvh_node));
+    if (ref_idx) std::memcpy(ref_idx, bo.ref_idx.data(), nr * sizeof(uint32_t));
+    if (end_of_node) std::memcpy(end_of_node, bo.end_of_node.data(), nr);
+    return ATLAS_RT_OK;
+}
+
 atlas_rt_status atlas_rt_world_create(atlas_rt_world** out) {
     if (!out) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_world_create: out is NULL");
     *out = new atlas_rt_world();
@@ -149,8 +173,10 @@ atlas_rt_status atlas_rt_world_update(atlas_rt_world* w, const atlas_rt_instance
     const size_t m = bo.ref_idx.size();
     std::vector<atlas_rt_gpu_bvh_instance> packed(m);
     std::vector<uint32_t> order(m);
+    std::vector<float> xform(12 * m);
     for (size_t i = 0; i < m; i++) {
         const atlas_rt_instance_desc& d = inst[bo.ref_idx[i]];
+        std::memcpy(&xform[12 * i], d.object_to_world, 12 * sizeof(float));
         atlas_rt_gpu_bvh_instance& g = packed[i];
         std::memcpy(g.inverse_matrix, d.world_to_object, sizeof(g.inverse_matrix));
         g.mesh_offset = int32_t(d.blas);
@@ -173,6 +199,8 @@ atlas_rt_status atlas_rt_world_update(atlas_rt_world* w, const atlas_rt_instance
     w->instances = std::move(packed);
     w->order = std::move(order);
     w->table = std::move(table);
+    w->blas_handles.assign(blases, blases + blas_count);
+    w->xform = std::move(xform);
     w->tlas_depth = bo.max_depth;
     w->max_depth = bo.max_depth + blas_depth;
     w->device = dev;

@@ -34,6 +34,8 @@ struct atlas_rt_world {
     std::vector<atlas_rt_gpu_bvh_instance> instances;
     std::vector<uint32_t> order;  // packed slot -> caller instance index
     std::vector<atlas_rt::BlasDev> table;
+    std::vector<const atlas_rt_blas*> blas_handles;  // borrowed; the caller keeps them alive
+    std::vector<float> xform;                        // object-to-world rows, 12 floats per packed instance
     uint32_t tlas_depth = 0;
     uint32_t max_depth = 0;  // tlas depth + deepest BLAS
     bool valid = false;

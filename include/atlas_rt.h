@@ -222,6 +222,78 @@ atlas_rt_status atlas_rt_trace_any_host(const atlas_rt_blas* blas, const atlas_r
  * HIP events recorded around the kernel on its own stream), milliseconds. */
 atlas_rt_status atlas_rt_last_kernel_ms(const atlas_rt_blas* blas, float* ms);
 
+/* Volume::BVH::BVH(aabbs, parallelBuild) (BVH.cpp:58-101): the AABB-only
+ * tree the engine builds over instances (64 bins, one item per leaf, the
+ * single-item special node of BVH.cpp:80-88).  Host only.  On input
+ * *node_count / *ref_count are the capacities of nodes / ref_idx+end_of_node;
+ * on output the sizes (count - 1 nodes and count references for count >= 2;
+ * one node and two references for count == 1).  Too small a capacity returns
+ * ATLAS_RT_E_INVALID_ARG with the required sizes written. */
+atlas_rt_status atlas_rt_aabb_tree_build(const atlas_rt_aabb* aabbs, uint32_t count, uint32_t flags,
+                                         atlas_rt_gpu_bvh_node* nodes, uint32_t* node_count, uint32_t* ref_idx,
+                                         uint8_t* end_of_node, uint32_t* ref_count);
+
+/* Builds `count` BLASes at once, one host thread per mesh (the engine builds
+ * each mesh's BVH as its own job, editor/App.cpp:210-222 -> Mesh::BuildBVH).
+ * tris[k] holds counts[k] triangles; out[k] receives the handle. */
+atlas_rt_status atlas_rt_blas_build_many(const atlas_rt_bvh_triangle* const* tris, const uint32_t* counts,
+                                         uint32_t count, uint32_t flags, atlas_rt_blas** out);
+
+/* ------------------------------------------------------------------------- */
+/* Batched traces: many independent BLAS / ray-batch pairs in one launch     */
+/* ------------------------------------------------------------------------- */
+/* No engine counterpart: the engine traces one ray buffer against one world
+ * per dispatch (RayTracingHelper::DispatchHitClosest).  A batch set is a
+ * table of (BLAS, ray range) pairs traced by one persistent launch: waves pull
+ * rays from a global counter over the concatenated ranges, so uneven batch
+ * sizes keep every CU busy. */
+typedef struct atlas_rt_batch {
+    uint32_t blas;       /* index into the BLAS table given at creation */
+    uint32_t ray_offset; /* first ray of the batch in the ray / hit arrays */
+    uint32_t ray_count;
+    uint32_t reserved;
+} atlas_rt_batch;
+
+typedef struct atlas_rt_batch_set atlas_rt_batch_set;
+
+/* Launch configuration chosen by the autotuner or forced by the caller. */
+typedef struct atlas_rt_launch_config {
+    int32_t kernel;       /* ATLAS_RT_KERNEL_*; AUTO = tuned */
+    uint32_t block;       /* work-group size: 64, 128 or 256 (0 = default) */
+    uint32_t refill;      /* persistent kernels: refill a wave below this many live lanes (0 = default) */
+    uint32_t reserved;
+} atlas_rt_launch_config;
+
+atlas_rt_status atlas_rt_batch_create(const atlas_rt_blas* const* blases, uint32_t blas_count,
+                                      const atlas_rt_batch* batches, uint32_t batch_count,
+                                      atlas_rt_batch_set** out);
+atlas_rt_status atlas_rt_batch_destroy(atlas_rt_batch_set* set);
+
+/* Closest hit for every ray of every batch: hits[i] for rays[i] (16 B
+ * records as atlas_rt_trace_closest_rays).  d_count (may be NULL) caps the
+ * total number of rays processed (read on the device), so one captured graph
+ * can serve uneven ray counts. */
+atlas_rt_status atlas_rt_batch_trace_closest(atlas_rt_batch_set* set, const atlas_rt_ray* d_rays,
+                                             atlas_rt_hit* d_hits, const uint32_t* d_count,
+                                             const atlas_rt_launch_config* config, void* stream);
+
+/* The autotuner's choice for this set and its trials (kernel x work-group
+ * size x refill threshold, timed with HIP events on the first AUTO launch).
+ * *n_trials is the capacity of trials/trial_ms on input, the trial count on
+ * output.  Any output may be NULL. */
+atlas_rt_status atlas_rt_batch_tuning(const atlas_rt_batch_set* set, atlas_rt_launch_config* best,
+                                      uint32_t* n_trials, atlas_rt_launch_config* trials, float* trial_ms);
+
+/* Device time of the last batched launch (HIP events on its stream), ms. */
+atlas_rt_status atlas_rt_batch_last_kernel_ms(atlas_rt_batch_set* set, float* ms);
+
+/* Records the batched trace (same arguments) into a HIP graph owned by the
+ * set; atlas_rt_batch_graph_launch replays it on a stream. */
+atlas_rt_status atlas_rt_batch_graph_capture(atlas_rt_batch_set* set, const atlas_rt_ray* d_rays,
+                                             atlas_rt_hit* d_hits, const uint32_t* d_count,
+                                             const atlas_rt_launch_config* config);
+atlas_rt_status atlas_rt_batch_graph_launch(atlas_rt_batch_set* set, void* stream);
+
 /* ------------------------------------------------------------------------- */
 /* World: TLAS over instances of BLASes (RayTracingWorld)                    */
 /* ------------------------------------------------------------------------- */
@@ -297,6 +369,87 @@ atlas_rt_status atlas_rt_world_trace_closest_rays(const atlas_rt_world* world, c
 atlas_rt_status atlas_rt_world_trace_any_rays(const atlas_rt_world* world, const atlas_rt_ray* d_rays,
                                               uint32_t* d_hit, uint32_t count, uint32_t mask, uint32_t flags,
                                               void* stream);
+
+/* ------------------------------------------------------------------------- */
+/* Path tracer (PathTracingRenderer + RayTracingHelper bounce loop)          */
+/* ------------------------------------------------------------------------- */
+
+/* Per-slot shading data of a BLAS: vertex normals (GPUTriangle n0..n2,
+ * MeshData.cpp:222-237, kept in float32 here) and the material index
+ * (GPUTriangle.d0.w).  normals: 9 floats per triangle in the caller's
+ * original order; material: one index per triangle or NULL (all 0). */
+atlas_rt_status atlas_rt_blas_set_shading(atlas_rt_blas* blas, const float* normals, const int32_t* material);
+
+/* RaytraceMaterial / GPUMaterial subset used by the bounce shader
+ * (RTStructures.h:51-78, raytracer/surface.hsh:22-44).  No textures. */
+typedef struct atlas_rt_material {
+    float base_color[3];
+    float opacity;
+    float emissive[3];
+    float roughness;
+    float metalness;
+    float ao;
+    float reflectance;
+    int32_t two_sided;
+} atlas_rt_material; /* 48 B */
+
+/* GPULight (RTStructures.h:105-110, RayTracingHelper::UpdateLights
+ * RayTracingHelper.cpp:436-500).  Directional lights only: `direction` is the
+ * direction the light travels (LightComponent direction, L = -direction),
+ * radiance = linear colour * intensity. */
+#define ATLAS_RT_LIGHT_DIRECTIONAL 0u
+typedef struct atlas_rt_light {
+    float direction[3];
+    uint32_t type;
+    float radiance[3];
+    float reserved;
+} atlas_rt_light; /* 32 B */
+
+/* RayGenUniforms (PathTracingRenderer.h:48-57): origin/right/bottom are the
+ * near-plane top-left corner and the spans to the top-right and bottom-left
+ * corners (PathTracingRenderer.cpp:157-162); location is the camera position. */
+typedef struct atlas_rt_camera {
+    float location[3];
+    float origin[3];
+    float right[3];
+    float bottom[3];
+} atlas_rt_camera;
+
+typedef struct atlas_rt_pt_params {
+    uint32_t width;
+    uint32_t height;
+    int32_t bounces;          /* maxBounces; bounces + 1 trace passes per frame */
+    float exposure;
+    float env_radiance[3];    /* constant environment (SampleEnvironmentMap) */
+    uint32_t flags;           /* ATLAS_RT_PT_* */
+} atlas_rt_pt_params;
+
+typedef struct atlas_rt_pathtracer atlas_rt_pathtracer;
+
+/* The world, its BLASes (with shading data) and the material table must
+ * outlive the path tracer; materials are indexed by instance
+ * material_offset + triangle material index. */
+atlas_rt_status atlas_rt_pathtracer_create(const atlas_rt_world* world, const atlas_rt_pt_params* params,
+                                           const atlas_rt_material* materials, uint32_t material_count,
+                                           const atlas_rt_light* lights, uint32_t light_count,
+                                           atlas_rt_pathtracer** out);
+atlas_rt_status atlas_rt_pathtracer_destroy(atlas_rt_pathtracer* pt);
+
+/* Renders `frames` progressive samples per pixel (one path per pixel per
+ * frame: raygen, then bounces + 1 x (closest-hit trace, shade, shadow-ray
+ * any-hit), then accumulate), asynchronously on `stream`.  The per-bounce
+ * RNG seeds follow Random::SampleFastUniformFloat's PCG32 stream
+ * (RandomHelper.cpp:8-51) and the sub-pixel jitter rayGen.csh:33-34. */
+atlas_rt_status atlas_rt_pathtracer_render(atlas_rt_pathtracer* pt, const atlas_rt_camera* camera, uint32_t frames,
+                                           void* stream);
+
+/* PathTracingRenderer::ResetSampleCount: clears the accumulator. */
+atlas_rt_status atlas_rt_pathtracer_reset(atlas_rt_pathtracer* pt);
+
+/* Accumulated radiance sum (rgb) and sample count (a) per pixel, row-major,
+ * width*height float4; d_out is a device buffer.  Synchronous on `stream`. */
+atlas_rt_status atlas_rt_pathtracer_read(atlas_rt_pathtracer* pt, float* d_out, void* stream);
+atlas_rt_status atlas_rt_pathtracer_sample_count(const atlas_rt_pathtracer* pt, int32_t* count);
 
 #ifdef __cplusplus
 }

@@ -234,6 +234,56 @@ def _f32(*xs):
     return [np.ascontiguousarray(x, dtype=np.float32) for x in xs]
 
 
+def pathtrace(tlas_nodes, instances, xform, blases, shading, materials, lights, env, cam, width, height, bounces,
+              frames, first_sample=0, pcg_draws=0, accum=None, threads=None):
+    """CPU restatement of the progressive path tracer.
+
+    blases: list of (nodes, tris) packed arrays; shading: list of (normals
+    (n, 9) float32, material (n,) int32) in BVH slot order; xform: (m, 12)
+    object-to-world rows per packed instance; cam: a ctypes camera struct
+    (location, origin, right, bottom).  Returns the accumulated (h, w, 4)
+    float32 image (rgb sum, sample count).
+    """
+    k = len(blases)
+    keep = []
+    node_ptrs, tri_ptrs = (C.c_void_p * k)(), (C.c_void_p * k)()
+    nrm_ptrs, mat_ptrs = (C.c_void_p * k)(), (C.c_void_p * k)()
+    node_counts = np.zeros(k, dtype=np.uint32)
+    tri_counts = np.zeros(k, dtype=np.uint32)
+    for i, ((bn, bt), (sn, sm)) in enumerate(zip(blases, shading)):
+        bn = np.ascontiguousarray(bn, dtype=NODE_DTYPE)
+        bt = np.ascontiguousarray(bt, dtype=GPU_TRI_DTYPE)
+        sn = np.ascontiguousarray(sn, dtype=np.float32)
+        sm = np.ascontiguousarray(sm, dtype=np.int32)
+        keep += [bn, bt, sn, sm]
+        node_ptrs[i] = bn.ctypes.data if len(bn) else None
+        tri_ptrs[i] = bt.ctypes.data
+        nrm_ptrs[i] = sn.ctypes.data
+        mat_ptrs[i] = sm.ctypes.data
+        node_counts[i], tri_counts[i] = len(bn), len(bt)
+    tlas_nodes = np.ascontiguousarray(tlas_nodes, dtype=NODE_DTYPE)
+    instances = np.ascontiguousarray(instances, dtype=INSTANCE_DTYPE)
+    xform = np.ascontiguousarray(xform, dtype=np.float32)
+    materials = np.ascontiguousarray(materials)
+    lights = np.ascontiguousarray(lights)
+    env = np.ascontiguousarray(env, dtype=np.float32)
+    if accum is None:
+        accum = np.zeros((height, width, 4), dtype=np.float32)
+    L = lib()
+    if not getattr(L, "_pt_bound", False):
+        vp, u32, i32 = C.c_void_p, C.c_uint32, C.c_int32
+        L.oracle_pathtrace.restype = None
+        L.oracle_pathtrace.argtypes = [vp, u32, vp, u32, vp, vp, vp, vp, vp, vp, u32, vp, vp, u32, vp, u32, vp, vp,
+                                       u32, u32, i32, u32, u32, C.c_uint64, vp, i32]
+        L._pt_bound = True
+    L.oracle_pathtrace(_p(tlas_nodes), len(tlas_nodes), _p(instances), len(instances), C.cast(node_ptrs, C.c_void_p),
+                       _p(node_counts), C.cast(tri_ptrs, C.c_void_p), _p(tri_counts), C.cast(nrm_ptrs, C.c_void_p),
+                       C.cast(mat_ptrs, C.c_void_p), k, _p(xform), _p(materials), len(materials),
+                       _p(lights) if len(lights) else None, len(lights), _p(env), C.addressof(cam), width, height,
+                       bounces, frames, first_sample, pcg_draws, _p(accum), threads or threads_default())
+    return accum
+
+
 def triangle_test(o, d, v0, v1, v2):
     args = _f32(o, d, v0, v1, v2)  # keep the arrays alive across the call
     sol = np.zeros(3, dtype=np.float32)

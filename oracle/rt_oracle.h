@@ -4,6 +4,7 @@
 #include <vector>
 #include <cstdint>
 
+#include "../include/atlas_rt.h"
 #include "rt_oracle_math.h"
 
 namespace oracle {
@@ -24,5 +25,40 @@ struct BuildResult {
 
 bool build_triangle_bvh(const std::vector<Box>& aabbs, const float* verts, uint32_t n, BuildResult& res);
 bool build_aabb_bvh(const std::vector<Box>& aabbs, BuildResult& res);
+
+// Traversal over the engine's packed arrays (rt_oracle_trace.cpp).
+struct TraceStats {
+    uint32_t node_visits = 0;  // inner nodes fetched (BLAS + TLAS)
+    uint32_t tri_tests = 0;    // triangles intersected
+    uint32_t inst_visits = 0;  // instances entered (CheckInstance calls)
+};
+
+struct HitRec {
+    float t = 0, u = 0, v = 0;
+    int32_t slot = -1;      // engine hitID (BVH slot in the BLAS)
+    int32_t instance = -1;  // engine hitInstanceID (TLAS instance slot)
+    float ho[3] = {0, 0, 0}, hd[3] = {0, 0, 0};  // object-space ray of the accepted test
+};
+
+struct BlasView {
+    const atlas_rt_gpu_bvh_node* nodes;
+    uint32_t node_count;
+    const atlas_rt_gpu_bvh_triangle* tris;
+    uint32_t tri_count;
+};
+
+struct World {
+    const atlas_rt_gpu_bvh_node* tlas;
+    uint32_t tlas_count;
+    const atlas_rt_gpu_bvh_instance* inst;
+    uint32_t inst_count;
+    const BlasView* blas;
+    uint32_t blas_count;
+};
+
+void closest_rgpu_world(const World& w, V3 o0, V3 d0, float tmin, float tmax, uint32_t mask, HitRec& h,
+                        TraceStats& st, std::vector<int32_t>& stack);
+bool any_rgpu_world(const World& w, V3 o0, V3 d0, float tmin, float tmax, uint32_t mask, TraceStats& st,
+                    std::vector<int32_t>& stack, bool restore_always);
 
 }  // namespace oracle

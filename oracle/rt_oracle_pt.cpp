@@ -1,0 +1,448 @@
+// TEST INFRASTRUCTURE - the oracle.  Only tests/, __graft_entry__.smoke() and
+// bench.py's cpu_baseline leg may load it, and only as the checker.
+//
+// CPU restatement of the engine's progressive path tracer, one path per pixel
+// per frame:
+//   PathTracingRenderer::Render (src/engine/renderer/PathTracingRenderer.cpp:29-292):
+//     per-bounce seeds from Random::SampleFastUniformFloat (PCG32,
+//     src/engine/common/RandomHelper.cpp:8-51), bounces + 1 passes;
+//   rayGen.csh:25-90 (jittered pinhole ray through the near-plane frame);
+//   rayHit.csh:55-339 (EvaluateBounce / EvaluateDirectLight / EvaluateIndirectLight /
+//     CheckVisibility), raytracer/surface.hsh:46-140, raytracer/direct.hsh:12-86;
+//   brdf/brdf.hsh, brdfEval.hsh, brdfSample.hsh, brdf/surface.hsh;
+//   common/random.hsh (hash RNG).
+// Traversal: HitClosest and HitAnyTransparency's traversal (opaque triangles)
+// from rt_oracle_trace.cpp.  The shadow ray is traced inline, as the engine
+// does.  Light selection draws its seed from the pixel index (the engine uses
+// the invocation index; with one light the draw cannot change the choice).
+// Payload radiance/throughput stay float32 (the engine packs them to half
+// floats between bounces).
+#include <cmath>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "rt_oracle.h"
+
+namespace oracle {
+namespace {
+
+const float kPI = 3.1415926535897932384626433832795f;
+const float kInvPI = 0.3183098861837906715377675267450f;
+const float kEps = 0.1f;
+const float kInf = 1.0e12f;
+
+inline V3 mul(V3 a, float s) { return V3{a.x * s, a.y * s, a.z * s}; }
+inline V3 neg(V3 a) { return V3{-a.x, -a.y, -a.z}; }
+inline V3 nrm(V3 a) {
+    float l = std::sqrt(dot(a, a));
+    return V3{a.x / l, a.y / l, a.z / l};
+}
+inline float sat(float x) { return std::fmin(std::fmax(x, 0.0f), 1.0f); }
+inline float clampf(float x, float lo, float hi) { return std::fmin(std::fmax(x, lo), hi); }
+inline float mixf(float a, float b, float t) { return a + t * (b - a); }
+
+// random.hsh:5-54
+inline uint32_t ohash(uint32_t x) {
+    x += (x << 10u);
+    x ^= (x >> 6u);
+    x += (x << 3u);
+    x ^= (x >> 11u);
+    x += (x << 15u);
+    return x;
+}
+inline uint32_t fbits(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return u;
+}
+inline float to_unit(uint32_t m) {
+    m = (m & 0x007FFFFFu) | 0x3F800000u;
+    float f;
+    std::memcpy(&f, &m, 4);
+    return f - 1.0f;
+}
+inline float rand2(float a, float b) { return to_unit(ohash(fbits(a) ^ ohash(fbits(b)))); }
+inline float rand_seq(float x, float& seed) {
+    float r = rand2(x, seed);
+    seed += 1.0f;
+    return r;
+}
+
+// RandomHelper.cpp:8-51
+struct Pcg {
+    uint64_t state;
+    uint32_t next() {
+        uint64_t x = state;
+        unsigned count = unsigned(x >> 59u);
+        state = x * 6364136223846793005ull + 1442695040888963407ull;
+        x ^= x >> 18;
+        uint32_t v = uint32_t(x >> 27u);
+        return (v >> count) | (v << ((0u - count) & 31u));
+    }
+    float fast_uniform() { return float(next()) / float(0xFFFFFFFFu) * 0.999999f; }
+};
+
+struct Mat {
+    V3 base, emissive;
+    float opacity, roughness, metalness, ao, reflectance;
+    int32_t two_sided;
+};
+
+struct Surf {
+    V3 P, Vv, N, L, H, Ng;
+    float NdotL, LdotH, NdotH, NdotV;
+    V3 F0;
+    float F90;
+    Mat m;
+};
+
+inline V3 schlick(V3 F0, float F90, float c) {
+    float p = std::pow(1.0f - c, 5.0f);
+    return F0 + mul(V3{F90, F90, F90} - F0, p);
+}
+void update(Surf& s) {
+    s.L = nrm(s.L);
+    s.Vv = nrm(s.Vv);
+    s.N = nrm(s.N);
+    s.H = nrm(s.L + s.Vv);
+    s.NdotL = sat(dot(s.N, s.L));
+    s.LdotH = sat(dot(s.L, s.H));
+    s.NdotH = sat(dot(s.N, s.H));
+    s.NdotV = sat(dot(s.N, s.Vv));
+    float f = 0.16f * (s.m.reflectance * s.m.reflectance);
+    s.F0 = V3{mixf(f, s.m.base.x, s.m.metalness), mixf(f, s.m.base.y, s.m.metalness), mixf(f, s.m.base.z, s.m.metalness)};
+    s.F90 = sat(50.0f * dot(s.F0, V3{0.333f, 0.333f, 0.333f}));
+}
+V3 diffuse(const Surf& s) {
+    float lr = std::fmax(s.m.roughness * s.m.roughness, 0.00001f);
+    float bias = mixf(0.0f, 0.5f, lr), factor = mixf(1.0f, 1.0f / 1.51f, lr);
+    float fd90 = bias + 2.0f * s.LdotH * s.LdotH * lr;
+    float ls = 1.0f + (fd90 - 1.0f) * std::pow(1.0f - s.NdotL, 5.0f);
+    float vs = 1.0f + (fd90 - 1.0f) * std::pow(1.0f - s.NdotV, 5.0f);
+    float d = ls * vs * factor;
+    return mul(mul(mul(s.m.base, 1.0f - s.m.metalness), d), kInvPI);
+}
+float vis_corr(float NdotL, float NdotV, float a) {
+    float a2 = a * a;
+    float gl = NdotV * std::sqrt((-NdotL * a2 + NdotL) * NdotL + a2);
+    float gv = NdotL * std::sqrt((-NdotV * a2 + NdotV) * NdotV + a2);
+    return 0.5f / (gl + gv + 0.0000001f);
+}
+float vis_sep(float c, float a) {
+    float a2 = a * a;
+    return 2.0f * c / (c + std::sqrt(a2 + (1.0f - a2) * c * c));
+}
+float ggx(float NdotH, float a) {
+    float a2 = a * a;
+    float f = (NdotH * a2 - NdotH) * NdotH + 1.0f;
+    return a2 / (f * f + 0.0000001f) * kInvPI;
+}
+V3 specular(const Surf& s) {
+    float r = std::fmax(s.m.roughness * s.m.roughness, 0.00001f);
+    V3 F = schlick(s.F0, s.F90, s.LdotH);
+    float G = vis_corr(s.NdotV, s.NdotL, r);
+    float D = ggx(s.NdotH, r);
+    return mul(mul(F, D), G);
+}
+void frame_of(V3 N, V3& t, V3& b, bool nb) {
+    V3 up = std::fabs(N.z) < 0.999f ? V3{0, 0, 1} : V3{1, 0, 0};
+    t = nrm(cross(up, N));
+    b = cross(N, t);
+    if (nb) b = nrm(b);
+}
+struct Bs {
+    V3 refl, L;
+    float pdf;
+};
+Bs sample_diffuse(Surf& s, float u0, float u1) {
+    float th = std::sqrt(u0), ph = 2.0f * kPI * u1;
+    V3 l{th * std::cos(ph), th * std::sin(ph), std::sqrt(1.0f - u0)};
+    V3 t, b;
+    frame_of(s.N, t, b, false);
+    s.L = nrm((mul(t, l.x) + mul(b, l.y)) + mul(s.N, l.z));
+    update(s);
+    return Bs{diffuse(s), s.L, s.NdotL / kPI};
+}
+Bs sample_specular(Surf& s, float u0, float u1) {
+    float alpha = s.m.roughness * s.m.roughness;
+    s.Vv = nrm(s.Vv);
+    V3 N = nrm(s.N), t, b;
+    frame_of(N, t, b, true);
+    V3 vt = nrm(V3{dot(s.Vv, t), dot(s.Vv, b), dot(s.Vv, N)});
+    V3 vh = nrm(V3{vt.x * alpha, vt.y * alpha, vt.z});
+    float phi = kPI * 2.0f * u0;
+    float z = (1.0f - u1) * (1.0f + vh.z) + -vh.z;
+    float r = std::sqrt(clampf(1.0f - z * z, 0.0f, 1.0f));
+    V3 c{std::cos(phi) * r, std::sin(phi) * r, z};
+    V3 Hh = c + vh;
+    V3 h{Hh.x * alpha, Hh.y * alpha, std::fmax(Hh.z, 0.0f)};
+    V3 M = nrm((mul(t, h.x) + mul(b, h.y)) + mul(N, h.z));
+    s.L = mul(M, 2.0f * dot(s.Vv, M)) - s.Vv;
+    update(s);
+    Bs o{V3{0, 0, 0}, s.L, 1.0f};
+    if (s.NdotL > 0.0f && s.LdotH > 0.0f) {
+        V3 F = schlick(s.F0, s.F90, s.LdotH);
+        float Vis = vis_corr(s.NdotV, s.NdotL, alpha);
+        float G1 = vis_sep(s.NdotV, alpha);
+        o.pdf = G1 / (4.0f * std::fabs(dot(s.Vv, s.N)));
+        o.refl = mul(F, Vis);
+    }
+    return o;
+}
+
+struct Scene {
+    World w;
+    std::vector<std::vector<float>> shade;  // per BLAS, slot order: n0, n1, n2 (9 floats)
+    std::vector<std::vector<int32_t>> mat;  // per BLAS, slot order
+    const float* xform;                     // 12 floats per packed instance
+    std::vector<Mat> mats;
+    struct Light {
+        V3 N, rad;
+        float pdf;
+    };
+    std::vector<Light> lights;
+    V3 env;
+    int32_t bounces;
+};
+
+inline V3 xf(const float* M, V3 p, float w) {
+    return V3{(M[0] * p.x + M[1] * p.y) + (M[2] * p.z + M[3] * w),
+              (M[4] * p.x + M[5] * p.y) + (M[6] * p.z + M[7] * w),
+              (M[8] * p.x + M[9] * p.y) + (M[10] * p.z + M[11] * w)};
+}
+
+V3 trace_path(const Scene& S, V3 o, V3 d, float pixel_seed, const float* seeds, std::vector<int32_t>& stack) {
+    V3 thr{1, 1, 1}, Lr{0, 0, 0};
+    for (int32_t bounce = 0; bounce <= S.bounces; bounce++) {
+        HitRec h;
+        TraceStats st;
+        closest_rgpu_world(S.w, o, d, 0.0f, kInf, ATLAS_RT_MASK_ALL, h, st, stack);
+        if (h.slot < 0) {
+            V3 e = S.env * thr;
+            Lr = Lr + V3{std::fmin(e.x, 10.0f), std::fmin(e.y, 10.0f), std::fmin(e.z, 10.0f)};
+            break;
+        }
+        const auto& in = S.w.inst[h.instance];
+        const int32_t mesh = in.mesh_offset;
+        const float* M = S.xform + 12 * size_t(h.instance);
+        const auto& tri = S.w.blas[mesh].tris[h.slot];
+        const float* ns = &S.shade[mesh][9 * size_t(h.slot)];
+        V3 v0 = xf(M, V3{tri.v0[0], tri.v0[1], tri.v0[2]}, 1.0f);
+        V3 v1 = xf(M, V3{tri.v1[0], tri.v1[1], tri.v1[2]}, 1.0f);
+        V3 v2 = xf(M, V3{tri.v2[0], tri.v2[1], tri.v2[2]}, 1.0f);
+        V3 n0 = nrm(xf(M, V3{ns[0], ns[1], ns[2]}, 0.0f));
+        V3 n1 = nrm(xf(M, V3{ns[3], ns[4], ns[5]}, 0.0f));
+        V3 n2 = nrm(xf(M, V3{ns[6], ns[7], ns[8]}, 0.0f));
+        const Mat& m = S.mats[size_t(S.mat[mesh][h.slot] + in.material_offset)];
+        V3 e0 = v1 - v0, e1 = v2 - v0, s = o - v0;
+        V3 p = cross(s, e0), q = cross(d, e1);
+        float den = dot(q, e0);
+        float dist = dot(p, e1) / den, bs = dot(q, s) / den, bt = dot(p, d) / den;
+        float br = 1.0f - bs - bt;
+        Surf sf;
+        sf.m = m;
+        sf.P = o + mul(d, dist);
+        V3 normal = nrm((mul(n0, br) + mul(n1, bs)) + mul(n2, bt));
+        V3 Ng = nrm(cross(v0 - v1, v0 - v2));
+        bool flip = dot(Ng, d) > 0.0f;
+        Ng = mul(Ng, (flip && m.two_sided > 0) ? -1.0f : 1.0f);
+        if (dot(normal, Ng) < 0.0f && m.two_sided > 0) normal = neg(normal);
+        sf.Vv = neg(d);
+        sf.N = nrm(normal);
+        sf.Ng = Ng;
+        sf.F0 = V3{mixf(0.04f, m.base.x, m.metalness), mixf(0.04f, m.base.y, m.metalness),
+                   mixf(0.04f, m.base.z, m.metalness)};
+        sf.F90 = 1.0f;
+        sf.NdotL = 0.0f;
+        if ((m.emissive.x + m.emissive.y) + m.emissive.z > 0.0f && bounce == 0) Lr = Lr + m.emissive;
+        // EvaluateDirectLight
+        V3 radiance{0, 0, 0};
+        if (!S.lights.empty()) {
+            float seed = seeds[bounce];
+            float total = 0.0f, lpdf = 1.0f;
+            size_t sel = 0;
+            for (size_t l = 0; l < S.lights.size(); l++) {
+                float brightness = dot(S.lights[l].rad, V3{0.33333f, 0.33333f, 0.33333f});
+                float wgt = clampf(brightness, 0.0000001f, 1.0f);
+                total += wgt;
+                if (rand_seq(pixel_seed, seed) < wgt / total) {
+                    lpdf = wgt;
+                    sel = l;
+                }
+            }
+            const auto& Lt = S.lights[sel];
+            lpdf = lpdf / total * Lt.pdf;
+            sf.L = nrm(neg(Lt.N));
+            update(sf);
+            V3 refl = mul(diffuse(sf) + specular(sf), m.opacity);
+            V3 rad = Lt.rad;
+            float vis = 0.0f;
+            if (sf.NdotL > 0.0f) {
+                TraceStats st2;
+                bool occluded = any_rgpu_world(S.w, sf.P + mul(sf.N, kEps), sf.L, 0.0f, kInf - 2.0f * kEps,
+                                               ATLAS_RT_MASK_SHADOW, st2, stack, true);
+                vis = occluded ? 0.0f : 1.0f;
+            }
+            V3 direct = mul(refl * mul(rad, vis), sf.NdotL);
+            direct = V3{direct.x / lpdf, direct.y / lpdf, direct.z / lpdf};
+            radiance = mul(thr, m.opacity) * direct;
+            if (bounce > 0) {
+                float rm = std::fmax(std::fmax(radiance.x, std::fmax(radiance.y, radiance.z)), 10.0f);
+                radiance = mul(radiance, 10.0f / rm);
+            }
+        }
+        Lr = Lr + radiance;
+        // EvaluateIndirectLight
+        V3 no = sf.P;
+        float seed = seeds[bounce];
+        float refract = clampf(1.0f - m.opacity, 0.1f, 0.9f);
+        refract = m.opacity == 1.0f ? 0.0f : refract;
+        float rnd = sat(rand_seq(pixel_seed, seed));
+        Bs B;
+        bool refracted = false;
+        if (rnd >= refract) {
+            rnd = rand_seq(pixel_seed, seed);
+            V3 F = schlick(sf.F0, sf.F90, sf.NdotV);
+            float spec = clampf(dot(F, V3{0.33333f, 0.33333f, 0.33333f}), 0.1f, 0.9f);
+            float u0 = rand_seq(pixel_seed, seed), u1 = rand_seq(pixel_seed, seed);
+            if (rnd < spec) {
+                B = sample_specular(sf, u0, u1);
+                B.refl = mul(B.refl, m.opacity);
+                B.pdf *= spec;
+            } else {
+                B = sample_diffuse(sf, u0, u1);
+                B.refl = mul(B.refl, (1.0f - m.metalness) * m.opacity);
+                B.pdf *= (1.0f - spec);
+            }
+            B.pdf *= (1.0f - refract);
+            no = no + mul(sf.Vv, kEps);
+        } else {
+            B.L = d;
+            B.pdf = refract;
+            B.refl = V3{1.0f - m.opacity, 1.0f - m.opacity, 1.0f - m.opacity};
+            sf.NdotL = 1.0f;
+            no = no - mul(sf.N, kEps);
+            refracted = true;
+        }
+        if (B.pdf > 0.0f && (B.refl.x + B.refl.y) + B.refl.z > 0.0f) {
+            V3 k = mul(B.refl, sf.NdotL);
+            thr = thr * V3{k.x / B.pdf, k.y / B.pdf, k.z / B.pdf};
+        } else {
+            thr = V3{0, 0, 0};
+        }
+        V3 nd = nrm(B.L);
+        thr = mul(thr, m.ao);
+        float prob = clampf(std::fmax(thr.x, std::fmax(thr.y, thr.z)), 0.01f, 0.99f);
+        prob = bounce < 3 ? std::fmin(3.0f * prob, 1.0f) : prob;
+        if (rand_seq(pixel_seed, seed) > prob)
+            thr = V3{0, 0, 0};
+        else if (dot(nd, sf.Ng) <= 0.0f && !refracted)
+            thr = V3{0, 0, 0};
+        else
+            thr = V3{thr.x / prob, thr.y / prob, thr.z / prob};
+        if ((thr.x + thr.y) + thr.z == 0.0f) break;
+        o = no;
+        d = nd;
+    }
+    return Lr;
+}
+
+}  // namespace
+}  // namespace oracle
+
+using namespace oracle;
+
+extern "C" {
+
+// shade_normals[k]: 9 floats per slot of BLAS k (BVH slot order);
+// shade_mat[k]: one material index per slot.  accum: width*height*4 floats
+// (rgb sum, count), accumulated over `frames` frames starting at sample
+// `first_sample`; pcg_draws: number of Random draws made before this call
+// (the PCG stream position).
+void oracle_pathtrace(const atlas_rt_gpu_bvh_node* tlas, uint32_t tlas_count, const atlas_rt_gpu_bvh_instance* inst,
+                      uint32_t inst_count, const atlas_rt_gpu_bvh_node* const* blas_nodes,
+                      const uint32_t* blas_node_counts, const atlas_rt_gpu_bvh_triangle* const* blas_tris,
+                      const uint32_t* blas_tri_counts, const float* const* shade_normals,
+                      const int32_t* const* shade_mat, uint32_t blas_count, const float* xform,
+                      const atlas_rt_material* mats, uint32_t n_mats, const atlas_rt_light* lights, uint32_t n_lights,
+                      const float* env, const atlas_rt_camera* cam, uint32_t width, uint32_t height, int32_t bounces,
+                      uint32_t frames, uint32_t first_sample, uint64_t pcg_draws, float* accum, int32_t threads) {
+    std::vector<BlasView> views(blas_count);
+    for (uint32_t k = 0; k < blas_count; k++)
+        views[k] = BlasView{blas_nodes[k], blas_node_counts[k], blas_tris[k], blas_tri_counts[k]};
+    Scene S;
+    S.w = World{tlas, tlas_count, inst, inst_count, views.data(), blas_count};
+    S.xform = xform;
+    S.shade.resize(blas_count);
+    S.mat.resize(blas_count);
+    for (uint32_t k = 0; k < blas_count; k++) {
+        S.shade[k].assign(shade_normals[k], shade_normals[k] + 9 * size_t(blas_tri_counts[k]));
+        S.mat[k].assign(shade_mat[k], shade_mat[k] + blas_tri_counts[k]);
+    }
+    for (uint32_t i = 0; i < n_mats; i++) {
+        const auto& m = mats[i];
+        S.mats.push_back(Mat{V3{m.base_color[0], m.base_color[1], m.base_color[2]},
+                             V3{m.emissive[0], m.emissive[1], m.emissive[2]}, m.opacity, m.roughness, m.metalness,
+                             m.ao, m.reflectance, m.two_sided});
+    }
+    // UpdateLights (RayTracingHelper.cpp:436-500)
+    float maxw = 0.0f, total = 0.0f;
+    std::vector<float> wts(n_lights);
+    for (uint32_t l = 0; l < n_lights; l++) {
+        const float* r = lights[l].radiance;
+        wts[l] = (r[0] * 0.3333f + r[1] * 0.3333f) + r[2] * 0.3333f;
+        maxw = std::fmax(maxw, wts[l]);
+    }
+    for (uint32_t l = 0; l < n_lights; l++) {
+        wts[l] = std::fmax(wts[l], 0.005f * maxw);
+        total += wts[l];
+    }
+    for (uint32_t l = 0; l < n_lights; l++)
+        S.lights.push_back({V3{lights[l].direction[0], lights[l].direction[1], lights[l].direction[2]},
+                            V3{lights[l].radiance[0], lights[l].radiance[1], lights[l].radiance[2]},
+                            wts[l] / total});
+    S.env = V3{env[0], env[1], env[2]};
+    S.bounces = bounces;
+    Pcg pcg{0ull + 1442695040888963407ull};
+    pcg.next();  // pcg32_init's discarded draw
+    for (uint64_t i = 0; i < pcg_draws; i++) pcg.next();
+    const V3 loc{cam->location[0], cam->location[1], cam->location[2]};
+    const V3 org{cam->origin[0], cam->origin[1], cam->origin[2]};
+    const V3 rgt{cam->right[0], cam->right[1], cam->right[2]};
+    const V3 bot{cam->bottom[0], cam->bottom[1], cam->bottom[2]};
+    for (uint32_t f = 0; f < frames; f++) {
+        std::vector<float> seeds(size_t(bounces) + 1);
+        for (int32_t b = 0; b <= bounces; b++) seeds[b] = pcg.fast_uniform();
+        const float sc = float(first_sample + f);
+        const float jx = rand2(sc, 0.0f), jy = rand2(sc, 1.0f);
+        const uint32_t n = width * height;
+        auto work = [&](uint32_t lo, uint32_t hi) {
+            std::vector<int32_t> stack;
+            for (uint32_t pix = lo; pix < hi; pix++) {
+                const uint32_t px = pix % width, py = pix / width;
+                float cx = (float(px) + jx) / float(width);
+                float cy = (float(py) + jy) / float(height);
+                V3 dir = nrm(((org + mul(rgt, cx)) + mul(bot, cy)) - loc);
+                V3 L = trace_path(S, loc, dir, float(pix), seeds.data(), stack);
+                accum[4 * size_t(pix)] += L.x;
+                accum[4 * size_t(pix) + 1] += L.y;
+                accum[4 * size_t(pix) + 2] += L.z;
+                accum[4 * size_t(pix) + 3] += 1.0f;
+            }
+        };
+        if (threads <= 1) {
+            work(0, n);
+        } else {
+            std::vector<std::thread> pool;
+            uint32_t chunk = (n + uint32_t(threads) - 1) / uint32_t(threads);
+            for (int t = 0; t < threads; t++) {
+                uint32_t lo = uint32_t(t) * chunk, hi = std::min(n, lo + chunk);
+                if (lo < hi) pool.emplace_back(work, lo, hi);
+            }
+            for (auto& th : pool) th.join();
+        }
+    }
+}
+
+}  // extern "C"

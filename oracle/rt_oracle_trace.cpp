@@ -19,29 +19,9 @@
 #include <cmath>
 
 #include "../include/atlas_rt.h"
-#include "rt_oracle_math.h"
+#include "rt_oracle.h"
 
 namespace oracle {
-
-struct TraceStats {
-    uint32_t node_visits = 0;  // inner nodes fetched (BLAS + TLAS)
-    uint32_t tri_tests = 0;    // triangles intersected
-    uint32_t inst_visits = 0;  // instances entered (CheckInstance calls)
-};
-
-struct HitRec {
-    float t = 0, u = 0, v = 0;
-    int32_t slot = -1;      // engine hitID (BVH slot in the BLAS)
-    int32_t instance = -1;  // engine hitInstanceID (TLAS instance slot)
-    float ho[3] = {0, 0, 0}, hd[3] = {0, 0, 0};  // object-space ray of the accepted test
-};
-
-struct BlasView {
-    const atlas_rt_gpu_bvh_node* nodes;
-    uint32_t node_count;
-    const atlas_rt_gpu_bvh_triangle* tris;
-    uint32_t tri_count;
-};
 
 static inline V3 node_lmin(const atlas_rt_gpu_bvh_node& n) { return V3{n.left_min[0], n.left_min[1], n.left_min[2]}; }
 static inline V3 node_lmax(const atlas_rt_gpu_bvh_node& n) { return V3{n.left_max[0], n.left_max[1], n.left_max[2]}; }
@@ -179,23 +159,14 @@ static bool any_rgpu_blas(const BlasView& b, V3 o, V3 d, float tmin, float tmax,
 // CheckInstance (bvh.hsh:172-273, 359-441).  The TLAS/BLAS mode switch uses the
 // engine's tlasIndex marker on a single stack.
 // ---------------------------------------------------------------------------
-struct World {
-    const atlas_rt_gpu_bvh_node* tlas;
-    uint32_t tlas_count;
-    const atlas_rt_gpu_bvh_instance* inst;
-    uint32_t inst_count;
-    const BlasView* blas;
-    uint32_t blas_count;
-};
-
 static inline float dot4(float ax, float ay, float az, float aw, const float* m) {
     return ((ax * m[0] + ay * m[1]) + az * m[2]) + aw * m[3];
 }
 
 static const uint32_t kTlasInvalid = 0xFFFFFFFFu;
 
-static void closest_rgpu_world(const World& w, V3 o0, V3 d0, float tmin, float tmax, uint32_t mask,
-                               HitRec& h, TraceStats& st, std::vector<int32_t>& stack) {
+void closest_rgpu_world(const World& w, V3 o0, V3 d0, float tmin, float tmax, uint32_t mask,
+                        HitRec& h, TraceStats& st, std::vector<int32_t>& stack) {
     h = HitRec();
     h.t = tmax;
     if (is_nan3(d0)) return;
@@ -292,8 +263,8 @@ static void closest_rgpu_world(const World& w, V3 o0, V3 d0, float tmin, float t
 // restore_always = true: HitAnyTransparency's traversal (bvh.hsh:466-470),
 // which restores on every TLAS step; with opaque triangles its result is this
 // any-hit test.
-static bool any_rgpu_world(const World& w, V3 o0, V3 d0, float tmin, float tmax, uint32_t mask,
-                           TraceStats& st, std::vector<int32_t>& stack, bool restore_always) {
+bool any_rgpu_world(const World& w, V3 o0, V3 d0, float tmin, float tmax, uint32_t mask, TraceStats& st,
+                    std::vector<int32_t>& stack, bool restore_always) {
     if (is_nan3(d0)) return false;
     stack.clear();
     V3 o = o0, d = d0;

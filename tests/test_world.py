@@ -172,7 +172,7 @@ def test_gpu_world_closest_any_bit_exact(torch_cuda, salt, n_inst):
         assert np.array_equal(d_any.cpu().numpy(), ar["prim_id"])
         ar2, _, _ = oracle_rt.trace_world(tl, inst, bl, rays, mask=mask, mode=oracle_rt.WORLD_ANY_RESTORE)
         assert np.array_equal(d_any2.cpu().numpy(), ar2["prim_id"])
-        if mask == _abi.MASK_ALL:
+        if mask == _abi.MASK_ALL and n_inst >= 24:
             assert (hits["prim_id"] >= 0).sum() > n // 20
 
 
