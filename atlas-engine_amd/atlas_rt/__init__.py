@@ -410,3 +410,67 @@ class PathTracer:
         n = C.c_int32()
         check(lib().atlas_rt_pathtracer_sample_count(self._h, C.byref(n)))
         return n.value
+
+
+class RadianceCache:
+    """Online-trained hash-grid + MLP radiance cache (atlas_rt_nrc_*).
+
+    Queries are (N, 12) float32 device tensors: (position xyz, roughness),
+    (outgoing direction xyz, unused), (normal xyz, unused); targets and
+    outputs are (N, 4) float32 (rgb, unused)."""
+
+    def __init__(self, bounds_min, bounds_max, levels=12, log2_table=19, base_resolution=16, max_resolution=2048,
+                 learning_rate=1e-2, seed=0):
+        p = _abi.NrcParams((C.c_float * 3)(*bounds_min), (C.c_float * 3)(*bounds_max), levels, log2_table,
+                           base_resolution, max_resolution, learning_rate, seed)
+        self.params = p
+        out = C.c_void_p()
+        check(lib().atlas_rt_nrc_create(C.byref(p), C.byref(out)))
+        self._h = out
+        m, t = C.c_uint32(), C.c_uint32()
+        check(lib().atlas_rt_nrc_sizes(self._h, C.byref(m), C.byref(t)))
+        self.mlp_size, self.table_size = m.value, t.value
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            check(lib().atlas_rt_nrc_destroy(self._h))
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def infer(self, d_queries, out=None, stream=None):
+        import torch
+        n = d_queries.shape[0]
+        if out is None:
+            out = torch.empty((n, 4), dtype=torch.float32, device=d_queries.device)
+        check(lib().atlas_rt_nrc_infer(self._h, _ptr(d_queries), _ptr(out), n, _stream_ptr(stream)))
+        return out
+
+    def train(self, d_queries, d_targets, d_loss=None, stream=None):
+        check(lib().atlas_rt_nrc_train(self._h, _ptr(d_queries), _ptr(d_targets), d_queries.shape[0],
+                                       _ptr(d_loss) if d_loss is not None else None, _stream_ptr(stream)))
+
+    def gradients(self, d_queries, d_targets, stream=None):
+        """(weight gradients (mlp_size,), table gradients (table_size,)), FP32 device tensors."""
+        import torch
+        gm = torch.empty(self.mlp_size, dtype=torch.float32, device=d_queries.device)
+        gt = torch.empty(self.table_size, dtype=torch.float32, device=d_queries.device)
+        check(lib().atlas_rt_nrc_gradients(self._h, _ptr(d_queries), _ptr(d_targets), d_queries.shape[0], _ptr(gm),
+                                           _ptr(gt), _stream_ptr(stream)))
+        return gm, gt
+
+    def get_params(self):
+        """(FP32 master weights, FP16 table as int16 bits) device tensors."""
+        import torch
+        m = torch.empty(self.mlp_size, dtype=torch.float32, device="cuda")
+        t = torch.empty(self.table_size, dtype=torch.int16, device="cuda")
+        check(lib().atlas_rt_nrc_get_params(self._h, _ptr(m), _ptr(t)))
+        return m, t
+
+    def set_params(self, mlp=None, table=None):
+        check(lib().atlas_rt_nrc_set_params(self._h, _ptr(mlp) if mlp is not None else None,
+                                            _ptr(table) if table is not None else None))

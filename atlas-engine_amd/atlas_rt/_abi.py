@@ -90,6 +90,13 @@ class PtParams(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("bounces", C.c_int32), ("exposure", C.c_float),
                 ("env_radiance", C.c_float * 3), ("flags", C.c_uint32)]
 
+class NrcParams(C.Structure):
+    """atlas_rt_nrc_params."""
+    _fields_ = [("bounds_min", C.c_float * 3), ("bounds_max", C.c_float * 3), ("levels", C.c_uint32),
+                ("log2_table", C.c_uint32), ("base_resolution", C.c_uint32), ("max_resolution", C.c_uint32),
+                ("learning_rate", C.c_float), ("seed", C.c_uint32)]
+
+
 assert NODE_DTYPE.itemsize == 64 and GPU_TRI_DTYPE.itemsize == 48
 assert INSTANCE_DTYPE.itemsize == 64 and PACKED_RAY_DTYPE.itemsize == 48
 assert BVH_TRI_DTYPE.itemsize == 40 and RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 16
@@ -141,4 +148,12 @@ PROTOTYPES = {
     "atlas_rt_pathtracer_reset": (status, [vp]),
     "atlas_rt_pathtracer_read": (status, [vp, vp, vp]),
     "atlas_rt_pathtracer_sample_count": (status, [vp, C.POINTER(i32)]),
+    "atlas_rt_nrc_create": (status, [vp, C.POINTER(vp)]),
+    "atlas_rt_nrc_destroy": (status, [vp]),
+    "atlas_rt_nrc_infer": (status, [vp, vp, vp, u32, vp]),
+    "atlas_rt_nrc_train": (status, [vp, vp, vp, u32, vp, vp]),
+    "atlas_rt_nrc_gradients": (status, [vp, vp, vp, u32, vp, vp, vp]),
+    "atlas_rt_nrc_get_params": (status, [vp, vp, vp]),
+    "atlas_rt_nrc_set_params": (status, [vp, vp, vp]),
+    "atlas_rt_nrc_sizes": (status, [vp, C.POINTER(u32), C.POINTER(u32)]),
 }

@@ -451,6 +451,62 @@ atlas_rt_status atlas_rt_pathtracer_reset(atlas_rt_pathtracer* pt);
 atlas_rt_status atlas_rt_pathtracer_read(atlas_rt_pathtracer* pt, float* d_out, void* stream);
 atlas_rt_status atlas_rt_pathtracer_sample_count(const atlas_rt_pathtracer* pt, int32_t* count);
 
+/* ------------------------------------------------------------------------- */
+/* Radiance cache (no engine counterpart)                                    */
+/* ------------------------------------------------------------------------- */
+/* An online-trained neural radiance cache: a multi-resolution hash-grid
+ * encoding of position (up to 12 levels x 2 features, 2^log2_table entries
+ * per level, FP16 table) concatenated with the outgoing direction, the
+ * shading normal, the roughness and a constant 1 (32 inputs) feeding a
+ * 5 x 64 ReLU MLP with 3 linear outputs (no biases).  Weights are BF16 on
+ * the matrix cores with FP32 accumulation; FP32 master weights and tables
+ * with Adam moments.
+ * Input vector: [0, 2*levels) hash features (level-major), zeros up to 24,
+ *   24..26 outgoing direction, 27..29 normal, 30 roughness, 31 = 1.
+ * Queries: 12 floats each, three float4: (position xyz, roughness),
+ *   (outgoing direction xyz, unused), (normal xyz, unused).
+ * Targets and outputs: 4 floats each (rgb radiance, unused). */
+typedef struct atlas_rt_nrc atlas_rt_nrc;
+
+typedef struct atlas_rt_nrc_params {
+    float bounds_min[3];   /* positions are normalised to this box */
+    float bounds_max[3];
+    uint32_t levels;       /* 1..12 */
+    uint32_t log2_table;   /* 10..24, e.g. 19 */
+    uint32_t base_resolution;   /* e.g. 16 */
+    uint32_t max_resolution;    /* e.g. 2048; level l has base * b^l cells, b = (max/base)^(1/(levels-1)) */
+    float learning_rate;   /* Adam step size, e.g. 1e-2 */
+    uint32_t seed;         /* initialisation seed */
+} atlas_rt_nrc_params;
+
+atlas_rt_status atlas_rt_nrc_create(const atlas_rt_nrc_params* params, atlas_rt_nrc** out);
+atlas_rt_status atlas_rt_nrc_destroy(atlas_rt_nrc* nrc);
+
+/* Predicted rgb radiance (4 floats per query, 4th unused) on `stream`. */
+atlas_rt_status atlas_rt_nrc_infer(atlas_rt_nrc* nrc, const float* d_queries, float* d_out, uint32_t count,
+                                   void* stream);
+
+/* One Adam step (beta1 0.9, beta2 0.99, eps 1e-15; table entries whose
+ * gradient is exactly zero are left untouched) on the relative L2 loss
+ * sum((y - t)^2 / (sg(y)^2 + 0.01)) / (3 count), sg = no gradient through
+ * the denominator.  If d_loss != NULL the batch loss is written there. */
+atlas_rt_status atlas_rt_nrc_train(atlas_rt_nrc* nrc, const float* d_queries, const float* d_targets,
+                                   uint32_t count, float* d_loss, void* stream);
+
+/* Gradients of the same loss without an update (for checking): d_grad_mlp
+ * receives the FP32 weight gradients in the master layout (layer by layer,
+ * K x N row-major), d_grad_table the FP32 table gradients (levels x 2^log2 x 2). */
+atlas_rt_status atlas_rt_nrc_gradients(atlas_rt_nrc* nrc, const float* d_queries, const float* d_targets,
+                                       uint32_t count, float* d_grad_mlp, float* d_grad_table, void* stream);
+
+/* Parameter access: FP32 master MLP weights (same layout as the gradients)
+ * and the FP16 table as uint16 bits.  Either pointer may be NULL.
+ * Device pointers.  set_params also resets the Adam moments and step. */
+atlas_rt_status atlas_rt_nrc_get_params(atlas_rt_nrc* nrc, float* d_mlp, uint16_t* d_table);
+atlas_rt_status atlas_rt_nrc_set_params(atlas_rt_nrc* nrc, const float* d_mlp, const uint16_t* d_table);
+/* Sizes in elements: MLP weights, table entries (levels x 2^log2 x 2). */
+atlas_rt_status atlas_rt_nrc_sizes(const atlas_rt_nrc* nrc, uint32_t* mlp_params, uint32_t* table_params);
+
 #ifdef __cplusplus
 }
 #endif
