@@ -411,6 +411,18 @@ class PathTracer:
         check(lib().atlas_rt_pathtracer_sample_count(self._h, C.byref(n)))
         return n.value
 
+    def set_cache(self, cache, train_stride=32, train_steps=4):
+        """Attach a RadianceCache (None: plain path tracing)."""
+        p = _abi.PtCacheParams(train_stride, train_steps)
+        check(lib().atlas_rt_pathtracer_set_cache(self._h, cache._h if cache is not None else None, C.byref(p)))
+        self._cache = cache
+
+    def cache_stats(self):
+        """(training samples, cache queries, last training loss) of the last frame."""
+        n, q, l = C.c_uint32(), C.c_uint32(), C.c_float()
+        check(lib().atlas_rt_pathtracer_cache_stats(self._h, C.byref(n), C.byref(q), C.byref(l)))
+        return n.value, q.value, l.value
+
 
 class RadianceCache:
     """Online-trained hash-grid + MLP radiance cache (atlas_rt_nrc_*).

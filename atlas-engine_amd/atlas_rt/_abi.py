@@ -97,6 +97,11 @@ class NrcParams(C.Structure):
                 ("learning_rate", C.c_float), ("seed", C.c_uint32)]
 
 
+class PtCacheParams(C.Structure):
+    """atlas_rt_pt_cache_params."""
+    _fields_ = [("train_stride", C.c_uint32), ("train_steps", C.c_uint32), ("reserved", C.c_uint32 * 2)]
+
+
 assert NODE_DTYPE.itemsize == 64 and GPU_TRI_DTYPE.itemsize == 48
 assert INSTANCE_DTYPE.itemsize == 64 and PACKED_RAY_DTYPE.itemsize == 48
 assert BVH_TRI_DTYPE.itemsize == 40 and RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 16
@@ -156,4 +161,6 @@ PROTOTYPES = {
     "atlas_rt_nrc_get_params": (status, [vp, vp, vp]),
     "atlas_rt_nrc_set_params": (status, [vp, vp, vp]),
     "atlas_rt_nrc_sizes": (status, [vp, C.POINTER(u32), C.POINTER(u32)]),
+    "atlas_rt_pathtracer_set_cache": (status, [vp, vp, vp]),
+    "atlas_rt_pathtracer_cache_stats": (status, [vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(f32)]),
 }

@@ -60,6 +60,26 @@ atlas_rt_status check_device(const atlas_rt_nrc* N, const char* fn) {
 
 }  // namespace
 
+atlas_rt_status atlas_rt::nrc::train_step(atlas_rt_nrc& N, const float4* queries, const float4* targets,
+                                          const uint32_t* d_count, uint32_t count, uint32_t slice, uint32_t slices,
+                                          float* d_loss, hipStream_t s) {
+    uint32_t blocks = 0;
+    atlas_rt_status st = hip_check(launch_train(N.dev, queries, targets, d_count, count, slice, slices, N.partial,
+                                                N.max_blocks, N.grad_tab, d_loss ? d_loss : N.loss, &blocks, s),
+                                   "nrc train launch");
+    if (st != ATLAS_RT_OK) return st;
+    N.step++;
+    // Adam bias correction folded into the step size
+    const double b1t = std::pow(0.9, double(N.step)), b2t = std::pow(0.99, double(N.step));
+    const float lr_t = float(double(N.params.learning_rate) * std::sqrt(1.0 - b2t) / (1.0 - b1t));
+    st = hip_check(launch_mlp_reduce(N.partial, blocks, nullptr, N.master, N.m_mlp, N.v_mlp, lr_t, s), "nrc adam");
+    if (st == ATLAS_RT_OK) st = hip_check(launch_pack(N.master, N.wn, N.wk, s), "nrc pack");
+    if (st == ATLAS_RT_OK)
+        st = hip_check(launch_table_adam(N.table32, N.table16, N.m_tab, N.v_tab, N.grad_tab, N.table_params, lr_t, s),
+                       "nrc table adam");
+    return st;
+}
+
 extern "C" {
 
 atlas_rt_status atlas_rt_nrc_create(const atlas_rt_nrc_params* params, atlas_rt_nrc** out) {
@@ -174,7 +194,7 @@ atlas_rt_status atlas_rt_nrc_infer(atlas_rt_nrc* nrc, const float* d_queries, fl
     atlas_rt_status st = check_device(nrc, "atlas_rt_nrc_infer");
     if (st != ATLAS_RT_OK) return st;
     return hip_check(launch_infer(nrc->dev, reinterpret_cast<const float4*>(d_queries), reinterpret_cast<float4*>(d_out),
-                                  count, static_cast<hipStream_t>(stream)),
+                                  nullptr, count, static_cast<hipStream_t>(stream)),
                      "nrc infer launch");
 }
 
@@ -192,23 +212,8 @@ atlas_rt_status atlas_rt_nrc_train(atlas_rt_nrc* nrc, const float* d_queries, co
     atlas_rt_status st = check_train_args(nrc, d_queries, d_targets, count, "atlas_rt_nrc_train");
     if (st != ATLAS_RT_OK) return st;
     if (count == 0) return ATLAS_RT_OK;
-    atlas_rt_nrc& N = *nrc;
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    uint32_t blocks = 0;
-    st = hip_check(launch_train(N.dev, reinterpret_cast<const float4*>(d_queries),
-                                reinterpret_cast<const float4*>(d_targets), count, N.partial, N.max_blocks, N.grad_tab,
-                                d_loss ? d_loss : N.loss, &blocks, s),
-                   "nrc train launch");
-    if (st != ATLAS_RT_OK) return st;
-    N.step++;
-    const double b1t = std::pow(0.9, double(N.step)), b2t = std::pow(0.99, double(N.step));
-    const float lr_t = float(double(N.params.learning_rate) * std::sqrt(1.0 - b2t) / (1.0 - b1t));
-    st = hip_check(launch_mlp_reduce(N.partial, blocks, nullptr, N.master, N.m_mlp, N.v_mlp, lr_t, s), "nrc adam");
-    if (st == ATLAS_RT_OK) st = hip_check(launch_pack(N.master, N.wn, N.wk, s), "nrc pack");
-    if (st == ATLAS_RT_OK)
-        st = hip_check(launch_table_adam(N.table32, N.table16, N.m_tab, N.v_tab, N.grad_tab, N.table_params, lr_t, s),
-                       "nrc table adam");
-    return st;
+    return train_step(*nrc, reinterpret_cast<const float4*>(d_queries), reinterpret_cast<const float4*>(d_targets),
+                      nullptr, count, 0, 1, d_loss, static_cast<hipStream_t>(stream));
 }
 
 atlas_rt_status atlas_rt_nrc_gradients(atlas_rt_nrc* nrc, const float* d_queries, const float* d_targets,
@@ -222,8 +227,8 @@ atlas_rt_status atlas_rt_nrc_gradients(atlas_rt_nrc* nrc, const float* d_queries
     uint32_t blocks = 0;
     if (st == ATLAS_RT_OK)
         st = hip_check(launch_train(N.dev, reinterpret_cast<const float4*>(d_queries),
-                                    reinterpret_cast<const float4*>(d_targets), count, N.partial, N.max_blocks,
-                                    d_grad_table, N.loss, &blocks, s),
+                                    reinterpret_cast<const float4*>(d_targets), nullptr, count, 0, 1, N.partial,
+                                    N.max_blocks, d_grad_table, N.loss, &blocks, s),
                        "nrc train launch");
     if (st == ATLAS_RT_OK)
         st = hip_check(launch_mlp_reduce(N.partial, blocks, d_grad_mlp, nullptr, nullptr, nullptr, 0.0f, s),

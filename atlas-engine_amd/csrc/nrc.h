@@ -73,14 +73,17 @@ struct NrcDev {
 };
 
 // Launchers; every launch goes to `stream`.
-hipError_t launch_infer(const NrcDev& d, const float4* queries, float4* out, uint32_t count, hipStream_t stream);
-// Forward + backward over `count` samples: per-block weight-gradient partial
-// sums into partial[blocks][kMasterSize], table gradients added into
-// grad_table, loss added into *loss (cleared first).  Returns the block count
-// used through *blocks.
-hipError_t launch_train(const NrcDev& d, const float4* queries, const float4* targets, uint32_t count,
-                        float* partial, uint32_t max_blocks, float* grad_table, float* loss, uint32_t* blocks,
+// Counts: with d_count set the kernels read the count from device memory
+// (at most `count`), else `count` itself.
+hipError_t launch_infer(const NrcDev& d, const float4* queries, float4* out, const uint32_t* d_count, uint32_t count,
                         hipStream_t stream);
+// Forward + backward over slice `slice` of `slices` equal slices of the
+// samples: per-block weight-gradient partial sums into
+// partial[blocks][kMasterSize], table gradients added into grad_table, loss
+// written to *loss.  Returns the block count used through *blocks.
+hipError_t launch_train(const NrcDev& d, const float4* queries, const float4* targets, const uint32_t* d_count,
+                        uint32_t count, uint32_t slice, uint32_t slices, float* partial, uint32_t max_blocks,
+                        float* grad_table, float* loss, uint32_t* blocks, hipStream_t stream);
 // Sum the partials; with m/v set, apply Adam to `master` (lr_t already
 // bias-corrected), else write the sums to grad_out.
 hipError_t launch_mlp_reduce(const float* partial, uint32_t blocks, float* grad_out, float* master, float* m, float* v,
@@ -94,6 +97,17 @@ hipError_t launch_fill(float* p, uint32_t n, float value, hipStream_t stream);
 hipError_t launch_table_from_f32(const float* t32, uint16_t* t16, uint32_t n, hipStream_t stream);
 hipError_t launch_table_from_f16(const uint16_t* t16, float* t32, uint32_t n, hipStream_t stream);
 
+}  // namespace nrc
+}  // namespace atlas_rt
+
+struct atlas_rt_nrc;
+
+namespace atlas_rt {
+namespace nrc {
+// One Adam step (forward/backward, weight-gradient reduction + update,
+// weight packing, table update) on a slice of the samples; see launch_train.
+atlas_rt_status train_step(atlas_rt_nrc& N, const float4* queries, const float4* targets, const uint32_t* d_count,
+                           uint32_t count, uint32_t slice, uint32_t slices, float* d_loss, hipStream_t s);
 }  // namespace nrc
 }  // namespace atlas_rt
 

@@ -291,10 +291,12 @@ constexpr int kInferQT = kInferQueriesPerWave / 16;
 constexpr int kTrainQT = kTrainQueriesPerWave / 16;
 static_assert(kTrainQT == 2, "the weight-gradient product takes K = 32 queries per chunk");
 
+// count = *d_count when d_count is set (device-resident counts), else n.
 __global__ __launch_bounds__(256) void k_nrc_infer(NrcDev P, const float4* __restrict__ q, float4* __restrict__ out,
-                                                   uint32_t count) {
+                                                   const uint32_t* __restrict__ d_count, uint32_t n) {
     constexpr int QT = kInferQT;
     const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
+    const uint32_t count = d_count ? min(*d_count, n) : n;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t base = wave * (16 * QT);
     if (base >= count) return;  // wave-uniform
@@ -329,12 +331,22 @@ __global__ __launch_bounds__(256) void k_nrc_infer(NrcDev P, const float4* __res
     }
 }
 
+// Trains on slice `slice` of `slices` equal slices of the samples; the
+// sample count is *d_count when d_count is set (at most n), else n.
 __global__ __launch_bounds__(kTrainBlock, 1) void k_nrc_train(NrcDev P, const float4* __restrict__ q,
-                                                               const float4* __restrict__ tgt, uint32_t count,
-                                                               float inv_norm, float* __restrict__ partial,
+                                                               const float4* __restrict__ tgt,
+                                                               const uint32_t* __restrict__ d_count, uint32_t n,
+                                                               uint32_t slice, uint32_t slices,
+                                                               float* __restrict__ partial,
                                                                float* __restrict__ grad_table, float* loss) {
     constexpr int QT = kTrainQT;
     constexpr int kWaves = kTrainBlock / 64;
+    const uint32_t total = d_count ? min(*d_count, n) : n;
+    const uint32_t first = uint32_t(uint64_t(total) * slice / slices);
+    const uint32_t count = uint32_t(uint64_t(total) * (slice + 1) / slices) - first;
+    const float inv_norm = count ? 1.0f / (3.0f * float(count)) : 0.0f;
+    q += 3 * size_t(first);
+    tgt += first;
     __shared__ float dW[kMasterSize];
     __shared__ __align__(16) uint16_t staging[kWaves][2][kStage];
     for (int j = threadIdx.x; j < kMasterSize; j += kTrainBlock) dW[j] = 0.0f;
@@ -524,25 +536,28 @@ __global__ void k_f16_to_f32(const uint16_t* __restrict__ a, float* __restrict__
 
 static inline uint32_t blocks_for(uint32_t n, uint32_t per) { return (n + per - 1) / per; }
 
-hipError_t launch_infer(const NrcDev& d, const float4* queries, float4* out, uint32_t count, hipStream_t stream) {
+hipError_t launch_infer(const NrcDev& d, const float4* queries, float4* out, const uint32_t* d_count, uint32_t count,
+                        hipStream_t stream) {
     if (count == 0) return hipSuccess;
     const uint32_t per_block = 256 / 64 * kInferQueriesPerWave;
-    hipLaunchKernelGGL(k_nrc_infer, dim3(blocks_for(count, per_block)), dim3(256), 0, stream, d, queries, out, count);
+    hipLaunchKernelGGL(k_nrc_infer, dim3(blocks_for(count, per_block)), dim3(256), 0, stream, d, queries, out, d_count,
+                       count);
     return hipGetLastError();
 }
 
-hipError_t launch_train(const NrcDev& d, const float4* queries, const float4* targets, uint32_t count,
-                        float* partial, uint32_t max_blocks, float* grad_table, float* loss, uint32_t* blocks,
-                        hipStream_t stream) {
+hipError_t launch_train(const NrcDev& d, const float4* queries, const float4* targets, const uint32_t* d_count,
+                        uint32_t count, uint32_t slice, uint32_t slices, float* partial, uint32_t max_blocks,
+                        float* grad_table, float* loss, uint32_t* blocks, hipStream_t stream) {
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, stream, loss, 1u, 0.0f);
-    const uint32_t chunks = blocks_for(count, kTrainQueriesPerWave);
+    // with a device-resident count the grid covers the largest slice
+    const uint32_t most = d_count ? blocks_for(count, slices) : count;
+    const uint32_t chunks = blocks_for(most, kTrainQueriesPerWave);
     uint32_t nb = blocks_for(chunks, kTrainBlock / 64);
     if (nb > max_blocks) nb = max_blocks;
-    if (nb == 0) nb = 1;  // count == 0 still writes zero partials
+    if (nb == 0) nb = 1;  // an empty slice still writes zero partials
     *blocks = nb;
-    const float inv_norm = count ? 1.0f / (3.0f * (float)count) : 0.0f;
-    hipLaunchKernelGGL(k_nrc_train, dim3(nb), dim3(kTrainBlock), 0, stream, d, queries, targets, count, inv_norm,
-                       partial, grad_table, loss);
+    hipLaunchKernelGGL(k_nrc_train, dim3(nb), dim3(kTrainBlock), 0, stream, d, queries, targets, d_count, count, slice,
+                       slices, partial, grad_table, loss);
     return hipGetLastError();
 }
 

@@ -1,6 +1,7 @@
 // Path tracer objects: PathTracingRenderer's host side (buffers, light list,
 // per-frame seeds, progressive accumulation) over the wavefront kernels.
 #include "pathtracer.h"
+#include "nrc.h"
 
 #include <algorithm>
 #include <cstring>
@@ -25,6 +26,7 @@ void free_all(PathTracer& P) {
         if (p) (void)hipFree(p);
     P.spill.release();
     P.flags.release();
+    free_cache_buffers(P);
 }
 
 // Random::Init seeds PCG32 with uint64(SampleUniformFloat()) = 0
@@ -36,6 +38,16 @@ uint64_t pcg_initial_state() {
 }
 
 }  // namespace
+
+void atlas_rt::pt::free_cache_buffers(PathTracer& P) {
+    void* ptrs[] = {P.rec, P.q_buf, P.q_meta, P.q_out, P.s_q, P.s_t};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    P.rec = P.q_buf = P.q_meta = P.q_out = P.s_q = P.s_t = nullptr;
+    P.nrc = nullptr;
+    P.train_cap = 0;
+    P.shade_args.cache = CacheArgs{};
+}
 
 extern "C" {
 
@@ -190,6 +202,63 @@ atlas_rt_status atlas_rt_pathtracer_read(atlas_rt_pathtracer* pt, float* d_out, 
 atlas_rt_status atlas_rt_pathtracer_sample_count(const atlas_rt_pathtracer* pt, int32_t* count) {
     if (!pt || !count) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_sample_count: NULL argument");
     *count = pt->P.host_samples;
+    return ATLAS_RT_OK;
+}
+
+atlas_rt_status atlas_rt_pathtracer_set_cache(atlas_rt_pathtracer* pt, atlas_rt_nrc* cache,
+                                              const atlas_rt_pt_cache_params* params) {
+    if (!pt) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_set_cache: NULL path tracer");
+    PathTracer& P = pt->P;
+    ATLAS_HIP_TRY(hipDeviceSynchronize());
+    free_cache_buffers(P);
+    if (!cache) return ATLAS_RT_OK;
+    if (!params) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_set_cache: NULL params");
+    if (P.bounces < 2) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_set_cache: needs bounces >= 2");
+    if (params->train_stride < 1 || params->train_steps > 64)
+        return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_set_cache: need train_stride >= 1, train_steps <= 64");
+    int dev = -1;
+    ATLAS_HIP_TRY(hipGetDevice(&dev));
+    if (dev != cache->device) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_set_cache: cache on another device");
+    const size_t pixels = size_t(P.width) * P.height;
+    // about pixels / stride training paths per frame; the hash selection is
+    // binomial, so leave twice that (the surplus pixels render normally)
+    const size_t cap = std::min(pixels, 2 * (pixels / params->train_stride) + 256);
+    const size_t recs = cap * size_t(P.bounces + 1);
+    atlas_rt_status st = ATLAS_RT_OK;
+    if (st == ATLAS_RT_OK) st = dev_alloc(P.rec, 5 * recs);
+    if (st == ATLAS_RT_OK) st = dev_alloc(P.q_buf, 3 * pixels);
+    if (st == ATLAS_RT_OK) st = dev_alloc(P.q_meta, pixels);
+    if (st == ATLAS_RT_OK) st = dev_alloc(P.q_out, pixels);
+    if (st == ATLAS_RT_OK) st = dev_alloc(P.s_q, 3 * recs);
+    if (st == ATLAS_RT_OK) st = dev_alloc(P.s_t, recs);
+    if (st != ATLAS_RT_OK) {
+        free_cache_buffers(P);
+        return st;
+    }
+    P.nrc = cache;
+    P.train_steps = params->train_steps;
+    P.train_cap = uint32_t(cap);
+    CacheArgs& C = P.shade_args.cache;
+    C.rec_q = P.rec;
+    C.rec_beta = P.rec + 3 * recs;
+    C.rec_add = P.rec + 4 * recs;
+    C.q_buf = P.q_buf;
+    C.q_meta = P.q_meta;
+    C.train_cap = uint32_t(cap);
+    C.train_stride = params->train_stride;
+    C.enabled = 1;
+    return ATLAS_RT_OK;
+}
+
+atlas_rt_status atlas_rt_pathtracer_cache_stats(atlas_rt_pathtracer* pt, uint32_t* train_samples, uint32_t* queries,
+                                                float* loss) {
+    if (!pt) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_cache_stats: NULL path tracer");
+    FrameCtl ctl;
+    ATLAS_HIP_TRY(hipDeviceSynchronize());
+    ATLAS_HIP_TRY(hipMemcpy(&ctl, pt->P.ctl, sizeof(ctl), hipMemcpyDeviceToHost));
+    if (train_samples) *train_samples = ctl.train_samples;
+    if (queries) *queries = ctl.queries;
+    if (loss) *loss = ctl.train_loss;
     return ATLAS_RT_OK;
 }
 

@@ -25,6 +25,7 @@ struct ShadeArgs {
     const int4* hits;
     float4* shadow;
     float4* Lpix;
+    CacheArgs cache;
 };
 
 struct PathTracer {
@@ -49,9 +50,20 @@ struct PathTracer {
     Scratch flags;
     ShadeArgs shade_args{};
     int32_t host_samples = 0;
+    // radiance cache (null: plain path tracing)
+    atlas_rt_nrc* nrc = nullptr;
+    uint32_t train_steps = 0;
+    uint32_t train_cap = 0;       // training paths per frame
+    float4* rec = nullptr;        // training records: 5 float4 each (query x3, beta, added radiance)
+    float4* q_buf = nullptr;      // render-path queries, 3 float4 each
+    float4* q_meta = nullptr;
+    float4* q_out = nullptr;
+    float4* s_q = nullptr;        // training samples: queries (3 float4) and targets
+    float4* s_t = nullptr;
 };
 
 atlas_rt_status launch_frame(PathTracer& P, const Camera& cam, hipStream_t s);
+void free_cache_buffers(PathTracer& P);
 
 }  // namespace pt
 }  // namespace atlas_rt

@@ -20,6 +20,7 @@
 // Every count lives on the device, so a whole frame can be graph-captured.
 #include <hip/hip_runtime.h>
 
+#include "nrc.h"
 #include "pathtracer.h"
 #include "pt_device.h"
 #include "trace_world_core.h"
@@ -48,13 +49,26 @@ __global__ void k_frame_begin(FrameCtl* ctl, int32_t bounces, uint32_t pixels) {
     ctl->paths[0] = pixels;
     for (int i = 1; i < 17; i++) ctl->paths[i] = 0;
     for (int i = 0; i < 16; i++) ctl->shadows[i] = 0;
+    ctl->train_paths = 0;
+    ctl->queries = 0;
+    ctl->train_samples = 0;
+}
+
+// Integer hash (murmur3 finaliser) choosing the frame's training pixels.
+__device__ __forceinline__ uint32_t mix32(uint32_t h) {
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
 }
 
 // rayGen.csh:25-90 with one sample per frame (z = 0): jittered pinhole ray
 // through the near-plane corner frame; the write index groups 8x8 tiles.
 __global__ __launch_bounds__(kBlock) void k_raygen(Camera cam, uint32_t width, uint32_t height,
-                                                   const FrameCtl* __restrict__ ctl, float4* __restrict__ paths,
-                                                   float4* __restrict__ Lpix) {
+                                                   FrameCtl* __restrict__ ctl, float4* __restrict__ paths,
+                                                   float4* __restrict__ Lpix, CacheArgs C, int32_t bounces) {
     const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
     if (g >= width * height) return;
     const uint32_t px = g % width, py = g / width;
@@ -76,8 +90,21 @@ __global__ __launch_bounds__(kBlock) void k_raygen(Camera cam, uint32_t width, u
     // ragged right / bottom tiles are packed behind the full ones
     if ((width % 8) || (height % 8)) idx = g;
     const uint32_t pixel = py * width + px;
+    // radiance cache: about one pixel in train_stride traces a training path
+    uint32_t train = 0;
+    if (C.enabled && mix32(pixel * 0x9e3779b1u + ctl->frame * 0x632be5abu) % C.train_stride == 0) {
+        const uint32_t t = atomicAdd(&ctl->train_paths, 1u);
+        if (t < C.train_cap) {
+            train = t + 1;
+            for (int32_t k = 0; k <= bounces; k++) {
+                const size_t r = size_t(t) * (bounces + 1) + k;
+                C.rec_beta[r] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                C.rec_add[r] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            }
+        }
+    }
     paths[3 * size_t(idx)] = make_float4(loc.x, loc.y, loc.z, __uint_as_float(pixel));
-    paths[3 * size_t(idx) + 1] = make_float4(dir.x, dir.y, dir.z, 0.0f);
+    paths[3 * size_t(idx) + 1] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(train));
     paths[3 * size_t(idx) + 2] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     Lpix[pixel] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
@@ -128,7 +155,11 @@ __global__ __launch_bounds__(kBlock) void k_shade(ShadeArgs A, int32_t bounce) {
     const float4 p2 = A.paths_in[3 * size_t(i) + 2];
     const int4 h = A.hits[i];
     const uint32_t pixel = __float_as_uint(p0.w);
-    V thr = mk(p2.x, p2.y, p2.z);
+    // radiance cache: 0 = render path, else training path index + 1
+    const uint32_t train = A.cache.enabled ? __float_as_uint(p1.w) : 0u;
+    const size_t rec = train ? size_t(train - 1) * (A.bounces + 1) + bounce : 0;
+    const V thr_in = mk(p2.x, p2.y, p2.z);
+    V thr = thr_in;
     const V ro = mk(p0.x, p0.y, p0.z), rd = mk(p1.x, p1.y, p1.z);
     float4 acc = A.Lpix[pixel];
     V add = mk(0.0f, 0.0f, 0.0f);
@@ -175,6 +206,25 @@ __global__ __launch_bounds__(kBlock) void k_shade(ShadeArgs A, int32_t bounce) {
         S.V_ = -rd;
         S.N = normalize(normal);
         S.Ng = Ng;
+        if (A.cache.enabled) {
+            const float4 q0 = make_float4(S.P.x, S.P.y, S.P.z, m.roughness);
+            const float4 q1 = make_float4(-rd.x, -rd.y, -rd.z, 0.0f);
+            const float4 q2 = make_float4(S.N.x, S.N.y, S.N.z, 0.0f);
+            if (train) {
+                A.cache.rec_q[3 * rec] = q0;
+                A.cache.rec_q[3 * rec + 1] = q1;
+                A.cache.rec_q[3 * rec + 2] = q2;
+                A.cache.rec_beta[rec] = make_float4(thr_in.x, thr_in.y, thr_in.z, 1.0f);
+            } else if (bounce == 1) {
+                // the render path ends here with the cache's radiance estimate
+                const uint32_t k = atomicAdd(&ctl->queries, 1u);
+                A.cache.q_buf[3 * size_t(k)] = q0;
+                A.cache.q_buf[3 * size_t(k) + 1] = q1;
+                A.cache.q_buf[3 * size_t(k) + 2] = q2;
+                A.cache.q_meta[k] = make_float4(thr_in.x, thr_in.y, thr_in.z, __uint_as_float(pixel));
+                return;
+            }
+        }
         const V base = mk(m.base[0], m.base[1], m.base[2]);
         S.F0 = mixv(mk(0.04f, 0.04f, 0.04f), base, m.metalness);
         S.F90 = 1.0f;
@@ -216,7 +266,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(ShadeArgs A, int32_t bounce) {
                 const V so = S.P + S.N * kEpsilon;
                 A.shadow[3 * size_t(k)] = make_float4(so.x, so.y, so.z, __uint_as_float(pixel));
                 A.shadow[3 * size_t(k) + 1] = make_float4(S.L.x, S.L.y, S.L.z, kInf - 2.0f * kEpsilon);
-                A.shadow[3 * size_t(k) + 2] = make_float4(c.x, c.y, c.z, 0.0f);
+                A.shadow[3 * size_t(k) + 2] = make_float4(c.x, c.y, c.z, __uint_as_float(train ? uint32_t(rec) + 1u : 0u));
             }
         }
         // EvaluateIndirectLight (rayHit.csh:237-325)
@@ -272,11 +322,12 @@ __global__ __launch_bounds__(kBlock) void k_shade(ShadeArgs A, int32_t bounce) {
     acc.y += add.y;
     acc.z += add.z;
     A.Lpix[pixel] = acc;
+    if (train) A.cache.rec_add[rec] = make_float4(add.x, add.y, add.z, 0.0f);
     const float energy = (thr.x + thr.y) + thr.z;
     if (energy == 0.0f || bounce == A.bounces) return;
     const uint32_t k = atomicAdd(&ctl->paths[bounce + 1], 1u);
     A.paths_out[3 * size_t(k)] = make_float4(no.x, no.y, no.z, p0.w);
-    A.paths_out[3 * size_t(k) + 1] = make_float4(nd.x, nd.y, nd.z, 0.0f);
+    A.paths_out[3 * size_t(k) + 1] = make_float4(nd.x, nd.y, nd.z, p1.w);
     A.paths_out[3 * size_t(k) + 2] = make_float4(thr.x, thr.y, thr.z, 0.0f);
 }
 
@@ -286,7 +337,7 @@ template <int BLOCK, int LDS_DEPTH>
 __global__ __launch_bounds__(BLOCK) void k_shadow(WorldDev W, const FrameCtl* __restrict__ ctl, int32_t bounce,
                                                   const float4* __restrict__ shadow, float4* __restrict__ Lpix,
                                                   int32_t* __restrict__ spill, uint32_t cap,
-                                                  uint32_t* __restrict__ flags) {
+                                                  uint32_t* __restrict__ flags, float4* __restrict__ rec_add) {
     __shared__ int32_t lds[LDS_DEPTH * BLOCK];
     const uint32_t gid = blockIdx.x * BLOCK + threadIdx.x;
     if (gid >= ctl->shadows[bounce]) return;
@@ -311,6 +362,55 @@ __global__ __launch_bounds__(BLOCK) void k_shadow(WorldDev W, const FrameCtl* __
     acc.y += c.y;
     acc.z += c.z;
     Lpix[pixel] = acc;
+    const uint32_t rec = __float_as_uint(c.w);
+    if (rec) {
+        float4 a = rec_add[rec - 1];
+        a.x += c.x;
+        a.y += c.y;
+        a.z += c.z;
+        rec_add[rec - 1] = a;
+    }
+}
+
+// Render paths ended at their second vertex: add throughput x cache radiance.
+__global__ __launch_bounds__(kBlock) void k_cache_apply(const FrameCtl* __restrict__ ctl,
+                                                        const float4* __restrict__ meta,
+                                                        const float4* __restrict__ pred, float4* __restrict__ Lpix) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= ctl->queries) return;
+    const float4 m = meta[i];
+    const float4 y = pred[i];
+    const uint32_t pixel = __float_as_uint(m.w);
+    float4 acc = Lpix[pixel];
+    acc.x += m.x * fmaxf(y.x, 0.0f);
+    acc.y += m.y * fmaxf(y.y, 0.0f);
+    acc.z += m.z * fmaxf(y.z, 0.0f);
+    Lpix[pixel] = acc;
+}
+
+// One training sample per surface vertex of every training path: the
+// radiance the path added from that bounce on, over the throughput that
+// reached the vertex.
+__global__ __launch_bounds__(kBlock) void k_build_targets(FrameCtl* __restrict__ ctl, CacheArgs C, int32_t bounces,
+                                                          float4* __restrict__ s_q, float4* __restrict__ s_t) {
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= min(ctl->train_paths, C.train_cap)) return;
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    for (int32_t k = bounces; k >= 0; k--) {
+        const size_t r = size_t(t) * (bounces + 1) + k;
+        const float4 a = C.rec_add[r];
+        sx += a.x;
+        sy += a.y;
+        sz += a.z;
+        const float4 b = C.rec_beta[r];
+        if (b.w == 0.0f) continue;
+        const uint32_t si = atomicAdd(&ctl->train_samples, 1u);
+        s_q[3 * size_t(si)] = C.rec_q[3 * r];
+        s_q[3 * size_t(si) + 1] = C.rec_q[3 * r + 1];
+        s_q[3 * size_t(si) + 2] = C.rec_q[3 * r + 2];
+        s_t[si] = make_float4(b.x > 0.0f ? sx / b.x : 0.0f, b.y > 0.0f ? sy / b.y : 0.0f,
+                              b.z > 0.0f ? sz / b.z : 0.0f, 0.0f);
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_accumulate(FrameCtl* ctl, const float4* __restrict__ Lpix,
@@ -325,7 +425,10 @@ __global__ __launch_bounds__(kBlock) void k_accumulate(FrameCtl* ctl, const floa
         a.w += 1.0f;
         accum[p] = a;
     }
-    if (p == 0) ctl->sample_count += 1;
+    if (p == 0) {
+        ctl->sample_count += 1;
+        ctl->frame += 1;
+    }
 }
 
 }  // namespace
@@ -334,8 +437,10 @@ atlas_rt_status launch_frame(PathTracer& P, const Camera& cam, hipStream_t s) {
     const uint32_t pixels = P.width * P.height;
     const uint32_t grid = (pixels + kBlock - 1) / kBlock;
     uint32_t* flags = reinterpret_cast<uint32_t*>(P.flags.ptr);
+    const CacheArgs& C = P.shade_args.cache;
     hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(64), 0, s, P.ctl, P.bounces, pixels);
-    hipLaunchKernelGGL(k_raygen, dim3(grid), dim3(kBlock), 0, s, cam, P.width, P.height, P.ctl, P.paths[0], P.Lpix);
+    hipLaunchKernelGGL(k_raygen, dim3(grid), dim3(kBlock), 0, s, cam, P.width, P.height, P.ctl, P.paths[0], P.Lpix, C,
+                       P.bounces);
     for (int32_t b = 0; b <= P.bounces; b++) {
         float4* in = P.paths[b & 1];
         float4* out = P.paths[(b + 1) & 1];
@@ -347,7 +452,21 @@ atlas_rt_status launch_frame(PathTracer& P, const Camera& cam, hipStream_t s) {
         hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kBlock), 0, s, A, b);
         if (P.light_count > 0)
             hipLaunchKernelGGL((k_shadow<kBlock, kLdsDepth>), dim3(grid), dim3(kBlock), 0, s, P.world_dev, P.ctl, b,
-                               P.shadow, P.Lpix, reinterpret_cast<int32_t*>(P.spill.ptr), P.cap, flags);
+                               P.shadow, P.Lpix, reinterpret_cast<int32_t*>(P.spill.ptr), P.cap, flags, C.rec_add);
+        if (C.enabled && b == 1) {
+            ATLAS_HIP_TRY(nrc::launch_infer(P.nrc->dev, P.q_buf, P.q_out, &P.ctl->queries, pixels, s));
+            hipLaunchKernelGGL(k_cache_apply, dim3(grid), dim3(kBlock), 0, s, P.ctl, P.q_meta, P.q_out, P.Lpix);
+        }
+    }
+    if (C.enabled && P.train_steps > 0) {
+        hipLaunchKernelGGL(k_build_targets, dim3((P.train_cap + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P.ctl, C,
+                           P.bounces, P.s_q, P.s_t);
+        const uint32_t max_samples = P.train_cap * uint32_t(P.bounces + 1);
+        for (uint32_t k = 0; k < P.train_steps; k++) {
+            atlas_rt_status st = nrc::train_step(*P.nrc, P.s_q, P.s_t, &P.ctl->train_samples, max_samples, k,
+                                                 P.train_steps, &P.ctl->train_loss, s);
+            if (st != ATLAS_RT_OK) return st;
+        }
     }
     hipLaunchKernelGGL(k_accumulate, dim3(grid), dim3(kBlock), 0, s, P.ctl, P.Lpix, P.accum, pixels);
     ATLAS_HIP_TRY(hipGetLastError());

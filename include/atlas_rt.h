@@ -507,6 +507,32 @@ atlas_rt_status atlas_rt_nrc_set_params(atlas_rt_nrc* nrc, const float* d_mlp, c
 /* Sizes in elements: MLP weights, table entries (levels x 2^log2 x 2). */
 atlas_rt_status atlas_rt_nrc_sizes(const atlas_rt_nrc* nrc, uint32_t* mlp_params, uint32_t* table_params);
 
+/* Path tracing with the cache.  With a cache attached, every camera path is
+ * traced to its second surface vertex and ended there with the cache's
+ * prediction of the radiance leaving that vertex towards the first (clamped
+ * at 0, times the path throughput).  About one pixel in train_stride per
+ * frame (a different random set each frame) instead traces the full
+ * `bounces` path as in plain mode; every surface vertex of such a path
+ * becomes a training sample: the radiance the rest of the path brought back
+ * to the pixel divided by the throughput that reached the vertex.  After the
+ * frame's bounces the cache takes train_steps Adam steps on those samples
+ * (train_steps = 0 freezes it).  cache = NULL detaches it: plain path
+ * tracing, the fallback.  Needs bounces >= 2.  The cache must outlive its
+ * use by the path tracer. */
+typedef struct atlas_rt_pt_cache_params {
+    uint32_t train_stride;   /* e.g. 32 */
+    uint32_t train_steps;    /* Adam steps per frame, e.g. 4 */
+    uint32_t reserved[2];
+} atlas_rt_pt_cache_params;
+
+atlas_rt_status atlas_rt_pathtracer_set_cache(atlas_rt_pathtracer* pt, atlas_rt_nrc* cache,
+                                              const atlas_rt_pt_cache_params* params);
+
+/* Counts of the last rendered frame (synchronous): training samples built,
+ * cache queries issued, and the loss of its last training step. */
+atlas_rt_status atlas_rt_pathtracer_cache_stats(atlas_rt_pathtracer* pt, uint32_t* train_samples, uint32_t* queries,
+                                                float* loss);
+
 #ifdef __cplusplus
 }
 #endif
