@@ -417,6 +417,15 @@ class PathTracer:
         check(lib().atlas_rt_pathtracer_set_cache(self._h, cache._h if cache is not None else None, C.byref(p)))
         self._cache = cache
 
+    def cache_samples(self, max_count):
+        """The last frame's training samples: ((n, 12), (n, 4)) float32 device tensors."""
+        import torch
+        q = torch.empty((max_count, 12), dtype=torch.float32, device="cuda")
+        t = torch.empty((max_count, 4), dtype=torch.float32, device="cuda")
+        n = C.c_uint32()
+        check(lib().atlas_rt_pathtracer_cache_samples(self._h, _ptr(q), _ptr(t), max_count, C.byref(n)))
+        return q[:n.value], t[:n.value]
+
     def cache_stats(self):
         """(training samples, cache queries, last training loss) of the last frame."""
         n, q, l = C.c_uint32(), C.c_uint32(), C.c_float()

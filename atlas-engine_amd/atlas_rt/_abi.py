@@ -163,4 +163,5 @@ PROTOTYPES = {
     "atlas_rt_nrc_sizes": (status, [vp, C.POINTER(u32), C.POINTER(u32)]),
     "atlas_rt_pathtracer_set_cache": (status, [vp, vp, vp]),
     "atlas_rt_pathtracer_cache_stats": (status, [vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(f32)]),
+    "atlas_rt_pathtracer_cache_samples": (status, [vp, vp, vp, u32, C.POINTER(u32)]),
 }

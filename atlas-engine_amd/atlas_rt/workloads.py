@@ -227,6 +227,29 @@ def c2_scene(n_meshes=8, n_instances=256, seed_salt=0):
     return meshes, mats, xf, which
 
 
+def c4_scene(seed_salt=0, extent=15.0, height=15.0):
+    """Radiance-cache workload: config 2's meshes, materials and 256
+    instances, with the translations drawn over a 30 x 30 x 30 box instead of
+    200 x 40 x 200 so that light inter-reflects between the blobs (in config
+    2's layout nearly every second-bounce ray escapes to the sky and there is
+    no path tail for a cache to replace)."""
+    g = rng(2, seed_salt)
+    meshes = [blob_mesh(g) for _ in range(8)]
+    mats = {
+        "albedo": g.uniform(0.1, 0.9, (8, 3)).astype(np.float32),
+        "roughness": g.uniform(0.05, 1.0, 8).astype(np.float32),
+        "metalness": np.zeros(8, np.float32),
+    }
+    xf = random_affine(g, 256, extent=extent, height=height)
+    which = g.integers(0, 8, 256)
+    return meshes, mats, xf, which
+
+
+def c4_camera(width=1280, height=720, eye=(0.0, 0.0, -17.0), pitch_deg=0.0):
+    """Eye at the edge of the dense instance cluster, looking in."""
+    return dict(eye=eye, yaw_deg=0.0, pitch_deg=pitch_deg, fov_y_deg=55.0, width=width, height=height)
+
+
 C2_SUN_DIR = np.array([-0.3, -1.0, -0.2]) / np.linalg.norm([-0.3, -1.0, -0.2])  # direction light travels
 
 

@@ -262,4 +262,24 @@ atlas_rt_status atlas_rt_pathtracer_cache_stats(atlas_rt_pathtracer* pt, uint32_
     return ATLAS_RT_OK;
 }
 
+atlas_rt_status atlas_rt_pathtracer_cache_samples(atlas_rt_pathtracer* pt, float* d_queries, float* d_targets,
+                                                  uint32_t max_count, uint32_t* count) {
+    if (!pt || !count) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_cache_samples: NULL argument");
+    PathTracer& P = pt->P;
+    *count = 0;
+    if (!P.nrc) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_cache_samples: no cache attached");
+    FrameCtl ctl;
+    ATLAS_HIP_TRY(hipDeviceSynchronize());
+    ATLAS_HIP_TRY(hipMemcpy(&ctl, P.ctl, sizeof(ctl), hipMemcpyDeviceToHost));
+    const uint32_t n = std::min(std::min(ctl.train_samples, P.train_cap * uint32_t(P.bounces + 1)), max_count);
+    if (n && (!d_queries || !d_targets))
+        return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_cache_samples: NULL buffer");
+    if (n) {
+        ATLAS_HIP_TRY(hipMemcpy(d_queries, P.s_q, size_t(n) * 3 * sizeof(float4), hipMemcpyDeviceToDevice));
+        ATLAS_HIP_TRY(hipMemcpy(d_targets, P.s_t, size_t(n) * sizeof(float4), hipMemcpyDeviceToDevice));
+    }
+    *count = n;
+    return ATLAS_RT_OK;
+}
+
 }  // extern "C"

@@ -458,7 +458,7 @@ atlas_rt_status launch_frame(PathTracer& P, const Camera& cam, hipStream_t s) {
             hipLaunchKernelGGL(k_cache_apply, dim3(grid), dim3(kBlock), 0, s, P.ctl, P.q_meta, P.q_out, P.Lpix);
         }
     }
-    if (C.enabled && P.train_steps > 0) {
+    if (C.enabled) {
         hipLaunchKernelGGL(k_build_targets, dim3((P.train_cap + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P.ctl, C,
                            P.bounces, P.s_q, P.s_t);
         const uint32_t max_samples = P.train_cap * uint32_t(P.bounces + 1);

@@ -533,6 +533,12 @@ atlas_rt_status atlas_rt_pathtracer_set_cache(atlas_rt_pathtracer* pt, atlas_rt_
 atlas_rt_status atlas_rt_pathtracer_cache_stats(atlas_rt_pathtracer* pt, uint32_t* train_samples, uint32_t* queries,
                                                 float* loss);
 
+/* The last frame's training samples (synchronous): up to max_count queries
+ * (12 floats each) and targets (4 floats each) into device buffers, in no
+ * particular order; *count receives how many were copied. */
+atlas_rt_status atlas_rt_pathtracer_cache_samples(atlas_rt_pathtracer* pt, float* d_queries, float* d_targets,
+                                                  uint32_t max_count, uint32_t* count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -234,16 +234,9 @@ def _f32(*xs):
     return [np.ascontiguousarray(x, dtype=np.float32) for x in xs]
 
 
-def pathtrace(tlas_nodes, instances, xform, blases, shading, materials, lights, env, cam, width, height, bounces,
-              frames, first_sample=0, pcg_draws=0, accum=None, threads=None):
-    """CPU restatement of the progressive path tracer.
-
-    blases: list of (nodes, tris) packed arrays; shading: list of (normals
-    (n, 9) float32, material (n,) int32) in BVH slot order; xform: (m, 12)
-    object-to-world rows per packed instance; cam: a ctypes camera struct
-    (location, origin, right, bottom).  Returns the accumulated (h, w, 4)
-    float32 image (rgb sum, sample count).
-    """
+def _scene_args(tlas_nodes, instances, xform, blases, shading, materials, lights, env):
+    """ctypes argument list of the scene part shared by the path tracers, and
+    the arrays it points into (keep them alive across the call)."""
     k = len(blases)
     keep = []
     node_ptrs, tri_ptrs = (C.c_void_p * k)(), (C.c_void_p * k)()
@@ -267,21 +260,66 @@ def pathtrace(tlas_nodes, instances, xform, blases, shading, materials, lights, 
     materials = np.ascontiguousarray(materials)
     lights = np.ascontiguousarray(lights)
     env = np.ascontiguousarray(env, dtype=np.float32)
+    keep += [node_ptrs, tri_ptrs, nrm_ptrs, mat_ptrs, node_counts, tri_counts, tlas_nodes, instances, xform,
+             materials, lights, env]
+    args = [_p(tlas_nodes), len(tlas_nodes), _p(instances), len(instances), C.cast(node_ptrs, C.c_void_p),
+            _p(node_counts), C.cast(tri_ptrs, C.c_void_p), _p(tri_counts), C.cast(nrm_ptrs, C.c_void_p),
+            C.cast(mat_ptrs, C.c_void_p), k, _p(xform), _p(materials), len(materials),
+            _p(lights) if len(lights) else None, len(lights), _p(env)]
+    return args, keep
+
+
+_SCENE_TYPES = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32,
+                C.c_void_p]
+
+
+def pathtrace(tlas_nodes, instances, xform, blases, shading, materials, lights, env, cam, width, height, bounces,
+              frames, first_sample=0, pcg_draws=0, accum=None, threads=None):
+    """CPU restatement of the progressive path tracer.
+
+    blases: list of (nodes, tris) packed arrays; shading: list of (normals
+    (n, 9) float32, material (n,) int32) in BVH slot order; xform: (m, 12)
+    object-to-world rows per packed instance; cam: a ctypes camera struct
+    (location, origin, right, bottom).  Returns the accumulated (h, w, 4)
+    float32 image (rgb sum, sample count).
+    """
+    args, keep = _scene_args(tlas_nodes, instances, xform, blases, shading, materials, lights, env)
     if accum is None:
         accum = np.zeros((height, width, 4), dtype=np.float32)
     L = lib()
     if not getattr(L, "_pt_bound", False):
         vp, u32, i32 = C.c_void_p, C.c_uint32, C.c_int32
         L.oracle_pathtrace.restype = None
-        L.oracle_pathtrace.argtypes = [vp, u32, vp, u32, vp, vp, vp, vp, vp, vp, u32, vp, vp, u32, vp, u32, vp, vp,
-                                       u32, u32, i32, u32, u32, C.c_uint64, vp, i32]
+        L.oracle_pathtrace.argtypes = _SCENE_TYPES + [vp, u32, u32, i32, u32, u32, C.c_uint64, vp, i32]
         L._pt_bound = True
-    L.oracle_pathtrace(_p(tlas_nodes), len(tlas_nodes), _p(instances), len(instances), C.cast(node_ptrs, C.c_void_p),
-                       _p(node_counts), C.cast(tri_ptrs, C.c_void_p), _p(tri_counts), C.cast(nrm_ptrs, C.c_void_p),
-                       C.cast(mat_ptrs, C.c_void_p), k, _p(xform), _p(materials), len(materials),
-                       _p(lights) if len(lights) else None, len(lights), _p(env), C.addressof(cam), width, height,
-                       bounces, frames, first_sample, pcg_draws, _p(accum), threads or threads_default())
+    L.oracle_pathtrace(*args, C.addressof(cam), width, height, bounces, frames, first_sample, pcg_draws, _p(accum),
+                       threads or threads_default())
     return accum
+
+
+def pathtrace_cache(tlas_nodes, instances, xform, blases, shading, materials, lights, env, cam, width, height,
+                    bounces, train_stride, frame_index=0, sample_index=0, pcg_draws=0, threads=None):
+    """One frame of the radiance-cache mode.  Returns (radiance (h, w, 3)
+    added before any cache term, query (h, w, 16): cache query (12), the
+    throughput reaching the vertex (3), flag; train (h, w) bool; rec
+    (h, w, bounces + 1, 20): per bounce of a training path its query (12),
+    throughput reaching it (3), surface flag, radiance added (3), 0)."""
+    args, keep = _scene_args(tlas_nodes, instances, xform, blases, shading, materials, lights, env)
+    L = lib()
+    if not getattr(L, "_ptc_bound", False):
+        vp, u32, i32 = C.c_void_p, C.c_uint32, C.c_int32
+        L.oracle_pathtrace_cache.restype = None
+        L.oracle_pathtrace_cache.argtypes = _SCENE_TYPES + [vp, u32, u32, i32, u32, u32, C.c_uint64, u32, vp, vp, vp,
+                                                            vp, i32]
+        L._ptc_bound = True
+    rad = np.zeros((height, width, 4), np.float32)
+    query = np.zeros((height, width, 16), np.float32)
+    train = np.zeros((height, width), np.int32)
+    rec = np.zeros((height, width, bounces + 1, 20), np.float32)
+    L.oracle_pathtrace_cache(*args, C.addressof(cam), width, height, bounces, frame_index, sample_index, pcg_draws,
+                             train_stride, _p(rad), _p(query), _p(train), _p(rec), threads or threads_default())
+    return rad[..., :3], query, train.astype(bool), rec
 
 
 def triangle_test(o, d, v0, v1, v2):

@@ -212,15 +212,36 @@ inline V3 xf(const float* M, V3 p, float w) {
               (M[8] * p.x + M[9] * p.y) + (M[10] * p.z + M[11] * w)};
 }
 
-V3 trace_path(const Scene& S, V3 o, V3 d, float pixel_seed, const float* seeds, std::vector<int32_t>& stack) {
+// Radiance-cache bookkeeping of one path (both pointers null in plain mode).
+// query: a render path that reaches a surface at bounce 1 stops there and
+//   writes the cache query (position, roughness, -direction, 0, normal, 0),
+//   the throughput reaching the vertex and a flag (16 floats).
+// rec: a training path writes, per bounce k, 20 floats: the vertex's query
+//   (12), the throughput reaching it with w = 1 for a surface hit (4), and
+//   the radiance bounce k added to the pixel (4).
+struct CacheOut {
+    float* query = nullptr;
+    float* rec = nullptr;
+};
+
+V3 trace_path(const Scene& S, V3 o, V3 d, float pixel_seed, const float* seeds, std::vector<int32_t>& stack,
+              const CacheOut& co = CacheOut{}) {
     V3 thr{1, 1, 1}, Lr{0, 0, 0};
     for (int32_t bounce = 0; bounce <= S.bounces; bounce++) {
+        const V3 thr_in = thr;
+        float* rec = co.rec ? co.rec + 20 * size_t(bounce) : nullptr;
         HitRec h;
         TraceStats st;
         closest_rgpu_world(S.w, o, d, 0.0f, kInf, ATLAS_RT_MASK_ALL, h, st, stack);
         if (h.slot < 0) {
             V3 e = S.env * thr;
-            Lr = Lr + V3{std::fmin(e.x, 10.0f), std::fmin(e.y, 10.0f), std::fmin(e.z, 10.0f)};
+            const V3 ec{std::fmin(e.x, 10.0f), std::fmin(e.y, 10.0f), std::fmin(e.z, 10.0f)};
+            Lr = Lr + ec;
+            if (rec) {
+                rec[16] = ec.x;
+                rec[17] = ec.y;
+                rec[18] = ec.z;
+            }
             break;
         }
         const auto& in = S.w.inst[h.instance];
@@ -251,11 +272,32 @@ V3 trace_path(const Scene& S, V3 o, V3 d, float pixel_seed, const float* seeds, 
         sf.Vv = neg(d);
         sf.N = nrm(normal);
         sf.Ng = Ng;
+        const float q12[12] = {sf.P.x, sf.P.y, sf.P.z, m.roughness, -d.x, -d.y, -d.z, 0.0f,
+                               sf.N.x, sf.N.y, sf.N.z, 0.0f};
+        if (co.query && bounce == 1) {
+            std::memcpy(co.query, q12, sizeof(q12));
+            co.query[12] = thr_in.x;
+            co.query[13] = thr_in.y;
+            co.query[14] = thr_in.z;
+            co.query[15] = 1.0f;
+            return Lr;
+        }
+        if (rec) {
+            std::memcpy(rec, q12, sizeof(q12));
+            rec[12] = thr_in.x;
+            rec[13] = thr_in.y;
+            rec[14] = thr_in.z;
+            rec[15] = 1.0f;
+        }
         sf.F0 = V3{mixf(0.04f, m.base.x, m.metalness), mixf(0.04f, m.base.y, m.metalness),
                    mixf(0.04f, m.base.z, m.metalness)};
         sf.F90 = 1.0f;
         sf.NdotL = 0.0f;
-        if ((m.emissive.x + m.emissive.y) + m.emissive.z > 0.0f && bounce == 0) Lr = Lr + m.emissive;
+        V3 emis{0, 0, 0};
+        if ((m.emissive.x + m.emissive.y) + m.emissive.z > 0.0f && bounce == 0) {
+            emis = m.emissive;
+            Lr = Lr + m.emissive;
+        }
         // EvaluateDirectLight
         V3 radiance{0, 0, 0};
         if (!S.lights.empty()) {
@@ -293,6 +335,12 @@ V3 trace_path(const Scene& S, V3 o, V3 d, float pixel_seed, const float* seeds, 
             }
         }
         Lr = Lr + radiance;
+        if (rec) {
+            // emission, then the shadow ray's radiance (the GPU's order)
+            rec[16] = emis.x + radiance.x;
+            rec[17] = emis.y + radiance.y;
+            rec[18] = emis.z + radiance.z;
+        }
         // EvaluateIndirectLight
         V3 no = sf.P;
         float seed = seeds[bounce];
@@ -353,25 +401,17 @@ V3 trace_path(const Scene& S, V3 o, V3 d, float pixel_seed, const float* seeds, 
 
 using namespace oracle;
 
-extern "C" {
+namespace {
 
-// shade_normals[k]: 9 floats per slot of BLAS k (BVH slot order);
-// shade_mat[k]: one material index per slot.  accum: width*height*4 floats
-// (rgb sum, count), accumulated over `frames` frames starting at sample
-// `first_sample`; pcg_draws: number of Random draws made before this call
-// (the PCG stream position).
-void oracle_pathtrace(const atlas_rt_gpu_bvh_node* tlas, uint32_t tlas_count, const atlas_rt_gpu_bvh_instance* inst,
-                      uint32_t inst_count, const atlas_rt_gpu_bvh_node* const* blas_nodes,
-                      const uint32_t* blas_node_counts, const atlas_rt_gpu_bvh_triangle* const* blas_tris,
-                      const uint32_t* blas_tri_counts, const float* const* shade_normals,
-                      const int32_t* const* shade_mat, uint32_t blas_count, const float* xform,
-                      const atlas_rt_material* mats, uint32_t n_mats, const atlas_rt_light* lights, uint32_t n_lights,
-                      const float* env, const atlas_rt_camera* cam, uint32_t width, uint32_t height, int32_t bounces,
-                      uint32_t frames, uint32_t first_sample, uint64_t pcg_draws, float* accum, int32_t threads) {
-    std::vector<BlasView> views(blas_count);
+void make_scene(Scene& S, std::vector<BlasView>& views, const atlas_rt_gpu_bvh_node* tlas, uint32_t tlas_count,
+                const atlas_rt_gpu_bvh_instance* inst, uint32_t inst_count, const atlas_rt_gpu_bvh_node* const* blas_nodes,
+                const uint32_t* blas_node_counts, const atlas_rt_gpu_bvh_triangle* const* blas_tris,
+                const uint32_t* blas_tri_counts, const float* const* shade_normals, const int32_t* const* shade_mat,
+                uint32_t blas_count, const float* xform, const atlas_rt_material* mats, uint32_t n_mats,
+                const atlas_rt_light* lights, uint32_t n_lights, const float* env, int32_t bounces) {
+    views.resize(blas_count);
     for (uint32_t k = 0; k < blas_count; k++)
         views[k] = BlasView{blas_nodes[k], blas_node_counts[k], blas_tris[k], blas_tri_counts[k]};
-    Scene S;
     S.w = World{tlas, tlas_count, inst, inst_count, views.data(), blas_count};
     S.xform = xform;
     S.shade.resize(blas_count);
@@ -404,45 +444,138 @@ void oracle_pathtrace(const atlas_rt_gpu_bvh_node* tlas, uint32_t tlas_count, co
                             wts[l] / total});
     S.env = V3{env[0], env[1], env[2]};
     S.bounces = bounces;
-    Pcg pcg{0ull + 1442695040888963407ull};
-    pcg.next();  // pcg32_init's discarded draw
-    for (uint64_t i = 0; i < pcg_draws; i++) pcg.next();
+}
+
+// Frame jitter and camera ray of pixel `pix` (rayGen.csh:25-90).
+V3 camera_dir(const atlas_rt_camera* cam, uint32_t pix, uint32_t width, uint32_t height, float jx, float jy) {
     const V3 loc{cam->location[0], cam->location[1], cam->location[2]};
     const V3 org{cam->origin[0], cam->origin[1], cam->origin[2]};
     const V3 rgt{cam->right[0], cam->right[1], cam->right[2]};
     const V3 bot{cam->bottom[0], cam->bottom[1], cam->bottom[2]};
+    const uint32_t px = pix % width, py = pix / width;
+    float cx = (float(px) + jx) / float(width);
+    float cy = (float(py) + jy) / float(height);
+    return nrm(((org + mul(rgt, cx)) + mul(bot, cy)) - loc);
+}
+
+template <class F>
+void parallel_pixels(uint32_t n, int32_t threads, F&& work) {
+    if (threads <= 1) {
+        work(0u, n);
+        return;
+    }
+    std::vector<std::thread> pool;
+    uint32_t chunk = (n + uint32_t(threads) - 1) / uint32_t(threads);
+    for (int t = 0; t < threads; t++) {
+        uint32_t lo = uint32_t(t) * chunk, hi = std::min(n, lo + chunk);
+        if (lo < hi) pool.emplace_back(work, lo, hi);
+    }
+    for (auto& th : pool) th.join();
+}
+
+// Training-pixel choice of the cache mode: a murmur3-finalised hash of the
+// pixel and frame index, one pixel in `stride`.
+uint32_t mix32(uint32_t h) {
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
+}
+
+}  // namespace
+
+extern "C" {
+
+// shade_normals[k]: 9 floats per slot of BLAS k (BVH slot order);
+// shade_mat[k]: one material index per slot.  accum: width*height*4 floats
+// (rgb sum, count), accumulated over `frames` frames starting at sample
+// `first_sample`; pcg_draws: number of Random draws made before this call
+// (the PCG stream position).
+void oracle_pathtrace(const atlas_rt_gpu_bvh_node* tlas, uint32_t tlas_count, const atlas_rt_gpu_bvh_instance* inst,
+                      uint32_t inst_count, const atlas_rt_gpu_bvh_node* const* blas_nodes,
+                      const uint32_t* blas_node_counts, const atlas_rt_gpu_bvh_triangle* const* blas_tris,
+                      const uint32_t* blas_tri_counts, const float* const* shade_normals,
+                      const int32_t* const* shade_mat, uint32_t blas_count, const float* xform,
+                      const atlas_rt_material* mats, uint32_t n_mats, const atlas_rt_light* lights, uint32_t n_lights,
+                      const float* env, const atlas_rt_camera* cam, uint32_t width, uint32_t height, int32_t bounces,
+                      uint32_t frames, uint32_t first_sample, uint64_t pcg_draws, float* accum, int32_t threads) {
+    Scene S;
+    std::vector<BlasView> views;
+    make_scene(S, views, tlas, tlas_count, inst, inst_count, blas_nodes, blas_node_counts, blas_tris, blas_tri_counts,
+               shade_normals, shade_mat, blas_count, xform, mats, n_mats, lights, n_lights, env, bounces);
+    Pcg pcg{0ull + 1442695040888963407ull};
+    pcg.next();  // pcg32_init's discarded draw
+    for (uint64_t i = 0; i < pcg_draws; i++) pcg.next();
+    const V3 loc{cam->location[0], cam->location[1], cam->location[2]};
     for (uint32_t f = 0; f < frames; f++) {
         std::vector<float> seeds(size_t(bounces) + 1);
         for (int32_t b = 0; b <= bounces; b++) seeds[b] = pcg.fast_uniform();
         const float sc = float(first_sample + f);
         const float jx = rand2(sc, 0.0f), jy = rand2(sc, 1.0f);
-        const uint32_t n = width * height;
-        auto work = [&](uint32_t lo, uint32_t hi) {
+        parallel_pixels(width * height, threads, [&](uint32_t lo, uint32_t hi) {
             std::vector<int32_t> stack;
             for (uint32_t pix = lo; pix < hi; pix++) {
-                const uint32_t px = pix % width, py = pix / width;
-                float cx = (float(px) + jx) / float(width);
-                float cy = (float(py) + jy) / float(height);
-                V3 dir = nrm(((org + mul(rgt, cx)) + mul(bot, cy)) - loc);
-                V3 L = trace_path(S, loc, dir, float(pix), seeds.data(), stack);
+                V3 L = trace_path(S, loc, camera_dir(cam, pix, width, height, jx, jy), float(pix), seeds.data(), stack);
                 accum[4 * size_t(pix)] += L.x;
                 accum[4 * size_t(pix) + 1] += L.y;
                 accum[4 * size_t(pix) + 2] += L.z;
                 accum[4 * size_t(pix) + 3] += 1.0f;
             }
-        };
-        if (threads <= 1) {
-            work(0, n);
-        } else {
-            std::vector<std::thread> pool;
-            uint32_t chunk = (n + uint32_t(threads) - 1) / uint32_t(threads);
-            for (int t = 0; t < threads; t++) {
-                uint32_t lo = uint32_t(t) * chunk, hi = std::min(n, lo + chunk);
-                if (lo < hi) pool.emplace_back(work, lo, hi);
-            }
-            for (auto& th : pool) th.join();
-        }
+        });
     }
+}
+
+// One frame of the radiance-cache mode (the first frame of a fresh path
+// tracer when frame_index = 0 and pcg_draws = 0): per pixel, radiance[4] is
+// what the path added before any cache term; a render path that reaches a
+// surface at bounce 1 writes query[16] (see CacheOut) instead of going on;
+// a training pixel (train[pix] = 1) traces the full path and writes
+// rec[(bounces + 1) * 20].
+void oracle_pathtrace_cache(const atlas_rt_gpu_bvh_node* tlas, uint32_t tlas_count,
+                            const atlas_rt_gpu_bvh_instance* inst, uint32_t inst_count,
+                            const atlas_rt_gpu_bvh_node* const* blas_nodes, const uint32_t* blas_node_counts,
+                            const atlas_rt_gpu_bvh_triangle* const* blas_tris, const uint32_t* blas_tri_counts,
+                            const float* const* shade_normals, const int32_t* const* shade_mat, uint32_t blas_count,
+                            const float* xform, const atlas_rt_material* mats, uint32_t n_mats,
+                            const atlas_rt_light* lights, uint32_t n_lights, const float* env,
+                            const atlas_rt_camera* cam, uint32_t width, uint32_t height, int32_t bounces,
+                            uint32_t frame_index, uint32_t sample_index, uint64_t pcg_draws, uint32_t train_stride,
+                            float* radiance, float* query, int32_t* train, float* rec, int32_t threads) {
+    Scene S;
+    std::vector<BlasView> views;
+    make_scene(S, views, tlas, tlas_count, inst, inst_count, blas_nodes, blas_node_counts, blas_tris, blas_tri_counts,
+               shade_normals, shade_mat, blas_count, xform, mats, n_mats, lights, n_lights, env, bounces);
+    Pcg pcg{0ull + 1442695040888963407ull};
+    pcg.next();
+    for (uint64_t i = 0; i < pcg_draws; i++) pcg.next();
+    std::vector<float> seeds(size_t(bounces) + 1);
+    for (int32_t b = 0; b <= bounces; b++) seeds[b] = pcg.fast_uniform();
+    const float sc = float(sample_index);
+    const float jx = rand2(sc, 0.0f), jy = rand2(sc, 1.0f);
+    const V3 loc{cam->location[0], cam->location[1], cam->location[2]};
+    const size_t per_rec = 20 * size_t(bounces + 1);
+    parallel_pixels(width * height, threads, [&](uint32_t lo, uint32_t hi) {
+        std::vector<int32_t> stack;
+        for (uint32_t pix = lo; pix < hi; pix++) {
+            const bool tr = mix32(pix * 0x9e3779b1u + frame_index * 0x632be5abu) % train_stride == 0;
+            train[pix] = tr ? 1 : 0;
+            CacheOut co;
+            if (tr) {
+                co.rec = rec + per_rec * pix;
+                std::fill(co.rec, co.rec + per_rec, 0.0f);
+            } else {
+                co.query = query + 16 * size_t(pix);
+                std::fill(co.query, co.query + 16, 0.0f);
+            }
+            V3 L = trace_path(S, loc, camera_dir(cam, pix, width, height, jx, jy), float(pix), seeds.data(), stack, co);
+            radiance[4 * size_t(pix)] = L.x;
+            radiance[4 * size_t(pix) + 1] = L.y;
+            radiance[4 * size_t(pix) + 2] = L.z;
+            radiance[4 * size_t(pix) + 3] = 1.0f;
+        }
+    });
 }
 
 }  // extern "C"

@@ -51,7 +51,7 @@ def oracle_scene(meshes, normals, mats_per_mesh, xf, which, mask=_abi.MASK_ALL |
                 descs=descs, boxes=boxes)
 
 
-def small_c2(n_meshes=4, n_inst=48, salt=0, extent=30.0):
+def small_c2(n_meshes=4, n_inst=48, salt=0, extent=30.0, height=8.0):
     g = W.rng(2, 500 + salt)
     meshes, normals = [], []
     for _ in range(n_meshes):
@@ -60,7 +60,7 @@ def small_c2(n_meshes=4, n_inst=48, salt=0, extent=30.0):
         normals.append(n)
     mats = W.c2_materials({"albedo": g.uniform(0.1, 0.9, (n_meshes, 3)).astype(np.float32),
                            "roughness": g.uniform(0.05, 1.0, n_meshes).astype(np.float32)})
-    xf = W.random_affine(g, n_inst, scale=(1.0, 4.0), extent=extent, height=8.0)
+    xf = W.random_affine(g, n_inst, scale=(1.0, 4.0), extent=extent, height=height)
     which = g.integers(0, n_meshes, n_inst)
     return meshes, normals, mats, xf, which
 
