@@ -26,6 +26,8 @@ KERNEL_BVH2_EXACT = 1
 KERNEL_BVH2_PERSISTENT = 2
 KERNEL_BVH4 = 3
 KERNEL_BVH8 = 4
+KERNEL_VARIANT_MASK = 0xF
+KERNEL_STACK_REGISTER = 0x10
 
 ANY_ENGINE_HITANY = 0
 ANY_RESTORE_ALWAYS = 1

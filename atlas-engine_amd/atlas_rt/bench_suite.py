@@ -1,0 +1,212 @@
+"""Secondary measurements of the benchmark (all on one GPU, inputs resident
+in device memory before each timed region):
+
+  c1           coherent primary rays over a 1024 x 1024 heightfield
+               (2,093,058 triangles) at 1920 x 1080, then one any-hit shadow
+               ray per hit toward a directional light: Mrays/s over both.
+  c3           512 independent BLAS (1k-64k triangles, log-uniform) each with
+               its own random ray batch (4k-256k rays, log-uniform), traced
+               by one autotuned persistent launch and by its captured graph.
+  cache_study  time to equal rel-MSE against a long plain render, plain
+               path tracer vs the radiance-cache path tracer, 1280 x 720,
+               8 bounces, sun + constant environment; on the config-2 layout
+               and on the dense radiance-cache layout.
+"""
+import math
+import time
+
+import numpy as np
+
+from . import HIT_DTYPE, Blas, BatchSet, PathTracer, RadianceCache, World, build_many, camera, instance_descs
+from . import workloads as W
+
+
+def _sync_time(torch, fn):
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    fn()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t
+
+
+def c1(torch, log):
+    verts, prim = W.c1()
+    t = time.perf_counter()
+    blas = Blas.build(verts)
+    build_s = time.perf_counter() - t
+    n = len(prim)
+    d_prim = torch.from_numpy(prim.view(np.float32).reshape(-1, 8).copy()).cuda()
+    d_hits = torch.empty((n, 4), dtype=torch.float32, device="cuda")
+    blas.trace_closest(d_prim, d_hits)
+    torch.cuda.synchronize()
+    hits = d_hits.cpu().numpy().view(HIT_DTYPE).reshape(-1)
+    shadow = W.shadow_rays(prim, hits, verts)
+    d_sh = torch.from_numpy(shadow.view(np.float32).reshape(-1, 8).copy()).cuda()
+    d_occ = torch.empty(n, dtype=torch.int32, device="cuda")
+    res = {}
+    for kernel in (1, 2, 3, 4):
+        for _ in range(2):
+            blas.trace_closest(d_prim, d_hits, kernel=kernel)
+        steps = 10
+        tp = _sync_time(torch, lambda: [blas.trace_closest(d_prim, d_hits, kernel=kernel) for _ in range(steps)])
+        res[kernel] = tp / steps
+    best = min(res, key=res.get)
+    for _ in range(2):
+        blas.trace_any(d_sh, d_occ)
+    steps = 10
+    ts = _sync_time(torch, lambda: [blas.trace_any(d_sh, d_occ) for _ in range(steps)]) / steps
+    n_shadow = int((hits["prim_id"] >= 0).sum())
+    total = res[best] + ts
+    log(f"c1: primary {n / res[best] / 1e6:.0f} Mrays/s, shadow {n / ts / 1e6:.0f} Mrays/s")
+    return {
+        "workload": "c1: 1024x1024 value-noise heightfield (2,093,058 tris), 1920x1080 pinhole at 30 deg pitch, "
+                    "plus one any-hit shadow ray per hit toward a directional light",
+        "triangles": int(len(verts)), "primary_rays": n, "shadow_rays": n_shadow,
+        "build_s": round(build_s, 3),
+        "primary_ms": round(res[best] * 1e3, 4), "primary_kernel": best,
+        "primary_ms_by_kernel": {str(k): round(v * 1e3, 4) for k, v in res.items()},
+        "shadow_ms": round(ts * 1e3, 4),
+        "mrays_per_s": round((n + n) / total / 1e6, 1),
+        "note": "shadow batch keeps one ray per pixel; misses carry NaN directions and exit at once",
+    }
+
+
+def c3(torch, log):
+    meshes, rays, table = W.c3()
+    t = time.perf_counter()
+    blases = build_many(meshes)
+    build_s = time.perf_counter() - t
+    bs = BatchSet(blases, [(b, o, c, 0) for b, o, c in table])
+    n = len(rays)
+    d_rays = torch.from_numpy(rays.view(np.float32).reshape(-1, 8).copy()).cuda()
+    d_hits = torch.empty((n, 4), dtype=torch.float32, device="cuda")
+    t = time.perf_counter()
+    bs.trace_closest(d_rays, d_hits)  # first AUTO launch runs the autotuner
+    torch.cuda.synchronize()
+    tune_s = time.perf_counter() - t
+    best, trials = bs.tuning()
+    steps = 5
+    tl = _sync_time(torch, lambda: [bs.trace_closest(d_rays, d_hits) for _ in range(steps)]) / steps
+    bs.graph_capture(d_rays, d_hits)
+    tg = _sync_time(torch, lambda: [bs.graph_launch() for _ in range(steps)]) / steps
+    log(f"c3: {n / tl / 1e6:.0f} Mrays/s launch, {n / tg / 1e6:.0f} Mrays/s graph, best {best}")
+    return {
+        "workload": "c3: 512 BLAS of 1k-64k triangles (log-uniform), each with its own 4k-256k random rays "
+                    "(log-uniform), one persistent launch",
+        "blas": len(meshes), "triangles": int(sum(len(m) for m in meshes)), "rays": n,
+        "build_s": round(build_s, 2), "autotune_s": round(tune_s, 2),
+        "best": best, "trials": [{"config": c, "ms": round(ms, 3)} for c, ms in trials],
+        "ms": round(tl * 1e3, 3), "graph_ms": round(tg * 1e3, 3),
+        "mrays_per_s": round(n / tl / 1e6, 1), "graph_mrays_per_s": round(n / tg / 1e6, 1),
+    }
+
+
+# ---------------------------------------------------------------------------
+# radiance cache vs plain path tracing
+# ---------------------------------------------------------------------------
+
+def _world(meshes, xf, which):
+    blases = []
+    for v, nrm in meshes:
+        b = Blas.build(v)
+        b.set_shading(nrm)
+        blases.append(b)
+    descs = instance_descs(xf, which)
+    descs["material_offset"] = which.astype(np.int32)
+    w = World()
+    w.update(descs, blases)
+    return w, blases
+
+
+def rel_mse(torch, img, ref):
+    """mean over pixels and channels of (x - r)^2 / (r^2 + 0.01), linear HDR."""
+    x = img[..., :3] / img[..., 3:4].clamp(min=1)
+    r = ref[..., :3] / ref[..., 3:4].clamp(min=1)
+    return float(torch.mean((x - r) ** 2 / (r * r + 0.01)).item())
+
+
+def _curve(torch, pt, cam, ref, checkpoints, t0=0.0):
+    """(spp, seconds, rel-MSE) after each checkpoint; rendering time only."""
+    out, t, done = [], t0, 0
+    for spp in checkpoints:
+        t += _sync_time(torch, lambda: pt.render(cam, spp - done))
+        done = spp
+        out.append((spp, t, rel_mse(torch, pt.read(), ref)))
+    return out
+
+
+def _time_to(curve, target):
+    """Seconds for a (spp, seconds, rel-MSE) curve to reach `target`,
+    interpolated in log-log between checkpoints; None if never reached."""
+    prev = None
+    for spp, t, e in curve:
+        if e <= target:
+            if prev is None or prev[2] <= e:
+                return t
+            lt = math.log(prev[1]) + (math.log(t) - math.log(prev[1])) * \
+                (math.log(prev[2]) - math.log(target)) / (math.log(prev[2]) - math.log(e))
+            return math.exp(lt)
+        prev = (spp, t, e)
+    return None
+
+
+def _study(torch, log, name, meshes, mats, xf, which, cc, bounds, ref_spp, bounces=8, train_stride=32, train_steps=2,
+           warmup=32, checkpoints=(1, 2, 4, 8, 16, 32, 64, 128, 256)):
+    w, blases = _world(meshes, xf, which)
+    mats = W.c2_materials(mats)
+    sun = W.sun_light()
+    width, height = cc["width"], cc["height"]
+    cam = camera(cc["eye"], cc["yaw_deg"], cc["pitch_deg"], cc["fov_y_deg"], width / height)
+    ref_pt = PathTracer(w, width, height, bounces, mats, sun, env=W.C2_ENV)
+    t_ref = 0.0
+    chunk = 256
+    for k in range(0, ref_spp, chunk):
+        t_ref += _sync_time(torch, lambda: ref_pt.render(cam, min(chunk, ref_spp - k)))
+        log(f"{name}: reference {k + chunk} / {ref_spp} spp")
+    ref = ref_pt.read().clone()
+    del ref_pt
+    plain = _curve(torch, PathTracer(w, width, height, bounces, mats, sun, env=W.C2_ENV), cam, ref, checkpoints)
+    rc = RadianceCache(bounds[0], bounds[1], levels=12, log2_table=19, base_resolution=16, max_resolution=2048,
+                       learning_rate=1e-2)
+    pt = PathTracer(w, width, height, bounces, mats, sun, env=W.C2_ENV)
+    pt.set_cache(rc, train_stride=train_stride, train_steps=train_steps)
+    t_warm = _sync_time(torch, lambda: pt.render(cam, warmup))
+    samples, queries, loss = pt.cache_stats()
+    pt.reset()
+    cache = _curve(torch, pt, cam, ref, checkpoints, t0=t_warm)
+    log(f"{name}: plain {plain[-1]}, cache {cache[-1]}")
+    speedups = {}
+    for spp, t_p, e_p in plain:
+        if spp < 4:
+            continue
+        t_c = _time_to(cache, e_p)
+        speedups[str(spp)] = None if t_c is None else round(t_p / t_c, 3)
+    head = speedups.get("64")
+    return {
+        "workload": name,
+        "resolution": [width, height], "bounces": bounces, "reference_spp": ref_spp,
+        "reference_ms_per_spp": round(t_ref / ref_spp * 1e3, 3),
+        "cache": {"levels": 12, "log2_table": 19, "train_stride": train_stride, "train_steps_per_frame": train_steps,
+                  "warmup_frames": warmup, "warmup_ms": round(t_warm * 1e3, 1),
+                  "samples_per_frame": samples, "queries_per_frame": queries, "last_loss": round(loss, 4)},
+        "plain_curve": [[s, round(t * 1e3, 2), round(e, 6)] for s, t, e in plain],
+        "cache_curve": [[s, round(t * 1e3, 2), round(e, 6)] for s, t, e in cache],
+        "speedup_to_plain_rel_mse_at_spp": speedups,
+        "speedup_at_plain_64spp": head,
+    }
+
+
+def cache_study(torch, log, ref_spp):
+    out = {"definition": "plain path tracer's rel-MSE vs the reference after N spp and its render time T_plain; "
+                         "the cache path tracer's render time T_cache (including its warm-up frames) to reach that "
+                         "rel-MSE, log-log interpolated between checkpoints; speedup = T_plain / T_cache (None: "
+                         "the cache's bias floor stays above the target)"}
+    meshes, mats, xf, which = W.c4_scene()
+    out["dense"] = _study(torch, log, "c4: config-2 meshes and materials, 256 instances in a 30-unit cube, camera at "
+                                      "its edge", meshes, mats, xf, which, W.c4_camera(),
+                          ((-20.0, -20.0, -20.0), (20.0, 20.0, 20.0)), ref_spp)
+    meshes, mats, xf, which = W.c2_scene()
+    out["c2_layout"] = _study(torch, log, "c2 layout: 256 instances over 200 x 40 x 200, camera outside", meshes, mats,
+                              xf, which, W.c2_camera(), ((-110.0, -30.0, -110.0), (110.0, 30.0, 110.0)), ref_spp)
+    out["speedup_at_plain_64spp"] = out["dense"]["speedup_at_plain_64spp"]
+    return out

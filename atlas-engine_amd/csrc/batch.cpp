@@ -28,18 +28,36 @@ atlas_rt_status upload(void*& dst, const std::vector<T>& v) {
 
 BatchLaunch to_launch(const atlas_rt_launch_config& c) {
     BatchLaunch L;
-    L.persistent = c.kernel != ATLAS_RT_KERNEL_BVH2_EXACT;
+    const int base = c.kernel & ATLAS_RT_KERNEL_VARIANT_MASK;
+    L.persistent = base != ATLAS_RT_KERNEL_BVH2_EXACT;
+    L.node_w = base == ATLAS_RT_KERNEL_BVH4 ? 4 : (base == ATLAS_RT_KERNEL_BVH8 ? 8 : 2);
+    L.reg = (c.kernel & ATLAS_RT_KERNEL_STACK_REGISTER) != 0;
     L.block = c.block ? c.block : 256;
     L.refill = c.refill ? c.refill : 40;
     return L;
 }
 
-// Candidate launch shapes: {persistent, grid} x {64, 128, 256} lanes x refill.
+bool valid_kernel(int32_t k) {
+    const int32_t base = k & ATLAS_RT_KERNEL_VARIANT_MASK;
+    return (k & ~int32_t(ATLAS_RT_KERNEL_VARIANT_MASK | ATLAS_RT_KERNEL_STACK_REGISTER)) == 0 &&
+           base >= ATLAS_RT_KERNEL_BVH2_EXACT && base <= ATLAS_RT_KERNEL_BVH8;
+}
+
+// Candidates: node width {2, 4, 8} x stack placement {LDS, register top} x
+// work-group size {64, 128, 256}, for binary nodes both launch shapes (one
+// ray per lane; resident waves at refill thresholds 16 / 32 / 48).
 std::vector<atlas_rt_launch_config> candidates() {
     std::vector<atlas_rt_launch_config> out;
+    const int32_t reg = ATLAS_RT_KERNEL_STACK_REGISTER;
     for (uint32_t block : {64u, 128u, 256u}) {
         out.push_back({ATLAS_RT_KERNEL_BVH2_EXACT, block, 0, 0});
+        out.push_back({ATLAS_RT_KERNEL_BVH2_EXACT | reg, block, 0, 0});
         for (uint32_t refill : {16u, 32u, 48u}) out.push_back({ATLAS_RT_KERNEL_BVH2_PERSISTENT, block, refill, 0});
+        out.push_back({ATLAS_RT_KERNEL_BVH2_PERSISTENT | reg, block, 32, 0});
+        for (int32_t wide : {ATLAS_RT_KERNEL_BVH4, ATLAS_RT_KERNEL_BVH8}) {
+            out.push_back({wide, block, 32, 0});
+            out.push_back({wide | reg, block, 32, 0});
+        }
     }
     return out;
 }
@@ -131,7 +149,8 @@ atlas_rt_status atlas_rt_batch_create(const atlas_rt_blas* const* blases, uint32
             return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_batch_create: BLAS " + std::to_string(k) +
                                                     " is NULL, host-only or on another device");
         }
-        s->table[k] = BlasDev{b->d_nodes, b->d_tris, b->d_vtx, uint32_t(b->nodes.size()), uint32_t(b->tris.size())};
+        s->table[k] = BlasDev{b->d_nodes,  b->d_tris, b->d_vtx, b->d_nodes4, b->d_nodes8, uint32_t(b->nodes.size()),
+                              uint32_t(b->tris.size())};
     }
     s->batches.assign(batches, batches + batch_count);
     s->prefix.resize(batch_count + 1);
@@ -146,6 +165,8 @@ atlas_rt_status atlas_rt_batch_create(const atlas_rt_blas* const* blases, uint32
         total += batches[i].ray_count;
         used[batches[i].blas] = 1;
         s->max_depth = std::max(s->max_depth, blases[batches[i].blas]->max_depth);
+        s->max_cap4 = std::max(s->max_cap4, blases[batches[i].blas]->cap4);
+        s->max_cap8 = std::max(s->max_cap8, blases[batches[i].blas]->cap8);
     }
     if (total >= (1ull << 32)) {
         delete s;
@@ -192,7 +213,7 @@ atlas_rt_status atlas_rt_batch_trace_closest(atlas_rt_batch_set* s, const atlas_
     base.stream = reinterpret_cast<hipStream_t>(stream);
     atlas_rt_launch_config c{};
     if (config && config->kernel != ATLAS_RT_KERNEL_AUTO) {
-        if (config->kernel != ATLAS_RT_KERNEL_BVH2_EXACT && config->kernel != ATLAS_RT_KERNEL_BVH2_PERSISTENT)
+        if (!valid_kernel(config->kernel))
             return fail(ATLAS_RT_E_UNSUPPORTED, "atlas_rt_batch_trace_closest: kernel variant not available");
         c = *config;
     } else {
@@ -241,6 +262,10 @@ atlas_rt_status atlas_rt_batch_graph_capture(atlas_rt_batch_set* s, const atlas_
     ATLAS_HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     atlas_rt_launch_config c{};
     if (config && config->kernel != ATLAS_RT_KERNEL_AUTO) {
+        if (!valid_kernel(config->kernel)) {
+            (void)hipStreamDestroy(cs);
+            return fail(ATLAS_RT_E_UNSUPPORTED, "atlas_rt_batch_graph_capture: kernel variant not available");
+        }
         c = *config;
     } else {
         if (!s->tuned) {

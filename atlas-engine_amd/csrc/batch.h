@@ -23,6 +23,8 @@ struct BatchLaunch {
     void* hits = nullptr;
     const uint32_t* d_count = nullptr;
     bool persistent = true;
+    int node_w = 2;       // 2, 4 or 8 children per node
+    bool reg = false;     // register-topped traversal stack
     uint32_t block = 256;
     uint32_t refill = 40;
     hipStream_t stream = nullptr;
@@ -36,6 +38,7 @@ struct atlas_rt_batch_set {
     std::vector<uint32_t> prefix;
     uint32_t total = 0;
     uint32_t max_depth = 0;
+    uint32_t max_cap4 = 1, max_cap8 = 1;  // wide stack bounds over the set's BLASes
     int device = -1;
     void* d_table = nullptr;
     void* d_batches = nullptr;

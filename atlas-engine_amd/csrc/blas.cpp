@@ -9,6 +9,7 @@
 #include "builder.h"
 #include "rt_hostmath.h"
 #include "rt_internal.h"
+#include "wide.h"
 
 namespace atlas_rt {
 
@@ -127,6 +128,23 @@ atlas_rt_status upload_blas(atlas_rt_blas& b) {
     b.d_vtx = reinterpret_cast<float4*>(dv);
     b.device = dev;
     b.on_device = true;
+    // wide nodes for the 4- and 8-wide kernels
+    std::vector<uint8_t> end_flags(nt);
+    for (size_t i = 0; i < nt; i++) end_flags[i] = b.tris[i].v0[3] > 0.0f ? 1 : 0;
+    for (int width : {4, 8}) {
+        const WideTree w = collapse_wide(b.nodes, end_flags, width, 4);
+        void* dw = nullptr;
+        if (w.node_count) {
+            e = hipMalloc(&dw, w.data.size() * sizeof(float));
+            if (e == hipSuccess) e = hipMemcpy(dw, w.data.data(), w.data.size() * sizeof(float), hipMemcpyHostToDevice);
+            if (e != hipSuccess) {
+                if (dw) hipFree(dw);
+                return hip_fail(e, "upload_blas (wide nodes)");
+            }
+        }
+        (width == 4 ? b.d_nodes4 : b.d_nodes8) = reinterpret_cast<float4*>(dw);
+        (width == 4 ? b.cap4 : b.cap8) = w.max_stack + 1;
+    }
     return ATLAS_RT_OK;
 }
 
@@ -249,6 +267,8 @@ atlas_rt_status atlas_rt_blas_destroy(atlas_rt_blas* b) {
     if (b->d_tris) hipFree(b->d_tris);
     if (b->d_vtx) hipFree(b->d_vtx);
     if (b->d_shade) hipFree(b->d_shade);
+    if (b->d_nodes4) hipFree(b->d_nodes4);
+    if (b->d_nodes8) hipFree(b->d_nodes8);
     b->spill.release();
     b->flags.release();
     b->timing.release();

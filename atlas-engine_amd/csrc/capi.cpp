@@ -19,8 +19,10 @@ static atlas_rt_status check_blas_ready(const atlas_rt_blas* b, const char* who)
 }
 
 static atlas_rt_status check_kernel(int32_t kernel, const char* who) {
-    if (kernel == ATLAS_RT_KERNEL_AUTO || kernel == ATLAS_RT_KERNEL_BVH2_EXACT ||
-        kernel == ATLAS_RT_KERNEL_BVH2_PERSISTENT)
+    const int32_t base = kernel & ATLAS_RT_KERNEL_VARIANT_MASK;
+    const int32_t extra = kernel & ~int32_t(ATLAS_RT_KERNEL_VARIANT_MASK | ATLAS_RT_KERNEL_STACK_REGISTER);
+    if (extra == 0 && base >= ATLAS_RT_KERNEL_AUTO && base <= ATLAS_RT_KERNEL_BVH8 &&
+        !(base == ATLAS_RT_KERNEL_AUTO && kernel != ATLAS_RT_KERNEL_AUTO))
         return ATLAS_RT_OK;
     return fail(ATLAS_RT_E_UNSUPPORTED, std::string(who) + ": kernel variant not available");
 }
@@ -78,7 +80,9 @@ atlas_rt_status atlas_rt_trace_any_rays(const atlas_rt_blas* blas, const atlas_r
     atlas_rt_status st = check_blas_ready(blas, "atlas_rt_trace_any_rays");
     if (st != ATLAS_RT_OK) return st;
     if (count && (!d_rays || !d_hit)) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_trace_any_rays: NULL buffer");
-    st = check_kernel(kernel, "atlas_rt_trace_any_rays");
+    if (kernel != ATLAS_RT_KERNEL_AUTO && kernel != ATLAS_RT_KERNEL_BVH2_EXACT &&
+        kernel != ATLAS_RT_KERNEL_BVH2_PERSISTENT)
+        return fail(ATLAS_RT_E_UNSUPPORTED, "atlas_rt_trace_any_rays: any hit runs the binary engine-order kernel");
     if (st != ATLAS_RT_OK) return st;
     TraceLaunch L;
     L.rays = d_rays;

@@ -54,6 +54,19 @@ __global__ void k_frame_begin(FrameCtl* ctl, int32_t bounces, uint32_t pixels) {
     ctl->train_samples = 0;
 }
 
+// Stream compaction slot: one atomic per wave for the active lanes that
+// want a slot (ballot + popcount); lanes keep their order within the wave.
+__device__ __forceinline__ uint32_t wave_alloc(uint32_t* counter, bool want) {
+    const uint64_t m = __ballot(want);
+    if (m == 0) return 0;
+    const uint32_t lane = threadIdx.x & 63u;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == uint32_t(leader)) base = atomicAdd(counter, uint32_t(__popcll(m)));
+    base = __shfl(base, leader, 64);
+    return base + uint32_t(__popcll(m & ((1ull << lane) - 1ull)));
+}
+
 // Integer hash (murmur3 finaliser) choosing the frame's training pixels.
 __device__ __forceinline__ uint32_t mix32(uint32_t h) {
     h ^= h >> 16;
@@ -92,9 +105,10 @@ __global__ __launch_bounds__(kBlock) void k_raygen(Camera cam, uint32_t width, u
     const uint32_t pixel = py * width + px;
     // radiance cache: about one pixel in train_stride traces a training path
     uint32_t train = 0;
-    if (C.enabled && mix32(pixel * 0x9e3779b1u + ctl->frame * 0x632be5abu) % C.train_stride == 0) {
-        const uint32_t t = atomicAdd(&ctl->train_paths, 1u);
-        if (t < C.train_cap) {
+    const bool pick = C.enabled && mix32(pixel * 0x9e3779b1u + ctl->frame * 0x632be5abu) % C.train_stride == 0;
+    if (C.enabled) {
+        const uint32_t t = wave_alloc(&ctl->train_paths, pick);
+        if (pick && t < C.train_cap) {
             train = t + 1;
             for (int32_t k = 0; k <= bounces; k++) {
                 const size_t r = size_t(t) * (bounces + 1) + k;
@@ -217,7 +231,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(ShadeArgs A, int32_t bounce) {
                 A.cache.rec_beta[rec] = make_float4(thr_in.x, thr_in.y, thr_in.z, 1.0f);
             } else if (bounce == 1) {
                 // the render path ends here with the cache's radiance estimate
-                const uint32_t k = atomicAdd(&ctl->queries, 1u);
+                const uint32_t k = wave_alloc(&ctl->queries, true);
                 A.cache.q_buf[3 * size_t(k)] = q0;
                 A.cache.q_buf[3 * size_t(k) + 1] = q1;
                 A.cache.q_buf[3 * size_t(k) + 2] = q2;
@@ -255,6 +269,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(ShadeArgs A, int32_t bounce) {
             update_surface(S);
             V refl = (eval_diffuse(S) + eval_specular(S)) * m.opacity;
             const V rad = mk(Lt.radiance[0], Lt.radiance[1], Lt.radiance[2]);
+            const uint32_t k = wave_alloc(&ctl->shadows[bounce], S.NdotL > 0.0f);
             if (S.NdotL > 0.0f) {
                 V direct = (refl * rad) * S.NdotL / lpdf;
                 V c = thr * m.opacity * direct;
@@ -262,7 +277,6 @@ __global__ __launch_bounds__(kBlock) void k_shade(ShadeArgs A, int32_t bounce) {
                     const float rm = fmaxf(fmaxf(c.x, fmaxf(c.y, c.z)), 10.0f);
                     c = c * (10.0f / rm);
                 }
-                const uint32_t k = atomicAdd(&ctl->shadows[bounce], 1u);
                 const V so = S.P + S.N * kEpsilon;
                 A.shadow[3 * size_t(k)] = make_float4(so.x, so.y, so.z, __uint_as_float(pixel));
                 A.shadow[3 * size_t(k) + 1] = make_float4(S.L.x, S.L.y, S.L.z, kInf - 2.0f * kEpsilon);
@@ -324,8 +338,9 @@ __global__ __launch_bounds__(kBlock) void k_shade(ShadeArgs A, int32_t bounce) {
     A.Lpix[pixel] = acc;
     if (train) A.cache.rec_add[rec] = make_float4(add.x, add.y, add.z, 0.0f);
     const float energy = (thr.x + thr.y) + thr.z;
-    if (energy == 0.0f || bounce == A.bounces) return;
-    const uint32_t k = atomicAdd(&ctl->paths[bounce + 1], 1u);
+    const bool more = !(energy == 0.0f || bounce == A.bounces);
+    const uint32_t k = wave_alloc(&ctl->paths[bounce + 1], more);
+    if (!more) return;
     A.paths_out[3 * size_t(k)] = make_float4(no.x, no.y, no.z, p0.w);
     A.paths_out[3 * size_t(k) + 1] = make_float4(nd.x, nd.y, nd.z, p1.w);
     A.paths_out[3 * size_t(k) + 2] = make_float4(thr.x, thr.y, thr.z, 0.0f);
@@ -404,7 +419,7 @@ __global__ __launch_bounds__(kBlock) void k_build_targets(FrameCtl* __restrict__
         sz += a.z;
         const float4 b = C.rec_beta[r];
         if (b.w == 0.0f) continue;
-        const uint32_t si = atomicAdd(&ctl->train_samples, 1u);
+        const uint32_t si = atomicAdd(&ctl->train_samples, 1u);  // one per vertex of ~1/stride paths
         s_q[3 * size_t(si)] = C.rec_q[3 * r];
         s_q[3 * size_t(si) + 1] = C.rec_q[3 * r + 1];
         s_q[3 * size_t(si) + 2] = C.rec_q[3 * r + 2];
@@ -435,15 +450,17 @@ __global__ __launch_bounds__(kBlock) void k_accumulate(FrameCtl* ctl, const floa
 
 atlas_rt_status launch_frame(PathTracer& P, const Camera& cam, hipStream_t s) {
     const uint32_t pixels = P.width * P.height;
-    const uint32_t grid = (pixels + kBlock - 1) / kBlock;
+    const uint32_t grid_all = (pixels + kBlock - 1) / kBlock;
     uint32_t* flags = reinterpret_cast<uint32_t*>(P.flags.ptr);
     const CacheArgs& C = P.shade_args.cache;
     hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(64), 0, s, P.ctl, P.bounces, pixels);
-    hipLaunchKernelGGL(k_raygen, dim3(grid), dim3(kBlock), 0, s, cam, P.width, P.height, P.ctl, P.paths[0], P.Lpix, C,
-                       P.bounces);
+    hipLaunchKernelGGL(k_raygen, dim3(grid_all), dim3(kBlock), 0, s, cam, P.width, P.height, P.ctl, P.paths[0], P.Lpix,
+                       C, P.bounces);
     for (int32_t b = 0; b <= P.bounces; b++) {
         float4* in = P.paths[b & 1];
         float4* out = P.paths[(b + 1) & 1];
+        // with the cache only training paths outlive bounce 1
+        const uint32_t grid = (C.enabled && b >= 2) ? (P.train_cap + kBlock - 1) / kBlock : grid_all;
         hipLaunchKernelGGL((k_trace<kBlock, kLdsDepth>), dim3(grid), dim3(kBlock), 0, s, P.world_dev,
                            &P.ctl->paths[b], in, P.hits, reinterpret_cast<int32_t*>(P.spill.ptr), P.cap, flags);
         ShadeArgs A = P.shade_args;
@@ -455,7 +472,7 @@ atlas_rt_status launch_frame(PathTracer& P, const Camera& cam, hipStream_t s) {
                                P.shadow, P.Lpix, reinterpret_cast<int32_t*>(P.spill.ptr), P.cap, flags, C.rec_add);
         if (C.enabled && b == 1) {
             ATLAS_HIP_TRY(nrc::launch_infer(P.nrc->dev, P.q_buf, P.q_out, &P.ctl->queries, pixels, s));
-            hipLaunchKernelGGL(k_cache_apply, dim3(grid), dim3(kBlock), 0, s, P.ctl, P.q_meta, P.q_out, P.Lpix);
+            hipLaunchKernelGGL(k_cache_apply, dim3(grid_all), dim3(kBlock), 0, s, P.ctl, P.q_meta, P.q_out, P.Lpix);
         }
     }
     if (C.enabled) {
@@ -468,7 +485,7 @@ atlas_rt_status launch_frame(PathTracer& P, const Camera& cam, hipStream_t s) {
             if (st != ATLAS_RT_OK) return st;
         }
     }
-    hipLaunchKernelGGL(k_accumulate, dim3(grid), dim3(kBlock), 0, s, P.ctl, P.Lpix, P.accum, pixels);
+    hipLaunchKernelGGL(k_accumulate, dim3(grid_all), dim3(kBlock), 0, s, P.ctl, P.Lpix, P.accum, pixels);
     ATLAS_HIP_TRY(hipGetLastError());
     return ATLAS_RT_OK;
 }

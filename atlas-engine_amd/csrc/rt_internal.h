@@ -23,12 +23,14 @@ atlas_rt_status hip_fail(hipError_t e, const char* what);
         if (_e != hipSuccess) return ::atlas_rt::hip_fail(_e, #expr); \
     } while (0)
 
-// Device view of one BLAS (32 B, also the element type of BLAS tables).
+// Device view of one BLAS (also the element type of BLAS tables).
 struct BlasDev {
     const float4* nodes;  // engine GPUBVHNode, 4 float4 per node
     const float4* tris;   // 3 float4 per slot: (v0, caller idx bits), (v1-v0, end flag), (v2-v0, 0)
     const float4* vtx;    // engine GPUBVHTriangle, 3 float4 per slot (v0, v1, v2)
-    uint32_t node_count;
+    const float4* nodes4; // 4-wide nodes collapsed from `nodes` (wide.h), root 0
+    const float4* nodes8; // 8-wide nodes
+    uint32_t node_count;  // binary nodes; 0 = the BLAS is a single leaf (root ~0)
     uint32_t tri_count;
 };
 
@@ -76,6 +78,11 @@ struct atlas_rt_blas {
     //  d_shade: optional, 3 x float4 per slot: (n0, material index bits),
     //           (n1, 0), (n2, 0) (the path tracer's shading data).
     float4* d_shade = nullptr;
+    //  d_nodes4 / d_nodes8: wide nodes (wide.h); cap4 / cap8 bound the
+    //  traversal stack of the wide kernels.
+    float4* d_nodes4 = nullptr;
+    float4* d_nodes8 = nullptr;
+    uint32_t cap4 = 1, cap8 = 1;
     atlas_rt::Scratch spill;
     atlas_rt::Scratch flags;  // [0] ray counter, [1] finished waves, [2] stack overflow flag
     atlas_rt::LaunchTiming timing;

@@ -3,7 +3,9 @@
 // Work item w in [0, total) belongs to the batch b with prefix[b] <= w <
 // prefix[b+1] (binary search over the exclusive prefix of the batch sizes);
 // its ray is rays[batch[b].ray_offset + (w - prefix[b])] and it is traced
-// against blases[batch[b].blas] with the engine-exact BVH2 step function.
+// against blases[batch[b].blas] with one of the step functions of
+// trace_core.h: binary nodes in engine order, or 4- / 8-wide nodes, each
+// with the stack in LDS or with its top in registers.
 //
 // Launch shapes (picked by the autotuner):
 //  * persistent<BLOCK>: resident waves pull work items from a global counter
@@ -36,7 +38,7 @@ __device__ __forceinline__ uint32_t find_batch(const uint32_t* __restrict__ pref
     return lo;
 }
 
-template <int BLOCK, int LDS_DEPTH>
+template <int BLOCK, int LDS_DEPTH, int NODE_W, bool REG>
 __global__ __launch_bounds__(BLOCK) void k_batch_persistent(BatchDev D, const uint32_t* __restrict__ d_count,
                                                             const float4* __restrict__ rays,
                                                             float4* __restrict__ hits, int32_t* __restrict__ spill,
@@ -74,7 +76,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_persistent(BatchDev D, const ui
                         if (any_nan3(r1.x, r1.y, r1.z)) {
                             hits[ray] = make_float4(tmax, 0.0f, 0.0f, __int_as_float(-1));
                         } else {
-                            lane_begin(L, B, r0, r1);
+                            lane_begin(L, B, r0, r1, NODE_W > 2);
                             active = true;
                         }
                     }
@@ -89,7 +91,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_persistent(BatchDev D, const ui
         const uint32_t threshold = exhausted ? 1u : refill;
         for (;;) {
             if (active) {
-                const int r = step_closest<BLOCK, LDS_DEPTH, false>(B, L, S, cap, dn, dt);
+                const int r = step_variant<NODE_W, REG, BLOCK, LDS_DEPTH, false>(B, L, S, cap, dn, dt);
                 if (r != kContinue) {
                     if (r == kOverflow) {
                         atomicOr(&ctl[2], 1u);
@@ -112,7 +114,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_persistent(BatchDev D, const ui
     }
 }
 
-template <int BLOCK, int LDS_DEPTH>
+template <int BLOCK, int LDS_DEPTH, int NODE_W, bool REG>
 __global__ __launch_bounds__(BLOCK) void k_batch_grid(BatchDev D, const uint32_t* __restrict__ d_count,
                                                       const float4* __restrict__ rays, float4* __restrict__ hits,
                                                       int32_t* __restrict__ spill, uint32_t cap,
@@ -130,12 +132,12 @@ __global__ __launch_bounds__(BLOCK) void k_batch_grid(BatchDev D, const uint32_t
     LaneState L;
     L.slot = -1;
     if (!any_nan3(r1.x, r1.y, r1.z)) {
-        lane_begin(L, B, r0, r1);
+        lane_begin(L, B, r0, r1, NODE_W > 2);
         const LaneStack<BLOCK, LDS_DEPTH> S{lds, spill, threadIdx.x, w, gridDim.x * BLOCK};
         uint32_t dn = 0, dt = 0;
         int r;
         do {
-            r = step_closest<BLOCK, LDS_DEPTH, false>(B, L, S, cap, dn, dt);
+            r = step_variant<NODE_W, REG, BLOCK, LDS_DEPTH, false>(B, L, S, cap, dn, dt);
         } while (r == kContinue);
         if (r == kOverflow) {
             atomicOr(&ctl[2], 1u);
@@ -151,6 +153,20 @@ struct Rows {
     static constexpr int value = BLOCK >= 256 ? 24 : 32;
 };
 
+template <int BLOCK, int NODE_W, bool REG>
+void launch_kernel(atlas_rt_batch_set& s, const BatchLaunch& L, uint32_t grid, uint32_t cap) {
+    constexpr int D = Rows<BLOCK>::value;
+    uint32_t* ctl = reinterpret_cast<uint32_t*>(s.flags.ptr);
+    if (L.persistent)
+        hipLaunchKernelGGL((k_batch_persistent<BLOCK, D, NODE_W, REG>), dim3(grid), dim3(BLOCK), 0, L.stream, s.dev(),
+                           L.d_count, reinterpret_cast<const float4*>(L.rays), reinterpret_cast<float4*>(L.hits),
+                           reinterpret_cast<int32_t*>(s.spill.ptr), cap, ctl, L.refill);
+    else
+        hipLaunchKernelGGL((k_batch_grid<BLOCK, D, NODE_W, REG>), dim3(grid), dim3(BLOCK), 0, L.stream, s.dev(),
+                           L.d_count, reinterpret_cast<const float4*>(L.rays), reinterpret_cast<float4*>(L.hits),
+                           reinterpret_cast<int32_t*>(s.spill.ptr), cap, ctl);
+}
+
 template <int BLOCK>
 atlas_rt_status launch_one(atlas_rt_batch_set& s, const BatchLaunch& L) {
     constexpr int D = Rows<BLOCK>::value;
@@ -162,20 +178,19 @@ atlas_rt_status launch_one(atlas_rt_batch_set& s, const BatchLaunch& L) {
         grid = (s.total + BLOCK - 1) / BLOCK;
         threads = grid * BLOCK;
     }
-    uint32_t cap = s.max_depth + 1;
+    const uint32_t cap = variant_cap(L.node_w, s.max_depth + 1, s.max_cap4, s.max_cap8);
     if (cap > uint32_t(D)) {
         atlas_rt_status st = s.spill.ensure(size_t(cap - D) * threads * sizeof(int32_t));
         if (st != ATLAS_RT_OK) return st;
     }
-    uint32_t* ctl = reinterpret_cast<uint32_t*>(s.flags.ptr);
-    if (L.persistent)
-        hipLaunchKernelGGL((k_batch_persistent<BLOCK, D>), dim3(grid), dim3(BLOCK), 0, L.stream, s.dev(), L.d_count,
-                           reinterpret_cast<const float4*>(L.rays), reinterpret_cast<float4*>(L.hits),
-                           reinterpret_cast<int32_t*>(s.spill.ptr), cap, ctl, L.refill);
+    if (!L.persistent && L.node_w != 2)
+        return fail(ATLAS_RT_E_UNSUPPORTED, "batched trace: wide nodes run in resident waves only");
+    if (L.node_w == 2)
+        L.reg ? launch_kernel<BLOCK, 2, true>(s, L, grid, cap) : launch_kernel<BLOCK, 2, false>(s, L, grid, cap);
+    else if (L.node_w == 4)
+        L.reg ? launch_kernel<BLOCK, 4, true>(s, L, grid, cap) : launch_kernel<BLOCK, 4, false>(s, L, grid, cap);
     else
-        hipLaunchKernelGGL((k_batch_grid<BLOCK, D>), dim3(grid), dim3(BLOCK), 0, L.stream, s.dev(), L.d_count,
-                           reinterpret_cast<const float4*>(L.rays), reinterpret_cast<float4*>(L.hits),
-                           reinterpret_cast<int32_t*>(s.spill.ptr), cap, ctl);
+        L.reg ? launch_kernel<BLOCK, 8, true>(s, L, grid, cap) : launch_kernel<BLOCK, 8, false>(s, L, grid, cap);
     ATLAS_HIP_TRY(hipGetLastError());
     return ATLAS_RT_OK;
 }

@@ -25,7 +25,7 @@ namespace {
 
 constexpr int kWave = 64;
 
-template <int BLOCK, int LDS_DEPTH, bool STATS>
+template <int BLOCK, int LDS_DEPTH, bool STATS, int NODE_W = 2, bool REG = false>
 __global__ __launch_bounds__(BLOCK) void k_blas_closest(BlasDev B, const float4* __restrict__ rays,
                                                         float4* __restrict__ hits, int32_t* __restrict__ slots,
                                                         uint2* __restrict__ visits, uint32_t count,
@@ -40,11 +40,11 @@ __global__ __launch_bounds__(BLOCK) void k_blas_closest(BlasDev B, const float4*
     L.slot = -1;
     uint32_t n_nodes = 0, n_tris = 0;
     if (!any_nan3(r1.x, r1.y, r1.z)) {
-        lane_begin(L, B, r0, r1);
+        lane_begin(L, B, r0, r1, NODE_W > 2);
         const LaneStack<BLOCK, LDS_DEPTH> S{lds, spill, threadIdx.x, gid, gridDim.x * BLOCK};
         int r;
         do {
-            r = step_closest<BLOCK, LDS_DEPTH, STATS>(B, L, S, cap, n_nodes, n_tris);
+            r = step_variant<NODE_W, REG, BLOCK, LDS_DEPTH, STATS>(B, L, S, cap, n_nodes, n_tris);
         } while (r == kContinue);
         if (r == kOverflow) {
             atomicOr(&ctl[2], 1u);
@@ -56,7 +56,7 @@ __global__ __launch_bounds__(BLOCK) void k_blas_closest(BlasDev B, const float4*
     if (STATS) visits[gid] = make_uint2(n_nodes, n_tris);
 }
 
-template <int BLOCK, int LDS_DEPTH>
+template <int BLOCK, int LDS_DEPTH, int NODE_W = 2, bool REG = false>
 __global__ __launch_bounds__(BLOCK) void k_blas_closest_persistent(BlasDev B, const float4* __restrict__ rays,
                                                                    float4* __restrict__ hits,
                                                                    int32_t* __restrict__ slots, uint32_t count,
@@ -90,7 +90,7 @@ __global__ __launch_bounds__(BLOCK) void k_blas_closest_persistent(BlasDev B, co
                             hits[idx] = make_float4(tmax, 0.0f, 0.0f, __int_as_float(-1));
                             if (slots) slots[idx] = -1;
                         } else {
-                            lane_begin(L, B, r0, r1);
+                            lane_begin(L, B, r0, r1, NODE_W > 2);
                             active = true;
                         }
                     }
@@ -105,7 +105,7 @@ __global__ __launch_bounds__(BLOCK) void k_blas_closest_persistent(BlasDev B, co
         const uint32_t threshold = exhausted ? 1u : refill;
         for (;;) {
             if (active) {
-                const int r = step_closest<BLOCK, LDS_DEPTH, false>(B, L, S, cap, dn, dt);
+                const int r = step_variant<NODE_W, REG, BLOCK, LDS_DEPTH, false>(B, L, S, cap, dn, dt);
                 if (r != kContinue) {
                     if (r == kOverflow) {
                         atomicOr(&ctl[2], 1u);
@@ -167,6 +167,8 @@ BlasDev blas_dev(const atlas_rt_blas& b) {
     d.nodes = b.d_nodes;
     d.tris = b.d_tris;
     d.vtx = b.d_vtx;
+    d.nodes4 = b.d_nodes4;
+    d.nodes8 = b.d_nodes8;
     d.node_count = uint32_t(b.nodes.size());
     d.tri_count = uint32_t(b.tris.size());
     return d;
@@ -188,9 +190,11 @@ int device_cu_count() {
     return cached;
 }
 
-static atlas_rt_status prepare(atlas_rt_blas& blas, uint32_t threads, uint32_t lds_depth, uint32_t& cap) {
-    // The stack never holds more entries than the deepest leaf's depth.
-    cap = blas.max_depth + 1;
+static atlas_rt_status prepare(atlas_rt_blas& blas, uint32_t threads, uint32_t lds_depth, uint32_t& cap,
+                               int node_w = 2) {
+    // The binary stack never holds more entries than the deepest leaf's
+    // depth; the wide bounds come from the collapse.
+    cap = variant_cap(node_w, blas.max_depth + 1, blas.cap4, blas.cap8);
     if (cap > lds_depth) {
         const size_t need = size_t(cap - lds_depth) * threads * sizeof(int32_t);
         atlas_rt_status st = blas.spill.ensure(need);
@@ -204,25 +208,52 @@ static atlas_rt_status prepare(atlas_rt_blas& blas, uint32_t threads, uint32_t l
     return ATLAS_RT_OK;
 }
 
+template <int NODE_W, bool REG>
+static void launch_persistent(const BlasDev& B, const TraceLaunch& L, uint32_t groups, int32_t* spill, uint32_t cap,
+                              uint32_t* ctl) {
+    hipLaunchKernelGGL((k_blas_closest_persistent<kPBlock, kPLdsDepth, NODE_W, REG>), dim3(groups), dim3(kPBlock), 0,
+                       L.stream, B, reinterpret_cast<const float4*>(L.rays), reinterpret_cast<float4*>(L.hits),
+                       L.slots, L.count, spill, cap, ctl, L.refill ? L.refill : 40u);
+}
+
 atlas_rt_status launch_blas_closest(atlas_rt_blas& blas, const TraceLaunch& L) {
     if (L.count == 0) return ATLAS_RT_OK;
     const BlasDev B = blas_dev(blas);
     uint32_t* ctl = reinterpret_cast<uint32_t*>(blas.flags.ptr);
-    const bool persistent = !L.visits && L.kernel == ATLAS_RT_KERNEL_BVH2_PERSISTENT;
+    const int base = L.kernel & ATLAS_RT_KERNEL_VARIANT_MASK;
+    const bool reg = (L.kernel & ATLAS_RT_KERNEL_STACK_REGISTER) != 0;
+    const bool persistent = !L.visits && base >= ATLAS_RT_KERNEL_BVH2_PERSISTENT;
+    const int node_w = base == ATLAS_RT_KERNEL_BVH4 ? 4 : (base == ATLAS_RT_KERNEL_BVH8 ? 8 : 2);
     atlas_rt_status st;
     uint32_t cap = 0;
     if (persistent) {
         // Resident grid: enough 256-lane groups for every CU several times over.
         const uint32_t groups = uint32_t(device_cu_count()) * 8;
-        st = prepare(blas, groups * kPBlock, kPLdsDepth, cap);
+        st = prepare(blas, groups * kPBlock, kPLdsDepth, cap, node_w);
         if (st != ATLAS_RT_OK) return st;
         ctl = reinterpret_cast<uint32_t*>(blas.flags.ptr);
         st = blas.timing.begin(L.stream);
         if (st != ATLAS_RT_OK) return st;
-        hipLaunchKernelGGL((k_blas_closest_persistent<kPBlock, kPLdsDepth>), dim3(groups), dim3(kPBlock), 0,
-                           L.stream, B, reinterpret_cast<const float4*>(L.rays), reinterpret_cast<float4*>(L.hits),
-                           L.slots, L.count, reinterpret_cast<int32_t*>(blas.spill.ptr), cap, ctl,
-                           L.refill ? L.refill : 40u);
+        int32_t* spill = reinterpret_cast<int32_t*>(blas.spill.ptr);
+        if (node_w == 2)
+            reg ? launch_persistent<2, true>(B, L, groups, spill, cap, ctl)
+                : launch_persistent<2, false>(B, L, groups, spill, cap, ctl);
+        else if (node_w == 4)
+            reg ? launch_persistent<4, true>(B, L, groups, spill, cap, ctl)
+                : launch_persistent<4, false>(B, L, groups, spill, cap, ctl);
+        else
+            reg ? launch_persistent<8, true>(B, L, groups, spill, cap, ctl)
+                : launch_persistent<8, false>(B, L, groups, spill, cap, ctl);
+    } else if (reg && !L.visits) {
+        const uint32_t grid = (L.count + kBlock - 1) / kBlock;
+        st = prepare(blas, grid * kBlock, kLdsDepth, cap);
+        if (st != ATLAS_RT_OK) return st;
+        ctl = reinterpret_cast<uint32_t*>(blas.flags.ptr);
+        st = blas.timing.begin(L.stream);
+        if (st != ATLAS_RT_OK) return st;
+        hipLaunchKernelGGL((k_blas_closest<kBlock, kLdsDepth, false, 2, true>), dim3(grid), dim3(kBlock), 0, L.stream,
+                           B, reinterpret_cast<const float4*>(L.rays), reinterpret_cast<float4*>(L.hits), L.slots,
+                           nullptr, L.count, reinterpret_cast<int32_t*>(blas.spill.ptr), cap, ctl);
     } else {
         const uint32_t grid = (L.count + kBlock - 1) / kBlock;
         st = prepare(blas, grid * kBlock, kLdsDepth, cap);

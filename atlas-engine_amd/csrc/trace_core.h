@@ -24,6 +24,19 @@
 
 namespace atlas_rt {
 
+// Entries of the register-resident top of the stack (register stack kernels).
+constexpr int kRegTop = 6;
+
+struct LaneState {
+    DevRay R;
+    float hitT;
+    int32_t node;   // >= 0 inner node, < 0: ~(current triangle slot)
+    int32_t slot;   // best hit slot (-1 none)
+    int32_t prim;   // caller index of the best hit
+    uint32_t sp;    // stack entries
+    int32_t top[kRegTop];  // register stack: top[0] is the top entry
+};
+
 // Stack: lane-contiguous LDS rows [LDS_DEPTH][BLOCK], then global spill rows
 // [cap - LDS_DEPTH][spill_stride].
 template <int BLOCK, int LDS_DEPTH>
@@ -46,13 +59,41 @@ struct LaneStack {
     }
 };
 
-struct LaneState {
-    DevRay R;
-    float hitT;
-    int32_t node;   // >= 0 inner node, < 0: ~(current triangle slot)
-    int32_t slot;   // best hit slot (-1 none)
-    int32_t prim;   // caller index of the best hit
-    uint32_t sp;    // stack entries
+// Stack placements for the generic step functions (push / pop on L.sp).
+// MemStack: every entry in the LDS rows / global spill of LaneStack.
+template <int BLOCK, int LDS_DEPTH>
+struct MemStack {
+    LaneStack<BLOCK, LDS_DEPTH> m;
+    __device__ __forceinline__ void push(LaneState& L, int32_t v) const {
+        m.put(L.sp, v);
+        L.sp++;
+    }
+    __device__ __forceinline__ int32_t pop(LaneState& L) const {
+        L.sp--;
+        return m.get(L.sp);
+    }
+};
+
+// RegStack: the top kRegTop entries live in VGPRs (shifted on push / pop, so
+// every register index is static); deeper entries go to LaneStack memory.
+template <int BLOCK, int LDS_DEPTH>
+struct RegStack {
+    LaneStack<BLOCK, LDS_DEPTH> m;
+    __device__ __forceinline__ void push(LaneState& L, int32_t v) const {
+        if (L.sp >= uint32_t(kRegTop)) m.put(L.sp - kRegTop, L.top[kRegTop - 1]);
+#pragma unroll
+        for (int i = kRegTop - 1; i > 0; i--) L.top[i] = L.top[i - 1];
+        L.top[0] = v;
+        L.sp++;
+    }
+    __device__ __forceinline__ int32_t pop(LaneState& L) const {
+        const int32_t v = L.top[0];
+#pragma unroll
+        for (int i = 0; i < kRegTop - 1; i++) L.top[i] = L.top[i + 1];
+        L.sp--;
+        if (L.sp >= uint32_t(kRegTop)) L.top[kRegTop - 1] = m.get(L.sp - kRegTop);
+        return v;
+    }
 };
 
 enum StepResult : int { kContinue = 0, kDone = 1, kOverflow = 2 };
@@ -149,10 +190,173 @@ __device__ __forceinline__ int step_any(const BlasDev& B, LaneState& L, const La
     return kContinue;
 }
 
-__device__ __forceinline__ void lane_begin(LaneState& L, const BlasDev& B, float4 r0, float4 r1) {
+// ---------------------------------------------------------------------------
+// Generic-stack and wide-node variants (the autotuner's other candidates).
+// Nearest hit distance and the set of reachable triangles are those of the
+// binary traversal (every wide child box is a binary node's box); the
+// winner among triangles hit at exactly the same t may differ.
+// ---------------------------------------------------------------------------
+
+// Leaf step shared by every variant: test triangle ~L.node, advance within
+// the leaf or pop.
+template <class STK>
+__device__ __forceinline__ int leaf_step_closest(const BlasDev& B, LaneState& L, const STK& S, uint32_t& n_tris,
+                                                 bool stats) {
+    const int32_t tp = ~L.node;
+    const float4* tr = B.tris + 3 * size_t(tp);
+    const float4 a = tr[0], b = tr[1], c = tr[2];
+    if (stats) n_tris++;
+    float t, u, v;
+    const bool geo = tri_uvt(L.R, a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z, t, u, v);
+    if (geo && t > L.R.tmin && t < L.hitT) {
+        L.hitT = t;
+        L.slot = tp;
+        L.prim = __float_as_int(a.w);
+    }
+    if (b.w > 0.0f) {
+        if (L.sp == 0) return kDone;
+        L.node = S.pop(L);
+    } else {
+        L.node = ~(tp + 1);
+    }
+    return kContinue;
+}
+
+// Binary nodes, engine order, any stack placement.
+template <bool STATS, class STK>
+__device__ __forceinline__ int step_closest_s(const BlasDev& B, LaneState& L, const STK& S, uint32_t cap,
+                                              uint32_t& n_nodes, uint32_t& n_tris) {
+    if (L.node < 0) return leaf_step_closest(B, L, S, n_tris, STATS);
+    const float4* np = B.nodes + 4 * size_t(L.node);
+    const float4 a = np[0], b = np[1], c = np[2], d = np[3];
+    if (STATS) n_nodes++;
+    float dl, dr;
+    const bool hl = slab_dist(L.R, a.x, a.y, a.z, a.w, b.x, b.y, L.hitT, dl);
+    const bool hr = slab_dist(L.R, b.z, b.w, c.x, c.y, c.z, c.w, L.hitT, dr);
+    const int32_t lp = __float_as_int(d.x), rp = __float_as_int(d.y);
+    const bool leftFirst = dl <= dr;
+    if (hl && hr) {
+        if (L.sp >= cap) return kOverflow;
+        S.push(L, leftFirst ? rp : lp);
+        L.node = leftFirst ? lp : rp;
+    } else if (hl || hr) {
+        L.node = leftFirst ? lp : rp;
+    } else {
+        if (L.sp == 0) return kDone;
+        L.node = S.pop(L);
+    }
+    return kContinue;
+}
+
+__device__ __forceinline__ float comp4(const float4& v, int k) {
+    return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w));
+}
+
+// Wide leaf step: L.node = ~((slot << 3) | code), code = triangles left - 1,
+// or 7: continue to the end-of-leaf flag.
+template <class STK>
+__device__ __forceinline__ int wide_leaf_step_closest(const BlasDev& B, LaneState& L, const STK& S, uint32_t& n_tris,
+                                                      bool stats) {
+    const uint32_t enc = uint32_t(~L.node);
+    const uint32_t tp = enc >> 3, code = enc & 7u;
+    const float4* tr = B.tris + 3 * size_t(tp);
+    const float4 a = tr[0], b = tr[1], c = tr[2];
+    if (stats) n_tris++;
+    float t, u, v;
+    const bool geo = tri_uvt(L.R, a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z, t, u, v);
+    if (geo && t > L.R.tmin && t < L.hitT) {
+        L.hitT = t;
+        L.slot = int32_t(tp);
+        L.prim = __float_as_int(a.w);
+    }
+    const bool last = code == 7u ? b.w > 0.0f : code == 0u;
+    if (last) {
+        if (L.sp == 0) return kDone;
+        L.node = S.pop(L);
+    } else {
+        L.node = ~int32_t(((tp + 1) << 3) | (code == 7u ? 7u : code - 1));
+    }
+    return kContinue;
+}
+
+// W-wide nodes (W = 4 or 8): test every child box, visit hit children
+// nearest first (equal entry distance: lower child index first).
+template <int W, bool STATS, class STK>
+__device__ __forceinline__ int step_closest_w(const BlasDev& B, LaneState& L, const STK& S, uint32_t cap,
+                                              uint32_t& n_nodes, uint32_t& n_tris) {
+    if (L.node < 0) return wide_leaf_step_closest(B, L, S, n_tris, STATS);
+    constexpr int G = W / 4, F = 7 * G + 1;
+    const float4* np = (W == 4 ? B.nodes4 : B.nodes8) + F * size_t(L.node);
+    float4 f[F];
+#pragma unroll
+    for (int i = 0; i < F; i++) f[i] = np[i];
+    if (STATS) n_nodes++;
+    const int cnt = __float_as_int(f[7 * G].x);
+    float d[W];
+    int32_t ptr[W];
+    bool h[W];
+    int hits = 0;
+#pragma unroll
+    for (int c = 0; c < W; c++) {
+        const int q = c >> 2, k = c & 3;
+        ptr[c] = __float_as_int(comp4(f[6 * G + q], k));
+        d[c] = 0.0f;
+        h[c] = c < cnt && slab_dist(L.R, comp4(f[q], k), comp4(f[G + q], k), comp4(f[2 * G + q], k),
+                                    comp4(f[3 * G + q], k), comp4(f[4 * G + q], k), comp4(f[5 * G + q], k), L.hitT,
+                                    d[c]);
+        hits += h[c] ? 1 : 0;
+    }
+    if (hits == 0) {
+        if (L.sp == 0) return kDone;
+        L.node = S.pop(L);
+        return kContinue;
+    }
+    if (L.sp + uint32_t(hits) - 1u > cap) return kOverflow;
+    int32_t srt[W];
+#pragma unroll
+    for (int r = 0; r < W; r++) srt[r] = 0;
+#pragma unroll
+    for (int c = 0; c < W; c++) {
+        int rank = 0;
+#pragma unroll
+        for (int j = 0; j < W; j++) rank += (h[j] && (d[j] < d[c] || (d[j] == d[c] && j < c))) ? 1 : 0;
+#pragma unroll
+        for (int r = 0; r < W; r++) srt[r] = (h[c] && rank == r) ? ptr[c] : srt[r];
+    }
+#pragma unroll
+    for (int r = W - 1; r >= 1; r--)
+        if (r < hits) S.push(L, srt[r]);
+    L.node = srt[0];
+    return kContinue;
+}
+
+// One closest-hit step of the variant (node width NODE_W in {2, 4, 8};
+// REG: register-topped stack).  NODE_W = 2 without REG is the engine-order
+// binary step above.
+template <int NODE_W, bool REG, int BLOCK, int LDS_DEPTH, bool STATS>
+__device__ __forceinline__ int step_variant(const BlasDev& B, LaneState& L, const LaneStack<BLOCK, LDS_DEPTH>& S,
+                                            uint32_t cap, uint32_t& n_nodes, uint32_t& n_tris) {
+    if constexpr (NODE_W == 2 && !REG)
+        return step_closest<BLOCK, LDS_DEPTH, STATS>(B, L, S, cap, n_nodes, n_tris);
+    else if constexpr (NODE_W == 2)
+        return step_closest_s<STATS>(B, L, RegStack<BLOCK, LDS_DEPTH>{S}, cap, n_nodes, n_tris);
+    else if constexpr (REG)
+        return step_closest_w<NODE_W, STATS>(B, L, RegStack<BLOCK, LDS_DEPTH>{S}, cap, n_nodes, n_tris);
+    else
+        return step_closest_w<NODE_W, STATS>(B, L, MemStack<BLOCK, LDS_DEPTH>{S}, cap, n_nodes, n_tris);
+}
+
+// Stack bound of a variant for one BLAS (entries).
+__host__ __device__ __forceinline__ uint32_t variant_cap(int node_w, uint32_t cap2, uint32_t cap4, uint32_t cap8) {
+    return node_w == 2 ? cap2 : (node_w == 4 ? cap4 : cap8);
+}
+
+// wide: the root of a single-leaf BLAS in the wide leaf encoding (run to
+// the end-of-leaf flag from slot 0).
+__device__ __forceinline__ void lane_begin(LaneState& L, const BlasDev& B, float4 r0, float4 r1, bool wide = false) {
     make_dev_ray(L.R, r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, r0.w);
     L.hitT = r1.w;
-    L.node = B.node_count ? 0 : ~0;
+    L.node = B.node_count ? 0 : (wide ? ~7 : ~0);
     L.slot = -1;
     L.prim = -1;
     L.sp = 0;

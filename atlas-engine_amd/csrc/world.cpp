@@ -147,6 +147,8 @@ atlas_rt_status atlas_rt_world_update(atlas_rt_world* w, const atlas_rt_instance
         table[k].nodes = b->d_nodes;
         table[k].tris = b->d_tris;
         table[k].vtx = b->d_vtx;
+        table[k].nodes4 = b->d_nodes4;
+        table[k].nodes8 = b->d_nodes8;
         table[k].node_count = uint32_t(b->nodes.size());
         table[k].tri_count = uint32_t(b->tris.size());
         blas_depth = std::max(blas_depth, b->max_depth);

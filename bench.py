@@ -26,8 +26,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "atlas-engine_amd"))
 
 HBM_PEAK_GBPS = 8000.0
+REG = 0x10
 KERNEL_NAMES = {1: "bvh2-exact grid (engine order, bit-exact)",
-                2: "bvh2-exact persistent wave-refill (engine order, bit-exact)"}
+                2: "bvh2-exact persistent wave-refill (engine order, bit-exact)",
+                3: "bvh4 persistent (wide nodes, same nearest hit)",
+                4: "bvh8 persistent (wide nodes, same nearest hit)",
+                1 | REG: "bvh2-exact grid, register stack top",
+                2 | REG: "bvh2-exact persistent, register stack top",
+                3 | REG: "bvh4 persistent, register stack top",
+                4 | REG: "bvh8 persistent, register stack top"}
 METRIC = "Mrays/s incoherent closest-hit; path-trace time to equal rel-MSE, cache vs plain"
 
 
@@ -39,7 +46,13 @@ def parse():
     ap.add_argument("--kernel", type=int, default=0, help="atlas_rt kernel variant (0 = auto)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU work for the baseline sample")
+    ap.add_argument("--no-extra", action="store_true", help="only the c0 headline (skip c1, c3 and the cache study)")
+    ap.add_argument("--ref-spp", type=int, default=4096, help="samples per pixel of the plain reference render")
     return ap.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def host_threads():
@@ -118,7 +131,7 @@ def main():
     variants = {}
     if kernel == 0:
         # pick the fastest traversal variant (outside the timed region)
-        for k in (1, 2):
+        for k in (1, 2, 3, 4, 1 | REG, 2 | REG, 3 | REG, 4 | REG):
             for _ in range(3):
                 blas.trace_closest(d_rays, d_hits, kernel=k, stream=stream)
             torch.cuda.synchronize()
@@ -191,6 +204,14 @@ def main():
         out["cpu_baseline"] = cb
         out["parity"] = parity
         out["gpu_vs_cpu"] = round(value / cb["value"], 2)
+    log(f"c0 done: {value:.1f} Mrays/s")
+    if not args.no_extra:
+        from atlas_rt import bench_suite
+        del d_rays, d_hits, d_vis, blas
+        torch.cuda.empty_cache()
+        out["c1"] = bench_suite.c1(torch, log)
+        out["c3"] = bench_suite.c3(torch, log)
+        out["radiance_cache"] = bench_suite.cache_study(torch, log, args.ref_spp)
     print(json.dumps(out), flush=True)
 
 

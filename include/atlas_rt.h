@@ -181,13 +181,24 @@ atlas_rt_status atlas_rt_blas_export(const atlas_rt_blas* blas, atlas_rt_gpu_bvh
 /* Bulk tracing against one BLAS                                             */
 /* ------------------------------------------------------------------------- */
 
-/* Traversal kernels.  ATLAS_RT_KERNEL_AUTO picks the autotuned variant for the
- * current device and BLAS shape. */
+/* Traversal kernels.  ATLAS_RT_KERNEL_AUTO picks the default (single BLAS)
+ * or the autotuned variant (batch sets).
+ *  BVH2_EXACT       one ray per lane over the engine's binary nodes, engine order
+ *  BVH2_PERSISTENT  resident waves refilled with new rays, same traversal
+ *  BVH4 / BVH8      resident waves over 4- / 8-wide nodes collapsed from the
+ *                   engine's tree: hit children visited nearest first
+ * Or'ing ATLAS_RT_KERNEL_STACK_REGISTER keeps the top of each lane's
+ * traversal stack in registers (the rest in LDS, then global memory) instead
+ * of all of it in LDS.  Every variant returns the engine's nearest hit
+ * distance bit for bit; BVH4/BVH8 may report a different triangle where two
+ * are hit at exactly the same t (the engine reports the first it tests). */
 #define ATLAS_RT_KERNEL_AUTO 0
 #define ATLAS_RT_KERNEL_BVH2_EXACT 1    /* engine order: near child first, left on ties */
 #define ATLAS_RT_KERNEL_BVH2_PERSISTENT 2
 #define ATLAS_RT_KERNEL_BVH4 3
 #define ATLAS_RT_KERNEL_BVH8 4
+#define ATLAS_RT_KERNEL_VARIANT_MASK 0xF
+#define ATLAS_RT_KERNEL_STACK_REGISTER 0x10
 
 /* Closest hit, engine semantics of HitClosest (bvh.hsh:191-273, CheckLeafClosest
  * :44-72, IntersectAABB/IntersectTriangle intersections.hsh:18-58): accept
@@ -277,8 +288,11 @@ atlas_rt_status atlas_rt_batch_trace_closest(atlas_rt_batch_set* set, const atla
                                              atlas_rt_hit* d_hits, const uint32_t* d_count,
                                              const atlas_rt_launch_config* config, void* stream);
 
-/* The autotuner's choice for this set and its trials (kernel x work-group
- * size x refill threshold, timed with HIP events on the first AUTO launch).
+/* The autotuner's choice for this set and its trials, timed with HIP events
+ * on the first AUTO launch: node width (2, 4, 8) x stack placement (LDS or
+ * register top) x work-group size (64, 128, 256), binary nodes also one ray
+ * per lane and at three refill thresholds.  Wide nodes run in resident waves
+ * only (a grid request returns ATLAS_RT_E_UNSUPPORTED).
  * *n_trials is the capacity of trials/trial_ms on input, the trial count on
  * output.  Any output may be NULL. */
 atlas_rt_status atlas_rt_batch_tuning(const atlas_rt_batch_set* set, atlas_rt_launch_config* best,

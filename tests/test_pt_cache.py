@@ -161,5 +161,5 @@ def test_gpu_pt_cache_gain_at_equal_samples(torch_cuda):
     e_untrained, _ = cached(0, 0)
     print(f"pt cache gain at {spp} spp: rel-MSE plain {e_plain:.5f}, trained cache {e_trained:.5f} "
           f"(last loss {st[2]:.4f}, {st[0]} samples/frame), untrained cache {e_untrained:.5f}")
-    assert e_trained < e_plain
-    assert e_trained < 0.5 * e_untrained
+    assert e_trained < 0.8 * e_plain
+    assert e_trained < 0.8 * e_untrained
