@@ -179,6 +179,21 @@ class Blas:
         return ms.value
 
 
+def aabb_tree_build(boxes, parallel=True):
+    """The engine's AABB-only tree (Volume::BVH(aabbs)) on the host: (nodes,
+    ref_idx, end_of_node) for (n, 6) float32 boxes (min xyz, max xyz)."""
+    boxes = np.ascontiguousarray(boxes, dtype=np.float32).reshape(-1, 6)
+    n = len(boxes)
+    cap_nodes, cap_refs = max(n, 1), max(n, 2)
+    nodes = np.zeros(cap_nodes, dtype=NODE_DTYPE)
+    ref_idx = np.zeros(cap_refs, dtype=np.uint32)
+    end = np.zeros(cap_refs, dtype=np.uint8)
+    nn, nr = C.c_uint32(cap_nodes), C.c_uint32(cap_refs)
+    check(lib().atlas_rt_aabb_tree_build(_ptr(boxes), n, 1 if parallel else 0, _ptr(nodes), C.byref(nn),
+                                         _ptr(ref_idx), _ptr(end), C.byref(nr)))
+    return nodes[:nn.value], ref_idx[:nr.value], end[:nr.value]
+
+
 def instance_descs(object_to_world, blas_index, mask=_abi.MASK_ALL | _abi.MASK_SHADOW, material_offset=0):
     """Instance descriptors from (n, 3, 4) or (n, 4, 4) object-to-world matrices.
 
