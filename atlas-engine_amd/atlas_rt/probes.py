@@ -1,0 +1,153 @@
+"""Fixed workloads for profiler runs (rocprofv3 --kernel-trace --stats, or
+one --pmc counter per run).  Run from the package directory's parent:
+
+    PYTHONPATH=atlas-engine_amd rocprofv3 --kernel-trace --stats -d out -o pt -- python3 -m atlas_rt.probes pt cache
+    PYTHONPATH=atlas-engine_amd rocprofv3 --pmc FETCH_SIZE -d out -o fetch -- python3 -m atlas_rt.probes pmc
+    python3 -m atlas_rt.probes summarize-pmc out
+
+  pt plain|cache  the dense radiance-cache scene (256 blob instances in a
+                  30-unit cube) at 1280 x 720, 8 bounces: 4 untimed frames,
+                  then 8 frames, plain or with the cache (1/32 training
+                  pixels, 2 Adam steps per frame)
+  nrc             cache training steps at 32 .. 262,144 samples and
+                  inference of 65,536 and 1,048,576 queries
+  pmc             a 256 MiB device-to-device copy (the FETCH_SIZE /
+                  WRITE_SIZE calibration) and the C0 closest-hit trace, 4 each
+"""
+import csv
+import glob
+import json
+import os
+import re
+import sys
+import time
+
+import numpy as np
+
+COPY_BYTES = 256 << 20
+
+
+def pt(mode):
+    import torch
+
+    from . import PathTracer, RadianceCache
+    from . import workloads as W
+    from .bench_suite import _world
+    meshes, mats_d, xf, which = W.c4_scene()
+    w, blases = _world(meshes, xf, which)
+    cc = W.c4_camera()
+    from . import camera
+    cam = camera(cc["eye"], cc["yaw_deg"], cc["pitch_deg"], cc["fov_y_deg"], cc["width"] / cc["height"])
+    p = PathTracer(w, cc["width"], cc["height"], 8, W.c2_materials(mats_d), W.sun_light(), env=W.C2_ENV)
+    if mode == "cache":
+        rc = RadianceCache((-20, -20, -20), (20, 20, 20), levels=12, log2_table=19, base_resolution=16,
+                           max_resolution=2048)
+        p.set_cache(rc, train_stride=32, train_steps=2)
+    p.render(cam, 4)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    p.render(cam, 8)
+    torch.cuda.synchronize()
+    print(f"pt {mode}: {(time.perf_counter() - t) / 8 * 1e3:.3f} ms/frame", flush=True)
+
+
+def nrc():
+    import torch
+
+    from . import RadianceCache
+    g = np.random.default_rng(0)
+
+    def q(n):
+        a = np.zeros((n, 12), np.float32)
+        a[:, :3] = g.uniform(-20, 20, (n, 3))
+        a[:, 3] = g.uniform(0, 1, n)
+        for k in (4, 8):
+            d = g.normal(size=(n, 3))
+            a[:, k:k + 3] = d / np.linalg.norm(d, axis=1, keepdims=True)
+        return torch.from_numpy(a).cuda()
+
+    rc = RadianceCache((-20, -20, -20), (20, 20, 20), levels=12, log2_table=19, base_resolution=16,
+                       max_resolution=2048)
+    for n in (32, 1024, 8192, 21504, 65536, 262144):
+        Q, T = q(n), torch.rand((n, 4), device="cuda")
+        rc.train(Q, T)
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        for _ in range(10):
+            rc.train(Q, T)
+        e.record()
+        torch.cuda.synchronize()
+        print(f"train step n={n}: {s.elapsed_time(e) / 10 * 1e3:.1f} us", flush=True)
+    for n in (65536, 1 << 20):
+        Q = q(n)
+        out = rc.infer(Q)
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        for _ in range(10):
+            rc.infer(Q, out)
+        e.record()
+        torch.cuda.synchronize()
+        t = s.elapsed_time(e) / 10
+        print(f"infer n={n}: {t * 1e3:.1f} us, {n / t / 1e6:.2f} Gq/s", flush=True)
+
+
+def pmc(kernel=2, reps=4):
+    import torch
+
+    from . import Blas
+    from . import workloads as W
+    src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    verts, rays = W.c0()
+    blas = Blas.build(verts)
+    d_rays = torch.from_numpy(rays.view(np.float32).reshape(-1, 8).copy()).cuda()
+    d_hits = torch.empty((len(rays), 4), dtype=torch.float32, device="cuda")
+    for _ in range(reps):
+        blas.trace_closest(d_rays, d_hits, kernel=kernel)
+    torch.cuda.synchronize()
+    print("pmc probe done", flush=True)
+
+
+def summarize_pmc(out_dir):
+    """Per-dispatch counter averages of the copy and the trace kernels, and
+    the trace kernel's traffic scaled by the copy's known bytes."""
+    acc = {}
+    for path in glob.glob(os.path.join(out_dir, "**", "*counter_collection.csv"), recursive=True):
+        with open(path) as f:
+            for row in csv.DictReader(f):
+                name = row.get("Kernel_Name", "")
+                kind = "copy" if re.search(r"copy", name, re.I) else ("trace" if "k_blas_closest" in name else None)
+                if kind is None:
+                    continue
+                key = (kind, row.get("Counter_Name", ""))
+                d = acc.setdefault(key, {})
+                disp = row.get("Dispatch_Id", str(len(d)))
+                d[disp] = d.get(disp, 0.0) + float(row.get("Counter_Value", 0) or 0)
+    out = {}
+    for (kind, counter), d in acc.items():
+        out[f"{kind}:{counter}_kib_per_dispatch"] = sum(d.values()) / max(1, len(d))
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        c, t = out.get(f"copy:{counter}_kib_per_dispatch"), out.get(f"trace:{counter}_kib_per_dispatch")
+        if c and t:
+            scale = (COPY_BYTES / 1024.0) / c
+            out[f"{counter}_scale"] = scale
+            out[f"trace_{counter}_bytes"] = t * scale * 1024.0
+    return out
+
+
+if __name__ == "__main__":
+    cmd = sys.argv[1] if len(sys.argv) > 1 else ""
+    if cmd == "pt":
+        pt(sys.argv[2] if len(sys.argv) > 2 else "plain")
+    elif cmd == "nrc":
+        nrc()
+    elif cmd == "pmc":
+        pmc()
+    elif cmd == "summarize-pmc":
+        print(json.dumps(summarize_pmc(sys.argv[2]), indent=1))
+    else:
+        print(__doc__)
+        sys.exit(2)
