@@ -55,6 +55,30 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_blas_closest_persistent_c0_pmc.md")
+
+
+def pmc_traffic():
+    """HBM traffic per launch of the persistent C0 kernel from the committed
+    FETCH_SIZE / WRITE_SIZE runs (calibrated against a copy kernel), plus the
+    L2 hit rate and VALU busy fraction of the same runs; None if absent."""
+    try:
+        text = open(PMC_SUMMARY).read()
+    except OSError:
+        return None
+    import re
+    out = {}
+    m = re.search(r"traffic_bytes_per_launch: (\d+)", text)
+    if not m:
+        return None
+    out["traffic"] = int(m.group(1))
+    m = re.search(r"L2 \(TCC\) hit rate: ([0-9.]+)%", text)
+    out["l2_hit_rate"] = float(m.group(1)) / 100 if m else None
+    m = re.search(r"VALU busy: .*= ([0-9.]+)%", text)
+    out["valu_busy"] = float(m.group(1)) / 100 if m else None
+    return out
+
+
 def host_threads():
     try:
         n = len(os.sched_getaffinity(0))
@@ -191,6 +215,7 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4),
             "traffic": None,
+            "traffic_source": None,
             "kernel_ms": round(kernel_ms, 4),
             "alg_bytes_per_launch": alg_bytes,
             "node_visits_per_ray": round(node_visits / n, 3),
@@ -199,6 +224,13 @@ def main():
         "build_s": round(build_s, 3),
         "hit_fraction": round(float((hits["prim_id"] >= 0).mean()), 4),
     }
+    pmc = pmc_traffic()
+    if pmc and kernel == 2:
+        out["roofline"]["traffic"] = pmc["traffic"]
+        out["roofline"]["traffic_source"] = "profiles/k_blas_closest_persistent_c0_pmc.md (FETCH_SIZE + WRITE_SIZE, copy-calibrated)"
+        out["roofline"]["traffic_GBps"] = round(pmc["traffic"] / (kernel_ms / 1000.0) / 1e9, 1)
+        out["roofline"]["l2_hit_rate"] = pmc["l2_hit_rate"]
+        out["roofline"]["valu_busy"] = pmc["valu_busy"]
     if not args.no_cpu_baseline:
         cb, parity = cpu_baseline(nodes, tris, orig, rays, hits, args.cpu_seconds)
         out["cpu_baseline"] = cb
