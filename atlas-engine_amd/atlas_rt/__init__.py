@@ -278,6 +278,18 @@ class World:
         check(lib().atlas_rt_trace_any(self._h, _ptr(d_in), _ptr(d_occluded), _ptr(d_count), max_count, tmin, tmax,
                                        mask, flags, _stream_ptr(stream)))
 
+    def trace_closest_transparency_packed(self, d_in, d_out, d_count, max_count, tmin=0.0, tmax=1e12,
+                                          mask=_abi.MASK_ALL, stream=None):
+        """HitClosestTransparency over PackedRay buffers (baked opacity)."""
+        check(lib().atlas_rt_trace_closest_transparency(self._h, _ptr(d_in), _ptr(d_out), _ptr(d_count), max_count,
+                                                        tmin, tmax, mask, _stream_ptr(stream)))
+
+    def trace_transparency_packed(self, d_in, d_trans, d_count, max_count, tmin=0.0, tmax=1e12,
+                                  mask=_abi.MASK_SHADOW, stream=None):
+        """HitAnyTransparency over PackedRay buffers: float32 transparency per ray."""
+        check(lib().atlas_rt_trace_transparency(self._h, _ptr(d_in), _ptr(d_trans), _ptr(d_count), max_count, tmin,
+                                                tmax, mask, _stream_ptr(stream)))
+
 
 def build_many(meshes, host_only=False):
     """Build one BLAS per (n, 9) vertex array, one host thread per mesh."""
@@ -357,6 +369,20 @@ def _blas_set_shading(self, normals, material=None):
 Blas.set_shading = _blas_set_shading
 
 
+def _blas_set_opacity(self, opacity=None, uvs=None):
+    """Baked per-triangle opacity ((n,), caller order; -1 = opacity map) and
+    texture coordinates ((n, 6): uv0, uv1, uv2).  Call after set_shading and
+    before creating a PathTracer over the world."""
+    if opacity is not None:
+        opacity = np.ascontiguousarray(opacity, dtype=np.float32).reshape(-1)
+    if uvs is not None:
+        uvs = np.ascontiguousarray(uvs, dtype=np.float32).reshape(-1, 6)
+    check(lib().atlas_rt_blas_set_opacity(self._h, _ptr(opacity), _ptr(uvs)))
+
+
+Blas.set_opacity = _blas_set_opacity
+
+
 def camera(eye, yaw_deg, pitch_deg, fov_y_deg, aspect, near=0.1):
     """atlas_rt_camera from an eye point and yaw/pitch (y up), built like
     CameraComponent::GetFrustumCorners' near-plane corners 4, 5, 6."""
@@ -425,6 +451,33 @@ class PathTracer:
         n = C.c_int32()
         check(lib().atlas_rt_pathtracer_sample_count(self._h, C.byref(n)))
         return n.value
+
+    def set_textures(self, textures):
+        """Opacity textures: a list of (height, width) uint8 arrays (R8 UNORM)."""
+        keep = [np.ascontiguousarray(t, dtype=np.uint8) for t in textures]
+        arr = (_abi.Texture * max(len(keep), 1))()
+        for i, t in enumerate(keep):
+            arr[i].height, arr[i].width = t.shape
+            arr[i].texels = t.ctypes.data
+        check(lib().atlas_rt_pathtracer_set_textures(self._h, arr if keep else None, len(keep)))
+
+    def set_emissive_lights(self, emitters):
+        """Triangle lights from emissive materials (PathTracingRenderer's
+        sampleEmissives): emitters (m, 4) float32 per material table entry,
+        (sRGB emissive colour, emissive intensity); None turns them off."""
+        if emitters is None:
+            check(lib().atlas_rt_pathtracer_set_emissive_lights(self._h, None, 0))
+            return
+        em = np.ascontiguousarray(emitters, dtype=np.float32).reshape(-1, 4)
+        check(lib().atlas_rt_pathtracer_set_emissive_lights(self._h, _ptr(em), len(em)))
+
+    def lights(self):
+        """The light list in use: atlas_rt_gpu_light records (GPU_LIGHT_DTYPE)."""
+        n = C.c_uint32()
+        check(lib().atlas_rt_pathtracer_lights(self._h, None, 0, C.byref(n)))
+        out = np.zeros(n.value, _abi.GPU_LIGHT_DTYPE)
+        check(lib().atlas_rt_pathtracer_lights(self._h, _ptr(out) if n.value else None, n.value, C.byref(n)))
+        return out
 
     def set_cache(self, cache, train_stride=32, train_steps=4):
         """Attach a RadianceCache (None: plain path tracing)."""

@@ -71,14 +71,22 @@ class LaunchConfig(C.Structure):
     _fields_ = [("kernel", C.c_int32), ("block", C.c_uint32), ("refill", C.c_uint32), ("reserved", C.c_uint32)]
 
 
-# atlas_rt_material (48 B), atlas_rt_light (32 B).
+# atlas_rt_material (64 B), atlas_rt_light (32 B), atlas_rt_gpu_light (64 B).
 MATERIAL_DTYPE = np.dtype([
     ("base_color", "<f4", 3), ("opacity", "<f4"), ("emissive", "<f4", 3), ("roughness", "<f4"),
     ("metalness", "<f4"), ("ao", "<f4"), ("reflectance", "<f4"), ("two_sided", "<i4"),
+    ("opacity_texture", "<i4"), ("invert_uvs", "<i4"), ("reserved", "<i4", 2),
 ])
 LIGHT_DTYPE = np.dtype([("direction", "<f4", 3), ("type", "<u4"), ("radiance", "<f4", 3), ("reserved", "<f4")])
-assert MATERIAL_DTYPE.itemsize == 48 and LIGHT_DTYPE.itemsize == 32
+GPU_LIGHT_DTYPE = np.dtype([("P", "<f4", 4), ("N", "<f4", 4), ("color", "<f4", 4), ("data", "<f4", 4)])
+assert MATERIAL_DTYPE.itemsize == 64 and LIGHT_DTYPE.itemsize == 32 and GPU_LIGHT_DTYPE.itemsize == 64
 LIGHT_DIRECTIONAL = 0
+LIGHT_TRIANGLE = 1
+
+
+class Texture(C.Structure):
+    """atlas_rt_texture (R8 UNORM, row-major, host memory)."""
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("texels", C.c_void_p)]
 
 
 class Camera(C.Structure):
@@ -149,6 +157,12 @@ PROTOTYPES = {
     "atlas_rt_batch_graph_launch": (status, [vp, vp]),
     "atlas_rt_aabb_tree_build": (status, [vp, u32, u32, vp, C.POINTER(u32), vp, vp, C.POINTER(u32)]),
     "atlas_rt_blas_set_shading": (status, [vp, vp, vp]),
+    "atlas_rt_blas_set_opacity": (status, [vp, vp, vp]),
+    "atlas_rt_trace_closest_transparency": (status, [vp, vp, vp, vp, u32, f32, f32, u32, vp]),
+    "atlas_rt_trace_transparency": (status, [vp, vp, vp, vp, u32, f32, f32, u32, vp]),
+    "atlas_rt_pathtracer_set_textures": (status, [vp, vp, u32]),
+    "atlas_rt_pathtracer_set_emissive_lights": (status, [vp, vp, u32]),
+    "atlas_rt_pathtracer_lights": (status, [vp, vp, u32, C.POINTER(u32)]),
     "atlas_rt_pathtracer_create": (status, [vp, vp, vp, u32, vp, u32, C.POINTER(vp)]),
     "atlas_rt_pathtracer_destroy": (status, [vp]),
     "atlas_rt_pathtracer_render": (status, [vp, vp, u32, vp]),

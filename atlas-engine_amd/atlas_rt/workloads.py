@@ -329,6 +329,7 @@ def c2_materials(mats):
     out["metalness"] = mats.get("metalness", np.zeros(n, np.float32))
     out["ao"] = 1.0
     out["reflectance"] = 0.5
+    out["opacity_texture"] = -1
     return out
 
 

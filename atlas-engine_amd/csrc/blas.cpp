@@ -267,6 +267,7 @@ atlas_rt_status atlas_rt_blas_destroy(atlas_rt_blas* b) {
     if (b->d_tris) hipFree(b->d_tris);
     if (b->d_vtx) hipFree(b->d_vtx);
     if (b->d_shade) hipFree(b->d_shade);
+    if (b->d_uv) hipFree(b->d_uv);
     if (b->d_nodes4) hipFree(b->d_nodes4);
     if (b->d_nodes8) hipFree(b->d_nodes8);
     b->spill.release();
@@ -282,10 +283,14 @@ atlas_rt_status atlas_rt_blas_set_shading(atlas_rt_blas* b, const float* normals
     // Reorder to BVH slot order (slot i holds caller triangle orig_idx[i]).
     const size_t n = b->tris.size();
     std::vector<float> packed(12 * n);
+    std::vector<float> host_n(9 * n);
+    std::vector<int32_t> host_m(n);
     for (size_t i = 0; i < n; i++) {
         const uint32_t src = b->orig_idx[i];
         const float* nv = normals + 9 * size_t(src);
         const int32_t mat = material ? material[src] : 0;
+        std::memcpy(&host_n[9 * i], nv, 9 * sizeof(float));
+        host_m[i] = mat;
         float* o = &packed[12 * i];
         o[0] = nv[0];
         o[1] = nv[1];
@@ -294,7 +299,7 @@ atlas_rt_status atlas_rt_blas_set_shading(atlas_rt_blas* b, const float* normals
         o[4] = nv[3];
         o[5] = nv[4];
         o[6] = nv[5];
-        o[7] = 0.0f;
+        o[7] = 1.0f;  // baked opacity (atlas_rt_blas_set_opacity)
         o[8] = nv[6];
         o[9] = nv[7];
         o[10] = nv[8];
@@ -308,7 +313,47 @@ atlas_rt_status atlas_rt_blas_set_shading(atlas_rt_blas* b, const float* normals
         return hip_fail(e, "atlas_rt_blas_set_shading");
     }
     if (b->d_shade) hipFree(b->d_shade);
+    if (b->d_uv) hipFree(b->d_uv);
     b->d_shade = reinterpret_cast<float4*>(d);
+    b->d_uv = nullptr;
+    b->has_alpha = false;
+    b->shade_normals.swap(host_n);
+    b->shade_mat.swap(host_m);
+    return ATLAS_RT_OK;
+}
+
+atlas_rt_status atlas_rt_blas_set_opacity(atlas_rt_blas* b, const float* opacity, const float* uvs) {
+    if (!b) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_blas_set_opacity: NULL handle");
+    if (!b->d_shade)
+        return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_blas_set_opacity: call atlas_rt_blas_set_shading first");
+    const size_t n = b->tris.size();
+    // baked opacity goes to the w lane of each slot's second shading float4
+    std::vector<float> op(n, 1.0f);
+    bool alpha = false;
+    for (size_t i = 0; i < n; i++) {
+        if (opacity) op[i] = opacity[b->orig_idx[i]];
+        alpha = alpha || op[i] != 1.0f;
+    }
+    float4* d_uv = nullptr;
+    if (uvs) {
+        std::vector<float> packed(8 * n, 0.0f);
+        for (size_t i = 0; i < n; i++) std::memcpy(&packed[8 * i], uvs + 6 * size_t(b->orig_idx[i]), 6 * sizeof(float));
+        ATLAS_HIP_TRY(hipMalloc(reinterpret_cast<void**>(&d_uv), packed.size() * sizeof(float)));
+        hipError_t e = hipMemcpy(d_uv, packed.data(), packed.size() * sizeof(float), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            hipFree(d_uv);
+            return hip_fail(e, "atlas_rt_blas_set_opacity");
+        }
+    }
+    hipError_t e = hipMemcpy2D(reinterpret_cast<char*>(b->d_shade) + 7 * sizeof(float), 12 * sizeof(float), op.data(),
+                               sizeof(float), sizeof(float), n, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        if (d_uv) hipFree(d_uv);
+        return hip_fail(e, "atlas_rt_blas_set_opacity");
+    }
+    if (b->d_uv) hipFree(b->d_uv);
+    b->d_uv = d_uv;
+    b->has_alpha = alpha;
     return ATLAS_RT_OK;
 }
 

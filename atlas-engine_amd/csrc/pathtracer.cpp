@@ -4,6 +4,7 @@
 #include "nrc.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 
 using namespace atlas_rt;
@@ -21,7 +22,8 @@ atlas_rt_status dev_alloc(T*& p, size_t count) {
 
 void free_all(PathTracer& P) {
     void* ptrs[] = {P.ctl, P.paths[0], P.paths[1], P.hits, P.shadow, P.Lpix, P.accum, P.xform,
-                    const_cast<float4**>(P.shade_table), P.materials, P.lights};
+                    const_cast<float4**>(P.shade_table), const_cast<float4**>(P.uv_table), P.materials, P.lights,
+                    P.tex, P.texels};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     P.spill.release();
@@ -37,7 +39,120 @@ uint64_t pcg_initial_state() {
     return state;
 }
 
+float bits_to_float(uint32_t u) {
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+struct H3 {
+    float x, y, z;
+};
+inline H3 operator+(H3 a, H3 b) { return H3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline H3 operator-(H3 a, H3 b) { return H3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline float hdot(H3 a, H3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+// glm's mat4 * vec4 over object-to-world rows (columns summed in order)
+inline H3 hxf(const float* M, H3 p, float w) {
+    return H3{((M[0] * p.x + M[1] * p.y) + M[2] * p.z) + M[3] * w, ((M[4] * p.x + M[5] * p.y) + M[6] * p.z) + M[7] * w,
+              ((M[8] * p.x + M[9] * p.y) + M[10] * p.z) + M[11] * w};
+}
+
+// RayTracingWorld::BuildTriangleLightsForMesh (RayTracingWorld.cpp:322-376)
+// per BLAS slot and UpdateTriangleLights (:378-407) per instance of the BLAS:
+// centroid and mean vertex normal moved to world space, Heron's area of the
+// object-space triangle, weight = area * brightness of linear(colour), radiance
+// = linear(linear(colour)) * intensity.  The instance index is the world's
+// TLAS slot of the instance.
+void triangle_lights(const atlas_rt_world& W, const float* emitters, uint32_t n_emitters, std::vector<Light>& out) {
+    for (size_t k = 0; k < W.blas_handles.size(); k++) {
+        const atlas_rt_blas& b = *W.blas_handles[k];
+        for (size_t i = 0; i < b.tris.size(); i++) {
+            const atlas_rt_gpu_bvh_triangle& tri = b.tris[i];
+            const float* ns = &b.shade_normals[9 * i];
+            for (size_t j = 0; j < W.instances.size(); j++) {
+                const atlas_rt_gpu_bvh_instance& in = W.instances[j];
+                if (in.mesh_offset != int32_t(k)) continue;
+                const int64_t e = int64_t(b.shade_mat[i]) + in.material_offset;
+                if (e < 0 || e >= int64_t(n_emitters)) continue;
+                const float* em = emitters + 4 * size_t(e);
+                const H3 lin{std::pow(em[0], 2.2f), std::pow(em[1], 2.2f), std::pow(em[2], 2.2f)};
+                const float brightness = hdot(lin, H3{0.3333f, 0.3333f, 0.3333f});
+                if (!(brightness > 0.0f)) continue;
+                const H3 v0{tri.v0[0], tri.v0[1], tri.v0[2]}, v1{tri.v1[0], tri.v1[1], tri.v1[2]},
+                    v2{tri.v2[0], tri.v2[1], tri.v2[2]};
+                const H3 c = (v0 + v1) + v2;
+                const H3 nsum = (H3{ns[0], ns[1], ns[2]} + H3{ns[3], ns[4], ns[5]}) + H3{ns[6], ns[7], ns[8]};
+                const H3 P{c.x / 3.0f, c.y / 3.0f, c.z / 3.0f}, N{nsum.x / 3.0f, nsum.y / 3.0f, nsum.z / 3.0f};
+                const H3 ea = v1 - v0, eb = v2 - v0, ec = v1 - v2;
+                const float a = std::sqrt(hdot(ea, ea)), bb = std::sqrt(hdot(eb, eb)), cc = std::sqrt(hdot(ec, ec));
+                const float p = 0.5f * ((a + bb) + cc);
+                const float area = std::sqrt(((p * (p - a)) * (p - bb)) * (p - cc));
+                const float* M = &W.xform[12 * j];
+                const H3 WP = hxf(M, P, 1.0f), WN = hxf(M, N, 0.0f);
+                Light L{};
+                L.P[0] = WP.x;
+                L.P[1] = WP.y;
+                L.P[2] = WP.z;
+                L.P[3] = 1.0f;
+                L.N[0] = WN.x;
+                L.N[1] = WN.y;
+                L.N[2] = WN.z;
+                L.color[0] = std::pow(lin.x, 2.2f) * em[3];
+                L.color[1] = std::pow(lin.y, 2.2f) * em[3];
+                L.color[2] = std::pow(lin.z, 2.2f) * em[3];
+                L.data[0] = bits_to_float((ATLAS_RT_LIGHT_TRIANGLE << 28) | uint32_t(i));
+                L.data[1] = area * brightness;
+                L.data[2] = area;
+                L.data[3] = bits_to_float(uint32_t(j));
+                out.push_back(L);
+            }
+        }
+    }
+}
+
+atlas_rt_status upload_lights(PathTracer& P) {
+    const uint32_t n = uint32_t(P.light_list.size());
+    if (n > P.light_cap) {
+        Light* d = nullptr;
+        ATLAS_HIP_TRY(hipMalloc(reinterpret_cast<void**>(&d), n * sizeof(Light)));
+        if (P.lights) (void)hipFree(P.lights);
+        P.lights = d;
+        P.light_cap = n;
+    }
+    if (n) ATLAS_HIP_TRY(hipMemcpy(P.lights, P.light_list.data(), n * sizeof(Light), hipMemcpyHostToDevice));
+    P.light_count = int32_t(n);
+    P.shade_args.lights = P.lights;
+    P.shade_args.light_count = P.light_count;
+    return ATLAS_RT_OK;
+}
+
 }  // namespace
+
+void atlas_rt::pt::build_light_list(PathTracer& P, const float* emitters, uint32_t emitter_count) {
+    // UpdateLights (RayTracingHelper.cpp:436-500): punctual lights weighted by
+    // brightness, then the triangle lights; every weight floored at 0.5% of
+    // the maximum, then normalised.
+    std::vector<Light>& L = P.light_list;
+    L.clear();
+    for (size_t l = 0; l < P.punctual.size(); l++) {
+        const atlas_rt_light& src = P.punctual[l];
+        Light g{};
+        g.P[3] = 1.0f;
+        std::memcpy(g.N, src.direction, 12);
+        std::memcpy(g.color, src.radiance, 12);
+        g.data[0] = bits_to_float((ATLAS_RT_LIGHT_DIRECTIONAL << 28) | uint32_t(l));
+        g.data[1] = (src.radiance[0] * 0.3333f + src.radiance[1] * 0.3333f) + src.radiance[2] * 0.3333f;
+        L.push_back(g);
+    }
+    if (emitters && emitter_count) triangle_lights(*P.world, emitters, emitter_count, L);
+    float maxw = 0.0f, total = 0.0f;
+    for (const Light& g : L) maxw = std::max(maxw, g.data[1]);
+    for (Light& g : L) {
+        g.data[1] = std::max(g.data[1], 0.005f * maxw);
+        total += g.data[1];
+    }
+    for (Light& g : L) g.data[1] /= total;
+}
 
 void atlas_rt::pt::free_cache_buffers(PathTracer& P) {
     void* ptrs[] = {P.rec, P.q_buf, P.q_meta, P.q_out, P.s_q, P.s_t};
@@ -74,8 +189,10 @@ atlas_rt_status atlas_rt_pathtracer_create(const atlas_rt_world* world, const at
     P.width = params->width;
     P.height = params->height;
     P.bounces = params->bounces;
-    P.light_count = int32_t(light_count);
+    P.punctual.assign(lights, lights + light_count);
+    P.material_count = material_count;
     P.world = world;
+    for (const atlas_rt_blas* b : world->blas_handles) P.alpha = P.alpha || b->has_alpha;
     P.world_dev = world->dev();
     P.cap = world->max_depth + 1;
     const size_t pixels = size_t(P.width) * P.height;
@@ -89,8 +206,8 @@ atlas_rt_status atlas_rt_pathtracer_create(const atlas_rt_world* world, const at
     if (st == ATLAS_RT_OK) st = dev_alloc(P.accum, pixels);
     if (st == ATLAS_RT_OK) st = dev_alloc(P.xform, 3 * world->instances.size());
     if (st == ATLAS_RT_OK) st = dev_alloc(P.shade_table, world->blas_handles.size());
+    if (st == ATLAS_RT_OK) st = dev_alloc(P.uv_table, world->blas_handles.size());
     if (st == ATLAS_RT_OK) st = dev_alloc(P.materials, material_count);
-    if (st == ATLAS_RT_OK) st = dev_alloc(P.lights, std::max<uint32_t>(light_count, 1));
     const uint32_t grid_threads = uint32_t((pixels + 127) / 128 * 128);
     if (st == ATLAS_RT_OK && P.cap > 32) st = P.spill.ensure(size_t(P.cap - 32) * grid_threads * sizeof(int32_t));
     if (st == ATLAS_RT_OK) st = P.flags.ensure(256);
@@ -99,27 +216,11 @@ atlas_rt_status atlas_rt_pathtracer_create(const atlas_rt_world* world, const at
         delete pt;
         return st;
     }
-    // Light list (UpdateLights, RayTracingHelper.cpp:436-500): weights are the
-    // brightness, floored at 0.5% of the maximum and normalised.
-    std::vector<Light> L(std::max<uint32_t>(light_count, 1));
-    float maxw = 0.0f, total = 0.0f;
-    for (uint32_t l = 0; l < light_count; l++) {
-        const float* r = lights[l].radiance;
-        L[l].pdf = (r[0] * 0.3333f + r[1] * 0.3333f) + r[2] * 0.3333f;
-        maxw = std::max(maxw, L[l].pdf);
+    std::vector<const float4*> shade(world->blas_handles.size()), uv(world->blas_handles.size());
+    for (size_t k = 0; k < shade.size(); k++) {
+        shade[k] = world->blas_handles[k]->d_shade;
+        uv[k] = world->blas_handles[k]->d_uv;
     }
-    for (uint32_t l = 0; l < light_count; l++) {
-        L[l].pdf = std::max(L[l].pdf, 0.005f * maxw);
-        total += L[l].pdf;
-    }
-    for (uint32_t l = 0; l < light_count; l++) {
-        std::memcpy(L[l].N, lights[l].direction, 12);
-        std::memcpy(L[l].radiance, lights[l].radiance, 12);
-        L[l].type = lights[l].type;
-        L[l].pdf /= total;
-    }
-    std::vector<const float4*> shade(world->blas_handles.size());
-    for (size_t k = 0; k < shade.size(); k++) shade[k] = world->blas_handles[k]->d_shade;
     FrameCtl ctl{};
     ctl.pcg_state = pcg_initial_state();
     hipError_t e = hipSuccess;
@@ -131,12 +232,20 @@ atlas_rt_status atlas_rt_pathtracer_create(const atlas_rt_world* world, const at
     if (e == hipSuccess)
         e = hipMemcpy(const_cast<float4**>(P.shade_table), shade.data(), shade.size() * sizeof(float4*),
                       hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(const_cast<float4**>(P.uv_table), uv.data(), uv.size() * sizeof(float4*), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(P.materials, materials, material_count * sizeof(Material), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(P.lights, L.data(), L.size() * sizeof(Light), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         free_all(P);
         delete pt;
         return hip_fail(e, "atlas_rt_pathtracer_create");
+    }
+    build_light_list(P, nullptr, 0);
+    st = upload_lights(P);
+    if (st != ATLAS_RT_OK) {
+        free_all(P);
+        delete pt;
+        return st;
     }
     ShadeArgs& A = P.shade_args;
     A.world = P.world_dev;
@@ -145,6 +254,7 @@ atlas_rt_status atlas_rt_pathtracer_create(const atlas_rt_world* world, const at
     A.materials = P.materials;
     A.lights = P.lights;
     A.light_count = P.light_count;
+    A.alpha = AlphaArgs{P.shade_table, P.uv_table, P.materials, nullptr, nullptr, 0u, 0u};
     A.bounces = P.bounces;
     std::memcpy(A.env, params->env_radiance, 12);
     A.ctl = P.ctl;
@@ -176,6 +286,66 @@ atlas_rt_status atlas_rt_pathtracer_render(atlas_rt_pathtracer* pt, const atlas_
         if (st != ATLAS_RT_OK) return st;
         pt->P.host_samples++;
     }
+    return ATLAS_RT_OK;
+}
+
+atlas_rt_status atlas_rt_pathtracer_set_textures(atlas_rt_pathtracer* pt, const atlas_rt_texture* textures,
+                                                 uint32_t count) {
+    if (!pt || (count && !textures)) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_set_textures: NULL argument");
+    PathTracer& P = pt->P;
+    std::vector<TexDesc> desc(count);
+    size_t bytes = 0;
+    for (uint32_t t = 0; t < count; t++) {
+        const atlas_rt_texture& T = textures[t];
+        if (!T.width || !T.height || !T.texels)
+            return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_set_textures: empty texture");
+        if (bytes + size_t(T.width) * T.height > 0xFFFFFFFFull)
+            return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_set_textures: more than 4 GiB of texels");
+        desc[t] = TexDesc{uint32_t(bytes), T.width, T.height, 0u};
+        bytes += size_t(T.width) * T.height;
+    }
+    ATLAS_HIP_TRY(hipDeviceSynchronize());
+    if (P.tex) (void)hipFree(P.tex);
+    if (P.texels) (void)hipFree(P.texels);
+    P.tex = nullptr;
+    P.texels = nullptr;
+    P.tex_count = 0;
+    P.shade_args.alpha.tex = nullptr;
+    P.shade_args.alpha.texels = nullptr;
+    P.shade_args.alpha.tex_count = 0;
+    if (!count) return ATLAS_RT_OK;
+    atlas_rt_status st = dev_alloc(P.tex, count);
+    if (st == ATLAS_RT_OK) st = dev_alloc(P.texels, bytes);
+    if (st != ATLAS_RT_OK) return st;
+    ATLAS_HIP_TRY(hipMemcpy(P.tex, desc.data(), count * sizeof(TexDesc), hipMemcpyHostToDevice));
+    for (uint32_t t = 0; t < count; t++)
+        ATLAS_HIP_TRY(hipMemcpy(P.texels + desc[t].offset, textures[t].texels,
+                                size_t(textures[t].width) * textures[t].height, hipMemcpyHostToDevice));
+    P.tex_count = count;
+    P.shade_args.alpha.tex = P.tex;
+    P.shade_args.alpha.texels = P.texels;
+    P.shade_args.alpha.tex_count = count;
+    return ATLAS_RT_OK;
+}
+
+atlas_rt_status atlas_rt_pathtracer_set_emissive_lights(atlas_rt_pathtracer* pt, const float* emitters,
+                                                        uint32_t count) {
+    if (!pt) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_set_emissive_lights: NULL handle");
+    PathTracer& P = pt->P;
+    if (emitters && count && count != P.material_count)
+        return fail(ATLAS_RT_E_SIZE_MISMATCH,
+                    "atlas_rt_pathtracer_set_emissive_lights: one emitter per material table entry expected");
+    ATLAS_HIP_TRY(hipDeviceSynchronize());
+    build_light_list(P, count ? emitters : nullptr, count);
+    return upload_lights(P);
+}
+
+atlas_rt_status atlas_rt_pathtracer_lights(const atlas_rt_pathtracer* pt, atlas_rt_gpu_light* out, uint32_t cap,
+                                           uint32_t* count) {
+    if (!pt || !count || (cap && !out)) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_lights: NULL argument");
+    const auto& L = pt->P.light_list;
+    *count = uint32_t(L.size());
+    std::memcpy(out, L.data(), std::min<size_t>(cap, L.size()) * sizeof(Light));
     return ATLAS_RT_OK;
 }
 

@@ -13,10 +13,11 @@ namespace pt {
 struct ShadeArgs {
     WorldDev world;
     const float4* xform;          // object-to-world rows, 3 float4 per packed instance
-    const float4* const* shade;   // per BLAS: 3 float4 per slot (n0 + material, n1, n2)
+    const float4* const* shade;   // per BLAS: 3 float4 per slot (n0 + material, n1 + opacity, n2)
     const Material* materials;
     const Light* lights;
     int32_t light_count;
+    AlphaArgs alpha;
     int32_t bounces;
     float env[3];
     FrameCtl* ctl;
@@ -44,8 +45,17 @@ struct PathTracer {
     float4* accum = nullptr;
     float4* xform = nullptr;
     const float4** shade_table = nullptr;
+    const float4** uv_table = nullptr;
     Material* materials = nullptr;
+    uint32_t material_count = 0;
     Light* lights = nullptr;
+    uint32_t light_cap = 0;
+    bool alpha = false;                     // some BLAS has non-opaque triangles
+    std::vector<atlas_rt_light> punctual;   // the lights given at creation
+    std::vector<Light> light_list;          // host copy of the device list
+    TexDesc* tex = nullptr;
+    uint8_t* texels = nullptr;
+    uint32_t tex_count = 0;
     Scratch spill;
     Scratch flags;
     ShadeArgs shade_args{};
@@ -63,6 +73,9 @@ struct PathTracer {
 };
 
 atlas_rt_status launch_frame(PathTracer& P, const Camera& cam, hipStream_t s);
+// Builds the light list (UpdateLights over the punctual lights and, with
+// emitters, the triangle lights) into P.light_list.
+void build_light_list(PathTracer& P, const float* emitters, uint32_t emitter_count);
 void free_cache_buffers(PathTracer& P);
 
 }  // namespace pt

@@ -7,19 +7,20 @@
 // wavefront kernels so that traversal and shading do not share registers:
 //   frame_begin  per-bounce seeds from the device-side PCG32 state, counters
 //   raygen       camera rays (tile-ordered, rayGen.csh:25-90)
-//   trace        closest hit of every live path (HitClosest, engine order)
+//   trace        closest hit of every live path (HitClosestTransparency,
+//                engine order; HitClosest when every triangle is opaque)
 //   shade        EvaluateBounce (rayHit.csh:160-325): env on miss, surface
-//                reconstruction, emission, the sun's contribution (queued as
-//                a shadow ray with its already-clamped radiance), BSDF
-//                sample, Russian roulette, stream compaction of survivors
-//   shadow       HitAnyTransparency-style any hit; visible -> add radiance
+//                reconstruction, emission, one light's contribution (queued
+//                as a shadow ray), BSDF sample, Russian roulette, stream
+//                compaction of survivors
+//   shadow       HitAnyTransparency (HitAny with the world ray restored when
+//                every triangle is opaque); adds radiance * transparency,
+//                clamped at bounces > 0 as EvaluateBounce does
 //   accumulate   per-pixel running sum (rayHit.csh:119-132, linear HDR)
-// With opaque materials and visibility in {0, 1} the deferred shadow test
-// adds exactly what the inline one would: the bounce clamp scales the
-// unoccluded radiance, which is then multiplied by visibility.
 // Every count lives on the device, so a whole frame can be graph-captured.
 #include <hip/hip_runtime.h>
 
+#include "alpha.h"
 #include "nrc.h"
 #include "pathtracer.h"
 #include "pt_device.h"
@@ -31,6 +32,7 @@ namespace pt {
 namespace {
 
 constexpr int kBlock = 128;
+constexpr int kShadeBlock = 256;  // raygen and shade: one compaction atomic per counter per block
 constexpr int kLdsDepth = 32;
 
 __device__ __forceinline__ uint32_t pcg32_next(uint64_t& state) {
@@ -54,17 +56,33 @@ __global__ void k_frame_begin(FrameCtl* ctl, int32_t bounces, uint32_t pixels) {
     ctl->train_samples = 0;
 }
 
-// Stream compaction slot: one atomic per wave for the active lanes that
-// want a slot (ballot + popcount); lanes keep their order within the wave.
-__device__ __forceinline__ uint32_t wave_alloc(uint32_t* counter, bool want) {
-    const uint64_t m = __ballot(want);
-    if (m == 0) return 0;
-    const uint32_t lane = threadIdx.x & 63u;
-    const int leader = __ffsll((unsigned long long)m) - 1;
-    uint32_t base = 0;
-    if (lane == uint32_t(leader)) base = atomicAdd(counter, uint32_t(__popcll(m)));
-    base = __shfl(base, leader, 64);
-    return base + uint32_t(__popcll(m & ((1ull << lane) - 1ull)));
+// Stream compaction for a whole block: one LDS atomic per wave and one
+// global atomic per counter per block.  Every thread of the block must call
+// it (threads with nothing to write pass want = false).
+template <int N>
+__device__ __forceinline__ void block_alloc(uint32_t* const (&ctr)[N], const bool (&want)[N], uint32_t (&slot)[N]) {
+    __shared__ uint32_t s_cnt[N], s_base[N];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u;
+    if (tid < uint32_t(N)) s_cnt[tid] = 0;
+    __syncthreads();
+    uint32_t woff[N], below[N];
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        const uint64_t m = __ballot(want[k]);
+        below[k] = uint32_t(__popcll(m & ((1ull << lane) - 1ull)));
+        woff[k] = 0;
+        if (m) {
+            const int leader = __ffsll((unsigned long long)m) - 1;
+            uint32_t off = 0;
+            if (lane == uint32_t(leader)) off = atomicAdd(&s_cnt[k], uint32_t(__popcll(m)));
+            woff[k] = __shfl(off, leader, 64);
+        }
+    }
+    __syncthreads();
+    if (tid < uint32_t(N) && s_cnt[tid]) s_base[tid] = atomicAdd(ctr[tid], s_cnt[tid]);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < N; k++) slot[k] = want[k] ? s_base[k] + woff[k] + below[k] : 0u;
 }
 
 // Integer hash (murmur3 finaliser) choosing the frame's training pixels.
@@ -79,11 +97,19 @@ __device__ __forceinline__ uint32_t mix32(uint32_t h) {
 
 // rayGen.csh:25-90 with one sample per frame (z = 0): jittered pinhole ray
 // through the near-plane corner frame; the write index groups 8x8 tiles.
-__global__ __launch_bounds__(kBlock) void k_raygen(Camera cam, uint32_t width, uint32_t height,
-                                                   FrameCtl* __restrict__ ctl, float4* __restrict__ paths,
-                                                   float4* __restrict__ Lpix, CacheArgs C, int32_t bounces) {
-    const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
-    if (g >= width * height) return;
+__global__ __launch_bounds__(kShadeBlock) void k_raygen(Camera cam, uint32_t width, uint32_t height,
+                                                        FrameCtl* __restrict__ ctl, float4* __restrict__ paths,
+                                                        float4* __restrict__ Lpix, CacheArgs C, int32_t bounces) {
+    const uint32_t g0 = blockIdx.x * kShadeBlock + threadIdx.x;
+    const bool valid = g0 < width * height;
+    // radiance cache: about one pixel in train_stride traces a training path
+    uint32_t* const ctr[1] = {&ctl->train_paths};
+    const bool pick[1] = {valid && C.enabled &&
+                          mix32(g0 * 0x9e3779b1u + ctl->frame * 0x632be5abu) % C.train_stride == 0};
+    uint32_t t[1];
+    if (C.enabled) block_alloc<1>(ctr, pick, t);
+    if (!valid) return;
+    const uint32_t g = g0;
     const uint32_t px = g % width, py = g / width;
     const float sc = float(ctl->sample_count);
     const float jx = rnd_pair(sc, 0.0f);
@@ -103,15 +129,12 @@ __global__ __launch_bounds__(kBlock) void k_raygen(Camera cam, uint32_t width, u
     // ragged right / bottom tiles are packed behind the full ones
     if ((width % 8) || (height % 8)) idx = g;
     const uint32_t pixel = py * width + px;
-    // radiance cache: about one pixel in train_stride traces a training path
     uint32_t train = 0;
-    const bool pick = C.enabled && mix32(pixel * 0x9e3779b1u + ctl->frame * 0x632be5abu) % C.train_stride == 0;
     if (C.enabled) {
-        const uint32_t t = wave_alloc(&ctl->train_paths, pick);
-        if (pick && t < C.train_cap) {
-            train = t + 1;
+        if (pick[0] && t[0] < C.train_cap) {
+            train = t[0] + 1;
             for (int32_t k = 0; k <= bounces; k++) {
-                const size_t r = size_t(t) * (bounces + 1) + k;
+                const size_t r = size_t(t[0]) * (bounces + 1) + k;
                 C.rec_beta[r] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 C.rec_add[r] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             }
@@ -124,11 +147,12 @@ __global__ __launch_bounds__(kBlock) void k_raygen(Camera cam, uint32_t width, u
 }
 
 // Closest hit of live paths: hit = (engine float32 t, slot, instance slot).
-template <int BLOCK, int LDS_DEPTH>
+// ALPHA: HitClosestTransparency (candidates need opacity > 0).
+template <int BLOCK, int LDS_DEPTH, bool ALPHA>
 __global__ __launch_bounds__(BLOCK) void k_trace(WorldDev W, const uint32_t* __restrict__ count_ptr,
                                                  const float4* __restrict__ paths, int4* __restrict__ hits,
                                                  int32_t* __restrict__ spill, uint32_t cap,
-                                                 uint32_t* __restrict__ flags) {
+                                                 uint32_t* __restrict__ flags, AlphaArgs A) {
     __shared__ int32_t lds[LDS_DEPTH * BLOCK];
     const uint32_t gid = blockIdx.x * BLOCK + threadIdx.x;
     if (gid >= *count_ptr) return;
@@ -140,9 +164,16 @@ __global__ __launch_bounds__(BLOCK) void k_trace(WorldDev W, const uint32_t* __r
     if (!any_nan3(p1.x, p1.y, p1.z)) {
         const LaneStack<BLOCK, LDS_DEPTH> S{lds, spill, threadIdx.x, gid, gridDim.x * BLOCK};
         int r;
-        do {
-            r = world_step_closest<BLOCK, LDS_DEPTH>(L, W, ATLAS_RT_MASK_ALL, S, cap);
-        } while (r == kContinue);
+        if (ALPHA) {
+            const AcceptOpaque acc{TriOpacity{W, A}};
+            do {
+                r = world_step_closest<BLOCK, LDS_DEPTH>(L, W, ATLAS_RT_MASK_ALL, S, cap, acc);
+            } while (r == kContinue);
+        } else {
+            do {
+                r = world_step_closest<BLOCK, LDS_DEPTH>(L, W, ATLAS_RT_MASK_ALL, S, cap);
+            } while (r == kContinue);
+        }
         if (r == kOverflow)
             atomicOr(&flags[2], 1u);
         else if (L.slot >= 0) {
@@ -160,10 +191,16 @@ __device__ __forceinline__ V row_xform(const float4* m, V p, float w) {
               (m[2].x * p.x + m[2].y * p.y) + (m[2].z * p.z + m[2].w * w));
 }
 
-__global__ __launch_bounds__(kBlock) void k_shade(ShadeArgs A, int32_t bounce) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+// Records a shaded path hands to the stream compaction of k_shade.
+struct ShadeOut {
+    bool query = false, shadow = false, cont = false;
+    float4 q[4];   // cache query (3 float4) + (throughput, pixel)
+    float4 sh[3];  // shadow ray
+    float4 pa[3];  // continued path
+};
+
+__device__ __forceinline__ void shade_one(const ShadeArgs& A, int32_t bounce, uint32_t i, ShadeOut& out) {
     FrameCtl* ctl = A.ctl;
-    if (i >= ctl->paths[bounce]) return;
     const float4 p0 = A.paths_in[3 * size_t(i)];
     const float4 p1 = A.paths_in[3 * size_t(i) + 1];
     const float4 p2 = A.paths_in[3 * size_t(i) + 2];
@@ -200,7 +237,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(ShadeArgs A, int32_t bounce) {
         const V n0 = normalize(row_xform(M, mk(s0.x, s0.y, s0.z), 0.0f));
         const V n1 = normalize(row_xform(M, mk(s1.x, s1.y, s1.z), 0.0f));
         const V n2 = normalize(row_xform(M, mk(s2.x, s2.y, s2.z), 0.0f));
-        const Material m = A.materials[__float_as_int(s0.w) + mat_off];
+        Material m = A.materials[__float_as_int(s0.w) + mat_off];
         // re-intersect for the barycentrics (surface.hsh:72-78)
         const V e0 = v1 - v0, e1 = v2 - v0, s = ro - v0;
         const V p = cross(s, e0), q = cross(rd, e1);
@@ -209,6 +246,12 @@ __global__ __launch_bounds__(kBlock) void k_shade(ShadeArgs A, int32_t bounce) {
         const float bs = dot(q, s) / den;
         const float bt = dot(p, rd) / den;
         const float br = 1.0f - bs - bt;
+        if (m.opacity_texture >= 0 && uint32_t(m.opacity_texture) < A.alpha.tex_count) {
+            // surface.hsh:86-105: texture coordinate from these barycentrics
+            float2 tc = uv_at(A.alpha.uv[mesh], slot, bs, bt);
+            if (m.invert_uvs > 0) tc.y = 1.0f - tc.y;
+            m.opacity *= sample_opacity(A.alpha, m.opacity_texture, tc.x, tc.y);
+        }
         Surface S;
         S.m = m;
         S.P = ro + rd * dist;
@@ -231,11 +274,11 @@ __global__ __launch_bounds__(kBlock) void k_shade(ShadeArgs A, int32_t bounce) {
                 A.cache.rec_beta[rec] = make_float4(thr_in.x, thr_in.y, thr_in.z, 1.0f);
             } else if (bounce == 1) {
                 // the render path ends here with the cache's radiance estimate
-                const uint32_t k = wave_alloc(&ctl->queries, true);
-                A.cache.q_buf[3 * size_t(k)] = q0;
-                A.cache.q_buf[3 * size_t(k) + 1] = q1;
-                A.cache.q_buf[3 * size_t(k) + 2] = q2;
-                A.cache.q_meta[k] = make_float4(thr_in.x, thr_in.y, thr_in.z, __uint_as_float(pixel));
+                out.query = true;
+                out.q[0] = q0;
+                out.q[1] = q1;
+                out.q[2] = q2;
+                out.q[3] = make_float4(thr_in.x, thr_in.y, thr_in.z, __uint_as_float(pixel));
                 return;
             }
         }
@@ -246,17 +289,30 @@ __global__ __launch_bounds__(kBlock) void k_shade(ShadeArgs A, int32_t bounce) {
         S.NdotL = 0.0f;
         if ((m.emissive[0] + m.emissive[1]) + m.emissive[2] > 0.0f && bounce == 0)
             add = add + mk(m.emissive[0], m.emissive[1], m.emissive[2]);
-        // EvaluateDirectLight (rayHit.csh:209-235): one light per bounce
+        // EvaluateDirectLight (rayHit.csh:209-235): GetLight / SampleLight
+        // (raytracer/direct.hsh:12-86).  The light's seed is the pixel index
+        // (the engine uses the invocation index, which depends on the order
+        // of the compaction atomics).
         if (A.light_count > 0) {
             float seed = ctl->seeds[bounce];
-            const float ray_seed = float(i);
+            const float ray_seed = float(pixel);
             float total = 0.0f, lpdf = 1.0f;
             int32_t sel = 0;
             for (int32_t l = 0; l < A.light_count; l++) {
                 const Light& Lt = A.lights[l];
-                const float brightness = dot(mk(Lt.radiance[0], Lt.radiance[1], Lt.radiance[2]),
+                const float brightness = dot(mk(Lt.color[0], Lt.color[1], Lt.color[2]),
                                              mk(0.33333f, 0.33333f, 0.33333f));
-                float w = clampf(brightness, 0.0000001f, 1.0f);
+                float w;
+                if ((__float_as_uint(Lt.data[0]) >> 28) == ATLAS_RT_LIGHT_TRIANGLE) {
+                    const V pl = mk(Lt.P[0], Lt.P[1], Lt.P[2]) - S.P;
+                    const float sq = dot(pl, pl);
+                    const V Ld = normalize(pl);
+                    const float ndl = fmaxf(dot(mk(Lt.N[0], Lt.N[1], Lt.N[2]), -Ld), 0.0f);
+                    w = ((brightness * Lt.data[2]) * ndl) / sq;
+                } else {
+                    w = brightness;
+                }
+                w = clampf(w, 0.0000001f, 1.0f);
                 total += w;
                 if (rnd2(ray_seed, seed) < w / total) {
                     lpdf = w;
@@ -264,23 +320,52 @@ __global__ __launch_bounds__(kBlock) void k_shade(ShadeArgs A, int32_t bounce) {
                 }
             }
             const Light& Lt = A.lights[sel];
-            lpdf = lpdf / total * Lt.pdf;
-            S.L = normalize(-mk(Lt.N[0], Lt.N[1], Lt.N[2]));
-            update_surface(S);
+            lpdf = lpdf / total * Lt.data[1];
+            float solid = 1.0f, ldist = kInf;
+            if ((__float_as_uint(Lt.data[0]) >> 28) == ATLAS_RT_LIGHT_TRIANGLE) {
+                const float r0 = rnd2(ray_seed, seed), r1 = rnd2(ray_seed, seed);
+                const int32_t li = __float_as_int(Lt.data[3]);
+                const int32_t lslot = int32_t(__float_as_uint(Lt.data[0]) & 0x0FFFFFFFu);
+                const int32_t lmesh = __float_as_int(A.world.instances[4 * size_t(li) + 3].x);
+                const float4* LM = A.xform + 3 * size_t(li);
+                const float4* lv = A.world.blases[lmesh].vtx + 3 * size_t(lslot);
+                const float4* ls = A.shade[lmesh] + 3 * size_t(lslot);
+                const float4 b0 = lv[0], b1 = lv[1], b2 = lv[2];
+                const float4 c0 = ls[0], c1 = ls[1], c2 = ls[2];
+                // TransformTriangle + SampleTriangleLight (raytracer/lights.hsh:11-31)
+                const V a0 = row_xform(LM, mk(b0.x, b0.y, b0.z), 1.0f);
+                const V a1 = row_xform(LM, mk(b1.x, b1.y, b1.z), 1.0f);
+                const V a2 = row_xform(LM, mk(b2.x, b2.y, b2.z), 1.0f);
+                const V m0 = normalize(row_xform(LM, mk(c0.x, c0.y, c0.z), 0.0f));
+                const V m1 = normalize(row_xform(LM, mk(c1.x, c1.y, c1.z), 0.0f));
+                const V m2 = normalize(row_xform(LM, mk(c2.x, c2.y, c2.z), 0.0f));
+                const float sr = sqrtf(r0);
+                const float bu = 1.0f - sr, bv = r1 * sr, bw = (1.0f - bu) - bv;
+                const V LP = (a0 * bu + a1 * bv) + a2 * bw;
+                const V LN = (m0 * bu + m1 * bv) + m2 * bw;
+                const float spdf = 1.0f / Lt.data[2];
+                const V pl = LP - S.P;
+                const float sq = dot(pl, pl);
+                ldist = sqrtf(sq);
+                S.L = mk(pl.x / ldist, pl.y / ldist, pl.z / ldist);
+                update_surface(S);
+                const float ndl = fmaxf(dot(LN, -S.L), 0.0f);
+                solid = ndl / (sq * spdf);
+            } else {
+                S.L = normalize(-mk(Lt.N[0], Lt.N[1], Lt.N[2]));
+                update_surface(S);
+            }
             V refl = (eval_diffuse(S) + eval_specular(S)) * m.opacity;
-            const V rad = mk(Lt.radiance[0], Lt.radiance[1], Lt.radiance[2]);
-            const uint32_t k = wave_alloc(&ctl->shadows[bounce], S.NdotL > 0.0f);
+            const V rad = mk(Lt.color[0], Lt.color[1], Lt.color[2]) * solid;
             if (S.NdotL > 0.0f) {
-                V direct = (refl * rad) * S.NdotL / lpdf;
-                V c = thr * m.opacity * direct;
-                if (bounce > 0) {
-                    const float rm = fmaxf(fmaxf(c.x, fmaxf(c.y, c.z)), 10.0f);
-                    c = c * (10.0f / rm);
-                }
+                // visibility and the bounce > 0 clamp are applied by k_shadow
+                const V direct = (refl * rad) * S.NdotL / lpdf;
+                const V c = thr * m.opacity * direct;
                 const V so = S.P + S.N * kEpsilon;
-                A.shadow[3 * size_t(k)] = make_float4(so.x, so.y, so.z, __uint_as_float(pixel));
-                A.shadow[3 * size_t(k) + 1] = make_float4(S.L.x, S.L.y, S.L.z, kInf - 2.0f * kEpsilon);
-                A.shadow[3 * size_t(k) + 2] = make_float4(c.x, c.y, c.z, __uint_as_float(train ? uint32_t(rec) + 1u : 0u));
+                out.shadow = true;
+                out.sh[0] = make_float4(so.x, so.y, so.z, __uint_as_float(pixel));
+                out.sh[1] = make_float4(S.L.x, S.L.y, S.L.z, ldist - 2.0f * kEpsilon);
+                out.sh[2] = make_float4(c.x, c.y, c.z, __uint_as_float(train ? uint32_t(rec) + 1u : 0u));
             }
         }
         // EvaluateIndirectLight (rayHit.csh:237-325)
@@ -338,21 +423,48 @@ __global__ __launch_bounds__(kBlock) void k_shade(ShadeArgs A, int32_t bounce) {
     A.Lpix[pixel] = acc;
     if (train) A.cache.rec_add[rec] = make_float4(add.x, add.y, add.z, 0.0f);
     const float energy = (thr.x + thr.y) + thr.z;
-    const bool more = !(energy == 0.0f || bounce == A.bounces);
-    const uint32_t k = wave_alloc(&ctl->paths[bounce + 1], more);
-    if (!more) return;
-    A.paths_out[3 * size_t(k)] = make_float4(no.x, no.y, no.z, p0.w);
-    A.paths_out[3 * size_t(k) + 1] = make_float4(nd.x, nd.y, nd.z, p1.w);
-    A.paths_out[3 * size_t(k) + 2] = make_float4(thr.x, thr.y, thr.z, 0.0f);
+    if (energy == 0.0f || bounce == A.bounces) return;
+    out.cont = true;
+    out.pa[0] = make_float4(no.x, no.y, no.z, p0.w);
+    out.pa[1] = make_float4(nd.x, nd.y, nd.z, p1.w);
+    out.pa[2] = make_float4(thr.x, thr.y, thr.z, 0.0f);
+}
+
+__global__ __launch_bounds__(kShadeBlock) void k_shade(ShadeArgs A, int32_t bounce) {
+    const uint32_t i = blockIdx.x * kShadeBlock + threadIdx.x;
+    FrameCtl* ctl = A.ctl;
+    ShadeOut o;
+    if (i < ctl->paths[bounce]) shade_one(A, bounce, i, o);
+    uint32_t* const ctr[3] = {&ctl->queries, &ctl->shadows[bounce], &ctl->paths[bounce + 1]};
+    const bool want[3] = {o.query, o.shadow, o.cont};
+    uint32_t k[3];
+    block_alloc<3>(ctr, want, k);
+    if (o.query) {
+        A.cache.q_buf[3 * size_t(k[0])] = o.q[0];
+        A.cache.q_buf[3 * size_t(k[0]) + 1] = o.q[1];
+        A.cache.q_buf[3 * size_t(k[0]) + 2] = o.q[2];
+        A.cache.q_meta[k[0]] = o.q[3];
+    }
+    if (o.shadow) {
+        A.shadow[3 * size_t(k[1])] = o.sh[0];
+        A.shadow[3 * size_t(k[1]) + 1] = o.sh[1];
+        A.shadow[3 * size_t(k[1]) + 2] = o.sh[2];
+    }
+    if (o.cont) {
+        A.paths_out[3 * size_t(k[2])] = o.pa[0];
+        A.paths_out[3 * size_t(k[2]) + 1] = o.pa[1];
+        A.paths_out[3 * size_t(k[2]) + 2] = o.pa[2];
+    }
 }
 
 // Shadow rays: HitAnyTransparency's traversal (restore every TLAS step),
 // INSTANCE_MASK_SHADOW, tmin 0 (rayHit.csh:327-339).
-template <int BLOCK, int LDS_DEPTH>
+template <int BLOCK, int LDS_DEPTH, bool ALPHA>
 __global__ __launch_bounds__(BLOCK) void k_shadow(WorldDev W, const FrameCtl* __restrict__ ctl, int32_t bounce,
                                                   const float4* __restrict__ shadow, float4* __restrict__ Lpix,
                                                   int32_t* __restrict__ spill, uint32_t cap,
-                                                  uint32_t* __restrict__ flags, float4* __restrict__ rec_add) {
+                                                  uint32_t* __restrict__ flags, float4* __restrict__ rec_add,
+                                                  AlphaArgs A) {
     __shared__ int32_t lds[LDS_DEPTH * BLOCK];
     const uint32_t gid = blockIdx.x * BLOCK + threadIdx.x;
     if (gid >= ctl->shadows[bounce]) return;
@@ -363,26 +475,42 @@ __global__ __launch_bounds__(BLOCK) void k_shadow(WorldDev W, const FrameCtl* __
     world_begin(L, make_float4(s0.x, s0.y, s0.z, 0.0f), make_float4(s1.x, s1.y, s1.z, s1.w));
     const LaneStack<BLOCK, LDS_DEPTH> S{lds, spill, threadIdx.x, gid, gridDim.x * BLOCK};
     int r;
-    do {
-        r = world_step_any<BLOCK, LDS_DEPTH, true>(L, W, ATLAS_RT_MASK_SHADOW, s1.w, S, cap);
-    } while (r == kContinue);
+    float vis;
+    if (ALPHA) {
+        const TriOpacity op{W, A};
+        do {
+            r = world_step_transparency<BLOCK, LDS_DEPTH>(L, W, ATLAS_RT_MASK_SHADOW, s1.w, S, cap, op);
+        } while (r == kContinue);
+        vis = L.trans;
+    } else {
+        do {
+            r = world_step_any<BLOCK, LDS_DEPTH, true>(L, W, ATLAS_RT_MASK_SHADOW, s1.w, S, cap);
+        } while (r == kContinue);
+        vis = L.slot >= 0 ? 0.0f : 1.0f;
+    }
     if (r == kOverflow) {
         atomicOr(&flags[2], 1u);
         return;
     }
-    if (L.slot >= 0) return;  // occluded
+    if (vis == 0.0f) return;  // occluded
+    V cc = mk(c.x, c.y, c.z) * vis;
+    if (bounce > 0) {
+        // EvaluateBounce's clamp of the direct radiance (rayHit.csh:196-201)
+        const float rm = fmaxf(fmaxf(cc.x, fmaxf(cc.y, cc.z)), 10.0f);
+        cc = cc * (10.0f / rm);
+    }
     const uint32_t pixel = __float_as_uint(s0.w);
     float4 acc = Lpix[pixel];
-    acc.x += c.x;
-    acc.y += c.y;
-    acc.z += c.z;
+    acc.x += cc.x;
+    acc.y += cc.y;
+    acc.z += cc.z;
     Lpix[pixel] = acc;
     const uint32_t rec = __float_as_uint(c.w);
     if (rec) {
         float4 a = rec_add[rec - 1];
-        a.x += c.x;
-        a.y += c.y;
-        a.z += c.z;
+        a.x += cc.x;
+        a.y += cc.y;
+        a.z += cc.z;
         rec_add[rec - 1] = a;
     }
 }
@@ -410,6 +538,10 @@ __global__ __launch_bounds__(kBlock) void k_build_targets(FrameCtl* __restrict__
                                                           float4* __restrict__ s_q, float4* __restrict__ s_t) {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= min(ctl->train_paths, C.train_cap)) return;
+    uint32_t n = 0;
+    for (int32_t k = 0; k <= bounces; k++) n += C.rec_beta[size_t(t) * (bounces + 1) + k].w != 0.0f;
+    if (n == 0) return;
+    uint32_t si = atomicAdd(&ctl->train_samples, n);  // one reservation per training path
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
     for (int32_t k = bounces; k >= 0; k--) {
         const size_t r = size_t(t) * (bounces + 1) + k;
@@ -419,12 +551,12 @@ __global__ __launch_bounds__(kBlock) void k_build_targets(FrameCtl* __restrict__
         sz += a.z;
         const float4 b = C.rec_beta[r];
         if (b.w == 0.0f) continue;
-        const uint32_t si = atomicAdd(&ctl->train_samples, 1u);  // one per vertex of ~1/stride paths
         s_q[3 * size_t(si)] = C.rec_q[3 * r];
         s_q[3 * size_t(si) + 1] = C.rec_q[3 * r + 1];
         s_q[3 * size_t(si) + 2] = C.rec_q[3 * r + 2];
         s_t[si] = make_float4(b.x > 0.0f ? sx / b.x : 0.0f, b.y > 0.0f ? sy / b.y : 0.0f,
                               b.z > 0.0f ? sz / b.z : 0.0f, 0.0f);
+        si++;
     }
 }
 
@@ -454,22 +586,32 @@ atlas_rt_status launch_frame(PathTracer& P, const Camera& cam, hipStream_t s) {
     uint32_t* flags = reinterpret_cast<uint32_t*>(P.flags.ptr);
     const CacheArgs& C = P.shade_args.cache;
     hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(64), 0, s, P.ctl, P.bounces, pixels);
-    hipLaunchKernelGGL(k_raygen, dim3(grid_all), dim3(kBlock), 0, s, cam, P.width, P.height, P.ctl, P.paths[0], P.Lpix,
-                       C, P.bounces);
+    hipLaunchKernelGGL(k_raygen, dim3((pixels + kShadeBlock - 1) / kShadeBlock), dim3(kShadeBlock), 0, s, cam, P.width,
+                       P.height, P.ctl, P.paths[0], P.Lpix, C, P.bounces);
     for (int32_t b = 0; b <= P.bounces; b++) {
         float4* in = P.paths[b & 1];
         float4* out = P.paths[(b + 1) & 1];
         // with the cache only training paths outlive bounce 1
         const uint32_t grid = (C.enabled && b >= 2) ? (P.train_cap + kBlock - 1) / kBlock : grid_all;
-        hipLaunchKernelGGL((k_trace<kBlock, kLdsDepth>), dim3(grid), dim3(kBlock), 0, s, P.world_dev,
-                           &P.ctl->paths[b], in, P.hits, reinterpret_cast<int32_t*>(P.spill.ptr), P.cap, flags);
+        int32_t* spill = reinterpret_cast<int32_t*>(P.spill.ptr);
+        const AlphaArgs& AA = P.shade_args.alpha;
+        if (P.alpha)
+            hipLaunchKernelGGL((k_trace<kBlock, kLdsDepth, true>), dim3(grid), dim3(kBlock), 0, s, P.world_dev,
+                               &P.ctl->paths[b], in, P.hits, spill, P.cap, flags, AA);
+        else
+            hipLaunchKernelGGL((k_trace<kBlock, kLdsDepth, false>), dim3(grid), dim3(kBlock), 0, s, P.world_dev,
+                               &P.ctl->paths[b], in, P.hits, spill, P.cap, flags, AA);
         ShadeArgs A = P.shade_args;
         A.paths_in = in;
         A.paths_out = out;
-        hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kBlock), 0, s, A, b);
-        if (P.light_count > 0)
-            hipLaunchKernelGGL((k_shadow<kBlock, kLdsDepth>), dim3(grid), dim3(kBlock), 0, s, P.world_dev, P.ctl, b,
-                               P.shadow, P.Lpix, reinterpret_cast<int32_t*>(P.spill.ptr), P.cap, flags, C.rec_add);
+        const uint32_t shade_grid = (grid * kBlock + kShadeBlock - 1) / kShadeBlock;
+        hipLaunchKernelGGL(k_shade, dim3(shade_grid), dim3(kShadeBlock), 0, s, A, b);
+        if (P.light_count > 0 && P.alpha)
+            hipLaunchKernelGGL((k_shadow<kBlock, kLdsDepth, true>), dim3(grid), dim3(kBlock), 0, s, P.world_dev,
+                               P.ctl, b, P.shadow, P.Lpix, spill, P.cap, flags, C.rec_add, AA);
+        else if (P.light_count > 0)
+            hipLaunchKernelGGL((k_shadow<kBlock, kLdsDepth, false>), dim3(grid), dim3(kBlock), 0, s, P.world_dev,
+                               P.ctl, b, P.shadow, P.Lpix, spill, P.cap, flags, C.rec_add, AA);
         if (C.enabled && b == 1) {
             ATLAS_HIP_TRY(nrc::launch_infer(P.nrc->dev, P.q_buf, P.q_out, &P.ctl->queries, pixels, s));
             hipLaunchKernelGGL(k_cache_apply, dim3(grid_all), dim3(kBlock), 0, s, P.ctl, P.q_meta, P.q_out, P.Lpix);

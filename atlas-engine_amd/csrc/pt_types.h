@@ -7,8 +7,8 @@
 namespace atlas_rt {
 namespace pt {
 
-// atlas_rt_material (48 B): the RaytraceMaterial fields the bounce shader
-// reads (raytracer/surface.hsh:22-44).
+// atlas_rt_material (64 B): the RaytraceMaterial fields the bounce shader
+// reads (raytracer/structures.hsh:107-135, surface.hsh:63-140).
 struct Material {
     float base[3];
     float opacity;
@@ -18,15 +18,39 @@ struct Material {
     float ao;
     float reflectance;
     int32_t two_sided;
+    int32_t opacity_texture;  // index into the texture table; outside it = none
+    int32_t invert_uvs;
+    int32_t reserved[2];
 };
 
-// Unpacked GPULight (raytracer/common.hsh:UnpackLight): N = direction the
-// light travels, pdf = normalised selection weight (UpdateLights).
+// atlas_rt_gpu_light / GPULight (64 B), read as UnpackLight does
+// (raytracer/common.hsh:100-121): data[0] = bits(type << 28 | slot),
+// data[1] = normalised selection weight, data[2] = area, data[3] = bits(instance).
 struct Light {
-    float N[3];
-    uint32_t type;
-    float radiance[3];
-    float pdf;
+    float P[4];
+    float N[4];
+    float color[4];
+    float data[4];
+};
+
+// One opacity texture of the path tracer's table: R8 texels at `offset`.
+struct TexDesc {
+    uint32_t offset;
+    uint32_t width;
+    uint32_t height;
+    uint32_t pad;
+};
+
+// What a kernel needs to evaluate a triangle's opacity.  materials = null:
+// opacity-map slots read as 1 (the world-level traces hold no material table).
+struct AlphaArgs {
+    const float4* const* shade;  // per BLAS: 3 float4 per slot (n0 + material, n1 + opacity, n2)
+    const float4* const* uv;     // per BLAS: 2 float4 per slot, or null
+    const Material* materials;
+    const TexDesc* tex;
+    const uint8_t* texels;
+    uint32_t tex_count;
+    uint32_t pad;
 };
 
 struct Camera {

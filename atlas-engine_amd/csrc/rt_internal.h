@@ -76,8 +76,13 @@ struct atlas_rt_blas {
     float4* d_tris = nullptr;
     float4* d_vtx = nullptr;
     //  d_shade: optional, 3 x float4 per slot: (n0, material index bits),
-    //           (n1, 0), (n2, 0) (the path tracer's shading data).
+    //           (n1, baked opacity), (n2, 0) (the path tracer's shading data);
+    //  d_uv   : optional, 2 x float4 per slot: (uv0, uv1), (uv2, 0, 0).
     float4* d_shade = nullptr;
+    float4* d_uv = nullptr;
+    bool has_alpha = false;             // some baked opacity is not 1
+    std::vector<float> shade_normals;   // host copy, 9 floats per slot
+    std::vector<int32_t> shade_mat;     // host copy, 1 per slot
     //  d_nodes4 / d_nodes8: wide nodes (wide.h); cap4 / cap8 bound the
     //  traversal stack of the wide kernels.
     float4* d_nodes4 = nullptr;

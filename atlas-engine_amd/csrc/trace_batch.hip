@@ -172,7 +172,7 @@ atlas_rt_status launch_one(atlas_rt_batch_set& s, const BatchLaunch& L) {
     constexpr int D = Rows<BLOCK>::value;
     uint32_t threads, grid;
     if (L.persistent) {
-        grid = uint32_t(device_cu_count()) * (2048u / BLOCK);
+        grid = uint32_t(device_cu_count()) * (1024u / BLOCK);  // 16 resident waves per CU
         threads = grid * BLOCK;
     } else {
         grid = (s.total + BLOCK - 1) / BLOCK;

@@ -12,6 +12,7 @@
 // are the same engine-exact arithmetic as the single-BLAS kernels.
 #include <hip/hip_runtime.h>
 
+#include "alpha.h"
 #include "rt_internal.h"
 #include "trace_world_core.h"
 
@@ -133,6 +134,71 @@ __global__ __launch_bounds__(BLOCK) void k_world_closest_packed(WorldDev W, cons
     out[3 * size_t(gid) + 2] = make_float4(hitDistance, __int_as_float(hitID), __int_as_float(hitInstance), p2.w);
 }
 
+// Opacity-aware PackedRay traces: out_rays != null runs
+// HitClosestTransparency (written like k_world_closest_packed), otherwise
+// HitAnyTransparency writes the transparency left along each ray.
+template <int BLOCK, int LDS_DEPTH>
+__global__ __launch_bounds__(BLOCK) void k_world_alpha_packed(WorldDev W, pt::AlphaArgs A, const float4* __restrict__ in,
+                                                              float4* __restrict__ out_rays,
+                                                              float* __restrict__ out_trans,
+                                                              const uint32_t* __restrict__ d_count, uint32_t max_count,
+                                                              float tmin, float tmax, uint32_t mask,
+                                                              int32_t* __restrict__ spill, uint32_t cap,
+                                                              uint32_t* __restrict__ ctl) {
+    __shared__ int32_t lds[LDS_DEPTH * BLOCK];
+    const uint32_t gid = blockIdx.x * BLOCK + threadIdx.x;
+    const uint32_t count = d_count ? min(*d_count, max_count) : max_count;
+    if (gid >= count) return;
+    const float4 p0 = in[3 * size_t(gid)];
+    const float4 p1 = in[3 * size_t(gid) + 1];
+    const pt::TriOpacity op{W, A};
+    const LaneStack<BLOCK, LDS_DEPTH> S{lds, spill, threadIdx.x, gid, gridDim.x * BLOCK};
+    if (!out_rays) {
+        float trans = 1.0f;
+        if (!any_nan3(p1.x, p1.y, p1.z)) {
+            WorldLane L;
+            world_begin(L, make_float4(p0.x, p0.y, p0.z, tmin), make_float4(p1.x, p1.y, p1.z, tmax));
+            int r;
+            do {
+                r = world_step_transparency<BLOCK, LDS_DEPTH>(L, W, mask, tmax, S, cap, op);
+            } while (r == kContinue);
+            if (r == kOverflow) atomicOr(&ctl[2], 1u);
+            trans = L.trans;
+        }
+        out_trans[gid] = trans;
+        return;
+    }
+    const float4 p2 = in[3 * size_t(gid) + 2];
+    const int32_t id = __float_as_int(p0.w);
+    float hitDistance = 0.0f;
+    int32_t hitID = -1;
+    int32_t hitInstance = __float_as_int(p2.z);
+    if (id >= 0) {
+        hitDistance = tmax;
+        if (!any_nan3(p1.x, p1.y, p1.z)) {
+            WorldLane L;
+            world_begin(L, make_float4(p0.x, p0.y, p0.z, tmin), make_float4(p1.x, p1.y, p1.z, tmax));
+            const pt::AcceptOpaque acc{op};
+            int r;
+            do {
+                r = world_step_closest<BLOCK, LDS_DEPTH>(L, W, mask, S, cap, acc);
+            } while (r == kContinue);
+            if (r == kOverflow) {
+                atomicOr(&ctl[2], 1u);
+            } else {
+                hitDistance = L.hitT;
+                if (L.slot >= 0) {
+                    hitID = L.slot;
+                    hitInstance = L.inst;
+                }
+            }
+        }
+    }
+    out_rays[3 * size_t(gid)] = p0;
+    out_rays[3 * size_t(gid) + 1] = p1;
+    out_rays[3 * size_t(gid) + 2] = make_float4(hitDistance, __int_as_float(hitID), __int_as_float(hitInstance), p2.w);
+}
+
 template <int BLOCK, int LDS_DEPTH, bool RESTORE_ALWAYS>
 __global__ __launch_bounds__(BLOCK) void k_world_any_packed(WorldDev W, const float4* __restrict__ in,
                                                             uint32_t* __restrict__ out,
@@ -226,6 +292,40 @@ atlas_rt_status launch_world_closest_packed(atlas_rt_world& w, const void* in, v
     hipLaunchKernelGGL((k_world_closest_packed<kBlock, kLdsDepth>), dim3(grid), dim3(kBlock), 0, stream, w.dev(),
                        reinterpret_cast<const float4*>(in), reinterpret_cast<float4*>(out), d_count, max_count, tmin,
                        tmax, mask, reinterpret_cast<int32_t*>(w.spill.ptr), cap,
+                       reinterpret_cast<uint32_t*>(w.flags.ptr));
+    ATLAS_HIP_TRY(hipGetLastError());
+    return ATLAS_RT_OK;
+}
+
+atlas_rt_status launch_world_alpha_packed(atlas_rt_world& w, const void* in, void* out_rays, float* out_trans,
+                                          const uint32_t* d_count, uint32_t max_count, float tmin, float tmax,
+                                          uint32_t mask, hipStream_t stream) {
+    if (!max_count) return ATLAS_RT_OK;
+    // per-BLAS shading and uv pointers, refreshed when a BLAS's buffers change
+    const size_t nb = w.blas_handles.size();
+    std::vector<const void*> tab(2 * nb);
+    for (size_t k = 0; k < nb; k++) {
+        if (!w.blas_handles[k]->d_shade)
+            return fail(ATLAS_RT_E_INVALID_ARG, "opacity-aware trace: a BLAS has no shading data");
+        tab[k] = w.blas_handles[k]->d_shade;
+        tab[nb + k] = w.blas_handles[k]->d_uv;
+    }
+    if (tab != w.alpha_tab_host) {
+        atlas_rt_status st = w.alpha_tab.ensure(tab.size() * sizeof(void*));
+        if (st != ATLAS_RT_OK) return st;
+        ATLAS_HIP_TRY(hipStreamSynchronize(stream));
+        ATLAS_HIP_TRY(hipMemcpy(w.alpha_tab.ptr, tab.data(), tab.size() * sizeof(void*), hipMemcpyHostToDevice));
+        w.alpha_tab_host = tab;
+    }
+    const float4* const* dtab = reinterpret_cast<const float4* const*>(w.alpha_tab.ptr);
+    const pt::AlphaArgs A{dtab, dtab + nb, nullptr, nullptr, nullptr, 0u, 0u};
+    const uint32_t grid = (max_count + kBlock - 1) / kBlock;
+    uint32_t cap;
+    atlas_rt_status st = world_prepare(w, grid * kBlock, cap);
+    if (st != ATLAS_RT_OK) return st;
+    hipLaunchKernelGGL((k_world_alpha_packed<kBlock, kLdsDepth>), dim3(grid), dim3(kBlock), 0, stream, w.dev(), A,
+                       reinterpret_cast<const float4*>(in), reinterpret_cast<float4*>(out_rays), out_trans, d_count,
+                       max_count, tmin, tmax, mask, reinterpret_cast<int32_t*>(w.spill.ptr), cap,
                        reinterpret_cast<uint32_t*>(w.flags.ptr));
     ATLAS_HIP_TRY(hipGetLastError());
     return ATLAS_RT_OK;

@@ -33,6 +33,13 @@ struct WorldLane {
     BlasDev B;
     // object-space ray of the best hit (for the reported t, u, v)
     float hox, hoy, hoz, hdx, hdy, hdz;
+    float trans;  // HitAnyTransparency's transparency
+};
+
+// Candidate filter of the closest-hit walk: HitClosest accepts every
+// geometric hit; HitClosestTransparency passes an opacity test instead.
+struct AcceptAll {
+    __device__ __forceinline__ bool operator()(int32_t, int32_t, float, float) const { return true; }
 };
 
 __device__ __forceinline__ float dot4(float x, float y, float z, float w, float4 m) {
@@ -69,6 +76,7 @@ __device__ __forceinline__ void world_begin(WorldLane& L, float4 r0, float4 r1) 
     L.cur_inst = -1;
     L.sp = 0;
     L.tlas_index = kTlasInvalid;
+    L.trans = 1.0f;
 }
 
 // CheckInstance (bvh.hsh:172-189); returns whether the mask admits it.
@@ -90,9 +98,13 @@ __device__ __forceinline__ bool enter_instance(WorldLane& L, const WorldDev& W, 
     return (uint32_t(__float_as_int(meta.w)) & mask) > 0;
 }
 
-template <int BLOCK, int LDS_DEPTH>
+// One step of HitClosest (bvh.hsh:191-273); with an opacity filter it is
+// HitClosestTransparency (bvh.hsh:275-357), whose leaf test additionally
+// needs opacity > 0 (CheckLeafClosestTransparency, bvh.hsh:106-137).
+template <int BLOCK, int LDS_DEPTH, class Accept = AcceptAll>
 __device__ __forceinline__ int world_step_closest(WorldLane& L, const WorldDev& W, uint32_t mask,
-                                                  const LaneStack<BLOCK, LDS_DEPTH>& S, uint32_t cap) {
+                                                  const LaneStack<BLOCK, LDS_DEPTH>& S, uint32_t cap,
+                                                  const Accept& accept = Accept()) {
     const uint32_t spE = L.sp + 1;  // the engine's stackPtr
     if (spE < L.tlas_index) {
         restore_world(L);
@@ -155,7 +167,7 @@ __device__ __forceinline__ int world_step_closest(WorldLane& L, const WorldDev& 
     const float4 a = tr[0], b = tr[1], c = tr[2];
     float t, u, v;
     const bool geo = tri_uvt(L.R, a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z, t, u, v);
-    if (geo && t > L.R.tmin && t < L.hitT) {
+    if (geo && t > L.R.tmin && t < L.hitT && accept(L.cur_inst, tp, u, v)) {
         L.hitT = t;
         L.slot = tp;
         L.inst = L.cur_inst;
@@ -260,5 +272,89 @@ __device__ __forceinline__ int world_step_any(WorldLane& L, const WorldDev& W, u
     return kContinue;
 }
 
+// One step of HitAnyTransparency (bvh.hsh:443-524) with the world ray
+// restored on every TLAS step.  A leaf is handled in one step, as
+// CheckLeafTransparency does (bvh.hsh:139-170): starting from the current
+// transparency it multiplies (1 - opacity) for every triangle hit in
+// (tmin, tmax), and the caller multiplies that product into the transparency
+// once more (bvh.hsh:496), so each leaf squares what came before it.  Below
+// 1e-6 the transparency is 0 and the walk ends.  `opacity(instance, slot, u, v)`
+// gives the triangle's opacity.
+template <int BLOCK, int LDS_DEPTH, class Opacity>
+__device__ __forceinline__ int world_step_transparency(WorldLane& L, const WorldDev& W, uint32_t mask, float tmax,
+                                                       const LaneStack<BLOCK, LDS_DEPTH>& S, uint32_t cap,
+                                                       const Opacity& opacity) {
+    const uint32_t spE = L.sp + 1;
+    if (spE < L.tlas_index) {
+        restore_world(L);
+        L.tlas_index = kTlasInvalid;
+        if (L.node < 0) {
+            if (enter_instance(L, W, mask)) {
+                L.tlas_index = spE;
+            } else {
+                if (L.sp == 0) return kDone;
+                L.sp--;
+                L.node = S.get(L.sp);
+            }
+            return kContinue;
+        }
+        const float4* np = W.tlas + 4 * size_t(L.node);
+        const float4 a = np[0], b = np[1], c = np[2], d = np[3];
+        float dl, dr;
+        const bool hl = slab_dist(L.R, a.x, a.y, a.z, a.w, b.x, b.y, tmax, dl);
+        const bool hr = slab_dist(L.R, b.z, b.w, c.x, c.y, c.z, c.w, tmax, dr);
+        const int32_t lp = __float_as_int(d.x), rp = __float_as_int(d.y);
+        if (hl && hr) {
+            if (L.sp >= cap) return kOverflow;
+            S.put(L.sp, rp);
+            L.sp++;
+            L.node = lp;
+        } else if (hl || hr) {
+            L.node = hl ? lp : rp;
+        } else {
+            if (L.sp == 0) return kDone;
+            L.sp--;
+            L.node = S.get(L.sp);
+        }
+        return kContinue;
+    }
+    if (L.node >= 0) {
+        const float4* np = L.B.nodes + 4 * size_t(L.node);
+        const float4 a = np[0], b = np[1], c = np[2], d = np[3];
+        float dl, dr;
+        const bool hl = slab_dist(L.R, a.x, a.y, a.z, a.w, b.x, b.y, tmax, dl);
+        const bool hr = slab_dist(L.R, b.z, b.w, c.x, c.y, c.z, c.w, tmax, dr);
+        const int32_t lp = __float_as_int(d.x), rp = __float_as_int(d.y);
+        if (hl && hr) {
+            if (L.sp >= cap) return kOverflow;
+            S.put(L.sp, rp);
+            L.sp++;
+            L.node = lp;
+        } else if (hl || hr) {
+            L.node = hl ? lp : rp;
+        } else {
+            if (L.sp == 0) return kDone;
+            L.sp--;
+            L.node = S.get(L.sp);
+        }
+        return kContinue;
+    }
+    float leaf = L.trans;
+    for (int32_t tp = ~L.node;; tp++) {
+        const float4* tr = L.B.tris + 3 * size_t(tp);
+        const float4 a = tr[0], b = tr[1], c = tr[2];
+        float t, u, v;
+        const bool geo = tri_uvt(L.R, a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z, t, u, v);
+        if (geo && t > L.R.tmin && t < tmax) leaf *= (1.0f - opacity(L.cur_inst, tp, u, v));
+        if (b.w > 0.0f) break;
+    }
+    L.trans *= leaf;
+    if (L.trans < 0.000001f) L.trans = 0.0f;
+    if (!(L.trans > 0.0f)) return kDone;
+    if (L.sp == 0) return kDone;
+    L.sp--;
+    L.node = S.get(L.sp);
+    return kContinue;
+}
 
 }  // namespace atlas_rt

@@ -124,6 +124,7 @@ atlas_rt_status atlas_rt_world_destroy(atlas_rt_world* w) {
     w->free_device();
     w->spill.release();
     w->flags.release();
+    w->alpha_tab.release();
     w->timing.release();
     delete w;
     return ATLAS_RT_OK;
@@ -238,6 +239,29 @@ atlas_rt_status atlas_rt_trace_closest(const atlas_rt_world* w, const atlas_rt_p
     if (max_count && (!d_in || !d_out)) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_trace_closest: NULL buffer");
     return launch_world_closest_packed(*const_cast<atlas_rt_world*>(w), d_in, d_out, d_count, max_count, tmin, tmax,
                                        mask, reinterpret_cast<hipStream_t>(stream));
+}
+
+atlas_rt_status atlas_rt_trace_closest_transparency(const atlas_rt_world* w, const atlas_rt_packed_ray* d_in,
+                                                    atlas_rt_packed_ray* d_out, const uint32_t* d_count,
+                                                    uint32_t max_count, float tmin, float tmax, uint32_t mask,
+                                                    void* stream) {
+    atlas_rt_status st = check_world(w, "atlas_rt_trace_closest_transparency");
+    if (st != ATLAS_RT_OK) return st;
+    if (max_count && (!d_in || !d_out))
+        return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_trace_closest_transparency: NULL buffer");
+    return launch_world_alpha_packed(*const_cast<atlas_rt_world*>(w), d_in, d_out, nullptr, d_count, max_count, tmin,
+                                     tmax, mask, reinterpret_cast<hipStream_t>(stream));
+}
+
+atlas_rt_status atlas_rt_trace_transparency(const atlas_rt_world* w, const atlas_rt_packed_ray* d_in,
+                                            float* d_transparency, const uint32_t* d_count, uint32_t max_count,
+                                            float tmin, float tmax, uint32_t mask, void* stream) {
+    atlas_rt_status st = check_world(w, "atlas_rt_trace_transparency");
+    if (st != ATLAS_RT_OK) return st;
+    if (max_count && (!d_in || !d_transparency))
+        return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_trace_transparency: NULL buffer");
+    return launch_world_alpha_packed(*const_cast<atlas_rt_world*>(w), d_in, nullptr, d_transparency, d_count,
+                                     max_count, tmin, tmax, mask, reinterpret_cast<hipStream_t>(stream));
 }
 
 atlas_rt_status atlas_rt_trace_any(const atlas_rt_world* w, const atlas_rt_packed_ray* d_in, uint32_t* d_occluded,

@@ -22,6 +22,9 @@ atlas_rt_status launch_world_any_rays(atlas_rt_world& w, const void* rays, uint3
 atlas_rt_status launch_world_closest_packed(atlas_rt_world& w, const void* in, void* out, const uint32_t* d_count,
                                             uint32_t max_count, float tmin, float tmax, uint32_t mask,
                                             hipStream_t stream);
+atlas_rt_status launch_world_alpha_packed(atlas_rt_world& w, const void* in, void* out_rays, float* out_trans,
+                                          const uint32_t* d_count, uint32_t max_count, float tmin, float tmax,
+                                          uint32_t mask, hipStream_t stream);
 atlas_rt_status launch_world_any_packed(atlas_rt_world& w, const void* in, uint32_t* out, const uint32_t* d_count,
                                         uint32_t max_count, float tmin, float tmax, uint32_t mask, uint32_t flags,
                                         hipStream_t stream);
@@ -48,6 +51,9 @@ struct atlas_rt_world {
     atlas_rt::Scratch spill;
     atlas_rt::Scratch flags;
     atlas_rt::LaunchTiming timing;
+    // per-BLAS shading / uv pointers for the opacity-aware traces
+    atlas_rt::Scratch alpha_tab;
+    std::vector<const void*> alpha_tab_host;
 
     atlas_rt::WorldDev dev() const {
         atlas_rt::WorldDev d;

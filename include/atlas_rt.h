@@ -357,6 +357,25 @@ atlas_rt_status atlas_rt_trace_closest(const atlas_rt_world* world, const atlas_
                                        atlas_rt_packed_ray* d_out, const uint32_t* d_count, uint32_t max_count,
                                        float tmin, float tmax, uint32_t mask, void* stream);
 
+/* DispatchHitClosest with opacityCheck = true (RayTracingHelper.cpp:351-353):
+ * as atlas_rt_trace_closest, but HitClosestTransparency (bvh.hsh:275-357)
+ * drops candidates whose opacity is <= 0.  Opacity is the BLASes' baked
+ * per-slot value (atlas_rt_blas_set_opacity); slots baked -1 (opacity map)
+ * read as 1 here, since only the path tracer holds materials and textures.
+ * Every BLAS of the world needs shading data (atlas_rt_blas_set_shading). */
+atlas_rt_status atlas_rt_trace_closest_transparency(const atlas_rt_world* world, const atlas_rt_packed_ray* d_in,
+                                                    atlas_rt_packed_ray* d_out, const uint32_t* d_count,
+                                                    uint32_t max_count, float tmin, float tmax, uint32_t mask,
+                                                    void* stream);
+
+/* HitAnyTransparency (bvh.hsh:443-524) per PackedRay: d_transparency[i] =
+ * the transparency left along the ray (1 = unoccluded, 0 = blocked), with
+ * the engine's per-leaf squaring (bvh.hsh:496) and its 1e-6 cut-off; 1 for
+ * a NaN direction.  Opacity as for atlas_rt_trace_closest_transparency. */
+atlas_rt_status atlas_rt_trace_transparency(const atlas_rt_world* world, const atlas_rt_packed_ray* d_in,
+                                            float* d_transparency, const uint32_t* d_count, uint32_t max_count,
+                                            float tmin, float tmax, uint32_t mask, void* stream);
+
 /* Any-hit traversal flavours.
  * ATLAS_RT_ANY_ENGINE_HITANY: HitAny (bvh.hsh:359-441) bit for bit, including
  *   that it restores the world-space ray only when returning from an entered
@@ -394,8 +413,24 @@ atlas_rt_status atlas_rt_world_trace_any_rays(const atlas_rt_world* world, const
  * original order; material: one index per triangle or NULL (all 0). */
 atlas_rt_status atlas_rt_blas_set_shading(atlas_rt_blas* blas, const float* normals, const int32_t* material);
 
+/* Per-triangle opacity and texture coordinates of a BLAS, used by the path
+ * tracer's opacity-aware traversal (HitClosestTransparency /
+ * HitAnyTransparency, raytracer/bvh.hsh:106-170,275-357,443-524).
+ * opacity: one value per triangle in the caller's original order, baked as
+ * MeshData::BuildBVH does (MeshData.cpp:141): the material's opacity, or -1
+ * where the material has an opacity map (the traversal then samples
+ * GetOpacity, raytracer/surface.hsh:147-160); NULL = all 1 (opaque).
+ * uvs: 6 floats per triangle (uv0, uv1, uv2; GPUTriangle d1/d2 texture
+ * coordinates, MeshData.cpp:222-237, kept in float32 here), or NULL (all 0).
+ * Call after atlas_rt_blas_set_shading. */
+atlas_rt_status atlas_rt_blas_set_opacity(atlas_rt_blas* blas, const float* opacity, const float* uvs);
+
 /* RaytraceMaterial / GPUMaterial subset used by the bounce shader
- * (RTStructures.h:51-78, raytracer/surface.hsh:22-44).  No textures. */
+ * (RTStructures.h:51-78, raytracer/structures.hsh:107-135, surface.hsh:63-140).
+ * opacity_texture indexes the path tracer's texture table
+ * (atlas_rt_pathtracer_set_textures); -1, or any index outside the table,
+ * means no opacity map (SampleOpacityBilinear returns 1).  Base colour,
+ * normal, roughness, metalness and AO maps are not supported. */
 typedef struct atlas_rt_material {
     float base_color[3];
     float opacity;
@@ -405,19 +440,44 @@ typedef struct atlas_rt_material {
     float ao;
     float reflectance;
     int32_t two_sided;
-} atlas_rt_material; /* 48 B */
+    int32_t opacity_texture;
+    int32_t invert_uvs;
+    int32_t reserved[2];
+} atlas_rt_material; /* 64 B */
 
-/* GPULight (RTStructures.h:105-110, RayTracingHelper::UpdateLights
- * RayTracingHelper.cpp:436-500).  Directional lights only: `direction` is the
- * direction the light travels (LightComponent direction, L = -direction),
+/* A punctual light as the engine's LightComponent describes it
+ * (RayTracingHelper::UpdateLights, RayTracingHelper.cpp:436-500): directional
+ * only; `direction` is the direction the light travels (L = -direction),
  * radiance = linear colour * intensity. */
-#define ATLAS_RT_LIGHT_DIRECTIONAL 0u
+#define ATLAS_RT_LIGHT_DIRECTIONAL 0u /* DIRECTIONAL_LIGHT (raytracer/common.hsh:11) */
+#define ATLAS_RT_LIGHT_TRIANGLE 1u    /* TRIANGLE_LIGHT (raytracer/common.hsh:12) */
 typedef struct atlas_rt_light {
     float direction[3];
     uint32_t type;
     float radiance[3];
     float reserved;
 } atlas_rt_light; /* 32 B */
+
+/* GPULight / PackedLight (RTStructures.h:105-110, raytracer/structures.hsh,
+ * unpacked by UnpackLight raytracer/common.hsh:100-121): the light list the
+ * bounce shader reads.  data[0] = bits(type << 28 | BVH slot of a triangle
+ * light), data[1] = normalised selection weight, data[2] = area,
+ * data[3] = bits(instance slot, in the world's TLAS order). */
+typedef struct atlas_rt_gpu_light {
+    float P[4];
+    float N[4];
+    float color[4];
+    float data[4];
+} atlas_rt_gpu_light; /* 64 B */
+
+/* An opacity texture: R8 UNORM texels, row-major, width * height bytes in
+ * host memory (copied).  Sampled bilinearly with repeat addressing at level 0
+ * (the engine's global sampler, MainRenderer.cpp:922-929). */
+typedef struct atlas_rt_texture {
+    uint32_t width;
+    uint32_t height;
+    const uint8_t* texels;
+} atlas_rt_texture;
 
 /* RayGenUniforms (PathTracingRenderer.h:48-57): origin/right/bottom are the
  * near-plane top-left corner and the spans to the top-right and bottom-left
@@ -456,6 +516,31 @@ atlas_rt_status atlas_rt_pathtracer_destroy(atlas_rt_pathtracer* pt);
  * (RandomHelper.cpp:8-51) and the sub-pixel jitter rayGen.csh:33-34. */
 atlas_rt_status atlas_rt_pathtracer_render(atlas_rt_pathtracer* pt, const atlas_rt_camera* camera, uint32_t frames,
                                            void* stream);
+
+/* Opacity textures referenced by atlas_rt_material.opacity_texture; replaces
+ * the previous table (the scene's bindless textures). */
+atlas_rt_status atlas_rt_pathtracer_set_textures(atlas_rt_pathtracer* pt, const atlas_rt_texture* textures,
+                                                 uint32_t count);
+
+/* PathTracingRenderer::sampleEmissives (PathTracingRenderer.h:46, passed to
+ * UpdateLights at PathTracingRenderer.cpp:65): adds one triangle light per
+ * (instance, BVH slot) whose emitter has a positive brightness, as
+ * RayTracingWorld::BuildTriangleLightsForMesh / UpdateTriangleLights
+ * (RayTracingWorld.cpp:322-407) build them, and renormalises the whole list
+ * (UpdateLights).  emitters: 4 floats per entry of the material table (the
+ * material's sRGB emissive colour and emissive intensity, which
+ * GPUMaterial.emissiveColor = linear(colour) * intensity comes from); the
+ * emitter of a triangle is its material index + the instance's
+ * material_offset.  emitters = NULL or count = 0 turns triangle lights off.
+ * Light order: directional lights, then BLAS by BLAS, slot by slot, the
+ * instances of that BLAS in TLAS order. */
+atlas_rt_status atlas_rt_pathtracer_set_emissive_lights(atlas_rt_pathtracer* pt, const float* emitters,
+                                                        uint32_t count);
+
+/* The light list in use (host copy).  *count receives its length; at most
+ * `cap` entries are written to `out` (which may be NULL when cap = 0). */
+atlas_rt_status atlas_rt_pathtracer_lights(const atlas_rt_pathtracer* pt, atlas_rt_gpu_light* out, uint32_t cap,
+                                           uint32_t* count);
 
 /* PathTracingRenderer::ResetSampleCount: clears the accumulator. */
 atlas_rt_status atlas_rt_pathtracer_reset(atlas_rt_pathtracer* pt);

@@ -269,57 +269,178 @@ def _scene_args(tlas_nodes, instances, xform, blases, shading, materials, lights
     return args, keep
 
 
+class _Texture(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("texels", C.c_void_p)]
+
+
+class _Surface(C.Structure):
+    _fields_ = [("opacity", C.c_void_p), ("uvs", C.c_void_p), ("textures", C.c_void_p),
+                ("texture_count", C.c_uint32), ("emitters", C.c_void_p), ("emitter_count", C.c_uint32),
+                ("lights_out", C.c_void_p), ("lights_cap", C.c_uint32), ("lights_count", C.c_void_p)]
+
+
+GPU_LIGHT_DTYPE = np.dtype([("P", "<f4", 4), ("N", "<f4", 4), ("color", "<f4", 4), ("data", "<f4", 4)])
+
+
+def _surface_arg(surface, n_blas, keep):
+    """OracleSurface from a dict with optional keys: opacity (per BLAS, slot
+    order, or None), uvs (per BLAS, (n, 6) slot order, or None), textures
+    (list of (h, w) uint8), emitters ((m, 4) float32), lights_cap (int: the
+    light list is returned in surface["lights"])."""
+    if surface is None:
+        return None
+    S = _Surface()
+    def per_blas(key, width):
+        vals = surface.get(key)
+        if vals is None:
+            return None
+        ptrs = (C.c_void_p * n_blas)()
+        for i, a in enumerate(vals):
+            if a is None:
+                continue
+            a = np.ascontiguousarray(a, dtype=np.float32).reshape(-1, width)
+            keep.append(a)
+            ptrs[i] = a.ctypes.data
+        keep.append(ptrs)
+        return C.cast(ptrs, C.c_void_p)
+    S.opacity = per_blas("opacity", 1)
+    S.uvs = per_blas("uvs", 6)
+    tex = surface.get("textures") or []
+    if tex:
+        arr = (_Texture * len(tex))()
+        for i, t in enumerate(tex):
+            t = np.ascontiguousarray(t, dtype=np.uint8)
+            keep.append(t)
+            arr[i].height, arr[i].width = t.shape
+            arr[i].texels = t.ctypes.data
+        keep.append(arr)
+        S.textures = C.cast(arr, C.c_void_p)
+        S.texture_count = len(tex)
+    em = surface.get("emitters")
+    if em is not None:
+        em = np.ascontiguousarray(em, dtype=np.float32).reshape(-1, 4)
+        keep.append(em)
+        S.emitters = em.ctypes.data
+        S.emitter_count = len(em)
+    cap = int(surface.get("lights_cap", 0))
+    if cap:
+        out = np.zeros(cap, GPU_LIGHT_DTYPE)
+        cnt = np.zeros(1, np.uint32)
+        keep += [out, cnt]
+        S.lights_out, S.lights_cap, S.lights_count = out.ctypes.data, cap, cnt.ctypes.data
+        surface["_lights"] = (out, cnt)
+    keep.append(S)
+    return C.addressof(S)
+
+
+def _collect_lights(surface):
+    if surface is not None and "_lights" in surface:
+        out, cnt = surface.pop("_lights")
+        surface["lights"] = out[:min(int(cnt[0]), len(out))].copy()
+
+
 _SCENE_TYPES = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                 C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32,
                 C.c_void_p]
 
 
 def pathtrace(tlas_nodes, instances, xform, blases, shading, materials, lights, env, cam, width, height, bounces,
-              frames, first_sample=0, pcg_draws=0, accum=None, threads=None):
+              frames, first_sample=0, pcg_draws=0, accum=None, threads=None, surface=None):
     """CPU restatement of the progressive path tracer.
 
     blases: list of (nodes, tris) packed arrays; shading: list of (normals
     (n, 9) float32, material (n,) int32) in BVH slot order; xform: (m, 12)
     object-to-world rows per packed instance; cam: a ctypes camera struct
-    (location, origin, right, bottom).  Returns the accumulated (h, w, 4)
-    float32 image (rgb sum, sample count).
+    (location, origin, right, bottom); surface: optional opacity, texture and
+    triangle-light inputs (see _surface_arg).  Returns the accumulated
+    (h, w, 4) float32 image (rgb sum, sample count).
     """
     args, keep = _scene_args(tlas_nodes, instances, xform, blases, shading, materials, lights, env)
+    ext = _surface_arg(surface, len(blases), keep)
     if accum is None:
         accum = np.zeros((height, width, 4), dtype=np.float32)
     L = lib()
     if not getattr(L, "_pt_bound", False):
         vp, u32, i32 = C.c_void_p, C.c_uint32, C.c_int32
         L.oracle_pathtrace.restype = None
-        L.oracle_pathtrace.argtypes = _SCENE_TYPES + [vp, u32, u32, i32, u32, u32, C.c_uint64, vp, i32]
+        L.oracle_pathtrace.argtypes = _SCENE_TYPES + [vp, u32, u32, i32, u32, u32, C.c_uint64, vp, i32, vp]
         L._pt_bound = True
     L.oracle_pathtrace(*args, C.addressof(cam), width, height, bounces, frames, first_sample, pcg_draws, _p(accum),
-                       threads or threads_default())
+                       threads or threads_default(), ext)
+    _collect_lights(surface)
     return accum
 
 
 def pathtrace_cache(tlas_nodes, instances, xform, blases, shading, materials, lights, env, cam, width, height,
-                    bounces, train_stride, frame_index=0, sample_index=0, pcg_draws=0, threads=None):
+                    bounces, train_stride, frame_index=0, sample_index=0, pcg_draws=0, threads=None, surface=None):
     """One frame of the radiance-cache mode.  Returns (radiance (h, w, 3)
     added before any cache term, query (h, w, 16): cache query (12), the
     throughput reaching the vertex (3), flag; train (h, w) bool; rec
     (h, w, bounces + 1, 20): per bounce of a training path its query (12),
     throughput reaching it (3), surface flag, radiance added (3), 0)."""
     args, keep = _scene_args(tlas_nodes, instances, xform, blases, shading, materials, lights, env)
+    ext = _surface_arg(surface, len(blases), keep)
     L = lib()
     if not getattr(L, "_ptc_bound", False):
         vp, u32, i32 = C.c_void_p, C.c_uint32, C.c_int32
         L.oracle_pathtrace_cache.restype = None
         L.oracle_pathtrace_cache.argtypes = _SCENE_TYPES + [vp, u32, u32, i32, u32, u32, C.c_uint64, u32, vp, vp, vp,
-                                                            vp, i32]
+                                                            vp, i32, vp]
         L._ptc_bound = True
     rad = np.zeros((height, width, 4), np.float32)
     query = np.zeros((height, width, 16), np.float32)
     train = np.zeros((height, width), np.int32)
     rec = np.zeros((height, width, bounces + 1, 20), np.float32)
     L.oracle_pathtrace_cache(*args, C.addressof(cam), width, height, bounces, frame_index, sample_index, pcg_draws,
-                             train_stride, _p(rad), _p(query), _p(train), _p(rec), threads or threads_default())
+                             train_stride, _p(rad), _p(query), _p(train), _p(rec), threads or threads_default(), ext)
+    _collect_lights(surface)
     return rad[..., :3], query, train.astype(bool), rec
+
+
+def sample_opacity(texels, x, y):
+    """Bilinear, repeat-addressed sample of an (h, w) uint8 opacity texture."""
+    texels = np.ascontiguousarray(texels, dtype=np.uint8)
+    L = lib()
+    if not getattr(L, "_so_bound", False):
+        L.oracle_sample_opacity.restype = C.c_float
+        L.oracle_sample_opacity.argtypes = [C.c_uint32, C.c_uint32, C.c_void_p, C.c_float, C.c_float]
+        L._so_bound = True
+    return float(L.oracle_sample_opacity(texels.shape[1], texels.shape[0], _p(texels), x, y))
+
+
+def trace_world_alpha(tlas_nodes, instances, blases, opacity, rays, mask=0xFFFFFFFF):
+    """Per ray: HitAnyTransparency's transparency and HitClosestTransparency's
+    (slot, instance slot, t), with baked per-slot opacity per BLAS.
+    blases: list of (nodes, tris); opacity: list of (n,) float32 in slot order."""
+    k = len(blases)
+    keep = []
+    node_ptrs, tri_ptrs, op_ptrs = (C.c_void_p * k)(), (C.c_void_p * k)(), (C.c_void_p * k)()
+    node_counts, tri_counts = np.zeros(k, np.uint32), np.zeros(k, np.uint32)
+    for i, ((bn, bt), op) in enumerate(zip(blases, opacity)):
+        bn = np.ascontiguousarray(bn, dtype=NODE_DTYPE)
+        bt = np.ascontiguousarray(bt, dtype=GPU_TRI_DTYPE)
+        op = np.ascontiguousarray(op, dtype=np.float32)
+        keep += [bn, bt, op]
+        node_ptrs[i] = bn.ctypes.data if len(bn) else None
+        tri_ptrs[i], op_ptrs[i] = bt.ctypes.data, op.ctypes.data
+        node_counts[i], tri_counts[i] = len(bn), len(bt)
+    tlas_nodes = np.ascontiguousarray(tlas_nodes, dtype=NODE_DTYPE)
+    instances = np.ascontiguousarray(instances, dtype=INSTANCE_DTYPE)
+    rays = np.ascontiguousarray(rays).view(np.float32).reshape(-1, 8)
+    n = len(rays)
+    tr, t = np.zeros(n, np.float32), np.zeros(n, np.float32)
+    slot, inst = np.zeros(n, np.int32), np.zeros(n, np.int32)
+    L = lib()
+    if not getattr(L, "_twa_bound", False):
+        vp, u32 = C.c_void_p, C.c_uint32
+        L.oracle_trace_world_alpha.restype = None
+        L.oracle_trace_world_alpha.argtypes = [vp, u32, vp, u32, vp, vp, vp, vp, u32, vp, vp, u32, u32, vp, vp, vp, vp]
+        L._twa_bound = True
+    L.oracle_trace_world_alpha(_p(tlas_nodes), len(tlas_nodes), _p(instances), len(instances),
+                               C.cast(node_ptrs, C.c_void_p), _p(node_counts), C.cast(tri_ptrs, C.c_void_p),
+                               _p(tri_counts), k, C.cast(op_ptrs, C.c_void_p), _p(rays), n, mask, _p(tr), _p(slot),
+                               _p(inst), _p(t))
+    return tr, slot, inst, t
 
 
 def triangle_test(o, d, v0, v1, v2):

@@ -56,8 +56,20 @@ struct World {
     uint32_t blas_count;
 };
 
+// Opacity of a candidate hit (instance slot, BVH slot, object-space u, v):
+// CheckLeafClosestTransparency / CheckLeafTransparency (bvh.hsh:106-170).
+struct Opacity {
+    const void* ctx;
+    float (*fn)(const void* ctx, int32_t instance, int32_t slot, float u, float v);
+};
+
+// op = nullptr: HitClosest; otherwise HitClosestTransparency (bvh.hsh:275-357),
+// which drops candidates whose opacity is <= 0.
 void closest_rgpu_world(const World& w, V3 o0, V3 d0, float tmin, float tmax, uint32_t mask, HitRec& h,
-                        TraceStats& st, std::vector<int32_t>& stack);
+                        TraceStats& st, std::vector<int32_t>& stack, const Opacity* op = nullptr);
+// HitAnyTransparency (bvh.hsh:443-524): the transparency left along the ray.
+float transparency_rgpu_world(const World& w, V3 o0, V3 d0, float tmin, float tmax, uint32_t mask,
+                              TraceStats& st, std::vector<int32_t>& stack, const Opacity& op);
 bool any_rgpu_world(const World& w, V3 o0, V3 d0, float tmin, float tmax, uint32_t mask, TraceStats& st,
                     std::vector<int32_t>& stack, bool restore_always);
 

@@ -11,18 +11,38 @@
 //     CheckVisibility), raytracer/surface.hsh:46-140, raytracer/direct.hsh:12-86;
 //   brdf/brdf.hsh, brdfEval.hsh, brdfSample.hsh, brdf/surface.hsh;
 //   common/random.hsh (hash RNG).
-// Traversal: HitClosest and HitAnyTransparency's traversal (opaque triangles)
-// from rt_oracle_trace.cpp.  The shadow ray is traced inline, as the engine
-// does.  Light selection draws its seed from the pixel index (the engine uses
-// the invocation index; with one light the draw cannot change the choice).
-// Payload radiance/throughput stay float32 (the engine packs them to half
-// floats between bounces).
+// Traversal: HitClosestTransparency and HitAnyTransparency
+// (raytracer/bvh.hsh:275-357, 443-524) from rt_oracle_trace.cpp; a world whose
+// triangles are all opaque takes HitClosest / HitAny, which give the same
+// results there.  The shadow ray is traced inline, as the engine does.  Light
+// selection and light sampling draw their seed from the pixel index (the
+// engine uses the invocation index of the compacted ray buffer).  Payload
+// radiance/throughput stay float32 (the engine packs them to half floats
+// between bounces).
+// Triangle lights: RayTracingWorld::BuildTriangleLightsForMesh /
+// UpdateTriangleLights (src/engine/raytracing/RayTracingWorld.cpp:322-407) and
+// RayTracingHelper::UpdateLights (src/engine/renderer/helper/RayTracingHelper.cpp:436-500).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <thread>
 #include <vector>
 
 #include "rt_oracle.h"
+
+// Optional surface and light inputs of the oracle path tracer (NULL: opaque
+// triangles, no textures, directional lights only).
+struct OracleSurface {
+    const float* const* opacity;  // per BLAS: baked opacity per slot, or NULL
+    const float* const* uvs;      // per BLAS: 6 floats per slot, or NULL
+    const atlas_rt_texture* textures;
+    uint32_t texture_count;
+    const float* emitters;        // per material: sRGB emissive rgb + intensity; NULL = no triangle lights
+    uint32_t emitter_count;
+    atlas_rt_gpu_light* lights_out;  // optional copy of the light list
+    uint32_t lights_cap;
+    uint32_t* lights_count;
+};
 
 namespace oracle {
 namespace {
@@ -86,8 +106,33 @@ struct Pcg {
 struct Mat {
     V3 base, emissive;
     float opacity, roughness, metalness, ao, reflectance;
-    int32_t two_sided;
+    int32_t two_sided, opacity_texture, invert_uvs;
 };
+
+struct Tex {
+    uint32_t w, h;
+    const uint8_t* texels;  // R8 UNORM, row-major
+};
+
+// SampleOpacityBilinear (raytracer/texture.hsh:18-25) with the engine's global
+// sampler (linear filter, repeat addressing, MainRenderer.cpp:922-929) at
+// level 0; an index outside the table reads as no texture (1.0).
+float sample_opacity(const std::vector<Tex>& tex, int32_t id, float x, float y) {
+    if (id < 0 || size_t(id) >= tex.size()) return 1.0f;
+    const Tex& T = tex[size_t(id)];
+    const float xw = x - std::floor(x), yw = y - std::floor(y);
+    const float fx = xw * float(T.w) - 0.5f, fy = yw * float(T.h) - 0.5f;
+    const float x0 = std::floor(fx), y0 = std::floor(fy);
+    const float a = fx - x0, b = fy - y0;
+    const int32_t ix = int32_t(x0), iy = int32_t(y0);
+    const uint32_t i0 = ix < 0 ? T.w - 1 : std::min(uint32_t(ix), T.w - 1);
+    const uint32_t j0 = iy < 0 ? T.h - 1 : std::min(uint32_t(iy), T.h - 1);
+    const uint32_t i1 = i0 + 1 >= T.w ? 0u : i0 + 1, j1 = j0 + 1 >= T.h ? 0u : j0 + 1;
+    auto at = [&](uint32_t i, uint32_t j) { return float(T.texels[size_t(j) * T.w + i]) / 255.0f; };
+    const float r0 = at(i0, j0) * (1.0f - a) + at(i1, j0) * a;
+    const float r1 = at(i0, j1) * (1.0f - a) + at(i1, j1) * a;
+    return r0 * (1.0f - b) + r1 * b;
+}
 
 struct Surf {
     V3 P, Vv, N, L, H, Ng;
@@ -193,15 +238,15 @@ Bs sample_specular(Surf& s, float u0, float u1) {
 
 struct Scene {
     World w;
-    std::vector<std::vector<float>> shade;  // per BLAS, slot order: n0, n1, n2 (9 floats)
-    std::vector<std::vector<int32_t>> mat;  // per BLAS, slot order
-    const float* xform;                     // 12 floats per packed instance
+    std::vector<std::vector<float>> shade;    // per BLAS, slot order: n0, n1, n2 (9 floats)
+    std::vector<std::vector<int32_t>> mat;    // per BLAS, slot order
+    std::vector<std::vector<float>> opacity;  // per BLAS, slot order (MeshData.cpp:141), empty = opaque
+    std::vector<std::vector<float>> uv;       // per BLAS, slot order: uv0, uv1, uv2 (6 floats), may be empty
+    bool alpha = false;                       // some triangle is not opaque
+    const float* xform;                       // 12 floats per packed instance
     std::vector<Mat> mats;
-    struct Light {
-        V3 N, rad;
-        float pdf;
-    };
-    std::vector<Light> lights;
+    std::vector<Tex> tex;
+    std::vector<atlas_rt_gpu_light> lights;   // the engine's GPULight list
     V3 env;
     int32_t bounces;
 };
@@ -210,6 +255,35 @@ inline V3 xf(const float* M, V3 p, float w) {
     return V3{(M[0] * p.x + M[1] * p.y) + (M[2] * p.z + M[3] * w),
               (M[4] * p.x + M[5] * p.y) + (M[6] * p.z + M[7] * w),
               (M[8] * p.x + M[9] * p.y) + (M[10] * p.z + M[11] * w)};
+}
+
+// glm's mat4 * vec4 (columns summed in order) over the object-to-world rows.
+inline V3 xf_glm(const float* M, V3 p, float w) {
+    return V3{((M[0] * p.x + M[1] * p.y) + M[2] * p.z) + M[3] * w,
+              ((M[4] * p.x + M[5] * p.y) + M[6] * p.z) + M[7] * w,
+              ((M[8] * p.x + M[9] * p.y) + M[10] * p.z) + M[11] * w};
+}
+
+inline V3 uv_at(const std::vector<float>& uv, int32_t slot, float s, float t) {
+    if (uv.empty()) return V3{0, 0, 0};
+    const float* q = &uv[6 * size_t(slot)];
+    const float r = (1.0f - s) - t;
+    return V3{(r * q[0] + s * q[2]) + t * q[4], (r * q[1] + s * q[3]) + t * q[5], 0.0f};
+}
+
+// The triangle's opacity as the traversal sees it: the baked value, or for
+// -1 GetOpacity (raytracer/surface.hsh:147-160) at the object-space (u, v).
+float tri_opacity(const void* ctx, int32_t instance, int32_t slot, float u, float v) {
+    const Scene& S = *static_cast<const Scene*>(ctx);
+    const auto& in = S.w.inst[instance];
+    const int32_t mesh = in.mesh_offset;
+    if (S.opacity[mesh].empty()) return 1.0f;
+    const float op = S.opacity[mesh][slot];
+    if (op >= 0.0f) return op;
+    const Mat& m = S.mats[size_t(S.mat[mesh][slot] + in.material_offset)];
+    V3 tc = uv_at(S.uv[mesh], slot, u, v);
+    if (m.invert_uvs > 0) tc.y = 1.0f - tc.y;
+    return sample_opacity(S.tex, m.opacity_texture, tc.x, tc.y) * m.opacity;
 }
 
 // Radiance-cache bookkeeping of one path (both pointers null in plain mode).
@@ -232,7 +306,8 @@ V3 trace_path(const Scene& S, V3 o, V3 d, float pixel_seed, const float* seeds, 
         float* rec = co.rec ? co.rec + 20 * size_t(bounce) : nullptr;
         HitRec h;
         TraceStats st;
-        closest_rgpu_world(S.w, o, d, 0.0f, kInf, ATLAS_RT_MASK_ALL, h, st, stack);
+        const Opacity op{&S, tri_opacity};
+        closest_rgpu_world(S.w, o, d, 0.0f, kInf, ATLAS_RT_MASK_ALL, h, st, stack, S.alpha ? &op : nullptr);
         if (h.slot < 0) {
             V3 e = S.env * thr;
             const V3 ec{std::fmin(e.x, 10.0f), std::fmin(e.y, 10.0f), std::fmin(e.z, 10.0f)};
@@ -255,12 +330,18 @@ V3 trace_path(const Scene& S, V3 o, V3 d, float pixel_seed, const float* seeds, 
         V3 n0 = nrm(xf(M, V3{ns[0], ns[1], ns[2]}, 0.0f));
         V3 n1 = nrm(xf(M, V3{ns[3], ns[4], ns[5]}, 0.0f));
         V3 n2 = nrm(xf(M, V3{ns[6], ns[7], ns[8]}, 0.0f));
-        const Mat& m = S.mats[size_t(S.mat[mesh][h.slot] + in.material_offset)];
+        Mat m = S.mats[size_t(S.mat[mesh][h.slot] + in.material_offset)];
         V3 e0 = v1 - v0, e1 = v2 - v0, s = o - v0;
         V3 p = cross(s, e0), q = cross(d, e1);
         float den = dot(q, e0);
         float dist = dot(p, e1) / den, bs = dot(q, s) / den, bt = dot(p, d) / den;
         float br = 1.0f - bs - bt;
+        if (m.opacity_texture >= 0 && size_t(m.opacity_texture) < S.tex.size()) {
+            // surface.hsh:86-105: texture coordinate from the world-space barycentrics
+            V3 tc = uv_at(S.uv[mesh], h.slot, bs, bt);
+            if (m.invert_uvs > 0) tc.y = 1.0f - tc.y;
+            m.opacity *= sample_opacity(S.tex, m.opacity_texture, tc.x, tc.y);
+        }
         Surf sf;
         sf.m = m;
         sf.P = o + mul(d, dist);
@@ -298,33 +379,82 @@ V3 trace_path(const Scene& S, V3 o, V3 d, float pixel_seed, const float* seeds, 
             emis = m.emissive;
             Lr = Lr + m.emissive;
         }
-        // EvaluateDirectLight
+        // EvaluateDirectLight (rayHit.csh:209-235), GetLight / SampleLight
+        // (raytracer/direct.hsh:12-86)
         V3 radiance{0, 0, 0};
         if (!S.lights.empty()) {
             float seed = seeds[bounce];
             float total = 0.0f, lpdf = 1.0f;
             size_t sel = 0;
             for (size_t l = 0; l < S.lights.size(); l++) {
-                float brightness = dot(S.lights[l].rad, V3{0.33333f, 0.33333f, 0.33333f});
-                float wgt = clampf(brightness, 0.0000001f, 1.0f);
+                const atlas_rt_gpu_light& Lt = S.lights[l];
+                const V3 lr{Lt.color[0], Lt.color[1], Lt.color[2]};
+                const float brightness = dot(lr, V3{0.33333f, 0.33333f, 0.33333f});
+                float wgt = 1.0f;
+                if ((fbits(Lt.data[0]) >> 28) == 1u) {
+                    const V3 pl = V3{Lt.P[0], Lt.P[1], Lt.P[2]} - sf.P;
+                    const float sq = dot(pl, pl);
+                    const V3 Ld = nrm(pl);
+                    const float ndl = std::fmax(dot(V3{Lt.N[0], Lt.N[1], Lt.N[2]}, neg(Ld)), 0.0f);
+                    wgt = ((brightness * Lt.data[2]) * ndl) / sq;
+                } else {
+                    wgt = brightness;
+                }
+                wgt = clampf(wgt, 0.0000001f, 1.0f);
                 total += wgt;
                 if (rand_seq(pixel_seed, seed) < wgt / total) {
                     lpdf = wgt;
                     sel = l;
                 }
             }
-            const auto& Lt = S.lights[sel];
-            lpdf = lpdf / total * Lt.pdf;
-            sf.L = nrm(neg(Lt.N));
-            update(sf);
+            const atlas_rt_gpu_light& Lt = S.lights[sel];
+            lpdf = lpdf / total * Lt.data[1];
+            float solid = 1.0f, ldist = kInf;
+            if ((fbits(Lt.data[0]) >> 28) == 1u) {
+                const float r0 = rand_seq(pixel_seed, seed), r1 = rand_seq(pixel_seed, seed);
+                const int32_t li = int32_t(fbits(Lt.data[3]));
+                const int32_t lslot = int32_t(fbits(Lt.data[0]) & 0x0FFFFFFFu);
+                const auto& lin = S.w.inst[li];
+                const float* LM = S.xform + 12 * size_t(li);
+                const auto& ltri = S.w.blas[lin.mesh_offset].tris[lslot];
+                const float* lns = &S.shade[lin.mesh_offset][9 * size_t(lslot)];
+                const V3 a0 = xf(LM, V3{ltri.v0[0], ltri.v0[1], ltri.v0[2]}, 1.0f);
+                const V3 a1 = xf(LM, V3{ltri.v1[0], ltri.v1[1], ltri.v1[2]}, 1.0f);
+                const V3 a2 = xf(LM, V3{ltri.v2[0], ltri.v2[1], ltri.v2[2]}, 1.0f);
+                const V3 m0 = nrm(xf(LM, V3{lns[0], lns[1], lns[2]}, 0.0f));
+                const V3 m1 = nrm(xf(LM, V3{lns[3], lns[4], lns[5]}, 0.0f));
+                const V3 m2 = nrm(xf(LM, V3{lns[6], lns[7], lns[8]}, 0.0f));
+                // SampleTriangleLight (raytracer/lights.hsh:11-31)
+                const float sr = std::sqrt(r0);
+                const float u = 1.0f - sr, v = r1 * sr, w = (1.0f - u) - v;
+                const V3 LP = (mul(a0, u) + mul(a1, v)) + mul(a2, w);
+                const V3 LN = (mul(m0, u) + mul(m1, v)) + mul(m2, w);
+                const float spdf = 1.0f / Lt.data[2];
+                const V3 pl = LP - sf.P;
+                const float sq = dot(pl, pl);
+                ldist = std::sqrt(sq);
+                sf.L = V3{pl.x / ldist, pl.y / ldist, pl.z / ldist};
+                update(sf);
+                const float ndl = std::fmax(dot(LN, neg(sf.L)), 0.0f);
+                solid = ndl / (sq * spdf);
+            } else {
+                sf.L = nrm(neg(V3{Lt.N[0], Lt.N[1], Lt.N[2]}));
+                update(sf);
+            }
             V3 refl = mul(diffuse(sf) + specular(sf), m.opacity);
-            V3 rad = Lt.rad;
+            V3 rad = mul(V3{Lt.color[0], Lt.color[1], Lt.color[2]}, solid);
             float vis = 0.0f;
             if (sf.NdotL > 0.0f) {
                 TraceStats st2;
-                bool occluded = any_rgpu_world(S.w, sf.P + mul(sf.N, kEps), sf.L, 0.0f, kInf - 2.0f * kEps,
-                                               ATLAS_RT_MASK_SHADOW, st2, stack, true);
-                vis = occluded ? 0.0f : 1.0f;
+                const V3 so = sf.P + mul(sf.N, kEps);
+                const float tmax = ldist - 2.0f * kEps;
+                if (S.alpha) {
+                    vis = transparency_rgpu_world(S.w, so, sf.L, 0.0f, tmax, ATLAS_RT_MASK_SHADOW, st2, stack,
+                                                  Opacity{&S, tri_opacity});
+                } else {
+                    bool occluded = any_rgpu_world(S.w, so, sf.L, 0.0f, tmax, ATLAS_RT_MASK_SHADOW, st2, stack, true);
+                    vis = occluded ? 0.0f : 1.0f;
+                }
             }
             V3 direct = mul(refl * mul(rad, vis), sf.NdotL);
             direct = V3{direct.x / lpdf, direct.y / lpdf, direct.z / lpdf};
@@ -403,12 +533,72 @@ using namespace oracle;
 
 namespace {
 
+uint32_t ubits(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return u;
+}
+float bitsf(uint32_t u) {
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+// Triangle lights of every instance: BuildTriangleLightsForMesh
+// (RayTracingWorld.cpp:322-376) per BLAS slot, then UpdateTriangleLights
+// (:378-407) per instance of that BLAS.  Emitter e = material index +
+// instance material_offset holds (sRGB emissive colour, emissive intensity).
+// Order: BLAS, then slot, then instance slot.
+void triangle_lights(const Scene& S, const float* emitters, uint32_t n_emitters,
+                     std::vector<atlas_rt_gpu_light>& out) {
+    for (uint32_t k = 0; k < S.w.blas_count; k++) {
+        const BlasView& b = S.w.blas[k];
+        for (uint32_t i = 0; i < b.tri_count; i++) {
+            const auto& tri = b.tris[i];
+            const float* ns = &S.shade[k][9 * size_t(i)];
+            for (uint32_t j = 0; j < S.w.inst_count; j++) {
+                const auto& in = S.w.inst[j];
+                if (in.mesh_offset != int32_t(k)) continue;
+                const int64_t e = int64_t(S.mat[k][i]) + in.material_offset;
+                if (e < 0 || e >= int64_t(n_emitters)) continue;
+                const float* em = emitters + 4 * size_t(e);
+                const V3 lin{std::pow(em[0], 2.2f), std::pow(em[1], 2.2f), std::pow(em[2], 2.2f)};
+                const float brightness = dot(lin, V3{0.3333f, 0.3333f, 0.3333f});
+                if (!(brightness > 0.0f)) continue;
+                const V3 v0{tri.v0[0], tri.v0[1], tri.v0[2]}, v1{tri.v1[0], tri.v1[1], tri.v1[2]},
+                    v2{tri.v2[0], tri.v2[1], tri.v2[2]};
+                const V3 c = (v0 + v1) + v2, nsum = (V3{ns[0], ns[1], ns[2]} + V3{ns[3], ns[4], ns[5]}) +
+                                                    V3{ns[6], ns[7], ns[8]};
+                const V3 P{c.x / 3.0f, c.y / 3.0f, c.z / 3.0f}, N{nsum.x / 3.0f, nsum.y / 3.0f, nsum.z / 3.0f};
+                const V3 ea = v1 - v0, eb = v2 - v0, ec = v1 - v2;
+                const float a = std::sqrt(dot(ea, ea)), bb = std::sqrt(dot(eb, eb)), cc = std::sqrt(dot(ec, ec));
+                const float p = 0.5f * ((a + bb) + cc);
+                const float area = std::sqrt(((p * (p - a)) * (p - bb)) * (p - cc));
+                const V3 col{std::pow(lin.x, 2.2f) * em[3], std::pow(lin.y, 2.2f) * em[3],
+                             std::pow(lin.z, 2.2f) * em[3]};
+                const float* M = S.xform + 12 * size_t(j);
+                const V3 WP = xf_glm(M, P, 1.0f), WN = xf_glm(M, N, 0.0f);
+                atlas_rt_gpu_light L{};
+                L.P[0] = WP.x; L.P[1] = WP.y; L.P[2] = WP.z; L.P[3] = 1.0f;
+                L.N[0] = WN.x; L.N[1] = WN.y; L.N[2] = WN.z; L.N[3] = 0.0f;
+                L.color[0] = col.x; L.color[1] = col.y; L.color[2] = col.z; L.color[3] = 0.0f;
+                L.data[0] = bitsf((1u << 28) | i);
+                L.data[1] = area * brightness;
+                L.data[2] = area;
+                L.data[3] = bitsf(j);
+                out.push_back(L);
+            }
+        }
+    }
+}
+
 void make_scene(Scene& S, std::vector<BlasView>& views, const atlas_rt_gpu_bvh_node* tlas, uint32_t tlas_count,
                 const atlas_rt_gpu_bvh_instance* inst, uint32_t inst_count, const atlas_rt_gpu_bvh_node* const* blas_nodes,
                 const uint32_t* blas_node_counts, const atlas_rt_gpu_bvh_triangle* const* blas_tris,
                 const uint32_t* blas_tri_counts, const float* const* shade_normals, const int32_t* const* shade_mat,
                 uint32_t blas_count, const float* xform, const atlas_rt_material* mats, uint32_t n_mats,
-                const atlas_rt_light* lights, uint32_t n_lights, const float* env, int32_t bounces) {
+                const atlas_rt_light* lights, uint32_t n_lights, const float* env, int32_t bounces,
+                const OracleSurface* ext) {
     views.resize(blas_count);
     for (uint32_t k = 0; k < blas_count; k++)
         views[k] = BlasView{blas_nodes[k], blas_node_counts[k], blas_tris[k], blas_tri_counts[k]};
@@ -416,32 +606,51 @@ void make_scene(Scene& S, std::vector<BlasView>& views, const atlas_rt_gpu_bvh_n
     S.xform = xform;
     S.shade.resize(blas_count);
     S.mat.resize(blas_count);
+    S.opacity.resize(blas_count);
+    S.uv.resize(blas_count);
     for (uint32_t k = 0; k < blas_count; k++) {
         S.shade[k].assign(shade_normals[k], shade_normals[k] + 9 * size_t(blas_tri_counts[k]));
         S.mat[k].assign(shade_mat[k], shade_mat[k] + blas_tri_counts[k]);
+        if (ext && ext->opacity && ext->opacity[k]) {
+            S.opacity[k].assign(ext->opacity[k], ext->opacity[k] + blas_tri_counts[k]);
+            for (float o : S.opacity[k]) S.alpha = S.alpha || o != 1.0f;
+        }
+        if (ext && ext->uvs && ext->uvs[k]) S.uv[k].assign(ext->uvs[k], ext->uvs[k] + 6 * size_t(blas_tri_counts[k]));
     }
     for (uint32_t i = 0; i < n_mats; i++) {
         const auto& m = mats[i];
         S.mats.push_back(Mat{V3{m.base_color[0], m.base_color[1], m.base_color[2]},
                              V3{m.emissive[0], m.emissive[1], m.emissive[2]}, m.opacity, m.roughness, m.metalness,
-                             m.ao, m.reflectance, m.two_sided});
+                             m.ao, m.reflectance, m.two_sided, m.opacity_texture, m.invert_uvs});
     }
-    // UpdateLights (RayTracingHelper.cpp:436-500)
-    float maxw = 0.0f, total = 0.0f;
-    std::vector<float> wts(n_lights);
+    if (ext)
+        for (uint32_t t = 0; t < ext->texture_count; t++)
+            S.tex.push_back(Tex{ext->textures[t].width, ext->textures[t].height, ext->textures[t].texels});
+    // UpdateLights (RayTracingHelper.cpp:436-500): punctual lights first, then
+    // the triangle lights; weights floored at 0.5% of the maximum, normalised.
     for (uint32_t l = 0; l < n_lights; l++) {
         const float* r = lights[l].radiance;
-        wts[l] = (r[0] * 0.3333f + r[1] * 0.3333f) + r[2] * 0.3333f;
-        maxw = std::fmax(maxw, wts[l]);
+        atlas_rt_gpu_light L{};
+        L.P[3] = 1.0f;
+        std::memcpy(L.N, lights[l].direction, 12);
+        std::memcpy(L.color, r, 12);
+        L.data[0] = bitsf((0u << 28) | l);
+        L.data[1] = (r[0] * 0.3333f + r[1] * 0.3333f) + r[2] * 0.3333f;
+        S.lights.push_back(L);
     }
-    for (uint32_t l = 0; l < n_lights; l++) {
-        wts[l] = std::fmax(wts[l], 0.005f * maxw);
-        total += wts[l];
+    if (ext && ext->emitters) triangle_lights(S, ext->emitters, ext->emitter_count, S.lights);
+    float maxw = 0.0f, total = 0.0f;
+    for (const auto& L : S.lights) maxw = std::fmax(maxw, L.data[1]);
+    for (auto& L : S.lights) {
+        L.data[1] = std::fmax(L.data[1], 0.005f * maxw);
+        total += L.data[1];
     }
-    for (uint32_t l = 0; l < n_lights; l++)
-        S.lights.push_back({V3{lights[l].direction[0], lights[l].direction[1], lights[l].direction[2]},
-                            V3{lights[l].radiance[0], lights[l].radiance[1], lights[l].radiance[2]},
-                            wts[l] / total});
+    for (auto& L : S.lights) L.data[1] /= total;
+    if (ext && ext->lights_out) {
+        const uint32_t n = uint32_t(std::min<size_t>(S.lights.size(), ext->lights_cap));
+        std::memcpy(ext->lights_out, S.lights.data(), n * sizeof(atlas_rt_gpu_light));
+        *ext->lights_count = uint32_t(S.lights.size());
+    }
     S.env = V3{env[0], env[1], env[2]};
     S.bounces = bounces;
 }
@@ -488,6 +697,49 @@ uint32_t mix32(uint32_t h) {
 
 extern "C" {
 
+// Bilinear opacity-texture sample (texture.hsh:18-25 with the global sampler).
+float oracle_sample_opacity(uint32_t width, uint32_t height, const uint8_t* texels, float x, float y) {
+    std::vector<Tex> t{Tex{width, height, texels}};
+    return sample_opacity(t, 0, x, y);
+}
+
+// Transparency along a world ray (HitAnyTransparency) and the closest hit of
+// HitClosestTransparency, with per-slot baked opacity per BLAS (no maps).
+struct OpacityTable {
+    const World* w;
+    const float* const* opacity;
+};
+static float table_opacity(const void* ctx, int32_t instance, int32_t slot, float, float) {
+    const OpacityTable& T = *static_cast<const OpacityTable*>(ctx);
+    return T.opacity[T.w->inst[instance].mesh_offset][slot];
+}
+
+void oracle_trace_world_alpha(const atlas_rt_gpu_bvh_node* tlas, uint32_t tlas_count,
+                              const atlas_rt_gpu_bvh_instance* inst, uint32_t inst_count,
+                              const atlas_rt_gpu_bvh_node* const* blas_nodes, const uint32_t* blas_node_counts,
+                              const atlas_rt_gpu_bvh_triangle* const* blas_tris, const uint32_t* blas_tri_counts,
+                              uint32_t blas_count, const float* const* opacity, const float* rays, uint32_t n,
+                              uint32_t mask, float* transparency, int32_t* slot, int32_t* instance, float* t) {
+    std::vector<BlasView> views(blas_count);
+    for (uint32_t k = 0; k < blas_count; k++)
+        views[k] = BlasView{blas_nodes[k], blas_node_counts[k], blas_tris[k], blas_tri_counts[k]};
+    World w{tlas, tlas_count, inst, inst_count, views.data(), blas_count};
+    OpacityTable T{&w, opacity};
+    const Opacity op{&T, table_opacity};
+    std::vector<int32_t> stack;
+    for (uint32_t i = 0; i < n; i++) {
+        const float* r = rays + 8 * size_t(i);
+        TraceStats st;
+        transparency[i] = transparency_rgpu_world(w, V3{r[0], r[1], r[2]}, V3{r[4], r[5], r[6]}, r[3], r[7], mask, st,
+                                                  stack, op);
+        HitRec h;
+        closest_rgpu_world(w, V3{r[0], r[1], r[2]}, V3{r[4], r[5], r[6]}, r[3], r[7], mask, h, st, stack, &op);
+        slot[i] = h.slot;
+        instance[i] = h.instance;
+        t[i] = h.t;
+    }
+}
+
 // shade_normals[k]: 9 floats per slot of BLAS k (BVH slot order);
 // shade_mat[k]: one material index per slot.  accum: width*height*4 floats
 // (rgb sum, count), accumulated over `frames` frames starting at sample
@@ -500,11 +752,12 @@ void oracle_pathtrace(const atlas_rt_gpu_bvh_node* tlas, uint32_t tlas_count, co
                       const int32_t* const* shade_mat, uint32_t blas_count, const float* xform,
                       const atlas_rt_material* mats, uint32_t n_mats, const atlas_rt_light* lights, uint32_t n_lights,
                       const float* env, const atlas_rt_camera* cam, uint32_t width, uint32_t height, int32_t bounces,
-                      uint32_t frames, uint32_t first_sample, uint64_t pcg_draws, float* accum, int32_t threads) {
+                      uint32_t frames, uint32_t first_sample, uint64_t pcg_draws, float* accum, int32_t threads,
+                      const OracleSurface* ext) {
     Scene S;
     std::vector<BlasView> views;
     make_scene(S, views, tlas, tlas_count, inst, inst_count, blas_nodes, blas_node_counts, blas_tris, blas_tri_counts,
-               shade_normals, shade_mat, blas_count, xform, mats, n_mats, lights, n_lights, env, bounces);
+               shade_normals, shade_mat, blas_count, xform, mats, n_mats, lights, n_lights, env, bounces, ext);
     Pcg pcg{0ull + 1442695040888963407ull};
     pcg.next();  // pcg32_init's discarded draw
     for (uint64_t i = 0; i < pcg_draws; i++) pcg.next();
@@ -542,11 +795,12 @@ void oracle_pathtrace_cache(const atlas_rt_gpu_bvh_node* tlas, uint32_t tlas_cou
                             const atlas_rt_light* lights, uint32_t n_lights, const float* env,
                             const atlas_rt_camera* cam, uint32_t width, uint32_t height, int32_t bounces,
                             uint32_t frame_index, uint32_t sample_index, uint64_t pcg_draws, uint32_t train_stride,
-                            float* radiance, float* query, int32_t* train, float* rec, int32_t threads) {
+                            float* radiance, float* query, int32_t* train, float* rec, int32_t threads,
+                            const OracleSurface* ext) {
     Scene S;
     std::vector<BlasView> views;
     make_scene(S, views, tlas, tlas_count, inst, inst_count, blas_nodes, blas_node_counts, blas_tris, blas_tri_counts,
-               shade_normals, shade_mat, blas_count, xform, mats, n_mats, lights, n_lights, env, bounces);
+               shade_normals, shade_mat, blas_count, xform, mats, n_mats, lights, n_lights, env, bounces, ext);
     Pcg pcg{0ull + 1442695040888963407ull};
     pcg.next();
     for (uint64_t i = 0; i < pcg_draws; i++) pcg.next();

@@ -166,7 +166,7 @@ static inline float dot4(float ax, float ay, float az, float aw, const float* m)
 static const uint32_t kTlasInvalid = 0xFFFFFFFFu;
 
 void closest_rgpu_world(const World& w, V3 o0, V3 d0, float tmin, float tmax, uint32_t mask,
-                        HitRec& h, TraceStats& st, std::vector<int32_t>& stack) {
+                        HitRec& h, TraceStats& st, std::vector<int32_t>& stack, const Opacity* op) {
     h = HitRec();
     h.t = tmax;
     if (is_nan3(d0)) return;
@@ -228,7 +228,8 @@ void closest_rgpu_world(const World& w, V3 o0, V3 d0, float tmin, float tmax, ui
                     V3 sol;
                     st.tri_tests++;
                     bool hit = tri_test(o, d, tri_v(tri, 0), tri_v(tri, 1), tri_v(tri, 2), sol);
-                    if (hit && sol.x > tmin && sol.x < leafTmax && sol.x < h.t) {
+                    if (hit && sol.x > tmin && sol.x < leafTmax && sol.x < h.t &&
+                        (!op || op->fn(op->ctx, curInstance, triPtr, sol.y, sol.z) > 0.0f)) {
                         h.t = sol.x;
                         h.u = sol.y;
                         h.v = sol.z;
@@ -341,6 +342,98 @@ bool any_rgpu_world(const World& w, V3 o0, V3 d0, float tmin, float tmax, uint32
             }
         }
     }
+}
+
+// HitAnyTransparency (bvh.hsh:443-524).  The world ray is restored on every
+// TLAS step.  At a leaf, CheckLeafTransparency starts from the current
+// transparency and multiplies (1 - opacity) for every triangle the ray hits in
+// (tmin, tmax); the caller then multiplies that product into the transparency
+// again (bvh.hsh:496), so each leaf squares what came before it.  Below 1e-6
+// the transparency becomes 0 and the walk ends.
+float transparency_rgpu_world(const World& w, V3 o0, V3 d0, float tmin, float tmax, uint32_t mask,
+                              TraceStats& st, std::vector<int32_t>& stack, const Opacity& op) {
+    if (is_nan3(d0)) return 1.0f;
+    float transparency = 1.0f;
+    stack.clear();
+    V3 o = o0, d = d0;
+    int32_t nodePtr = 0;
+    int32_t meshPtr = 0;
+    int32_t curInstance = -1;
+    uint32_t tlasIndex = kTlasInvalid;
+    auto pop = [&](int32_t& np) -> bool {
+        if (stack.empty()) return false;
+        np = stack.back();
+        stack.pop_back();
+        return true;
+    };
+    while (transparency > 0.0f) {
+        uint32_t sp = uint32_t(stack.size()) + 1;
+        if (sp < tlasIndex) {
+            o = o0;
+            d = d0;
+            tlasIndex = kTlasInvalid;
+            if (nodePtr < 0) {
+                const auto& in = w.inst[~nodePtr];
+                st.inst_visits++;
+                o = V3{dot4(o0.x, o0.y, o0.z, 1.0f, in.inverse_matrix[0]), dot4(o0.x, o0.y, o0.z, 1.0f, in.inverse_matrix[1]),
+                       dot4(o0.x, o0.y, o0.z, 1.0f, in.inverse_matrix[2])};
+                d = V3{dot4(d0.x, d0.y, d0.z, 0.0f, in.inverse_matrix[0]), dot4(d0.x, d0.y, d0.z, 0.0f, in.inverse_matrix[1]),
+                       dot4(d0.x, d0.y, d0.z, 0.0f, in.inverse_matrix[2])};
+                curInstance = ~nodePtr;
+                meshPtr = in.mesh_offset;
+                nodePtr = w.blas[meshPtr].node_count ? 0 : ~0;
+                if ((in.mask & mask) > 0)
+                    tlasIndex = sp;
+                else if (!pop(nodePtr))
+                    break;
+            } else {
+                const auto& n = w.tlas[nodePtr];
+                st.node_visits++;
+                float dl, dr;
+                bool iL = slab_gpu(o, d, node_lmin(n), node_lmax(n), tmin, tmax, dl);
+                bool iR = slab_gpu(o, d, node_rmin(n), node_rmax(n), tmin, tmax, dr);
+                int32_t next = iL ? n.left_ptr : n.right_ptr;
+                if (!iR && !iL) {
+                    if (!pop(next)) break;
+                }
+                if (iR && iL) stack.push_back(n.right_ptr);
+                nodePtr = next;
+            }
+        } else {
+            const BlasView& b = w.blas[meshPtr];
+            if (nodePtr < 0) {
+                float leaf = transparency;
+                int32_t triPtr = ~nodePtr;
+                bool eon = false;
+                while (!eon) {
+                    const auto& tri = b.tris[triPtr];
+                    eon = tri.v0[3] > 0.0f;
+                    V3 sol;
+                    st.tri_tests++;
+                    bool hit = tri_test(o, d, tri_v(tri, 0), tri_v(tri, 1), tri_v(tri, 2), sol);
+                    if (hit && sol.x > tmin && sol.x < tmax)
+                        leaf *= (1.0f - op.fn(op.ctx, curInstance, triPtr, sol.y, sol.z));
+                    triPtr++;
+                }
+                transparency *= leaf;
+                if (transparency < 0.000001f) transparency = 0.0f;
+                if (!pop(nodePtr)) break;
+            } else {
+                const auto& n = b.nodes[nodePtr];
+                st.node_visits++;
+                float dl, dr;
+                bool iL = slab_gpu(o, d, node_lmin(n), node_lmax(n), tmin, tmax, dl);
+                bool iR = slab_gpu(o, d, node_rmin(n), node_rmax(n), tmin, tmax, dr);
+                int32_t next = iL ? n.left_ptr : n.right_ptr;
+                if (!iR && !iL) {
+                    if (!pop(next)) break;
+                }
+                if (iR && iL) stack.push_back(n.right_ptr);
+                nodePtr = next;
+            }
+        }
+    }
+    return transparency;
 }
 
 // ---------------------------------------------------------------------------

@@ -24,11 +24,12 @@ def oracle_scene(meshes, normals, mats_per_mesh, xf, which, mask=_abi.MASK_ALL |
     """Oracle-only world + shading arrays (builders restated in the oracle)."""
     descs = atlas_rt.instance_descs(xf, which, mask=mask)
     descs["material_offset"] = which.astype(np.int32)
-    blases, shading = [], []
+    blases, shading, refs = [], [], []
     for v, n in zip(meshes, normals):
         ob = oracle_rt.build_triangles(v)
         blases.append((ob.nodes, ob.tris))
         shading.append((n[ob.ref_idx], np.zeros(len(ob.ref_idx), np.int32)))
+        refs.append(ob.ref_idx)
     boxes = np.zeros((len(descs), 6), np.float32)
     for i, dsc in enumerate(descs):
         tri = meshes[dsc["blas"]].reshape(-1, 3).astype(np.float64)
@@ -48,7 +49,7 @@ def oracle_scene(meshes, normals, mats_per_mesh, xf, which, mask=_abi.MASK_ALL |
         inst[i]["mask"] = descs[r]["mask"]
         xform[i] = descs[r]["object_to_world"].reshape(-1)
     return dict(tlas=tl.nodes, inst=inst, xform=xform, blases=blases, shading=shading, materials=mats_per_mesh,
-                descs=descs, boxes=boxes)
+                descs=descs, boxes=boxes, refs=refs, tlas_order=tl.ref_idx)
 
 
 def small_c2(n_meshes=4, n_inst=48, salt=0, extent=30.0, height=8.0):
@@ -75,3 +76,29 @@ def product_scene(meshes, normals, xf, which, boxes):
     w = atlas_rt.World()
     w.update(descs, blases, world_aabbs=boxes)
     return w, blases
+
+
+def quad_xy(z, size=1.0, normal_z=-1.0):
+    """Two triangles spanning [-size, size]^2 in the plane z, vertex normals
+    (0, 0, normal_z)."""
+    s = size
+    a, b, c, d = [-s, -s, z], [s, -s, z], [s, s, z], [-s, s, z]
+    verts = np.array([a + c + b, a + d + c], dtype=np.float32)
+    normals = np.tile(np.array([0, 0, normal_z] * 3, np.float32), (2, 1))
+    return verts, normals
+
+
+def sphere_uvs(verts):
+    """Per-triangle (uv0, uv1, uv2) from each vertex's direction (longitude,
+    colatitude), (n, 6) float32."""
+    p = verts.reshape(-1, 3).astype(np.float64)
+    r = np.linalg.norm(p, axis=1) + 1e-12
+    u = np.arctan2(p[:, 2], p[:, 0]) / (2 * np.pi) + 0.5
+    v = np.arccos(np.clip(p[:, 1] / r, -1, 1)) / np.pi
+    return np.stack([u, v], axis=1).reshape(-1, 6).astype(np.float32)
+
+
+def checker(n=8, lo=0, hi=255):
+    """(n, n) uint8 checkerboard."""
+    i, j = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    return np.where((i + j) % 2 == 0, hi, lo).astype(np.uint8)
