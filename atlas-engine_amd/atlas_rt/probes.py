@@ -27,7 +27,7 @@ import numpy as np
 COPY_BYTES = 256 << 20
 
 
-def pt(mode):
+def pt(mode, stride=32, steps=2):
     import torch
 
     from . import PathTracer, RadianceCache
@@ -42,7 +42,7 @@ def pt(mode):
     if mode == "cache":
         rc = RadianceCache((-20, -20, -20), (20, 20, 20), levels=12, log2_table=19, base_resolution=16,
                            max_resolution=2048)
-        p.set_cache(rc, train_stride=32, train_steps=2)
+        p.set_cache(rc, train_stride=stride, train_steps=steps)
     p.render(cam, 4)
     torch.cuda.synchronize()
     t = time.perf_counter()
@@ -141,11 +141,11 @@ def summarize_pmc(out_dir):
 if __name__ == "__main__":
     cmd = sys.argv[1] if len(sys.argv) > 1 else ""
     if cmd == "pt":
-        pt(sys.argv[2] if len(sys.argv) > 2 else "plain")
+        pt(sys.argv[2] if len(sys.argv) > 2 else "plain", *[int(a) for a in sys.argv[3:5]])
     elif cmd == "nrc":
         nrc()
     elif cmd == "pmc":
-        pmc()
+        pmc(int(sys.argv[2]) if len(sys.argv) > 2 else 2)
     elif cmd == "summarize-pmc":
         print(json.dumps(summarize_pmc(sys.argv[2]), indent=1))
     else:

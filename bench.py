@@ -55,15 +55,21 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_blas_closest_persistent_c0_pmc.md")
+# Hardware-counter summary of one traversal variant (kernel id -> file): the
+# traffic it reports belongs to that kernel only.
+PMC_SUMMARIES = {18: os.path.join(ROOT, "profiles", "k_blas_closest_persistent_regstack_c0_pmc.md")}
 
 
-def pmc_traffic():
-    """HBM traffic per launch of the persistent C0 kernel from the committed
+def pmc_traffic(kernel):
+    """HBM traffic per launch of the selected C0 kernel from the committed
     FETCH_SIZE / WRITE_SIZE runs (calibrated against a copy kernel), plus the
-    L2 hit rate and VALU busy fraction of the same runs; None if absent."""
+    L2 hit rate and VALU busy fraction of the same runs; None if that kernel
+    has no counter summary."""
+    path = PMC_SUMMARIES.get(kernel)
+    if path is None:
+        return None
     try:
-        text = open(PMC_SUMMARY).read()
+        text = open(path).read()
     except OSError:
         return None
     import re
@@ -72,6 +78,7 @@ def pmc_traffic():
     if not m:
         return None
     out["traffic"] = int(m.group(1))
+    out["source"] = os.path.relpath(path, ROOT) + " (FETCH_SIZE + WRITE_SIZE, copy-calibrated)"
     m = re.search(r"L2 \(TCC\) hit rate: ([0-9.]+)%", text)
     out["l2_hit_rate"] = float(m.group(1)) / 100 if m else None
     m = re.search(r"VALU busy: .*= ([0-9.]+)%", text)
@@ -224,10 +231,10 @@ def main():
         "build_s": round(build_s, 3),
         "hit_fraction": round(float((hits["prim_id"] >= 0).mean()), 4),
     }
-    pmc = pmc_traffic()
-    if pmc and kernel == 2:
+    pmc = pmc_traffic(kernel)
+    if pmc:
         out["roofline"]["traffic"] = pmc["traffic"]
-        out["roofline"]["traffic_source"] = "profiles/k_blas_closest_persistent_c0_pmc.md (FETCH_SIZE + WRITE_SIZE, copy-calibrated)"
+        out["roofline"]["traffic_source"] = pmc["source"]
         out["roofline"]["traffic_GBps"] = round(pmc["traffic"] / (kernel_ms / 1000.0) / 1e9, 1)
         out["roofline"]["l2_hit_rate"] = pmc["l2_hit_rate"]
         out["roofline"]["valu_busy"] = pmc["valu_busy"]
