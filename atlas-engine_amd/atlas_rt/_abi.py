@@ -28,6 +28,7 @@ KERNEL_BVH4 = 3
 KERNEL_BVH8 = 4
 KERNEL_VARIANT_MASK = 0xF
 KERNEL_STACK_REGISTER = 0x10
+KERNEL_SLAB_FLOAT = 0x20
 
 ANY_ENGINE_HITANY = 0
 ANY_RESTORE_ALWAYS = 1

@@ -25,6 +25,7 @@ struct BatchLaunch {
     bool persistent = true;
     int node_w = 2;       // 2, 4 or 8 children per node
     bool reg = false;     // register-topped traversal stack
+    bool fslab = false;   // widened float slab test (binary nodes)
     uint32_t block = 256;
     uint32_t refill = 40;
     hipStream_t stream = nullptr;

@@ -38,10 +38,17 @@ struct DevRay {
     float ox, oy, oz;
     float dx, dy, dz;
     double rx, ry, rz;  // 1 / d, correctly rounded double
+    float fx, fy, fz;   // 1 / d rounded to float (the widened float slab test)
     float tmin;
+    bool fast;          // every |d| is 0 or in [2^-126, 2^126]: fx, fy, fz are accurate
 };
 
 __device__ __forceinline__ float qdiv(float a, double r) { return (float)((double)a * r); }
+
+__device__ __forceinline__ bool fast_component(float d) {
+    const float a = fabsf(d);
+    return d == 0.0f || (a >= 1.1754943508222875e-38f && a <= 8.507059173023462e+37f);
+}
 
 __device__ __forceinline__ void make_dev_ray(DevRay& R, float ox, float oy, float oz, float dx, float dy, float dz,
                                              float tmin) {
@@ -54,7 +61,11 @@ __device__ __forceinline__ void make_dev_ray(DevRay& R, float ox, float oy, floa
     R.rx = 1.0 / (double)dx;
     R.ry = 1.0 / (double)dy;
     R.rz = 1.0 / (double)dz;
+    R.fx = (float)R.rx;
+    R.fy = (float)R.ry;
+    R.fz = (float)R.rz;
     R.tmin = tmin;
+    R.fast = fast_component(dx) && fast_component(dy) && fast_component(dz);
 }
 
 // Slab test against one box; returns hit and writes the engine's `dist`.
@@ -73,6 +84,41 @@ __device__ __forceinline__ bool slab_dist(const DevRay& R, float lx, float ly, f
     const bool hit = tminf <= tmaxf;
     dist = hit ? tminf : tmax;
     return hit;
+}
+
+// Widened float slab test of one inner node (the SLAB_FLOAT kernel variants).
+// A slab distance computed as (b - o) * RN(1/d) is within 2^-22 of its
+// magnitude of the correctly rounded (b - o) / d when 1/d is a normal float;
+// min / max keep that bound for the entry and exit distances.  A box counts
+// as hit when entry - exit <= 2^-20 (|entry| + |exit|), so every box the
+// exact test enters is entered here too (a few more may be); infinities are
+// the exact ones and never widen.  Child order uses the float distances.
+// Triangles stay exact, so the nearest hit is the exact traversal's; only
+// which of several triangles at exactly the same t wins may differ, and a
+// triangle whose box the exact test culls at a face may be found.
+__device__ __forceinline__ void node_test_float(const DevRay& R, const float4& a, const float4& b, const float4& c,
+                                                float tmax, bool& hl, bool& hr, bool& left_first) {
+    if (!R.fast) {
+        float dl, dr;
+        hl = slab_dist(R, a.x, a.y, a.z, a.w, b.x, b.y, tmax, dl);
+        hr = slab_dist(R, b.z, b.w, c.x, c.y, c.z, c.w, tmax, dr);
+        left_first = dl <= dr;
+        return;
+    }
+    const float l0x = (a.x - R.ox) * R.fx, l0y = (a.y - R.oy) * R.fy, l0z = (a.z - R.oz) * R.fz;
+    const float l1x = (a.w - R.ox) * R.fx, l1y = (b.x - R.oy) * R.fy, l1z = (b.y - R.oz) * R.fz;
+    const float r0x = (b.z - R.ox) * R.fx, r0y = (b.w - R.oy) * R.fy, r0z = (c.x - R.oz) * R.fz;
+    const float r1x = (c.y - R.ox) * R.fx, r1y = (c.z - R.oy) * R.fy, r1z = (c.w - R.oz) * R.fz;
+    const float lo_l = fmaxf(fmaxf(R.tmin, fminf(l0x, l1x)), fmaxf(fminf(l0y, l1y), fminf(l0z, l1z)));
+    const float hi_l = fminf(fminf(tmax, fmaxf(l0x, l1x)), fminf(fmaxf(l0y, l1y), fmaxf(l0z, l1z)));
+    const float lo_r = fmaxf(fmaxf(R.tmin, fminf(r0x, r1x)), fmaxf(fminf(r0y, r1y), fminf(r0z, r1z)));
+    const float hi_r = fminf(fminf(tmax, fmaxf(r0x, r1x)), fminf(fmaxf(r0y, r1y), fmaxf(r0z, r1z)));
+    constexpr float kWiden = 9.5367431640625e-07f;   // 2^-20
+    constexpr float kFloor = 7.52316384526264e-37f;   // 2^-120, for subnormal distances
+    constexpr float kBig = 3.4028234663852886e+38f;   // FLT_MAX: keeps infinities out of the margin
+    hl = lo_l - hi_l <= kWiden * (fminf(fabsf(lo_l), kBig) + fminf(fabsf(hi_l), kBig)) + kFloor;
+    hr = lo_r - hi_r <= kWiden * (fminf(fabsf(lo_r), kBig) + fminf(fabsf(hi_r), kBig)) + kFloor;
+    left_first = (hl ? lo_l : tmax) <= (hr ? lo_r : tmax);
 }
 
 // Triangle test on the precomputed layout (v0, e0 = v1 - v0, e1 = v2 - v0; the

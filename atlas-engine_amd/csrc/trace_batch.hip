@@ -38,7 +38,7 @@ __device__ __forceinline__ uint32_t find_batch(const uint32_t* __restrict__ pref
     return lo;
 }
 
-template <int BLOCK, int LDS_DEPTH, int NODE_W, bool REG>
+template <int BLOCK, int LDS_DEPTH, int NODE_W, bool REG, bool FLOAT_SLAB>
 __global__ __launch_bounds__(BLOCK) void k_batch_persistent(BatchDev D, const uint32_t* __restrict__ d_count,
                                                             const float4* __restrict__ rays,
                                                             float4* __restrict__ hits, int32_t* __restrict__ spill,
@@ -91,7 +91,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_persistent(BatchDev D, const ui
         const uint32_t threshold = exhausted ? 1u : refill;
         for (;;) {
             if (active) {
-                const int r = step_variant<NODE_W, REG, BLOCK, LDS_DEPTH, false>(B, L, S, cap, dn, dt);
+                const int r = step_variant<NODE_W, REG, BLOCK, LDS_DEPTH, false, FLOAT_SLAB>(B, L, S, cap, dn, dt);
                 if (r != kContinue) {
                     if (r == kOverflow) {
                         atomicOr(&ctl[2], 1u);
@@ -114,7 +114,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_persistent(BatchDev D, const ui
     }
 }
 
-template <int BLOCK, int LDS_DEPTH, int NODE_W, bool REG>
+template <int BLOCK, int LDS_DEPTH, int NODE_W, bool REG, bool FLOAT_SLAB>
 __global__ __launch_bounds__(BLOCK) void k_batch_grid(BatchDev D, const uint32_t* __restrict__ d_count,
                                                       const float4* __restrict__ rays, float4* __restrict__ hits,
                                                       int32_t* __restrict__ spill, uint32_t cap,
@@ -137,7 +137,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_grid(BatchDev D, const uint32_t
         uint32_t dn = 0, dt = 0;
         int r;
         do {
-            r = step_variant<NODE_W, REG, BLOCK, LDS_DEPTH, false>(B, L, S, cap, dn, dt);
+            r = step_variant<NODE_W, REG, BLOCK, LDS_DEPTH, false, FLOAT_SLAB>(B, L, S, cap, dn, dt);
         } while (r == kContinue);
         if (r == kOverflow) {
             atomicOr(&ctl[2], 1u);
@@ -153,16 +153,18 @@ struct Rows {
     static constexpr int value = BLOCK >= 256 ? 24 : 32;
 };
 
-template <int BLOCK, int NODE_W, bool REG>
+template <int BLOCK, int NODE_W, bool REG, bool FLOAT_SLAB = false>
 void launch_kernel(atlas_rt_batch_set& s, const BatchLaunch& L, uint32_t grid, uint32_t cap) {
     constexpr int D = Rows<BLOCK>::value;
     uint32_t* ctl = reinterpret_cast<uint32_t*>(s.flags.ptr);
     if (L.persistent)
-        hipLaunchKernelGGL((k_batch_persistent<BLOCK, D, NODE_W, REG>), dim3(grid), dim3(BLOCK), 0, L.stream, s.dev(),
+        hipLaunchKernelGGL((k_batch_persistent<BLOCK, D, NODE_W, REG, FLOAT_SLAB>), dim3(grid), dim3(BLOCK), 0,
+                           L.stream, s.dev(),
                            L.d_count, reinterpret_cast<const float4*>(L.rays), reinterpret_cast<float4*>(L.hits),
                            reinterpret_cast<int32_t*>(s.spill.ptr), cap, ctl, L.refill);
     else
-        hipLaunchKernelGGL((k_batch_grid<BLOCK, D, NODE_W, REG>), dim3(grid), dim3(BLOCK), 0, L.stream, s.dev(),
+        hipLaunchKernelGGL((k_batch_grid<BLOCK, D, NODE_W, REG, FLOAT_SLAB>), dim3(grid), dim3(BLOCK), 0, L.stream,
+                           s.dev(),
                            L.d_count, reinterpret_cast<const float4*>(L.rays), reinterpret_cast<float4*>(L.hits),
                            reinterpret_cast<int32_t*>(s.spill.ptr), cap, ctl);
 }
@@ -185,7 +187,10 @@ atlas_rt_status launch_one(atlas_rt_batch_set& s, const BatchLaunch& L) {
     }
     if (!L.persistent && L.node_w != 2)
         return fail(ATLAS_RT_E_UNSUPPORTED, "batched trace: wide nodes run in resident waves only");
-    if (L.node_w == 2)
+    if (L.node_w == 2 && L.fslab)
+        L.reg ? launch_kernel<BLOCK, 2, true, true>(s, L, grid, cap)
+              : launch_kernel<BLOCK, 2, false, true>(s, L, grid, cap);
+    else if (L.node_w == 2)
         L.reg ? launch_kernel<BLOCK, 2, true>(s, L, grid, cap) : launch_kernel<BLOCK, 2, false>(s, L, grid, cap);
     else if (L.node_w == 4)
         L.reg ? launch_kernel<BLOCK, 4, true>(s, L, grid, cap) : launch_kernel<BLOCK, 4, false>(s, L, grid, cap);

@@ -25,7 +25,7 @@ namespace {
 
 constexpr int kWave = 64;
 
-template <int BLOCK, int LDS_DEPTH, bool STATS, int NODE_W = 2, bool REG = false>
+template <int BLOCK, int LDS_DEPTH, bool STATS, int NODE_W = 2, bool REG = false, bool FLOAT_SLAB = false>
 __global__ __launch_bounds__(BLOCK) void k_blas_closest(BlasDev B, const float4* __restrict__ rays,
                                                         float4* __restrict__ hits, int32_t* __restrict__ slots,
                                                         uint2* __restrict__ visits, uint32_t count,
@@ -44,7 +44,7 @@ __global__ __launch_bounds__(BLOCK) void k_blas_closest(BlasDev B, const float4*
         const LaneStack<BLOCK, LDS_DEPTH> S{lds, spill, threadIdx.x, gid, gridDim.x * BLOCK};
         int r;
         do {
-            r = step_variant<NODE_W, REG, BLOCK, LDS_DEPTH, STATS>(B, L, S, cap, n_nodes, n_tris);
+            r = step_variant<NODE_W, REG, BLOCK, LDS_DEPTH, STATS, FLOAT_SLAB>(B, L, S, cap, n_nodes, n_tris);
         } while (r == kContinue);
         if (r == kOverflow) {
             atomicOr(&ctl[2], 1u);
@@ -56,7 +56,7 @@ __global__ __launch_bounds__(BLOCK) void k_blas_closest(BlasDev B, const float4*
     if (STATS) visits[gid] = make_uint2(n_nodes, n_tris);
 }
 
-template <int BLOCK, int LDS_DEPTH, int NODE_W = 2, bool REG = false>
+template <int BLOCK, int LDS_DEPTH, int NODE_W = 2, bool REG = false, bool FLOAT_SLAB = false>
 __global__ __launch_bounds__(BLOCK) void k_blas_closest_persistent(BlasDev B, const float4* __restrict__ rays,
                                                                    float4* __restrict__ hits,
                                                                    int32_t* __restrict__ slots, uint32_t count,
@@ -105,7 +105,7 @@ __global__ __launch_bounds__(BLOCK) void k_blas_closest_persistent(BlasDev B, co
         const uint32_t threshold = exhausted ? 1u : refill;
         for (;;) {
             if (active) {
-                const int r = step_variant<NODE_W, REG, BLOCK, LDS_DEPTH, false>(B, L, S, cap, dn, dt);
+                const int r = step_variant<NODE_W, REG, BLOCK, LDS_DEPTH, false, FLOAT_SLAB>(B, L, S, cap, dn, dt);
                 if (r != kContinue) {
                     if (r == kOverflow) {
                         atomicOr(&ctl[2], 1u);
@@ -208,10 +208,11 @@ static atlas_rt_status prepare(atlas_rt_blas& blas, uint32_t threads, uint32_t l
     return ATLAS_RT_OK;
 }
 
-template <int NODE_W, bool REG>
+template <int NODE_W, bool REG, bool FLOAT_SLAB = false>
 static void launch_persistent(const BlasDev& B, const TraceLaunch& L, uint32_t groups, int32_t* spill, uint32_t cap,
                               uint32_t* ctl) {
-    hipLaunchKernelGGL((k_blas_closest_persistent<kPBlock, kPLdsDepth, NODE_W, REG>), dim3(groups), dim3(kPBlock), 0,
+    hipLaunchKernelGGL((k_blas_closest_persistent<kPBlock, kPLdsDepth, NODE_W, REG, FLOAT_SLAB>), dim3(groups),
+                       dim3(kPBlock), 0,
                        L.stream, B, reinterpret_cast<const float4*>(L.rays), reinterpret_cast<float4*>(L.hits),
                        L.slots, L.count, spill, cap, ctl, L.refill ? L.refill : 40u);
 }
@@ -222,6 +223,7 @@ atlas_rt_status launch_blas_closest(atlas_rt_blas& blas, const TraceLaunch& L) {
     uint32_t* ctl = reinterpret_cast<uint32_t*>(blas.flags.ptr);
     const int base = L.kernel & ATLAS_RT_KERNEL_VARIANT_MASK;
     const bool reg = (L.kernel & ATLAS_RT_KERNEL_STACK_REGISTER) != 0;
+    const bool fslab = (L.kernel & ATLAS_RT_KERNEL_SLAB_FLOAT) != 0;
     const bool persistent = !L.visits && base >= ATLAS_RT_KERNEL_BVH2_PERSISTENT;
     const int node_w = base == ATLAS_RT_KERNEL_BVH4 ? 4 : (base == ATLAS_RT_KERNEL_BVH8 ? 8 : 2);
     atlas_rt_status st;
@@ -237,7 +239,10 @@ atlas_rt_status launch_blas_closest(atlas_rt_blas& blas, const TraceLaunch& L) {
         st = blas.timing.begin(L.stream);
         if (st != ATLAS_RT_OK) return st;
         int32_t* spill = reinterpret_cast<int32_t*>(blas.spill.ptr);
-        if (node_w == 2)
+        if (node_w == 2 && fslab)
+            reg ? launch_persistent<2, true, true>(B, L, groups, spill, cap, ctl)
+                : launch_persistent<2, false, true>(B, L, groups, spill, cap, ctl);
+        else if (node_w == 2)
             reg ? launch_persistent<2, true>(B, L, groups, spill, cap, ctl)
                 : launch_persistent<2, false>(B, L, groups, spill, cap, ctl);
         else if (node_w == 4)
@@ -246,16 +251,25 @@ atlas_rt_status launch_blas_closest(atlas_rt_blas& blas, const TraceLaunch& L) {
         else
             reg ? launch_persistent<8, true>(B, L, groups, spill, cap, ctl)
                 : launch_persistent<8, false>(B, L, groups, spill, cap, ctl);
-    } else if (reg && !L.visits) {
+    } else if ((reg || fslab) && !L.visits) {
         const uint32_t grid = (L.count + kBlock - 1) / kBlock;
         st = prepare(blas, grid * kBlock, kLdsDepth, cap);
         if (st != ATLAS_RT_OK) return st;
         ctl = reinterpret_cast<uint32_t*>(blas.flags.ptr);
         st = blas.timing.begin(L.stream);
         if (st != ATLAS_RT_OK) return st;
-        hipLaunchKernelGGL((k_blas_closest<kBlock, kLdsDepth, false, 2, true>), dim3(grid), dim3(kBlock), 0, L.stream,
-                           B, reinterpret_cast<const float4*>(L.rays), reinterpret_cast<float4*>(L.hits), L.slots,
-                           nullptr, L.count, reinterpret_cast<int32_t*>(blas.spill.ptr), cap, ctl);
+        const auto* rays = reinterpret_cast<const float4*>(L.rays);
+        auto* hits = reinterpret_cast<float4*>(L.hits);
+        int32_t* spill = reinterpret_cast<int32_t*>(blas.spill.ptr);
+        if (reg && fslab)
+            hipLaunchKernelGGL((k_blas_closest<kBlock, kLdsDepth, false, 2, true, true>), dim3(grid), dim3(kBlock), 0,
+                               L.stream, B, rays, hits, L.slots, nullptr, L.count, spill, cap, ctl);
+        else if (reg)
+            hipLaunchKernelGGL((k_blas_closest<kBlock, kLdsDepth, false, 2, true>), dim3(grid), dim3(kBlock), 0,
+                               L.stream, B, rays, hits, L.slots, nullptr, L.count, spill, cap, ctl);
+        else
+            hipLaunchKernelGGL((k_blas_closest<kBlock, kLdsDepth, false, 2, false, true>), dim3(grid), dim3(kBlock), 0,
+                               L.stream, B, rays, hits, L.slots, nullptr, L.count, spill, cap, ctl);
     } else {
         const uint32_t grid = (L.count + kBlock - 1) / kBlock;
         st = prepare(blas, grid * kBlock, kLdsDepth, cap);

@@ -222,19 +222,25 @@ __device__ __forceinline__ int leaf_step_closest(const BlasDev& B, LaneState& L,
     return kContinue;
 }
 
-// Binary nodes, engine order, any stack placement.
-template <bool STATS, class STK>
+// Binary nodes, engine order, any stack placement.  FLOAT_SLAB: the widened
+// float slab test (node_test_float) instead of the engine-exact one.
+template <bool STATS, bool FLOAT_SLAB, class STK>
 __device__ __forceinline__ int step_closest_s(const BlasDev& B, LaneState& L, const STK& S, uint32_t cap,
                                               uint32_t& n_nodes, uint32_t& n_tris) {
     if (L.node < 0) return leaf_step_closest(B, L, S, n_tris, STATS);
     const float4* np = B.nodes + 4 * size_t(L.node);
     const float4 a = np[0], b = np[1], c = np[2], d = np[3];
     if (STATS) n_nodes++;
-    float dl, dr;
-    const bool hl = slab_dist(L.R, a.x, a.y, a.z, a.w, b.x, b.y, L.hitT, dl);
-    const bool hr = slab_dist(L.R, b.z, b.w, c.x, c.y, c.z, c.w, L.hitT, dr);
+    bool hl, hr, leftFirst;
+    if (FLOAT_SLAB) {
+        node_test_float(L.R, a, b, c, L.hitT, hl, hr, leftFirst);
+    } else {
+        float dl, dr;
+        hl = slab_dist(L.R, a.x, a.y, a.z, a.w, b.x, b.y, L.hitT, dl);
+        hr = slab_dist(L.R, b.z, b.w, c.x, c.y, c.z, c.w, L.hitT, dr);
+        leftFirst = dl <= dr;
+    }
     const int32_t lp = __float_as_int(d.x), rp = __float_as_int(d.y);
-    const bool leftFirst = dl <= dr;
     if (hl && hr) {
         if (L.sp >= cap) return kOverflow;
         S.push(L, leftFirst ? rp : lp);
@@ -331,15 +337,19 @@ __device__ __forceinline__ int step_closest_w(const BlasDev& B, LaneState& L, co
 }
 
 // One closest-hit step of the variant (node width NODE_W in {2, 4, 8};
-// REG: register-topped stack).  NODE_W = 2 without REG is the engine-order
+// REG: register-topped stack; FLOAT_SLAB: widened float slab test, binary
+// nodes only).  NODE_W = 2 without REG or FLOAT_SLAB is the engine-order
 // binary step above.
-template <int NODE_W, bool REG, int BLOCK, int LDS_DEPTH, bool STATS>
+template <int NODE_W, bool REG, int BLOCK, int LDS_DEPTH, bool STATS, bool FLOAT_SLAB = false>
 __device__ __forceinline__ int step_variant(const BlasDev& B, LaneState& L, const LaneStack<BLOCK, LDS_DEPTH>& S,
                                             uint32_t cap, uint32_t& n_nodes, uint32_t& n_tris) {
-    if constexpr (NODE_W == 2 && !REG)
+    static_assert(NODE_W == 2 || !FLOAT_SLAB, "the float slab test is for binary nodes");
+    if constexpr (NODE_W == 2 && !REG && !FLOAT_SLAB)
         return step_closest<BLOCK, LDS_DEPTH, STATS>(B, L, S, cap, n_nodes, n_tris);
+    else if constexpr (NODE_W == 2 && REG)
+        return step_closest_s<STATS, FLOAT_SLAB>(B, L, RegStack<BLOCK, LDS_DEPTH>{S}, cap, n_nodes, n_tris);
     else if constexpr (NODE_W == 2)
-        return step_closest_s<STATS>(B, L, RegStack<BLOCK, LDS_DEPTH>{S}, cap, n_nodes, n_tris);
+        return step_closest_s<STATS, FLOAT_SLAB>(B, L, MemStack<BLOCK, LDS_DEPTH>{S}, cap, n_nodes, n_tris);
     else if constexpr (REG)
         return step_closest_w<NODE_W, STATS>(B, L, RegStack<BLOCK, LDS_DEPTH>{S}, cap, n_nodes, n_tris);
     else

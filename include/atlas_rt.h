@@ -199,6 +199,13 @@ atlas_rt_status atlas_rt_blas_export(const atlas_rt_blas* blas, atlas_rt_gpu_bvh
 #define ATLAS_RT_KERNEL_BVH8 4
 #define ATLAS_RT_KERNEL_VARIANT_MASK 0xF
 #define ATLAS_RT_KERNEL_STACK_REGISTER 0x10
+/* With the binary kernels (1, 2): the widened float slab test.  Slab distances
+ * use the float reciprocal of the direction and a box counts as hit within
+ * 2^-20 of its entry / exit distances, so every box the engine-exact test
+ * enters is entered; triangles stay exact.  Same nearest hit as the engine
+ * order; which triangle wins an exact equal-t tie may differ, and a hit that
+ * the engine's slab test culls at a box face may be found. */
+#define ATLAS_RT_KERNEL_SLAB_FLOAT 0x20
 
 /* Closest hit, engine semantics of HitClosest (bvh.hsh:191-273, CheckLeafClosest
  * :44-72, IntersectAABB/IntersectTriangle intersections.hsh:18-58): accept
