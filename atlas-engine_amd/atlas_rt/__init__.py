@@ -303,6 +303,20 @@ def build_many(meshes, host_only=False):
     return [Blas(out[i]) for i in range(k)]
 
 
+def build_many_device(meshes, stream=None):
+    """GPU linear-BVH build (atlas_rt_blas_build_device) of one BLAS per (n, 9)
+    vertex array; returns (blases, GPU milliseconds of the build kernels)."""
+    arrs = [np.ascontiguousarray(v, dtype=np.float32).reshape(-1, 9) for v in meshes]
+    k = len(arrs)
+    ptrs = (C.c_void_p * k)(*[a.ctypes.data for a in arrs])
+    counts = np.array([len(a) for a in arrs], dtype=np.uint32)
+    out = (C.c_void_p * k)()
+    ms = C.c_float()
+    check(lib().atlas_rt_blas_build_device(C.cast(ptrs, C.c_void_p), _ptr(counts), k, 0, _stream_ptr(stream),
+                                           C.cast(out, C.c_void_p), C.byref(ms)))
+    return [Blas(out[i]) for i in range(k)], ms.value
+
+
 class BatchSet:
     """Many (BLAS, ray range) pairs traced by one persistent launch."""
 

@@ -159,6 +159,7 @@ PROTOTYPES = {
     "atlas_rt_aabb_tree_build": (status, [vp, u32, u32, vp, C.POINTER(u32), vp, vp, C.POINTER(u32)]),
     "atlas_rt_blas_set_shading": (status, [vp, vp, vp]),
     "atlas_rt_blas_set_opacity": (status, [vp, vp, vp]),
+    "atlas_rt_blas_build_device": (status, [vp, vp, u32, u32, vp, vp, C.POINTER(f32)]),
     "atlas_rt_trace_closest_transparency": (status, [vp, vp, vp, vp, u32, f32, f32, u32, vp]),
     "atlas_rt_trace_transparency": (status, [vp, vp, vp, vp, u32, f32, f32, u32, vp]),
     "atlas_rt_pathtracer_set_textures": (status, [vp, vp, u32]),

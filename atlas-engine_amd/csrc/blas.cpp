@@ -128,7 +128,13 @@ atlas_rt_status upload_blas(atlas_rt_blas& b) {
     b.d_vtx = reinterpret_cast<float4*>(dv);
     b.device = dev;
     b.on_device = true;
-    // wide nodes for the 4- and 8-wide kernels
+    return upload_wide(b);
+}
+
+// Wide nodes for the 4- and 8-wide kernels, collapsed from the host copy.
+atlas_rt_status upload_wide(atlas_rt_blas& b) {
+    const size_t nt = b.tris.size();
+    hipError_t e = hipSuccess;
     std::vector<uint8_t> end_flags(nt);
     for (size_t i = 0; i < nt; i++) end_flags[i] = b.tris[i].v0[3] > 0.0f ? 1 : 0;
     for (int width : {4, 8}) {

@@ -114,5 +114,7 @@ atlas_rt_status launch_blas_any(atlas_rt_blas& blas, const TraceLaunch& L);
 
 // Upload the engine-layout arrays of `blas` to the current device.
 atlas_rt_status upload_blas(atlas_rt_blas& blas);
+// The 4- and 8-wide node arrays of a BLAS whose host copy is filled.
+atlas_rt_status upload_wide(atlas_rt_blas& blas);
 
 }  // namespace atlas_rt

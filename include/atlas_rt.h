@@ -257,6 +257,21 @@ atlas_rt_status atlas_rt_aabb_tree_build(const atlas_rt_aabb* aabbs, uint32_t co
 atlas_rt_status atlas_rt_blas_build_many(const atlas_rt_bvh_triangle* const* tris, const uint32_t* counts,
                                          uint32_t count, uint32_t flags, atlas_rt_blas** out);
 
+/* GPU builder (no engine counterpart; it serves the same callers as
+ * atlas_rt_blas_build_many: Mesh::BuildBVH jobs, editor/App.cpp:210-222).
+ * verts[k]: counts[k] triangles of 9 floats (v0, v1, v2) in the caller's
+ * order, host memory; at most 2^24 triangles per mesh.  A linear BVH (30-bit
+ * Morton codes over the centroid bounds, Karras radix tree), flattened like
+ * BVHBuilder::Flatten (BVH.cpp:408-441): pre-order, the larger-area child
+ * left, one triangle per leaf.  Same node and triangle format as
+ * atlas_rt_blas_build, so every trace entry point accepts the result; the
+ * tree differs from the SAH/SBVH one (nearest hits are the same, slots and
+ * traversal cost differ).  Host copies are mirrored for export.  device_ms
+ * (may be NULL) receives the GPU time of the build kernels, uploads excluded.
+ * Synchronous on `stream`. */
+atlas_rt_status atlas_rt_blas_build_device(const float* const* verts, const uint32_t* counts, uint32_t count,
+                                           uint32_t flags, void* stream, atlas_rt_blas** out, float* device_ms);
+
 /* ------------------------------------------------------------------------- */
 /* Batched traces: many independent BLAS / ray-batch pairs in one launch     */
 /* ------------------------------------------------------------------------- */

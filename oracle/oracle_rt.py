@@ -397,6 +397,36 @@ def pathtrace_cache(tlas_nodes, instances, xform, blases, shading, materials, li
     return rad[..., :3], query, train.astype(bool), rec
 
 
+def build_lbvh(verts):
+    """CPU restatement of the GPU linear-BVH builder: (nodes, tris, orig,
+    max_depth) in the engine's packed layout."""
+    verts = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1, 9)
+    n = len(verts)
+    nodes = np.zeros(max(n - 1, 0), dtype=NODE_DTYPE)
+    tris = np.zeros(n, dtype=GPU_TRI_DTYPE)
+    orig = np.zeros(n, dtype=np.uint32)
+    depth = np.zeros(1, dtype=np.uint32)
+    L = lib()
+    if not getattr(L, "_lbvh_bound", False):
+        L.oracle_lbvh_build.restype = None
+        L.oracle_lbvh_build.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.oracle_radix_tree.restype = None
+        L.oracle_radix_tree.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p]
+        L._lbvh_bound = True
+    L.oracle_lbvh_build(_p(verts), n, _p(nodes) if len(nodes) else None, _p(tris), _p(orig), _p(depth))
+    return nodes, tris, orig, int(depth[0])
+
+
+def radix_tree(keys):
+    """Children (n - 1, 2) of the binary radix tree of sorted distinct uint64
+    keys (Karras' per-node search): >= 0 inner node, < 0 ~leaf."""
+    keys = np.ascontiguousarray(keys, dtype=np.uint64)
+    out = np.zeros((max(len(keys) - 1, 0), 2), dtype=np.int32)
+    build_lbvh(np.zeros((1, 9), np.float32))  # binds the entry points
+    lib().oracle_radix_tree(_p(keys), len(keys), _p(out) if len(out) else None)
+    return out
+
+
 def sample_opacity(texels, x, y):
     """Bilinear, repeat-addressed sample of an (h, w) uint8 opacity texture."""
     texels = np.ascontiguousarray(texels, dtype=np.uint8)
