@@ -5,14 +5,17 @@ one --pmc counter per run).  Run from the package directory's parent:
     PYTHONPATH=atlas-engine_amd rocprofv3 --pmc FETCH_SIZE -d out -o fetch -- python3 -m atlas_rt.probes pmc
     python3 -m atlas_rt.probes summarize-pmc out
 
-  pt plain|cache  the dense radiance-cache scene (256 blob instances in a
+  pt plain|cache [stride steps]
+                  the dense radiance-cache scene (256 blob instances in a
                   30-unit cube) at 1280 x 720, 8 bounces: 4 untimed frames,
                   then 8 frames, plain or with the cache (1/32 training
-                  pixels, 2 Adam steps per frame)
+                  pixels, 2 Adam steps per frame, unless given)
   nrc             cache training steps at 32 .. 262,144 samples and
                   inference of 65,536 and 1,048,576 queries
-  pmc             a 256 MiB device-to-device copy (the FETCH_SIZE /
+  pmc [kernel]    a 256 MiB device-to-device copy (the FETCH_SIZE /
                   WRITE_SIZE calibration) and the C0 closest-hit trace, 4 each
+  lbvh            three GPU linear-BVH builds of the C3 meshes (512 meshes,
+                  ~7.9M triangles)
 """
 import csv
 import glob
@@ -138,12 +141,30 @@ def summarize_pmc(out_dir):
     return out
 
 
+def lbvh(reps=3):
+    """GPU linear-BVH builds of the C3 meshes (512 meshes, ~7.9M triangles)."""
+    import torch  # the HIP runtime comes up through torch, as in every other probe
+
+    torch.cuda.init()
+    from . import build_many_device
+    from . import workloads as W
+    meshes, _, _ = W.c3()
+    for r in range(reps):
+        t = time.perf_counter()
+        blases, ms = build_many_device(meshes)
+        wall = time.perf_counter() - t
+        print(f"lbvh build {r}: {ms:.2f} ms GPU, {wall * 1e3:.1f} ms wall", flush=True)
+        del blases
+
+
 if __name__ == "__main__":
     cmd = sys.argv[1] if len(sys.argv) > 1 else ""
     if cmd == "pt":
         pt(sys.argv[2] if len(sys.argv) > 2 else "plain", *[int(a) for a in sys.argv[3:5]])
     elif cmd == "nrc":
         nrc()
+    elif cmd == "lbvh":
+        lbvh()
     elif cmd == "pmc":
         pmc(int(sys.argv[2]) if len(sys.argv) > 2 else 2)
     elif cmd == "summarize-pmc":

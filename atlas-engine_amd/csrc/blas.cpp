@@ -269,13 +269,15 @@ atlas_rt_status atlas_rt_blas_create(const atlas_rt_gpu_bvh_node* nodes, uint32_
 
 atlas_rt_status atlas_rt_blas_destroy(atlas_rt_blas* b) {
     if (!b) return ATLAS_RT_OK;
-    if (b->d_nodes) hipFree(b->d_nodes);
-    if (b->d_tris) hipFree(b->d_tris);
-    if (b->d_vtx) hipFree(b->d_vtx);
+    if (!b->arena) {
+        if (b->d_nodes) hipFree(b->d_nodes);
+        if (b->d_tris) hipFree(b->d_tris);
+        if (b->d_vtx) hipFree(b->d_vtx);
+        if (b->d_nodes4) hipFree(b->d_nodes4);
+        if (b->d_nodes8) hipFree(b->d_nodes8);
+    }
     if (b->d_shade) hipFree(b->d_shade);
     if (b->d_uv) hipFree(b->d_uv);
-    if (b->d_nodes4) hipFree(b->d_nodes4);
-    if (b->d_nodes8) hipFree(b->d_nodes8);
     b->spill.release();
     b->flags.release();
     b->timing.release();

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -88,6 +89,10 @@ struct atlas_rt_blas {
     float4* d_nodes4 = nullptr;
     float4* d_nodes8 = nullptr;
     uint32_t cap4 = 1, cap8 = 1;
+    //  arena: set for BLASes of one device build, whose d_nodes, d_tris,
+    //  d_vtx, d_nodes4 and d_nodes8 are slices of shared allocations that the
+    //  last BLAS of the batch frees.
+    std::shared_ptr<void> arena;
     atlas_rt::Scratch spill;
     atlas_rt::Scratch flags;  // [0] ray counter, [1] finished waves, [2] stack overflow flag
     atlas_rt::LaunchTiming timing;

@@ -177,11 +177,20 @@ def test_gpu_lbvh_equals_oracle(torch_cuda):
         assert b.info()[2] == depth, (len(verts), b.info(), depth)
 
 
+def _fp32_t(rays, tris, i, slot):
+    tr = tris[slot]
+    hit, sol = oracle_rt.triangle_test(rays[i]["origin"], rays[i]["direction"], tr["v0"][:3], tr["v1"][:3],
+                                       tr["v2"][:3])
+    return sol[0] if hit else np.inf
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("kernel", [1, 2, 2 | 0x10])
+@pytest.mark.parametrize("kernel", [1, 2, 2 | 0x10, 3, 4 | 0x10])
 def test_gpu_lbvh_traces_match_oracle(torch_cuda, kernel):
     """The engine-order kernels on a device-built tree give the oracle's
-    traversal result on the same tree, bit for bit."""
+    traversal result on the same tree, bit for bit; the wide kernels (over the
+    wide nodes collapsed from the device-built tree) find every engine hit at
+    a float32 t no larger, with identical records where the slot agrees."""
     print(f"case lbvh traces kernel={kernel}", flush=True)
     torch = torch_cuda
     verts, rays = rt_cases.soup(65536, 262144, 120)
@@ -193,9 +202,20 @@ def test_gpu_lbvh_traces_match_oracle(torch_cuda, kernel):
     d_slots = torch.empty(len(rays), dtype=torch.int32, device="cuda")
     b.trace_closest(d_rays, d_hits, d_slots, kernel=kernel)
     torch.cuda.synchronize()
-    assert d_hits.cpu().numpy().tobytes() == hr.tobytes()
-    assert np.array_equal(d_slots.cpu().numpy(), sr)
     assert (sr >= 0).mean() > 0.3
+    h = d_hits.cpu().numpy().view(atlas_rt.HIT_DTYPE).reshape(-1)
+    s = d_slots.cpu().numpy()
+    if kernel & 0xF in (1, 2):
+        assert h.tobytes() == hr.tobytes()
+        assert np.array_equal(s, sr)
+        return
+    assert np.all(s[sr >= 0] >= 0), "wide traversal lost a hit"
+    diff = np.nonzero((s != sr) & (sr >= 0))[0]
+    for i in diff:
+        assert _fp32_t(rays, tris, i, s[i]) <= _fp32_t(rays, tris, i, sr[i]), i
+    same = (s == sr) & (sr >= 0)
+    assert np.array_equal(rt_cases.hit_bits(h[same]), rt_cases.hit_bits(hr[same]))
+    assert len(diff) < 0.001 * len(rays)
 
 
 @pytest.mark.gpu
