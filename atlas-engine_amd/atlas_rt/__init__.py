@@ -303,6 +303,20 @@ def build_many(meshes, host_only=False):
     return [Blas(out[i]) for i in range(k)]
 
 
+def sort_rays_by_direction(d_rays, d_out, d_perm=None, d_bin_offsets=None, count=None, stream=None):
+    """Stable sort of (n, 8) float32 device rays into d_out by the octahedral
+    bin of their direction (atlas_rt_sort_rays_by_direction); optional int32
+    permutation (d_out[i] = d_rays[d_perm[i]]) and 65 bin offsets."""
+    import torch
+    n = d_rays.shape[0] if count is None else count
+    need = C.c_size_t()
+    check(lib().atlas_rt_sort_rays_workspace(n, C.byref(need)))
+    ws = torch.empty(max(need.value, 4), dtype=torch.uint8, device=d_rays.device)
+    check(lib().atlas_rt_sort_rays_by_direction(_ptr(d_rays), n, _ptr(d_out), _ptr(d_perm), _ptr(d_bin_offsets),
+                                                _ptr(ws), need.value, _stream_ptr(stream)))
+    return ws
+
+
 def build_many_device(meshes, stream=None):
     """GPU linear-BVH build (atlas_rt_blas_build_device) of one BLAS per (n, 9)
     vertex array; returns (blases, GPU milliseconds of the build kernels)."""
@@ -424,8 +438,10 @@ def camera(eye, yaw_deg, pitch_deg, fov_y_deg, aspect, near=0.1):
 class PathTracer:
     """Progressive wavefront path tracer over a World (PathTracingRenderer)."""
 
-    def __init__(self, world, width, height, bounces, materials, lights, env=(0.0, 0.0, 0.0), exposure=1.0):
-        p = _abi.PtParams(width, height, bounces, exposure, (C.c_float * 3)(*env), 0)
+    def __init__(self, world, width, height, bounces, materials, lights, env=(0.0, 0.0, 0.0), exposure=1.0,
+                 binning=False):
+        p = _abi.PtParams(width, height, bounces, exposure, (C.c_float * 3)(*env),
+                          _abi.PT_RAY_BINNING if binning else 0)
         self._mats = np.ascontiguousarray(materials, dtype=_abi.MATERIAL_DTYPE)
         self._lights = np.ascontiguousarray(lights, dtype=_abi.LIGHT_DTYPE)
         self._world = world

@@ -30,6 +30,9 @@ KERNEL_VARIANT_MASK = 0xF
 KERNEL_STACK_REGISTER = 0x10
 KERNEL_SLAB_FLOAT = 0x20
 
+PT_RAY_BINNING = 0x1
+RAY_BINS = 64
+
 ANY_ENGINE_HITANY = 0
 ANY_RESTORE_ALWAYS = 1
 
@@ -135,6 +138,8 @@ PROTOTYPES = {
     "atlas_rt_blas_export": (status, [vp, vp, vp, vp]),
     "atlas_rt_trace_closest_rays": (status, [vp, vp, vp, vp, u32, i32, vp]),
     "atlas_rt_trace_closest_stats": (status, [vp, vp, vp, vp, vp, u32, vp]),
+    "atlas_rt_sort_rays_workspace": (status, [u32, C.POINTER(C.c_size_t)]),
+    "atlas_rt_sort_rays_by_direction": (status, [vp, u32, vp, vp, vp, vp, C.c_size_t, vp]),
     "atlas_rt_trace_any_rays": (status, [vp, vp, vp, u32, i32, vp]),
     "atlas_rt_trace_closest_host": (status, [vp, vp, vp, u32]),
     "atlas_rt_trace_any_host": (status, [vp, vp, vp, u32]),

@@ -4,6 +4,9 @@ in device memory before each timed region):
   c1           coherent primary rays over a 1024 x 1024 heightfield
                (2,093,058 triangles) at 1920 x 1080, then one any-hit shadow
                ray per hit toward a directional light: Mrays/s over both.
+  c2           two-level path trace, 256 instances of 8 meshes, 1280 x 720,
+               4 bounces, sun + constant environment: ms per frame (one
+               sample per pixel), without and with ray binning.
   c3           512 independent BLAS (1k-64k triangles, log-uniform) each with
                its own random ray batch (4k-256k rays, log-uniform), traced
                by one autotuned persistent launch and by its captured graph.
@@ -69,6 +72,38 @@ def c1(torch, log):
         "shadow_ms": round(ts * 1e3, 4),
         "mrays_per_s": round((n + n) / total / 1e6, 1),
         "note": "shadow batch keeps one ray per pixel; misses carry NaN directions and exit at once",
+    }
+
+
+def c2(torch, log, frames=16):
+    meshes, mats_d, xf, which = W.c2_scene()
+    w, blases = _world(meshes, xf, which)
+    cc = W.c2_camera()
+    cam = camera(cc["eye"], cc["yaw_deg"], cc["pitch_deg"], cc["fov_y_deg"], cc["width"] / cc["height"])
+    mats = W.c2_materials(mats_d)
+    res = {}
+    imgs = {}
+    for binning in (False, True):
+        pt = PathTracer(w, cc["width"], cc["height"], 4, mats, W.sun_light(), env=W.C2_ENV, binning=binning)
+        pt.render(cam, 2)
+        pt.reset()
+        t = _sync_time(torch, lambda: pt.render(cam, frames))
+        res[binning] = t / frames
+        imgs[binning] = pt.read().cpu().numpy()
+        del pt
+    same = bool(imgs[False].tobytes() == imgs[True].tobytes())
+    px = cc["width"] * cc["height"]
+    log(f"c2: {res[False] * 1e3:.3f} ms/frame plain, {res[True] * 1e3:.3f} ms/frame binned (same image: {same})")
+    return {
+        "workload": "c2: 256 instances of 8 random-blob meshes, 1280x720, 4 bounces, sun + constant env, "
+                    "one sample per pixel per frame",
+        "triangles": int(sum(len(m[0]) for m in meshes)),
+        "instanced_triangles": int(sum(len(meshes[k][0]) for k in which)), "instances": int(len(which)),
+        "ms_per_frame": round(res[False] * 1e3, 3),
+        "mpaths_per_s": round(px / res[False] / 1e6, 1),
+        "binned_ms_per_frame": round(res[True] * 1e3, 3),
+        "binning_speedup": round(res[False] / res[True], 3),
+        "binned_image_identical": same,
     }
 
 

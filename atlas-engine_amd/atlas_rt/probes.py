@@ -16,6 +16,9 @@ one --pmc counter per run).  Run from the package directory's parent:
                   WRITE_SIZE calibration) and the C0 closest-hit trace, 4 each
   lbvh            three GPU linear-BVH builds of the C3 meshes (512 meshes,
                   ~7.9M triangles)
+  c2 [binned]     the config-2 path trace (256 instances, 1280 x 720,
+                  4 bounces): 2 untimed frames, then 16, optionally with
+                  ray binning
 """
 import csv
 import glob
@@ -41,7 +44,8 @@ def pt(mode, stride=32, steps=2):
     cc = W.c4_camera()
     from . import camera
     cam = camera(cc["eye"], cc["yaw_deg"], cc["pitch_deg"], cc["fov_y_deg"], cc["width"] / cc["height"])
-    p = PathTracer(w, cc["width"], cc["height"], 8, W.c2_materials(mats_d), W.sun_light(), env=W.C2_ENV)
+    p = PathTracer(w, cc["width"], cc["height"], 8, W.c2_materials(mats_d), W.sun_light(), env=W.C2_ENV,
+                   binning=(mode == "binned"))
     if mode == "cache":
         rc = RadianceCache((-20, -20, -20), (20, 20, 20), levels=12, log2_table=19, base_resolution=16,
                            max_resolution=2048)
@@ -52,6 +56,28 @@ def pt(mode, stride=32, steps=2):
     p.render(cam, 8)
     torch.cuda.synchronize()
     print(f"pt {mode}: {(time.perf_counter() - t) / 8 * 1e3:.3f} ms/frame", flush=True)
+
+
+def c2(binning):
+    """Path tracing of the C2 scene (1280 x 720, 4 bounces), 16 frames."""
+    import torch
+
+    torch.cuda.init()
+    from . import PathTracer, camera
+    from . import workloads as W
+    from .bench_suite import _world
+    meshes, mats_d, xf, which = W.c2_scene()
+    w, blases = _world(meshes, xf, which)
+    cc = W.c2_camera()
+    cam = camera(cc["eye"], cc["yaw_deg"], cc["pitch_deg"], cc["fov_y_deg"], cc["width"] / cc["height"])
+    p = PathTracer(w, cc["width"], cc["height"], 4, W.c2_materials(mats_d), W.sun_light(), env=W.C2_ENV,
+                   binning=binning)
+    p.render(cam, 2)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    p.render(cam, 16)
+    torch.cuda.synchronize()
+    print(f"c2 binning={binning}: {(time.perf_counter() - t) / 16 * 1e3:.3f} ms/frame", flush=True)
 
 
 def nrc():
@@ -161,6 +187,8 @@ if __name__ == "__main__":
     cmd = sys.argv[1] if len(sys.argv) > 1 else ""
     if cmd == "pt":
         pt(sys.argv[2] if len(sys.argv) > 2 else "plain", *[int(a) for a in sys.argv[3:5]])
+    elif cmd == "c2":
+        c2(len(sys.argv) > 2 and sys.argv[2] == "binned")
     elif cmd == "nrc":
         nrc()
     elif cmd == "lbvh":

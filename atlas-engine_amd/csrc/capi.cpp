@@ -1,6 +1,7 @@
 // extern "C" trace entry points over one BLAS, and library-level queries.
 #include <cstring>
 
+#include "binning.h"
 #include "rt_internal.h"
 
 namespace atlas_rt {
@@ -58,6 +59,33 @@ atlas_rt_status atlas_rt_trace_closest_rays(const atlas_rt_blas* blas, const atl
     L.kernel = kernel;
     L.stream = reinterpret_cast<hipStream_t>(stream);
     return launch_blas_closest(*const_cast<atlas_rt_blas*>(blas), L);
+}
+
+atlas_rt_status atlas_rt_sort_rays_workspace(uint32_t count, size_t* bytes) {
+    if (!bytes) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_sort_rays_workspace: NULL argument");
+    *bytes = bin_sort_workspace(count);
+    return ATLAS_RT_OK;
+}
+
+atlas_rt_status atlas_rt_sort_rays_by_direction(const atlas_rt_ray* d_rays, uint32_t count, atlas_rt_ray* d_out,
+                                                uint32_t* d_perm, uint32_t* d_bin_offsets, void* d_workspace,
+                                                size_t workspace_bytes, void* stream) {
+    if (count && (!d_rays || !d_out || !d_workspace))
+        return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_sort_rays_by_direction: NULL buffer");
+    if (count && d_rays < d_out + count && d_out < d_rays + count)
+        return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_sort_rays_by_direction: input and output overlap");
+    if (workspace_bytes < bin_sort_workspace(count))
+        return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_sort_rays_by_direction: workspace smaller than "
+                                            "atlas_rt_sort_rays_workspace(count)");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (count == 0) {
+        if (d_bin_offsets) ATLAS_HIP_TRY(hipMemsetAsync(d_bin_offsets, 0, (kRayBins + 1) * sizeof(uint32_t), s));
+        return ATLAS_RT_OK;
+    }
+    const hipError_t e = launch_bin_sort(reinterpret_cast<const float4*>(d_rays), reinterpret_cast<float4*>(d_out), 2,
+                                         nullptr, count, count, d_workspace, d_perm, d_bin_offsets, s);
+    if (e != hipSuccess) return hip_fail(e, "atlas_rt_sort_rays_by_direction");
+    return ATLAS_RT_OK;
 }
 
 atlas_rt_status atlas_rt_trace_closest_stats(const atlas_rt_blas* blas, const atlas_rt_ray* d_rays,

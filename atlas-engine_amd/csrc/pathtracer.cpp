@@ -1,6 +1,7 @@
 // Path tracer objects: PathTracingRenderer's host side (buffers, light list,
 // per-frame seeds, progressive accumulation) over the wavefront kernels.
 #include "pathtracer.h"
+#include "binning.h"
 #include "nrc.h"
 
 #include <algorithm>
@@ -23,11 +24,12 @@ atlas_rt_status dev_alloc(T*& p, size_t count) {
 void free_all(PathTracer& P) {
     void* ptrs[] = {P.ctl, P.paths[0], P.paths[1], P.hits, P.shadow, P.Lpix, P.accum, P.xform,
                     const_cast<float4**>(P.shade_table), const_cast<float4**>(P.uv_table), P.materials, P.lights,
-                    P.tex, P.texels};
+                    P.tex, P.texels, P.binned};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     P.spill.release();
     P.flags.release();
+    P.bin_ws.release();
     free_cache_buffers(P);
 }
 
@@ -184,8 +186,11 @@ atlas_rt_status atlas_rt_pathtracer_create(const atlas_rt_world* world, const at
     for (uint32_t l = 0; l < light_count; l++)
         if (lights[l].type != ATLAS_RT_LIGHT_DIRECTIONAL)
             return fail(ATLAS_RT_E_UNSUPPORTED, "atlas_rt_pathtracer_create: only directional lights");
+    if (params->flags & ~ATLAS_RT_PT_RAY_BINNING)
+        return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_create: unknown flag bits");
     auto* pt = new atlas_rt_pathtracer();
     PathTracer& P = pt->P;
+    P.binning = (params->flags & ATLAS_RT_PT_RAY_BINNING) != 0;
     P.width = params->width;
     P.height = params->height;
     P.bounces = params->bounces;
@@ -211,6 +216,8 @@ atlas_rt_status atlas_rt_pathtracer_create(const atlas_rt_world* world, const at
     const uint32_t grid_threads = uint32_t((pixels + 127) / 128 * 128);
     if (st == ATLAS_RT_OK && P.cap > 32) st = P.spill.ensure(size_t(P.cap - 32) * grid_threads * sizeof(int32_t));
     if (st == ATLAS_RT_OK) st = P.flags.ensure(256);
+    if (st == ATLAS_RT_OK && P.binning) st = dev_alloc(P.binned, 3 * pixels);
+    if (st == ATLAS_RT_OK && P.binning) st = P.bin_ws.ensure(bin_sort_workspace(uint32_t(pixels)));
     if (st != ATLAS_RT_OK) {
         free_all(P);
         delete pt;

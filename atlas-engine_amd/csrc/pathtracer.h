@@ -58,6 +58,11 @@ struct PathTracer {
     uint32_t tex_count = 0;
     Scratch spill;
     Scratch flags;
+    // ray binning (ATLAS_RT_PT_RAY_BINNING): each bounce's paths sorted into
+    // `binned` by direction bin before the trace
+    bool binning = false;
+    float4* binned = nullptr;
+    Scratch bin_ws;
     ShadeArgs shade_args{};
     int32_t host_samples = 0;
     // radiance cache (null: plain path tracing)

@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include "alpha.h"
+#include "binning.h"
 #include "nrc.h"
 #include "pathtracer.h"
 #include "pt_device.h"
@@ -591,6 +592,13 @@ atlas_rt_status launch_frame(PathTracer& P, const Camera& cam, hipStream_t s) {
     for (int32_t b = 0; b <= P.bounces; b++) {
         float4* in = P.paths[b & 1];
         float4* out = P.paths[(b + 1) & 1];
+        if (P.binning) {
+            // coherence sort of this bounce's live paths (the engine sorts
+            // before each trace dispatch, RayTracingHelper.cpp:304-344)
+            ATLAS_HIP_TRY(launch_bin_sort(in, P.binned, 3, &P.ctl->paths[b], 0, pixels, P.bin_ws.ptr, nullptr,
+                                          nullptr, s));
+            in = P.binned;
+        }
         // with the cache only training paths outlive bounce 1
         const uint32_t grid = (C.enabled && b >= 2) ? (P.train_cap + kBlock - 1) / kBlock : grid_all;
         int32_t* spill = reinterpret_cast<int32_t*>(P.spill.ptr);

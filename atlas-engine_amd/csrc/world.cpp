@@ -48,7 +48,7 @@ static void transform_box(const float m[3][4], const atlas_rt_gpu_bvh_node* root
     const float cx[8] = {b.lo[0], b.lo[0], b.hi[0], b.hi[0], b.lo[0], b.lo[0], b.hi[0], b.hi[0]};
     const float cy[8] = {b.lo[1], b.lo[1], b.lo[1], b.lo[1], b.hi[1], b.hi[1], b.hi[1], b.hi[1]};
     const float cz[8] = {b.lo[2], b.hi[2], b.lo[2], b.hi[2], b.lo[2], b.hi[2], b.lo[2], b.hi[2]};
-    float lo[3], hi[3];
+    float lo[3] = {0.0f, 0.0f, 0.0f}, hi[3] = {0.0f, 0.0f, 0.0f};  // set at i == 0
     for (int i = 0; i < 8; i++) {
         float p[3];
         for (int r = 0; r < 3; r++)

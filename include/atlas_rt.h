@@ -220,6 +220,22 @@ atlas_rt_status atlas_rt_trace_closest_rays(const atlas_rt_blas* blas, const atl
  * algorithmic bytes of a trace: 64 B per node (GPUBVHNode) and 48 B per
  * triangle test (GPUBVHTriangle).  Used to price the roofline; the profiler
  * counters of this profiling launch are not representative of the plain one. */
+/* Ray coherence sorting (raytracer/binning.csh, binningOffset.csh; bin of
+ * raytracer/tracing.hsh:18-21): a stable counting sort of `count` device rays
+ * into d_out by the octahedral bin of their direction (common/octahedron.hsh:
+ * 23-37; 8 x 8 cells, bin = cy * 8 + cx).  The engine truncates a map
+ * coordinate of exactly 1 to cell 8, past its grid; here the cell is 7.  A
+ * NaN direction falls in bin 0.  The engine's order inside a bin comes from
+ * atomics; here rays keep their input order, so the result is deterministic.
+ * d_perm (optional): d_out[i] = d_rays[d_perm[i]], to map hits back.
+ * d_bin_offsets (optional, 65 entries): first output index of each bin, then
+ * count.  d_workspace: atlas_rt_sort_rays_workspace(count) bytes of device
+ * memory; the call allocates nothing.  Asynchronous on `stream`. */
+atlas_rt_status atlas_rt_sort_rays_workspace(uint32_t count, size_t* bytes);
+atlas_rt_status atlas_rt_sort_rays_by_direction(const atlas_rt_ray* d_rays, uint32_t count, atlas_rt_ray* d_out,
+                                                uint32_t* d_perm, uint32_t* d_bin_offsets, void* d_workspace,
+                                                size_t workspace_bytes, void* stream);
+
 atlas_rt_status atlas_rt_trace_closest_stats(const atlas_rt_blas* blas, const atlas_rt_ray* d_rays,
                                              atlas_rt_hit* d_hits, int32_t* d_slot, uint32_t* d_visits,
                                              uint32_t count, void* stream);
@@ -510,6 +526,11 @@ typedef struct atlas_rt_camera {
     float right[3];
     float bottom[3];
 } atlas_rt_camera;
+
+/* atlas_rt_pt_params.flags: sort every bounce's rays by direction bin before
+ * tracing (the engine's useRayBinning, RayTracingHelper.h:53, switched off
+ * there; see atlas_rt_sort_rays_by_direction).  The image is unchanged. */
+#define ATLAS_RT_PT_RAY_BINNING 0x1u
 
 typedef struct atlas_rt_pt_params {
     uint32_t width;

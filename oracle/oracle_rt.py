@@ -427,6 +427,40 @@ def radix_tree(keys):
     return out
 
 
+def _bind_binning():
+    L = lib()
+    if not getattr(L, "_bin_bound", False):
+        L.oracle_ray_bin.restype = C.c_uint32
+        L.oracle_ray_bin.argtypes = [C.c_float, C.c_float, C.c_float]
+        L.oracle_ray_bins.restype = None
+        L.oracle_ray_bins.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p]
+        L.oracle_sort_rays_by_direction.restype = None
+        L.oracle_sort_rays_by_direction.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]
+        L._bin_bound = True
+    return L
+
+
+def ray_bins(dirs):
+    """Octahedral direction bin (0..63) of each (n, 3) float32 direction."""
+    dirs = np.ascontiguousarray(dirs, dtype=np.float32).reshape(-1, 3)
+    out = np.zeros(len(dirs), dtype=np.uint32)
+    if len(dirs):
+        _bind_binning().oracle_ray_bins(_p(dirs), len(dirs), _p(out))
+    return out
+
+
+def sort_rays_by_direction(rays):
+    """(sorted rays, perm, 65 bin offsets): stable order by direction bin."""
+    rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+    out = np.zeros_like(rays)
+    perm = np.zeros(len(rays), dtype=np.uint32)
+    offsets = np.zeros(65, dtype=np.uint32)
+    _bind_binning().oracle_sort_rays_by_direction(_p(rays) if len(rays) else None, len(rays),
+                                                  _p(out) if len(rays) else None, _p(perm) if len(rays) else None,
+                                                  _p(offsets))
+    return out, perm, offsets
+
+
 def sample_opacity(texels, x, y):
     """Bilinear, repeat-addressed sample of an (h, w) uint8 opacity texture."""
     texels = np.ascontiguousarray(texels, dtype=np.uint8)
