@@ -233,6 +233,17 @@ class World:
         check(lib().atlas_rt_world_update(self._h, _ptr(descs), len(descs), _ptr(world_aabbs),
                                           C.cast(table, C.c_void_p), len(self._blases)))
 
+    def refit(self, descs, stream=None):
+        """Device refit with new matrices (atlas_rt_world_refit): descs is a
+        numpy array (copied to the device) or a uint8 CUDA tensor holding
+        INSTANCE_DESC_DTYPE records, in the order of the last update."""
+        import torch
+        if isinstance(descs, np.ndarray):
+            descs = np.ascontiguousarray(descs, dtype=_abi.INSTANCE_DESC_DTYPE)
+            descs = torch.from_numpy(descs.view(np.uint8).copy()).cuda()
+        check(lib().atlas_rt_world_refit(self._h, _ptr(descs), _stream_ptr(stream)))
+        return descs
+
     def close(self):
         if self._h is not None and self._h.value:
             check(lib().atlas_rt_world_destroy(self._h))

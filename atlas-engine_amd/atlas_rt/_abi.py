@@ -147,6 +147,7 @@ PROTOTYPES = {
     "atlas_rt_world_create": (status, [C.POINTER(vp)]),
     "atlas_rt_world_destroy": (status, [vp]),
     "atlas_rt_world_update": (status, [vp, vp, u32, vp, vp, u32]),
+    "atlas_rt_world_refit": (status, [vp, vp, vp]),
     "atlas_rt_world_info": (status, [vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32)]),
     "atlas_rt_world_export": (status, [vp, vp, vp, vp]),
     "atlas_rt_trace_closest": (status, [vp, vp, vp, vp, u32, f32, f32, u32, vp]),

@@ -22,7 +22,7 @@ atlas_rt_status dev_alloc(T*& p, size_t count) {
 }
 
 void free_all(PathTracer& P) {
-    void* ptrs[] = {P.ctl, P.paths[0], P.paths[1], P.hits, P.shadow, P.Lpix, P.accum, P.xform,
+    void* ptrs[] = {P.ctl, P.paths[0], P.paths[1], P.hits, P.shadow, P.Lpix, P.accum,
                     const_cast<float4**>(P.shade_table), const_cast<float4**>(P.uv_table), P.materials, P.lights,
                     P.tex, P.texels, P.binned};
     for (void* p : ptrs)
@@ -197,6 +197,7 @@ atlas_rt_status atlas_rt_pathtracer_create(const atlas_rt_world* world, const at
     P.punctual.assign(lights, lights + light_count);
     P.material_count = material_count;
     P.world = world;
+    P.xform = reinterpret_cast<const float4*>(world->d_xform);  // the world's, kept current by refits
     for (const atlas_rt_blas* b : world->blas_handles) P.alpha = P.alpha || b->has_alpha;
     P.world_dev = world->dev();
     P.cap = world->max_depth + 1;
@@ -209,7 +210,6 @@ atlas_rt_status atlas_rt_pathtracer_create(const atlas_rt_world* world, const at
     if (st == ATLAS_RT_OK) st = dev_alloc(P.shadow, 3 * pixels);
     if (st == ATLAS_RT_OK) st = dev_alloc(P.Lpix, pixels);
     if (st == ATLAS_RT_OK) st = dev_alloc(P.accum, pixels);
-    if (st == ATLAS_RT_OK) st = dev_alloc(P.xform, 3 * world->instances.size());
     if (st == ATLAS_RT_OK) st = dev_alloc(P.shade_table, world->blas_handles.size());
     if (st == ATLAS_RT_OK) st = dev_alloc(P.uv_table, world->blas_handles.size());
     if (st == ATLAS_RT_OK) st = dev_alloc(P.materials, material_count);
@@ -234,8 +234,6 @@ atlas_rt_status atlas_rt_pathtracer_create(const atlas_rt_world* world, const at
     e = hipMemcpy(P.ctl, &ctl, sizeof(ctl), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(P.accum, 0, pixels * sizeof(float4));
     if (e == hipSuccess) e = hipMemset(P.flags.ptr, 0, 256);
-    if (e == hipSuccess)
-        e = hipMemcpy(P.xform, world->xform.data(), world->xform.size() * sizeof(float), hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(const_cast<float4**>(P.shade_table), shade.data(), shade.size() * sizeof(float4*),
                       hipMemcpyHostToDevice);
@@ -343,6 +341,9 @@ atlas_rt_status atlas_rt_pathtracer_set_emissive_lights(atlas_rt_pathtracer* pt,
         return fail(ATLAS_RT_E_SIZE_MISMATCH,
                     "atlas_rt_pathtracer_set_emissive_lights: one emitter per material table entry expected");
     ATLAS_HIP_TRY(hipDeviceSynchronize());
+    // the instance transforms of the world's last update or refit
+    atlas_rt_status st = const_cast<atlas_rt_world*>(P.world)->sync_host();
+    if (st != ATLAS_RT_OK) return st;
     build_light_list(P, count ? emitters : nullptr, count);
     return upload_lights(P);
 }

@@ -43,7 +43,7 @@ struct PathTracer {
     float4* shadow = nullptr;
     float4* Lpix = nullptr;
     float4* accum = nullptr;
-    float4* xform = nullptr;
+    const float4* xform = nullptr;      // borrowed: the world's transform table
     const float4** shade_table = nullptr;
     const float4** uv_table = nullptr;
     Material* materials = nullptr;

@@ -16,17 +16,31 @@ int default_threads();
 using namespace atlas_rt;
 
 void atlas_rt_world::free_device() {
-    if (d_tlas) hipFree(d_tlas);
-    if (d_instances) hipFree(d_instances);
-    if (d_table) hipFree(d_table);
-    if (d_order) hipFree(d_order);
-    d_tlas = d_instances = d_table = d_order = nullptr;
+    void* ptrs[] = {d_tlas, d_instances, d_table, d_order, d_xform, d_blas_box, d_inst_box, d_refit_leaves,
+                    d_refit_nodes};
+    for (void* p : ptrs)
+        if (p) hipFree(p);
+    d_tlas = d_instances = d_table = d_order = d_xform = nullptr;
+    d_blas_box = d_inst_box = d_refit_leaves = d_refit_nodes = nullptr;
+    refit_level_off.clear();
+    host_stale = false;
 }
 
-// AABB::Transform (AABB.cpp:35-60) with glm's mat4 * vec4 evaluation order
-// ((m0*x + m1*y) + (m2*z + m3*w)).
-static void transform_box(const float m[3][4], const atlas_rt_gpu_bvh_node* root, uint32_t root_count,
-                          const atlas_rt_gpu_bvh_triangle* tris, uint32_t tri_count, float out[6]) {
+atlas_rt_status atlas_rt_world::sync_host() {
+    if (!host_stale) return ATLAS_RT_OK;
+    ATLAS_HIP_TRY(hipDeviceSynchronize());
+    ATLAS_HIP_TRY(hipMemcpy(tlas.data(), d_tlas, tlas.size() * sizeof(tlas[0]), hipMemcpyDeviceToHost));
+    ATLAS_HIP_TRY(
+        hipMemcpy(instances.data(), d_instances, instances.size() * sizeof(instances[0]), hipMemcpyDeviceToHost));
+    ATLAS_HIP_TRY(hipMemcpy(xform.data(), d_xform, xform.size() * sizeof(float), hipMemcpyDeviceToHost));
+    host_stale = false;
+    return ATLAS_RT_OK;
+}
+
+// The object-space box AABB::Transform starts from: the union of the root's
+// two child boxes, or of the vertices of a BLAS that is a single leaf.
+static HBox blas_object_box(const atlas_rt_gpu_bvh_node* root, uint32_t root_count,
+                            const atlas_rt_gpu_bvh_triangle* tris, uint32_t tri_count) {
     HBox b = empty_box();
     if (root_count) {
         HBox l, r;
@@ -45,6 +59,14 @@ static void transform_box(const float m[3][4], const atlas_rt_gpu_bvh_node* root
             box_grow_point(b, tris[i].v2);
         }
     }
+    return b;
+}
+
+// AABB::Transform (AABB.cpp:35-60) with glm's mat4 * vec4 evaluation order
+// ((m0*x + m1*y) + (m2*z + m3*w)).
+static void transform_box(const float m[3][4], const atlas_rt_gpu_bvh_node* root, uint32_t root_count,
+                          const atlas_rt_gpu_bvh_triangle* tris, uint32_t tri_count, float out[6]) {
+    const HBox b = blas_object_box(root, root_count, tris, tri_count);
     const float cx[8] = {b.lo[0], b.lo[0], b.hi[0], b.hi[0], b.lo[0], b.lo[0], b.hi[0], b.hi[0]};
     const float cy[8] = {b.lo[1], b.lo[1], b.lo[1], b.lo[1], b.hi[1], b.hi[1], b.hi[1], b.hi[1]};
     const float cz[8] = {b.lo[2], b.hi[2], b.lo[2], b.hi[2], b.lo[2], b.hi[2], b.lo[2], b.hi[2]};
@@ -194,10 +216,56 @@ atlas_rt_status atlas_rt_world_update(atlas_rt_world* w, const atlas_rt_instance
     if (st == ATLAS_RT_OK) st = upload_vec(w->d_instances, packed);
     if (st == ATLAS_RT_OK) st = upload_vec(w->d_table, table);
     if (st == ATLAS_RT_OK) st = upload_vec(w->d_order, order);
+    if (st == ATLAS_RT_OK) st = upload_vec(w->d_xform, xform);
+    // refit schedule: BLAS object boxes, per-node leaf ranges, nodes by depth
+    std::vector<float> blas_box(8 * size_t(blas_count), 0.0f);
+    for (uint32_t k = 0; k < blas_count; k++) {
+        const atlas_rt_blas* b = blases[k];
+        const HBox ob = blas_object_box(b->nodes.empty() ? nullptr : &b->nodes[0], uint32_t(b->nodes.size()),
+                                        b->tris.data(), uint32_t(b->tris.size()));
+        for (int a = 0; a < 3; a++) {
+            blas_box[8 * size_t(k) + a] = ob.lo[a];
+            blas_box[8 * size_t(k) + 4 + a] = ob.hi[a];
+        }
+    }
+    const size_t nn = bo.nodes.size();
+    std::vector<int32_t> leaves(4 * nn, 0);
+    std::vector<uint32_t> depth(nn, 0);
+    uint32_t deepest = 0;
+    for (size_t g = 0; g < nn; g++) {
+        const int32_t ptr[2] = {bo.nodes[g].left_ptr, bo.nodes[g].right_ptr};
+        for (int side = 0; side < 2; side++) {
+            if (ptr[side] >= 0) {
+                depth[size_t(ptr[side])] = depth[g] + 1;
+                deepest = std::max(deepest, depth[g] + 1);
+            } else if (side == 1 && ptr[1] == ptr[0]) {
+                // a single instance: the builder's root node points both
+                // children at it and leaves the second box zero (count -1: kept)
+                leaves[4 * g + 3] = -1;
+            } else {
+                uint32_t first = uint32_t(~ptr[side]), last = first;
+                while (!bo.end_of_node[last]) last++;
+                leaves[4 * g + 2 * side] = int32_t(first);
+                leaves[4 * g + 2 * side + 1] = int32_t(last - first + 1);
+            }
+        }
+    }
+    std::vector<uint32_t> level_off(deepest + 2, 0), by_level(nn);
+    for (size_t g = 0; g < nn; g++) level_off[deepest - depth[g] + 1]++;
+    for (size_t l = 1; l < level_off.size(); l++) level_off[l] += level_off[l - 1];
+    {
+        std::vector<uint32_t> next(level_off.begin(), level_off.end() - 1);
+        for (size_t g = 0; g < nn; g++) by_level[next[deepest - depth[g]]++] = uint32_t(g);
+    }
+    if (st == ATLAS_RT_OK) st = upload_vec(w->d_blas_box, blas_box);
+    if (st == ATLAS_RT_OK) st = upload_vec(w->d_refit_leaves, leaves);
+    if (st == ATLAS_RT_OK) st = upload_vec(w->d_refit_nodes, by_level);
+    if (st == ATLAS_RT_OK) st = upload_vec(w->d_inst_box, std::vector<float>(8 * m, 0.0f));
     if (st != ATLAS_RT_OK) {
         w->free_device();
         return st;
     }
+    w->refit_level_off = std::move(level_off);
     w->tlas = std::move(bo.nodes);
     w->instances = std::move(packed);
     w->order = std::move(order);
@@ -209,6 +277,13 @@ atlas_rt_status atlas_rt_world_update(atlas_rt_world* w, const atlas_rt_instance
     w->device = dev;
     w->valid = true;
     return ATLAS_RT_OK;
+}
+
+atlas_rt_status atlas_rt_world_refit(atlas_rt_world* w, const atlas_rt_instance_desc* d_instances, void* stream) {
+    atlas_rt_status st = check_world(w, "atlas_rt_world_refit");
+    if (st != ATLAS_RT_OK) return st;
+    if (!d_instances) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_world_refit: NULL instances");
+    return launch_world_refit(*w, d_instances, reinterpret_cast<hipStream_t>(stream));
 }
 
 atlas_rt_status atlas_rt_world_info(const atlas_rt_world* w, uint32_t* tlas_node_count, uint32_t* instance_count,
@@ -223,6 +298,8 @@ atlas_rt_status atlas_rt_world_info(const atlas_rt_world* w, uint32_t* tlas_node
 atlas_rt_status atlas_rt_world_export(const atlas_rt_world* w, atlas_rt_gpu_bvh_node* tlas_nodes,
                                       atlas_rt_gpu_bvh_instance* instances, uint32_t* instance_order) {
     if (!w) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_world_export: NULL world");
+    atlas_rt_status st = const_cast<atlas_rt_world*>(w)->sync_host();
+    if (st != ATLAS_RT_OK) return st;
     if (tlas_nodes && !w->tlas.empty()) std::memcpy(tlas_nodes, w->tlas.data(), w->tlas.size() * sizeof(*tlas_nodes));
     if (instances && !w->instances.empty())
         std::memcpy(instances, w->instances.data(), w->instances.size() * sizeof(*instances));

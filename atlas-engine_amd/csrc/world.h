@@ -25,6 +25,9 @@ atlas_rt_status launch_world_closest_packed(atlas_rt_world& w, const void* in, v
 atlas_rt_status launch_world_alpha_packed(atlas_rt_world& w, const void* in, void* out_rays, float* out_trans,
                                           const uint32_t* d_count, uint32_t max_count, float tmin, float tmax,
                                           uint32_t mask, hipStream_t stream);
+// Device TLAS refit with new instance matrices (refit.hip).
+atlas_rt_status launch_world_refit(atlas_rt_world& w, const atlas_rt_instance_desc* d_desc, hipStream_t stream);
+
 atlas_rt_status launch_world_any_packed(atlas_rt_world& w, const void* in, uint32_t* out, const uint32_t* d_count,
                                         uint32_t max_count, float tmin, float tmax, uint32_t mask, uint32_t flags,
                                         hipStream_t stream);
@@ -48,6 +51,18 @@ struct atlas_rt_world {
     void* d_instances = nullptr;
     void* d_table = nullptr;
     void* d_order = nullptr;
+    void* d_xform = nullptr;  // object-to-world rows, 3 float4 per packed instance
+    // refit schedule, fixed by the last update: per BLAS its object box (2
+    // float4), per slot a world box (2 float4), per TLAS node (int4) the
+    // instance range of a leaf child (left first, count, right first,
+    // count; count 0 = inner child), and the nodes grouped by depth,
+    // deepest first
+    void* d_blas_box = nullptr;
+    void* d_inst_box = nullptr;
+    void* d_refit_leaves = nullptr;
+    void* d_refit_nodes = nullptr;
+    std::vector<uint32_t> refit_level_off;
+    bool host_stale = false;  // a refit changed the device copies after the host ones
     atlas_rt::Scratch spill;
     atlas_rt::Scratch flags;
     atlas_rt::LaunchTiming timing;
@@ -64,4 +79,6 @@ struct atlas_rt_world {
         return d;
     }
     void free_device();
+    // Brings tlas, instances and xform up to date after a refit.
+    atlas_rt_status sync_host();
 };

@@ -377,6 +377,21 @@ atlas_rt_status atlas_rt_world_update(atlas_rt_world* world, const atlas_rt_inst
                                       uint32_t instance_count, const atlas_rt_aabb* world_aabbs,
                                       const atlas_rt_blas* const* blases, uint32_t blas_count);
 
+/* Per-frame update without a rebuild (the engine rebuilds its TLAS every
+ * frame in RayTracingWorld::Update, RayTracingWorld.cpp:32-162; this keeps
+ * the last update's tree and instance order).  d_instances: device memory,
+ * the same instances in the caller's order of the last atlas_rt_world_update;
+ * only object_to_world and world_to_object are read.  On `stream`: every
+ * instance record takes its new inverse matrix, every instance its new world
+ * box (as atlas_rt_world_update computes it with world_aabbs NULL), and every
+ * TLAS node's child boxes the union of the boxes below, level by level.
+ * Nearest hits are those of a rebuild; traversal cost grows as instances move
+ * away from where the tree was built, so rebuild now and then.  A path tracer
+ * created on the world sees the new transforms; its triangle lights move only
+ * after atlas_rt_pathtracer_set_emissive_lights.  Stream-ordered: traces on
+ * other streams must not overlap it. */
+atlas_rt_status atlas_rt_world_refit(atlas_rt_world* world, const atlas_rt_instance_desc* d_instances, void* stream);
+
 atlas_rt_status atlas_rt_world_info(const atlas_rt_world* world, uint32_t* tlas_node_count,
                                     uint32_t* instance_count, uint32_t* max_depth);
 /* Copies the packed TLAS nodes and GPUBVHInstance records (TLAS leaf order)

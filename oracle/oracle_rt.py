@@ -427,6 +427,35 @@ def radix_tree(keys):
     return out
 
 
+def world_refit(tlas, inst, xform, order, descs, blases):
+    """The device refit restated: copies of (tlas, inst, xform) after moving
+    the instances to descs (caller order), and the instances' world boxes
+    ((n, 6), slot order); blases: (nodes, tris) per BLAS of the world's table."""
+    tlas = np.ascontiguousarray(tlas).copy()
+    inst = np.ascontiguousarray(inst).copy()
+    xform = np.ascontiguousarray(xform, dtype=np.float32).copy()
+    order = np.ascontiguousarray(order, dtype=np.uint32)
+    descs = np.ascontiguousarray(descs)
+    L = lib()
+    if not getattr(L, "_refit_bound", False):
+        L.oracle_world_refit.restype = None
+        L.oracle_world_refit.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
+                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_void_p]
+        L._refit_bound = True
+    k = len(blases)
+    keep = [(np.ascontiguousarray(n), np.ascontiguousarray(t)) for n, t in blases]
+    np_ptrs = (C.c_void_p * k)(*[n.ctypes.data if len(n) else None for n, _ in keep])
+    tp_ptrs = (C.c_void_p * k)(*[t.ctypes.data for _, t in keep])
+    ncnt = np.array([len(n) for n, _ in keep], np.uint32)
+    tcnt = np.array([len(t) for _, t in keep], np.uint32)
+    boxes = np.zeros((len(inst), 6), np.float32)
+    L.oracle_world_refit(_p(tlas) if len(tlas) else None, len(tlas), _p(inst), _p(xform), _p(order), len(inst),
+                         _p(descs), C.cast(np_ptrs, C.c_void_p), _p(ncnt), C.cast(tp_ptrs, C.c_void_p), _p(tcnt),
+                         _p(boxes))
+    return tlas, inst, xform, boxes
+
+
 def _bind_binning():
     L = lib()
     if not getattr(L, "_bin_bound", False):
