@@ -7,6 +7,11 @@ in device memory before each timed region):
   c2           two-level path trace, 256 instances of 8 meshes, 1280 x 720,
                4 bounces, sun + constant environment: ms per frame (one
                sample per pixel), without and with ray binning.
+  tlas_update  per-frame TLAS update of C2's meshes at 256 and 65,536
+               instances: the host rebuild (atlas_rt_world_update) against
+               the device refit (atlas_rt_world_refit, matrices resident), and
+               1M random rays traced after a small motion on the refitted and
+               on the rebuilt tree.
   c3           512 independent BLAS (1k-64k triangles, log-uniform) each with
                its own random ray batch (4k-256k rays, log-uniform), traced
                by one autotuned persistent launch and by its captured graph.
@@ -105,6 +110,55 @@ def c2(torch, log, frames=16):
         "binning_speedup": round(res[False] / res[True], 3),
         "binned_image_identical": same,
     }
+
+
+def tlas_update(torch, log, sizes=(256, 65536), reps=10):
+    meshes, _, _, _ = W.c2_scene()
+    blases = []
+    for v, nrm in meshes:
+        b = Blas.build(v)
+        blases.append(b)
+    out = {"workload": "C2's 8 blob meshes; instances random affine over [-100, 100] x [-20, 20] x [-100, 100] "
+                       "at 256 instances, each side scaled by (n / 256)^(1/3) above; motion = every instance "
+                       "translated by U[-0.5, 0.5]^3",
+           "sizes": []}
+    for n in sizes:
+        g = W.rng(2, 4000 + n)
+        grow = (n / 256) ** (1.0 / 3.0)
+        xf = W.random_affine(g, n, extent=100.0 * grow, height=20.0 * grow)
+        which = g.integers(0, len(meshes), n)
+        descs = instance_descs(xf, which)
+        w = World()
+        w.update(descs, blases)
+        t_host = min(_sync_time(torch, lambda: w.update(descs, blases)) for _ in range(3))
+        xf2 = xf.copy()
+        xf2[:, :, 3] += g.uniform(-0.5, 0.5, (n, 3))
+        moved = instance_descs(xf2, which)
+        d_moved = torch.from_numpy(moved.view(np.uint8).copy()).cuda()
+        w.refit(d_moved)
+        t_refit = _sync_time(torch, lambda: [w.refit(d_moved) for _ in range(reps)]) / reps
+        rays = W.random_rays(1 << 20, g, extent=100.0 * grow)
+        d_rays = torch.from_numpy(rays.view(np.float32).reshape(-1, 8).copy()).cuda()
+        d_hits = torch.empty((len(rays), 4), dtype=torch.float32, device="cuda")
+        w.trace_closest_rays(d_rays, d_hits)
+        t_tr_refit = _sync_time(torch, lambda: [w.trace_closest_rays(d_rays, d_hits) for _ in range(3)]) / 3
+        h_refit = d_hits.cpu().numpy().view(HIT_DTYPE).reshape(-1).copy()
+        w2 = World()
+        w2.update(moved, blases)
+        w2.trace_closest_rays(d_rays, d_hits)
+        t_tr_build = _sync_time(torch, lambda: [w2.trace_closest_rays(d_rays, d_hits) for _ in range(3)]) / 3
+        h_build = d_hits.cpu().numpy().view(HIT_DTYPE).reshape(-1)
+        same_t = bool(np.array_equal(h_refit["t"].view(np.int32), h_build["t"].view(np.int32)))
+        log(f"tlas {n}: host rebuild {t_host * 1e3:.2f} ms, device refit {t_refit * 1e3:.3f} ms; trace "
+            f"{t_tr_refit * 1e3:.2f} ms refit vs {t_tr_build * 1e3:.2f} ms rebuilt (same t: {same_t})")
+        out["sizes"].append({
+            "instances": n, "host_rebuild_ms": round(t_host * 1e3, 3), "device_refit_ms": round(t_refit * 1e3, 4),
+            "trace_1m_rays_after_motion_ms": {"refit": round(t_tr_refit * 1e3, 3),
+                                             "rebuilt": round(t_tr_build * 1e3, 3)},
+            "same_hit_distances": same_t,
+        })
+        del w, w2
+    return out
 
 
 def c3(torch, log):

@@ -261,6 +261,7 @@ def main():
         torch.cuda.empty_cache()
         out["c1"] = bench_suite.c1(torch, log)
         out["c2"] = bench_suite.c2(torch, log)
+        out["tlas_update"] = bench_suite.tlas_update(torch, log)
         out["c3"] = bench_suite.c3(torch, log)
         out["radiance_cache"] = bench_suite.cache_study(torch, log, args.ref_spp)
     print(json.dumps(out), flush=True)
