@@ -446,6 +446,45 @@ def camera(eye, yaw_deg, pitch_deg, fov_y_deg, aspect, near=0.1):
     return cam
 
 
+def view_matrix(eye, yaw_deg, pitch_deg):
+    """4x4 world-to-view matrix (numpy, acting on column vectors) of the same
+    eye and orientation as camera(): right-handed, looking down -z."""
+    import math
+    pitch, yaw = math.radians(pitch_deg), math.radians(yaw_deg)
+    fwd = np.array([math.cos(pitch) * math.sin(yaw), -math.sin(pitch), math.cos(pitch) * math.cos(yaw)])
+    right = np.cross(fwd, [0.0, 1.0, 0.0])
+    right /= np.linalg.norm(right)
+    up = np.cross(right, fwd)
+    eye = np.asarray(eye, dtype=np.float64)
+    m = np.eye(4)
+    m[0, :3], m[1, :3], m[2, :3] = right, up, -fwd
+    m[:3, 3] = -m[:3, :3] @ eye
+    return m
+
+
+def projection_matrix(fov_y_deg, aspect, near=0.1, far=1000.0):
+    """4x4 perspective projection (numpy), depth zero to one, y down (Vulkan)."""
+    import math
+    f = 1.0 / math.tan(math.radians(fov_y_deg) / 2.0)
+    m = np.zeros((4, 4))
+    m[0, 0] = f / aspect
+    m[1, 1] = -f
+    m[2, 2] = far / (near - far)
+    m[2, 3] = near * far / (near - far)
+    m[3, 2] = -1.0
+    return m
+
+
+def frame_matrices(view, proj, last_view_proj, jitter=(0.0, 0.0)):
+    """atlas_rt_frame_matrices from numpy 4x4 matrices acting on column
+    vectors (stored column-major, as glm does)."""
+    fm = _abi.FrameMatrices()
+    for name, m in (("view", view), ("projection", proj), ("last_view_projection", last_view_proj)):
+        getattr(fm, name)[:] = np.asarray(m, np.float64).T.astype(np.float32).ravel()
+    fm.jitter[:] = np.asarray(jitter, np.float32)
+    return fm
+
+
 class PathTracer:
     """Progressive wavefront path tracer over a World (PathTracingRenderer)."""
 
@@ -476,6 +515,32 @@ class PathTracer:
 
     def render(self, cam, frames=1, stream=None):
         check(lib().atlas_rt_pathtracer_render(self._h, C.byref(cam), frames, _stream_ptr(stream)))
+
+    def set_realtime(self, samples_per_frame=4, history_length_max=32, history_clip_max=0.2,
+                     current_clip_factor=2.0, max_radiance=65535.0, enable=True):
+        """Realtime mode (defaults of PathTracingRenderer.h:38-43), or off."""
+        p = _abi.RealtimeParams(samples_per_frame, history_length_max, history_clip_max, current_clip_factor,
+                                max_radiance)
+        check(lib().atlas_rt_pathtracer_set_realtime(self._h, C.byref(p) if enable else None))
+
+    def render_realtime(self, cam, matrices, stream=None):
+        check(lib().atlas_rt_pathtracer_render_realtime(self._h, C.byref(cam), C.byref(matrices),
+                                                        _stream_ptr(stream)))
+
+    def read_realtime(self, buffer, stream=None):
+        """One of this frame's realtime buffers (_abi.RT_*) as a CUDA tensor."""
+        import torch
+        shape, dtype = {
+            _abi.RT_RESOLVED: ((self.height, self.width, 4), torch.float32),
+            _abi.RT_FRAME: ((self.height, self.width, 4), torch.float32),
+            _abi.RT_DEPTH: ((self.height, self.width), torch.float32),
+            _abi.RT_VELOCITY: ((self.height, self.width), torch.int32),
+            _abi.RT_NORMAL: ((self.height, self.width), torch.int32),
+            _abi.RT_MATERIAL: ((self.height, self.width), torch.int16),
+        }[buffer]
+        out = torch.empty(shape, dtype=dtype, device="cuda")
+        check(lib().atlas_rt_pathtracer_read_realtime(self._h, buffer, _ptr(out), _stream_ptr(stream)))
+        return out
 
     def reset(self):
         check(lib().atlas_rt_pathtracer_reset(self._h))

@@ -79,7 +79,7 @@ class LaunchConfig(C.Structure):
 MATERIAL_DTYPE = np.dtype([
     ("base_color", "<f4", 3), ("opacity", "<f4"), ("emissive", "<f4", 3), ("roughness", "<f4"),
     ("metalness", "<f4"), ("ao", "<f4"), ("reflectance", "<f4"), ("two_sided", "<i4"),
-    ("opacity_texture", "<i4"), ("invert_uvs", "<i4"), ("reserved", "<i4", 2),
+    ("opacity_texture", "<i4"), ("invert_uvs", "<i4"), ("id", "<i4"), ("reserved", "<i4"),
 ])
 LIGHT_DTYPE = np.dtype([("direction", "<f4", 3), ("type", "<u4"), ("radiance", "<f4", 3), ("reserved", "<f4")])
 GPU_LIGHT_DTYPE = np.dtype([("P", "<f4", 4), ("N", "<f4", 4), ("color", "<f4", 4), ("data", "<f4", 4)])
@@ -97,6 +97,21 @@ class Camera(C.Structure):
     """atlas_rt_camera."""
     _fields_ = [("location", C.c_float * 3), ("origin", C.c_float * 3), ("right", C.c_float * 3),
                 ("bottom", C.c_float * 3)]
+
+
+class RealtimeParams(C.Structure):
+    """atlas_rt_realtime_params (defaults: PathTracingRenderer.h:38-43)."""
+    _fields_ = [("samples_per_frame", C.c_int32), ("history_length_max", C.c_int32),
+                ("history_clip_max", C.c_float), ("current_clip_factor", C.c_float), ("max_radiance", C.c_float)]
+
+
+class FrameMatrices(C.Structure):
+    """atlas_rt_frame_matrices: glm column-major mat4 (m[column][row])."""
+    _fields_ = [("view", C.c_float * 16), ("projection", C.c_float * 16), ("last_view_projection", C.c_float * 16),
+                ("jitter", C.c_float * 2), ("reserved", C.c_float * 2)]
+
+
+RT_RESOLVED, RT_FRAME, RT_DEPTH, RT_VELOCITY, RT_NORMAL, RT_MATERIAL = range(6)
 
 
 class PtParams(C.Structure):
@@ -169,6 +184,9 @@ PROTOTYPES = {
     "atlas_rt_trace_closest_transparency": (status, [vp, vp, vp, vp, u32, f32, f32, u32, vp]),
     "atlas_rt_trace_transparency": (status, [vp, vp, vp, vp, u32, f32, f32, u32, vp]),
     "atlas_rt_pathtracer_set_textures": (status, [vp, vp, u32]),
+    "atlas_rt_pathtracer_set_realtime": (status, [vp, vp]),
+    "atlas_rt_pathtracer_render_realtime": (status, [vp, vp, vp, vp]),
+    "atlas_rt_pathtracer_read_realtime": (status, [vp, u32, vp, vp]),
     "atlas_rt_pathtracer_set_emissive_lights": (status, [vp, vp, u32]),
     "atlas_rt_pathtracer_lights": (status, [vp, vp, u32, C.POINTER(u32)]),
     "atlas_rt_pathtracer_create": (status, [vp, vp, vp, u32, vp, u32, C.POINTER(vp)]),

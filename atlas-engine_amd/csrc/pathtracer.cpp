@@ -21,6 +21,48 @@ atlas_rt_status dev_alloc(T*& p, size_t count) {
     return ATLAS_RT_OK;
 }
 
+// Buffers sized by the paths per frame: both path queues, hits, shadow rays,
+// radiance per ray ID, the traversal stack spill and the binning buffers.
+atlas_rt_status alloc_ray_buffers(PathTracer& P, uint32_t rays) {
+    void* old[] = {P.paths[0], P.paths[1], P.hits, P.shadow, P.Lpix, P.binned};
+    for (void* p : old)
+        if (p) (void)hipFree(p);
+    P.paths[0] = P.paths[1] = nullptr;
+    P.hits = nullptr;
+    P.shadow = P.Lpix = P.binned = nullptr;
+    P.rays = rays;
+    atlas_rt_status st = dev_alloc(P.paths[0], 3 * size_t(rays));
+    if (st == ATLAS_RT_OK) st = dev_alloc(P.paths[1], 3 * size_t(rays));
+    if (st == ATLAS_RT_OK) st = dev_alloc(P.hits, rays);
+    if (st == ATLAS_RT_OK) st = dev_alloc(P.shadow, 3 * size_t(rays));
+    if (st == ATLAS_RT_OK) st = dev_alloc(P.Lpix, rays);
+    const size_t grid_threads = (size_t(rays) + 127) / 128 * 128;
+    if (st == ATLAS_RT_OK && P.cap > 32) st = P.spill.ensure(size_t(P.cap - 32) * grid_threads * sizeof(int32_t));
+    if (st == ATLAS_RT_OK && P.binning) st = dev_alloc(P.binned, 3 * size_t(rays));
+    if (st == ATLAS_RT_OK && P.binning) st = P.bin_ws.ensure(bin_sort_workspace(rays));
+    ShadeArgs& A = P.shade_args;
+    A.hits = P.hits;
+    A.shadow = P.shadow;
+    A.Lpix = P.Lpix;
+    return st;
+}
+
+void free_realtime(PathTracer& P) {
+    void* ptrs[] = {P.rt_frame, P.rt_radiance[0], P.rt_radiance[1], P.rt_depth[0], P.rt_depth[1], P.rt_velocity[0],
+                    P.rt_velocity[1], P.rt_normal[0], P.rt_normal[1], P.rt_material[0], P.rt_material[1],
+                    P.rt_last_xform};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    P.rt_frame = P.rt_last_xform = nullptr;
+    for (int k = 0; k < 2; k++) {
+        P.rt_radiance[k] = nullptr;
+        P.rt_depth[k] = nullptr;
+        P.rt_velocity[k] = P.rt_normal[k] = nullptr;
+        P.rt_material[k] = nullptr;
+    }
+    P.realtime = false;
+}
+
 void free_all(PathTracer& P) {
     void* ptrs[] = {P.ctl, P.paths[0], P.paths[1], P.hits, P.shadow, P.Lpix, P.accum,
                     const_cast<float4**>(P.shade_table), const_cast<float4**>(P.uv_table), P.materials, P.lights,
@@ -31,6 +73,7 @@ void free_all(PathTracer& P) {
     P.flags.release();
     P.bin_ws.release();
     free_cache_buffers(P);
+    free_realtime(P);
 }
 
 // Random::Init seeds PCG32 with uint64(SampleUniformFloat()) = 0
@@ -204,20 +247,12 @@ atlas_rt_status atlas_rt_pathtracer_create(const atlas_rt_world* world, const at
     const size_t pixels = size_t(P.width) * P.height;
     atlas_rt_status st = ATLAS_RT_OK;
     if (st == ATLAS_RT_OK) st = dev_alloc(P.ctl, 1);
-    if (st == ATLAS_RT_OK) st = dev_alloc(P.paths[0], 3 * pixels);
-    if (st == ATLAS_RT_OK) st = dev_alloc(P.paths[1], 3 * pixels);
-    if (st == ATLAS_RT_OK) st = dev_alloc(P.hits, pixels);
-    if (st == ATLAS_RT_OK) st = dev_alloc(P.shadow, 3 * pixels);
-    if (st == ATLAS_RT_OK) st = dev_alloc(P.Lpix, pixels);
+    if (st == ATLAS_RT_OK) st = alloc_ray_buffers(P, uint32_t(pixels));
     if (st == ATLAS_RT_OK) st = dev_alloc(P.accum, pixels);
     if (st == ATLAS_RT_OK) st = dev_alloc(P.shade_table, world->blas_handles.size());
     if (st == ATLAS_RT_OK) st = dev_alloc(P.uv_table, world->blas_handles.size());
     if (st == ATLAS_RT_OK) st = dev_alloc(P.materials, material_count);
-    const uint32_t grid_threads = uint32_t((pixels + 127) / 128 * 128);
-    if (st == ATLAS_RT_OK && P.cap > 32) st = P.spill.ensure(size_t(P.cap - 32) * grid_threads * sizeof(int32_t));
     if (st == ATLAS_RT_OK) st = P.flags.ensure(256);
-    if (st == ATLAS_RT_OK && P.binning) st = dev_alloc(P.binned, 3 * pixels);
-    if (st == ATLAS_RT_OK && P.binning) st = P.bin_ws.ensure(bin_sort_workspace(uint32_t(pixels)));
     if (st != ATLAS_RT_OK) {
         free_all(P);
         delete pt;
@@ -291,6 +326,90 @@ atlas_rt_status atlas_rt_pathtracer_render(atlas_rt_pathtracer* pt, const atlas_
         if (st != ATLAS_RT_OK) return st;
         pt->P.host_samples++;
     }
+    return ATLAS_RT_OK;
+}
+
+atlas_rt_status atlas_rt_pathtracer_set_realtime(atlas_rt_pathtracer* pt, const atlas_rt_realtime_params* params) {
+    if (!pt) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_set_realtime: NULL path tracer");
+    PathTracer& P = pt->P;
+    if (params && P.nrc)
+        return fail(ATLAS_RT_E_UNSUPPORTED, "atlas_rt_pathtracer_set_realtime: not with a radiance cache");
+    if (params && (params->samples_per_frame < 1 || params->samples_per_frame > 64 ||
+                   params->history_length_max < 1 || !(params->max_radiance > 0.0f)))
+        return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_set_realtime: need 1 <= samples_per_frame <= 64, "
+                                            "history_length_max >= 1, max_radiance > 0");
+    ATLAS_HIP_TRY(hipDeviceSynchronize());
+    free_realtime(P);
+    const size_t pixels = size_t(P.width) * P.height;
+    const uint32_t rays = uint32_t(pixels * (params ? size_t(params->samples_per_frame) : 1u));
+    atlas_rt_status st = alloc_ray_buffers(P, rays);
+    if (st != ATLAS_RT_OK || !params) return st;
+    const size_t inst = P.world->instances.size();
+    if (st == ATLAS_RT_OK) st = dev_alloc(P.rt_frame, pixels);
+    for (int k = 0; k < 2 && st == ATLAS_RT_OK; k++) {
+        st = dev_alloc(P.rt_radiance[k], pixels);
+        if (st == ATLAS_RT_OK) st = dev_alloc(P.rt_depth[k], pixels);
+        if (st == ATLAS_RT_OK) st = dev_alloc(P.rt_velocity[k], pixels);
+        if (st == ATLAS_RT_OK) st = dev_alloc(P.rt_normal[k], pixels);
+        if (st == ATLAS_RT_OK) st = dev_alloc(P.rt_material[k], pixels);
+    }
+    if (st == ATLAS_RT_OK) st = dev_alloc(P.rt_last_xform, 3 * inst);
+    if (st != ATLAS_RT_OK) {
+        free_realtime(P);
+        return st;
+    }
+    // empty history: zero radiance and length, G-buffer of zeros
+    for (int k = 0; k < 2; k++) {
+        ATLAS_HIP_TRY(hipMemset(P.rt_radiance[k], 0, pixels * sizeof(float4)));
+        ATLAS_HIP_TRY(hipMemset(P.rt_depth[k], 0, pixels * sizeof(float)));
+        ATLAS_HIP_TRY(hipMemset(P.rt_velocity[k], 0, pixels * sizeof(uint32_t)));
+        ATLAS_HIP_TRY(hipMemset(P.rt_normal[k], 0, pixels * sizeof(uint32_t)));
+        ATLAS_HIP_TRY(hipMemset(P.rt_material[k], 0, pixels * sizeof(uint16_t)));
+    }
+    ATLAS_HIP_TRY(hipMemset(P.rt_frame, 0, pixels * sizeof(float4)));
+    // no motion before the first frame
+    ATLAS_HIP_TRY(hipMemcpy(P.rt_last_xform, P.world->d_xform, 3 * inst * sizeof(float4), hipMemcpyDeviceToDevice));
+    P.rt = *params;
+    P.rt_cur = 0;
+    P.rt_frames = 0;
+    P.realtime = true;
+    return ATLAS_RT_OK;
+}
+
+atlas_rt_status atlas_rt_pathtracer_render_realtime(atlas_rt_pathtracer* pt, const atlas_rt_camera* camera,
+                                                    const atlas_rt_frame_matrices* matrices, void* stream) {
+    if (!pt || !camera || !matrices) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_render_realtime: NULL argument");
+    if (!pt->P.realtime)
+        return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_render_realtime: realtime mode is off "
+                                            "(atlas_rt_pathtracer_set_realtime)");
+    Camera cam;
+    std::memcpy(cam.loc, camera->location, 12);
+    std::memcpy(cam.origin, camera->origin, 12);
+    std::memcpy(cam.right, camera->right, 12);
+    std::memcpy(cam.bottom, camera->bottom, 12);
+    return launch_frame_realtime(pt->P, cam, *matrices, reinterpret_cast<hipStream_t>(stream));
+}
+
+atlas_rt_status atlas_rt_pathtracer_read_realtime(atlas_rt_pathtracer* pt, uint32_t buffer, void* d_out, void* stream) {
+    if (!pt || !d_out) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_read_realtime: NULL argument");
+    PathTracer& P = pt->P;
+    if (!P.realtime) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_read_realtime: realtime mode is off");
+    const size_t pixels = size_t(P.width) * P.height;
+    const uint32_t c = P.rt_cur;
+    const void* src = nullptr;
+    size_t bytes = 0;
+    switch (buffer) {
+        case ATLAS_RT_RT_RESOLVED: src = P.rt_radiance[c]; bytes = pixels * sizeof(float4); break;
+        case ATLAS_RT_RT_FRAME: src = P.rt_frame; bytes = pixels * sizeof(float4); break;
+        case ATLAS_RT_RT_DEPTH: src = P.rt_depth[c]; bytes = pixels * sizeof(float); break;
+        case ATLAS_RT_RT_VELOCITY: src = P.rt_velocity[c]; bytes = pixels * sizeof(uint32_t); break;
+        case ATLAS_RT_RT_NORMAL: src = P.rt_normal[c]; bytes = pixels * sizeof(uint32_t); break;
+        case ATLAS_RT_RT_MATERIAL: src = P.rt_material[c]; bytes = pixels * sizeof(uint16_t); break;
+        default: return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_read_realtime: unknown buffer");
+    }
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    ATLAS_HIP_TRY(hipMemcpyAsync(d_out, src, bytes, hipMemcpyDeviceToDevice, s));
+    ATLAS_HIP_TRY(hipStreamSynchronize(s));
     return ATLAS_RT_OK;
 }
 
@@ -391,6 +510,7 @@ atlas_rt_status atlas_rt_pathtracer_set_cache(atlas_rt_pathtracer* pt, atlas_rt_
     free_cache_buffers(P);
     if (!cache) return ATLAS_RT_OK;
     if (!params) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_set_cache: NULL params");
+    if (P.realtime) return fail(ATLAS_RT_E_UNSUPPORTED, "atlas_rt_pathtracer_set_cache: not in realtime mode");
     if (P.bounces < 2) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_set_cache: needs bounces >= 2");
     if (params->train_stride < 1 || params->train_steps > 64)
         return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_set_cache: need train_stride >= 1, train_steps <= 64");

@@ -25,8 +25,9 @@ struct ShadeArgs {
     float4* paths_out;
     const int4* hits;
     float4* shadow;
-    float4* Lpix;
+    float4* Lpix;                 // radiance per ray ID
     CacheArgs cache;
+    RealtimeArgs rt;
 };
 
 struct PathTracer {
@@ -65,6 +66,19 @@ struct PathTracer {
     Scratch bin_ws;
     ShadeArgs shade_args{};
     int32_t host_samples = 0;
+    uint32_t rays = 0;            // paths per frame: pixels, times the samples per frame in realtime mode
+    // realtime mode (atlas_rt_pathtracer_set_realtime)
+    bool realtime = false;
+    atlas_rt_realtime_params rt{};
+    uint32_t rt_cur = 0;          // which set of the pairs below is this frame's
+    uint32_t rt_frames = 0;
+    float4* rt_frame = nullptr;   // this frame's radiance, as the temporal pass fetches it
+    float4* rt_radiance[2] = {nullptr, nullptr};  // resolved radiance + history length
+    float* rt_depth[2] = {nullptr, nullptr};
+    uint32_t* rt_velocity[2] = {nullptr, nullptr};
+    uint32_t* rt_normal[2] = {nullptr, nullptr};
+    uint16_t* rt_material[2] = {nullptr, nullptr};
+    float4* rt_last_xform = nullptr;  // the world's transforms of the previous frame
     // radiance cache (null: plain path tracing)
     atlas_rt_nrc* nrc = nullptr;
     uint32_t train_steps = 0;
@@ -78,6 +92,29 @@ struct PathTracer {
 };
 
 atlas_rt_status launch_frame(PathTracer& P, const Camera& cam, hipStream_t s);
+// The temporal resolve's inputs and output (pt_temporal.hip,
+// pathtracer/temporal.csh).
+struct TemporalArgs {
+    uint32_t width, height;
+    const float4* frame;       // this frame's radiance (FetchTexel)
+    const float* depth;
+    const uint32_t* velocity;  // 2 x binary16
+    const uint32_t* normal;    // 2 x binary16
+    const uint16_t* material;
+    const float* hdepth;       // last frame's G-buffer
+    const uint32_t* hnormal;
+    const uint16_t* hmaterial;
+    const float4* hradiance;   // inAccumImage: last frame's resolve
+    float4* out;               // outAccumImage
+    float history_clip_max, current_clip_factor, max_history_length;
+};
+atlas_rt_status launch_temporal(const TemporalArgs& T, hipStream_t s);
+
+// One realtime frame.
+atlas_rt_status launch_frame_realtime(PathTracer& P, const Camera& cam, const atlas_rt_frame_matrices& M,
+                                      hipStream_t s);
+// The bounce loop shared by both modes: bounces + 1 x (trace, shade, shadow).
+atlas_rt_status launch_bounces(PathTracer& P, hipStream_t s);
 // Builds the light list (UpdateLights over the punctual lights and, with
 // emitters, the triangle lights) into P.light_list.
 void build_light_list(PathTracer& P, const float* emitters, uint32_t emitter_count);

@@ -19,6 +19,9 @@
 //   accumulate   per-pixel running sum (rayHit.csh:119-132, linear HDR)
 // Every count lives on the device, so a whole frame can be graph-captured.
 #include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+
+#include <cstring>
 
 #include "alpha.h"
 #include "binning.h"
@@ -147,6 +150,69 @@ __global__ __launch_bounds__(kShadeBlock) void k_raygen(Camera cam, uint32_t wid
     Lpix[pixel] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
+// rayGen.csh with REALTIME: spf rays per pixel through the pixel centre
+// shifted by a quarter of the camera jitter (rayGen.csh:39-40), ray ID =
+// pixel * spf + sample; written in 8x8-tile order, the samples of a pixel
+// next to each other.
+__global__ __launch_bounds__(kShadeBlock) void k_raygen_rt(Camera cam, uint32_t width, uint32_t height, uint32_t spf,
+                                                           float jx, float jy, float4* __restrict__ paths,
+                                                           float4* __restrict__ Lpix) {
+    const uint32_t g = blockIdx.x * kShadeBlock + threadIdx.x;
+    if (g >= width * height * spf) return;
+    const uint32_t pixel = g / spf, sample = g % spf;
+    const uint32_t px = pixel % width, py = pixel / width;
+    const float cx = jx * 0.25f + (float(px) + 0.5f) / float(width);
+    const float cy = jy * 0.25f + (float(py) + 0.5f) / float(height);
+    const V o = mk(cam.origin[0], cam.origin[1], cam.origin[2]);
+    const V r = mk(cam.right[0], cam.right[1], cam.right[2]);
+    const V b = mk(cam.bottom[0], cam.bottom[1], cam.bottom[2]);
+    const V loc = mk(cam.loc[0], cam.loc[1], cam.loc[2]);
+    const V dir = normalize(((o + r * cx) + b * cy) - loc);
+    const uint32_t tiles_x = (width + 7) / 8;
+    uint32_t slot = ((py / 8) * tiles_x + (px / 8)) * 64 + (py % 8) * 8 + (px % 8);
+    if ((width % 8) || (height % 8)) slot = pixel;
+    const size_t idx = size_t(slot) * spf + sample;
+    paths[3 * idx] = make_float4(loc.x, loc.y, loc.z, __uint_as_float(g));
+    paths[3 * idx + 1] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(0u));
+    paths[3 * idx + 2] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    Lpix[g] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+__device__ __forceinline__ float glsl_max(float x, float y) { return (x < y) ? y : x; }
+__device__ __forceinline__ float glsl_min(float x, float y) { return (y < x) ? y : x; }
+
+// The frame's radiance per pixel as temporal.csh's FetchTexel reads it: one
+// sample stores its radiance (rayHit.csh:133-137); more are quantised to
+// 0xFFFFFFFF / spf steps of max_radiance, summed as integers (the engine's
+// imageAtomicAdd, rayHit.csh:139-149) and decoded; then max(., 0).
+__global__ __launch_bounds__(kBlock) void k_frame_rt(const float4* __restrict__ Lray, uint32_t pixels, uint32_t spf,
+                                                     float max_radiance, float4* __restrict__ frame) {
+    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= pixels) return;
+    float c[3];
+    if (spf == 1) {
+        const float4 L = Lray[p];
+        c[0] = L.x;
+        c[1] = L.y;
+        c[2] = L.z;
+    } else {
+        const uint32_t per = 0xFFFFFFFFu / spf;
+        uint32_t sum[3] = {0u, 0u, 0u};
+        for (uint32_t k = 0; k < spf; k++) {
+            const float4 L = Lray[size_t(p) * spf + k];
+            const float v[3] = {L.x, L.y, L.z};
+            for (int a = 0; a < 3; a++) {
+                const float n = glsl_min(glsl_max(v[a] / max_radiance, 0.0f), 1.0f);
+                const float q = n * float(per);  // in [0, 2^31]; NaN radiance counts as 0
+                const uint32_t u = q >= 0.0f ? uint32_t(q) : 0u;
+                sum[a] += u < per ? u : per;
+            }
+        }
+        for (int a = 0; a < 3; a++) c[a] = float(sum[a]) / float(per) / float(spf) * max_radiance;
+    }
+    frame[p] = make_float4(glsl_max(c[0], 0.0f), glsl_max(c[1], 0.0f), glsl_max(c[2], 0.0f), 0.0f);
+}
+
 // Closest hit of live paths: hit = (engine float32 t, slot, instance slot).
 // ALPHA: HitClosestTransparency (candidates need opacity > 0).
 template <int BLOCK, int LDS_DEPTH, bool ALPHA>
@@ -192,6 +258,49 @@ __device__ __forceinline__ V row_xform(const float4* m, V p, float w) {
               (m[2].x * p.x + m[2].y * p.y) + (m[2].z * p.z + m[2].w * w));
 }
 
+// glm mat4 (column-major, m[4 * column + row]) times (x, y, z, w), each row
+// as ((m0 x + m1 y) + (m2 z + m3 w)).
+__device__ __forceinline__ float4 mat4_mul(const float* m, float x, float y, float z, float w) {
+    float r[4];
+    for (int k = 0; k < 4; k++) r[k] = (m[k] * x + m[4 + k] * y) + (m[8 + k] * z + m[12 + k] * w);
+    return make_float4(r[0], r[1], r[2], r[3]);
+}
+
+__device__ __forceinline__ uint32_t pack_half2(float a, float b) {
+    return uint32_t(__half_as_ushort(__float2half_rn(a))) | (uint32_t(__half_as_ushort(__float2half_rn(b))) << 16);
+}
+
+// The realtime G-buffer of a pixel (rayHit.csh:76-117): velocity from the
+// surface point's object-space position under this and last frame's
+// matrices, view-space depth, EncodeNormal of the geometry normal
+// (common/normalencode.hsh:3-15), the material ID.  inv / last: the
+// instance's inverse rows and last object-to-world rows (identity on a miss).
+__device__ __forceinline__ void write_gbuffer(const RealtimeArgs& R, uint32_t pixel, bool hit, V P, V Ng,
+                                              uint32_t material, const float4* inv, const float4* last) {
+    const float ox = dot4(P.x, P.y, P.z, 1.0f, inv[0]);
+    const float oy = dot4(P.x, P.y, P.z, 1.0f, inv[1]);
+    const float oz = dot4(P.x, P.y, P.z, 1.0f, inv[2]);
+    const V lp = row_xform(last, mk(ox, oy, oz), 1.0f);
+    const float4 vs = mat4_mul(R.view, P.x, P.y, P.z, 1.0f);
+    const float4 pc = mat4_mul(R.proj, vs.x, vs.y, vs.z, vs.w);
+    const float4 pl = mat4_mul(R.last_vp, lp.x, lp.y, lp.z, 1.0f);
+    const float vx = (pl.x / pl.w - pc.x / pc.w) * 0.5f;
+    const float vy = (pl.y / pl.w - pc.y / pc.w) * 0.5f;
+    R.velocity[pixel] = pack_half2(vx, vy);
+    R.depth[pixel] = hit ? vs.z : 1.0e12f;
+    R.material[pixel] = uint16_t(material);
+    const float l1 = (fabsf(Ng.x) + fabsf(Ng.y)) + fabsf(Ng.z);
+    float nx = Ng.x / l1, nz = Ng.z / l1;
+    if (Ng.y < 0.0f) {
+        const float a = nx, b = nz;
+        nx = (a >= 0.0f ? 1.0f : -1.0f) * (1.0f - fabsf(b));
+        nz = (b >= 0.0f ? 1.0f : -1.0f) * (1.0f - fabsf(a));
+    }
+    R.normal[pixel] = pack_half2(nx, nz);
+}
+
+__constant__ float4 kIdentityRows[3] = {{1.0f, 0.0f, 0.0f, 0.0f}, {0.0f, 1.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 1.0f, 0.0f}};
+
 // Records a shaded path hands to the stream compaction of k_shade.
 struct ShadeOut {
     bool query = false, shadow = false, cont = false;
@@ -206,17 +315,23 @@ __device__ __forceinline__ void shade_one(const ShadeArgs& A, int32_t bounce, ui
     const float4 p1 = A.paths_in[3 * size_t(i) + 1];
     const float4 p2 = A.paths_in[3 * size_t(i) + 2];
     const int4 h = A.hits[i];
-    const uint32_t pixel = __float_as_uint(p0.w);
+    // ray ID: the pixel in progressive mode, pixel * samples per frame +
+    // sample in realtime mode (ray.ID, rayGen.csh:45)
+    const uint32_t id = __float_as_uint(p0.w);
     // radiance cache: 0 = render path, else training path index + 1
     const uint32_t train = A.cache.enabled ? __float_as_uint(p1.w) : 0u;
     const size_t rec = train ? size_t(train - 1) * (A.bounces + 1) + bounce : 0;
     const V thr_in = mk(p2.x, p2.y, p2.z);
     V thr = thr_in;
     const V ro = mk(p0.x, p0.y, p0.z), rd = mk(p1.x, p1.y, p1.z);
-    float4 acc = A.Lpix[pixel];
+    float4 acc = A.Lpix[id];
     V add = mk(0.0f, 0.0f, 0.0f);
     V no = ro, nd = rd;
+    const bool gbuffer = A.rt.enabled && bounce == 0 && id % A.rt.spf == 0;
     if (h.y < 0) {
+        if (gbuffer)
+            write_gbuffer(A.rt, id / A.rt.spf, false, ro + rd * 100.0f, mk(1.0f, 0.0f, 0.0f), 65535u, kIdentityRows,
+                          kIdentityRows);
         // miss: min(env * throughput, 10) (rayHit.csh:166-172)
         const V e = mk(A.env[0], A.env[1], A.env[2]) * thr;
         add = mk(fminf(e.x, 10.0f), fminf(e.y, 10.0f), fminf(e.z, 10.0f));
@@ -264,6 +379,9 @@ __device__ __forceinline__ void shade_one(const ShadeArgs& A, int32_t bounce, ui
         S.V_ = -rd;
         S.N = normalize(normal);
         S.Ng = Ng;
+        if (gbuffer)
+            write_gbuffer(A.rt, id / A.rt.spf, true, S.P, Ng, uint32_t(m.id), A.world.instances + 4 * size_t(inst),
+                          A.rt.last_xform + 3 * size_t(inst));
         if (A.cache.enabled) {
             const float4 q0 = make_float4(S.P.x, S.P.y, S.P.z, m.roughness);
             const float4 q1 = make_float4(-rd.x, -rd.y, -rd.z, 0.0f);
@@ -279,7 +397,7 @@ __device__ __forceinline__ void shade_one(const ShadeArgs& A, int32_t bounce, ui
                 out.q[0] = q0;
                 out.q[1] = q1;
                 out.q[2] = q2;
-                out.q[3] = make_float4(thr_in.x, thr_in.y, thr_in.z, __uint_as_float(pixel));
+                out.q[3] = make_float4(thr_in.x, thr_in.y, thr_in.z, __uint_as_float(id));
                 return;
             }
         }
@@ -291,12 +409,12 @@ __device__ __forceinline__ void shade_one(const ShadeArgs& A, int32_t bounce, ui
         if ((m.emissive[0] + m.emissive[1]) + m.emissive[2] > 0.0f && bounce == 0)
             add = add + mk(m.emissive[0], m.emissive[1], m.emissive[2]);
         // EvaluateDirectLight (rayHit.csh:209-235): GetLight / SampleLight
-        // (raytracer/direct.hsh:12-86).  The light's seed is the pixel index
-        // (the engine uses the invocation index, which depends on the order
-        // of the compaction atomics).
+        // (raytracer/direct.hsh:12-86).  The light's seed is the ray ID (the
+        // engine uses the invocation index, which depends on the order of
+        // the compaction atomics).
         if (A.light_count > 0) {
             float seed = ctl->seeds[bounce];
-            const float ray_seed = float(pixel);
+            const float ray_seed = float(id);
             float total = 0.0f, lpdf = 1.0f;
             int32_t sel = 0;
             for (int32_t l = 0; l < A.light_count; l++) {
@@ -364,7 +482,7 @@ __device__ __forceinline__ void shade_one(const ShadeArgs& A, int32_t bounce, ui
                 const V c = thr * m.opacity * direct;
                 const V so = S.P + S.N * kEpsilon;
                 out.shadow = true;
-                out.sh[0] = make_float4(so.x, so.y, so.z, __uint_as_float(pixel));
+                out.sh[0] = make_float4(so.x, so.y, so.z, __uint_as_float(id));
                 out.sh[1] = make_float4(S.L.x, S.L.y, S.L.z, ldist - 2.0f * kEpsilon);
                 out.sh[2] = make_float4(c.x, c.y, c.z, __uint_as_float(train ? uint32_t(rec) + 1u : 0u));
             }
@@ -372,7 +490,7 @@ __device__ __forceinline__ void shade_one(const ShadeArgs& A, int32_t bounce, ui
         // EvaluateIndirectLight (rayHit.csh:237-325)
         no = S.P;
         float seed = ctl->seeds[bounce];
-        const float ray_seed = float(pixel);
+        const float ray_seed = float(id);
         float refract = clampf(1.0f - m.opacity, 0.1f, 0.9f);
         refract = m.opacity == 1.0f ? 0.0f : refract;
         float rnd = saturate(rnd2(ray_seed, seed));
@@ -421,7 +539,7 @@ __device__ __forceinline__ void shade_one(const ShadeArgs& A, int32_t bounce, ui
     acc.x += add.x;
     acc.y += add.y;
     acc.z += add.z;
-    A.Lpix[pixel] = acc;
+    A.Lpix[id] = acc;
     if (train) A.cache.rec_add[rec] = make_float4(add.x, add.y, add.z, 0.0f);
     const float energy = (thr.x + thr.y) + thr.z;
     if (energy == 0.0f || bounce == A.bounces) return;
@@ -581,21 +699,18 @@ __global__ __launch_bounds__(kBlock) void k_accumulate(FrameCtl* ctl, const floa
 
 }  // namespace
 
-atlas_rt_status launch_frame(PathTracer& P, const Camera& cam, hipStream_t s) {
-    const uint32_t pixels = P.width * P.height;
-    const uint32_t grid_all = (pixels + kBlock - 1) / kBlock;
+atlas_rt_status launch_bounces(PathTracer& P, hipStream_t s) {
+    const uint32_t rays = P.rays;
+    const uint32_t grid_all = (rays + kBlock - 1) / kBlock;
     uint32_t* flags = reinterpret_cast<uint32_t*>(P.flags.ptr);
     const CacheArgs& C = P.shade_args.cache;
-    hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(64), 0, s, P.ctl, P.bounces, pixels);
-    hipLaunchKernelGGL(k_raygen, dim3((pixels + kShadeBlock - 1) / kShadeBlock), dim3(kShadeBlock), 0, s, cam, P.width,
-                       P.height, P.ctl, P.paths[0], P.Lpix, C, P.bounces);
     for (int32_t b = 0; b <= P.bounces; b++) {
         float4* in = P.paths[b & 1];
         float4* out = P.paths[(b + 1) & 1];
         if (P.binning) {
             // coherence sort of this bounce's live paths (the engine sorts
             // before each trace dispatch, RayTracingHelper.cpp:304-344)
-            ATLAS_HIP_TRY(launch_bin_sort(in, P.binned, 3, &P.ctl->paths[b], 0, pixels, P.bin_ws.ptr, nullptr,
+            ATLAS_HIP_TRY(launch_bin_sort(in, P.binned, 3, &P.ctl->paths[b], 0, rays, P.bin_ws.ptr, nullptr,
                                           nullptr, s));
             in = P.binned;
         }
@@ -621,10 +736,75 @@ atlas_rt_status launch_frame(PathTracer& P, const Camera& cam, hipStream_t s) {
             hipLaunchKernelGGL((k_shadow<kBlock, kLdsDepth, false>), dim3(grid), dim3(kBlock), 0, s, P.world_dev,
                                P.ctl, b, P.shadow, P.Lpix, spill, P.cap, flags, C.rec_add, AA);
         if (C.enabled && b == 1) {
-            ATLAS_HIP_TRY(nrc::launch_infer(P.nrc->dev, P.q_buf, P.q_out, &P.ctl->queries, pixels, s));
+            ATLAS_HIP_TRY(nrc::launch_infer(P.nrc->dev, P.q_buf, P.q_out, &P.ctl->queries, rays, s));
             hipLaunchKernelGGL(k_cache_apply, dim3(grid_all), dim3(kBlock), 0, s, P.ctl, P.q_meta, P.q_out, P.Lpix);
         }
     }
+    ATLAS_HIP_TRY(hipGetLastError());
+    return ATLAS_RT_OK;
+}
+
+atlas_rt_status launch_frame_realtime(PathTracer& P, const Camera& cam, const atlas_rt_frame_matrices& M,
+                                      hipStream_t s) {
+    const uint32_t pixels = P.width * P.height;
+    const uint32_t spf = uint32_t(P.rt.samples_per_frame);
+    // PathTracerRenderTarget::Swap: last frame's buffers become the history
+    P.rt_cur ^= 1u;
+    const uint32_t cur = P.rt_cur, hist = cur ^ 1u;
+    ShadeArgs& A = P.shade_args;
+    A.rt.depth = P.rt_depth[cur];
+    A.rt.velocity = P.rt_velocity[cur];
+    A.rt.normal = P.rt_normal[cur];
+    A.rt.material = P.rt_material[cur];
+    A.rt.last_xform = P.rt_last_xform;
+    std::memcpy(A.rt.view, M.view, sizeof(A.rt.view));
+    std::memcpy(A.rt.proj, M.projection, sizeof(A.rt.proj));
+    std::memcpy(A.rt.last_vp, M.last_view_projection, sizeof(A.rt.last_vp));
+    A.rt.spf = spf;
+    A.rt.enabled = 1;
+    hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(64), 0, s, P.ctl, P.bounces, P.rays);
+    hipLaunchKernelGGL(k_raygen_rt, dim3((P.rays + kShadeBlock - 1) / kShadeBlock), dim3(kShadeBlock), 0, s, cam,
+                       P.width, P.height, spf, M.jitter[0], M.jitter[1], P.paths[0], P.Lpix);
+    const atlas_rt_status st = launch_bounces(P, s);
+    A.rt.enabled = 0;
+    if (st != ATLAS_RT_OK) return st;
+    hipLaunchKernelGGL(k_frame_rt, dim3((pixels + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P.Lpix, pixels, spf,
+                       P.rt.max_radiance, P.rt_frame);
+    TemporalArgs T;
+    T.width = P.width;
+    T.height = P.height;
+    T.frame = P.rt_frame;
+    T.depth = P.rt_depth[cur];
+    T.velocity = P.rt_velocity[cur];
+    T.normal = P.rt_normal[cur];
+    T.material = P.rt_material[cur];
+    T.hdepth = P.rt_depth[hist];
+    T.hnormal = P.rt_normal[hist];
+    T.hmaterial = P.rt_material[hist];
+    T.hradiance = P.rt_radiance[hist];
+    T.out = P.rt_radiance[cur];
+    T.history_clip_max = P.rt.history_clip_max;
+    T.current_clip_factor = P.rt.current_clip_factor;
+    T.max_history_length = float(P.rt.history_length_max);
+    const atlas_rt_status ts = launch_temporal(T, s);
+    if (ts != ATLAS_RT_OK) return ts;
+    // this frame's transforms are the next frame's last ones
+    const size_t n = 3 * P.world->instances.size();
+    ATLAS_HIP_TRY(hipMemcpyAsync(P.rt_last_xform, P.world->d_xform, n * sizeof(float4), hipMemcpyDeviceToDevice, s));
+    P.rt_frames++;
+    ATLAS_HIP_TRY(hipGetLastError());
+    return ATLAS_RT_OK;
+}
+
+atlas_rt_status launch_frame(PathTracer& P, const Camera& cam, hipStream_t s) {
+    const uint32_t pixels = P.width * P.height;
+    const uint32_t grid_all = (pixels + kBlock - 1) / kBlock;
+    const CacheArgs& C = P.shade_args.cache;
+    hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(64), 0, s, P.ctl, P.bounces, pixels);
+    hipLaunchKernelGGL(k_raygen, dim3((pixels + kShadeBlock - 1) / kShadeBlock), dim3(kShadeBlock), 0, s, cam, P.width,
+                       P.height, P.ctl, P.paths[0], P.Lpix, C, P.bounces);
+    const atlas_rt_status bs = launch_bounces(P, s);
+    if (bs != ATLAS_RT_OK) return bs;
     if (C.enabled) {
         hipLaunchKernelGGL(k_build_targets, dim3((P.train_cap + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P.ctl, C,
                            P.bounces, P.s_q, P.s_t);

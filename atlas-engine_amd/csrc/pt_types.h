@@ -20,7 +20,8 @@ struct Material {
     int32_t two_sided;
     int32_t opacity_texture;  // index into the texture table; outside it = none
     int32_t invert_uvs;
-    int32_t reserved[2];
+    int32_t id;  // RaytraceMaterial.ID: the realtime G-buffer's material index
+    int32_t reserved;
 };
 
 // atlas_rt_gpu_light / GPULight (64 B), read as UnpackLight does
@@ -76,6 +77,22 @@ struct FrameCtl {
     uint32_t train_samples;  // training samples built this frame
     float train_loss;        // loss of the frame's last training step
     uint32_t pad2[3];
+};
+
+// Realtime mode's G-buffer, written by the first bounce of each pixel's
+// first sample (rayHit.csh:76-117); enabled = 0 outside realtime mode.
+// Matrices are glm column-major mat4 (m[4 * column + row]).
+struct RealtimeArgs {
+    float* depth;             // view-space z, the shaders' INF (1e12) on a miss
+    uint32_t* velocity;       // 2 x binary16: (ndc last - ndc now) / 2
+    uint32_t* normal;         // 2 x binary16: EncodeNormal(geometry normal)
+    uint16_t* material;       // material ID, 65535 on a miss
+    const float4* last_xform; // last frame's object-to-world rows, 3 float4 per instance slot
+    float view[16];
+    float proj[16];
+    float last_vp[16];
+    uint32_t spf;             // samples per frame: ray ID = pixel * spf + sample
+    uint32_t enabled;
 };
 
 // Radiance-cache buffers the bounce kernels write (all null in plain mode).

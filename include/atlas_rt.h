@@ -495,7 +495,9 @@ typedef struct atlas_rt_material {
     int32_t two_sided;
     int32_t opacity_texture;
     int32_t invert_uvs;
-    int32_t reserved[2];
+    int32_t id;           /* RaytraceMaterial.ID (hash % 65535, RayTracingWorld.cpp:194): the
+                             realtime G-buffer's material index (16 bits kept) */
+    int32_t reserved;
 } atlas_rt_material; /* 64 B */
 
 /* A punctual light as the engine's LightComponent describes it
@@ -574,6 +576,52 @@ atlas_rt_status atlas_rt_pathtracer_destroy(atlas_rt_pathtracer* pt);
  * (RandomHelper.cpp:8-51) and the sub-pixel jitter rayGen.csh:33-34. */
 atlas_rt_status atlas_rt_pathtracer_render(atlas_rt_pathtracer* pt, const atlas_rt_camera* camera, uint32_t frames,
                                            void* stream);
+
+/* Realtime mode (PathTracingRenderer with realTime, PathTracingRenderer.cpp:
+ * 29-292): every frame traces samples_per_frame paths per pixel through the
+ * pixel centre plus the camera jitter (rayGen.csh:39-40), writes a G-buffer at
+ * the first bounce (rayHit.csh:76-117: velocity from this and last frame's
+ * matrices, view-space depth, octahedral geometry normal, material ID), sums
+ * the samples (float radiance for one sample, else the engine's quantised
+ * sum, rayHit.csh:133-149) and resolves them against the history
+ * (pathtracer/temporal.csh).  Defaults in PathTracingRenderer.h:38-43. */
+typedef struct atlas_rt_realtime_params {
+    int32_t samples_per_frame;   /* realTimeSamplesPerFrame (4) */
+    int32_t history_length_max;  /* historyLengthMax (32) */
+    float history_clip_max;      /* historyClipMax (0.2) */
+    float current_clip_factor;   /* currentClipFactor (2.0) */
+    float max_radiance;          /* maxRadiance (65535) */
+} atlas_rt_realtime_params;
+
+/* The frame's camera matrices as the engine's globalData holds them: glm
+ * column-major mat4 (m[column][row]); jitter = jitterCurrent. */
+typedef struct atlas_rt_frame_matrices {
+    float view[4][4];
+    float projection[4][4];
+    float last_view_projection[4][4];
+    float jitter[2];
+    float reserved[2];
+} atlas_rt_frame_matrices;
+
+#define ATLAS_RT_RT_RESOLVED 0u  /* float4 per pixel: resolved radiance, history length (outAccumImage) */
+#define ATLAS_RT_RT_FRAME 1u     /* float4 per pixel: this frame's radiance as temporal.csh fetches it */
+#define ATLAS_RT_RT_DEPTH 2u     /* float per pixel: view-space z, +inf on a miss */
+#define ATLAS_RT_RT_VELOCITY 3u  /* 2 x binary16 per pixel (R16G16_SFLOAT) */
+#define ATLAS_RT_RT_NORMAL 4u    /* 2 x binary16 per pixel: EncodeNormal(geometry normal) */
+#define ATLAS_RT_RT_MATERIAL 5u  /* uint16 per pixel: material ID, 65535 on a miss */
+
+/* Switches realtime mode on (params) or off (NULL); resets the history.
+ * Not with a radiance cache attached. */
+atlas_rt_status atlas_rt_pathtracer_set_realtime(atlas_rt_pathtracer* pt, const atlas_rt_realtime_params* params);
+/* One realtime frame on `stream`.  Instances' last-frame matrices are the
+ * world's transforms before its latest atlas_rt_world_refit (the current ones
+ * after an update). */
+atlas_rt_status atlas_rt_pathtracer_render_realtime(atlas_rt_pathtracer* pt, const atlas_rt_camera* camera,
+                                                    const atlas_rt_frame_matrices* matrices, void* stream);
+/* Copies one of this frame's ATLAS_RT_RT_* buffers (width * height elements)
+ * to device memory d_out. */
+atlas_rt_status atlas_rt_pathtracer_read_realtime(atlas_rt_pathtracer* pt, uint32_t buffer, void* d_out,
+                                                  void* stream);
 
 /* Opacity textures referenced by atlas_rt_material.opacity_texture; replaces
  * the previous table (the scene's bindless textures). */

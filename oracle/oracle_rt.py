@@ -371,6 +371,73 @@ def pathtrace(tlas_nodes, instances, xform, blases, shading, materials, lights, 
     return accum
 
 
+def pathtrace_realtime(tlas_nodes, instances, xform, blases, shading, materials, lights, env, cam, width, height,
+                       bounces, spf, matrices, last_xform, max_radiance=65535.0, pcg_draws=0, threads=None,
+                       surface=None):
+    """One realtime frame: (frame (h, w, 4) radiance as the temporal pass
+    fetches it, depth (h, w), velocity (h, w) uint32 binary16 pairs, normal
+    (h, w) uint32 binary16 pairs, material (h, w) uint16).  matrices: a
+    ctypes atlas_rt_frame_matrices; last_xform: (m, 12) last frame's rows."""
+    args, keep = _scene_args(tlas_nodes, instances, xform, blases, shading, materials, lights, env)
+    ext = _surface_arg(surface, len(blases), keep)
+    L = lib()
+    if not getattr(L, "_ptr_bound", False):
+        vp, u32, i32 = C.c_void_p, C.c_uint32, C.c_int32
+        L.oracle_pathtrace_realtime.restype = None
+        L.oracle_pathtrace_realtime.argtypes = _SCENE_TYPES + [vp, u32, u32, i32, u32, C.c_uint64, vp, vp, vp, vp, vp,
+                                                               C.c_float, vp, vp, vp, vp, vp, i32, vp]
+        L._ptr_bound = True
+    frame = np.zeros((height, width, 4), np.float32)
+    depth = np.zeros((height, width), np.float32)
+    vel = np.zeros((height, width), np.uint32)
+    nrm = np.zeros((height, width), np.uint32)
+    mat = np.zeros((height, width), np.uint16)
+    last = np.ascontiguousarray(last_xform, dtype=np.float32)
+    m = matrices
+    L.oracle_pathtrace_realtime(*args, C.addressof(cam), width, height, bounces, spf, pcg_draws,
+                                C.addressof(m.view), C.addressof(m.projection), C.addressof(m.last_view_projection),
+                                C.addressof(m.jitter), _p(last), max_radiance, _p(frame), _p(depth), _p(vel), _p(nrm),
+                                _p(mat), threads or threads_default(), ext)
+    _collect_lights(surface)
+    return frame, depth, vel, nrm, mat
+
+
+def temporal(frame, depth, velocity, normal, material, hdepth, hnormal, hmaterial, hradiance,
+             history_clip_max=0.2, current_clip_factor=2.0, max_history_length=32.0):
+    """The temporal resolve restated: (h, w, 4) (resolved rgb, history length)."""
+    h, w = depth.shape
+    L = lib()
+    if not getattr(L, "_tmp_bound", False):
+        L.oracle_temporal.restype = None
+        L.oracle_temporal.argtypes = [C.c_uint32, C.c_uint32] + [C.c_void_p] * 11
+        L._tmp_bound = True
+    arrs = [np.ascontiguousarray(a, dtype=t) for a, t in
+            ((frame, np.float32), (depth, np.float32), (velocity, np.uint32), (normal, np.uint32),
+             (material, np.uint16), (hdepth, np.float32), (hnormal, np.uint32), (hmaterial, np.uint16),
+             (hradiance, np.float32))]
+    params = np.array([history_clip_max, current_clip_factor, max_history_length], np.float32)
+    out = np.zeros((h, w, 4), np.float32)
+    L.oracle_temporal(w, h, *[_p(a) for a in arrs], _p(params), _p(out))
+    return out
+
+
+def f2h(values):
+    """binary16 bits (round to nearest even) of float32 values, by the oracle."""
+    L = lib()
+    if not getattr(L, "_f2h_bound", False):
+        L.oracle_f2h.restype = C.c_uint16
+        L.oracle_f2h.argtypes = [C.c_float]
+        L.oracle_h2f.restype = C.c_float
+        L.oracle_h2f.argtypes = [C.c_uint16]
+        L._f2h_bound = True
+    return np.array([L.oracle_f2h(float(v)) for v in np.asarray(values, np.float32).ravel()], np.uint16)
+
+
+def h2f(bits):
+    f2h([0.0])
+    return np.array([lib().oracle_h2f(int(b)) for b in np.asarray(bits, np.uint16).ravel()], np.float32)
+
+
 def pathtrace_cache(tlas_nodes, instances, xform, blases, shading, materials, lights, env, cam, width, height,
                     bounces, train_stride, frame_index=0, sample_index=0, pcg_draws=0, threads=None, surface=None):
     """One frame of the radiance-cache mode.  Returns (radiance (h, w, 3)

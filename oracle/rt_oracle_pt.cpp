@@ -107,6 +107,7 @@ struct Mat {
     V3 base, emissive;
     float opacity, roughness, metalness, ao, reflectance;
     int32_t two_sided, opacity_texture, invert_uvs;
+    int32_t id;  // RaytraceMaterial.ID
 };
 
 struct Tex {
@@ -298,8 +299,16 @@ struct CacheOut {
     float* rec = nullptr;
 };
 
+// The first bounce's surface, for the realtime G-buffer (rayHit.csh:76-117).
+struct Primary {
+    bool hit = false;
+    V3 P{0, 0, 0}, Ng{0, 0, 0};
+    int32_t material_id = 0;
+    int32_t instance = -1;
+};
+
 V3 trace_path(const Scene& S, V3 o, V3 d, float pixel_seed, const float* seeds, std::vector<int32_t>& stack,
-              const CacheOut& co = CacheOut{}) {
+              const CacheOut& co = CacheOut{}, Primary* prim = nullptr) {
     V3 thr{1, 1, 1}, Lr{0, 0, 0};
     for (int32_t bounce = 0; bounce <= S.bounces; bounce++) {
         const V3 thr_in = thr;
@@ -309,6 +318,7 @@ V3 trace_path(const Scene& S, V3 o, V3 d, float pixel_seed, const float* seeds, 
         const Opacity op{&S, tri_opacity};
         closest_rgpu_world(S.w, o, d, 0.0f, kInf, ATLAS_RT_MASK_ALL, h, st, stack, S.alpha ? &op : nullptr);
         if (h.slot < 0) {
+            if (prim && bounce == 0) prim->hit = false;
             V3 e = S.env * thr;
             const V3 ec{std::fmin(e.x, 10.0f), std::fmin(e.y, 10.0f), std::fmin(e.z, 10.0f)};
             Lr = Lr + ec;
@@ -353,6 +363,13 @@ V3 trace_path(const Scene& S, V3 o, V3 d, float pixel_seed, const float* seeds, 
         sf.Vv = neg(d);
         sf.N = nrm(normal);
         sf.Ng = Ng;
+        if (prim && bounce == 0) {
+            prim->hit = true;
+            prim->P = sf.P;
+            prim->Ng = Ng;
+            prim->material_id = m.id;
+            prim->instance = h.instance;
+        }
         const float q12[12] = {sf.P.x, sf.P.y, sf.P.z, m.roughness, -d.x, -d.y, -d.z, 0.0f,
                                sf.N.x, sf.N.y, sf.N.z, 0.0f};
         if (co.query && bounce == 1) {
@@ -621,7 +638,7 @@ void make_scene(Scene& S, std::vector<BlasView>& views, const atlas_rt_gpu_bvh_n
         const auto& m = mats[i];
         S.mats.push_back(Mat{V3{m.base_color[0], m.base_color[1], m.base_color[2]},
                              V3{m.emissive[0], m.emissive[1], m.emissive[2]}, m.opacity, m.roughness, m.metalness,
-                             m.ao, m.reflectance, m.two_sided, m.opacity_texture, m.invert_uvs});
+                             m.ao, m.reflectance, m.two_sided, m.opacity_texture, m.invert_uvs, m.id});
     }
     if (ext)
         for (uint32_t t = 0; t < ext->texture_count; t++)
@@ -828,6 +845,164 @@ void oracle_pathtrace_cache(const atlas_rt_gpu_bvh_node* tlas, uint32_t tlas_cou
             radiance[4 * size_t(pix) + 1] = L.y;
             radiance[4 * size_t(pix) + 2] = L.z;
             radiance[4 * size_t(pix) + 3] = 1.0f;
+        }
+    });
+}
+
+// IEEE binary16 of a float, round to nearest even (the R16G16_SFLOAT store).
+uint16_t oracle_f2h(float f) {
+    uint32_t x;
+    std::memcpy(&x, &f, 4);
+    const uint32_t sign = (x >> 16) & 0x8000u, mant = x & 0x7FFFFFu;
+    const int32_t exp = int32_t((x >> 23) & 0xFFu);
+    if (exp == 0xFF) return uint16_t(sign | 0x7C00u | (mant ? 0x200u | (mant >> 13) : 0u));
+    const int32_t e = exp - 127 + 15;
+    if (e >= 31) return uint16_t(sign | 0x7C00u);
+    if (e <= 0) {
+        if (e < -10) return uint16_t(sign);
+        const uint32_t m = mant | 0x800000u;
+        const uint32_t shift = uint32_t(14 - e);
+        uint32_t h = m >> shift;
+        const uint32_t rem = m & ((1u << shift) - 1u), half = 1u << (shift - 1u);
+        if (rem > half || (rem == half && (h & 1u))) h++;
+        return uint16_t(sign | h);
+    }
+    uint32_t h = (uint32_t(e) << 10) | (mant >> 13);
+    const uint32_t rem = mant & 0x1FFFu;
+    if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) h++;
+    return uint16_t(sign | h);
+}
+
+float oracle_h2f(uint16_t h) {
+    const uint32_t sign = uint32_t(h & 0x8000u) << 16;
+    const uint32_t exp = (h >> 10) & 0x1Fu, mant = h & 0x3FFu;
+    uint32_t x;
+    if (exp == 0) {
+        if (mant == 0) {
+            x = sign;
+        } else {  // subnormal: renormalise
+            int32_t e = -1;
+            uint32_t m = mant;
+            do {
+                e++;
+                m <<= 1;
+            } while (!(m & 0x400u));
+            x = sign | (uint32_t(127 - 15 - e) << 23) | ((m & 0x3FFu) << 13);
+        }
+    } else if (exp == 31) {
+        x = sign | 0x7F800000u | (mant << 13);
+    } else {
+        x = sign | ((exp - 15 + 127) << 23) | (mant << 13);
+    }
+    float f;
+    std::memcpy(&f, &x, 4);
+    return f;
+}
+
+// One realtime frame (PathTracingRenderer with realTime; rayGen.csh and
+// rayHit.csh with REALTIME): spf paths per pixel (ray ID = pixel * spf +
+// sample) through the pixel centre plus jitter / 4, seeds after pcg_draws
+// draws; per pixel the radiance as temporal.csh's FetchTexel reads it
+// (frame[4]), and from sample 0's first bounce the G-buffer: view-space depth
+// (1e12 on a miss), velocity and EncodeNormal as binary16 pairs (low half =
+// first component), material ID (65535 on a miss).  view / proj / last_vp:
+// glm column-major mat4; last_xform: last frame's object-to-world rows (12
+// floats per instance slot).
+void oracle_pathtrace_realtime(const atlas_rt_gpu_bvh_node* tlas, uint32_t tlas_count,
+                               const atlas_rt_gpu_bvh_instance* inst, uint32_t inst_count,
+                               const atlas_rt_gpu_bvh_node* const* blas_nodes, const uint32_t* blas_node_counts,
+                               const atlas_rt_gpu_bvh_triangle* const* blas_tris, const uint32_t* blas_tri_counts,
+                               const float* const* shade_normals, const int32_t* const* shade_mat,
+                               uint32_t blas_count, const float* xform, const atlas_rt_material* mats,
+                               uint32_t n_mats, const atlas_rt_light* lights, uint32_t n_lights, const float* env,
+                               const atlas_rt_camera* cam, uint32_t width, uint32_t height, int32_t bounces,
+                               uint32_t spf, uint64_t pcg_draws, const float* view, const float* proj,
+                               const float* last_vp, const float* jitter, const float* last_xform,
+                               float max_radiance, float* frame, float* depth, uint32_t* velocity,
+                               uint32_t* normal, uint16_t* material, int32_t threads, const OracleSurface* ext) {
+    Scene S;
+    std::vector<BlasView> views;
+    make_scene(S, views, tlas, tlas_count, inst, inst_count, blas_nodes, blas_node_counts, blas_tris, blas_tri_counts,
+               shade_normals, shade_mat, blas_count, xform, mats, n_mats, lights, n_lights, env, bounces, ext);
+    Pcg pcg{0ull + 1442695040888963407ull};
+    pcg.next();
+    for (uint64_t i = 0; i < pcg_draws; i++) pcg.next();
+    std::vector<float> seeds(size_t(bounces) + 1);
+    for (int32_t b = 0; b <= bounces; b++) seeds[b] = pcg.fast_uniform();
+    const V3 loc{cam->location[0], cam->location[1], cam->location[2]};
+    const V3 org{cam->origin[0], cam->origin[1], cam->origin[2]};
+    const V3 rgt{cam->right[0], cam->right[1], cam->right[2]};
+    const V3 bot{cam->bottom[0], cam->bottom[1], cam->bottom[2]};
+    auto gmax = [](float x, float y) { return (x < y) ? y : x; };
+    auto gmin = [](float x, float y) { return (y < x) ? y : x; };
+    auto mat4 = [](const float* m, float x, float y, float z, float w, float out[4]) {
+        for (int k = 0; k < 4; k++) out[k] = (m[k] * x + m[4 + k] * y) + (m[8 + k] * z + m[12 + k] * w);
+    };
+    parallel_pixels(width * height, threads, [&](uint32_t lo, uint32_t hi) {
+        std::vector<int32_t> stack;
+        for (uint32_t pix = lo; pix < hi; pix++) {
+            const uint32_t px = pix % width, py = pix / width;
+            const float cx = jitter[0] * 0.25f + (float(px) + 0.5f) / float(width);
+            const float cy = jitter[1] * 0.25f + (float(py) + 0.5f) / float(height);
+            const V3 dir = nrm(((org + mul(rgt, cx)) + mul(bot, cy)) - loc);
+            Primary prim;
+            std::vector<V3> L(spf);
+            for (uint32_t k = 0; k < spf; k++)
+                L[k] = trace_path(S, loc, dir, float(pix * spf + k), seeds.data(), stack, CacheOut{},
+                                  k == 0 ? &prim : nullptr);
+            float c[3];
+            if (spf == 1) {
+                c[0] = L[0].x;
+                c[1] = L[0].y;
+                c[2] = L[0].z;
+            } else {
+                const uint32_t per = 0xFFFFFFFFu / spf;
+                uint32_t sum[3] = {0u, 0u, 0u};
+                for (uint32_t k = 0; k < spf; k++) {
+                    const float v[3] = {L[k].x, L[k].y, L[k].z};
+                    for (int a = 0; a < 3; a++) {
+                        const float q = gmin(gmax(v[a] / max_radiance, 0.0f), 1.0f) * float(per);
+                        const uint32_t u = q >= 0.0f ? uint32_t(q) : 0u;
+                        sum[a] += std::min(u, per);
+                    }
+                }
+                for (int a = 0; a < 3; a++) c[a] = float(sum[a]) / float(per) / float(spf) * max_radiance;
+            }
+            for (int a = 0; a < 3; a++) frame[4 * size_t(pix) + a] = gmax(c[a], 0.0f);
+            frame[4 * size_t(pix) + 3] = 0.0f;
+            // G-buffer (rayHit.csh:76-117)
+            V3 P, Ng;
+            float inv[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0}, last[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+            uint32_t mat_id = 65535u;
+            if (prim.hit) {
+                P = prim.P;
+                Ng = prim.Ng;
+                mat_id = uint32_t(prim.material_id) & 0xFFFFu;
+                std::memcpy(inv, inst[prim.instance].inverse_matrix, 48);
+                std::memcpy(last, last_xform + 12 * size_t(prim.instance), 48);
+            } else {
+                P = loc + mul(dir, 100.0f);
+                Ng = V3{1.0f, 0.0f, 0.0f};
+            }
+            const V3 obj = xf_glm(inv, P, 1.0f);
+            const V3 lp = xf(last, obj, 1.0f);
+            float vs[4], pc[4], pl[4];
+            mat4(view, P.x, P.y, P.z, 1.0f, vs);
+            mat4(proj, vs[0], vs[1], vs[2], vs[3], pc);
+            mat4(last_vp, lp.x, lp.y, lp.z, 1.0f, pl);
+            const float vx = (pl[0] / pl[3] - pc[0] / pc[3]) * 0.5f;
+            const float vy = (pl[1] / pl[3] - pc[1] / pc[3]) * 0.5f;
+            velocity[pix] = uint32_t(oracle_f2h(vx)) | (uint32_t(oracle_f2h(vy)) << 16);
+            depth[pix] = prim.hit ? vs[2] : 1.0e12f;
+            material[pix] = uint16_t(mat_id);
+            const float l1 = (std::fabs(Ng.x) + std::fabs(Ng.y)) + std::fabs(Ng.z);
+            float nx = Ng.x / l1, nz = Ng.z / l1;
+            if (Ng.y < 0.0f) {
+                const float a = nx, b = nz;
+                nx = (a >= 0.0f ? 1.0f : -1.0f) * (1.0f - std::fabs(b));
+                nz = (b >= 0.0f ? 1.0f : -1.0f) * (1.0f - std::fabs(a));
+            }
+            normal[pix] = uint32_t(oracle_f2h(nx)) | (uint32_t(oracle_f2h(nz)) << 16);
         }
     });
 }
