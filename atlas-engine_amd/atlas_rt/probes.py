@@ -19,6 +19,8 @@ one --pmc counter per run).  Run from the package directory's parent:
   c2 [binned]     the config-2 path trace (256 instances, 1280 x 720,
                   4 bounces): 2 untimed frames, then 16, optionally with
                   ray binning
+  realtime        the same scene in realtime mode (4 samples per pixel,
+                  G-buffer, temporal resolve): 2 untimed frames, then 16
 """
 import csv
 import glob
@@ -78,6 +80,33 @@ def c2(binning):
     p.render(cam, 16)
     torch.cuda.synchronize()
     print(f"c2 binning={binning}: {(time.perf_counter() - t) / 16 * 1e3:.3f} ms/frame", flush=True)
+
+
+def realtime():
+    """Realtime frames of the C2 scene: 4 samples per pixel, temporal resolve."""
+    import torch
+
+    torch.cuda.init()
+    from . import PathTracer, camera, frame_matrices, projection_matrix, view_matrix
+    from . import workloads as W
+    from .bench_suite import _world
+    meshes, mats_d, xf, which = W.c2_scene()
+    w, blases = _world(meshes, xf, which)
+    cc = W.c2_camera()
+    cam = camera(cc["eye"], cc["yaw_deg"], cc["pitch_deg"], cc["fov_y_deg"], cc["width"] / cc["height"])
+    p = PathTracer(w, cc["width"], cc["height"], 4, W.c2_materials(mats_d), W.sun_light(), env=W.C2_ENV)
+    p.set_realtime(samples_per_frame=4)
+    view = view_matrix(cc["eye"], cc["yaw_deg"], cc["pitch_deg"])
+    proj = projection_matrix(cc["fov_y_deg"], cc["width"] / cc["height"])
+    fm = frame_matrices(view, proj, proj @ view)
+    for _ in range(2):
+        p.render_realtime(cam, fm)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(16):
+        p.render_realtime(cam, fm)
+    torch.cuda.synchronize()
+    print(f"c2 realtime: {(time.perf_counter() - t) / 16 * 1e3:.3f} ms/frame", flush=True)
 
 
 def nrc():
@@ -189,6 +218,8 @@ if __name__ == "__main__":
         pt(sys.argv[2] if len(sys.argv) > 2 else "plain", *[int(a) for a in sys.argv[3:5]])
     elif cmd == "c2":
         c2(len(sys.argv) > 2 and sys.argv[2] == "binned")
+    elif cmd == "realtime":
+        realtime()
     elif cmd == "nrc":
         nrc()
     elif cmd == "lbvh":
