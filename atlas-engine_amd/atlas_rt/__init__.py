@@ -422,6 +422,16 @@ def _blas_set_opacity(self, opacity=None, uvs=None):
 Blas.set_opacity = _blas_set_opacity
 
 
+def _blas_set_tangents(self, tangents):
+    """Per-triangle tangent and bitangent ((n, 6), caller order) for normal maps."""
+    tangents = np.ascontiguousarray(tangents, dtype=np.float32).reshape(-1, 6)
+    self._tangents = tangents
+    check(lib().atlas_rt_blas_set_tangents(self._h, _ptr(tangents)))
+
+
+Blas.set_tangents = _blas_set_tangents
+
+
 def camera(eye, yaw_deg, pitch_deg, fov_y_deg, aspect, near=0.1):
     """atlas_rt_camera from an eye point and yaw/pitch (y up), built like
     CameraComponent::GetFrustumCorners' near-plane corners 4, 5, 6."""
@@ -559,11 +569,13 @@ class PathTracer:
         return n.value
 
     def set_textures(self, textures):
-        """Opacity textures: a list of (height, width) uint8 arrays (R8 UNORM)."""
+        """Material textures: a list of (height, width) or (height, width,
+        channels) uint8 arrays (UNORM8, 1-4 channels)."""
         keep = [np.ascontiguousarray(t, dtype=np.uint8) for t in textures]
         arr = (_abi.Texture * max(len(keep), 1))()
         for i, t in enumerate(keep):
-            arr[i].height, arr[i].width = t.shape
+            arr[i].height, arr[i].width = t.shape[:2]
+            arr[i].channels = t.shape[2] if t.ndim == 3 else 1
             arr[i].texels = t.ctypes.data
         check(lib().atlas_rt_pathtracer_set_textures(self._h, arr if keep else None, len(keep)))
 

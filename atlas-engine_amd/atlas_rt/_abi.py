@@ -79,18 +79,23 @@ class LaunchConfig(C.Structure):
 MATERIAL_DTYPE = np.dtype([
     ("base_color", "<f4", 3), ("opacity", "<f4"), ("emissive", "<f4", 3), ("roughness", "<f4"),
     ("metalness", "<f4"), ("ao", "<f4"), ("reflectance", "<f4"), ("two_sided", "<i4"),
-    ("opacity_texture", "<i4"), ("invert_uvs", "<i4"), ("id", "<i4"), ("reserved", "<i4"),
+    ("opacity_texture", "<i4"), ("invert_uvs", "<i4"), ("id", "<i4"), ("normal_scale", "<f4"),
+    ("base_color_texture", "<i4"), ("roughness_texture", "<i4"), ("metalness_texture", "<i4"),
+    ("ao_texture", "<i4"), ("normal_texture", "<i4"), ("reserved", "<i4", 3),
 ])
+TEXTURE_FIELDS = ("base_color_texture", "opacity_texture", "roughness_texture", "metalness_texture", "ao_texture",
+                  "normal_texture")
 LIGHT_DTYPE = np.dtype([("direction", "<f4", 3), ("type", "<u4"), ("radiance", "<f4", 3), ("reserved", "<f4")])
 GPU_LIGHT_DTYPE = np.dtype([("P", "<f4", 4), ("N", "<f4", 4), ("color", "<f4", 4), ("data", "<f4", 4)])
-assert MATERIAL_DTYPE.itemsize == 64 and LIGHT_DTYPE.itemsize == 32 and GPU_LIGHT_DTYPE.itemsize == 64
+assert MATERIAL_DTYPE.itemsize == 96 and LIGHT_DTYPE.itemsize == 32 and GPU_LIGHT_DTYPE.itemsize == 64
 LIGHT_DIRECTIONAL = 0
 LIGHT_TRIANGLE = 1
 
 
 class Texture(C.Structure):
-    """atlas_rt_texture (R8 UNORM, row-major, host memory)."""
-    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("texels", C.c_void_p)]
+    """atlas_rt_texture (UNORM8, 1-4 channels, row-major, host memory)."""
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("texels", C.c_void_p), ("channels", C.c_uint32),
+                ("reserved", C.c_uint32)]
 
 
 class Camera(C.Structure):
@@ -180,6 +185,7 @@ PROTOTYPES = {
     "atlas_rt_aabb_tree_build": (status, [vp, u32, u32, vp, C.POINTER(u32), vp, vp, C.POINTER(u32)]),
     "atlas_rt_blas_set_shading": (status, [vp, vp, vp]),
     "atlas_rt_blas_set_opacity": (status, [vp, vp, vp]),
+    "atlas_rt_blas_set_tangents": (status, [vp, vp]),
     "atlas_rt_blas_build_device": (status, [vp, vp, u32, u32, vp, vp, C.POINTER(f32)]),
     "atlas_rt_trace_closest_transparency": (status, [vp, vp, vp, vp, u32, f32, f32, u32, vp]),
     "atlas_rt_trace_transparency": (status, [vp, vp, vp, vp, u32, f32, f32, u32, vp]),

@@ -329,7 +329,10 @@ def c2_materials(mats):
     out["metalness"] = mats.get("metalness", np.zeros(n, np.float32))
     out["ao"] = 1.0
     out["reflectance"] = 0.5
-    out["opacity_texture"] = -1
+    out["normal_scale"] = 1.0
+    for f in ("base_color_texture", "opacity_texture", "roughness_texture", "metalness_texture", "ao_texture",
+              "normal_texture"):
+        out[f] = -1
     return out
 
 

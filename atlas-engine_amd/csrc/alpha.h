@@ -13,12 +13,13 @@
 namespace atlas_rt {
 namespace pt {
 
-// SampleOpacityBilinear (raytracer/texture.hsh:18-25) with the engine's global
-// sampler (linear filter, repeat addressing, MainRenderer.cpp:922-929) at
-// level 0 over R8 texels; an index outside the table reads as 1.
-__device__ __forceinline__ float sample_opacity(const AlphaArgs& A, int32_t id, float x, float y) {
-    if (id < 0 || uint32_t(id) >= A.tex_count) return 1.0f;
+// textureLod with the engine's global sampler (linear filter, repeat
+// addressing, MainRenderer.cpp:922-929) at level 0 over UNORM8 texels, as
+// the Sample*Bilinear functions of raytracer/texture.hsh read it; channels a
+// texture lacks read g = b = 0, a = 1.  Callers check the index.
+__device__ __forceinline__ float4 sample_texture(const AlphaArgs& A, int32_t id, float x, float y) {
     const TexDesc T = A.tex[id];
+    const uint32_t ch = T.channels ? T.channels : 1u;
     const float xw = x - floorf(x), yw = y - floorf(y);
     const float fx = xw * float(T.width) - 0.5f, fy = yw * float(T.height) - 0.5f;
     const float x0 = floorf(fx), y0 = floorf(fy);
@@ -28,13 +29,26 @@ __device__ __forceinline__ float sample_opacity(const AlphaArgs& A, int32_t id, 
     const uint32_t j0 = iy < 0 ? T.height - 1 : min(uint32_t(iy), T.height - 1);
     const uint32_t i1 = i0 + 1 >= T.width ? 0u : i0 + 1, j1 = j0 + 1 >= T.height ? 0u : j0 + 1;
     const uint8_t* px = A.texels + T.offset;
-    const float t00 = float(px[size_t(j0) * T.width + i0]) / 255.0f;
-    const float t10 = float(px[size_t(j0) * T.width + i1]) / 255.0f;
-    const float t01 = float(px[size_t(j1) * T.width + i0]) / 255.0f;
-    const float t11 = float(px[size_t(j1) * T.width + i1]) / 255.0f;
-    const float r0 = t00 * (1.0f - a) + t10 * a;
-    const float r1 = t01 * (1.0f - a) + t11 * a;
-    return r0 * (1.0f - b) + r1 * b;
+    float out[4] = {0.0f, 0.0f, 0.0f, 1.0f};
+    for (uint32_t c = 0; c < ch; c++) {
+        const float t00 = float(px[(size_t(j0) * T.width + i0) * ch + c]) / 255.0f;
+        const float t10 = float(px[(size_t(j0) * T.width + i1) * ch + c]) / 255.0f;
+        const float t01 = float(px[(size_t(j1) * T.width + i0) * ch + c]) / 255.0f;
+        const float t11 = float(px[(size_t(j1) * T.width + i1) * ch + c]) / 255.0f;
+        const float r0 = t00 * (1.0f - a) + t10 * a;
+        const float r1 = t01 * (1.0f - a) + t11 * a;
+        out[c] = r0 * (1.0f - b) + r1 * b;
+    }
+    return make_float4(out[0], out[1], out[2], out[3]);
+}
+
+__device__ __forceinline__ bool has_texture(const AlphaArgs& A, int32_t id) {
+    return id >= 0 && uint32_t(id) < A.tex_count;
+}
+
+// SampleOpacityBilinear (raytracer/texture.hsh:18-25): .r, 1 without a map.
+__device__ __forceinline__ float sample_opacity(const AlphaArgs& A, int32_t id, float x, float y) {
+    return has_texture(A, id) ? sample_texture(A, id, x, y).x : 1.0f;
 }
 
 // r * uv0 + s * uv1 + t * uv2 with r = 1 - s - t (surface.hsh:81-87).

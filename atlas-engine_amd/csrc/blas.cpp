@@ -278,6 +278,7 @@ atlas_rt_status atlas_rt_blas_destroy(atlas_rt_blas* b) {
     }
     if (b->d_shade) hipFree(b->d_shade);
     if (b->d_uv) hipFree(b->d_uv);
+    if (b->d_tangent) hipFree(b->d_tangent);
     b->spill.release();
     b->flags.release();
     b->timing.release();
@@ -327,6 +328,29 @@ atlas_rt_status atlas_rt_blas_set_shading(atlas_rt_blas* b, const float* normals
     b->has_alpha = false;
     b->shade_normals.swap(host_n);
     b->shade_mat.swap(host_m);
+    return ATLAS_RT_OK;
+}
+
+atlas_rt_status atlas_rt_blas_set_tangents(atlas_rt_blas* b, const float* tangents) {
+    if (!b || !tangents) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_blas_set_tangents: NULL argument");
+    if (!b->d_shade)
+        return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_blas_set_tangents: call atlas_rt_blas_set_shading first");
+    const size_t n = b->tris.size();
+    std::vector<float> packed(8 * n, 0.0f);
+    for (size_t i = 0; i < n; i++) {
+        const float* t = tangents + 6 * size_t(b->orig_idx[i]);
+        std::memcpy(&packed[8 * i], t, 12);
+        std::memcpy(&packed[8 * i + 4], t + 3, 12);
+    }
+    void* d = nullptr;
+    ATLAS_HIP_TRY(hipMalloc(&d, packed.size() * sizeof(float)));
+    const hipError_t e = hipMemcpy(d, packed.data(), packed.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        hipFree(d);
+        return hip_fail(e, "atlas_rt_blas_set_tangents");
+    }
+    if (b->d_tangent) hipFree(b->d_tangent);
+    b->d_tangent = reinterpret_cast<float4*>(d);
     return ATLAS_RT_OK;
 }
 

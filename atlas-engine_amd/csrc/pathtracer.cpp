@@ -65,7 +65,8 @@ void free_realtime(PathTracer& P) {
 
 void free_all(PathTracer& P) {
     void* ptrs[] = {P.ctl, P.paths[0], P.paths[1], P.hits, P.shadow, P.Lpix, P.accum,
-                    const_cast<float4**>(P.shade_table), const_cast<float4**>(P.uv_table), P.materials, P.lights,
+                    const_cast<float4**>(P.shade_table), const_cast<float4**>(P.uv_table),
+                    const_cast<float4**>(P.tangent_table), P.materials, P.lights,
                     P.tex, P.texels, P.binned};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -251,6 +252,7 @@ atlas_rt_status atlas_rt_pathtracer_create(const atlas_rt_world* world, const at
     if (st == ATLAS_RT_OK) st = dev_alloc(P.accum, pixels);
     if (st == ATLAS_RT_OK) st = dev_alloc(P.shade_table, world->blas_handles.size());
     if (st == ATLAS_RT_OK) st = dev_alloc(P.uv_table, world->blas_handles.size());
+    if (st == ATLAS_RT_OK) st = dev_alloc(P.tangent_table, world->blas_handles.size());
     if (st == ATLAS_RT_OK) st = dev_alloc(P.materials, material_count);
     if (st == ATLAS_RT_OK) st = P.flags.ensure(256);
     if (st != ATLAS_RT_OK) {
@@ -258,10 +260,12 @@ atlas_rt_status atlas_rt_pathtracer_create(const atlas_rt_world* world, const at
         delete pt;
         return st;
     }
-    std::vector<const float4*> shade(world->blas_handles.size()), uv(world->blas_handles.size());
+    std::vector<const float4*> shade(world->blas_handles.size()), uv(world->blas_handles.size()),
+        tangent(world->blas_handles.size());
     for (size_t k = 0; k < shade.size(); k++) {
         shade[k] = world->blas_handles[k]->d_shade;
         uv[k] = world->blas_handles[k]->d_uv;
+        tangent[k] = world->blas_handles[k]->d_tangent;
     }
     FrameCtl ctl{};
     ctl.pcg_state = pcg_initial_state();
@@ -274,6 +278,9 @@ atlas_rt_status atlas_rt_pathtracer_create(const atlas_rt_world* world, const at
                       hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(const_cast<float4**>(P.uv_table), uv.data(), uv.size() * sizeof(float4*), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(const_cast<float4**>(P.tangent_table), tangent.data(), tangent.size() * sizeof(float4*),
+                      hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(P.materials, materials, material_count * sizeof(Material), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         free_all(P);
@@ -291,6 +298,7 @@ atlas_rt_status atlas_rt_pathtracer_create(const atlas_rt_world* world, const at
     A.world = P.world_dev;
     A.xform = P.xform;
     A.shade = P.shade_table;
+    A.tangent = P.tangent_table;
     A.materials = P.materials;
     A.lights = P.lights;
     A.light_count = P.light_count;
@@ -423,10 +431,12 @@ atlas_rt_status atlas_rt_pathtracer_set_textures(atlas_rt_pathtracer* pt, const 
         const atlas_rt_texture& T = textures[t];
         if (!T.width || !T.height || !T.texels)
             return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_set_textures: empty texture");
-        if (bytes + size_t(T.width) * T.height > 0xFFFFFFFFull)
+        if (T.channels > 4) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_set_textures: 1 to 4 channels");
+        const uint32_t ch = T.channels ? T.channels : 1u;
+        if (bytes + size_t(T.width) * T.height * ch > 0xFFFFFFFFull)
             return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_set_textures: more than 4 GiB of texels");
-        desc[t] = TexDesc{uint32_t(bytes), T.width, T.height, 0u};
-        bytes += size_t(T.width) * T.height;
+        desc[t] = TexDesc{uint32_t(bytes), T.width, T.height, ch};
+        bytes += size_t(T.width) * T.height * ch;
     }
     ATLAS_HIP_TRY(hipDeviceSynchronize());
     if (P.tex) (void)hipFree(P.tex);
@@ -444,7 +454,8 @@ atlas_rt_status atlas_rt_pathtracer_set_textures(atlas_rt_pathtracer* pt, const 
     ATLAS_HIP_TRY(hipMemcpy(P.tex, desc.data(), count * sizeof(TexDesc), hipMemcpyHostToDevice));
     for (uint32_t t = 0; t < count; t++)
         ATLAS_HIP_TRY(hipMemcpy(P.texels + desc[t].offset, textures[t].texels,
-                                size_t(textures[t].width) * textures[t].height, hipMemcpyHostToDevice));
+                                size_t(textures[t].width) * textures[t].height * desc[t].channels,
+                                hipMemcpyHostToDevice));
     P.tex_count = count;
     P.shade_args.alpha.tex = P.tex;
     P.shade_args.alpha.texels = P.texels;

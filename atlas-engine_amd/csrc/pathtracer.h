@@ -14,6 +14,7 @@ struct ShadeArgs {
     WorldDev world;
     const float4* xform;          // object-to-world rows, 3 float4 per packed instance
     const float4* const* shade;   // per BLAS: 3 float4 per slot (n0 + material, n1 + opacity, n2)
+    const float4* const* tangent; // per BLAS: 2 float4 per slot (tangent, bitangent), or null
     const Material* materials;
     const Light* lights;
     int32_t light_count;
@@ -47,6 +48,7 @@ struct PathTracer {
     const float4* xform = nullptr;      // borrowed: the world's transform table
     const float4** shade_table = nullptr;
     const float4** uv_table = nullptr;
+    const float4** tangent_table = nullptr;
     Material* materials = nullptr;
     uint32_t material_count = 0;
     Light* lights = nullptr;

@@ -362,11 +362,27 @@ __device__ __forceinline__ void shade_one(const ShadeArgs& A, int32_t bounce, ui
         const float bs = dot(q, s) / den;
         const float bt = dot(p, rd) / den;
         const float br = 1.0f - bs - bt;
-        if (m.opacity_texture >= 0 && uint32_t(m.opacity_texture) < A.alpha.tex_count) {
-            // surface.hsh:86-105: texture coordinate from these barycentrics
-            float2 tc = uv_at(A.alpha.uv[mesh], slot, bs, bt);
+        // material maps (surface.hsh:86-124) at the texture coordinate of
+        // these barycentrics
+        const AlphaArgs& TX = A.alpha;
+        float2 tc = make_float2(0.0f, 0.0f);
+        if (has_texture(TX, m.base_color_texture) || has_texture(TX, m.opacity_texture) ||
+            has_texture(TX, m.roughness_texture) || has_texture(TX, m.metalness_texture) ||
+            has_texture(TX, m.ao_texture) || has_texture(TX, m.normal_texture)) {
+            tc = uv_at(TX.uv[mesh], slot, bs, bt);
             if (m.invert_uvs > 0) tc.y = 1.0f - tc.y;
-            m.opacity *= sample_opacity(A.alpha, m.opacity_texture, tc.x, tc.y);
+            if (has_texture(TX, m.base_color_texture)) {
+                const float4 c = sample_texture(TX, m.base_color_texture, tc.x, tc.y);
+                m.base[0] *= c.x;
+                m.base[1] *= c.y;
+                m.base[2] *= c.z;
+            }
+            if (has_texture(TX, m.opacity_texture)) m.opacity *= sample_texture(TX, m.opacity_texture, tc.x, tc.y).x;
+            if (has_texture(TX, m.roughness_texture))
+                m.roughness *= sample_texture(TX, m.roughness_texture, tc.x, tc.y).x;
+            if (has_texture(TX, m.metalness_texture))
+                m.metalness *= sample_texture(TX, m.metalness_texture, tc.x, tc.y).x;
+            if (has_texture(TX, m.ao_texture)) m.ao *= sample_texture(TX, m.ao_texture, tc.x, tc.y).x;
         }
         Surface S;
         S.m = m;
@@ -375,6 +391,24 @@ __device__ __forceinline__ void shade_one(const ShadeArgs& A, int32_t bounce, ui
         V Ng = normalize(cross(v0 - v1, v0 - v2));
         const bool flip = dot(Ng, rd) > 0.0f;
         Ng = Ng * ((flip && m.two_sided > 0) ? -1.0f : 1.0f);
+        if (has_texture(TX, m.normal_texture)) {
+            // TBN = (tangent, +-bitangent, interpolated normal); without a
+            // tangent frame both are zero
+            V tg = mk(0.0f, 0.0f, 0.0f), bg = mk(0.0f, 0.0f, 0.0f);
+            const float4* tp = A.tangent ? A.tangent[mesh] : nullptr;
+            if (tp) {
+                const float4 ta = tp[2 * size_t(slot)], tb = tp[2 * size_t(slot) + 1];
+                tg = normalize(row_xform(M, mk(ta.x, ta.y, ta.z), 0.0f));
+                bg = normalize(row_xform(M, mk(tb.x, tb.y, tb.z), 0.0f));
+            }
+            const V bit = m.invert_uvs > 0 ? bg : -bg;
+            const float4 c = sample_texture(TX, m.normal_texture, tc.x, tc.y);
+            const float tx = 2.0f * c.x - 1.0f, ty = 2.0f * c.y - 1.0f, tz = 2.0f * c.z - 1.0f;
+            V tn = (tg * tx + bit * ty) + normal * tz;
+            tn = tn / glsl_max(sqrtf(dot(tn, tn)), 0.00001f);
+            const float w = glsl_min(glsl_max(m.normal_scale, 0.0f), 1.0f);
+            normal = normal * (1.0f - w) + tn * w;
+        }
         if (dot(normal, Ng) < 0.0f && m.two_sided > 0) normal = -normal;
         S.V_ = -rd;
         S.N = normalize(normal);

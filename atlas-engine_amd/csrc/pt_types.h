@@ -7,7 +7,7 @@
 namespace atlas_rt {
 namespace pt {
 
-// atlas_rt_material (64 B): the RaytraceMaterial fields the bounce shader
+// atlas_rt_material (96 B): the RaytraceMaterial fields the bounce shader
 // reads (raytracer/structures.hsh:107-135, surface.hsh:63-140).
 struct Material {
     float base[3];
@@ -21,7 +21,13 @@ struct Material {
     int32_t opacity_texture;  // index into the texture table; outside it = none
     int32_t invert_uvs;
     int32_t id;  // RaytraceMaterial.ID: the realtime G-buffer's material index
-    int32_t reserved;
+    float normal_scale;
+    int32_t base_color_texture;
+    int32_t roughness_texture;
+    int32_t metalness_texture;
+    int32_t ao_texture;
+    int32_t normal_texture;
+    int32_t reserved[3];
 };
 
 // atlas_rt_gpu_light / GPULight (64 B), read as UnpackLight does
@@ -34,12 +40,13 @@ struct Light {
     float data[4];
 };
 
-// One opacity texture of the path tracer's table: R8 texels at `offset`.
+// One texture of the path tracer's table: UNORM8 texels at `offset`,
+// `channels` per texel.
 struct TexDesc {
     uint32_t offset;
     uint32_t width;
     uint32_t height;
-    uint32_t pad;
+    uint32_t channels;
 };
 
 // What a kernel needs to evaluate a triangle's opacity.  materials = null:

@@ -81,6 +81,8 @@ struct atlas_rt_blas {
     //  d_uv   : optional, 2 x float4 per slot: (uv0, uv1), (uv2, 0, 0).
     float4* d_shade = nullptr;
     float4* d_uv = nullptr;
+    //  d_tangent: optional, 2 x float4 per slot: (tangent, 0), (bitangent, 0).
+    float4* d_tangent = nullptr;
     bool has_alpha = false;             // some baked opacity is not 1
     std::vector<float> shade_normals;   // host copy, 9 floats per slot
     std::vector<int32_t> shade_mat;     // host copy, 1 per slot

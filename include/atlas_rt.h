@@ -478,12 +478,25 @@ atlas_rt_status atlas_rt_blas_set_shading(atlas_rt_blas* blas, const float* norm
  * Call after atlas_rt_blas_set_shading. */
 atlas_rt_status atlas_rt_blas_set_opacity(atlas_rt_blas* blas, const float* opacity, const float* uvs);
 
+/* Per-triangle tangent frame for normal maps (GPUTriangle's tangent and
+ * bitangent, MeshData.cpp:222-237; transformed and normalised per instance as
+ * TransformTriangle does, surface.hsh:46-61): 6 floats per triangle (t, bt) in
+ * the caller's original order.  Without it both are zero.  Call after
+ * atlas_rt_blas_set_shading. */
+atlas_rt_status atlas_rt_blas_set_tangents(atlas_rt_blas* blas, const float* tangents);
+
 /* RaytraceMaterial / GPUMaterial subset used by the bounce shader
  * (RTStructures.h:51-78, raytracer/structures.hsh:107-135, surface.hsh:63-140).
- * opacity_texture indexes the path tracer's texture table
+ * The *_texture fields index the path tracer's texture table
  * (atlas_rt_pathtracer_set_textures); -1, or any index outside the table,
- * means no opacity map (SampleOpacityBilinear returns 1).  Base colour,
- * normal, roughness, metalness and AO maps are not supported. */
+ * means no map (the Sample*Bilinear functions of raytracer/texture.hsh
+ * return 1).  Maps are sampled at the triangle's texture coordinate
+ * (atlas_rt_blas_set_opacity's uvs) as GetSurfaceParameters does
+ * (raytracer/surface.hsh:86-130): base colour .rgb, opacity / roughness /
+ * metalness / AO .r multiply the material's values; the normal map's
+ * 2 * rgb - 1 goes through (tangent, +-bitangent, normal)
+ * (atlas_rt_blas_set_tangents) and is mixed in by saturate(normal_scale).
+ * Vertex colours are not supported. */
 typedef struct atlas_rt_material {
     float base_color[3];
     float opacity;
@@ -497,8 +510,14 @@ typedef struct atlas_rt_material {
     int32_t invert_uvs;
     int32_t id;           /* RaytraceMaterial.ID (hash % 65535, RayTracingWorld.cpp:194): the
                              realtime G-buffer's material index (16 bits kept) */
-    int32_t reserved;
-} atlas_rt_material; /* 64 B */
+    float normal_scale;   /* normalScale: weight of the normal map */
+    int32_t base_color_texture;  /* texture table indices, -1 or outside the table = none */
+    int32_t roughness_texture;
+    int32_t metalness_texture;
+    int32_t ao_texture;
+    int32_t normal_texture;
+    int32_t reserved[3];
+} atlas_rt_material; /* 96 B */
 
 /* A punctual light as the engine's LightComponent describes it
  * (RayTracingHelper::UpdateLights, RayTracingHelper.cpp:436-500): directional
@@ -525,13 +544,16 @@ typedef struct atlas_rt_gpu_light {
     float data[4];
 } atlas_rt_gpu_light; /* 64 B */
 
-/* An opacity texture: R8 UNORM texels, row-major, width * height bytes in
- * host memory (copied).  Sampled bilinearly with repeat addressing at level 0
- * (the engine's global sampler, MainRenderer.cpp:922-929). */
+/* A material texture: UNORM8 texels with `channels` channels, row-major,
+ * width * height * channels bytes in host memory (copied).  Sampled
+ * bilinearly with repeat addressing at level 0 (the engine's global sampler,
+ * MainRenderer.cpp:922-929). */
 typedef struct atlas_rt_texture {
     uint32_t width;
     uint32_t height;
     const uint8_t* texels;
+    uint32_t channels;  /* 1 (or 0) .. 4 UNORM8 channels per texel; missing g, b read 0 and a reads 1 */
+    uint32_t reserved;
 } atlas_rt_texture;
 
 /* RayGenUniforms (PathTracingRenderer.h:48-57): origin/right/bottom are the

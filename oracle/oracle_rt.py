@@ -270,13 +270,15 @@ def _scene_args(tlas_nodes, instances, xform, blases, shading, materials, lights
 
 
 class _Texture(C.Structure):
-    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("texels", C.c_void_p)]
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("texels", C.c_void_p), ("channels", C.c_uint32),
+                ("reserved", C.c_uint32)]
 
 
 class _Surface(C.Structure):
     _fields_ = [("opacity", C.c_void_p), ("uvs", C.c_void_p), ("textures", C.c_void_p),
                 ("texture_count", C.c_uint32), ("emitters", C.c_void_p), ("emitter_count", C.c_uint32),
-                ("lights_out", C.c_void_p), ("lights_cap", C.c_uint32), ("lights_count", C.c_void_p)]
+                ("lights_out", C.c_void_p), ("lights_cap", C.c_uint32), ("lights_count", C.c_void_p),
+                ("tangents", C.c_void_p)]
 
 
 GPU_LIGHT_DTYPE = np.dtype([("P", "<f4", 4), ("N", "<f4", 4), ("color", "<f4", 4), ("data", "<f4", 4)])
@@ -284,8 +286,9 @@ GPU_LIGHT_DTYPE = np.dtype([("P", "<f4", 4), ("N", "<f4", 4), ("color", "<f4", 4
 
 def _surface_arg(surface, n_blas, keep):
     """OracleSurface from a dict with optional keys: opacity (per BLAS, slot
-    order, or None), uvs (per BLAS, (n, 6) slot order, or None), textures
-    (list of (h, w) uint8), emitters ((m, 4) float32), lights_cap (int: the
+    order, or None), uvs (per BLAS, (n, 6) slot order, or None), tangents
+    (per BLAS, (n, 6) slot order, or None), textures (list of (h, w) or
+    (h, w, channels) uint8), emitters ((m, 4) float32), lights_cap (int: the
     light list is returned in surface["lights"])."""
     if surface is None:
         return None
@@ -305,13 +308,15 @@ def _surface_arg(surface, n_blas, keep):
         return C.cast(ptrs, C.c_void_p)
     S.opacity = per_blas("opacity", 1)
     S.uvs = per_blas("uvs", 6)
+    S.tangents = per_blas("tangents", 6)
     tex = surface.get("textures") or []
     if tex:
         arr = (_Texture * len(tex))()
         for i, t in enumerate(tex):
             t = np.ascontiguousarray(t, dtype=np.uint8)
             keep.append(t)
-            arr[i].height, arr[i].width = t.shape
+            arr[i].height, arr[i].width = t.shape[:2]
+            arr[i].channels = t.shape[2] if t.ndim == 3 else 1
             arr[i].texels = t.ctypes.data
         keep.append(arr)
         S.textures = C.cast(arr, C.c_void_p)

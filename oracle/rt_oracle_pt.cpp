@@ -42,6 +42,7 @@ struct OracleSurface {
     atlas_rt_gpu_light* lights_out;  // optional copy of the light list
     uint32_t lights_cap;
     uint32_t* lights_count;
+    const float* const* tangents;  // per BLAS: 6 floats per slot (tangent, bitangent), or NULL
 };
 
 namespace oracle {
@@ -108,19 +109,21 @@ struct Mat {
     float opacity, roughness, metalness, ao, reflectance;
     int32_t two_sided, opacity_texture, invert_uvs;
     int32_t id;  // RaytraceMaterial.ID
+    float normal_scale;
+    int32_t base_color_texture, roughness_texture, metalness_texture, ao_texture, normal_texture;
 };
 
 struct Tex {
     uint32_t w, h;
-    const uint8_t* texels;  // R8 UNORM, row-major
+    const uint8_t* texels;  // UNORM8, row-major, `ch` channels per texel
+    uint32_t ch = 1;
 };
 
-// SampleOpacityBilinear (raytracer/texture.hsh:18-25) with the engine's global
-// sampler (linear filter, repeat addressing, MainRenderer.cpp:922-929) at
-// level 0; an index outside the table reads as no texture (1.0).
-float sample_opacity(const std::vector<Tex>& tex, int32_t id, float x, float y) {
-    if (id < 0 || size_t(id) >= tex.size()) return 1.0f;
-    const Tex& T = tex[size_t(id)];
+// textureLod with the engine's global sampler (linear filter, repeat
+// addressing, MainRenderer.cpp:922-929) at level 0, as the Sample*Bilinear
+// functions of raytracer/texture.hsh read it: missing channels read g = b = 0,
+// a = 1.  The caller checks the index.
+void sample_tex(const Tex& T, float x, float y, float out[4]) {
     const float xw = x - std::floor(x), yw = y - std::floor(y);
     const float fx = xw * float(T.w) - 0.5f, fy = yw * float(T.h) - 0.5f;
     const float x0 = std::floor(fx), y0 = std::floor(fy);
@@ -129,10 +132,25 @@ float sample_opacity(const std::vector<Tex>& tex, int32_t id, float x, float y) 
     const uint32_t i0 = ix < 0 ? T.w - 1 : std::min(uint32_t(ix), T.w - 1);
     const uint32_t j0 = iy < 0 ? T.h - 1 : std::min(uint32_t(iy), T.h - 1);
     const uint32_t i1 = i0 + 1 >= T.w ? 0u : i0 + 1, j1 = j0 + 1 >= T.h ? 0u : j0 + 1;
-    auto at = [&](uint32_t i, uint32_t j) { return float(T.texels[size_t(j) * T.w + i]) / 255.0f; };
-    const float r0 = at(i0, j0) * (1.0f - a) + at(i1, j0) * a;
-    const float r1 = at(i0, j1) * (1.0f - a) + at(i1, j1) * a;
-    return r0 * (1.0f - b) + r1 * b;
+    out[0] = out[1] = out[2] = 0.0f;
+    out[3] = 1.0f;
+    for (uint32_t c = 0; c < T.ch; c++) {
+        auto at = [&](uint32_t i, uint32_t j) { return float(T.texels[(size_t(j) * T.w + i) * T.ch + c]) / 255.0f; };
+        const float r0 = at(i0, j0) * (1.0f - a) + at(i1, j0) * a;
+        const float r1 = at(i0, j1) * (1.0f - a) + at(i1, j1) * a;
+        out[c] = r0 * (1.0f - b) + r1 * b;
+    }
+}
+
+bool has_tex(const std::vector<Tex>& tex, int32_t id) { return id >= 0 && size_t(id) < tex.size(); }
+
+// SampleOpacityBilinear (raytracer/texture.hsh:18-25): .r; an index outside
+// the table reads as no texture (1.0).
+float sample_opacity(const std::vector<Tex>& tex, int32_t id, float x, float y) {
+    if (!has_tex(tex, id)) return 1.0f;
+    float v[4];
+    sample_tex(tex[size_t(id)], x, y, v);
+    return v[0];
 }
 
 struct Surf {
@@ -243,6 +261,7 @@ struct Scene {
     std::vector<std::vector<int32_t>> mat;    // per BLAS, slot order
     std::vector<std::vector<float>> opacity;  // per BLAS, slot order (MeshData.cpp:141), empty = opaque
     std::vector<std::vector<float>> uv;       // per BLAS, slot order: uv0, uv1, uv2 (6 floats), may be empty
+    std::vector<std::vector<float>> tangent;  // per BLAS, slot order: tangent, bitangent (6 floats), may be empty
     bool alpha = false;                       // some triangle is not opaque
     const float* xform;                       // 12 floats per packed instance
     std::vector<Mat> mats;
@@ -346,11 +365,25 @@ V3 trace_path(const Scene& S, V3 o, V3 d, float pixel_seed, const float* seeds, 
         float den = dot(q, e0);
         float dist = dot(p, e1) / den, bs = dot(q, s) / den, bt = dot(p, d) / den;
         float br = 1.0f - bs - bt;
-        if (m.opacity_texture >= 0 && size_t(m.opacity_texture) < S.tex.size()) {
-            // surface.hsh:86-105: texture coordinate from the world-space barycentrics
-            V3 tc = uv_at(S.uv[mesh], h.slot, bs, bt);
+        // material maps (surface.hsh:86-124) at the texture coordinate of the
+        // world-space barycentrics
+        V3 tc{0, 0, 0};
+        const int32_t maps[6] = {m.base_color_texture, m.opacity_texture, m.roughness_texture,
+                                 m.metalness_texture, m.ao_texture, m.normal_texture};
+        bool textured = false;
+        for (int32_t id : maps) textured = textured || has_tex(S.tex, id);
+        if (textured) {
+            tc = uv_at(S.uv[mesh], h.slot, bs, bt);
             if (m.invert_uvs > 0) tc.y = 1.0f - tc.y;
+            float v[4];
+            if (has_tex(S.tex, m.base_color_texture)) {
+                sample_tex(S.tex[size_t(m.base_color_texture)], tc.x, tc.y, v);
+                m.base = V3{m.base.x * v[0], m.base.y * v[1], m.base.z * v[2]};
+            }
             m.opacity *= sample_opacity(S.tex, m.opacity_texture, tc.x, tc.y);
+            if (has_tex(S.tex, m.roughness_texture)) m.roughness *= sample_opacity(S.tex, m.roughness_texture, tc.x, tc.y);
+            if (has_tex(S.tex, m.metalness_texture)) m.metalness *= sample_opacity(S.tex, m.metalness_texture, tc.x, tc.y);
+            if (has_tex(S.tex, m.ao_texture)) m.ao *= sample_opacity(S.tex, m.ao_texture, tc.x, tc.y);
         }
         Surf sf;
         sf.m = m;
@@ -359,6 +392,26 @@ V3 trace_path(const Scene& S, V3 o, V3 d, float pixel_seed, const float* seeds, 
         V3 Ng = nrm(cross(v0 - v1, v0 - v2));
         bool flip = dot(Ng, d) > 0.0f;
         Ng = mul(Ng, (flip && m.two_sided > 0) ? -1.0f : 1.0f);
+        if (has_tex(S.tex, m.normal_texture)) {
+            // TBN = mat3(tangent, +-bitangent, normal); texNormal /= max(length, 1e-5);
+            // mix by saturate(normalScale) (surface.hsh:111-124)
+            V3 tg{0, 0, 0}, bg{0, 0, 0};
+            if (!S.tangent[mesh].empty()) {
+                const float* tp = &S.tangent[mesh][6 * size_t(h.slot)];
+                tg = nrm(xf(M, V3{tp[0], tp[1], tp[2]}, 0.0f));
+                bg = nrm(xf(M, V3{tp[3], tp[4], tp[5]}, 0.0f));
+            }
+            const V3 bit = m.invert_uvs > 0 ? bg : neg(bg);
+            float v[4];
+            sample_tex(S.tex[size_t(m.normal_texture)], tc.x, tc.y, v);
+            const float tx = 2.0f * v[0] - 1.0f, ty = 2.0f * v[1] - 1.0f, tz = 2.0f * v[2] - 1.0f;
+            V3 tn = (mul(tg, tx) + mul(bit, ty)) + mul(normal, tz);
+            const float len = std::sqrt(dot(tn, tn));
+            const float den = (len < 0.00001f) ? 0.00001f : len;
+            tn = V3{tn.x / den, tn.y / den, tn.z / den};
+            const float wgt = (m.normal_scale < 0.0f) ? 0.0f : ((1.0f < m.normal_scale) ? 1.0f : m.normal_scale);
+            normal = mul(normal, 1.0f - wgt) + mul(tn, wgt);
+        }
         if (dot(normal, Ng) < 0.0f && m.two_sided > 0) normal = neg(normal);
         sf.Vv = neg(d);
         sf.N = nrm(normal);
@@ -625,6 +678,7 @@ void make_scene(Scene& S, std::vector<BlasView>& views, const atlas_rt_gpu_bvh_n
     S.mat.resize(blas_count);
     S.opacity.resize(blas_count);
     S.uv.resize(blas_count);
+    S.tangent.resize(blas_count);
     for (uint32_t k = 0; k < blas_count; k++) {
         S.shade[k].assign(shade_normals[k], shade_normals[k] + 9 * size_t(blas_tri_counts[k]));
         S.mat[k].assign(shade_mat[k], shade_mat[k] + blas_tri_counts[k]);
@@ -633,16 +687,21 @@ void make_scene(Scene& S, std::vector<BlasView>& views, const atlas_rt_gpu_bvh_n
             for (float o : S.opacity[k]) S.alpha = S.alpha || o != 1.0f;
         }
         if (ext && ext->uvs && ext->uvs[k]) S.uv[k].assign(ext->uvs[k], ext->uvs[k] + 6 * size_t(blas_tri_counts[k]));
+        if (ext && ext->tangents && ext->tangents[k])
+            S.tangent[k].assign(ext->tangents[k], ext->tangents[k] + 6 * size_t(blas_tri_counts[k]));
     }
     for (uint32_t i = 0; i < n_mats; i++) {
         const auto& m = mats[i];
         S.mats.push_back(Mat{V3{m.base_color[0], m.base_color[1], m.base_color[2]},
                              V3{m.emissive[0], m.emissive[1], m.emissive[2]}, m.opacity, m.roughness, m.metalness,
-                             m.ao, m.reflectance, m.two_sided, m.opacity_texture, m.invert_uvs, m.id});
+                             m.ao, m.reflectance, m.two_sided, m.opacity_texture, m.invert_uvs, m.id,
+                             m.normal_scale, m.base_color_texture, m.roughness_texture, m.metalness_texture,
+                             m.ao_texture, m.normal_texture});
     }
     if (ext)
         for (uint32_t t = 0; t < ext->texture_count; t++)
-            S.tex.push_back(Tex{ext->textures[t].width, ext->textures[t].height, ext->textures[t].texels});
+            S.tex.push_back(Tex{ext->textures[t].width, ext->textures[t].height, ext->textures[t].texels,
+                                ext->textures[t].channels ? ext->textures[t].channels : 1u});
     // UpdateLights (RayTracingHelper.cpp:436-500): punctual lights first, then
     // the triangle lights; weights floored at 0.5% of the maximum, normalised.
     for (uint32_t l = 0; l < n_lights; l++) {
@@ -716,7 +775,7 @@ extern "C" {
 
 // Bilinear opacity-texture sample (texture.hsh:18-25 with the global sampler).
 float oracle_sample_opacity(uint32_t width, uint32_t height, const uint8_t* texels, float x, float y) {
-    std::vector<Tex> t{Tex{width, height, texels}};
+    std::vector<Tex> t{Tex{width, height, texels, 1u}};
     return sample_opacity(t, 0, x, y);
 }
 
