@@ -63,7 +63,8 @@ def log(msg):
 
 # Hardware-counter summary of one traversal variant (kernel id -> file): the
 # traffic it reports belongs to that kernel only.
-PMC_SUMMARIES = {18: os.path.join(ROOT, "profiles", "k_blas_closest_persistent_regstack_c0_pmc.md")}
+PMC_SUMMARIES = {18: os.path.join(ROOT, "profiles", "k_blas_closest_persistent_regstack_c0_pmc.md"),
+                 50: os.path.join(ROOT, "profiles", "k_blas_closest_persistent_regstack_fslab_c0_pmc.md")}
 
 
 def pmc_traffic(kernel):
