@@ -29,6 +29,7 @@ KERNEL_BVH8 = 4
 KERNEL_VARIANT_MASK = 0xF
 KERNEL_STACK_REGISTER = 0x10
 KERNEL_SLAB_FLOAT = 0x20
+KERNEL_LEAF_BATCH = 0x40
 
 PT_RAY_BINNING = 0x1
 RAY_BINS = 64

@@ -33,6 +33,7 @@ BatchLaunch to_launch(const atlas_rt_launch_config& c) {
     L.node_w = base == ATLAS_RT_KERNEL_BVH4 ? 4 : (base == ATLAS_RT_KERNEL_BVH8 ? 8 : 2);
     L.reg = (c.kernel & ATLAS_RT_KERNEL_STACK_REGISTER) != 0;
     L.fslab = (c.kernel & ATLAS_RT_KERNEL_SLAB_FLOAT) != 0;
+    L.leaf_batch = (c.kernel & ATLAS_RT_KERNEL_LEAF_BATCH) != 0;
     L.block = c.block ? c.block : 256;
     L.refill = c.refill ? c.refill : 40;
     return L;
@@ -42,19 +43,21 @@ bool valid_kernel(int32_t k) {
     const int32_t base = k & ATLAS_RT_KERNEL_VARIANT_MASK;
     const bool fslab_ok = !(k & ATLAS_RT_KERNEL_SLAB_FLOAT) || base == ATLAS_RT_KERNEL_BVH2_EXACT ||
                           base == ATLAS_RT_KERNEL_BVH2_PERSISTENT;
-    return (k & ~int32_t(ATLAS_RT_KERNEL_VARIANT_MASK | ATLAS_RT_KERNEL_STACK_REGISTER |
-                         ATLAS_RT_KERNEL_SLAB_FLOAT)) == 0 &&
-           fslab_ok && base >= ATLAS_RT_KERNEL_BVH2_EXACT && base <= ATLAS_RT_KERNEL_BVH8;
+    const bool batch_ok = !(k & ATLAS_RT_KERNEL_LEAF_BATCH) || base == ATLAS_RT_KERNEL_BVH2_PERSISTENT;
+    return (k & ~int32_t(ATLAS_RT_KERNEL_VARIANT_MASK | ATLAS_RT_KERNEL_STACK_REGISTER | ATLAS_RT_KERNEL_SLAB_FLOAT |
+                         ATLAS_RT_KERNEL_LEAF_BATCH)) == 0 &&
+           fslab_ok && batch_ok && base >= ATLAS_RT_KERNEL_BVH2_EXACT && base <= ATLAS_RT_KERNEL_BVH8;
 }
 
 // Candidates: node width {2, 4, 8} x stack placement {LDS, register top} x
 // work-group size {64, 128, 256}, for binary nodes both launch shapes (one
-// ray per lane; resident waves at refill thresholds 16 / 32 / 48) and both
-// slab tests (engine-exact; widened float).
+// ray per lane; resident waves at refill thresholds 16 / 32 / 48), both slab
+// tests (engine-exact; widened float) and leaf-batched resident waves.
 std::vector<atlas_rt_launch_config> candidates() {
     std::vector<atlas_rt_launch_config> out;
     const int32_t reg = ATLAS_RT_KERNEL_STACK_REGISTER;
     const int32_t fs = ATLAS_RT_KERNEL_SLAB_FLOAT;
+    const int32_t lb = ATLAS_RT_KERNEL_LEAF_BATCH;
     for (uint32_t block : {64u, 128u, 256u}) {
         out.push_back({ATLAS_RT_KERNEL_BVH2_EXACT, block, 0, 0});
         out.push_back({ATLAS_RT_KERNEL_BVH2_EXACT | reg, block, 0, 0});
@@ -62,6 +65,9 @@ std::vector<atlas_rt_launch_config> candidates() {
         for (uint32_t refill : {16u, 32u, 48u}) out.push_back({ATLAS_RT_KERNEL_BVH2_PERSISTENT, block, refill, 0});
         out.push_back({ATLAS_RT_KERNEL_BVH2_PERSISTENT | reg, block, 32, 0});
         out.push_back({ATLAS_RT_KERNEL_BVH2_PERSISTENT | reg | fs, block, 32, 0});
+        out.push_back({ATLAS_RT_KERNEL_BVH2_PERSISTENT | lb, block, 32, 0});
+        out.push_back({ATLAS_RT_KERNEL_BVH2_PERSISTENT | reg | lb, block, 32, 0});
+        out.push_back({ATLAS_RT_KERNEL_BVH2_PERSISTENT | fs | lb, block, 32, 0});
         for (int32_t wide : {ATLAS_RT_KERNEL_BVH4, ATLAS_RT_KERNEL_BVH8}) {
             out.push_back({wide, block, 32, 0});
             out.push_back({wide | reg, block, 32, 0});

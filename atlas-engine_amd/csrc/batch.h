@@ -26,6 +26,7 @@ struct BatchLaunch {
     int node_w = 2;       // 2, 4 or 8 children per node
     bool reg = false;     // register-topped traversal stack
     bool fslab = false;   // widened float slab test (binary nodes)
+    bool leaf_batch = false;  // leaf-batched resident waves (binary nodes)
     uint32_t block = 256;
     uint32_t refill = 40;
     hipStream_t stream = nullptr;

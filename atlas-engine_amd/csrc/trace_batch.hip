@@ -38,7 +38,7 @@ __device__ __forceinline__ uint32_t find_batch(const uint32_t* __restrict__ pref
     return lo;
 }
 
-template <int BLOCK, int LDS_DEPTH, int NODE_W, bool REG, bool FLOAT_SLAB>
+template <int BLOCK, int LDS_DEPTH, int NODE_W, bool REG, bool FLOAT_SLAB, int LEAF_DIV>
 __global__ __launch_bounds__(BLOCK) void k_batch_persistent(BatchDev D, const uint32_t* __restrict__ d_count,
                                                             const float4* __restrict__ rays,
                                                             float4* __restrict__ hits, int32_t* __restrict__ spill,
@@ -54,7 +54,6 @@ __global__ __launch_bounds__(BLOCK) void k_batch_persistent(BatchDev D, const ui
     bool exhausted = false;
     uint32_t ray = 0;
     float tmax = 0.0f;
-    uint32_t dn = 0, dt = 0;
     for (;;) {
         if (!exhausted) {
             const uint64_t idle = __ballot(!active);
@@ -90,16 +89,14 @@ __global__ __launch_bounds__(BLOCK) void k_batch_persistent(BatchDev D, const ui
         }
         const uint32_t threshold = exhausted ? 1u : refill;
         for (;;) {
-            if (active) {
-                const int r = step_variant<NODE_W, REG, BLOCK, LDS_DEPTH, false, FLOAT_SLAB>(B, L, S, cap, dn, dt);
-                if (r != kContinue) {
-                    if (r == kOverflow) {
-                        atomicOr(&ctl[2], 1u);
-                        L.slot = -1;
-                    }
-                    hits[ray] = lane_hit_record(B, L, tmax);
-                    active = false;
+            const int r = wave_step<NODE_W, REG, BLOCK, LDS_DEPTH, FLOAT_SLAB, LEAF_DIV>(B, L, S, cap, active);
+            if (r != kContinue) {
+                if (r == kOverflow) {
+                    atomicOr(&ctl[2], 1u);
+                    L.slot = -1;
                 }
+                hits[ray] = lane_hit_record(B, L, tmax);
+                active = false;
             }
             if (uint32_t(__popcll(__ballot(active))) < threshold) break;
         }
@@ -147,18 +144,22 @@ __global__ __launch_bounds__(BLOCK) void k_batch_grid(BatchDev D, const uint32_t
     hits[ray] = lane_hit_record(B, L, r1.w);
 }
 
+// Leaf-batched waves take triangle steps once a quarter of their live lanes
+// wait at a leaf.
+constexpr int kLeafDiv = 4;
+
 // LDS stack rows per block size (LDS bytes = rows * BLOCK * 4).
 template <int BLOCK>
 struct Rows {
     static constexpr int value = BLOCK >= 256 ? 24 : 32;
 };
 
-template <int BLOCK, int NODE_W, bool REG, bool FLOAT_SLAB = false>
+template <int BLOCK, int NODE_W, bool REG, bool FLOAT_SLAB = false, int LEAF_DIV = 0>
 void launch_kernel(atlas_rt_batch_set& s, const BatchLaunch& L, uint32_t grid, uint32_t cap) {
     constexpr int D = Rows<BLOCK>::value;
     uint32_t* ctl = reinterpret_cast<uint32_t*>(s.flags.ptr);
     if (L.persistent)
-        hipLaunchKernelGGL((k_batch_persistent<BLOCK, D, NODE_W, REG, FLOAT_SLAB>), dim3(grid), dim3(BLOCK), 0,
+        hipLaunchKernelGGL((k_batch_persistent<BLOCK, D, NODE_W, REG, FLOAT_SLAB, LEAF_DIV>), dim3(grid), dim3(BLOCK), 0,
                            L.stream, s.dev(),
                            L.d_count, reinterpret_cast<const float4*>(L.rays), reinterpret_cast<float4*>(L.hits),
                            reinterpret_cast<int32_t*>(s.spill.ptr), cap, ctl, L.refill);
@@ -187,7 +188,14 @@ atlas_rt_status launch_one(atlas_rt_batch_set& s, const BatchLaunch& L) {
     }
     if (!L.persistent && L.node_w != 2)
         return fail(ATLAS_RT_E_UNSUPPORTED, "batched trace: wide nodes run in resident waves only");
-    if (L.node_w == 2 && L.fslab)
+    if (L.node_w == 2 && L.leaf_batch && L.persistent) {
+        if (L.fslab)
+            L.reg ? launch_kernel<BLOCK, 2, true, true, kLeafDiv>(s, L, grid, cap)
+                  : launch_kernel<BLOCK, 2, false, true, kLeafDiv>(s, L, grid, cap);
+        else
+            L.reg ? launch_kernel<BLOCK, 2, true, false, kLeafDiv>(s, L, grid, cap)
+                  : launch_kernel<BLOCK, 2, false, false, kLeafDiv>(s, L, grid, cap);
+    } else if (L.node_w == 2 && L.fslab)
         L.reg ? launch_kernel<BLOCK, 2, true, true>(s, L, grid, cap)
               : launch_kernel<BLOCK, 2, false, true>(s, L, grid, cap);
     else if (L.node_w == 2)

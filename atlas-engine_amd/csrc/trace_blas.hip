@@ -56,7 +56,8 @@ __global__ __launch_bounds__(BLOCK) void k_blas_closest(BlasDev B, const float4*
     if (STATS) visits[gid] = make_uint2(n_nodes, n_tris);
 }
 
-template <int BLOCK, int LDS_DEPTH, int NODE_W = 2, bool REG = false, bool FLOAT_SLAB = false>
+// LEAF_DIV > 0: leaf-batched waves (wave_step).
+template <int BLOCK, int LDS_DEPTH, int NODE_W = 2, bool REG = false, bool FLOAT_SLAB = false, int LEAF_DIV = 0>
 __global__ __launch_bounds__(BLOCK) void k_blas_closest_persistent(BlasDev B, const float4* __restrict__ rays,
                                                                    float4* __restrict__ hits,
                                                                    int32_t* __restrict__ slots, uint32_t count,
@@ -71,7 +72,6 @@ __global__ __launch_bounds__(BLOCK) void k_blas_closest_persistent(BlasDev B, co
     bool exhausted = false;
     uint32_t idx = 0;
     float tmax = 0.0f;
-    uint32_t dn = 0, dt = 0;
     for (;;) {
         if (!exhausted) {
             const uint64_t idle = __ballot(!active);
@@ -104,17 +104,15 @@ __global__ __launch_bounds__(BLOCK) void k_blas_closest_persistent(BlasDev B, co
         }
         const uint32_t threshold = exhausted ? 1u : refill;
         for (;;) {
-            if (active) {
-                const int r = step_variant<NODE_W, REG, BLOCK, LDS_DEPTH, false, FLOAT_SLAB>(B, L, S, cap, dn, dt);
-                if (r != kContinue) {
-                    if (r == kOverflow) {
-                        atomicOr(&ctl[2], 1u);
-                        L.slot = -1;
-                    }
-                    hits[idx] = lane_hit_record(B, L, tmax);
-                    if (slots) slots[idx] = L.slot;
-                    active = false;
+            const int r = wave_step<NODE_W, REG, BLOCK, LDS_DEPTH, FLOAT_SLAB, LEAF_DIV>(B, L, S, cap, active);
+            if (r != kContinue) {
+                if (r == kOverflow) {
+                    atomicOr(&ctl[2], 1u);
+                    L.slot = -1;
                 }
+                hits[idx] = lane_hit_record(B, L, tmax);
+                if (slots) slots[idx] = L.slot;
+                active = false;
             }
             if (uint32_t(__popcll(__ballot(active))) < threshold) break;
         }
@@ -208,14 +206,18 @@ static atlas_rt_status prepare(atlas_rt_blas& blas, uint32_t threads, uint32_t l
     return ATLAS_RT_OK;
 }
 
-template <int NODE_W, bool REG, bool FLOAT_SLAB = false>
+template <int NODE_W, bool REG, bool FLOAT_SLAB = false, int LEAF_DIV = 0>
 static void launch_persistent(const BlasDev& B, const TraceLaunch& L, uint32_t groups, int32_t* spill, uint32_t cap,
                               uint32_t* ctl) {
-    hipLaunchKernelGGL((k_blas_closest_persistent<kPBlock, kPLdsDepth, NODE_W, REG, FLOAT_SLAB>), dim3(groups),
+    hipLaunchKernelGGL((k_blas_closest_persistent<kPBlock, kPLdsDepth, NODE_W, REG, FLOAT_SLAB, LEAF_DIV>), dim3(groups),
                        dim3(kPBlock), 0,
                        L.stream, B, reinterpret_cast<const float4*>(L.rays), reinterpret_cast<float4*>(L.hits),
                        L.slots, L.count, spill, cap, ctl, L.refill ? L.refill : 40u);
 }
+
+// Leaf batching: triangle steps once a quarter of a wave's live lanes wait at
+// a leaf.
+constexpr int kLeafDiv = 4;
 
 atlas_rt_status launch_blas_closest(atlas_rt_blas& blas, const TraceLaunch& L) {
     if (L.count == 0) return ATLAS_RT_OK;
@@ -224,6 +226,7 @@ atlas_rt_status launch_blas_closest(atlas_rt_blas& blas, const TraceLaunch& L) {
     const int base = L.kernel & ATLAS_RT_KERNEL_VARIANT_MASK;
     const bool reg = (L.kernel & ATLAS_RT_KERNEL_STACK_REGISTER) != 0;
     const bool fslab = (L.kernel & ATLAS_RT_KERNEL_SLAB_FLOAT) != 0;
+    const bool leaf_batch = (L.kernel & ATLAS_RT_KERNEL_LEAF_BATCH) != 0;
     const bool persistent = !L.visits && base >= ATLAS_RT_KERNEL_BVH2_PERSISTENT;
     const int node_w = base == ATLAS_RT_KERNEL_BVH4 ? 4 : (base == ATLAS_RT_KERNEL_BVH8 ? 8 : 2);
     atlas_rt_status st;
@@ -239,7 +242,14 @@ atlas_rt_status launch_blas_closest(atlas_rt_blas& blas, const TraceLaunch& L) {
         st = blas.timing.begin(L.stream);
         if (st != ATLAS_RT_OK) return st;
         int32_t* spill = reinterpret_cast<int32_t*>(blas.spill.ptr);
-        if (node_w == 2 && fslab)
+        if (node_w == 2 && leaf_batch) {
+            if (fslab)
+                reg ? launch_persistent<2, true, true, kLeafDiv>(B, L, groups, spill, cap, ctl)
+                    : launch_persistent<2, false, true, kLeafDiv>(B, L, groups, spill, cap, ctl);
+            else
+                reg ? launch_persistent<2, true, false, kLeafDiv>(B, L, groups, spill, cap, ctl)
+                    : launch_persistent<2, false, false, kLeafDiv>(B, L, groups, spill, cap, ctl);
+        } else if (node_w == 2 && fslab)
             reg ? launch_persistent<2, true, true>(B, L, groups, spill, cap, ctl)
                 : launch_persistent<2, false, true>(B, L, groups, spill, cap, ctl);
         else if (node_w == 2)

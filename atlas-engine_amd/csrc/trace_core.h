@@ -222,12 +222,12 @@ __device__ __forceinline__ int leaf_step_closest(const BlasDev& B, LaneState& L,
     return kContinue;
 }
 
-// Binary nodes, engine order, any stack placement.  FLOAT_SLAB: the widened
-// float slab test (node_test_float) instead of the engine-exact one.
+// Inner-node step of a binary tree (L.node >= 0), engine order, any stack
+// placement.  FLOAT_SLAB: the widened float slab test (node_test_float)
+// instead of the engine-exact one.
 template <bool STATS, bool FLOAT_SLAB, class STK>
-__device__ __forceinline__ int step_closest_s(const BlasDev& B, LaneState& L, const STK& S, uint32_t cap,
-                                              uint32_t& n_nodes, uint32_t& n_tris) {
-    if (L.node < 0) return leaf_step_closest(B, L, S, n_tris, STATS);
+__device__ __forceinline__ int node_step_closest(const BlasDev& B, LaneState& L, const STK& S, uint32_t cap,
+                                                 uint32_t& n_nodes) {
     const float4* np = B.nodes + 4 * size_t(L.node);
     const float4 a = np[0], b = np[1], c = np[2], d = np[3];
     if (STATS) n_nodes++;
@@ -252,6 +252,14 @@ __device__ __forceinline__ int step_closest_s(const BlasDev& B, LaneState& L, co
         L.node = S.pop(L);
     }
     return kContinue;
+}
+
+// One step of a binary tree: an inner node or one triangle.
+template <bool STATS, bool FLOAT_SLAB, class STK>
+__device__ __forceinline__ int step_closest_s(const BlasDev& B, LaneState& L, const STK& S, uint32_t cap,
+                                              uint32_t& n_nodes, uint32_t& n_tris) {
+    if (L.node < 0) return leaf_step_closest(B, L, S, n_tris, STATS);
+    return node_step_closest<STATS, FLOAT_SLAB>(B, L, S, cap, n_nodes);
 }
 
 __device__ __forceinline__ float comp4(const float4& v, int k) {
@@ -354,6 +362,56 @@ __device__ __forceinline__ int step_variant(const BlasDev& B, LaneState& L, cons
         return step_closest_w<NODE_W, STATS>(B, L, RegStack<BLOCK, LDS_DEPTH>{S}, cap, n_nodes, n_tris);
     else
         return step_closest_w<NODE_W, STATS>(B, L, MemStack<BLOCK, LDS_DEPTH>{S}, cap, n_nodes, n_tris);
+}
+
+// The two halves of a binary step, for kernels that batch leaf work across a
+// wave: an inner-node step (L.node >= 0) and a triangle step (L.node < 0).
+template <bool REG, int BLOCK, int LDS_DEPTH, bool FLOAT_SLAB>
+__device__ __forceinline__ int node_step_variant(const BlasDev& B, LaneState& L, const LaneStack<BLOCK, LDS_DEPTH>& S,
+                                                 uint32_t cap) {
+    uint32_t n_nodes = 0;
+    if constexpr (REG)
+        return node_step_closest<false, FLOAT_SLAB>(B, L, RegStack<BLOCK, LDS_DEPTH>{S}, cap, n_nodes);
+    else
+        return node_step_closest<false, FLOAT_SLAB>(B, L, MemStack<BLOCK, LDS_DEPTH>{S}, cap, n_nodes);
+}
+
+template <bool REG, int BLOCK, int LDS_DEPTH>
+__device__ __forceinline__ int leaf_step_variant(const BlasDev& B, LaneState& L, const LaneStack<BLOCK, LDS_DEPTH>& S) {
+    uint32_t n_tris = 0;
+    if constexpr (REG)
+        return leaf_step_closest(B, L, RegStack<BLOCK, LDS_DEPTH>{S}, n_tris, false);
+    else
+        return leaf_step_closest(B, L, MemStack<BLOCK, LDS_DEPTH>{S}, n_tris, false);
+}
+
+// One wave iteration of a resident-wave kernel for an active lane.  LEAF_DIV
+// = 0: every active lane takes one step of its variant.  LEAF_DIV > 0
+// (binary nodes): the wave takes triangle steps only when at least
+// 1 / LEAF_DIV of its live lanes wait at a leaf, and inner-node steps for the
+// other lanes otherwise, so the triangle test is not paid by the whole wave
+// whenever one lane reaches a leaf.  Each lane's own sequence of steps is
+// unchanged.  Every lane of the wave must call it; returns kContinue for a
+// lane that did not step.
+template <int NODE_W, bool REG, int BLOCK, int LDS_DEPTH, bool FLOAT_SLAB, int LEAF_DIV>
+__device__ __forceinline__ int wave_step(const BlasDev& B, LaneState& L, const LaneStack<BLOCK, LDS_DEPTH>& S,
+                                         uint32_t cap, bool active) {
+    if constexpr (LEAF_DIV > 0) {
+        static_assert(NODE_W == 2, "leaf batching is for binary nodes");
+        const bool at_leaf = active && L.node < 0;
+        const uint32_t nl = uint32_t(__popcll(__ballot(at_leaf)));
+        const uint32_t na = uint32_t(__popcll(__ballot(active)));
+        if (nl * LEAF_DIV >= na) {  // wave-uniform
+            if (at_leaf) return leaf_step_variant<REG, BLOCK, LDS_DEPTH>(B, L, S);
+        } else if (active && !at_leaf) {
+            return node_step_variant<REG, BLOCK, LDS_DEPTH, FLOAT_SLAB>(B, L, S, cap);
+        }
+        return kContinue;
+    } else {
+        uint32_t dn = 0, dt = 0;
+        if (!active) return kContinue;
+        return step_variant<NODE_W, REG, BLOCK, LDS_DEPTH, false, FLOAT_SLAB>(B, L, S, cap, dn, dt);
+    }
 }
 
 // Stack bound of a variant for one BLAS (entries).

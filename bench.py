@@ -28,7 +28,8 @@ sys.path.insert(0, os.path.join(ROOT, "atlas-engine_amd"))
 HBM_PEAK_GBPS = 8000.0
 REG = 0x10
 FSLAB = 0x20
-EXACT_KERNELS = (1, 2, 1 | REG, 2 | REG)  # engine order and arithmetic: bit-exact slots
+LEAFB = 0x40
+EXACT_KERNELS = (1, 2, 1 | REG, 2 | REG, 2 | LEAFB, 2 | REG | LEAFB)  # engine order and arithmetic: bit-exact
 KERNEL_NAMES = {1: "bvh2-exact grid (engine order, bit-exact)",
                 2: "bvh2-exact persistent wave-refill (engine order, bit-exact)",
                 3: "bvh4 persistent (wide nodes, same nearest hit)",
@@ -40,7 +41,12 @@ KERNEL_NAMES = {1: "bvh2-exact grid (engine order, bit-exact)",
                 1 | FSLAB: "bvh2 grid, widened float slab (same nearest hit)",
                 2 | FSLAB: "bvh2 persistent, widened float slab (same nearest hit)",
                 1 | REG | FSLAB: "bvh2 grid, register stack top, widened float slab (same nearest hit)",
-                2 | REG | FSLAB: "bvh2 persistent, register stack top, widened float slab (same nearest hit)"}
+                2 | REG | FSLAB: "bvh2 persistent, register stack top, widened float slab (same nearest hit)",
+                2 | LEAFB: "bvh2-exact persistent, leaf-batched waves",
+                2 | REG | LEAFB: "bvh2-exact persistent, register stack top, leaf-batched waves",
+                2 | FSLAB | LEAFB: "bvh2 persistent, widened float slab, leaf-batched waves (same nearest hit)",
+                2 | REG | FSLAB | LEAFB: "bvh2 persistent, register stack top, widened float slab, leaf-batched "
+                                         "waves (same nearest hit)"}
 METRIC = "Mrays/s incoherent closest-hit; path-trace time to equal rel-MSE, cache vs plain"
 
 
@@ -64,7 +70,8 @@ def log(msg):
 # Hardware-counter summary of one traversal variant (kernel id -> file): the
 # traffic it reports belongs to that kernel only.
 PMC_SUMMARIES = {18: os.path.join(ROOT, "profiles", "k_blas_closest_persistent_regstack_c0_pmc.md"),
-                 50: os.path.join(ROOT, "profiles", "k_blas_closest_persistent_regstack_fslab_c0_pmc.md")}
+                 50: os.path.join(ROOT, "profiles", "k_blas_closest_persistent_regstack_fslab_c0_pmc.md"),
+                 66: os.path.join(ROOT, "profiles", "k_blas_closest_persistent_leafbatch_c0_pmc.md")}
 
 
 def pmc_traffic(kernel):
@@ -170,7 +177,7 @@ def main():
     if kernel == 0:
         # pick the fastest traversal variant (outside the timed region)
         for k in (1, 2, 3, 4, 1 | REG, 2 | REG, 3 | REG, 4 | REG, 1 | FSLAB, 2 | FSLAB, 1 | REG | FSLAB,
-                  2 | REG | FSLAB):
+                  2 | REG | FSLAB, 2 | LEAFB, 2 | REG | LEAFB, 2 | FSLAB | LEAFB, 2 | REG | FSLAB | LEAFB):
             for _ in range(3):
                 blas.trace_closest(d_rays, d_hits, kernel=k, stream=stream)
             torch.cuda.synchronize()

@@ -206,6 +206,12 @@ atlas_rt_status atlas_rt_blas_export(const atlas_rt_blas* blas, atlas_rt_gpu_bvh
  * order; which triangle wins an exact equal-t tie may differ, and a hit that
  * the engine's slab test culls at a box face may be found. */
 #define ATLAS_RT_KERNEL_SLAB_FLOAT 0x20
+/* Or'ing ATLAS_RT_KERNEL_LEAF_BATCH into ATLAS_RT_KERNEL_BVH2_PERSISTENT
+ * (with or without the two bits above) makes a wave take triangle steps only
+ * once a quarter of its live lanes wait at a leaf, inner-node steps
+ * otherwise.  Every ray's own traversal is unchanged, so results are those of
+ * the same kernel without the bit. */
+#define ATLAS_RT_KERNEL_LEAF_BATCH 0x40
 
 /* Closest hit, engine semantics of HitClosest (bvh.hsh:191-273, CheckLeafClosest
  * :44-72, IntersectAABB/IntersectTriangle intersections.hsh:18-58): accept

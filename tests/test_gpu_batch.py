@@ -43,8 +43,10 @@ def test_batched_all_launch_shapes_bit_exact(torch_cuda):
     bs = atlas_rt.BatchSet(blases, [(b, o, c, 0) for b, o, c in table])
     exp = _expected(meshes, rays, table, blases)
     reg = _abi.KERNEL_STACK_REGISTER
+    lb = _abi.KERNEL_LEAF_BATCH
     for kernel in (_abi.KERNEL_BVH2_EXACT, _abi.KERNEL_BVH2_PERSISTENT, _abi.KERNEL_BVH2_EXACT | reg,
-                   _abi.KERNEL_BVH2_PERSISTENT | reg):
+                   _abi.KERNEL_BVH2_PERSISTENT | reg, _abi.KERNEL_BVH2_PERSISTENT | lb,
+                   _abi.KERNEL_BVH2_PERSISTENT | reg | lb):
         for block in (64, 128, 256):
             got = _run(torch, bs, rays, kernel=kernel, block=block, refill=24)
             assert np.array_equal(rt_cases.hit_bits(got), rt_cases.hit_bits(exp)), (kernel, block)
@@ -53,7 +55,8 @@ def test_batched_all_launch_shapes_bit_exact(torch_cuda):
     # last bits, so compare hit/miss and the float32 t through the oracle)
     fs = _abi.KERNEL_SLAB_FLOAT
     for kernel in (_abi.KERNEL_BVH4, _abi.KERNEL_BVH8, _abi.KERNEL_BVH4 | reg, _abi.KERNEL_BVH8 | reg,
-                   _abi.KERNEL_BVH2_EXACT | reg | fs, _abi.KERNEL_BVH2_PERSISTENT | reg | fs):
+                   _abi.KERNEL_BVH2_EXACT | reg | fs, _abi.KERNEL_BVH2_PERSISTENT | reg | fs,
+                   _abi.KERNEL_BVH2_PERSISTENT | fs | lb):
         for block in (64, 256):
             got = _run(torch, bs, rays, kernel=kernel, block=block, refill=32)
             diff = np.nonzero(np.any(rt_cases.hit_bits(got) != rt_cases.hit_bits(exp), axis=1))[0]
@@ -64,9 +67,9 @@ def test_batched_all_launch_shapes_bit_exact(torch_cuda):
     assert np.array_equal(got["prim_id"] >= 0, exp["prim_id"] >= 0)
     assert np.allclose(got["t"], exp["t"], rtol=1e-6, atol=0)
     best, trials = bs.tuning()
-    assert len(trials) >= 36 and best["block"] in (64, 128, 256)
+    assert len(trials) >= 45 and best["block"] in (64, 128, 256)
     kinds = {int(t[0]["kernel"]) for t in trials}
-    assert {1, 2, 3, 4, 1 | reg, 3 | reg, 4 | reg} <= kinds
+    assert {1, 2, 3, 4, 1 | reg, 3 | reg, 4 | reg, 2 | lb, 2 | reg | lb} <= kinds
 
 
 def test_batched_device_count_and_graph_replay(torch_cuda):

@@ -63,12 +63,14 @@ def test_closest_bit_exact_random(torch_cuda, n_tris, n_rays, salt):
     _assert_same(h, s, hr, sr, f"closest {n_tris}x{n_rays}")
 
 
+@pytest.mark.parametrize("kernel", [2, 2 | 0x40, 2 | 0x50])
 @pytest.mark.parametrize("n_rays", [1, 63, 1000, 65536, 1048576])
-def test_persistent_kernel_bit_exact(torch_cuda, n_rays):
-    """The persistent wave-refill kernel returns the engine's hits, including
-    batches smaller than one wave, and its self-resetting ray counter works
+def test_persistent_kernel_bit_exact(torch_cuda, n_rays, kernel):
+    """The persistent wave-refill kernels (plain, leaf-batched, leaf-batched
+    with the register-topped stack) return the engine's hits, including
+    batches smaller than one wave, and the self-resetting ray counter works
     across consecutive launches (each call is made twice)."""
-    print(f"case persistent n_rays={n_rays}", flush=True)
+    print(f"case persistent n_rays={n_rays} kernel={kernel}", flush=True)
     verts, rays = rt_cases.soup(65536, n_rays, 61)
     if n_rays >= 1000:
         rays[::97]["direction"] = np.nan  # NaN rays are misses
@@ -76,8 +78,8 @@ def test_persistent_kernel_bit_exact(torch_cuda, n_rays):
     nodes, tris, orig = blas.export()
     hr, sr = oracle_rt.trace_closest(nodes, tris, orig, rays)
     for _ in range(2):
-        h, s = _gpu_closest(torch_cuda, blas, rays, kernel=atlas_rt._abi.KERNEL_BVH2_PERSISTENT)
-        _assert_same(h, s, hr, sr, f"persistent {n_rays}")
+        h, s = _gpu_closest(torch_cuda, blas, rays, kernel=kernel)
+        _assert_same(h, s, hr, sr, f"persistent {n_rays} kernel {kernel}")
 
 
 @pytest.mark.parametrize("n_tris,salt", [(4096, 11), (65536, 12)])
@@ -239,7 +241,7 @@ def _fp32_t(rays, tris, i, slot):
     return sol[0] if hit else np.inf
 
 
-@pytest.mark.parametrize("kernel", [3, 4, 3 | 0x10, 4 | 0x10, 2 | 0x10, 2 | 0x20, 2 | 0x30, 1 | 0x30])
+@pytest.mark.parametrize("kernel", [3, 4, 3 | 0x10, 4 | 0x10, 2 | 0x10, 2 | 0x20, 2 | 0x30, 1 | 0x30, 2 | 0x70])
 def test_variant_kernels_edge_rays_and_deep_chain(torch_cuda, kernel):
     """Vertex- and edge-aimed rays (non-watertight culling cases) and an
     80-deep tree whose stack spills past the register top and LDS rows.
