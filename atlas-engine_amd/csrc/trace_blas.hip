@@ -219,8 +219,17 @@ static void launch_persistent(const BlasDev& B, const TraceLaunch& L, uint32_t g
 // a leaf.
 constexpr int kLeafDiv = 4;
 
-atlas_rt_status launch_blas_closest(atlas_rt_blas& blas, const TraceLaunch& L) {
-    if (L.count == 0) return ATLAS_RT_OK;
+// ATLAS_RT_KERNEL_AUTO: resident leaf-batched waves (the fastest
+// engine-exact variant on C0) for large batches, one ray per lane for small
+// ones, where a resident grid would mostly idle.
+constexpr uint32_t kAutoPersistentRays = 65536;
+
+atlas_rt_status launch_blas_closest(atlas_rt_blas& blas, const TraceLaunch& L0) {
+    if (L0.count == 0) return ATLAS_RT_OK;
+    TraceLaunch L = L0;
+    if (L.kernel == ATLAS_RT_KERNEL_AUTO && !L.visits)
+        L.kernel = L.count >= kAutoPersistentRays ? (ATLAS_RT_KERNEL_BVH2_PERSISTENT | ATLAS_RT_KERNEL_LEAF_BATCH)
+                                                  : ATLAS_RT_KERNEL_BVH2_EXACT;
     const BlasDev B = blas_dev(blas);
     uint32_t* ctl = reinterpret_cast<uint32_t*>(blas.flags.ptr);
     const int base = L.kernel & ATLAS_RT_KERNEL_VARIANT_MASK;

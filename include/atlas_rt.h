@@ -181,8 +181,10 @@ atlas_rt_status atlas_rt_blas_export(const atlas_rt_blas* blas, atlas_rt_gpu_bvh
 /* Bulk tracing against one BLAS                                             */
 /* ------------------------------------------------------------------------- */
 
-/* Traversal kernels.  ATLAS_RT_KERNEL_AUTO picks the default (single BLAS)
- * or the autotuned variant (batch sets).
+/* Traversal kernels.  ATLAS_RT_KERNEL_AUTO picks the default (single BLAS:
+ * leaf-batched resident waves, BVH2_PERSISTENT | LEAF_BATCH, from 65,536 rays
+ * up; one ray per lane, BVH2_EXACT, below) or the autotuned variant (batch
+ * sets).
  *  BVH2_EXACT       one ray per lane over the engine's binary nodes, engine order
  *  BVH2_PERSISTENT  resident waves refilled with new rays, same traversal
  *  BVH4 / BVH8      resident waves over 4- / 8-wide nodes collapsed from the
