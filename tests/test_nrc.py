@@ -2,7 +2,7 @@
 oracle (CPU), and the GPU kernels (fused encoding + BF16 MFMA MLP forward,
 backward, Adam) against it.
 
-The engine has no radiance cache; the oracle is the only judge here."""
+The engine has no radiance cache, so the FP64 oracle is the only reference here."""
 import math
 
 import numpy as np
