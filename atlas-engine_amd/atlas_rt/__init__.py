@@ -496,12 +496,18 @@ def frame_matrices(view, proj, last_view_proj, jitter=(0.0, 0.0)):
 
 
 class PathTracer:
-    """Progressive wavefront path tracer over a World (PathTracingRenderer)."""
+    """Progressive wavefront path tracer over a World (PathTracingRenderer).
+
+    frames_in_flight: progressive frames overlapped on the tracer's own
+    streams (1-4, None for the library's default); the image does not depend
+    on it."""
 
     def __init__(self, world, width, height, bounces, materials, lights, env=(0.0, 0.0, 0.0), exposure=1.0,
-                 binning=False):
-        p = _abi.PtParams(width, height, bounces, exposure, (C.c_float * 3)(*env),
-                          _abi.PT_RAY_BINNING if binning else 0)
+                 binning=False, frames_in_flight=None):
+        flags = _abi.PT_RAY_BINNING if binning else 0
+        if frames_in_flight is not None:
+            flags |= int(frames_in_flight) << _abi.PT_FRAMES_IN_FLIGHT_SHIFT
+        p = _abi.PtParams(width, height, bounces, exposure, (C.c_float * 3)(*env), flags)
         self._mats = np.ascontiguousarray(materials, dtype=_abi.MATERIAL_DTYPE)
         self._lights = np.ascontiguousarray(lights, dtype=_abi.LIGHT_DTYPE)
         self._world = world
@@ -597,9 +603,11 @@ class PathTracer:
         check(lib().atlas_rt_pathtracer_lights(self._h, _ptr(out) if n.value else None, n.value, C.byref(n)))
         return out
 
-    def set_cache(self, cache, train_stride=32, train_steps=4):
-        """Attach a RadianceCache (None: plain path tracing)."""
-        p = _abi.PtCacheParams(train_stride, train_steps)
+    def set_cache(self, cache, train_stride=32, train_steps=4, query_lag=0):
+        """Attach a RadianceCache (None: plain path tracing).  query_lag 1:
+        frame i of a render call queries the network as frame i - 2's training
+        left it, so training overlaps the next frame (0: frame i - 1's)."""
+        p = _abi.PtCacheParams(train_stride, train_steps, query_lag, 0)
         check(lib().atlas_rt_pathtracer_set_cache(self._h, cache._h if cache is not None else None, C.byref(p)))
         self._cache = cache
 

@@ -32,6 +32,7 @@ KERNEL_SLAB_FLOAT = 0x20
 KERNEL_LEAF_BATCH = 0x40
 
 PT_RAY_BINNING = 0x1
+PT_FRAMES_IN_FLIGHT_SHIFT = 8  # flags bits 8-11: progressive frames in flight (0 = default)
 RAY_BINS = 64
 
 ANY_ENGINE_HITANY = 0
@@ -134,7 +135,8 @@ class NrcParams(C.Structure):
 
 class PtCacheParams(C.Structure):
     """atlas_rt_pt_cache_params."""
-    _fields_ = [("train_stride", C.c_uint32), ("train_steps", C.c_uint32), ("reserved", C.c_uint32 * 2)]
+    _fields_ = [("train_stride", C.c_uint32), ("train_steps", C.c_uint32), ("query_lag", C.c_uint32),
+                ("reserved", C.c_uint32)]
 
 
 assert NODE_DTYPE.itemsize == 64 and GPU_TRI_DTYPE.itemsize == 48

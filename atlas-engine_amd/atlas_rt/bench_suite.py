@@ -89,15 +89,19 @@ def c2(torch, log, frames=16):
     mats = W.c2_materials(mats_d)
     res = {}
     imgs = {}
-    for binning in (False, True):
-        pt = PathTracer(w, cc["width"], cc["height"], 4, mats, W.sun_light(), env=W.C2_ENV, binning=binning)
+    # (ray binning, frames in flight): the library's default, binned, and one frame at a time
+    for binning, fif in ((False, None), (True, None), (False, 1)):
+        pt = PathTracer(w, cc["width"], cc["height"], 4, mats, W.sun_light(), env=W.C2_ENV, binning=binning,
+                        frames_in_flight=fif)
         pt.render(cam, 2)
         pt.reset()
         t = _sync_time(torch, lambda: pt.render(cam, frames))
-        res[binning] = t / frames
-        imgs[binning] = pt.read().cpu().numpy()
+        res[(binning, fif)] = t / frames
+        imgs[(binning, fif)] = pt.read().cpu().numpy()
         del pt
-    same = bool(imgs[False].tobytes() == imgs[True].tobytes())
+    same = bool(imgs[(False, None)].tobytes() == imgs[(True, None)].tobytes())
+    same_serial = bool(imgs[(False, None)].tobytes() == imgs[(False, 1)].tobytes())
+    res[False], res[True] = res[(False, None)], res[(True, None)]
     px = cc["width"] * cc["height"]
     # realtime mode at the engine's defaults (4 samples per pixel, temporal resolve)
     from . import frame_matrices, projection_matrix, view_matrix
@@ -111,7 +115,8 @@ def c2(torch, log, frames=16):
     t_rt = _sync_time(torch, lambda: [pt.render_realtime(cam, fm) for _ in range(frames)]) / frames
     del pt
     log(f"c2 realtime: {t_rt * 1e3:.3f} ms/frame at 4 samples per pixel with the temporal resolve")
-    log(f"c2: {res[False] * 1e3:.3f} ms/frame plain, {res[True] * 1e3:.3f} ms/frame binned (same image: {same})")
+    log(f"c2: {res[False] * 1e3:.3f} ms/frame plain, {res[True] * 1e3:.3f} ms/frame binned (same image: {same}), "
+        f"{res[(False, 1)] * 1e3:.3f} ms/frame one frame at a time (same image: {same_serial})")
     return {
         "workload": "c2: 256 instances of 8 random-blob meshes, 1280x720, 4 bounces, sun + constant env, "
                     "one sample per pixel per frame",
@@ -119,6 +124,9 @@ def c2(torch, log, frames=16):
         "instanced_triangles": int(sum(len(meshes[k][0]) for k in which)), "instances": int(len(which)),
         "ms_per_frame": round(res[False] * 1e3, 3),
         "mpaths_per_s": round(px / res[False] / 1e6, 1),
+        "frames_in_flight": 3,
+        "one_frame_at_a_time_ms_per_frame": round(res[(False, 1)] * 1e3, 3),
+        "in_flight_image_identical": same_serial,
         "binned_ms_per_frame": round(res[True] * 1e3, 3),
         "binning_speedup": round(res[False] / res[True], 3),
         "binned_image_identical": same,
@@ -273,8 +281,8 @@ def _time_to(curve, target):
     return None
 
 
-# cache settings compared: (training-pixel stride, Adam steps per frame, warm-up frames)
-CACHE_SETTINGS = ((32, 2, 32), (64, 1, 8), (128, 1, 8))
+# cache settings compared: (training-pixel stride, Adam steps per frame, warm-up frames, query lag)
+CACHE_SETTINGS = ((32, 2, 32, 1), (64, 1, 8, 1), (128, 1, 8, 1), (128, 1, 8, 0))
 
 
 def _study(torch, log, name, meshes, mats, xf, which, cc, bounds, ref_spp, bounces=8,
@@ -294,11 +302,11 @@ def _study(torch, log, name, meshes, mats, xf, which, cc, bounds, ref_spp, bounc
     del ref_pt
     plain = _curve(torch, PathTracer(w, width, height, bounces, mats, sun, env=W.C2_ENV), cam, ref, checkpoints)
     runs = []
-    for train_stride, train_steps, warmup in CACHE_SETTINGS:
+    for train_stride, train_steps, warmup, lag in CACHE_SETTINGS:
         rc = RadianceCache(bounds[0], bounds[1], levels=12, log2_table=19, base_resolution=16, max_resolution=2048,
                            learning_rate=1e-2)
         pt = PathTracer(w, width, height, bounces, mats, sun, env=W.C2_ENV)
-        pt.set_cache(rc, train_stride=train_stride, train_steps=train_steps)
+        pt.set_cache(rc, train_stride=train_stride, train_steps=train_steps, query_lag=lag)
         t_warm = _sync_time(torch, lambda: pt.render(cam, warmup))
         samples, queries, loss = pt.cache_stats()
         pt.reset()
@@ -309,9 +317,10 @@ def _study(torch, log, name, meshes, mats, xf, which, cc, bounds, ref_spp, bounc
                 continue
             t_c = _time_to(cache, e_p)
             speedups[str(spp)] = None if t_c is None else round(t_p / t_c, 3)
-        log(f"{name}: stride {train_stride} steps {train_steps} warm-up {warmup}: speedups {speedups}")
+        log(f"{name}: stride {train_stride} steps {train_steps} warm-up {warmup} lag {lag}: speedups {speedups}")
         runs.append({
             "train_stride": train_stride, "train_steps_per_frame": train_steps, "warmup_frames": warmup,
+            "query_lag": lag,
             "warmup_ms": round(t_warm * 1e3, 1), "samples_per_frame": samples, "queries_per_frame": queries,
             "last_loss": round(loss, 4),
             "cache_curve": [[s, round(t * 1e3, 2), round(e, 6)] for s, t, e in cache],

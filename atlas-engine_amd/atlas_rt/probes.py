@@ -5,11 +5,11 @@ one --pmc counter per run).  Run from the package directory's parent:
     PYTHONPATH=atlas-engine_amd rocprofv3 --pmc FETCH_SIZE -d out -o fetch -- python3 -m atlas_rt.probes pmc
     python3 -m atlas_rt.probes summarize-pmc out
 
-  pt plain|cache [stride steps]
+  pt plain|cache [stride steps [lag]]
                   the dense radiance-cache scene (256 blob instances in a
                   30-unit cube) at 1280 x 720, 8 bounces: 4 untimed frames,
                   then 8 frames, plain or with the cache (1/32 training
-                  pixels, 2 Adam steps per frame, unless given)
+                  pixels, 2 Adam steps per frame, query lag 0, unless given)
   nrc             cache training steps at 32 .. 262,144 samples and
                   inference of 65,536 and 1,048,576 queries
   pmc [kernel]    a 256 MiB device-to-device copy (the FETCH_SIZE /
@@ -19,6 +19,8 @@ one --pmc counter per run).  Run from the package directory's parent:
   c2 [binned]     the config-2 path trace (256 instances, 1280 x 720,
                   4 bounces): 2 untimed frames, then 16, optionally with
                   ray binning
+  (pt and c2 take fif=N anywhere after the command: N progressive frames
+  in flight instead of the library's default)
   realtime        the same scene in realtime mode (4 samples per pixel,
                   G-buffer, temporal resolve): 2 untimed frames, then 16
 """
@@ -35,7 +37,15 @@ import numpy as np
 COPY_BYTES = 256 << 20
 
 
-def pt(mode, stride=32, steps=2):
+def _fif():
+    """fif=N from the command line, else None (the library's default)."""
+    for a in sys.argv[2:]:
+        if a.startswith("fif="):
+            return int(a[4:])
+    return None
+
+
+def pt(mode, stride=32, steps=2, lag=0):
     import torch
 
     from . import PathTracer, RadianceCache
@@ -47,17 +57,18 @@ def pt(mode, stride=32, steps=2):
     from . import camera
     cam = camera(cc["eye"], cc["yaw_deg"], cc["pitch_deg"], cc["fov_y_deg"], cc["width"] / cc["height"])
     p = PathTracer(w, cc["width"], cc["height"], 8, W.c2_materials(mats_d), W.sun_light(), env=W.C2_ENV,
-                   binning=(mode == "binned"))
+                   binning=(mode == "binned"), frames_in_flight=_fif())
     if mode == "cache":
         rc = RadianceCache((-20, -20, -20), (20, 20, 20), levels=12, log2_table=19, base_resolution=16,
                            max_resolution=2048)
-        p.set_cache(rc, train_stride=stride, train_steps=steps)
+        p.set_cache(rc, train_stride=stride, train_steps=steps, query_lag=lag)
     p.render(cam, 4)
     torch.cuda.synchronize()
     t = time.perf_counter()
     p.render(cam, 8)
     torch.cuda.synchronize()
-    print(f"pt {mode}: {(time.perf_counter() - t) / 8 * 1e3:.3f} ms/frame", flush=True)
+    lag_s = f" lag={lag}" if mode == "cache" else ""
+    print(f"pt {mode} fif={_fif()}{lag_s}: {(time.perf_counter() - t) / 8 * 1e3:.3f} ms/frame", flush=True)
 
 
 def c2(binning):
@@ -73,13 +84,13 @@ def c2(binning):
     cc = W.c2_camera()
     cam = camera(cc["eye"], cc["yaw_deg"], cc["pitch_deg"], cc["fov_y_deg"], cc["width"] / cc["height"])
     p = PathTracer(w, cc["width"], cc["height"], 4, W.c2_materials(mats_d), W.sun_light(), env=W.C2_ENV,
-                   binning=binning)
+                   binning=binning, frames_in_flight=_fif())
     p.render(cam, 2)
     torch.cuda.synchronize()
     t = time.perf_counter()
     p.render(cam, 16)
     torch.cuda.synchronize()
-    print(f"c2 binning={binning}: {(time.perf_counter() - t) / 16 * 1e3:.3f} ms/frame", flush=True)
+    print(f"c2 binning={binning} fif={_fif()}: {(time.perf_counter() - t) / 16 * 1e3:.3f} ms/frame", flush=True)
 
 
 def realtime():
@@ -215,9 +226,10 @@ def lbvh(reps=3):
 if __name__ == "__main__":
     cmd = sys.argv[1] if len(sys.argv) > 1 else ""
     if cmd == "pt":
-        pt(sys.argv[2] if len(sys.argv) > 2 else "plain", *[int(a) for a in sys.argv[3:5]])
+        args = [a for a in sys.argv[2:] if not a.startswith("fif=")]
+        pt(args[0] if args else "plain", *[int(a) for a in args[1:4]])
     elif cmd == "c2":
-        c2(len(sys.argv) > 2 and sys.argv[2] == "binned")
+        c2("binned" in sys.argv[2:])
     elif cmd == "realtime":
         realtime()
     elif cmd == "nrc":

@@ -21,30 +21,65 @@ atlas_rt_status dev_alloc(T*& p, size_t count) {
     return ATLAS_RT_OK;
 }
 
-// Buffers sized by the paths per frame: both path queues, hits, shadow rays,
-// radiance per ray ID, the traversal stack spill and the binning buffers.
-atlas_rt_status alloc_ray_buffers(PathTracer& P, uint32_t rays) {
-    void* old[] = {P.paths[0], P.paths[1], P.hits, P.shadow, P.Lpix, P.binned};
+// Slot k's buffers sized by the paths per frame: both path queues, hits,
+// shadow rays, radiance per ray ID, the traversal stack spill and the
+// binning buffers.
+atlas_rt_status alloc_slot_rays(PathTracer& P, uint32_t k, uint32_t rays) {
+    FrameSlot& S = P.slot[k];
+    void* old[] = {S.paths[0], S.paths[1], S.hits, S.shadow, S.Lpix, S.binned};
     for (void* p : old)
         if (p) (void)hipFree(p);
-    P.paths[0] = P.paths[1] = nullptr;
-    P.hits = nullptr;
-    P.shadow = P.Lpix = P.binned = nullptr;
-    P.rays = rays;
-    atlas_rt_status st = dev_alloc(P.paths[0], 3 * size_t(rays));
-    if (st == ATLAS_RT_OK) st = dev_alloc(P.paths[1], 3 * size_t(rays));
-    if (st == ATLAS_RT_OK) st = dev_alloc(P.hits, rays);
-    if (st == ATLAS_RT_OK) st = dev_alloc(P.shadow, 3 * size_t(rays));
-    if (st == ATLAS_RT_OK) st = dev_alloc(P.Lpix, rays);
+    S.paths[0] = S.paths[1] = nullptr;
+    S.hits = nullptr;
+    S.shadow = S.Lpix = S.binned = nullptr;
+    atlas_rt_status st = dev_alloc(S.paths[0], 3 * size_t(rays));
+    if (st == ATLAS_RT_OK) st = dev_alloc(S.paths[1], 3 * size_t(rays));
+    if (st == ATLAS_RT_OK) st = dev_alloc(S.hits, rays);
+    if (st == ATLAS_RT_OK) st = dev_alloc(S.shadow, 3 * size_t(rays));
+    if (st == ATLAS_RT_OK) st = dev_alloc(S.Lpix, rays);
     const size_t grid_threads = (size_t(rays) + 127) / 128 * 128;
-    if (st == ATLAS_RT_OK && P.cap > 32) st = P.spill.ensure(size_t(P.cap - 32) * grid_threads * sizeof(int32_t));
-    if (st == ATLAS_RT_OK && P.binning) st = dev_alloc(P.binned, 3 * size_t(rays));
-    if (st == ATLAS_RT_OK && P.binning) st = P.bin_ws.ensure(bin_sort_workspace(rays));
-    ShadeArgs& A = P.shade_args;
-    A.hits = P.hits;
-    A.shadow = P.shadow;
-    A.Lpix = P.Lpix;
+    if (st == ATLAS_RT_OK && P.cap > 32) st = S.spill.ensure(size_t(P.cap - 32) * grid_threads * sizeof(int32_t));
+    if (st == ATLAS_RT_OK && P.binning) st = dev_alloc(S.binned, 3 * size_t(rays));
+    if (st == ATLAS_RT_OK && P.binning) st = S.bin_ws.ensure(bin_sort_workspace(rays));
     return st;
+}
+
+// A slot's frame block, flag words, stream and events.
+atlas_rt_status alloc_slot(PathTracer& P, uint32_t k, uint32_t rays) {
+    FrameSlot& S = P.slot[k];
+    atlas_rt_status st = dev_alloc(S.ctl, 1);
+    if (st == ATLAS_RT_OK) st = S.flags.ensure(256);
+    if (st == ATLAS_RT_OK) st = alloc_slot_rays(P, k, rays);
+    if (st != ATLAS_RT_OK) return st;
+    ATLAS_HIP_TRY(hipMemset(S.ctl, 0, sizeof(FrameCtl)));
+    ATLAS_HIP_TRY(hipMemset(S.flags.ptr, 0, 256));
+    ATLAS_HIP_TRY(hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking));
+    ATLAS_HIP_TRY(hipEventCreateWithFlags(&S.ev_begin, hipEventDisableTiming));
+    ATLAS_HIP_TRY(hipEventCreateWithFlags(&S.ev_train, hipEventDisableTiming));
+    ATLAS_HIP_TRY(hipEventCreateWithFlags(&S.ev_acc, hipEventDisableTiming));
+    return ATLAS_RT_OK;
+}
+
+void free_slot_cache(FrameSlot& S) {
+    void* ptrs[] = {S.rec, S.q_buf, S.q_meta, S.q_out, S.s_q, S.s_t};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    S.rec = S.q_buf = S.q_meta = S.q_out = S.s_q = S.s_t = nullptr;
+}
+
+void free_slot(FrameSlot& S) {
+    void* ptrs[] = {S.ctl, S.paths[0], S.paths[1], S.hits, S.shadow, S.Lpix, S.binned};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    S.spill.release();
+    S.flags.release();
+    S.bin_ws.release();
+    free_slot_cache(S);
+    if (S.stream) (void)hipStreamDestroy(S.stream);
+    hipEvent_t evs[] = {S.ev_begin, S.ev_train, S.ev_acc};
+    for (hipEvent_t e : evs)
+        if (e) (void)hipEventDestroy(e);
+    S = FrameSlot{};
 }
 
 void free_realtime(PathTracer& P) {
@@ -64,17 +99,15 @@ void free_realtime(PathTracer& P) {
 }
 
 void free_all(PathTracer& P) {
-    void* ptrs[] = {P.ctl, P.paths[0], P.paths[1], P.hits, P.shadow, P.Lpix, P.accum,
-                    const_cast<float4**>(P.shade_table), const_cast<float4**>(P.uv_table),
-                    const_cast<float4**>(P.tangent_table), P.materials, P.lights,
-                    P.tex, P.texels, P.binned};
+    void* ptrs[] = {P.ctl, P.accum, const_cast<float4**>(P.shade_table), const_cast<float4**>(P.uv_table),
+                    const_cast<float4**>(P.tangent_table), P.materials, P.lights, P.tex, P.texels};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
-    P.spill.release();
-    P.flags.release();
-    P.bin_ws.release();
     free_cache_buffers(P);
     free_realtime(P);
+    for (FrameSlot& S : P.slot) free_slot(S);
+    if (P.ev_start) (void)hipEventDestroy(P.ev_start);
+    P.ev_start = nullptr;
 }
 
 // Random::Init seeds PCG32 with uint64(SampleUniformFloat()) = 0
@@ -201,10 +234,13 @@ void atlas_rt::pt::build_light_list(PathTracer& P, const float* emitters, uint32
 }
 
 void atlas_rt::pt::free_cache_buffers(PathTracer& P) {
-    void* ptrs[] = {P.rec, P.q_buf, P.q_meta, P.q_out, P.s_q, P.s_t};
-    for (void* p : ptrs)
-        if (p) (void)hipFree(p);
-    P.rec = P.q_buf = P.q_meta = P.q_out = P.s_q = P.s_t = nullptr;
+    for (FrameSlot& S : P.slot) free_slot_cache(S);
+    for (int j = 0; j < 2; j++) {
+        if (P.snap_table[j]) (void)hipFree(P.snap_table[j]);
+        if (P.snap_wn[j]) (void)hipFree(P.snap_wn[j]);
+        P.snap_table[j] = P.snap_wn[j] = nullptr;
+    }
+    P.query_lag = 0;
     P.nrc = nullptr;
     P.train_cap = 0;
     P.shade_args.cache = CacheArgs{};
@@ -230,11 +266,15 @@ atlas_rt_status atlas_rt_pathtracer_create(const atlas_rt_world* world, const at
     for (uint32_t l = 0; l < light_count; l++)
         if (lights[l].type != ATLAS_RT_LIGHT_DIRECTIONAL)
             return fail(ATLAS_RT_E_UNSUPPORTED, "atlas_rt_pathtracer_create: only directional lights");
-    if (params->flags & ~ATLAS_RT_PT_RAY_BINNING)
+    if (params->flags & ~(ATLAS_RT_PT_RAY_BINNING | ATLAS_RT_PT_FRAMES_IN_FLIGHT_MASK))
         return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_create: unknown flag bits");
+    const uint32_t slots = (params->flags & ATLAS_RT_PT_FRAMES_IN_FLIGHT_MASK) >> ATLAS_RT_PT_FRAMES_IN_FLIGHT_SHIFT;
+    if (slots > kMaxFrameSlots)
+        return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_create: at most 4 frames in flight");
     auto* pt = new atlas_rt_pathtracer();
     PathTracer& P = pt->P;
     P.binning = (params->flags & ATLAS_RT_PT_RAY_BINNING) != 0;
+    P.slots = slots ? slots : kDefaultFrameSlots;
     P.width = params->width;
     P.height = params->height;
     P.bounces = params->bounces;
@@ -247,14 +287,16 @@ atlas_rt_status atlas_rt_pathtracer_create(const atlas_rt_world* world, const at
     P.cap = world->max_depth + 1;
     const size_t pixels = size_t(P.width) * P.height;
     atlas_rt_status st = ATLAS_RT_OK;
+    P.rays = uint32_t(pixels);
     if (st == ATLAS_RT_OK) st = dev_alloc(P.ctl, 1);
-    if (st == ATLAS_RT_OK) st = alloc_ray_buffers(P, uint32_t(pixels));
+    for (uint32_t k = 0; k < P.slots && st == ATLAS_RT_OK; k++) st = alloc_slot(P, k, P.rays);
+    if (st == ATLAS_RT_OK && hipEventCreateWithFlags(&P.ev_start, hipEventDisableTiming) != hipSuccess)
+        st = fail(ATLAS_RT_E_HIP, "atlas_rt_pathtracer_create: hipEventCreateWithFlags failed");
     if (st == ATLAS_RT_OK) st = dev_alloc(P.accum, pixels);
     if (st == ATLAS_RT_OK) st = dev_alloc(P.shade_table, world->blas_handles.size());
     if (st == ATLAS_RT_OK) st = dev_alloc(P.uv_table, world->blas_handles.size());
     if (st == ATLAS_RT_OK) st = dev_alloc(P.tangent_table, world->blas_handles.size());
     if (st == ATLAS_RT_OK) st = dev_alloc(P.materials, material_count);
-    if (st == ATLAS_RT_OK) st = P.flags.ensure(256);
     if (st != ATLAS_RT_OK) {
         free_all(P);
         delete pt;
@@ -272,7 +314,6 @@ atlas_rt_status atlas_rt_pathtracer_create(const atlas_rt_world* world, const at
     hipError_t e = hipSuccess;
     e = hipMemcpy(P.ctl, &ctl, sizeof(ctl), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(P.accum, 0, pixels * sizeof(float4));
-    if (e == hipSuccess) e = hipMemset(P.flags.ptr, 0, 256);
     if (e == hipSuccess)
         e = hipMemcpy(const_cast<float4**>(P.shade_table), shade.data(), shade.size() * sizeof(float4*),
                       hipMemcpyHostToDevice);
@@ -305,16 +346,13 @@ atlas_rt_status atlas_rt_pathtracer_create(const atlas_rt_world* world, const at
     A.alpha = AlphaArgs{P.shade_table, P.uv_table, P.materials, nullptr, nullptr, 0u, 0u};
     A.bounces = P.bounces;
     std::memcpy(A.env, params->env_radiance, 12);
-    A.ctl = P.ctl;
-    A.hits = P.hits;
-    A.shadow = P.shadow;
-    A.Lpix = P.Lpix;
     *out = pt;
     return ATLAS_RT_OK;
 }
 
 atlas_rt_status atlas_rt_pathtracer_destroy(atlas_rt_pathtracer* pt) {
     if (!pt) return ATLAS_RT_OK;
+    (void)hipDeviceSynchronize();  // frames may still run on the slot streams
     free_all(pt->P);
     delete pt;
     return ATLAS_RT_OK;
@@ -329,11 +367,36 @@ atlas_rt_status atlas_rt_pathtracer_render(atlas_rt_pathtracer* pt, const atlas_
     std::memcpy(cam.right, camera->right, 12);
     std::memcpy(cam.bottom, camera->bottom, 12);
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    for (uint32_t f = 0; f < frames; f++) {
-        atlas_rt_status st = launch_frame(pt->P, cam, s);
+    PathTracer& P = pt->P;
+    if (!frames) return ATLAS_RT_OK;
+    // frames rotate over the slots, each on its slot's stream: they start
+    // after the caller's earlier work on `s`, and `s` waits for all of them
+    const uint32_t n = P.realtime ? 1u : P.slots;  // realtime mode has one slot sized for its rays
+    if (P.nrc && P.query_lag) {
+        // frames 0 and 1 of the call query the network as the call finds it
+        atlas_rt_status st = refresh_snapshots(P, s);
         if (st != ATLAS_RT_OK) return st;
-        pt->P.host_samples++;
     }
+    ATLAS_HIP_TRY(hipEventRecord(P.ev_start, s));
+    uint32_t used = 0;
+    FrameOrder order;
+    for (uint32_t f = 0; f < frames; f++) {
+        const uint32_t k = P.next_slot % n;
+        if (!(used & (1u << k))) {
+            ATLAS_HIP_TRY(hipStreamWaitEvent(P.slot[k].stream, P.ev_start, 0));
+            used |= 1u << k;
+        }
+        order.index = f;
+        atlas_rt_status st = launch_frame(P, cam, k, order);
+        if (st != ATLAS_RT_OK) return st;
+        order.prev2 = order.prev;
+        order.prev = int32_t(k);
+        P.last_slot = k;
+        P.next_slot = (k + 1) % n;
+        P.host_samples++;
+    }
+    for (uint32_t k = 0; k < n; k++)
+        if (used & (1u << k)) ATLAS_HIP_TRY(hipStreamWaitEvent(s, P.slot[k].ev_acc, 0));
     return ATLAS_RT_OK;
 }
 
@@ -350,7 +413,9 @@ atlas_rt_status atlas_rt_pathtracer_set_realtime(atlas_rt_pathtracer* pt, const 
     free_realtime(P);
     const size_t pixels = size_t(P.width) * P.height;
     const uint32_t rays = uint32_t(pixels * (params ? size_t(params->samples_per_frame) : 1u));
-    atlas_rt_status st = alloc_ray_buffers(P, rays);
+    P.rays = rays;
+    P.next_slot = P.last_slot = 0;
+    atlas_rt_status st = alloc_slot_rays(P, 0, rays);
     if (st != ATLAS_RT_OK || !params) return st;
     const size_t inst = P.world->instances.size();
     if (st == ATLAS_RT_OK) st = dev_alloc(P.rt_frame, pixels);
@@ -525,6 +590,8 @@ atlas_rt_status atlas_rt_pathtracer_set_cache(atlas_rt_pathtracer* pt, atlas_rt_
     if (P.bounces < 2) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_set_cache: needs bounces >= 2");
     if (params->train_stride < 1 || params->train_steps > 64)
         return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_set_cache: need train_stride >= 1, train_steps <= 64");
+    if (params->query_lag > 1 || params->reserved)
+        return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_set_cache: query_lag is 0 or 1, reserved is 0");
     int dev = -1;
     ATLAS_HIP_TRY(hipGetDevice(&dev));
     if (dev != cache->device) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_set_cache: cache on another device");
@@ -534,12 +601,19 @@ atlas_rt_status atlas_rt_pathtracer_set_cache(atlas_rt_pathtracer* pt, atlas_rt_
     const size_t cap = std::min(pixels, 2 * (pixels / params->train_stride) + 256);
     const size_t recs = cap * size_t(P.bounces + 1);
     atlas_rt_status st = ATLAS_RT_OK;
-    if (st == ATLAS_RT_OK) st = dev_alloc(P.rec, 5 * recs);
-    if (st == ATLAS_RT_OK) st = dev_alloc(P.q_buf, 3 * pixels);
-    if (st == ATLAS_RT_OK) st = dev_alloc(P.q_meta, pixels);
-    if (st == ATLAS_RT_OK) st = dev_alloc(P.q_out, pixels);
-    if (st == ATLAS_RT_OK) st = dev_alloc(P.s_q, 3 * recs);
-    if (st == ATLAS_RT_OK) st = dev_alloc(P.s_t, recs);
+    for (uint32_t k = 0; k < P.slots && st == ATLAS_RT_OK; k++) {
+        FrameSlot& S = P.slot[k];
+        st = dev_alloc(S.rec, 5 * recs);
+        if (st == ATLAS_RT_OK) st = dev_alloc(S.q_buf, 3 * pixels);
+        if (st == ATLAS_RT_OK) st = dev_alloc(S.q_meta, pixels);
+        if (st == ATLAS_RT_OK) st = dev_alloc(S.q_out, pixels);
+        if (st == ATLAS_RT_OK) st = dev_alloc(S.s_q, 3 * recs);
+        if (st == ATLAS_RT_OK) st = dev_alloc(S.s_t, recs);
+    }
+    for (int j = 0; j < 2 && params->query_lag && st == ATLAS_RT_OK; j++) {
+        st = dev_alloc(P.snap_table[j], cache->table_params);
+        if (st == ATLAS_RT_OK) st = dev_alloc(P.snap_wn[j], size_t(nrc::kWnSize));
+    }
     if (st != ATLAS_RT_OK) {
         free_cache_buffers(P);
         return st;
@@ -547,12 +621,9 @@ atlas_rt_status atlas_rt_pathtracer_set_cache(atlas_rt_pathtracer* pt, atlas_rt_
     P.nrc = cache;
     P.train_steps = params->train_steps;
     P.train_cap = uint32_t(cap);
+    P.query_lag = params->query_lag;
+    // the slot's record and query buffers are bound per frame
     CacheArgs& C = P.shade_args.cache;
-    C.rec_q = P.rec;
-    C.rec_beta = P.rec + 3 * recs;
-    C.rec_add = P.rec + 4 * recs;
-    C.q_buf = P.q_buf;
-    C.q_meta = P.q_meta;
     C.train_cap = uint32_t(cap);
     C.train_stride = params->train_stride;
     C.enabled = 1;
@@ -562,9 +633,10 @@ atlas_rt_status atlas_rt_pathtracer_set_cache(atlas_rt_pathtracer* pt, atlas_rt_
 atlas_rt_status atlas_rt_pathtracer_cache_stats(atlas_rt_pathtracer* pt, uint32_t* train_samples, uint32_t* queries,
                                                 float* loss) {
     if (!pt) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_cache_stats: NULL path tracer");
+    // the last frame's statistics
     FrameCtl ctl;
     ATLAS_HIP_TRY(hipDeviceSynchronize());
-    ATLAS_HIP_TRY(hipMemcpy(&ctl, pt->P.ctl, sizeof(ctl), hipMemcpyDeviceToHost));
+    ATLAS_HIP_TRY(hipMemcpy(&ctl, pt->P.slot[pt->P.last_slot].ctl, sizeof(ctl), hipMemcpyDeviceToHost));
     if (train_samples) *train_samples = ctl.train_samples;
     if (queries) *queries = ctl.queries;
     if (loss) *loss = ctl.train_loss;
@@ -577,15 +649,17 @@ atlas_rt_status atlas_rt_pathtracer_cache_samples(atlas_rt_pathtracer* pt, float
     PathTracer& P = pt->P;
     *count = 0;
     if (!P.nrc) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_cache_samples: no cache attached");
+    // the last frame's samples
+    const FrameSlot& S = P.slot[P.last_slot];
     FrameCtl ctl;
     ATLAS_HIP_TRY(hipDeviceSynchronize());
-    ATLAS_HIP_TRY(hipMemcpy(&ctl, P.ctl, sizeof(ctl), hipMemcpyDeviceToHost));
+    ATLAS_HIP_TRY(hipMemcpy(&ctl, S.ctl, sizeof(ctl), hipMemcpyDeviceToHost));
     const uint32_t n = std::min(std::min(ctl.train_samples, P.train_cap * uint32_t(P.bounces + 1)), max_count);
     if (n && (!d_queries || !d_targets))
         return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_cache_samples: NULL buffer");
     if (n) {
-        ATLAS_HIP_TRY(hipMemcpy(d_queries, P.s_q, size_t(n) * 3 * sizeof(float4), hipMemcpyDeviceToDevice));
-        ATLAS_HIP_TRY(hipMemcpy(d_targets, P.s_t, size_t(n) * sizeof(float4), hipMemcpyDeviceToDevice));
+        ATLAS_HIP_TRY(hipMemcpy(d_queries, S.s_q, size_t(n) * 3 * sizeof(float4), hipMemcpyDeviceToDevice));
+        ATLAS_HIP_TRY(hipMemcpy(d_targets, S.s_t, size_t(n) * sizeof(float4), hipMemcpyDeviceToDevice));
     }
     *count = n;
     return ATLAS_RT_OK;

@@ -5,6 +5,7 @@
 
 #include "pt_types.h"
 #include "rt_internal.h"
+#include "nrc.h"
 #include "world.h"
 
 namespace atlas_rt {
@@ -31,6 +32,37 @@ struct ShadeArgs {
     RealtimeArgs rt;
 };
 
+// One frame's working state.  Progressive rendering keeps several frames in
+// flight, each on its own stream with its own set of these buffers: the last
+// bounces of a frame trace few rays and wait on their traversal latency, so
+// the next frame's first bounces fill the GPU meanwhile.  Events keep the
+// results those of frames run one after another (see launch_frame).
+struct FrameSlot {
+    FrameCtl* ctl = nullptr;      // this frame's seeds, counts and cache statistics
+    float4* paths[2] = {nullptr, nullptr};
+    int4* hits = nullptr;
+    float4* shadow = nullptr;
+    float4* Lpix = nullptr;
+    Scratch spill;
+    Scratch flags;
+    float4* binned = nullptr;
+    Scratch bin_ws;
+    // radiance cache
+    float4* rec = nullptr;        // training records: 5 float4 each (query x3, beta, added radiance)
+    float4* q_buf = nullptr;      // render-path queries, 3 float4 each
+    float4* q_meta = nullptr;
+    float4* q_out = nullptr;
+    float4* s_q = nullptr;        // training samples: queries (3 float4) and targets
+    float4* s_t = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_begin = nullptr;   // after k_frame_begin: the frame's seeds are drawn
+    hipEvent_t ev_train = nullptr;   // after the frame's last training step
+    hipEvent_t ev_acc = nullptr;     // after the frame's accumulation
+};
+
+constexpr uint32_t kMaxFrameSlots = 4;
+constexpr uint32_t kDefaultFrameSlots = 3;
+
 struct PathTracer {
     uint32_t width = 0, height = 0;
     int32_t bounces = 0;
@@ -39,11 +71,12 @@ struct PathTracer {
     WorldDev world_dev{};
     uint32_t cap = 0;
     // device buffers
-    FrameCtl* ctl = nullptr;
-    float4* paths[2] = {nullptr, nullptr};
-    int4* hits = nullptr;
-    float4* shadow = nullptr;
-    float4* Lpix = nullptr;
+    FrameCtl* ctl = nullptr;      // Random's PCG state, samples and frames so far
+    FrameSlot slot[kMaxFrameSlots];
+    uint32_t slots = 1;           // frames in flight during progressive rendering
+    uint32_t next_slot = 0;       // the slot of the next progressive frame
+    uint32_t last_slot = 0;       // the slot of the last frame rendered
+    hipEvent_t ev_start = nullptr;
     float4* accum = nullptr;
     const float4* xform = nullptr;      // borrowed: the world's transform table
     const float4** shade_table = nullptr;
@@ -59,14 +92,10 @@ struct PathTracer {
     TexDesc* tex = nullptr;
     uint8_t* texels = nullptr;
     uint32_t tex_count = 0;
-    Scratch spill;
-    Scratch flags;
     // ray binning (ATLAS_RT_PT_RAY_BINNING): each bounce's paths sorted into
-    // `binned` by direction bin before the trace
+    // the slot's `binned` by direction bin before the trace
     bool binning = false;
-    float4* binned = nullptr;
-    Scratch bin_ws;
-    ShadeArgs shade_args{};
+    ShadeArgs shade_args{};       // every field but the slot's buffers
     int32_t host_samples = 0;
     uint32_t rays = 0;            // paths per frame: pixels, times the samples per frame in realtime mode
     // realtime mode (atlas_rt_pathtracer_set_realtime)
@@ -85,15 +114,23 @@ struct PathTracer {
     atlas_rt_nrc* nrc = nullptr;
     uint32_t train_steps = 0;
     uint32_t train_cap = 0;       // training paths per frame
-    float4* rec = nullptr;        // training records: 5 float4 each (query x3, beta, added radiance)
-    float4* q_buf = nullptr;      // render-path queries, 3 float4 each
-    float4* q_meta = nullptr;
-    float4* q_out = nullptr;
-    float4* s_q = nullptr;        // training samples: queries (3 float4) and targets
-    float4* s_t = nullptr;
+    uint32_t query_lag = 0;       // 1: frames query a snapshot of the network two frames old
+    uint16_t* snap_table[2] = {nullptr, nullptr};  // FP16 table snapshots (query_lag 1)
+    uint16_t* snap_wn[2] = {nullptr, nullptr};     // BF16 forward fragment snapshots
 };
 
-atlas_rt_status launch_frame(PathTracer& P, const Camera& cam, hipStream_t s);
+// Where a progressive frame sits in its render call: the slots of the two
+// frames before it (-1: none in this call) and its index in the call.
+struct FrameOrder {
+    int32_t prev = -1;
+    int32_t prev2 = -1;
+    uint32_t index = 0;
+};
+
+// One progressive frame in slot `k` on that slot's stream.
+atlas_rt_status launch_frame(PathTracer& P, const Camera& cam, uint32_t k, const FrameOrder& order);
+// query_lag 1: both network snapshots from the live network, on `s`.
+atlas_rt_status refresh_snapshots(PathTracer& P, hipStream_t s);
 // The temporal resolve's inputs and output (pt_temporal.hip,
 // pathtracer/temporal.csh).
 struct TemporalArgs {
@@ -115,8 +152,11 @@ atlas_rt_status launch_temporal(const TemporalArgs& T, hipStream_t s);
 // One realtime frame.
 atlas_rt_status launch_frame_realtime(PathTracer& P, const Camera& cam, const atlas_rt_frame_matrices& M,
                                       hipStream_t s);
-// The bounce loop shared by both modes: bounces + 1 x (trace, shade, shadow).
-atlas_rt_status launch_bounces(PathTracer& P, hipStream_t s);
+// The bounce loop shared by both modes: bounces + 1 x (trace, shade, shadow)
+// in slot `k`.  With the cache, the inference waits for `train_ready` when it
+// is not null and evaluates `net` (null: the live network).
+atlas_rt_status launch_bounces(PathTracer& P, uint32_t k, hipStream_t s, hipEvent_t train_ready,
+                               const nrc::NrcDev* net);
 // Builds the light list (UpdateLights over the punctual lights and, with
 // emitters, the triangle lights) into P.light_list.
 void build_light_list(PathTracer& P, const float* emitters, uint32_t emitter_count);

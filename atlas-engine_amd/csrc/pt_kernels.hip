@@ -48,10 +48,22 @@ __device__ __forceinline__ uint32_t pcg32_next(uint64_t& state) {
     return (v >> count) | (v << ((32u - count) & 31u));
 }
 
-__global__ void k_frame_begin(FrameCtl* ctl, int32_t bounces, uint32_t pixels) {
+// A frame's seeds and counters: Random's PCG stream is global, so the
+// seeds come from the master block in frame order; the slot's block holds
+// this frame's copy of the sample and frame counts.  Progressive frames
+// (advance = 1) count themselves; realtime frames do not.
+__global__ void k_frame_begin(FrameCtl* master, FrameCtl* ctl, int32_t bounces, uint32_t pixels, uint32_t advance) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    uint64_t state = master->pcg_state;
     for (int32_t b = 0; b <= bounces && b < 16; b++)
-        ctl->seeds[b] = float(pcg32_next(ctl->pcg_state)) / float(0xFFFFFFFFu) * 0.999999f;
+        ctl->seeds[b] = float(pcg32_next(state)) / float(0xFFFFFFFFu) * 0.999999f;
+    master->pcg_state = state;
+    ctl->sample_count = master->sample_count;
+    ctl->frame = master->frame;
+    if (advance) {
+        master->sample_count += 1;
+        master->frame += 1;
+    }
     ctl->paths[0] = pixels;
     for (int i = 1; i < 17; i++) ctl->paths[i] = 0;
     for (int i = 0; i < 16; i++) ctl->shadows[i] = 0;
@@ -713,8 +725,8 @@ __global__ __launch_bounds__(kBlock) void k_build_targets(FrameCtl* __restrict__
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_accumulate(FrameCtl* ctl, const float4* __restrict__ Lpix,
-                                                       float4* __restrict__ accum, uint32_t pixels) {
+__global__ __launch_bounds__(kBlock) void k_accumulate(const float4* __restrict__ Lpix, float4* __restrict__ accum,
+                                                       uint32_t pixels) {
     const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
     if (p < pixels) {
         const float4 l = Lpix[p];
@@ -725,53 +737,72 @@ __global__ __launch_bounds__(kBlock) void k_accumulate(FrameCtl* ctl, const floa
         a.w += 1.0f;
         accum[p] = a;
     }
-    if (p == 0) {
-        ctl->sample_count += 1;
-        ctl->frame += 1;
+}
+
+// The shade kernel's arguments with slot k's buffers.
+ShadeArgs slot_args(const PathTracer& P, uint32_t k) {
+    const FrameSlot& S = P.slot[k];
+    ShadeArgs A = P.shade_args;
+    A.ctl = S.ctl;
+    A.hits = S.hits;
+    A.shadow = S.shadow;
+    A.Lpix = S.Lpix;
+    if (A.cache.enabled) {
+        const size_t recs = size_t(P.train_cap) * size_t(P.bounces + 1);
+        A.cache.rec_q = S.rec;
+        A.cache.rec_beta = S.rec + 3 * recs;
+        A.cache.rec_add = S.rec + 4 * recs;
+        A.cache.q_buf = S.q_buf;
+        A.cache.q_meta = S.q_meta;
     }
+    return A;
 }
 
 }  // namespace
 
-atlas_rt_status launch_bounces(PathTracer& P, hipStream_t s) {
+atlas_rt_status launch_bounces(PathTracer& P, uint32_t k, hipStream_t s, hipEvent_t train_ready,
+                               const nrc::NrcDev* net) {
+    FrameSlot& S = P.slot[k];
     const uint32_t rays = P.rays;
     const uint32_t grid_all = (rays + kBlock - 1) / kBlock;
-    uint32_t* flags = reinterpret_cast<uint32_t*>(P.flags.ptr);
-    const CacheArgs& C = P.shade_args.cache;
+    uint32_t* flags = reinterpret_cast<uint32_t*>(S.flags.ptr);
+    const ShadeArgs base = slot_args(P, k);
+    const CacheArgs& C = base.cache;
     for (int32_t b = 0; b <= P.bounces; b++) {
-        float4* in = P.paths[b & 1];
-        float4* out = P.paths[(b + 1) & 1];
+        float4* in = S.paths[b & 1];
+        float4* out = S.paths[(b + 1) & 1];
         if (P.binning) {
             // coherence sort of this bounce's live paths (the engine sorts
             // before each trace dispatch, RayTracingHelper.cpp:304-344)
-            ATLAS_HIP_TRY(launch_bin_sort(in, P.binned, 3, &P.ctl->paths[b], 0, rays, P.bin_ws.ptr, nullptr,
+            ATLAS_HIP_TRY(launch_bin_sort(in, S.binned, 3, &S.ctl->paths[b], 0, rays, S.bin_ws.ptr, nullptr,
                                           nullptr, s));
-            in = P.binned;
+            in = S.binned;
         }
         // with the cache only training paths outlive bounce 1
         const uint32_t grid = (C.enabled && b >= 2) ? (P.train_cap + kBlock - 1) / kBlock : grid_all;
-        int32_t* spill = reinterpret_cast<int32_t*>(P.spill.ptr);
-        const AlphaArgs& AA = P.shade_args.alpha;
+        int32_t* spill = reinterpret_cast<int32_t*>(S.spill.ptr);
+        const AlphaArgs& AA = base.alpha;
         if (P.alpha)
             hipLaunchKernelGGL((k_trace<kBlock, kLdsDepth, true>), dim3(grid), dim3(kBlock), 0, s, P.world_dev,
-                               &P.ctl->paths[b], in, P.hits, spill, P.cap, flags, AA);
+                               &S.ctl->paths[b], in, S.hits, spill, P.cap, flags, AA);
         else
             hipLaunchKernelGGL((k_trace<kBlock, kLdsDepth, false>), dim3(grid), dim3(kBlock), 0, s, P.world_dev,
-                               &P.ctl->paths[b], in, P.hits, spill, P.cap, flags, AA);
-        ShadeArgs A = P.shade_args;
+                               &S.ctl->paths[b], in, S.hits, spill, P.cap, flags, AA);
+        ShadeArgs A = base;
         A.paths_in = in;
         A.paths_out = out;
         const uint32_t shade_grid = (grid * kBlock + kShadeBlock - 1) / kShadeBlock;
         hipLaunchKernelGGL(k_shade, dim3(shade_grid), dim3(kShadeBlock), 0, s, A, b);
         if (P.light_count > 0 && P.alpha)
             hipLaunchKernelGGL((k_shadow<kBlock, kLdsDepth, true>), dim3(grid), dim3(kBlock), 0, s, P.world_dev,
-                               P.ctl, b, P.shadow, P.Lpix, spill, P.cap, flags, C.rec_add, AA);
+                               S.ctl, b, S.shadow, S.Lpix, spill, P.cap, flags, C.rec_add, AA);
         else if (P.light_count > 0)
             hipLaunchKernelGGL((k_shadow<kBlock, kLdsDepth, false>), dim3(grid), dim3(kBlock), 0, s, P.world_dev,
-                               P.ctl, b, P.shadow, P.Lpix, spill, P.cap, flags, C.rec_add, AA);
+                               S.ctl, b, S.shadow, S.Lpix, spill, P.cap, flags, C.rec_add, AA);
         if (C.enabled && b == 1) {
-            ATLAS_HIP_TRY(nrc::launch_infer(P.nrc->dev, P.q_buf, P.q_out, &P.ctl->queries, rays, s));
-            hipLaunchKernelGGL(k_cache_apply, dim3(grid_all), dim3(kBlock), 0, s, P.ctl, P.q_meta, P.q_out, P.Lpix);
+            if (train_ready) ATLAS_HIP_TRY(hipStreamWaitEvent(s, train_ready, 0));
+            ATLAS_HIP_TRY(nrc::launch_infer(net ? *net : P.nrc->dev, S.q_buf, S.q_out, &S.ctl->queries, rays, s));
+            hipLaunchKernelGGL(k_cache_apply, dim3(grid_all), dim3(kBlock), 0, s, S.ctl, S.q_meta, S.q_out, S.Lpix);
         }
     }
     ATLAS_HIP_TRY(hipGetLastError());
@@ -796,13 +827,15 @@ atlas_rt_status launch_frame_realtime(PathTracer& P, const Camera& cam, const at
     std::memcpy(A.rt.last_vp, M.last_view_projection, sizeof(A.rt.last_vp));
     A.rt.spf = spf;
     A.rt.enabled = 1;
-    hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(64), 0, s, P.ctl, P.bounces, P.rays);
+    // realtime frames run one at a time in slot 0 on the caller's stream
+    FrameSlot& S = P.slot[0];
+    hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(64), 0, s, P.ctl, S.ctl, P.bounces, P.rays, 0u);
     hipLaunchKernelGGL(k_raygen_rt, dim3((P.rays + kShadeBlock - 1) / kShadeBlock), dim3(kShadeBlock), 0, s, cam,
-                       P.width, P.height, spf, M.jitter[0], M.jitter[1], P.paths[0], P.Lpix);
-    const atlas_rt_status st = launch_bounces(P, s);
+                       P.width, P.height, spf, M.jitter[0], M.jitter[1], S.paths[0], S.Lpix);
+    const atlas_rt_status st = launch_bounces(P, 0, s, nullptr, nullptr);
     A.rt.enabled = 0;
     if (st != ATLAS_RT_OK) return st;
-    hipLaunchKernelGGL(k_frame_rt, dim3((pixels + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P.Lpix, pixels, spf,
+    hipLaunchKernelGGL(k_frame_rt, dim3((pixels + kBlock - 1) / kBlock), dim3(kBlock), 0, s, S.Lpix, pixels, spf,
                        P.rt.max_radiance, P.rt_frame);
     TemporalArgs T;
     T.width = P.width;
@@ -830,27 +863,77 @@ atlas_rt_status launch_frame_realtime(PathTracer& P, const Camera& cam, const at
     return ATLAS_RT_OK;
 }
 
-atlas_rt_status launch_frame(PathTracer& P, const Camera& cam, hipStream_t s) {
+// Frames in flight give the results of frames run one after another
+// because everything they share is ordered by events:
+// - the master block's PCG state and counters: frame f draws its seeds after
+//   frame f - 1 did (ev_begin);
+// - the accumulation image: frame f adds its radiance after frame f - 1
+//   (ev_acc), so every pixel sums its samples in frame order;
+// - the radiance cache: frame f trains after frame f - 1 trained, and
+//   queries the network after frame f - 1's training (query_lag 0) or
+//   frame f - 2's, through the snapshot that training left (query_lag 1)
+//   (ev_train).
+// Everything else a frame touches is its slot's own.
+atlas_rt_status launch_frame(PathTracer& P, const Camera& cam, uint32_t k, const FrameOrder& order) {
+    FrameSlot& S = P.slot[k];
+    hipStream_t s = S.stream;
     const uint32_t pixels = P.width * P.height;
     const uint32_t grid_all = (pixels + kBlock - 1) / kBlock;
-    const CacheArgs& C = P.shade_args.cache;
-    hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(64), 0, s, P.ctl, P.bounces, pixels);
+    const ShadeArgs A = slot_args(P, k);
+    const CacheArgs& C = A.cache;
+    const FrameSlot* before = order.prev >= 0 ? &P.slot[order.prev] : nullptr;
+    const FrameSlot* before2 = order.prev2 >= 0 ? &P.slot[order.prev2] : nullptr;
+    if (before) ATLAS_HIP_TRY(hipStreamWaitEvent(s, before->ev_begin, 0));
+    hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(64), 0, s, P.ctl, S.ctl, P.bounces, pixels, 1u);
+    ATLAS_HIP_TRY(hipEventRecord(S.ev_begin, s));
     hipLaunchKernelGGL(k_raygen, dim3((pixels + kShadeBlock - 1) / kShadeBlock), dim3(kShadeBlock), 0, s, cam, P.width,
-                       P.height, P.ctl, P.paths[0], P.Lpix, C, P.bounces);
-    const atlas_rt_status bs = launch_bounces(P, s);
+                       P.height, S.ctl, S.paths[0], S.Lpix, C, P.bounces);
+    const bool lag = C.enabled && P.query_lag;
+    nrc::NrcDev snap{};
+    if (lag) {
+        snap = P.nrc->dev;
+        snap.table = P.snap_table[order.index & 1u];
+        snap.wn = P.snap_wn[order.index & 1u];
+    }
+    const FrameSlot* ready = lag ? before2 : before;
+    const atlas_rt_status bs = launch_bounces(P, k, s, ready ? ready->ev_train : nullptr, lag ? &snap : nullptr);
     if (bs != ATLAS_RT_OK) return bs;
     if (C.enabled) {
-        hipLaunchKernelGGL(k_build_targets, dim3((P.train_cap + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P.ctl, C,
-                           P.bounces, P.s_q, P.s_t);
+        hipLaunchKernelGGL(k_build_targets, dim3((P.train_cap + kBlock - 1) / kBlock), dim3(kBlock), 0, s, S.ctl, C,
+                           P.bounces, S.s_q, S.s_t);
+        if (before) ATLAS_HIP_TRY(hipStreamWaitEvent(s, before->ev_train, 0));
         const uint32_t max_samples = P.train_cap * uint32_t(P.bounces + 1);
-        for (uint32_t k = 0; k < P.train_steps; k++) {
-            atlas_rt_status st = nrc::train_step(*P.nrc, P.s_q, P.s_t, &P.ctl->train_samples, max_samples, k,
-                                                 P.train_steps, &P.ctl->train_loss, s);
+        for (uint32_t t = 0; t < P.train_steps; t++) {
+            atlas_rt_status st = nrc::train_step(*P.nrc, S.s_q, S.s_t, &S.ctl->train_samples, max_samples, t,
+                                                 P.train_steps, &S.ctl->train_loss, s);
             if (st != ATLAS_RT_OK) return st;
         }
+        if (lag) {
+            // the network this training left, for the frame two after this one
+            const atlas_rt_nrc& N = *P.nrc;
+            const uint32_t j = order.index & 1u;
+            ATLAS_HIP_TRY(hipMemcpyAsync(P.snap_table[j], N.table16, size_t(N.table_params) * sizeof(uint16_t),
+                                         hipMemcpyDeviceToDevice, s));
+            ATLAS_HIP_TRY(hipMemcpyAsync(P.snap_wn[j], N.wn, size_t(nrc::kWnSize) * sizeof(uint16_t),
+                                         hipMemcpyDeviceToDevice, s));
+        }
     }
-    hipLaunchKernelGGL(k_accumulate, dim3(grid_all), dim3(kBlock), 0, s, P.ctl, P.Lpix, P.accum, pixels);
+    ATLAS_HIP_TRY(hipEventRecord(S.ev_train, s));
+    if (before) ATLAS_HIP_TRY(hipStreamWaitEvent(s, before->ev_acc, 0));
+    hipLaunchKernelGGL(k_accumulate, dim3(grid_all), dim3(kBlock), 0, s, S.Lpix, P.accum, pixels);
+    ATLAS_HIP_TRY(hipEventRecord(S.ev_acc, s));
     ATLAS_HIP_TRY(hipGetLastError());
+    return ATLAS_RT_OK;
+}
+
+atlas_rt_status refresh_snapshots(PathTracer& P, hipStream_t s) {
+    const atlas_rt_nrc& N = *P.nrc;
+    for (int j = 0; j < 2; j++) {
+        ATLAS_HIP_TRY(hipMemcpyAsync(P.snap_table[j], N.table16, size_t(N.table_params) * sizeof(uint16_t),
+                                     hipMemcpyDeviceToDevice, s));
+        ATLAS_HIP_TRY(hipMemcpyAsync(P.snap_wn[j], N.wn, size_t(nrc::kWnSize) * sizeof(uint16_t),
+                                     hipMemcpyDeviceToDevice, s));
+    }
     return ATLAS_RT_OK;
 }
 

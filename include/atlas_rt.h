@@ -578,6 +578,15 @@ typedef struct atlas_rt_camera {
  * tracing (the engine's useRayBinning, RayTracingHelper.h:53, switched off
  * there; see atlas_rt_sort_rays_by_direction).  The image is unchanged. */
 #define ATLAS_RT_PT_RAY_BINNING 0x1u
+/* atlas_rt_pt_params.flags bits 8-11: progressive frames in flight (1-4; 0
+ * picks 3).  No engine counterpart: the engine renders one frame per call.
+ * Each frame in flight has its own stream and path buffers; the last
+ * bounces of a frame trace few rays, so the next frames' first bounces keep
+ * the GPU busy meanwhile.  Events order the frames' seeds, accumulation and
+ * cache training, so the image is the same bit for bit at every setting. */
+#define ATLAS_RT_PT_FRAMES_IN_FLIGHT_SHIFT 8
+#define ATLAS_RT_PT_FRAMES_IN_FLIGHT_MASK 0xF00u
+#define ATLAS_RT_PT_FRAMES_IN_FLIGHT(n) ((uint32_t)(n) << ATLAS_RT_PT_FRAMES_IN_FLIGHT_SHIFT)
 
 typedef struct atlas_rt_pt_params {
     uint32_t width;
@@ -603,7 +612,11 @@ atlas_rt_status atlas_rt_pathtracer_destroy(atlas_rt_pathtracer* pt);
  * frame: raygen, then bounces + 1 x (closest-hit trace, shade, shadow-ray
  * any-hit), then accumulate), asynchronously on `stream`.  The per-bounce
  * RNG seeds follow Random::SampleFastUniformFloat's PCG32 stream
- * (RandomHelper.cpp:8-51) and the sub-pixel jitter rayGen.csh:33-34. */
+ * (RandomHelper.cpp:8-51) and the sub-pixel jitter rayGen.csh:33-34.
+ * Frames run on the path tracer's own streams (ATLAS_RT_PT_FRAMES_IN_FLIGHT),
+ * after the work already queued on `stream`, and `stream` waits for them:
+ * work queued on `stream` afterwards sees the image.  Calls on different
+ * streams are ordered by the caller. */
 atlas_rt_status atlas_rt_pathtracer_render(atlas_rt_pathtracer* pt, const atlas_rt_camera* camera, uint32_t frames,
                                            void* stream);
 
@@ -753,11 +766,18 @@ atlas_rt_status atlas_rt_nrc_sizes(const atlas_rt_nrc* nrc, uint32_t* mlp_params
  * frame's bounces the cache takes train_steps Adam steps on those samples
  * (train_steps = 0 freezes it).  cache = NULL detaches it: plain path
  * tracing, the fallback.  Needs bounces >= 2.  The cache must outlive its
- * use by the path tracer. */
+ * use by the path tracer.
+ * query_lag 0: a frame queries the network as the previous frame's training
+ * left it, so each frame's queries wait for that training.  query_lag 1:
+ * frame i of a render call queries the network as frame i - 2's training
+ * left it (frames 0 and 1 of the call: as the call found it), so a frame's
+ * training overlaps the next frame's bounces when frames are in flight.
+ * Both give the same image at every ATLAS_RT_PT_FRAMES_IN_FLIGHT setting. */
 typedef struct atlas_rt_pt_cache_params {
     uint32_t train_stride;   /* e.g. 32 */
     uint32_t train_steps;    /* Adam steps per frame, e.g. 4 */
-    uint32_t reserved[2];
+    uint32_t query_lag;      /* 0 or 1 */
+    uint32_t reserved;       /* 0 */
 } atlas_rt_pt_cache_params;
 
 atlas_rt_status atlas_rt_pathtracer_set_cache(atlas_rt_pathtracer* pt, atlas_rt_nrc* cache,
