@@ -388,7 +388,12 @@ atlas_rt_status atlas_rt_pathtracer_render(atlas_rt_pathtracer* pt, const atlas_
         }
         order.index = f;
         atlas_rt_status st = launch_frame(P, cam, k, order);
-        if (st != ATLAS_RT_OK) return st;
+        if (st != ATLAS_RT_OK) {
+            // leave `s` behind whatever was queued, then report
+            for (uint32_t j = 0; j < n; j++)
+                if (used & (1u << j)) (void)hipStreamWaitEvent(s, P.slot[j].ev_acc, 0);
+            return st;
+        }
         order.prev2 = order.prev;
         order.prev = int32_t(k);
         P.last_slot = k;

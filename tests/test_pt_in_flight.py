@@ -115,3 +115,33 @@ def test_gpu_frames_in_flight_same_cache_training(torch_cuda, query_lag):
         diff = np.abs(c[k] - a[k]).mean()
         print(f"{'image' if k == 0 else 'weights'}: serial vs serial {noise:.3e}, 3 in flight vs serial {diff:.3e}")
         assert diff <= 4.0 * noise + 1e-9 * np.abs(a[k]).mean()
+
+
+@pytest.mark.gpu
+def test_gpu_frames_in_flight_alpha_and_triangle_lights(torch_cuda):
+    """The opacity-aware trace and shadow kernels and triangle lights: the
+    same image bit for bit for 1 and 3 frames in flight."""
+    print("case pt frames in flight alpha + triangle lights", flush=True)
+    meshes, normals, mats, xf, which = pt_cases.small_c2(n_meshes=4, n_inst=40, salt=7, extent=20.0, height=6.0)
+    g = np.random.default_rng(7)
+    choices = np.array([0.0, 0.3, 0.7, 1.0], np.float32)
+    ops = [choices[g.integers(0, 4, len(v))] for v in meshes]
+    S = pt_cases.oracle_scene(meshes, normals, mats, xf, which)
+    w_, blases = pt_cases.product_scene(meshes, normals, xf, which, S["boxes"])
+    for b, op in zip(blases, ops):
+        b.set_opacity(op)
+    emit = np.zeros((len(mats), 4), np.float32)
+    emit[0] = (1.0, 0.9, 0.7, 5.0)   # every triangle of mesh 0's instances emits
+    W_, H_ = 96, 64
+    cam = atlas_rt.camera((0.0, 15.0, -70.0), 0.0, 12.0, 50.0, W_ / H_)
+    imgs = {}
+    for n in (1, 3):
+        pt = atlas_rt.PathTracer(w_, W_, H_, 4, mats, W.sun_light(radiance=(0.5, 0.5, 0.5)), env=(0.02, 0.02, 0.03),
+                                 frames_in_flight=n)
+        pt.set_emissive_lights(emit)
+        assert len(pt.lights()) > 1
+        pt.render(cam, 5)
+        imgs[n] = pt.read().cpu().numpy()
+        pt.close()
+    assert imgs[1][..., :3].max() > 0 and np.all(imgs[1][..., 3] == 5)
+    assert imgs[3].tobytes() == imgs[1].tobytes()
