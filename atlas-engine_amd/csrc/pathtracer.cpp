@@ -372,14 +372,18 @@ atlas_rt_status atlas_rt_pathtracer_render(atlas_rt_pathtracer* pt, const atlas_
     // frames rotate over the slots, each on its slot's stream: they start
     // after the caller's earlier work on `s`, and `s` waits for all of them
     const uint32_t n = P.realtime ? 1u : P.slots;  // realtime mode has one slot sized for its rays
+    // the frames of earlier calls come first whatever stream they were queued on
+    FrameOrder order;
+    order.prev = P.hist_prev;
+    order.prev2 = P.hist_prev2;
     if (P.nrc && P.query_lag) {
         // frames 0 and 1 of the call query the network as the call finds it
+        if (order.prev >= 0) ATLAS_HIP_TRY(hipStreamWaitEvent(s, P.slot[order.prev].ev_train, 0));
         atlas_rt_status st = refresh_snapshots(P, s);
         if (st != ATLAS_RT_OK) return st;
     }
     ATLAS_HIP_TRY(hipEventRecord(P.ev_start, s));
     uint32_t used = 0;
-    FrameOrder order;
     for (uint32_t f = 0; f < frames; f++) {
         const uint32_t k = P.next_slot % n;
         if (!(used & (1u << k))) {
@@ -396,6 +400,8 @@ atlas_rt_status atlas_rt_pathtracer_render(atlas_rt_pathtracer* pt, const atlas_
         }
         order.prev2 = order.prev;
         order.prev = int32_t(k);
+        P.hist_prev2 = order.prev2;
+        P.hist_prev = order.prev;
         P.last_slot = k;
         P.next_slot = (k + 1) % n;
         P.host_samples++;
@@ -420,6 +426,7 @@ atlas_rt_status atlas_rt_pathtracer_set_realtime(atlas_rt_pathtracer* pt, const 
     const uint32_t rays = uint32_t(pixels * (params ? size_t(params->samples_per_frame) : 1u));
     P.rays = rays;
     P.next_slot = P.last_slot = 0;
+    P.hist_prev = P.hist_prev2 = -1;
     atlas_rt_status st = alloc_slot_rays(P, 0, rays);
     if (st != ATLAS_RT_OK || !params) return st;
     const size_t inst = P.world->instances.size();

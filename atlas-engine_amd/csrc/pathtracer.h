@@ -76,6 +76,8 @@ struct PathTracer {
     uint32_t slots = 1;           // frames in flight during progressive rendering
     uint32_t next_slot = 0;       // the slot of the next progressive frame
     uint32_t last_slot = 0;       // the slot of the last frame rendered
+    int32_t hist_prev = -1;       // slots of the last two progressive frames (-1: none yet):
+    int32_t hist_prev2 = -1;      // the next call's first frames are ordered after them
     hipEvent_t ev_start = nullptr;
     float4* accum = nullptr;
     const float4* xform = nullptr;      // borrowed: the world's transform table
@@ -119,8 +121,8 @@ struct PathTracer {
     uint16_t* snap_wn[2] = {nullptr, nullptr};     // BF16 forward fragment snapshots
 };
 
-// Where a progressive frame sits in its render call: the slots of the two
-// frames before it (-1: none in this call) and its index in the call.
+// Where a progressive frame sits: the slots of the two frames before it
+// (-1: none yet) and its index in its render call.
 struct FrameOrder {
     int32_t prev = -1;
     int32_t prev2 = -1;

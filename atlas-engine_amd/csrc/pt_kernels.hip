@@ -864,7 +864,9 @@ atlas_rt_status launch_frame_realtime(PathTracer& P, const Camera& cam, const at
 }
 
 // Frames in flight give the results of frames run one after another
-// because everything they share is ordered by events:
+// because everything they share is ordered by events (also across render
+// calls: the first frames of a call wait for the last frames of the one
+// before):
 // - the master block's PCG state and counters: frame f draws its seeds after
 //   frame f - 1 did (ev_begin);
 // - the accumulation image: frame f adds its radiance after frame f - 1

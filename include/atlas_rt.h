@@ -614,9 +614,9 @@ atlas_rt_status atlas_rt_pathtracer_destroy(atlas_rt_pathtracer* pt);
  * RNG seeds follow Random::SampleFastUniformFloat's PCG32 stream
  * (RandomHelper.cpp:8-51) and the sub-pixel jitter rayGen.csh:33-34.
  * Frames run on the path tracer's own streams (ATLAS_RT_PT_FRAMES_IN_FLIGHT),
- * after the work already queued on `stream`, and `stream` waits for them:
- * work queued on `stream` afterwards sees the image.  Calls on different
- * streams are ordered by the caller. */
+ * after the work already queued on `stream` and after the frames of earlier
+ * calls, and `stream` waits for them: work queued on `stream` afterwards
+ * sees the image. */
 atlas_rt_status atlas_rt_pathtracer_render(atlas_rt_pathtracer* pt, const atlas_rt_camera* camera, uint32_t frames,
                                            void* stream);
 

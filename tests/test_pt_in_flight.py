@@ -145,3 +145,27 @@ def test_gpu_frames_in_flight_alpha_and_triangle_lights(torch_cuda):
         pt.close()
     assert imgs[1][..., :3].max() > 0 and np.all(imgs[1][..., 3] == 5)
     assert imgs[3].tobytes() == imgs[1].tobytes()
+
+
+@pytest.mark.gpu
+def test_gpu_frames_in_flight_calls_on_different_streams(torch_cuda):
+    """Render calls queued on two different streams without any
+    synchronisation between them: the second call's first frames still
+    follow the first call's frames (seeds, accumulation), so the image equals
+    one serial call on the default stream."""
+    print("case pt frames in flight two streams", flush=True)
+    torch = torch_cuda
+    w_, blases, mats = _plain_scene(salt=11)
+    W_, H_ = 96, 64
+    cam = atlas_rt.camera((0.0, 15.0, -70.0), 0.0, 12.0, 50.0, W_ / H_)
+    ref = atlas_rt.PathTracer(w_, W_, H_, 4, mats, W.sun_light(), env=ENV, frames_in_flight=1)
+    ref.render(cam, 7)
+    img_ref = ref.read().cpu().numpy()
+    ref.close()
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    pt = atlas_rt.PathTracer(w_, W_, H_, 4, mats, W.sun_light(), env=ENV, frames_in_flight=3)
+    pt.render(cam, 3, stream=s1)
+    pt.render(cam, 4, stream=s2)
+    img = pt.read(stream=s2).cpu().numpy()
+    pt.close()
+    assert img.tobytes() == img_ref.tobytes()
