@@ -472,7 +472,19 @@ atlas_rt_status atlas_rt_pathtracer_render_realtime(atlas_rt_pathtracer* pt, con
     std::memcpy(cam.origin, camera->origin, 12);
     std::memcpy(cam.right, camera->right, 12);
     std::memcpy(cam.bottom, camera->bottom, 12);
-    return launch_frame_realtime(pt->P, cam, *matrices, reinterpret_cast<hipStream_t>(stream));
+    PathTracer& P = pt->P;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    // slot 0's buffers and the PCG stream: after the frames before, on any stream
+    if (P.hist_prev >= 0) ATLAS_HIP_TRY(hipStreamWaitEvent(s, P.slot[P.hist_prev].ev_acc, 0));
+    const atlas_rt_status st = launch_frame_realtime(P, cam, *matrices, s);
+    if (st != ATLAS_RT_OK) return st;
+    // the frames after this one wait for all of it
+    FrameSlot& S = P.slot[0];
+    ATLAS_HIP_TRY(hipEventRecord(S.ev_begin, s));
+    ATLAS_HIP_TRY(hipEventRecord(S.ev_train, s));
+    ATLAS_HIP_TRY(hipEventRecord(S.ev_acc, s));
+    P.hist_prev = P.hist_prev2 = 0;
+    return ATLAS_RT_OK;
 }
 
 atlas_rt_status atlas_rt_pathtracer_read_realtime(atlas_rt_pathtracer* pt, uint32_t buffer, void* d_out, void* stream) {
