@@ -104,8 +104,10 @@ __global__ __launch_bounds__(kTile * kTile) void k_temporal(TemporalArgs T) {
     const int x0 = int(blockIdx.x) * kTile - kRadius, y0 = int(blockIdx.y) * kTile - kRadius;
     for (int i = int(threadIdx.x); i < kSide * kSide; i += kTile * kTile) {
         const uint32_t idx = at(T, x0 + i % kSide, y0 + i / kSide);
+        // each tap's YCoCg once per tile entry instead of once per pixel that reads it
         const float4 c = T.frame[idx];
-        tile[i] = make_float4(c.x, c.y, c.z, T.depth[idx]);
+        const F3 r = to_ycocg(f3(c.x, c.y, c.z));
+        tile[i] = make_float4(r.x, r.y, r.z, T.depth[idx]);
     }
     __syncthreads();
     const int lx = int(threadIdx.x) % kTile, ly = int(threadIdx.x) / kTile;
@@ -120,12 +122,15 @@ __global__ __launch_bounds__(kTile * kTile) void k_temporal(TemporalArgs T) {
     // ComputeVarianceMinMax: i over x, j over y
     F3 m1 = f3(0.0f, 0.0f, 0.0f), m2 = f3(0.0f, 0.0f, 0.0f);
     float total = 0.0f;
+    // the per-tap division by phi as a product with its reciprocal, and exp
+    // on the hardware exp2 (GLSL allows 2.5 ulp for a division and
+    // 3 + 2 |x| ulp for exp)
+    const float inv_phi = 1.0f / gmax(1.0f, fabsf(0.025f * c.depth));
     for (int i = -kRadius; i <= kRadius; i++)
         for (int j = -kRadius; j <= kRadius; j++) {
             const float4 t = tile[(ly + kRadius + j) * kSide + (lx + kRadius + i)];
-            const F3 r = to_ycocg(f3(t.x, t.y, t.z));
-            const float phi = gmax(1.0f, fabsf(0.025f * c.depth));
-            const float w = gmin(1.0f, expf(-fabsf(c.depth - t.w) / phi));
+            const F3 r = f3(t.x, t.y, t.z);
+            const float w = gmin(1.0f, __expf(-fabsf(c.depth - t.w) * inv_phi));
             m1 = f3(m1.x + r.x * w, m1.y + r.y * w, m1.z + r.z * w);
             m2 = f3(m2.x + (r.x * r.x) * w, m2.y + (r.y * r.y) * w, m2.z + (r.z * r.z) * w);
             total += w;
