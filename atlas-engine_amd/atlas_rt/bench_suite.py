@@ -282,7 +282,8 @@ def _time_to(curve, target):
 
 
 # cache settings compared: (training-pixel stride, Adam steps per frame, warm-up frames, query lag)
-CACHE_SETTINGS = ((128, 1, 8, 1), (512, 1, 8, 1), (2048, 1, 8, 1), (4096, 1, 8, 1), (128, 1, 8, 0))
+CACHE_SETTINGS = ((128, 1, 8, 1), (512, 1, 8, 1), (2048, 1, 8, 1), (2048, 1, 2, 1), (2048, 2, 2, 1),
+                  (4096, 1, 8, 1), (128, 1, 8, 0))
 
 
 def _study(torch, log, name, meshes, mats, xf, which, cc, bounds, ref_spp, bounces=8,
