@@ -178,6 +178,14 @@ class Blas:
         check(lib().atlas_rt_last_kernel_ms(self._h, C.byref(ms)))
         return ms.value
 
+    def set_stack_limit(self, entries):
+        """Cap the traversal stack at `entries` (0: sized from the tree)."""
+        check(lib().atlas_rt_blas_set_stack_limit(self._h, int(entries)))
+
+    def trace_status(self, stream=None):
+        """The overflow status of the traces since the last call (waits for `stream`)."""
+        return lib().atlas_rt_blas_trace_status(self._h, _stream_ptr(stream))
+
 
 def aabb_tree_build(boxes, parallel=True):
     """The engine's AABB-only tree (Volume::BVH(aabbs)) on the host: (nodes,
@@ -267,6 +275,14 @@ class World:
         order = np.zeros(i, dtype=np.uint32)
         check(lib().atlas_rt_world_export(self._h, _ptr(nodes) if n else None, _ptr(inst), _ptr(order)))
         return nodes, inst, order
+
+    def set_stack_limit(self, entries):
+        """Cap the traversal stack of the world's traces (0: sized from the depths)."""
+        check(lib().atlas_rt_world_set_stack_limit(self._h, int(entries)))
+
+    def trace_status(self, stream=None):
+        """The overflow status of the world's traces since the last call."""
+        return lib().atlas_rt_world_trace_status(self._h, _stream_ptr(stream))
 
     def trace_closest_rays(self, d_rays, d_hits, d_inst=None, count=None, mask=_abi.MASK_ALL, stream=None):
         n = count if count is not None else d_rays.numel() // 8
@@ -568,6 +584,10 @@ class PathTracer:
             out = torch.empty((self.height, self.width, 4), dtype=torch.float32, device="cuda")
         check(lib().atlas_rt_pathtracer_read(self._h, _ptr(out), _stream_ptr(stream)))
         return out
+
+    def status(self, stream=None):
+        """The overflow status of the frames since the last call (waits for them)."""
+        return lib().atlas_rt_pathtracer_status(self._h, _stream_ptr(stream))
 
     def sample_count(self):
         n = C.c_int32()

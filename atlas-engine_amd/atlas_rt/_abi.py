@@ -167,6 +167,11 @@ PROTOTYPES = {
     "atlas_rt_trace_closest_host": (status, [vp, vp, vp, u32]),
     "atlas_rt_trace_any_host": (status, [vp, vp, vp, u32]),
     "atlas_rt_last_kernel_ms": (status, [vp, C.POINTER(f32)]),
+    "atlas_rt_blas_set_stack_limit": (status, [vp, u32]),
+    "atlas_rt_blas_trace_status": (status, [vp, vp]),
+    "atlas_rt_world_set_stack_limit": (status, [vp, u32]),
+    "atlas_rt_world_trace_status": (status, [vp, vp]),
+    "atlas_rt_pathtracer_status": (status, [vp, vp]),
     "atlas_rt_world_create": (status, [C.POINTER(vp)]),
     "atlas_rt_world_destroy": (status, [vp]),
     "atlas_rt_world_update": (status, [vp, vp, u32, vp, vp, u32]),

@@ -31,8 +31,8 @@ size_t bin_sort_workspace(uint32_t cap);
 // atlas_rt_ray; rec = 3: a path record), direction in float4 [1].xyz.
 // The count is *d_count when d_count is non-null (a device counter), else
 // count; at most cap.  perm (optional): out[i] = in[perm[i]].  bin_offsets
-// (optional, 65 entries): the first output index of each bin, then the count.
-// The input and output must not overlap.
+// (optional, 65 entries): the first output index of each bin, then the count
+// (all zero when cap is 0).  The input and output must not overlap.
 hipError_t launch_bin_sort(const float4* in, float4* out, int rec, const uint32_t* d_count, uint32_t count,
                            uint32_t cap, void* workspace, uint32_t* perm, uint32_t* bin_offsets, hipStream_t s);
 

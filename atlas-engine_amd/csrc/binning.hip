@@ -140,10 +140,19 @@ uint32_t tiles_for(uint32_t cap) { return uint32_t((size_t(cap) + kTile - 1) / k
 
 size_t bin_sort_workspace(uint32_t cap) { return size_t(kRayBins) * (tiles_for(cap) ? tiles_for(cap) : 1) * 4; }
 
+// Bin offsets of an empty sort: all zero.  A kernel rather than a memset, so
+// that a captured graph of the call holds no memset node.
+__global__ void k_bin_offsets_zero(uint32_t* __restrict__ bin_offsets) {
+    if (threadIdx.x <= kRayBins) bin_offsets[threadIdx.x] = 0;
+}
+
 hipError_t launch_bin_sort(const float4* in, float4* out, int rec, const uint32_t* d_count, uint32_t count,
                            uint32_t cap, void* workspace, uint32_t* perm, uint32_t* bin_offsets, hipStream_t s) {
     const uint32_t tiles = tiles_for(cap);
-    if (tiles == 0) return hipSuccess;
+    if (tiles == 0) {
+        if (bin_offsets) hipLaunchKernelGGL(k_bin_offsets_zero, dim3(1), dim3(128), 0, s, bin_offsets);
+        return hipGetLastError();
+    }
     uint32_t* hist = static_cast<uint32_t*>(workspace);
     hipLaunchKernelGGL(k_bin_hist, dim3(tiles), dim3(kB), 0, s, in, rec, d_count, count, cap, tiles, hist);
     hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(kScanThreads), 0, s, hist, tiles, d_count, count, cap, bin_offsets);

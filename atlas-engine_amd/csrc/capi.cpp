@@ -10,6 +10,20 @@ const char* last_error();
 
 using namespace atlas_rt;
 
+atlas_rt_status atlas_rt::take_overflow_flag(void* flags, void* stream, const char* who) {
+    if (!flags) return ATLAS_RT_OK;
+    ATLAS_HIP_TRY(hipStreamSynchronize(reinterpret_cast<hipStream_t>(stream)));
+    uint32_t* word = reinterpret_cast<uint32_t*>(flags) + 2;
+    uint32_t v = 0;
+    ATLAS_HIP_TRY(hipMemcpy(&v, word, sizeof(v), hipMemcpyDeviceToHost));
+    if (!v) return ATLAS_RT_OK;
+    const uint32_t zero = 0;
+    ATLAS_HIP_TRY(hipMemcpy(word, &zero, sizeof(zero), hipMemcpyHostToDevice));
+    return fail(ATLAS_RT_E_STACK_OVERFLOW, std::string(who) +
+                                               ": a ray needed more traversal stack entries than the limit; it was "
+                                               "reported as a miss");
+}
+
 static atlas_rt_status check_blas_ready(const atlas_rt_blas* b, const char* who) {
     if (!b) return fail(ATLAS_RT_E_INVALID_ARG, std::string(who) + ": NULL BLAS handle");
     if (!b->on_device) return fail(ATLAS_RT_E_INVALID_ARG, std::string(who) + ": BLAS was built host-only");
@@ -79,10 +93,6 @@ atlas_rt_status atlas_rt_sort_rays_by_direction(const atlas_rt_ray* d_rays, uint
         return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_sort_rays_by_direction: workspace smaller than "
                                             "atlas_rt_sort_rays_workspace(count)");
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    if (count == 0) {
-        if (d_bin_offsets) ATLAS_HIP_TRY(hipMemsetAsync(d_bin_offsets, 0, (kRayBins + 1) * sizeof(uint32_t), s));
-        return ATLAS_RT_OK;
-    }
     const hipError_t e = launch_bin_sort(reinterpret_cast<const float4*>(d_rays), reinterpret_cast<float4*>(d_out), 2,
                                          nullptr, count, count, d_workspace, d_perm, d_bin_offsets, s);
     if (e != hipSuccess) return hip_fail(e, "atlas_rt_sort_rays_by_direction");
@@ -179,6 +189,17 @@ atlas_rt_status atlas_rt_trace_any_host(const atlas_rt_blas* blas, const atlas_r
     hipFree(dr);
     hipFree(dh);
     return st;
+}
+
+atlas_rt_status atlas_rt_blas_set_stack_limit(atlas_rt_blas* blas, uint32_t entries) {
+    if (!blas) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_blas_set_stack_limit: NULL BLAS handle");
+    blas->stack_limit = entries;
+    return ATLAS_RT_OK;
+}
+
+atlas_rt_status atlas_rt_blas_trace_status(atlas_rt_blas* blas, void* stream) {
+    if (!blas) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_blas_trace_status: NULL BLAS handle");
+    return take_overflow_flag(blas->flags.ptr, stream, "atlas_rt_blas_trace_status");
 }
 
 atlas_rt_status atlas_rt_last_kernel_ms(const atlas_rt_blas* blas, float* ms) {

@@ -246,6 +246,112 @@ void atlas_rt::pt::free_cache_buffers(PathTracer& P) {
     P.shade_args.cache = CacheArgs{};
 }
 
+atlas_rt_status atlas_rt::pt::apply_stack_cap(PathTracer& P) {
+    const uint32_t cap = P.world->stack_cap();
+    if (cap == P.cap && P.cap_bound) return ATLAS_RT_OK;
+    // a larger spill replaces the old one, which frames in flight may still use
+    if (P.cap_bound && cap > P.cap) ATLAS_HIP_TRY(hipDeviceSynchronize());
+    const size_t pixels = size_t(P.width) * P.height;
+    for (uint32_t k = 0; k < P.slots && cap > 32; k++) {
+        if (!P.slot[k].ctl) continue;
+        const size_t rays = k == 0 ? P.rays : pixels;  // realtime mode renders in slot 0
+        const size_t grid_threads = (rays + 127) / 128 * 128;
+        atlas_rt_status st = P.slot[k].spill.ensure(size_t(cap - 32) * grid_threads * sizeof(int32_t));
+        if (st != ATLAS_RT_OK) return st;
+    }
+    P.cap = cap;
+    P.cap_bound = true;
+    return ATLAS_RT_OK;
+}
+
+atlas_rt_status atlas_rt::pt::bind_world(PathTracer& P) {
+    const atlas_rt_world& W = *P.world;
+    if (P.world_gen == W.generation) return ATLAS_RT_OK;
+    if (!W.valid) return fail(ATLAS_RT_E_INVALID_ARG, "path tracer: the world has no instances (failed update?)");
+    for (const atlas_rt_blas* b : W.blas_handles)
+        if (!b->d_shade)
+            return fail(ATLAS_RT_E_INVALID_ARG, "path tracer: a BLAS of the updated world has no shading data "
+                                                "(atlas_rt_blas_set_shading)");
+    // frames in flight still read the tables and the realtime history
+    ATLAS_HIP_TRY(hipDeviceSynchronize());
+    const size_t nb = W.blas_handles.size();
+    if (nb != P.table_count) {
+        const void* old[] = {P.shade_table, P.uv_table, P.tangent_table};
+        for (const void* p : old)
+            if (p) (void)hipFree(const_cast<void*>(p));
+        P.shade_table = P.uv_table = P.tangent_table = nullptr;
+        P.table_count = 0;
+        atlas_rt_status st = dev_alloc(P.shade_table, nb);
+        if (st == ATLAS_RT_OK) st = dev_alloc(P.uv_table, nb);
+        if (st == ATLAS_RT_OK) st = dev_alloc(P.tangent_table, nb);
+        if (st != ATLAS_RT_OK) return st;
+        P.table_count = nb;
+    }
+    std::vector<const float4*> shade(nb), uv(nb), tangent(nb);
+    bool alpha = false;
+    for (size_t k = 0; k < nb; k++) {
+        shade[k] = W.blas_handles[k]->d_shade;
+        uv[k] = W.blas_handles[k]->d_uv;
+        tangent[k] = W.blas_handles[k]->d_tangent;
+        alpha = alpha || W.blas_handles[k]->has_alpha;
+    }
+    ATLAS_HIP_TRY(hipMemcpy(const_cast<float4**>(P.shade_table), shade.data(), nb * sizeof(float4*),
+                            hipMemcpyHostToDevice));
+    ATLAS_HIP_TRY(hipMemcpy(const_cast<float4**>(P.uv_table), uv.data(), nb * sizeof(float4*), hipMemcpyHostToDevice));
+    ATLAS_HIP_TRY(hipMemcpy(const_cast<float4**>(P.tangent_table), tangent.data(), nb * sizeof(float4*),
+                            hipMemcpyHostToDevice));
+    P.alpha = alpha;
+    P.world_dev = W.dev();
+    P.xform = reinterpret_cast<const float4*>(W.d_xform);  // the world's, kept current by refits
+    {
+        atlas_rt_status st = apply_stack_cap(P);
+        if (st != ATLAS_RT_OK) return st;
+    }
+    ShadeArgs& A = P.shade_args;
+    A.world = P.world_dev;
+    A.xform = P.xform;
+    A.shade = P.shade_table;
+    A.tangent = P.tangent_table;
+    A.alpha.shade = P.shade_table;
+    A.alpha.uv = P.uv_table;
+    if (P.realtime && P.rt_last_xform) {
+        // The update re-packs the instances: the previous frame's transform of
+        // each instance moves to its new slot (matched by caller index); a new
+        // instance starts without motion.
+        const size_t m_old = P.world_order.size(), m_new = W.order.size();
+        std::vector<float> last(12 * m_old), moved(12 * m_new);
+        if (m_old)
+            ATLAS_HIP_TRY(hipMemcpy(last.data(), P.rt_last_xform, 12 * m_old * sizeof(float), hipMemcpyDeviceToHost));
+        std::vector<int64_t> where;
+        for (size_t j = 0; j < m_old; j++) {
+            if (P.world_order[j] >= where.size()) where.resize(P.world_order[j] + 1, -1);
+            where[P.world_order[j]] = int64_t(j);
+        }
+        for (size_t j = 0; j < m_new; j++) {
+            const uint32_t id = W.order[j];
+            const int64_t src = id < where.size() ? where[id] : -1;
+            std::memcpy(&moved[12 * j], src >= 0 ? &last[12 * size_t(src)] : &W.xform[12 * j], 12 * sizeof(float));
+        }
+        if (m_new != m_old) {
+            (void)hipFree(P.rt_last_xform);
+            P.rt_last_xform = nullptr;
+            atlas_rt_status st = dev_alloc(P.rt_last_xform, 3 * m_new);
+            if (st != ATLAS_RT_OK) return st;
+        }
+        ATLAS_HIP_TRY(hipMemcpy(P.rt_last_xform, moved.data(), 12 * m_new * sizeof(float), hipMemcpyHostToDevice));
+    }
+    // the triangle lights name instance slots and carry world positions
+    atlas_rt_status st = ATLAS_RT_OK;
+    if (!P.emitters.empty()) st = const_cast<atlas_rt_world&>(W).sync_host();
+    if (st != ATLAS_RT_OK) return st;
+    build_light_list(P, P.emitters.empty() ? nullptr : P.emitters.data(), uint32_t(P.emitters.size() / 4));
+    st = upload_lights(P);
+    if (st != ATLAS_RT_OK) return st;
+    P.world_order = W.order;
+    P.world_gen = W.generation;
+    return ATLAS_RT_OK;
+}
+
 extern "C" {
 
 atlas_rt_status atlas_rt_pathtracer_create(const atlas_rt_world* world, const atlas_rt_pt_params* params,
@@ -280,11 +386,9 @@ atlas_rt_status atlas_rt_pathtracer_create(const atlas_rt_world* world, const at
     P.bounces = params->bounces;
     P.punctual.assign(lights, lights + light_count);
     P.material_count = material_count;
+    P.world_gen = ~world->generation;
     P.world = world;
-    P.xform = reinterpret_cast<const float4*>(world->d_xform);  // the world's, kept current by refits
-    for (const atlas_rt_blas* b : world->blas_handles) P.alpha = P.alpha || b->has_alpha;
-    P.world_dev = world->dev();
-    P.cap = world->max_depth + 1;
+    P.cap = world->stack_cap();  // slot spills are sized for it; bind_world rebinds on every update
     const size_t pixels = size_t(P.width) * P.height;
     atlas_rt_status st = ATLAS_RT_OK;
     P.rays = uint32_t(pixels);
@@ -293,59 +397,34 @@ atlas_rt_status atlas_rt_pathtracer_create(const atlas_rt_world* world, const at
     if (st == ATLAS_RT_OK && hipEventCreateWithFlags(&P.ev_start, hipEventDisableTiming) != hipSuccess)
         st = fail(ATLAS_RT_E_HIP, "atlas_rt_pathtracer_create: hipEventCreateWithFlags failed");
     if (st == ATLAS_RT_OK) st = dev_alloc(P.accum, pixels);
-    if (st == ATLAS_RT_OK) st = dev_alloc(P.shade_table, world->blas_handles.size());
-    if (st == ATLAS_RT_OK) st = dev_alloc(P.uv_table, world->blas_handles.size());
-    if (st == ATLAS_RT_OK) st = dev_alloc(P.tangent_table, world->blas_handles.size());
     if (st == ATLAS_RT_OK) st = dev_alloc(P.materials, material_count);
     if (st != ATLAS_RT_OK) {
         free_all(P);
         delete pt;
         return st;
     }
-    std::vector<const float4*> shade(world->blas_handles.size()), uv(world->blas_handles.size()),
-        tangent(world->blas_handles.size());
-    for (size_t k = 0; k < shade.size(); k++) {
-        shade[k] = world->blas_handles[k]->d_shade;
-        uv[k] = world->blas_handles[k]->d_uv;
-        tangent[k] = world->blas_handles[k]->d_tangent;
-    }
     FrameCtl ctl{};
     ctl.pcg_state = pcg_initial_state();
     hipError_t e = hipSuccess;
     e = hipMemcpy(P.ctl, &ctl, sizeof(ctl), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(P.accum, 0, pixels * sizeof(float4));
-    if (e == hipSuccess)
-        e = hipMemcpy(const_cast<float4**>(P.shade_table), shade.data(), shade.size() * sizeof(float4*),
-                      hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-        e = hipMemcpy(const_cast<float4**>(P.uv_table), uv.data(), uv.size() * sizeof(float4*), hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-        e = hipMemcpy(const_cast<float4**>(P.tangent_table), tangent.data(), tangent.size() * sizeof(float4*),
-                      hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(P.materials, materials, material_count * sizeof(Material), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         free_all(P);
         delete pt;
         return hip_fail(e, "atlas_rt_pathtracer_create");
     }
-    build_light_list(P, nullptr, 0);
-    st = upload_lights(P);
+    ShadeArgs& A = P.shade_args;
+    A.materials = P.materials;
+    A.alpha = AlphaArgs{nullptr, nullptr, P.materials, nullptr, nullptr, 0u, 0u};
+    A.bounces = P.bounces;
+    std::memcpy(A.env, params->env_radiance, 12);
+    st = bind_world(P);
     if (st != ATLAS_RT_OK) {
         free_all(P);
         delete pt;
         return st;
     }
-    ShadeArgs& A = P.shade_args;
-    A.world = P.world_dev;
-    A.xform = P.xform;
-    A.shade = P.shade_table;
-    A.tangent = P.tangent_table;
-    A.materials = P.materials;
-    A.lights = P.lights;
-    A.light_count = P.light_count;
-    A.alpha = AlphaArgs{P.shade_table, P.uv_table, P.materials, nullptr, nullptr, 0u, 0u};
-    A.bounces = P.bounces;
-    std::memcpy(A.env, params->env_radiance, 12);
     *out = pt;
     return ATLAS_RT_OK;
 }
@@ -369,6 +448,9 @@ atlas_rt_status atlas_rt_pathtracer_render(atlas_rt_pathtracer* pt, const atlas_
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     PathTracer& P = pt->P;
     if (!frames) return ATLAS_RT_OK;
+    atlas_rt_status bst = bind_world(P);
+    if (bst == ATLAS_RT_OK) bst = apply_stack_cap(P);  // a limit set on the world since the bind
+    if (bst != ATLAS_RT_OK) return bst;
     // frames rotate over the slots, each on its slot's stream: they start
     // after the caller's earlier work on `s`, and `s` waits for all of them
     const uint32_t n = P.realtime ? 1u : P.slots;  // realtime mode has one slot sized for its rays
@@ -420,6 +502,8 @@ atlas_rt_status atlas_rt_pathtracer_set_realtime(atlas_rt_pathtracer* pt, const 
                    params->history_length_max < 1 || !(params->max_radiance > 0.0f)))
         return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_set_realtime: need 1 <= samples_per_frame <= 64, "
                                             "history_length_max >= 1, max_radiance > 0");
+    atlas_rt_status bst = bind_world(P);
+    if (bst != ATLAS_RT_OK) return bst;
     ATLAS_HIP_TRY(hipDeviceSynchronize());
     free_realtime(P);
     const size_t pixels = size_t(P.width) * P.height;
@@ -474,6 +558,9 @@ atlas_rt_status atlas_rt_pathtracer_render_realtime(atlas_rt_pathtracer* pt, con
     std::memcpy(cam.bottom, camera->bottom, 12);
     PathTracer& P = pt->P;
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    atlas_rt_status bst = bind_world(P);
+    if (bst == ATLAS_RT_OK) bst = apply_stack_cap(P);
+    if (bst != ATLAS_RT_OK) return bst;
     // slot 0's buffers and the PCG stream: after the frames before, on any stream
     if (P.hist_prev >= 0) ATLAS_HIP_TRY(hipStreamWaitEvent(s, P.slot[P.hist_prev].ev_acc, 0));
     const atlas_rt_status st = launch_frame_realtime(P, cam, *matrices, s);
@@ -560,10 +647,16 @@ atlas_rt_status atlas_rt_pathtracer_set_emissive_lights(atlas_rt_pathtracer* pt,
         return fail(ATLAS_RT_E_SIZE_MISMATCH,
                     "atlas_rt_pathtracer_set_emissive_lights: one emitter per material table entry expected");
     ATLAS_HIP_TRY(hipDeviceSynchronize());
-    // the instance transforms of the world's last update or refit
-    atlas_rt_status st = const_cast<atlas_rt_world*>(P.world)->sync_host();
+    if (emitters && count)
+        P.emitters.assign(emitters, emitters + 4 * size_t(count));
+    else
+        P.emitters.clear();
+    atlas_rt_status st = bind_world(P);  // after an update this also builds the list
     if (st != ATLAS_RT_OK) return st;
-    build_light_list(P, count ? emitters : nullptr, count);
+    // the instance transforms of the world's last update or refit
+    st = const_cast<atlas_rt_world*>(P.world)->sync_host();
+    if (st != ATLAS_RT_OK) return st;
+    build_light_list(P, P.emitters.empty() ? nullptr : P.emitters.data(), uint32_t(P.emitters.size() / 4));
     return upload_lights(P);
 }
 
@@ -594,6 +687,18 @@ atlas_rt_status atlas_rt_pathtracer_read(atlas_rt_pathtracer* pt, float* d_out, 
     ATLAS_HIP_TRY(hipMemcpyAsync(d_out, P.accum, size_t(P.width) * P.height * sizeof(float4), hipMemcpyDeviceToDevice, s));
     ATLAS_HIP_TRY(hipStreamSynchronize(s));
     return ATLAS_RT_OK;
+}
+
+atlas_rt_status atlas_rt_pathtracer_status(atlas_rt_pathtracer* pt, void* stream) {
+    if (!pt) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_status: NULL path tracer");
+    ATLAS_HIP_TRY(hipStreamSynchronize(reinterpret_cast<hipStream_t>(stream)));
+    ATLAS_HIP_TRY(hipDeviceSynchronize());  // frames in flight run on the slot streams
+    atlas_rt_status out = ATLAS_RT_OK;
+    for (uint32_t k = 0; k < kMaxFrameSlots; k++) {
+        const atlas_rt_status st = take_overflow_flag(pt->P.slot[k].flags.ptr, nullptr, "atlas_rt_pathtracer_status");
+        if (st != ATLAS_RT_OK) out = st;
+    }
+    return out;
 }
 
 atlas_rt_status atlas_rt_pathtracer_sample_count(const atlas_rt_pathtracer* pt, int32_t* count) {

@@ -68,8 +68,14 @@ struct PathTracer {
     int32_t bounces = 0;
     int32_t light_count = 0;
     const atlas_rt_world* world = nullptr;
+    // The world's device arrays as of its update number world_gen; every
+    // call that launches work rebinds them when the world has been updated
+    // since (bind_world).
+    uint64_t world_gen = 0;
+    std::vector<uint32_t> world_order;  // the world's instance order at that update
     WorldDev world_dev{};
-    uint32_t cap = 0;
+    uint32_t cap = 0;             // traversal stack entries (the world's stack_cap())
+    bool cap_bound = false;       // the slot spills are sized for cap
     // device buffers
     FrameCtl* ctl = nullptr;      // Random's PCG state, samples and frames so far
     FrameSlot slot[kMaxFrameSlots];
@@ -81,6 +87,7 @@ struct PathTracer {
     hipEvent_t ev_start = nullptr;
     float4* accum = nullptr;
     const float4* xform = nullptr;      // borrowed: the world's transform table
+    size_t table_count = 0;             // entries of the three per-BLAS tables below
     const float4** shade_table = nullptr;
     const float4** uv_table = nullptr;
     const float4** tangent_table = nullptr;
@@ -91,6 +98,7 @@ struct PathTracer {
     bool alpha = false;                     // some BLAS has non-opaque triangles
     std::vector<atlas_rt_light> punctual;   // the lights given at creation
     std::vector<Light> light_list;          // host copy of the device list
+    std::vector<float> emitters;            // the last emissive table given (empty: none)
     TexDesc* tex = nullptr;
     uint8_t* texels = nullptr;
     uint32_t tex_count = 0;
@@ -163,6 +171,14 @@ atlas_rt_status launch_bounces(PathTracer& P, uint32_t k, hipStream_t s, hipEven
 // emitters, the triangle lights) into P.light_list.
 void build_light_list(PathTracer& P, const float* emitters, uint32_t emitter_count);
 void free_cache_buffers(PathTracer& P);
+// Binds the world's current device arrays if an update replaced them since
+// the last bind (waits for the device first): BLAS tables, transform table,
+// stack bound and spill, the triangle lights, and in realtime mode the
+// previous frame's transforms, carried over per caller instance index.
+atlas_rt_status bind_world(PathTracer& P);
+// Follows the world's stack bound (atlas_rt_world_set_stack_limit): the
+// stack entries of the traces and the slot spills sized for them.
+atlas_rt_status apply_stack_cap(PathTracer& P);
 
 }  // namespace pt
 }  // namespace atlas_rt

@@ -98,6 +98,9 @@ struct atlas_rt_blas {
     atlas_rt::Scratch spill;
     atlas_rt::Scratch flags;  // [0] ray counter, [1] finished waves, [2] stack overflow flag
     atlas_rt::LaunchTiming timing;
+    // Most traversal stack entries per ray (atlas_rt_blas_set_stack_limit);
+    // 0: sized from the tree, which never overflows.
+    uint32_t stack_limit = 0;
 };
 
 namespace atlas_rt {
@@ -115,6 +118,14 @@ struct TraceLaunch {
 };
 
 int device_cu_count();
+
+// The stack entries a trace may use: what the tree needs, or fewer when the
+// caller set a limit (limit 0: none).
+inline uint32_t limit_cap(uint32_t need, uint32_t limit) { return limit && limit < need ? limit : need; }
+
+// Waits for `stream`, then reads and clears the stack-overflow word
+// flags[2] of a flag block (NULL: no trace has run yet).
+atlas_rt_status take_overflow_flag(void* flags, void* stream, const char* who);
 
 atlas_rt_status launch_blas_closest(atlas_rt_blas& blas, const TraceLaunch& L);
 atlas_rt_status launch_blas_any(atlas_rt_blas& blas, const TraceLaunch& L);

@@ -192,7 +192,7 @@ static atlas_rt_status prepare(atlas_rt_blas& blas, uint32_t threads, uint32_t l
                                int node_w = 2) {
     // The binary stack never holds more entries than the deepest leaf's
     // depth; the wide bounds come from the collapse.
-    cap = variant_cap(node_w, blas.max_depth + 1, blas.cap4, blas.cap8);
+    cap = limit_cap(variant_cap(node_w, blas.max_depth + 1, blas.cap4, blas.cap8), blas.stack_limit);
     if (cap > lds_depth) {
         const size_t need = size_t(cap - lds_depth) * threads * sizeof(int32_t);
         atlas_rt_status st = blas.spill.ensure(need);

@@ -233,7 +233,7 @@ constexpr int kLdsDepth = 32;
 }  // namespace
 
 static atlas_rt_status world_prepare(atlas_rt_world& w, uint32_t threads, uint32_t& cap) {
-    cap = w.max_depth + 1;
+    cap = w.stack_cap();
     if (cap > uint32_t(kLdsDepth)) {
         atlas_rt_status st = w.spill.ensure(size_t(cap - kLdsDepth) * threads * sizeof(int32_t));
         if (st != ATLAS_RT_OK) return st;

@@ -212,6 +212,7 @@ atlas_rt_status atlas_rt_world_update(atlas_rt_world* w, const atlas_rt_instance
     }
     w->free_device();
     w->valid = false;
+    w->generation++;
     atlas_rt_status st = upload_vec(w->d_tlas, bo.nodes);
     if (st == ATLAS_RT_OK) st = upload_vec(w->d_instances, packed);
     if (st == ATLAS_RT_OK) st = upload_vec(w->d_table, table);
@@ -284,6 +285,17 @@ atlas_rt_status atlas_rt_world_refit(atlas_rt_world* w, const atlas_rt_instance_
     if (st != ATLAS_RT_OK) return st;
     if (!d_instances) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_world_refit: NULL instances");
     return launch_world_refit(*w, d_instances, reinterpret_cast<hipStream_t>(stream));
+}
+
+atlas_rt_status atlas_rt_world_set_stack_limit(atlas_rt_world* w, uint32_t entries) {
+    if (!w) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_world_set_stack_limit: NULL world");
+    w->stack_limit = entries;
+    return ATLAS_RT_OK;
+}
+
+atlas_rt_status atlas_rt_world_trace_status(atlas_rt_world* w, void* stream) {
+    if (!w) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_world_trace_status: NULL world");
+    return take_overflow_flag(w->flags.ptr, stream, "atlas_rt_world_trace_status");
 }
 
 atlas_rt_status atlas_rt_world_info(const atlas_rt_world* w, uint32_t* tlas_node_count, uint32_t* instance_count,

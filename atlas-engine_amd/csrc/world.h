@@ -45,6 +45,14 @@ struct atlas_rt_world {
     uint32_t tlas_depth = 0;
     uint32_t max_depth = 0;  // tlas depth + deepest BLAS
     bool valid = false;
+    // Bumped by every atlas_rt_world_update, which replaces the device arrays
+    // (a refit rewrites them in place): objects that keep device pointers of
+    // the world rebind when it changes.
+    uint64_t generation = 0;
+    // Most traversal stack entries per ray (atlas_rt_world_set_stack_limit);
+    // 0: sized from the TLAS and BLAS depths, which never overflows.
+    uint32_t stack_limit = 0;
+    uint32_t stack_cap() const { return atlas_rt::limit_cap(max_depth + 1, stack_limit); }
 
     int device = -1;
     void* d_tlas = nullptr;

@@ -60,6 +60,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU work for the baseline sample")
     ap.add_argument("--no-extra", action="store_true", help="only the c0 headline (skip c1, c3 and the cache study)")
     ap.add_argument("--ref-spp", type=int, default=4096, help="samples per pixel of the plain reference render")
+    ap.add_argument("--probe", nargs="+", metavar="ARG",
+                    help="run one fixed profiler workload instead of the benchmark (see PROBES)")
     return ap.parse_args()
 
 
@@ -100,48 +102,459 @@ def pmc_traffic(kernel):
     return out
 
 
-def host_threads():
+def host_cpu():
+    """The host the CPU leg runs on: usable cores, nproc and the CPU model
+    (lscpu's "Model name", read from /proc/cpuinfo)."""
     try:
-        n = len(os.sched_getaffinity(0))
+        usable = len(os.sched_getaffinity(0))
     except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(n, 16))
+        usable = os.cpu_count() or 1
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"threads": max(1, usable), "nproc": os.cpu_count(), "model": model}
 
 
-def cpu_baseline(nodes, tris, orig, rays, gpu_hits, budget_s):
-    """Oracle (restated engine traversal) on the host cores, on a bounded sample."""
+def _oracle():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_rt
+    return oracle_rt
 
-    threads = host_threads()
-    probe = rays[: min(len(rays), 32768)]
-    t0 = time.perf_counter()
-    oracle_rt.trace_closest(nodes, tris, orig, probe, threads=threads)
-    rate = len(probe) / max(time.perf_counter() - t0, 1e-6)
-    n = int(min(len(rays), max(65536, rate * budget_s / 6.0)))
-    sample = rays[:n]
+
+def _median_rate(fn, n, reps=5):
+    """n / median seconds of `reps` runs after one warm-up."""
     times = []
-    ref = None
-    for i in range(6):  # 1 warm-up + median of 5
+    for i in range(reps + 1):
         t0 = time.perf_counter()
-        ref, _ = oracle_rt.trace_closest(nodes, tris, orig, sample, threads=threads)
+        fn()
         if i:
             times.append(time.perf_counter() - t0)
-    med = float(np.median(times))
-    mism = int(np.sum(ref["prim_id"] != gpu_hits["prim_id"][:n]))
-    tbits = int(np.sum(ref["t"].view(np.int32) != gpu_hits["t"].view(np.int32)[:n]))
-    return {
-        "value": round(n / med / 1e6, 4),
+    return n / float(np.median(times))
+
+
+def _hit_bits(h):
+    return np.stack([h["t"].view(np.int32), h["u"].view(np.int32), h["v"].view(np.int32), h["prim_id"]], axis=1)
+
+
+def _rcpu_vs_rgpu(O, nodes, tris, orig, rays, threads):
+    """R-CPU (BVH::GetIntersection, BVH.cpp:103-174) against R-GPU
+    (HitClosest, bvh.hsh:191-273) on the same tree and rays: rays whose slot
+    differs, split into equal engine t (a tie the two child orders resolve
+    differently) and different t (R-CPU accepts hits below tmin and uses the
+    reciprocal slab test)."""
+    _, sg, tg = O.trace_closest(nodes, tris, orig, rays, raw_t=True, threads=threads)
+    _, sc, tc = O.trace_closest(nodes, tris, orig, rays, mode=O.MODE_RCPU, raw_t=True, threads=threads)
+    diff = sg != sc
+    same_t = diff & (tg.view(np.int32) == tc.view(np.int32))
+    return {"rays": int(len(rays)), "disagreements": int(diff.sum()), "at_equal_t": int(same_t.sum()),
+            "at_other_t": int((diff & ~same_t).sum())}
+
+
+def cpu_baseline(nodes, tris, orig, rays, gpu_hits, gpu_slots, budget_s):
+    """The oracle's restatement of the engine traversal on the host cores,
+    on a bounded sample of the c0 rays: R-GPU on every usable core (the
+    headline, median of 5 after a warm-up), R-GPU on one core, and R-CPU on
+    every core.  Parity of the GPU hits against R-GPU over all rays."""
+    O = _oracle()
+    host = host_cpu()
+    threads = host["threads"]
+    probe = rays[: min(len(rays), 32768)]
+    t0 = time.perf_counter()
+    O.trace_closest(nodes, tris, orig, probe, threads=threads)
+    rate = len(probe) / max(time.perf_counter() - t0, 1e-6)
+    n = int(min(len(rays), max(65536, rate * budget_s / 12.0)))
+    sample = rays[:n]
+    value = _median_rate(lambda: O.trace_closest(nodes, tris, orig, sample, threads=threads), n) / 1e6
+    rcpu = _median_rate(lambda: O.trace_closest(nodes, tris, orig, sample, mode=O.MODE_RCPU, threads=threads),
+                        n) / 1e6
+    t0 = time.perf_counter()
+    O.trace_closest(nodes, tris, orig, probe, threads=1)
+    rate1 = len(probe) / max(time.perf_counter() - t0, 1e-6)
+    n1 = int(min(len(rays), max(16384, rate1 * budget_s / 12.0)))
+    one = _median_rate(lambda: O.trace_closest(nodes, tris, orig, rays[:n1], threads=1), n1, reps=3) / 1e6
+    ref, sref, ties = O.trace_closest(nodes, tris, orig, rays, ties=True, threads=threads)
+    mism = int(np.sum(np.any(_hit_bits(ref) != _hit_bits(gpu_hits), axis=1) | (sref != gpu_slots)))
+    cb = {
+        "value": round(value, 4),
         "unit": "Mrays/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"first {n} of the {len(rays)} c0 rays, R-GPU traversal restated in C++ "
-                  f"(-O3, no FMA contraction), {threads} threads, median of 5 after 1 warm-up",
-    }, {"checked_rays": n, "prim_id_mismatches": mism, "t_bit_mismatches": tbits}
+        "sample": f"first {n} of the {len(rays)} c0 rays, R-GPU traversal (bvh.hsh:191-273) restated in C++ "
+                  f"(-O3, no FMA contraction), one std::thread per usable host core ({threads}), median of 5 "
+                  f"after 1 warm-up",
+        "host_cpu": host,
+        "one_core_mrays_per_s": round(one, 4),
+        "one_core_sample": f"first {n1} rays, 1 thread, median of 3 after 1 warm-up",
+        "rcpu_mrays_per_s": round(rcpu, 4),
+        "rcpu_note": "R-CPU semantics (BVH::GetIntersection, BVH.cpp:103-174: reciprocal slab, distance-culled "
+                     "stack) on the same sample and cores",
+    }
+    parity = {"checked_rays": int(len(rays)), "mismatches": mism, "equal_t_ties": int(ties.sum()),
+              "rcpu_vs_rgpu": _rcpu_vs_rgpu(O, nodes, tris, orig, rays, threads)}
+    return cb, parity
+
+
+def cpu_configs(extra, threads):
+    """CPU leg of the secondary configurations: each GPU result checked
+    against the oracle (hit records and slots bit for bit, equal-t ties
+    counted) and the oracle's traversal timed on the host cores (one run)."""
+    O = _oracle()
+    out = {}
+    c = extra.get("c1", {}).pop("_check", None)
+    if c:
+        t0 = time.perf_counter()
+        ref, sref, ties = O.trace_closest(c["nodes"], c["tris"], c["orig"], c["primary"], ties=True, threads=threads)
+        t_p = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        occ = O.trace_any(c["nodes"], c["tris"], c["shadow"], threads=threads)
+        t_s = time.perf_counter() - t0
+        n_p, n_s = len(c["primary"]), len(c["shadow"])
+        out["c1"] = {
+            "cpu_mrays_per_s": round((n_p + n_s) / (t_p + t_s) / 1e6, 3), "cores": threads,
+            "primary_mismatches": int(np.sum(np.any(_hit_bits(ref) != _hit_bits(c["hits"]), axis=1) |
+                                             (sref != c["slots"]))),
+            "shadow_mismatches": int(np.sum(occ != c["occluded"])),
+            "checked_primary": n_p, "checked_shadow": n_s, "equal_t_ties": int(ties.sum()),
+            "rcpu_vs_rgpu": _rcpu_vs_rgpu(O, c["nodes"], c["tris"], c["orig"], c["primary"], threads),
+        }
+    c = extra.get("c2", {}).pop("_check", None)
+    if c:
+        t0 = time.perf_counter()
+        ref, _, inst, ties = O.trace_world(c["tlas"], c["instances"], c["blases"], c["primary"], ties=True,
+                                           threads=threads)
+        t_w = time.perf_counter() - t0
+        order = c["order"]
+        expect_inst = np.where(inst >= 0, order[np.maximum(inst, 0)].astype(np.int32), -1)
+        out["c2"] = {
+            "cpu_mrays_per_s": round(len(c["primary"]) / t_w / 1e6, 3), "cores": threads,
+            "workload": "bounce-0 camera rays (one per pixel centre) through TLAS + instances + BLAS",
+            "mismatches": int(np.sum(np.any(_hit_bits(ref) != _hit_bits(c["hits"]), axis=1) |
+                                     (expect_inst != c["instance"]))),
+            "checked_rays": int(len(c["primary"])), "equal_t_ties": int(ties.sum()),
+            "rcpu_vs_rgpu": "none: the engine's CPU traversal (BVH::GetIntersection) is single-level",
+        }
+    c = extra.get("c3", {}).pop("_check", None)
+    if c:
+        mism = ties = nr = 0
+        t_all = 0.0
+        agg = {"rays": 0, "disagreements": 0, "at_equal_t": 0, "at_other_t": 0}
+        for (nodes, tris, orig), rays, hits in c["batches"]:
+            t0 = time.perf_counter()
+            ref, sref, tie = O.trace_closest(nodes, tris, orig, rays, ties=True, threads=threads)
+            t_all += time.perf_counter() - t0
+            mism += int(np.sum(np.any(_hit_bits(ref) != _hit_bits(hits), axis=1)))
+            ties += int(tie.sum())
+            nr += len(rays)
+            for k, v in _rcpu_vs_rgpu(O, nodes, tris, orig, rays, threads).items():
+                agg[k] += v
+        out["c3"] = {
+            "cpu_mrays_per_s": round(nr / t_all / 1e6, 3), "cores": threads,
+            "sample": f"the first {len(c['batches'])} of the 512 (BLAS, ray batch) pairs, {nr} rays",
+            "mismatches": mism, "checked_rays": nr, "equal_t_ties": ties, "rcpu_vs_rgpu": agg,
+        }
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Fixed profiler workloads (bench.py --probe ...), for rocprofv3 runs:
+#   rocprofv3 --kernel-trace --stats -d out -o c0 -- python3 bench.py --probe pmc 66
+#   rocprofv3 --pmc FETCH_SIZE -d out/fetch -o c0 -- python3 bench.py --probe pmc 66
+#   python3 bench.py --probe summarize-pmc out
+# ---------------------------------------------------------------------------
+PROBES = """probes:
+  pt plain|cache [stride steps [lag]]
+                  the dense radiance-cache scene (256 blob instances in a
+                  30-unit cube) at 1280 x 720, 8 bounces: 4 untimed frames,
+                  then 8 frames, plain or with the cache (1/32 training
+                  pixels, 2 Adam steps per frame, query lag 0, unless given)
+  nrc             cache training steps at 32 .. 262,144 samples and
+                  inference of 65,536 and 1,048,576 queries
+  pmc [kernel]    a 256 MiB device-to-device copy (the FETCH_SIZE /
+                  WRITE_SIZE calibration) and the c0 closest-hit trace, 4 each
+  c1 [kernel]     the c1 primary rays (kernel, default 66) and shadow rays, 4 each
+  c3 [kernel]     the c3 batched trace with one launch configuration
+                  (default: the engine-exact persistent leaf-batched kernel), 4 times
+  lbvh            three GPU linear-BVH builds of the c3 meshes (512 meshes,
+                  ~7.9M triangles)
+  c2 [binned]     the config-2 path trace (256 instances, 1280 x 720,
+                  4 bounces): 2 untimed frames, then 16, optionally with
+                  ray binning
+  (pt and c2 take fif=N anywhere after the command: N progressive frames
+  in flight instead of the library's default)
+  realtime        the same scene in realtime mode (4 samples per pixel,
+                  G-buffer, temporal resolve): 2 untimed frames, then 16
+  summarize-pmc DIR   per-dispatch counter averages of a --pmc run's CSVs"""
+
+COPY_BYTES = 256 << 20
+
+
+def _fif(argv):
+    for a in argv:
+        if a.startswith("fif="):
+            return int(a[4:])
+    return None
+
+
+def probe_pt(mode, stride=32, steps=2, lag=0, fif=None):
+    import torch
+
+    from atlas_rt import PathTracer, RadianceCache, camera
+    from atlas_rt import workloads as W
+    from atlas_rt.bench_suite import _world
+    meshes, mats_d, xf, which = W.c4_scene()
+    w, blases = _world(meshes, xf, which)
+    cc = W.c4_camera()
+    cam = camera(cc["eye"], cc["yaw_deg"], cc["pitch_deg"], cc["fov_y_deg"], cc["width"] / cc["height"])
+    p = PathTracer(w, cc["width"], cc["height"], 8, W.c2_materials(mats_d), W.sun_light(), env=W.C2_ENV,
+                   binning=(mode == "binned"), frames_in_flight=fif)
+    if mode == "cache":
+        rc = RadianceCache((-20, -20, -20), (20, 20, 20), levels=12, log2_table=19, base_resolution=16,
+                           max_resolution=2048)
+        p.set_cache(rc, train_stride=stride, train_steps=steps, query_lag=lag)
+    p.render(cam, 4)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    p.render(cam, 8)
+    torch.cuda.synchronize()
+    lag_s = f" lag={lag}" if mode == "cache" else ""
+    print(f"pt {mode} fif={fif}{lag_s}: {(time.perf_counter() - t) / 8 * 1e3:.3f} ms/frame", flush=True)
+
+
+def probe_c2(binning, fif=None):
+    import torch
+
+    torch.cuda.init()
+    from atlas_rt import PathTracer, camera
+    from atlas_rt import workloads as W
+    from atlas_rt.bench_suite import _world
+    meshes, mats_d, xf, which = W.c2_scene()
+    w, blases = _world(meshes, xf, which)
+    cc = W.c2_camera()
+    cam = camera(cc["eye"], cc["yaw_deg"], cc["pitch_deg"], cc["fov_y_deg"], cc["width"] / cc["height"])
+    p = PathTracer(w, cc["width"], cc["height"], 4, W.c2_materials(mats_d), W.sun_light(), env=W.C2_ENV,
+                   binning=binning, frames_in_flight=fif)
+    p.render(cam, 2)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    p.render(cam, 16)
+    torch.cuda.synchronize()
+    print(f"c2 binning={binning} fif={fif}: {(time.perf_counter() - t) / 16 * 1e3:.3f} ms/frame", flush=True)
+
+
+def probe_realtime():
+    import torch
+
+    torch.cuda.init()
+    from atlas_rt import PathTracer, camera, frame_matrices, projection_matrix, view_matrix
+    from atlas_rt import workloads as W
+    from atlas_rt.bench_suite import _world
+    meshes, mats_d, xf, which = W.c2_scene()
+    w, blases = _world(meshes, xf, which)
+    cc = W.c2_camera()
+    cam = camera(cc["eye"], cc["yaw_deg"], cc["pitch_deg"], cc["fov_y_deg"], cc["width"] / cc["height"])
+    p = PathTracer(w, cc["width"], cc["height"], 4, W.c2_materials(mats_d), W.sun_light(), env=W.C2_ENV)
+    p.set_realtime(samples_per_frame=4)
+    view = view_matrix(cc["eye"], cc["yaw_deg"], cc["pitch_deg"])
+    proj = projection_matrix(cc["fov_y_deg"], cc["width"] / cc["height"])
+    fm = frame_matrices(view, proj, proj @ view)
+    for _ in range(2):
+        p.render_realtime(cam, fm)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(16):
+        p.render_realtime(cam, fm)
+    torch.cuda.synchronize()
+    print(f"c2 realtime: {(time.perf_counter() - t) / 16 * 1e3:.3f} ms/frame", flush=True)
+
+
+def probe_nrc():
+    import torch
+
+    from atlas_rt import RadianceCache
+    g = np.random.default_rng(0)
+
+    def q(n):
+        a = np.zeros((n, 12), np.float32)
+        a[:, :3] = g.uniform(-20, 20, (n, 3))
+        a[:, 3] = g.uniform(0, 1, n)
+        for k in (4, 8):
+            d = g.normal(size=(n, 3))
+            a[:, k:k + 3] = d / np.linalg.norm(d, axis=1, keepdims=True)
+        return torch.from_numpy(a).cuda()
+
+    rc = RadianceCache((-20, -20, -20), (20, 20, 20), levels=12, log2_table=19, base_resolution=16,
+                       max_resolution=2048)
+    for n in (32, 1024, 8192, 21504, 65536, 262144):
+        Q, T = q(n), torch.rand((n, 4), device="cuda")
+        rc.train(Q, T)
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        for _ in range(10):
+            rc.train(Q, T)
+        e.record()
+        torch.cuda.synchronize()
+        print(f"train step n={n}: {s.elapsed_time(e) / 10 * 1e3:.1f} us", flush=True)
+    for n in (65536, 1 << 20):
+        Q = q(n)
+        out = rc.infer(Q)
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        for _ in range(10):
+            rc.infer(Q, out)
+        e.record()
+        torch.cuda.synchronize()
+        t = s.elapsed_time(e) / 10
+        print(f"infer n={n}: {t * 1e3:.1f} us, {n / t / 1e6:.2f} Gq/s", flush=True)
+
+
+def probe_pmc(kernel=2, reps=4):
+    import torch
+
+    import atlas_rt
+    from atlas_rt import workloads as W
+    src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    verts, rays = W.c0()
+    blas = atlas_rt.Blas.build(verts)
+    d_rays = torch.from_numpy(rays.view(np.float32).reshape(-1, 8).copy()).cuda()
+    d_hits = torch.empty((len(rays), 4), dtype=torch.float32, device="cuda")
+    for _ in range(reps):
+        blas.trace_closest(d_rays, d_hits, kernel=kernel)
+    torch.cuda.synchronize()
+    print("pmc probe done", flush=True)
+
+
+def probe_c1(kernel=66, reps=4):
+    import torch
+
+    import atlas_rt
+    from atlas_rt import workloads as W
+    src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    verts, prim = W.c1()
+    blas = atlas_rt.Blas.build(verts)
+    d_prim = torch.from_numpy(prim.view(np.float32).reshape(-1, 8).copy()).cuda()
+    d_hits = torch.empty((len(prim), 4), dtype=torch.float32, device="cuda")
+    for _ in range(reps):
+        blas.trace_closest(d_prim, d_hits, kernel=kernel)
+    torch.cuda.synchronize()
+    hits = d_hits.cpu().numpy().view(atlas_rt.HIT_DTYPE).reshape(-1)
+    shadow = np.ascontiguousarray(W.shadow_rays(prim, hits, verts)[hits["prim_id"] >= 0])
+    d_sh = torch.from_numpy(shadow.view(np.float32).reshape(-1, 8).copy()).cuda()
+    d_occ = torch.empty(len(shadow), dtype=torch.int32, device="cuda")
+    for _ in range(reps):
+        blas.trace_any(d_sh, d_occ)
+    torch.cuda.synchronize()
+    print(f"c1 probe done: {len(prim)} primary, {len(shadow)} shadow rays", flush=True)
+
+
+def probe_c3(kernel=66, block=256, refill=32, reps=4):
+    import torch
+
+    import atlas_rt
+    from atlas_rt import workloads as W
+    src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    meshes, rays, table = W.c3()
+    blases = atlas_rt.build_many(meshes)
+    bs = atlas_rt.BatchSet(blases, [(b, o, c, 0) for b, o, c in table])
+    d_rays = torch.from_numpy(rays.view(np.float32).reshape(-1, 8).copy()).cuda()
+    d_hits = torch.empty((len(rays), 4), dtype=torch.float32, device="cuda")
+    for _ in range(reps):
+        bs.trace_closest(d_rays, d_hits, kernel=kernel, block=block, refill=refill)
+    torch.cuda.synchronize()
+    print(f"c3 probe done: {len(rays)} rays, kernel {kernel} block {block}", flush=True)
+
+
+def summarize_pmc(out_dir, kernel_re=r"k_blas_closest|k_batch"):
+    """Per-dispatch counter averages of the copy and the trace kernels, and
+    the trace kernel's traffic scaled by the copy's known bytes."""
+    import csv
+    import glob
+    import re
+    acc = {}
+    for path in glob.glob(os.path.join(out_dir, "**", "*counter_collection.csv"), recursive=True):
+        with open(path) as f:
+            for row in csv.DictReader(f):
+                name = row.get("Kernel_Name", "")
+                kind = "copy" if re.search(r"copy", name, re.I) else ("trace" if re.search(kernel_re, name) else None)
+                if kind is None:
+                    continue
+                key = (kind, row.get("Counter_Name", ""))
+                d = acc.setdefault(key, {})
+                disp = row.get("Dispatch_Id", str(len(d)))
+                d[disp] = d.get(disp, 0.0) + float(row.get("Counter_Value", 0) or 0)
+    out = {}
+    for (kind, counter), d in acc.items():
+        out[f"{kind}:{counter}_kib_per_dispatch"] = sum(d.values()) / max(1, len(d))
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        c, t = out.get(f"copy:{counter}_kib_per_dispatch"), out.get(f"trace:{counter}_kib_per_dispatch")
+        if c and t:
+            scale = (COPY_BYTES / 1024.0) / c
+            out[f"{counter}_scale"] = scale
+            out[f"trace_{counter}_bytes"] = t * scale * 1024.0
+    return out
+
+
+def probe_lbvh(reps=3):
+    import torch
+
+    torch.cuda.init()
+    from atlas_rt import build_many_device
+    from atlas_rt import workloads as W
+    meshes, _, _ = W.c3()
+    for r in range(reps):
+        t = time.perf_counter()
+        blases, ms = build_many_device(meshes)
+        wall = time.perf_counter() - t
+        print(f"lbvh build {r}: {ms:.2f} ms GPU, {wall * 1e3:.1f} ms wall", flush=True)
+        del blases
+
+
+def probe(argv):
+    cmd, rest = argv[0], argv[1:]
+    fif = _fif(rest)
+    args = [a for a in rest if not a.startswith("fif=")]
+    if cmd == "pt":
+        probe_pt(args[0] if args else "plain", *[int(a) for a in args[1:4]], fif=fif)
+    elif cmd == "c2":
+        probe_c2("binned" in args, fif=fif)
+    elif cmd == "realtime":
+        probe_realtime()
+    elif cmd == "nrc":
+        probe_nrc()
+    elif cmd == "lbvh":
+        probe_lbvh()
+    elif cmd == "pmc":
+        probe_pmc(int(args[0]) if args else 2)
+    elif cmd == "c1":
+        probe_c1(int(args[0]) if args else 66)
+    elif cmd == "c3":
+        probe_c3(*[int(a) for a in args[:3]])
+    elif cmd == "summarize-pmc":
+        print(json.dumps(summarize_pmc(args[0], *args[1:2]), indent=1))
+    else:
+        print(PROBES)
+        sys.exit(2)
 
 
 def main():
     args = parse()
+    if args.probe:
+        return probe(args.probe)
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU sharding is design-only "
                                    "(rays and BLAS batches are independent), run with --gpus 1"}))
@@ -210,7 +623,11 @@ def main():
     ms_per_step = wall * 1000.0 / args.steps
     value = n / (ms_per_step / 1000.0) / 1e6
 
+    d_slots = torch.empty(n, dtype=torch.int32, device="cuda")
+    blas.trace_closest(d_rays, d_hits, d_slots, kernel=kernel, stream=stream)
+    torch.cuda.synchronize()
     hits = d_hits.cpu().numpy().view(atlas_rt.HIT_DTYPE).reshape(-1)
+    slots = d_slots.cpu().numpy()
     achieved = alg_bytes / (kernel_ms / 1000.0) / 1e9
     out = {
         "metric": METRIC,
@@ -258,19 +675,27 @@ def main():
         out["roofline"]["l2_hit_rate"] = pmc["l2_hit_rate"]
         out["roofline"]["valu_busy"] = pmc["valu_busy"]
     if not args.no_cpu_baseline:
-        cb, parity = cpu_baseline(nodes, tris, orig, rays, hits, args.cpu_seconds)
+        cb, parity = cpu_baseline(nodes, tris, orig, rays, hits, slots, args.cpu_seconds)
         out["cpu_baseline"] = cb
         out["parity"] = parity
         out["gpu_vs_cpu"] = round(value / cb["value"], 2)
+        out["gpu_vs_cpu_one_core"] = round(value / cb["one_core_mrays_per_s"], 1)
+        log(f"c0 cpu: {cb['value']:.2f} Mrays/s on {cb['cores']} threads, {cb['one_core_mrays_per_s']:.3f} on one; "
+            f"parity mismatches {parity['mismatches']}, ties {parity['equal_t_ties']}")
     log(f"c0 done: {value:.1f} Mrays/s")
     if not args.no_extra:
         from atlas_rt import bench_suite
-        del d_rays, d_hits, d_vis, blas
+        del d_rays, d_hits, d_vis, d_slots, blas
         torch.cuda.empty_cache()
         out["c1"] = bench_suite.c1(torch, log)
         out["c2"] = bench_suite.c2(torch, log)
         out["tlas_update"] = bench_suite.tlas_update(torch, log)
         out["c3"] = bench_suite.c3(torch, log)
+        if not args.no_cpu_baseline:
+            out["cpu_configs"] = cpu_configs(out, host_cpu()["threads"])
+            log(f"cpu configs: {json.dumps(out['cpu_configs'])}")
+        for k in ("c1", "c2", "c3"):
+            out[k].pop("_check", None)
         out["radiance_cache"] = bench_suite.cache_study(torch, log, args.ref_spp)
     print(json.dumps(out), flush=True)
 

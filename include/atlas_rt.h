@@ -260,6 +260,21 @@ atlas_rt_status atlas_rt_trace_closest_host(const atlas_rt_blas* blas, const atl
 atlas_rt_status atlas_rt_trace_any_host(const atlas_rt_blas* blas, const atlas_rt_ray* rays,
                                         uint32_t* hit, uint32_t count);
 
+/* Traversal stack bound.  By default (entries = 0) every trace of the BLAS
+ * sizes its stack from the tree's depth (LDS rows, then a global spill), so
+ * no ray ever runs out of stack.  entries > 0 caps the stack at that many
+ * entries, as the engine's fixed 32-entry stack does (bvh.hsh:16, which has
+ * no overflow check): a ray that would push past the cap stops, is reported
+ * as a miss (prim_id -1, t = tmax, slot -1), and raises the BLAS's overflow
+ * flag.  Set it while no trace of the BLAS is in flight. */
+atlas_rt_status atlas_rt_blas_set_stack_limit(atlas_rt_blas* blas, uint32_t entries);
+
+/* Waits for `stream`, then reads and clears the BLAS's overflow flag:
+ * ATLAS_RT_E_STACK_OVERFLOW if a trace of this BLAS since the last call
+ * stopped a ray at the stack limit, ATLAS_RT_OK otherwise.  The engine has
+ * no counterpart: its stack overflow goes unnoticed. */
+atlas_rt_status atlas_rt_blas_trace_status(atlas_rt_blas* blas, void* stream);
+
 /* Per-launch timing of the last trace call on this BLAS (device time between
  * HIP events recorded around the kernel on its own stream), milliseconds. */
 atlas_rt_status atlas_rt_last_kernel_ms(const atlas_rt_blas* blas, float* ms);
@@ -399,6 +414,13 @@ atlas_rt_status atlas_rt_world_update(atlas_rt_world* world, const atlas_rt_inst
  * after atlas_rt_pathtracer_set_emissive_lights.  Stream-ordered: traces on
  * other streams must not overlap it. */
 atlas_rt_status atlas_rt_world_refit(atlas_rt_world* world, const atlas_rt_instance_desc* d_instances, void* stream);
+
+/* The traversal stack bound and overflow flag of a world's traces (TLAS and
+ * BLAS levels share one stack, as in bvh.hsh), with the meaning of
+ * atlas_rt_blas_set_stack_limit / atlas_rt_blas_trace_status.  The path
+ * tracer's traces follow the limit of its world. */
+atlas_rt_status atlas_rt_world_set_stack_limit(atlas_rt_world* world, uint32_t entries);
+atlas_rt_status atlas_rt_world_trace_status(atlas_rt_world* world, void* stream);
 
 atlas_rt_status atlas_rt_world_info(const atlas_rt_world* world, uint32_t* tlas_node_count,
                                     uint32_t* instance_count, uint32_t* max_depth);
@@ -698,6 +720,11 @@ atlas_rt_status atlas_rt_pathtracer_reset(atlas_rt_pathtracer* pt);
  * width*height float4; d_out is a device buffer.  Synchronous on `stream`. */
 atlas_rt_status atlas_rt_pathtracer_read(atlas_rt_pathtracer* pt, float* d_out, void* stream);
 atlas_rt_status atlas_rt_pathtracer_sample_count(const atlas_rt_pathtracer* pt, int32_t* count);
+/* Waits for `stream` and every frame in flight, then reads and clears the
+ * path tracer's overflow flags: ATLAS_RT_E_STACK_OVERFLOW if a frame since
+ * the last call stopped a path at its world's stack limit
+ * (atlas_rt_world_set_stack_limit; such a path ends as a miss). */
+atlas_rt_status atlas_rt_pathtracer_status(atlas_rt_pathtracer* pt, void* stream);
 
 /* ------------------------------------------------------------------------- */
 /* Radiance cache (no engine counterpart)                                    */

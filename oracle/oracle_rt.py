@@ -61,12 +61,12 @@ def lib():
         L.oracle_bvh_depth.restype = u32
         L.oracle_bvh_depth.argtypes = [vp, u32]
         L.oracle_trace_closest.restype = None
-        L.oracle_trace_closest.argtypes = [vp, u32, vp, vp, u32, vp, u32, vp, vp, vp, vp, vp, i32, i32]
+        L.oracle_trace_closest.argtypes = [vp, u32, vp, vp, u32, vp, u32, vp, vp, vp, vp, vp, vp, i32, i32]
         L.oracle_trace_any.restype = None
         L.oracle_trace_any.argtypes = [vp, u32, vp, u32, vp, u32, vp, vp, vp, i32, i32]
         L.oracle_trace_world.restype = None
         L.oracle_trace_world.argtypes = [vp, u32, vp, u32, vp, vp, vp, vp, vp, u32, vp, u32, u32, i32,
-                                         vp, vp, vp, vp, vp, vp, vp, i32]
+                                         vp, vp, vp, vp, vp, vp, vp, vp, i32]
         L.oracle_triangle_test.restype = i32
         L.oracle_triangle_test.argtypes = [vp, vp, vp, vp, vp, vp]
         L.oracle_slab_gpu.restype = i32
@@ -135,12 +135,15 @@ def build_aabbs(aabbs):
     return OracleBVH(nodes, None, ref, eon)
 
 
-def trace_closest(bvh_nodes, bvh_tris, orig_idx, rays, mode=MODE_RGPU, threads=None, stats=False, raw_t=False):
+def trace_closest(bvh_nodes, bvh_tris, orig_idx, rays, mode=MODE_RGPU, threads=None, stats=False, raw_t=False,
+                  ties=False):
     """Closest hit on the packed arrays.
 
-    Returns (hits, slots[, raw_t][, node_visits, tri_tests]).  hits carry the
-    refined (t, u, v) of the selected triangle; raw_t is the engine's float32
-    hit distance that drove the traversal.
+    Returns (hits, slots[, raw_t][, node_visits, tri_tests][, ties]).  hits
+    carry the refined (t, u, v) of the selected triangle; raw_t is the
+    engine's float32 hit distance that drove the traversal; ties (R-GPU mode)
+    flags rays whose hit had a rival triangle at exactly the same float32 t,
+    so the traversal order chose the winner.
     """
     rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
     bvh_nodes = np.ascontiguousarray(bvh_nodes, dtype=NODE_DTYPE)
@@ -151,15 +154,18 @@ def trace_closest(bvh_nodes, bvh_tris, orig_idx, rays, mode=MODE_RGPU, threads=N
     rt = np.zeros(n, dtype=np.float32) if raw_t else None
     nv = np.zeros(n, dtype=np.uint32) if stats else None
     tt = np.zeros(n, dtype=np.uint32) if stats else None
+    tie = np.zeros(n, dtype=np.uint8) if ties else None
     orig = None if orig_idx is None else np.ascontiguousarray(orig_idx, dtype=np.uint32)
     lib().oracle_trace_closest(_p(bvh_nodes) if len(bvh_nodes) else None, len(bvh_nodes), _p(bvh_tris), _p(orig),
-                               len(bvh_tris), _p(rays), n, _p(hits), _p(slots), _p(rt), _p(nv), _p(tt), mode,
-                               threads or threads_default())
+                               len(bvh_tris), _p(rays), n, _p(hits), _p(slots), _p(rt), _p(nv), _p(tt), _p(tie),
+                               mode, threads or threads_default())
     out = (hits, slots)
     if raw_t:
         out += (rt,)
     if stats:
         out += (nv, tt)
+    if ties:
+        out += (tie.astype(bool),)
     return out
 
 
@@ -180,12 +186,12 @@ WORLD_CLOSEST, WORLD_ANY, WORLD_BRUTE, WORLD_ANY_RESTORE = 0, 1, 2, 3
 
 
 def trace_world(tlas_nodes, instances, blases, rays, mask=1 << 7, any_hit=False, threads=None, stats=False,
-                mode=None, raw_t=False):
+                mode=None, raw_t=False, ties=False):
     """Two-level trace.  blases: list of (nodes, tris, orig_idx) packed arrays.
 
     mode: WORLD_CLOSEST (R-GPU HitClosest), WORLD_ANY (HitAny; hits.prim_id =
     1/0) or WORLD_BRUTE (every instance x triangle).  Returns (hits, slots,
-    instance slots[, raw_t][, node_visits, tri_tests, instance_visits]).
+    instance slots[, raw_t][, node_visits, tri_tests, instance_visits][, ties]).
     """
     if mode is None:
         mode = WORLD_ANY if any_hit else WORLD_CLOSEST
@@ -217,16 +223,19 @@ def trace_world(tlas_nodes, instances, blases, rays, mask=1 << 7, any_hit=False,
     nv = np.zeros(n, dtype=np.uint32) if stats else None
     tt = np.zeros(n, dtype=np.uint32) if stats else None
     iv = np.zeros(n, dtype=np.uint32) if stats else None
+    tie = np.zeros(n, dtype=np.uint8) if ties else None
     lib().oracle_trace_world(_p(tlas_nodes), len(tlas_nodes), _p(instances), len(instances),
                              C.cast(node_ptrs, C.c_void_p), _p(node_counts), C.cast(tri_ptrs, C.c_void_p),
                              C.cast(orig_ptrs, C.c_void_p), _p(tri_counts), k, _p(rays), n, mask,
-                             mode, _p(hits), _p(slots), _p(inst), _p(rt), _p(nv), _p(tt), _p(iv),
+                             mode, _p(hits), _p(slots), _p(inst), _p(rt), _p(nv), _p(tt), _p(iv), _p(tie),
                              threads or threads_default())
     out = (hits, slots, inst)
     if raw_t:
         out += (rt,)
     if stats:
         out += (nv, tt, iv)
+    if ties:
+        out += (tie.astype(bool),)
     return out
 
 

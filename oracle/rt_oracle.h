@@ -31,6 +31,12 @@ struct TraceStats {
     uint32_t node_visits = 0;  // inner nodes fetched (BLAS + TLAS)
     uint32_t tri_tests = 0;    // triangles intersected
     uint32_t inst_visits = 0;  // instances entered (CheckInstance calls)
+    // Equal-t ties: a triangle that passed every accept test except the
+    // strict t < hitDistance because its t equals the current best (so the
+    // traversal order picked the winner).  tie_t is the distance of the last
+    // such event; the ray's final hit is a tie iff tie && tie_t == hit t.
+    bool tie = false;
+    float tie_t = 0.0f;
 };
 
 struct HitRec {

@@ -82,6 +82,10 @@ static void closest_rgpu_blas(const BlasView& b, V3 o, V3 d, float tmin, float t
                 V3 sol;
                 st.tri_tests++;
                 bool hit = tri_test(o, d, tri_v(tri, 0), tri_v(tri, 1), tri_v(tri, 2), sol);
+                if (hit && sol.x > tmin && h.slot >= 0 && sol.x == h.t) {
+                    st.tie = true;
+                    st.tie_t = sol.x;
+                }
                 if (hit && sol.x > tmin && sol.x < leafTmax) {
                     if (sol.x < h.t) {
                         h.t = sol.x;
@@ -228,6 +232,11 @@ void closest_rgpu_world(const World& w, V3 o0, V3 d0, float tmin, float tmax, ui
                     V3 sol;
                     st.tri_tests++;
                     bool hit = tri_test(o, d, tri_v(tri, 0), tri_v(tri, 1), tri_v(tri, 2), sol);
+                    if (hit && sol.x > tmin && h.slot >= 0 && sol.x == h.t &&
+                        (!op || op->fn(op->ctx, curInstance, triPtr, sol.y, sol.z) > 0.0f)) {
+                        st.tie = true;
+                        st.tie_t = sol.x;
+                    }
                     if (hit && sol.x > tmin && sol.x < leafTmax && sol.x < h.t &&
                         (!op || op->fn(op->ctx, curInstance, triPtr, sol.y, sol.z) > 0.0f)) {
                         h.t = sol.x;
@@ -666,7 +675,8 @@ extern "C" {
 void oracle_trace_closest(const atlas_rt_gpu_bvh_node* nodes, uint32_t node_count,
                           const atlas_rt_gpu_bvh_triangle* tris, const uint32_t* orig_idx, uint32_t tri_count,
                           const atlas_rt_ray* rays, uint32_t n, atlas_rt_hit* hits, int32_t* slots, float* raw_t,
-                          uint32_t* node_visits, uint32_t* tri_tests, int32_t mode, int32_t threads) {
+                          uint32_t* node_visits, uint32_t* tri_tests, uint8_t* ties, int32_t mode,
+                          int32_t threads) {
     BlasView b{nodes, node_count, tris, tri_count};
     parallel_rays(n, threads, [&](uint32_t lo, uint32_t hi) {
         std::vector<int32_t> stack;
@@ -688,6 +698,7 @@ void oracle_trace_closest(const atlas_rt_gpu_bvh_node* nodes, uint32_t node_coun
             write_hit(h, tris, orig_idx, r, hits, slots, raw_t, i);
             if (node_visits) node_visits[i] = st.node_visits;
             if (tri_tests) tri_tests[i] = st.tri_tests;
+            if (ties) ties[i] = (h.slot >= 0 && st.tie && st.tie_t == h.t) ? 1 : 0;
         }
     });
 }
@@ -731,7 +742,7 @@ void oracle_trace_world(const atlas_rt_gpu_bvh_node* tlas, uint32_t tlas_count,
                         const uint32_t* blas_tri_counts, uint32_t blas_count, const atlas_rt_ray* rays, uint32_t n,
                         uint32_t mask, int32_t mode, atlas_rt_hit* hits, int32_t* slots, int32_t* instances,
                         float* raw_t, uint32_t* node_visits, uint32_t* tri_tests, uint32_t* inst_visits,
-                        int32_t threads) {
+                        uint8_t* ties, int32_t threads) {
     const int32_t any = mode == 1 || mode == 3;
     std::vector<BlasView> views(blas_count);
     for (uint32_t k = 0; k < blas_count; k++)
@@ -771,6 +782,7 @@ void oracle_trace_world(const atlas_rt_gpu_bvh_node* tlas, uint32_t tlas_count,
                 if (slots) slots[i] = h.slot;
                 if (instances) instances[i] = h.instance;
                 if (raw_t) raw_t[i] = h.t;
+                if (ties) ties[i] = (h.slot >= 0 && st.tie && st.tie_t == h.t) ? 1 : 0;
             }
             if (node_visits) node_visits[i] = st.node_visits;
             if (tri_tests) tri_tests[i] = st.tri_tests;

@@ -315,6 +315,95 @@ def test_rcpu_vs_rgpu_disagreements_are_tmin_or_ties():
     assert np.all(tc1[d] <= 1e-4)
 
 
+def _edge_aimed_rays(verts, n, salt):
+    """Rays from above aimed at the midpoints of triangle edges, where two
+    triangles of a grid mesh meet and may report the same float32 t."""
+    g = np.random.default_rng(salt)
+    tri = verts.reshape(-1, 3, 3).astype(np.float64)
+    pick = g.integers(0, len(tri), n)
+    k = g.integers(0, 3, n)
+    a, b = tri[pick, k], tri[pick, (k + 1) % 3]
+    tgt = 0.5 * (a + b)
+    o = tgt + np.column_stack([g.uniform(-6, 6, n), g.uniform(15, 40, n), g.uniform(-6, 6, n)])
+    d = tgt - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return rt_cases.make_rays(o.astype(np.float32), d.astype(np.float32))
+
+
+def test_oracle_tie_flags_match_brute_force():
+    """The tie flag marks a hit whose float32 t another tested slot also
+    reaches (the traversal order then picks the slot).  Brute force over every
+    slot except the winner's finds such a rival independently of the tree:
+    every flagged ray has one, and a rival the flag misses is one that the
+    traversal dropped at a box boundary."""
+    verts = W.heightfield(grid=17, amplitude=4.0)
+    rays = _edge_aimed_rays(verts, 6000, 17)
+    ob = oracle_rt.build_triangles(verts)
+    h, s, rt, tie = oracle_rt.trace_closest(ob.nodes, ob.tris, ob.ref_idx, rays, raw_t=True, ties=True)
+    rival = np.zeros(len(rays), dtype=bool)
+    for w in np.unique(s[s >= 0]):
+        idx = np.nonzero(s == w)[0]
+        others = np.delete(ob.tris, w)
+        _, _, rtb = oracle_rt.trace_closest(ob.nodes, others, None, rays[idx], mode=oracle_rt.MODE_BRUTE, raw_t=True)
+        rival[idx] = rtb.view(np.int32) == rt[idx].view(np.int32)
+    assert tie.sum() >= 50, tie.sum()
+    assert np.all(rival[tie])
+    # Each unflagged rival lies on an edge the winner shares, i.e. on the face
+    # of its own leaf box, and the traversal never tests it: once the ray's
+    # distance is the hit t, some box on the rival's root-to-leaf path either
+    # fails the slab test or is entered at exactly t.  The latter drops the
+    # box too: a missed sibling's distance is the current t as well, so the
+    # engine's near-first rule (hitL <= hitR, bvh.hsh:235) walks into the
+    # missed sibling and pushes nothing.
+    parent = {}
+    for i, n in enumerate(ob.nodes):
+        parent[int(n["left_ptr"])] = (i, "left")
+        parent[int(n["right_ptr"])] = (i, "right")
+    eon = ob.tris["v0"][:, 3] > 0  # end-of-leaf flag
+    leaf_start = np.zeros(len(ob.tris), dtype=np.int64)
+    for k in range(1, len(ob.tris)):
+        leaf_start[k] = leaf_start[k - 1] if not eon[k - 1] else k
+
+    def culled(i, slot):
+        o, d, t = rays[i]["origin"], rays[i]["direction"], rt[i]
+        key = ~int(leaf_start[slot])
+        while key in parent:
+            node, side = parent[key]
+            n = ob.nodes[node]
+            ok, dist = oracle_rt.slab_test(o, d, n[side + "_min"], n[side + "_max"], rays[i]["tmin"], t)
+            if not ok or np.float32(dist) == t:
+                return True
+            key = node
+        return False
+
+    missed = np.nonzero(rival & ~tie)[0]
+    for i in missed:
+        o, d = rays[i]["origin"], rays[i]["direction"]
+        for slot, tr in enumerate(ob.tris):
+            if slot == s[i]:
+                continue
+            hit, sol = oracle_rt.triangle_test(o, d, tr["v0"][:3], tr["v1"][:3], tr["v2"][:3])
+            if hit and np.float32(sol[0]) == rt[i]:
+                assert culled(i, slot), (i, slot)
+    print(f"{tie.sum()} flagged ties, {len(missed)} rivals dropped at a box boundary")
+    # random rays through a triangle soup almost never tie
+    verts2, rays2 = rt_cases.soup(4096, 20000, 18)
+    ob2 = oracle_rt.build_triangles(verts2)
+    *_, tie2 = oracle_rt.trace_closest(ob2.nodes, ob2.tris, ob2.ref_idx, rays2, ties=True)
+    assert tie2.sum() == 0
+
+
+def test_oracle_tie_flags_on_duplicated_triangles():
+    """Every triangle present twice: each hit has a rival at the same t."""
+    verts, rays = rt_cases.soup(8192, 20000, 19)
+    dup = np.concatenate([verts, verts])
+    ob = oracle_rt.build_triangles(dup)
+    h, s, tie = oracle_rt.trace_closest(ob.nodes, ob.tris, ob.ref_idx, rays, ties=True)
+    hit = s >= 0
+    assert hit.sum() > 1000
+    assert np.array_equal(tie, hit)
+
+
 def test_chain_tree_oracle():
     nodes, tris, rays = rt_cases.chain_tree(80)
     h, _ = oracle_rt.trace_closest(nodes, tris, np.arange(len(tris), dtype=np.uint32), rays)

@@ -118,3 +118,40 @@ def test_gpu_pt_progressive_and_reset(torch_cuda):
     img3 = pt.read().cpu().numpy()
     close = np.all(np.abs(img3[..., :3] - ref1[..., :3]) <= 1e-4 + 1e-3 * np.abs(ref1[..., :3]), axis=-1)
     assert close.mean() > 0.97 and np.all(img3[..., 3] == 1)
+
+
+@pytest.mark.gpu
+def test_gpu_pt_renders_the_world_after_an_update(torch_cuda):
+    """RayTracingWorld::Update then PathTracingRenderer::Render: an update
+    replaces the world's device arrays (here with fewer BLASes and fewer
+    instances), and the next render traces the new world, per pixel against
+    the oracle's render of it."""
+    print("case pt after world update", flush=True)
+    meshes, normals, mats, xf, which = pt_cases.small_c2(salt=5)
+    S = pt_cases.oracle_scene(meshes, normals, mats, xf, which)
+    w_, blases = pt_cases.product_scene(meshes, normals, xf, which, S["boxes"])
+    W_, H_, bounces = 64, 48, 3
+    cam = _cam((0.0, 15.0, -70.0), 0.0, 12.0, W_, H_)
+    sun = W.sun_light()
+    pt = atlas_rt.PathTracer(w_, W_, H_, bounces, mats, sun, env=ENV)
+    pt.render(cam, 2)
+    before = pt.read().cpu().numpy()
+    g = W.rng(2, 905)
+    xf2 = W.random_affine(g, 30, scale=(1.0, 4.0), extent=30.0, height=8.0)
+    which2 = g.integers(0, 3, 30)
+    S2 = pt_cases.oracle_scene(meshes[:3], normals[:3], mats, xf2, which2)
+    descs2 = atlas_rt.instance_descs(xf2, which2)
+    descs2["material_offset"] = which2.astype(np.int32)
+    w_.update(descs2, blases[:3], world_aabbs=S2["boxes"])
+    tl, inst, _ = w_.export()
+    assert tl.tobytes() == S2["tlas"].tobytes() and inst.tobytes() == S2["inst"].tobytes()
+    pt.reset()
+    pt.render(cam, 2)
+    img = pt.read().cpu().numpy()
+    ref = _oracle_render(S2, cam, W_, H_, bounces, 2, sun, pcg_draws=2 * (bounces + 1))
+    a, r = img[..., :3].astype(np.float64), ref[..., :3].astype(np.float64)
+    close = np.all(np.abs(a - r) <= 1e-4 + 1e-3 * np.abs(r), axis=-1)
+    print("pt after update: fraction of matching pixels", close.mean())
+    assert close.mean() > 0.97 and abs(a.mean() - r.mean()) <= 0.01 * r.mean()
+    assert not np.allclose(before[..., :3], img[..., :3])  # the scene did change
+    pt.close()

@@ -203,3 +203,49 @@ def test_gpu_realtime_matches_oracle(torch_cuda, spf):
         last_xform = _xform(S)
         last_vp = vp
     pt.close()
+
+
+@pytest.mark.gpu
+def test_gpu_realtime_after_world_update(torch_cuda):
+    """An update between realtime frames re-packs the instances (here 8 fewer
+    plus 4 new ones): each kept instance's velocity still comes from its own
+    previous-frame matrix (matched by its position in the update's instance
+    array), new instances have none, and the G-buffer equals the oracle's."""
+    print("case realtime after update", flush=True)
+    S = _scene()
+    W_, H_, bounces = 96, 64, 2
+    pt = atlas_rt.PathTracer(S["world"], W_, H_, bounces, S["mats"], W.sun_light(), env=ENV)
+    pt.set_realtime(samples_per_frame=1)
+    proj = atlas_rt.projection_matrix(50.0, W_ / H_)
+    eye, yaw, pitch = (0.0, 15.0, -70.0), 0.0, 12.0
+    view = atlas_rt.view_matrix(eye, yaw, pitch)
+    vp = proj @ view
+    cam = atlas_rt.camera(eye, yaw, pitch, 50.0, W_ / H_)
+    pt.render_realtime(cam, atlas_rt.frame_matrices(view, proj, vp))
+    g = W.rng(2, 906)
+    n_old = len(S["descs"])
+    keep = n_old - 8
+    xf2 = np.concatenate([S["xf"][:keep], W.random_affine(g, 4, scale=(1.0, 4.0), extent=30.0, height=8.0)])
+    xf2[:keep, :, 3] += g.uniform(-0.5, 0.5, (keep, 3))
+    which2 = np.concatenate([S["which"][:keep], g.integers(0, len(S["meshes"]), 4)])
+    descs2 = atlas_rt.instance_descs(xf2, which2)
+    descs2["material_offset"] = which2.astype(np.int32)
+    S["world"].update(descs2, S["blases"])
+    S["cur_descs"] = descs2
+    _, _, order = S["world"].export()
+    last_xform = np.stack([(S["descs"][r] if r < keep else descs2[r])["object_to_world"].reshape(-1)
+                           for r in order]).astype(np.float32)
+    pt.render_realtime(cam, atlas_rt.frame_matrices(view, proj, vp))
+    got = {k: pt.read_realtime(k).cpu().numpy() for k in range(6)}
+    frame, depth, vel, nrm, mat = _oracle_frame(S, cam, atlas_rt.frame_matrices(view, proj, vp), W_, H_, bounces, 1,
+                                                last_xform, 1)
+    assert np.array_equal(got[_abi.RT_DEPTH].view(np.uint32), depth.view(np.uint32))
+    assert np.array_equal(got[_abi.RT_MATERIAL].view(np.uint16), mat)
+    assert np.array_equal(got[_abi.RT_NORMAL].view(np.uint32), nrm)
+    assert np.array_equal(got[_abi.RT_VELOCITY].view(np.uint32), vel)
+    v = oracle_rt.h2f((vel & 0xFFFF).astype(np.uint16))
+    assert np.abs(v).max() > 0  # the kept instances moved
+    a, r = got[_abi.RT_FRAME][..., :3].astype(np.float64), frame[..., :3].astype(np.float64)
+    close = np.all(np.abs(a - r) <= 1e-4 + 1e-3 * np.abs(r), axis=-1)
+    assert close.mean() > 0.97
+    pt.close()
