@@ -284,6 +284,15 @@ class World:
         """The overflow status of the world's traces since the last call."""
         return lib().atlas_rt_world_trace_status(self._h, _stream_ptr(stream))
 
+    def set_history(self, last_object_to_world):
+        """Last frame's object-to-world rows (n, 3, 4) of the last update's
+        instances in its order (includeObjectHistory); None clears it."""
+        if last_object_to_world is None:
+            check(lib().atlas_rt_world_set_history(self._h, None, 0))
+            return
+        m = np.ascontiguousarray(last_object_to_world, dtype=np.float32).reshape(-1, 12)
+        check(lib().atlas_rt_world_set_history(self._h, _ptr(m), len(m)))
+
     def trace_closest_rays(self, d_rays, d_hits, d_inst=None, count=None, mask=_abi.MASK_ALL, stream=None):
         n = count if count is not None else d_rays.numel() // 8
         check(lib().atlas_rt_world_trace_closest_rays(self._h, _ptr(d_rays), _ptr(d_hits), _ptr(d_inst), n, mask,
@@ -512,15 +521,18 @@ def frame_matrices(view, proj, last_view_proj, jitter=(0.0, 0.0)):
 
 
 class PathTracer:
-    """Progressive wavefront path tracer over a World (PathTracingRenderer).
+    """Progressive path tracer over a World (PathTracingRenderer).
 
     frames_in_flight: progressive frames overlapped on the tracer's own
-    streams (1-4, None for the library's default); the image does not depend
-    on it."""
+    streams (1-4, None for the library's default); wavefront: bounce-by-bounce
+    launches instead of the resident frame kernel.  The image depends on
+    neither."""
 
     def __init__(self, world, width, height, bounces, materials, lights, env=(0.0, 0.0, 0.0), exposure=1.0,
-                 binning=False, frames_in_flight=None):
+                 binning=False, frames_in_flight=None, wavefront=False):
         flags = _abi.PT_RAY_BINNING if binning else 0
+        if wavefront:
+            flags |= _abi.PT_WAVEFRONT
         if frames_in_flight is not None:
             flags |= int(frames_in_flight) << _abi.PT_FRAMES_IN_FLIGHT_SHIFT
         p = _abi.PtParams(width, height, bounces, exposure, (C.c_float * 3)(*env), flags)

@@ -32,6 +32,7 @@ KERNEL_SLAB_FLOAT = 0x20
 KERNEL_LEAF_BATCH = 0x40
 
 PT_RAY_BINNING = 0x1
+PT_WAVEFRONT = 0x2
 PT_FRAMES_IN_FLIGHT_SHIFT = 8  # flags bits 8-11: progressive frames in flight (0 = default)
 RAY_BINS = 64
 
@@ -171,6 +172,7 @@ PROTOTYPES = {
     "atlas_rt_blas_trace_status": (status, [vp, vp]),
     "atlas_rt_world_set_stack_limit": (status, [vp, u32]),
     "atlas_rt_world_trace_status": (status, [vp, vp]),
+    "atlas_rt_world_set_history": (status, [vp, vp, u32]),
     "atlas_rt_pathtracer_status": (status, [vp, vp]),
     "atlas_rt_world_create": (status, [C.POINTER(vp)]),
     "atlas_rt_world_destroy": (status, [vp]),

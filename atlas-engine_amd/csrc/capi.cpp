@@ -137,58 +137,20 @@ atlas_rt_status atlas_rt_trace_any_rays(const atlas_rt_blas* blas, const atlas_r
 
 atlas_rt_status atlas_rt_trace_closest_host(const atlas_rt_blas* blas, const atlas_rt_ray* rays, atlas_rt_hit* hits,
                                             uint32_t count) {
-    atlas_rt_status st = check_blas_ready(blas, "atlas_rt_trace_closest_host");
-    if (st != ATLAS_RT_OK) return st;
+    if (!blas) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_trace_closest_host: NULL BLAS handle");
     if (count == 0) return ATLAS_RT_OK;
     if (!rays || !hits) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_trace_closest_host: NULL buffer");
-    void *dr = nullptr, *dh = nullptr;
-    ATLAS_HIP_TRY(hipMalloc(&dr, size_t(count) * sizeof(atlas_rt_ray)));
-    hipError_t e = hipMalloc(&dh, size_t(count) * sizeof(atlas_rt_hit));
-    if (e != hipSuccess) {
-        hipFree(dr);
-        return hip_fail(e, "hipMalloc(hits)");
-    }
-    st = ATLAS_RT_OK;
-    e = hipMemcpy(dr, rays, size_t(count) * sizeof(atlas_rt_ray), hipMemcpyHostToDevice);
-    if (e != hipSuccess) st = hip_fail(e, "hipMemcpy(rays)");
-    if (st == ATLAS_RT_OK)
-        st = atlas_rt_trace_closest_rays(blas, reinterpret_cast<atlas_rt_ray*>(dr), reinterpret_cast<atlas_rt_hit*>(dh),
-                                         nullptr, count, ATLAS_RT_KERNEL_AUTO, nullptr);
-    if (st == ATLAS_RT_OK) {
-        e = hipMemcpy(hits, dh, size_t(count) * sizeof(atlas_rt_hit), hipMemcpyDeviceToHost);
-        if (e != hipSuccess) st = hip_fail(e, "hipMemcpy(hits)");
-    }
-    hipFree(dr);
-    hipFree(dh);
-    return st;
+    host_trace_closest(*blas, rays, hits, count);
+    return ATLAS_RT_OK;
 }
 
 atlas_rt_status atlas_rt_trace_any_host(const atlas_rt_blas* blas, const atlas_rt_ray* rays, uint32_t* hit,
                                         uint32_t count) {
-    atlas_rt_status st = check_blas_ready(blas, "atlas_rt_trace_any_host");
-    if (st != ATLAS_RT_OK) return st;
+    if (!blas) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_trace_any_host: NULL BLAS handle");
     if (count == 0) return ATLAS_RT_OK;
     if (!rays || !hit) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_trace_any_host: NULL buffer");
-    void *dr = nullptr, *dh = nullptr;
-    ATLAS_HIP_TRY(hipMalloc(&dr, size_t(count) * sizeof(atlas_rt_ray)));
-    hipError_t e = hipMalloc(&dh, size_t(count) * sizeof(uint32_t));
-    if (e != hipSuccess) {
-        hipFree(dr);
-        return hip_fail(e, "hipMalloc(hit)");
-    }
-    st = ATLAS_RT_OK;
-    e = hipMemcpy(dr, rays, size_t(count) * sizeof(atlas_rt_ray), hipMemcpyHostToDevice);
-    if (e != hipSuccess) st = hip_fail(e, "hipMemcpy(rays)");
-    if (st == ATLAS_RT_OK)
-        st = atlas_rt_trace_any_rays(blas, reinterpret_cast<atlas_rt_ray*>(dr), reinterpret_cast<uint32_t*>(dh), count,
-                                     ATLAS_RT_KERNEL_AUTO, nullptr);
-    if (st == ATLAS_RT_OK) {
-        e = hipMemcpy(hit, dh, size_t(count) * sizeof(uint32_t), hipMemcpyDeviceToHost);
-        if (e != hipSuccess) st = hip_fail(e, "hipMemcpy(hit)");
-    }
-    hipFree(dr);
-    hipFree(dh);
-    return st;
+    host_trace_any(*blas, rays, hit, count);
+    return ATLAS_RT_OK;
 }
 
 atlas_rt_status atlas_rt_blas_set_stack_limit(atlas_rt_blas* blas, uint32_t entries) {

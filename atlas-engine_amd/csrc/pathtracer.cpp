@@ -37,8 +37,7 @@ atlas_rt_status alloc_slot_rays(PathTracer& P, uint32_t k, uint32_t rays) {
     if (st == ATLAS_RT_OK) st = dev_alloc(S.hits, rays);
     if (st == ATLAS_RT_OK) st = dev_alloc(S.shadow, 3 * size_t(rays));
     if (st == ATLAS_RT_OK) st = dev_alloc(S.Lpix, rays);
-    const size_t grid_threads = (size_t(rays) + 127) / 128 * 128;
-    if (st == ATLAS_RT_OK && P.cap > 32) st = S.spill.ensure(size_t(P.cap - 32) * grid_threads * sizeof(int32_t));
+    if (st == ATLAS_RT_OK) st = S.spill.ensure(slot_spill_bytes(rays, P.cap));
     if (st == ATLAS_RT_OK && P.binning) st = dev_alloc(S.binned, 3 * size_t(rays));
     if (st == ATLAS_RT_OK && P.binning) st = S.bin_ws.ensure(bin_sort_workspace(rays));
     return st;
@@ -246,17 +245,29 @@ void atlas_rt::pt::free_cache_buffers(PathTracer& P) {
     P.shade_args.cache = CacheArgs{};
 }
 
+size_t atlas_rt::pt::slot_spill_bytes(uint32_t rays, uint32_t cap) {
+    size_t need = 0;
+    if (cap > uint32_t(kGridLds)) {
+        const size_t grid_threads = (size_t(rays) + kGridBlock - 1) / kGridBlock * kGridBlock;
+        need = size_t(cap - kGridLds) * grid_threads * sizeof(int32_t);
+    }
+    if (cap > uint32_t(kResidentLds)) {
+        const size_t lanes = size_t(device_cu_count()) * kResidentGroupsPerCu * kResidentBlock;
+        need = std::max(need, size_t(cap - kResidentLds) * lanes * sizeof(int32_t));
+    }
+    return need;
+}
+
 atlas_rt_status atlas_rt::pt::apply_stack_cap(PathTracer& P) {
     const uint32_t cap = P.world->stack_cap();
     if (cap == P.cap && P.cap_bound) return ATLAS_RT_OK;
     // a larger spill replaces the old one, which frames in flight may still use
     if (P.cap_bound && cap > P.cap) ATLAS_HIP_TRY(hipDeviceSynchronize());
     const size_t pixels = size_t(P.width) * P.height;
-    for (uint32_t k = 0; k < P.slots && cap > 32; k++) {
+    for (uint32_t k = 0; k < P.slots; k++) {
         if (!P.slot[k].ctl) continue;
         const size_t rays = k == 0 ? P.rays : pixels;  // realtime mode renders in slot 0
-        const size_t grid_threads = (rays + 127) / 128 * 128;
-        atlas_rt_status st = P.slot[k].spill.ensure(size_t(cap - 32) * grid_threads * sizeof(int32_t));
+        atlas_rt_status st = P.slot[k].spill.ensure(slot_spill_bytes(uint32_t(rays), cap));
         if (st != ATLAS_RT_OK) return st;
     }
     P.cap = cap;
@@ -266,7 +277,7 @@ atlas_rt_status atlas_rt::pt::apply_stack_cap(PathTracer& P) {
 
 atlas_rt_status atlas_rt::pt::bind_world(PathTracer& P) {
     const atlas_rt_world& W = *P.world;
-    if (P.world_gen == W.generation) return ATLAS_RT_OK;
+    if (P.world_gen == W.generation && P.history_serial == W.history_serial) return ATLAS_RT_OK;
     if (!W.valid) return fail(ATLAS_RT_E_INVALID_ARG, "path tracer: the world has no instances (failed update?)");
     for (const atlas_rt_blas* b : W.blas_handles)
         if (!b->d_shade)
@@ -315,20 +326,27 @@ atlas_rt_status atlas_rt::pt::bind_world(PathTracer& P) {
     A.alpha.shade = P.shade_table;
     A.alpha.uv = P.uv_table;
     if (P.realtime && P.rt_last_xform) {
-        // The update re-packs the instances: the previous frame's transform of
-        // each instance moves to its new slot (matched by caller index); a new
-        // instance starts without motion.
+        // The update re-packs the instances.  With an object history for this
+        // update (atlas_rt_world_set_history) each slot takes its instance's
+        // last matrix from it; otherwise the previous frame's transform of
+        // each instance moves to its new slot, matched by caller index, and
+        // an index the previous update did not have starts without motion.
         const size_t m_old = P.world_order.size(), m_new = W.order.size();
         std::vector<float> last(12 * m_old), moved(12 * m_new);
-        if (m_old)
+        const bool history = W.history_gen == W.generation && W.history.size() == 12 * m_new;
+        if (m_old && !history)
             ATLAS_HIP_TRY(hipMemcpy(last.data(), P.rt_last_xform, 12 * m_old * sizeof(float), hipMemcpyDeviceToHost));
         std::vector<int64_t> where;
-        for (size_t j = 0; j < m_old; j++) {
+        for (size_t j = 0; j < m_old && !history; j++) {
             if (P.world_order[j] >= where.size()) where.resize(P.world_order[j] + 1, -1);
             where[P.world_order[j]] = int64_t(j);
         }
         for (size_t j = 0; j < m_new; j++) {
             const uint32_t id = W.order[j];
+            if (history) {
+                std::memcpy(&moved[12 * j], &W.history[12 * size_t(id)], 12 * sizeof(float));
+                continue;
+            }
             const int64_t src = id < where.size() ? where[id] : -1;
             std::memcpy(&moved[12 * j], src >= 0 ? &last[12 * size_t(src)] : &W.xform[12 * j], 12 * sizeof(float));
         }
@@ -349,6 +367,7 @@ atlas_rt_status atlas_rt::pt::bind_world(PathTracer& P) {
     if (st != ATLAS_RT_OK) return st;
     P.world_order = W.order;
     P.world_gen = W.generation;
+    P.history_serial = W.history_serial;
     return ATLAS_RT_OK;
 }
 
@@ -372,7 +391,7 @@ atlas_rt_status atlas_rt_pathtracer_create(const atlas_rt_world* world, const at
     for (uint32_t l = 0; l < light_count; l++)
         if (lights[l].type != ATLAS_RT_LIGHT_DIRECTIONAL)
             return fail(ATLAS_RT_E_UNSUPPORTED, "atlas_rt_pathtracer_create: only directional lights");
-    if (params->flags & ~(ATLAS_RT_PT_RAY_BINNING | ATLAS_RT_PT_FRAMES_IN_FLIGHT_MASK))
+    if (params->flags & ~(ATLAS_RT_PT_RAY_BINNING | ATLAS_RT_PT_WAVEFRONT | ATLAS_RT_PT_FRAMES_IN_FLIGHT_MASK))
         return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_create: unknown flag bits");
     const uint32_t slots = (params->flags & ATLAS_RT_PT_FRAMES_IN_FLIGHT_MASK) >> ATLAS_RT_PT_FRAMES_IN_FLIGHT_SHIFT;
     if (slots > kMaxFrameSlots)
@@ -380,6 +399,7 @@ atlas_rt_status atlas_rt_pathtracer_create(const atlas_rt_world* world, const at
     auto* pt = new atlas_rt_pathtracer();
     PathTracer& P = pt->P;
     P.binning = (params->flags & ATLAS_RT_PT_RAY_BINNING) != 0;
+    P.wavefront = (params->flags & ATLAS_RT_PT_WAVEFRONT) != 0;
     P.slots = slots ? slots : kDefaultFrameSlots;
     P.width = params->width;
     P.height = params->height;

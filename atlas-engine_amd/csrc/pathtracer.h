@@ -63,6 +63,19 @@ struct FrameSlot {
 constexpr uint32_t kMaxFrameSlots = 4;
 constexpr uint32_t kDefaultFrameSlots = 3;
 
+// The resident frame kernel's launch shape: 256-lane groups, four per CU
+// (four waves per SIMD), a 24-entry LDS stack per lane; deeper entries spill
+// to the slot's spill rows, one column per resident lane.
+constexpr int kResidentBlock = 256;
+constexpr int kResidentLds = 24;
+constexpr uint32_t kResidentGroupsPerCu = 4;
+// The grid-shaped kernels' stack: 128-lane groups, 32 LDS entries.
+constexpr int kGridBlock = 128;
+constexpr int kGridLds = 32;
+// Spill bytes a slot needs for `rays` paths with `cap` stack entries, for
+// either launch shape.
+size_t slot_spill_bytes(uint32_t rays, uint32_t cap);
+
 struct PathTracer {
     uint32_t width = 0, height = 0;
     int32_t bounces = 0;
@@ -72,6 +85,7 @@ struct PathTracer {
     // call that launches work rebinds them when the world has been updated
     // since (bind_world).
     uint64_t world_gen = 0;
+    uint64_t history_serial = 0;        // the world's object history last taken
     std::vector<uint32_t> world_order;  // the world's instance order at that update
     WorldDev world_dev{};
     uint32_t cap = 0;             // traversal stack entries (the world's stack_cap())
@@ -105,6 +119,9 @@ struct PathTracer {
     // ray binning (ATLAS_RT_PT_RAY_BINNING): each bounce's paths sorted into
     // the slot's `binned` by direction bin before the trace
     bool binning = false;
+    // ATLAS_RT_PT_WAVEFRONT: bounce-by-bounce launches even where the
+    // resident frame kernel could run
+    bool wavefront = false;
     ShadeArgs shade_args{};       // every field but the slot's buffers
     int32_t host_samples = 0;
     uint32_t rays = 0;            // paths per frame: pixels, times the samples per frame in realtime mode

@@ -70,6 +70,7 @@ __global__ void k_frame_begin(FrameCtl* master, FrameCtl* ctl, int32_t bounces, 
     ctl->train_paths = 0;
     ctl->queries = 0;
     ctl->train_samples = 0;
+    ctl->pull = 0;
 }
 
 // Stream compaction for a whole block: one LDS atomic per wave and one
@@ -321,12 +322,12 @@ struct ShadeOut {
     float4 pa[3];  // continued path
 };
 
-__device__ __forceinline__ void shade_one(const ShadeArgs& A, int32_t bounce, uint32_t i, ShadeOut& out) {
+// EvaluateBounce of one path: p0 = (origin, ray ID bits), p1 = (direction,
+// training index bits), p2 = (throughput, 0); h = (t bits, hit slot or -1,
+// instance slot, 0).
+__device__ __forceinline__ void shade_path(const ShadeArgs& A, int32_t bounce, float4 p0, float4 p1, float4 p2, int4 h,
+                                           ShadeOut& out) {
     FrameCtl* ctl = A.ctl;
-    const float4 p0 = A.paths_in[3 * size_t(i)];
-    const float4 p1 = A.paths_in[3 * size_t(i) + 1];
-    const float4 p2 = A.paths_in[3 * size_t(i) + 2];
-    const int4 h = A.hits[i];
     // ray ID: the pixel in progressive mode, pixel * samples per frame +
     // sample in realtime mode (ray.ID, rayGen.csh:45)
     const uint32_t id = __float_as_uint(p0.w);
@@ -595,6 +596,11 @@ __device__ __forceinline__ void shade_one(const ShadeArgs& A, int32_t bounce, ui
     out.pa[2] = make_float4(thr.x, thr.y, thr.z, 0.0f);
 }
 
+__device__ __forceinline__ void shade_one(const ShadeArgs& A, int32_t bounce, uint32_t i, ShadeOut& out) {
+    shade_path(A, bounce, A.paths_in[3 * size_t(i)], A.paths_in[3 * size_t(i) + 1], A.paths_in[3 * size_t(i) + 2],
+               A.hits[i], out);
+}
+
 __global__ __launch_bounds__(kShadeBlock) void k_shade(ShadeArgs A, int32_t bounce) {
     const uint32_t i = blockIdx.x * kShadeBlock + threadIdx.x;
     FrameCtl* ctl = A.ctl;
@@ -680,6 +686,209 @@ __global__ __launch_bounds__(BLOCK) void k_shadow(WorldDev W, const FrameCtl* __
     }
 }
 
+// ---------------------------------------------------------------------------
+// Resident frame kernel: one path per lane from camera ray to its last
+// bounce.  A resident grid of waves takes paths from the frame's `pull`
+// counter in the raygen kernels' tile order.  Each lane walks its path as
+// trace (closest hit) -> shade -> shadow ray (any hit) -> next bounce, so no
+// launch waits for the slowest path of a bounce before the next bounce
+// starts: the frame's only tail is its slowest whole path.  Lanes that trace
+// closest hits and lanes that trace shadow rays share one step body over
+// opaque worlds (world_step_mixed).  Shading is batched per wave: the wave
+// keeps stepping until a quarter of its busy lanes have finished a walk (or
+// all have), then those lanes shade or add their shadow ray's light
+// together.  Paths refill idle lanes between batches (one atomic per wave).
+//
+// Every path runs the same arithmetic in the same order as in the
+// bounce-by-bounce kernels (raygen, k_trace, shade_path, k_shadow), and
+// everything a path writes (Lpix, the G-buffer) is its own, so the image is
+// the same bit for bit.
+// ---------------------------------------------------------------------------
+constexpr int kFrameBlock = kResidentBlock;
+constexpr int kFrameLds = kResidentLds;
+constexpr uint32_t kShadeDiv = 4;
+
+// The pixel of the k-th path slot of the raygen kernels' 8x8-tile order.
+__device__ __forceinline__ uint32_t tile_pixel(uint32_t k, uint32_t width, uint32_t height) {
+    if ((width % 8) || (height % 8)) return k;
+    const uint32_t tiles_x = width / 8;
+    const uint32_t tile = k / 64, in = k % 64;
+    const uint32_t px = (tile % tiles_x) * 8 + in % 8;
+    const uint32_t py = (tile / tiles_x) * 8 + in / 8;
+    return py * width + px;
+}
+
+struct FrameGen {
+    Camera cam;
+    uint32_t width, height;
+    uint32_t rays;        // paths this frame
+    uint32_t spf;         // realtime: samples per pixel (ray ID = pixel * spf + sample); 0: progressive
+    float jx, jy;         // realtime camera jitter
+};
+
+template <bool ALPHA>
+__global__ __launch_bounds__(kFrameBlock, 4) void k_frame_persistent(ShadeArgs A, FrameGen G, int32_t* __restrict__ spill,
+                                                                  uint32_t cap, uint32_t* __restrict__ flags) {
+    __shared__ int32_t lds[kFrameLds * kFrameBlock];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u;
+    const LaneStack<kFrameBlock, kFrameLds> S{lds, spill, tid, blockIdx.x * kFrameBlock + tid,
+                                              gridDim.x * kFrameBlock};
+    FrameCtl* ctl = A.ctl;
+    const WorldDev W = A.world;
+    const AlphaArgs& AA = A.alpha;
+    // progressive jitter: the same for every pixel of the frame (rayGen.csh:33-34)
+    const float sc = float(ctl->sample_count);
+    const float pjx = rnd_pair(sc, 0.0f), pjy = rnd_pair(sc, 1.0f);
+    const V o = mk(G.cam.origin[0], G.cam.origin[1], G.cam.origin[2]);
+    const V rgt = mk(G.cam.right[0], G.cam.right[1], G.cam.right[2]);
+    const V btm = mk(G.cam.bottom[0], G.cam.bottom[1], G.cam.bottom[2]);
+    const V loc = mk(G.cam.loc[0], G.cam.loc[1], G.cam.loc[2]);
+
+    WorldLane L;
+    world_begin(L, make_float4(0.0f, 0.0f, 0.0f, 0.0f), make_float4(1.0f, 0.0f, 0.0f, 0.0f));
+    int phase = 0;       // 0 idle, 1 closest hit, 2 shadow ray
+    bool ready = false;  // the walk of this phase has ended (result in L / rstat)
+    int rstat = kDone;
+    int32_t bounce = 0;
+    float4 p0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), p1 = p0, p2 = p0;
+    float4 shc = p0;     // the shadow ray's light before visibility
+    bool pending = false;  // the path continues after its shadow ray (p0..p2 hold its next ray)
+    bool exhausted = false;
+
+    for (;;) {
+        if (!exhausted) {
+            const uint64_t idle = __ballot(phase == 0);
+            if (idle) {
+                const uint32_t n = uint32_t(__popcll(idle));
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(&ctl->pull, n);
+                base = __shfl(base, 0, 64);
+                if (phase == 0) {
+                    const uint32_t k = base + uint32_t(__popcll(idle & ((1ull << lane) - 1ull)));
+                    if (k < G.rays) {
+                        // rayGen.csh:25-90, as k_raygen / k_raygen_rt write slot k
+                        uint32_t id;
+                        V dir;
+                        if (G.spf == 0) {
+                            const uint32_t pixel = tile_pixel(k, G.width, G.height);
+                            const uint32_t px = pixel % G.width, py = pixel / G.width;
+                            const float cx = (float(px) + pjx) / float(G.width);
+                            const float cy = (float(py) + pjy) / float(G.height);
+                            dir = normalize(((o + rgt * cx) + btm * cy) - loc);
+                            id = pixel;
+                        } else {
+                            const uint32_t pixel = tile_pixel(k / G.spf, G.width, G.height);
+                            const uint32_t px = pixel % G.width, py = pixel / G.width;
+                            const float cx = G.jx * 0.25f + (float(px) + 0.5f) / float(G.width);
+                            const float cy = G.jy * 0.25f + (float(py) + 0.5f) / float(G.height);
+                            dir = normalize(((o + rgt * cx) + btm * cy) - loc);
+                            id = pixel * G.spf + k % G.spf;
+                        }
+                        A.Lpix[id] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                        p0 = make_float4(loc.x, loc.y, loc.z, __uint_as_float(id));
+                        p1 = make_float4(dir.x, dir.y, dir.z, __uint_as_float(0u));
+                        p2 = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+                        bounce = 0;
+                        phase = 1;
+                        world_begin(L, make_float4(p0.x, p0.y, p0.z, 0.0f), make_float4(p1.x, p1.y, p1.z, kInf));
+                        ready = any_nan3(p1.x, p1.y, p1.z);
+                        rstat = kDone;
+                    }
+                }
+                if (base + n >= G.rays) exhausted = true;
+            }
+        }
+        if (!__any(phase != 0)) {
+            if (exhausted) break;
+            continue;
+        }
+        // walk until enough lanes have ended their walk
+        for (;;) {
+            if (phase != 0 && !ready) {
+                int r;
+                if (ALPHA) {
+                    if (phase == 1) {
+                        const AcceptOpaque acc{TriOpacity{W, AA}};
+                        r = world_step_closest<kFrameBlock, kFrameLds>(L, W, ATLAS_RT_MASK_ALL, S, cap, acc);
+                    } else {
+                        const TriOpacity op{W, AA};
+                        r = world_step_transparency<kFrameBlock, kFrameLds>(L, W, ATLAS_RT_MASK_SHADOW, L.hitT, S,
+                                                                            cap, op);
+                    }
+                } else {
+                    r = world_step_mixed<kFrameBlock, kFrameLds>(L, W, phase == 1 ? ATLAS_RT_MASK_ALL
+                                                                                   : ATLAS_RT_MASK_SHADOW,
+                                                                 phase == 2, S, cap);
+                }
+                if (r != kContinue) {
+                    ready = true;
+                    rstat = r;
+                }
+            }
+            const uint32_t busy = uint32_t(__popcll(__ballot(phase != 0)));
+            const uint32_t done = uint32_t(__popcll(__ballot(ready)));
+            if (done == busy || done * kShadeDiv >= busy) break;
+        }
+        if (!ready) continue;
+        if (phase == 1) {
+            // k_trace's hit record, then EvaluateBounce
+            int32_t slot = -1, inst = -1;
+            if (rstat == kOverflow)
+                atomicOr(&flags[2], 1u);
+            else if (L.slot >= 0) {
+                slot = L.slot;
+                inst = L.inst;
+            }
+            ShadeOut so;
+            shade_path(A, bounce, p0, p1, p2, make_int4(__float_as_int(L.hitT), slot, inst, 0), so);
+            pending = so.cont;
+            if (so.cont) {
+                p0 = so.pa[0];
+                p1 = so.pa[1];
+                p2 = so.pa[2];
+            }
+            if (so.shadow) {
+                shc = so.sh[2];
+                world_begin(L, make_float4(so.sh[0].x, so.sh[0].y, so.sh[0].z, 0.0f),
+                            make_float4(so.sh[1].x, so.sh[1].y, so.sh[1].z, so.sh[1].w));
+                phase = 2;
+                ready = false;
+                continue;
+            }
+        } else {
+            // k_shadow: the light of the shadow ray times its visibility
+            if (rstat == kOverflow) {
+                atomicOr(&flags[2], 1u);
+            } else {
+                const float vis = ALPHA ? L.trans : (L.slot >= 0 ? 0.0f : 1.0f);
+                if (vis != 0.0f) {
+                    V cc = mk(shc.x, shc.y, shc.z) * vis;
+                    if (bounce > 0) {
+                        const float rm = fmaxf(fmaxf(cc.x, fmaxf(cc.y, cc.z)), 10.0f);
+                        cc = cc * (10.0f / rm);
+                    }
+                    const uint32_t id = __float_as_uint(p0.w);
+                    float4 acc = A.Lpix[id];
+                    acc.x += cc.x;
+                    acc.y += cc.y;
+                    acc.z += cc.z;
+                    A.Lpix[id] = acc;
+                }
+            }
+        }
+        if (pending) {
+            bounce++;
+            world_begin(L, make_float4(p0.x, p0.y, p0.z, 0.0f), make_float4(p1.x, p1.y, p1.z, kInf));
+            phase = 1;
+            ready = any_nan3(p1.x, p1.y, p1.z);
+            rstat = kDone;
+        } else {
+            phase = 0;
+            ready = false;
+        }
+    }
+}
+
 // Render paths ended at their second vertex: add throughput x cache radiance.
 __global__ __launch_bounds__(kBlock) void k_cache_apply(const FrameCtl* __restrict__ ctl,
                                                         const float4* __restrict__ meta,
@@ -758,6 +967,23 @@ ShadeArgs slot_args(const PathTracer& P, uint32_t k) {
     return A;
 }
 
+// Whether a frame runs as one resident kernel: plain path tracing (the
+// radiance cache and ray binning need the bounce-by-bounce queues).
+bool resident_frame(const PathTracer& P) { return !P.shade_args.cache.enabled && !P.binning && !P.wavefront; }
+
+atlas_rt_status launch_resident(PathTracer& P, uint32_t k, const ShadeArgs& A, const FrameGen& G, hipStream_t s) {
+    FrameSlot& S = P.slot[k];
+    const uint32_t groups = uint32_t(device_cu_count()) * kResidentGroupsPerCu;
+    int32_t* spill = reinterpret_cast<int32_t*>(S.spill.ptr);
+    uint32_t* flags = reinterpret_cast<uint32_t*>(S.flags.ptr);
+    if (P.alpha)
+        hipLaunchKernelGGL(k_frame_persistent<true>, dim3(groups), dim3(kFrameBlock), 0, s, A, G, spill, P.cap, flags);
+    else
+        hipLaunchKernelGGL(k_frame_persistent<false>, dim3(groups), dim3(kFrameBlock), 0, s, A, G, spill, P.cap, flags);
+    ATLAS_HIP_TRY(hipGetLastError());
+    return ATLAS_RT_OK;
+}
+
 }  // namespace
 
 atlas_rt_status launch_bounces(PathTracer& P, uint32_t k, hipStream_t s, hipEvent_t train_ready,
@@ -830,9 +1056,15 @@ atlas_rt_status launch_frame_realtime(PathTracer& P, const Camera& cam, const at
     // realtime frames run one at a time in slot 0 on the caller's stream
     FrameSlot& S = P.slot[0];
     hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(64), 0, s, P.ctl, S.ctl, P.bounces, P.rays, 0u);
-    hipLaunchKernelGGL(k_raygen_rt, dim3((P.rays + kShadeBlock - 1) / kShadeBlock), dim3(kShadeBlock), 0, s, cam,
-                       P.width, P.height, spf, M.jitter[0], M.jitter[1], S.paths[0], S.Lpix);
-    const atlas_rt_status st = launch_bounces(P, 0, s, nullptr, nullptr);
+    atlas_rt_status st;
+    if (resident_frame(P)) {
+        const FrameGen G{cam, P.width, P.height, P.rays, spf, M.jitter[0], M.jitter[1]};
+        st = launch_resident(P, 0, slot_args(P, 0), G, s);
+    } else {
+        hipLaunchKernelGGL(k_raygen_rt, dim3((P.rays + kShadeBlock - 1) / kShadeBlock), dim3(kShadeBlock), 0, s, cam,
+                           P.width, P.height, spf, M.jitter[0], M.jitter[1], S.paths[0], S.Lpix);
+        st = launch_bounces(P, 0, s, nullptr, nullptr);
+    }
     A.rt.enabled = 0;
     if (st != ATLAS_RT_OK) return st;
     hipLaunchKernelGGL(k_frame_rt, dim3((pixels + kBlock - 1) / kBlock), dim3(kBlock), 0, s, S.Lpix, pixels, spf,
@@ -888,6 +1120,17 @@ atlas_rt_status launch_frame(PathTracer& P, const Camera& cam, uint32_t k, const
     if (before) ATLAS_HIP_TRY(hipStreamWaitEvent(s, before->ev_begin, 0));
     hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(64), 0, s, P.ctl, S.ctl, P.bounces, pixels, 1u);
     ATLAS_HIP_TRY(hipEventRecord(S.ev_begin, s));
+    if (resident_frame(P)) {
+        const FrameGen G{cam, P.width, P.height, pixels, 0u, 0.0f, 0.0f};
+        const atlas_rt_status rs = launch_resident(P, k, A, G, s);
+        if (rs != ATLAS_RT_OK) return rs;
+        ATLAS_HIP_TRY(hipEventRecord(S.ev_train, s));
+        if (before) ATLAS_HIP_TRY(hipStreamWaitEvent(s, before->ev_acc, 0));
+        hipLaunchKernelGGL(k_accumulate, dim3(grid_all), dim3(kBlock), 0, s, S.Lpix, P.accum, pixels);
+        ATLAS_HIP_TRY(hipEventRecord(S.ev_acc, s));
+        ATLAS_HIP_TRY(hipGetLastError());
+        return ATLAS_RT_OK;
+    }
     hipLaunchKernelGGL(k_raygen, dim3((pixels + kShadeBlock - 1) / kShadeBlock), dim3(kShadeBlock), 0, s, cam, P.width,
                        P.height, S.ctl, S.paths[0], S.Lpix, C, P.bounces);
     const bool lag = C.enabled && P.query_lag;

@@ -83,7 +83,8 @@ struct FrameCtl {
     uint32_t queries;        // cache queries issued this frame
     uint32_t train_samples;  // training samples built this frame
     float train_loss;        // loss of the frame's last training step
-    uint32_t pad2[3];
+    uint32_t pull;           // paths taken so far by the resident frame kernel
+    uint32_t pad2[2];
 };
 
 // Realtime mode's G-buffer, written by the first bounce of each pixel's

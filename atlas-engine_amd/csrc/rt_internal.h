@@ -128,6 +128,10 @@ inline uint32_t limit_cap(uint32_t need, uint32_t limit) { return limit && limit
 atlas_rt_status take_overflow_flag(void* flags, void* stream, const char* who);
 
 atlas_rt_status launch_blas_closest(atlas_rt_blas& blas, const TraceLaunch& L);
+// BVH::GetIntersection / GetIntersectionAny on the host over the BLAS's host
+// copies (host_trace.cpp).
+void host_trace_closest(const atlas_rt_blas& b, const atlas_rt_ray* rays, atlas_rt_hit* hits, uint32_t count);
+void host_trace_any(const atlas_rt_blas& b, const atlas_rt_ray* rays, uint32_t* hit, uint32_t count);
 atlas_rt_status launch_blas_any(atlas_rt_blas& blas, const TraceLaunch& L);
 
 // Upload the engine-layout arrays of `blas` to the current device.

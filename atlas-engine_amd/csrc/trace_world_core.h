@@ -272,6 +272,85 @@ __device__ __forceinline__ int world_step_any(WorldLane& L, const WorldDev& W, u
     return kContinue;
 }
 
+// One step of either walk over opaque triangles, chosen per lane, so that a
+// wave whose lanes trace closest hits and shadow rays at the same time runs
+// one step body instead of both:
+//   any = false: world_step_closest (HitClosest);
+//   any = true:  world_step_any<RESTORE_ALWAYS = true> (HitAnyTransparency's
+//                traversal with every triangle opaque) with tmax = L.hitT,
+//                which world_begin set and which the walk never lowers before
+//                it ends at its first accepted triangle.
+// Both restore the world ray on every TLAS step.  Left-first order of the
+// any-hit walk is the closest walk's order with "left is nearer" replaced by
+// "left was hit", so the two share every load and slab test.
+template <int BLOCK, int LDS_DEPTH>
+__device__ __forceinline__ int world_step_mixed(WorldLane& L, const WorldDev& W, uint32_t mask, bool any,
+                                                const LaneStack<BLOCK, LDS_DEPTH>& S, uint32_t cap) {
+    const uint32_t spE = L.sp + 1;
+    const bool top = spE < L.tlas_index;
+    if (top) {
+        restore_world(L);
+        L.tlas_index = kTlasInvalid;
+        if (L.node < 0) {
+            if (enter_instance(L, W, mask)) {
+                L.tlas_index = spE;
+            } else {
+                if (L.sp == 0) return kDone;
+                L.sp--;
+                L.node = S.get(L.sp);
+            }
+            return kContinue;
+        }
+    }
+    if (top || L.node >= 0) {
+        const float4* np = (top ? W.tlas : L.B.nodes) + 4 * size_t(L.node);
+        const float4 a = np[0], b = np[1], c = np[2], d = np[3];
+        float dl, dr;
+        const bool hl = slab_dist(L.R, a.x, a.y, a.z, a.w, b.x, b.y, L.hitT, dl);
+        const bool hr = slab_dist(L.R, b.z, b.w, c.x, c.y, c.z, c.w, L.hitT, dr);
+        const int32_t lp = __float_as_int(d.x), rp = __float_as_int(d.y);
+        const bool leftFirst = any ? hl : dl <= dr;
+        if (hl && hr) {
+            if (L.sp >= cap) return kOverflow;
+            S.put(L.sp, leftFirst ? rp : lp);
+            L.sp++;
+            L.node = leftFirst ? lp : rp;
+        } else if (hl || hr) {
+            L.node = leftFirst ? lp : rp;
+        } else {
+            if (L.sp == 0) return kDone;
+            L.sp--;
+            L.node = S.get(L.sp);
+        }
+        return kContinue;
+    }
+    const int32_t tp = ~L.node;
+    const float4* tr = L.B.tris + 3 * size_t(tp);
+    const float4 a = tr[0], b = tr[1], c = tr[2];
+    float t, u, v;
+    const bool geo = tri_uvt(L.R, a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z, t, u, v);
+    if (geo && t > L.R.tmin && t < L.hitT) {
+        L.hitT = t;
+        L.slot = tp;
+        L.inst = L.cur_inst;
+        if (any) return kDone;
+        L.hox = L.R.ox;
+        L.hoy = L.R.oy;
+        L.hoz = L.R.oz;
+        L.hdx = L.R.dx;
+        L.hdy = L.R.dy;
+        L.hdz = L.R.dz;
+    }
+    if (b.w > 0.0f) {
+        if (L.sp == 0) return kDone;
+        L.sp--;
+        L.node = S.get(L.sp);
+    } else {
+        L.node = ~(tp + 1);
+    }
+    return kContinue;
+}
+
 // One step of HitAnyTransparency (bvh.hsh:443-524) with the world ray
 // restored on every TLAS step.  A leaf is handled in one step, as
 // CheckLeafTransparency does (bvh.hsh:139-170): starting from the current

@@ -210,6 +210,8 @@ atlas_rt_status atlas_rt_world_update(atlas_rt_world* w, const atlas_rt_instance
         g.mask = d.mask;
         order[i] = bo.ref_idx[i];
     }
+    // traces and frames in flight on any stream still read the arrays freed here
+    ATLAS_HIP_TRY(hipDeviceSynchronize());
     w->free_device();
     w->valid = false;
     w->generation++;
@@ -290,6 +292,24 @@ atlas_rt_status atlas_rt_world_refit(atlas_rt_world* w, const atlas_rt_instance_
 atlas_rt_status atlas_rt_world_set_stack_limit(atlas_rt_world* w, uint32_t entries) {
     if (!w) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_world_set_stack_limit: NULL world");
     w->stack_limit = entries;
+    return ATLAS_RT_OK;
+}
+
+atlas_rt_status atlas_rt_world_set_history(atlas_rt_world* w, const float* last_object_to_world, uint32_t n) {
+    if (!w) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_world_set_history: NULL world");
+    if (!w->valid) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_world_set_history: the world has no instances");
+    if (!last_object_to_world) {
+        w->history.clear();
+        w->history_gen = ~uint64_t(0);
+        w->history_serial++;
+        return ATLAS_RT_OK;
+    }
+    if (n != w->order.size())
+        return fail(ATLAS_RT_E_SIZE_MISMATCH, "atlas_rt_world_set_history: " + std::to_string(n) +
+                                                  " matrices for " + std::to_string(w->order.size()) + " instances");
+    w->history.assign(last_object_to_world, last_object_to_world + 12 * size_t(n));
+    w->history_gen = w->generation;
+    w->history_serial++;
     return ATLAS_RT_OK;
 }
 

@@ -53,6 +53,12 @@ struct atlas_rt_world {
     // 0: sized from the TLAS and BLAS depths, which never overflows.
     uint32_t stack_limit = 0;
     uint32_t stack_cap() const { return atlas_rt::limit_cap(max_depth + 1, stack_limit); }
+    // Last frame's object-to-world rows per instance in the caller's order of
+    // the last update (atlas_rt_world_set_history); valid while
+    // history_gen == generation.
+    std::vector<float> history;
+    uint64_t history_gen = ~uint64_t(0);
+    uint64_t history_serial = 0;  // bumped by every atlas_rt_world_set_history
 
     int device = -1;
     void* d_tlas = nullptr;

@@ -12,11 +12,13 @@
 //  * The trees are built by the library's builder, bit-identical to the
 //    engine's BVHBuilder; `nodes`, `data` (BVH order, endOfNode set) and
 //    `refs` hold what the engine's members would hold.
-//  * GetIntersection / GetIntersectionAny trace on the GPU with the engine's
-//    shader semantics (HitClosest / HitAny, data/shader/raytracer/bvh.hsh),
-//    which is what the engine renders with; the engine's own CPU twin
-//    (BVH.cpp:103-217) differs in tmin handling and tie order.  The stack
-//    argument is accepted for source compatibility and unused.
+//  * GetIntersection / GetIntersectionAny run the engine's own CPU traversal
+//    (BVH.cpp:103-217: float-reciprocal slab test, distance-culled stack,
+//    triangles accepted from t = 0) on the host through
+//    atlas_rt_trace_closest_host / _any_host, also for host-only builds; the
+//    GPU kernels (atlas_rt_trace_closest_rays) have the shaders' HitClosest
+//    semantics instead.  The stack argument is accepted for source
+//    compatibility and unused.
 //  * A size mismatch between aabbs and data leaves the BVH empty, as in the
 //    engine (BVH.cpp:18-19); status() reports why.
 //  * vec3 is glm::vec3 when ATLAS_RT_FACADE_USE_GLM is defined (the engine
@@ -144,7 +146,7 @@ class BVH {
             refs[i].idx = src;
             refs[i].endOfNode = this->data[i].endOfNode;
             refs[i].aabb = aabbs[src];
-            if (!slot_of_.count(data[src].idx)) slot_of_[data[src].idx] = i;
+            if (!slot_of_.count(src)) slot_of_[src] = i;  // hits name the input position
         }
     }
 
@@ -206,7 +208,7 @@ class BVH {
         return hit != 0;
     }
 
-    // Batched form of GetIntersection (one GPU launch for all rays).
+    // Batched form of GetIntersection (the host's hardware threads share the rays).
     std::vector<atlas_rt_hit> GetIntersections(const std::vector<Ray>& rays) {
         std::vector<atlas_rt_ray> r(rays.size());
         for (size_t i = 0; i < rays.size(); i++) r[i] = to_ray(rays[i]);
@@ -267,7 +269,7 @@ class BVH {
     }
 
     std::unique_ptr<atlas_rt_blas, Deleter> blas_;
-    std::unordered_map<uint32_t, uint32_t> slot_of_;
+    std::unordered_map<uint32_t, uint32_t> slot_of_;  // input position -> first BVH slot holding it
     atlas_rt_status status_ = ATLAS_RT_OK;
 };
 

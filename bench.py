@@ -286,7 +286,8 @@ PROBES = """probes:
                   4 bounces): 2 untimed frames, then 16, optionally with
                   ray binning
   (pt and c2 take fif=N anywhere after the command: N progressive frames
-  in flight instead of the library's default)
+  in flight instead of the library's default; pt, c2 and realtime take
+  `wavefront`: bounce-by-bounce launches instead of the resident frame kernel)
   realtime        the same scene in realtime mode (4 samples per pixel,
                   G-buffer, temporal resolve): 2 untimed frames, then 16
   summarize-pmc DIR   per-dispatch counter averages of a --pmc run's CSVs"""
@@ -301,7 +302,7 @@ def _fif(argv):
     return None
 
 
-def probe_pt(mode, stride=32, steps=2, lag=0, fif=None):
+def probe_pt(mode, stride=32, steps=2, lag=0, fif=None, wavefront=False):
     import torch
 
     from atlas_rt import PathTracer, RadianceCache, camera
@@ -312,7 +313,7 @@ def probe_pt(mode, stride=32, steps=2, lag=0, fif=None):
     cc = W.c4_camera()
     cam = camera(cc["eye"], cc["yaw_deg"], cc["pitch_deg"], cc["fov_y_deg"], cc["width"] / cc["height"])
     p = PathTracer(w, cc["width"], cc["height"], 8, W.c2_materials(mats_d), W.sun_light(), env=W.C2_ENV,
-                   binning=(mode == "binned"), frames_in_flight=fif)
+                   binning=(mode == "binned"), frames_in_flight=fif, wavefront=wavefront)
     if mode == "cache":
         rc = RadianceCache((-20, -20, -20), (20, 20, 20), levels=12, log2_table=19, base_resolution=16,
                            max_resolution=2048)
@@ -323,10 +324,11 @@ def probe_pt(mode, stride=32, steps=2, lag=0, fif=None):
     p.render(cam, 8)
     torch.cuda.synchronize()
     lag_s = f" lag={lag}" if mode == "cache" else ""
-    print(f"pt {mode} fif={fif}{lag_s}: {(time.perf_counter() - t) / 8 * 1e3:.3f} ms/frame", flush=True)
+    print(f"pt {mode} fif={fif} wavefront={wavefront}{lag_s}: {(time.perf_counter() - t) / 8 * 1e3:.3f} ms/frame",
+          flush=True)
 
 
-def probe_c2(binning, fif=None):
+def probe_c2(binning, fif=None, wavefront=False):
     import torch
 
     torch.cuda.init()
@@ -338,16 +340,17 @@ def probe_c2(binning, fif=None):
     cc = W.c2_camera()
     cam = camera(cc["eye"], cc["yaw_deg"], cc["pitch_deg"], cc["fov_y_deg"], cc["width"] / cc["height"])
     p = PathTracer(w, cc["width"], cc["height"], 4, W.c2_materials(mats_d), W.sun_light(), env=W.C2_ENV,
-                   binning=binning, frames_in_flight=fif)
+                   binning=binning, frames_in_flight=fif, wavefront=wavefront)
     p.render(cam, 2)
     torch.cuda.synchronize()
     t = time.perf_counter()
     p.render(cam, 16)
     torch.cuda.synchronize()
-    print(f"c2 binning={binning} fif={fif}: {(time.perf_counter() - t) / 16 * 1e3:.3f} ms/frame", flush=True)
+    print(f"c2 binning={binning} fif={fif} wavefront={wavefront}: {(time.perf_counter() - t) / 16 * 1e3:.3f} "
+          "ms/frame", flush=True)
 
 
-def probe_realtime():
+def probe_realtime(wavefront=False):
     import torch
 
     torch.cuda.init()
@@ -358,7 +361,8 @@ def probe_realtime():
     w, blases = _world(meshes, xf, which)
     cc = W.c2_camera()
     cam = camera(cc["eye"], cc["yaw_deg"], cc["pitch_deg"], cc["fov_y_deg"], cc["width"] / cc["height"])
-    p = PathTracer(w, cc["width"], cc["height"], 4, W.c2_materials(mats_d), W.sun_light(), env=W.C2_ENV)
+    p = PathTracer(w, cc["width"], cc["height"], 4, W.c2_materials(mats_d), W.sun_light(), env=W.C2_ENV,
+                   wavefront=wavefront)
     p.set_realtime(samples_per_frame=4)
     view = view_matrix(cc["eye"], cc["yaw_deg"], cc["pitch_deg"])
     proj = projection_matrix(cc["fov_y_deg"], cc["width"] / cc["height"])
@@ -370,7 +374,7 @@ def probe_realtime():
     for _ in range(16):
         p.render_realtime(cam, fm)
     torch.cuda.synchronize()
-    print(f"c2 realtime: {(time.perf_counter() - t) / 16 * 1e3:.3f} ms/frame", flush=True)
+    print(f"c2 realtime wavefront={wavefront}: {(time.perf_counter() - t) / 16 * 1e3:.3f} ms/frame", flush=True)
 
 
 def probe_nrc():
@@ -527,13 +531,14 @@ def probe_lbvh(reps=3):
 def probe(argv):
     cmd, rest = argv[0], argv[1:]
     fif = _fif(rest)
-    args = [a for a in rest if not a.startswith("fif=")]
+    wf = "wavefront" in rest
+    args = [a for a in rest if not a.startswith("fif=") and a != "wavefront"]
     if cmd == "pt":
-        probe_pt(args[0] if args else "plain", *[int(a) for a in args[1:4]], fif=fif)
+        probe_pt(args[0] if args else "plain", *[int(a) for a in args[1:4]], fif=fif, wavefront=wf)
     elif cmd == "c2":
-        probe_c2("binned" in args, fif=fif)
+        probe_c2("binned" in args, fif=fif, wavefront=wf)
     elif cmd == "realtime":
-        probe_realtime()
+        probe_realtime(wavefront=wf)
     elif cmd == "nrc":
         probe_nrc()
     elif cmd == "lbvh":

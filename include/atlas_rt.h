@@ -253,8 +253,17 @@ atlas_rt_status atlas_rt_trace_closest_stats(const atlas_rt_blas* blas, const at
 atlas_rt_status atlas_rt_trace_any_rays(const atlas_rt_blas* blas, const atlas_rt_ray* d_rays,
                                         uint32_t* d_hit, uint32_t count, int32_t kernel, void* stream);
 
-/* Host-buffer convenience for CPU callers (BVH::GetIntersection, BVH.cpp:103-174):
- * copies rays in, traces on the device, copies hits back, synchronises. */
+/* The engine's CPU traversal for CPU callers, on the host, with host buffers:
+ * BVH::GetIntersection / GetIntersectionAny (BVH.cpp:103-217) over Ray::Intersects
+ * (Ray.cpp:28-76).  These are the engine's CPU semantics, not HitClosest's:
+ * float-reciprocal slab test, a stack of (node, entry distance) with entries
+ * beyond the current hit skipped, both hit children pushed nearer last (left
+ * last on ties), triangles accepted at t >= 0 (tMin bounds only the boxes)
+ * and t < current t; any hit pops left first and stops at the first t <
+ * tMax.  prim_id is the caller's index, (t, u, v) the FP32 solution; a hit is
+ * reported when t < the ray's tMax (the engine compares against the tMax it
+ * overwrote during the walk, BVH.cpp:141,172).  Works for host-only BLASes;
+ * large batches run on the host's hardware threads. */
 atlas_rt_status atlas_rt_trace_closest_host(const atlas_rt_blas* blas, const atlas_rt_ray* rays,
                                             atlas_rt_hit* hits, uint32_t count);
 atlas_rt_status atlas_rt_trace_any_host(const atlas_rt_blas* blas, const atlas_rt_ray* rays,
@@ -421,6 +430,19 @@ atlas_rt_status atlas_rt_world_refit(atlas_rt_world* world, const atlas_rt_insta
  * tracer's traces follow the limit of its world. */
 atlas_rt_status atlas_rt_world_set_stack_limit(atlas_rt_world* world, uint32_t entries);
 atlas_rt_status atlas_rt_world_trace_status(atlas_rt_world* world, void* stream);
+
+/* Object history of the last update, RayTracingWorld::Update with
+ * includeObjectHistory (RayTracingWorld.cpp:116-117, read as
+ * instanceLastMatrices by rayHit.csh:80-101): last_object_to_world holds
+ * instance_count x 12 floats, rows 0..2 of each instance's
+ * TransformComponent::lastGlobalMatrix, in the caller's order of the last
+ * atlas_rt_world_update.  A realtime path tracer's next frame takes its
+ * velocities from them.  Without a history for the current update, a
+ * realtime frame after an update matches instances by their index in the
+ * update's array (an instance at an index the previous array did not have
+ * starts without motion).  NULL clears the history. */
+atlas_rt_status atlas_rt_world_set_history(atlas_rt_world* world, const float* last_object_to_world,
+                                           uint32_t instance_count);
 
 atlas_rt_status atlas_rt_world_info(const atlas_rt_world* world, uint32_t* tlas_node_count,
                                     uint32_t* instance_count, uint32_t* max_depth);
@@ -600,6 +622,13 @@ typedef struct atlas_rt_camera {
  * tracing (the engine's useRayBinning, RayTracingHelper.h:53, switched off
  * there; see atlas_rt_sort_rays_by_direction).  The image is unchanged. */
 #define ATLAS_RT_PT_RAY_BINNING 0x1u
+/* atlas_rt_pt_params.flags: run every bounce as separate closest-hit,
+ * shade and shadow-ray launches over compacted path queues (the engine's
+ * dispatch shape, RayTracingHelper.cpp:224-405) instead of the default
+ * resident frame kernel, which walks each path from its camera ray to its
+ * last bounce in one lane.  Both give the same image bit for bit.  The
+ * radiance cache and ray binning always use the bounce-by-bounce launches. */
+#define ATLAS_RT_PT_WAVEFRONT 0x2u
 /* atlas_rt_pt_params.flags bits 8-11: progressive frames in flight (1-4; 0
  * picks 3).  No engine counterpart: the engine renders one frame per call.
  * Each frame in flight has its own stream and path buffers; the last

@@ -45,12 +45,19 @@ def test_version_and_errors_without_device():
     assert L.atlas_rt_blas_build(None, tris.ctypes.data, 1, 0, None) == _abi.E_INVALID_ARG
 
 
-def test_trace_on_host_only_blas_is_an_error():
+def test_device_trace_on_host_only_blas_is_an_error():
+    """A host-only BLAS has no device arrays: the device entry points refuse
+    it (before touching any buffer), while the engine's CPU traversal
+    (atlas_rt_trace_closest_host) runs on its host copy."""
     verts, _ = rt_cases.soup(64, 1, 1)
     blas = atlas_rt.Blas.build(verts, host_only=True)
-    with pytest.raises(atlas_rt.AtlasRTError) as e:
-        blas.trace_closest_host(np.zeros(4, dtype=atlas_rt.RAY_DTYPE))
-    assert e.value.code == _abi.E_INVALID_ARG and "host-only" in str(e.value)
+    rays = np.zeros(4, dtype=atlas_rt.RAY_DTYPE)
+    hits = np.zeros(4, dtype=atlas_rt.HIT_DTYPE)
+    L = atlas_rt.lib()
+    st = L.atlas_rt_trace_closest_rays(blas.handle, rays.ctypes.data, hits.ctypes.data, None, 4, 0, None)
+    assert st == _abi.E_INVALID_ARG and b"host-only" in L.atlas_rt_last_error()
+    h = blas.trace_closest_host(rays)
+    assert np.all(h["prim_id"] == -1)
 
 
 def test_create_rejects_malformed_trees():

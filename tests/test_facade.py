@@ -1,7 +1,9 @@
-"""The C++ drop-in facade (atlas-engine_amd/facade/atlas_rt_volume.h) keeps
-the engine's Volume::BVH call sites source-compatible; the compiled check
-program (built by the package Makefile) runs host-only on the CPU and
-traces through the C ABI on the GPU."""
+"""The C++ drop-in facades keep the engine's call sites source-compatible:
+atlas-engine_amd/facade/atlas_rt_volume.h (Volume::BVH: builds and the
+engine's CPU traversal, with or without a device) and
+atlas_rt_raytracing.h (RayTracingWorld, PathTracingRenderer: world updates
+and rendering on the GPU).  The compiled check programs are built by the
+package Makefile."""
 import os
 import subprocess
 
@@ -9,20 +11,38 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BIN = os.path.join(ROOT, "atlas-engine_amd", "_build", "facade_test")
+RT_BIN = os.path.join(ROOT, "atlas-engine_amd", "_build", "raytracing_facade_test")
+NO_DEVICE = {"HIP_VISIBLE_DEVICES": "-1", "ROCR_VISIBLE_DEVICES": "-1"}
 
 
-def _run(env_extra):
-    assert os.path.exists(BIN), "facade_test not built (run __graft_entry__.build())"
+def _run(env_extra, binary=BIN):
+    assert os.path.exists(binary), f"{os.path.basename(binary)} not built (run __graft_entry__.build())"
     env = dict(os.environ, **env_extra)
-    p = subprocess.run([BIN], capture_output=True, text=True, env=env, timeout=120)
-    assert p.returncode == 0, f"facade_test exit {p.returncode}: {p.stdout} {p.stderr}"
+    p = subprocess.run([binary], capture_output=True, text=True, env=env, timeout=120)
+    assert p.returncode == 0, f"{os.path.basename(binary)} exit {p.returncode}: {p.stdout} {p.stderr}"
     assert "facade ok" in p.stdout
     return p.stdout
 
 
 def test_facade_host_only():
-    out = _run({"HIP_VISIBLE_DEVICES": "-1", "ROCR_VISIBLE_DEVICES": "-1"})
+    out = _run(NO_DEVICE)
+    assert "host only" in out and "hits" in out
+
+
+def test_raytracing_facade_host_only():
+    """RayTracingWorld / PathTracingRenderer call sites compile and link."""
+    out = _run(NO_DEVICE, RT_BIN)
     assert "host only" in out
+
+
+@pytest.mark.gpu
+def test_raytracing_facade_renders_on_gpu(torch_cuda):
+    """Scene::Timestep -> PathTracingRenderer::Render through the facades gives
+    the C ABI's image bit for bit; a camera move restarts the samples; realtime
+    mode resolves a lit frame."""
+    print("case raytracing facade", flush=True)
+    out = _run({}, RT_BIN)
+    assert "equals the C ABI" in out
 
 
 @pytest.mark.gpu

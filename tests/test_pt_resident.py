@@ -1,0 +1,151 @@
+"""The resident frame kernel against the bounce-by-bounce launches.
+
+By default a frame runs as one resident kernel that walks every path from
+its camera ray to its last bounce in one lane (trace, shade, shadow ray, next
+bounce).  ATLAS_RT_PT_WAVEFRONT runs the engine's dispatch shape instead: one
+closest-hit launch, one shade launch and one shadow-ray launch per bounce over
+compacted path queues.  Every path computes the same arithmetic in the same
+order either way and writes only its own pixel, so the accumulated image, the
+realtime frame, its G-buffer and the resolve must be equal bit for bit: over
+an opaque world (the closest-hit and shadow walks share one step body), over
+cut-outs, translucency and an opacity map (the transparency walks), with
+triangle lights, and in realtime mode with four samples per pixel.  The
+bounce-by-bounce image is itself pinned per pixel against the oracle by
+test_pt.py, test_pt_alpha_lights.py and test_realtime.py.
+"""
+import numpy as np
+import pytest
+
+import atlas_rt
+import pt_cases
+from atlas_rt import _abi
+from atlas_rt import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+ENV = (0.25, 0.3, 0.4)
+
+
+def _both(make, render, read):
+    out = []
+    for wavefront in (False, True):
+        pt = make(wavefront)
+        render(pt)
+        out.append(read(pt))
+        pt.close()
+    return out
+
+
+def _plain(salt=3, n_inst=48):
+    meshes, normals, mats, xf, which = pt_cases.small_c2(n_inst=n_inst, salt=salt)
+    S = pt_cases.oracle_scene(meshes, normals, mats, xf, which)
+    w_, blases = pt_cases.product_scene(meshes, normals, xf, which, S["boxes"])
+    return w_, blases, mats
+
+
+@pytest.mark.parametrize("fif", [1, 3])
+def test_resident_equals_wavefront_opaque(torch_cuda, fif):
+    print(f"case resident opaque fif={fif}", flush=True)
+    w_, blases, mats = _plain()
+    W_, H_ = 160, 96
+    cam = atlas_rt.camera((0.0, 15.0, -70.0), 0.0, 12.0, 50.0, W_ / H_)
+
+    def render(pt):
+        pt.render(cam, 3)
+        pt.render(cam, 2)
+
+    a, b = _both(lambda wf: atlas_rt.PathTracer(w_, W_, H_, 4, mats, W.sun_light(), env=ENV, frames_in_flight=fif,
+                                                wavefront=wf),
+                 render, lambda pt: pt.read().cpu().numpy())
+    assert np.all(a[..., 3] == 5) and a[..., :3].max() > 0
+    diff = np.nonzero(np.any(a.view(np.uint32) != b.view(np.uint32), axis=-1))
+    assert len(diff[0]) == 0, f"{len(diff[0])} pixels differ, first {list(zip(*diff))[:5]}"
+
+
+def test_resident_equals_wavefront_ragged_and_deep(torch_cuda):
+    """A frame size that is not a multiple of the 8x8 tiles (paths in row
+    order) and eight bounces."""
+    print("case resident ragged", flush=True)
+    w_, blases, mats = _plain(salt=7, n_inst=64)
+    W_, H_ = 101, 67
+    cam = atlas_rt.camera((0.0, 12.0, -60.0), 8.0, 10.0, 60.0, W_ / H_)
+    a, b = _both(lambda wf: atlas_rt.PathTracer(w_, W_, H_, 8, mats, W.sun_light(), env=ENV, wavefront=wf),
+                 lambda pt: pt.render(cam, 3), lambda pt: pt.read().cpu().numpy())
+    assert a.tobytes() == b.tobytes()
+
+
+def test_resident_equals_wavefront_opacity_and_lamps(torch_cuda):
+    """Cut-outs, a translucent material, an opacity map and emissive lamp
+    quads (triangle lights): the transparency walks."""
+    print("case resident opacity lamps", flush=True)
+    meshes, normals, mats, xf, which = pt_cases.small_c2(n_meshes=4, n_inst=40, salt=11)
+    qv, qn = pt_cases.quad_mesh(60.0)
+    lv, ln = pt_cases.quad_xy(0.0, size=2.0, normal_z=-1.0)
+    meshes += [qv, lv]
+    normals += [qn, ln]
+    floor = np.eye(3, 4)[None] + np.array([[[0, 0, 0, 0], [0, 0, 0, -6.0], [0, 0, 0, 0]]])
+    lamp = np.zeros((3, 3, 4))
+    for i in range(3):
+        ang = 2 * np.pi * i / 3
+        lamp[i, :, :3] = [[1, 0, 0], [0, 0, -1], [0, 1, 0]]
+        lamp[i, :, 3] = (18 * np.cos(ang), 14.0, 18 * np.sin(ang))
+    xf = np.concatenate([xf, floor, lamp])
+    which = np.concatenate([which, [len(meshes) - 2], np.full(3, len(meshes) - 1)])
+    extra = W.c2_materials({"albedo": np.full((2, 3), 0.6, np.float32), "roughness": np.ones(2, np.float32)})
+    mats = np.concatenate([mats, extra])
+    mats["opacity"][1] = 0.5
+    mats["opacity_texture"][2] = 0
+    mats["opacity"][2] = 0.9
+    emit = np.zeros((len(mats), 4), np.float32)
+    emit[-1] = (1.0, 0.9, 0.7, 30.0)
+    mats["emissive"][-1] = emit[-1, :3] ** 2.2 * emit[-1, 3]
+    g = np.random.default_rng(3)
+    S = pt_cases.oracle_scene(meshes, normals, mats, xf, which)
+    w_, blases = pt_cases.product_scene(meshes, normals, xf, which, S["boxes"])
+    for k, (b, v) in enumerate(zip(blases, meshes)):
+        op = np.full(len(v), mats["opacity"][k], np.float32)
+        if k == 0:
+            op[g.random(len(v)) < 0.3] = 0.0
+        if k == 2:
+            op[:] = -1.0
+        b.set_opacity(op, pt_cases.sphere_uvs(v))
+    W_, H_ = 96, 64
+    cam = atlas_rt.camera((0.0, 15.0, -70.0), 0.0, 12.0, 50.0, W_ / H_)
+
+    def make(wf):
+        pt = atlas_rt.PathTracer(w_, W_, H_, 4, mats, W.sun_light(radiance=(0.5, 0.5, 0.5)), env=W.C2_ENV,
+                                 wavefront=wf)
+        pt.set_textures([pt_cases.checker(8)])
+        pt.set_emissive_lights(emit)
+        return pt
+
+    a, b = _both(make, lambda pt: pt.render(cam, 3), lambda pt: pt.read().cpu().numpy())
+    assert a[..., :3].max() > 0
+    assert a.tobytes() == b.tobytes()
+
+
+def test_resident_equals_wavefront_realtime(torch_cuda):
+    print("case resident realtime", flush=True)
+    w_, blases, mats = _plain(salt=4)
+    W_, H_ = 96, 64
+    proj = atlas_rt.projection_matrix(50.0, W_ / H_)
+    poses = [((0.0, 15.0, -70.0), 0.0, 12.0), ((1.5, 15.5, -69.0), 2.0, 11.0)]
+
+    def make(wf):
+        pt = atlas_rt.PathTracer(w_, W_, H_, 3, mats, W.sun_light(), env=ENV, wavefront=wf)
+        pt.set_realtime(samples_per_frame=4)
+        return pt
+
+    def render(pt):
+        last = None
+        for f, (eye, yaw, pitch) in enumerate(poses):
+            view = atlas_rt.view_matrix(eye, yaw, pitch)
+            vp = proj @ view
+            fm = atlas_rt.frame_matrices(view, proj, vp if last is None else last, jitter=(0.3 * f - 0.1, 0.2))
+            pt.render_realtime(atlas_rt.camera(eye, yaw, pitch, 50.0, W_ / H_), fm)
+            last = vp
+
+    a, b = _both(make, render, lambda pt: [pt.read_realtime(k).cpu().numpy() for k in range(6)])
+    for k in range(6):
+        assert a[k].tobytes() == b[k].tobytes(), k
+    assert a[_abi.RT_FRAME][..., :3].max() > 0

@@ -206,12 +206,17 @@ def test_gpu_realtime_matches_oracle(torch_cuda, spf):
 
 
 @pytest.mark.gpu
-def test_gpu_realtime_after_world_update(torch_cuda):
-    """An update between realtime frames re-packs the instances (here 8 fewer
-    plus 4 new ones): each kept instance's velocity still comes from its own
-    previous-frame matrix (matched by its position in the update's instance
-    array), new instances have none, and the G-buffer equals the oracle's."""
-    print("case realtime after update", flush=True)
+@pytest.mark.parametrize("history", [False, True])
+def test_gpu_realtime_after_world_update(torch_cuda, history):
+    """An update between realtime frames re-packs the instances.  Without an
+    object history the update drops the last 8 instances and moves the others:
+    each kept instance's velocity comes from its own previous-frame matrix,
+    matched by its index in the update's array.  With one
+    (atlas_rt_world_set_history, the engine's includeObjectHistory) the update
+    also appends 4 new instances whose last matrices are their current ones,
+    and every instance's velocity comes from the matrix the caller gave.  The
+    G-buffer equals the oracle's bit for bit."""
+    print(f"case realtime after update history={history}", flush=True)
     S = _scene()
     W_, H_, bounces = 96, 64, 2
     pt = atlas_rt.PathTracer(S["world"], W_, H_, bounces, S["mats"], W.sun_light(), env=ENV)
@@ -225,16 +230,20 @@ def test_gpu_realtime_after_world_update(torch_cuda):
     g = W.rng(2, 906)
     n_old = len(S["descs"])
     keep = n_old - 8
-    xf2 = np.concatenate([S["xf"][:keep], W.random_affine(g, 4, scale=(1.0, 4.0), extent=30.0, height=8.0)])
+    added = 4 if history else 0
+    xf2 = np.concatenate([S["xf"][:keep], W.random_affine(g, 4, scale=(1.0, 4.0), extent=30.0, height=8.0)[:added]])
     xf2[:keep, :, 3] += g.uniform(-0.5, 0.5, (keep, 3))
-    which2 = np.concatenate([S["which"][:keep], g.integers(0, len(S["meshes"]), 4)])
+    which2 = np.concatenate([S["which"][:keep], g.integers(0, len(S["meshes"]), 4)[:added]])
     descs2 = atlas_rt.instance_descs(xf2, which2)
     descs2["material_offset"] = which2.astype(np.int32)
     S["world"].update(descs2, S["blases"])
+    last_caller = np.stack([(S["descs"][r] if r < keep else descs2[r])["object_to_world"].reshape(12)
+                            for r in range(len(descs2))]).astype(np.float32)
+    if history:
+        S["world"].set_history(last_caller)
     S["cur_descs"] = descs2
     _, _, order = S["world"].export()
-    last_xform = np.stack([(S["descs"][r] if r < keep else descs2[r])["object_to_world"].reshape(-1)
-                           for r in order]).astype(np.float32)
+    last_xform = last_caller[order]
     pt.render_realtime(cam, atlas_rt.frame_matrices(view, proj, vp))
     got = {k: pt.read_realtime(k).cpu().numpy() for k in range(6)}
     frame, depth, vel, nrm, mat = _oracle_frame(S, cam, atlas_rt.frame_matrices(view, proj, vp), W_, H_, bounces, 1,
