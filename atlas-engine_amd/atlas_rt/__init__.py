@@ -457,6 +457,16 @@ def _blas_set_tangents(self, tangents):
 Blas.set_tangents = _blas_set_tangents
 
 
+def _blas_set_triangles(self, records):
+    """The engine's GPUTriangle records (GPU_TRIANGLE_DTYPE, BVH slot order)
+    as the path tracer's shading data (atlas_rt_blas_set_triangles)."""
+    r = np.ascontiguousarray(records, dtype=_abi.GPU_TRIANGLE_DTYPE)
+    check(lib().atlas_rt_blas_set_triangles(self._h, _ptr(r), len(r)))
+
+
+Blas.set_triangles = _blas_set_triangles
+
+
 def camera(eye, yaw_deg, pitch_deg, fov_y_deg, aspect, near=0.1):
     """atlas_rt_camera from an eye point and yaw/pitch (y up), built like
     CameraComponent::GetFrustumCorners' near-plane corners 4, 5, 6."""

@@ -84,8 +84,14 @@ MATERIAL_DTYPE = np.dtype([
     ("metalness", "<f4"), ("ao", "<f4"), ("reflectance", "<f4"), ("two_sided", "<i4"),
     ("opacity_texture", "<i4"), ("invert_uvs", "<i4"), ("id", "<i4"), ("normal_scale", "<f4"),
     ("base_color_texture", "<i4"), ("roughness_texture", "<i4"), ("metalness_texture", "<i4"),
-    ("ao_texture", "<i4"), ("normal_texture", "<i4"), ("reserved", "<i4", 3),
+    ("ao_texture", "<i4"), ("normal_texture", "<i4"), ("use_vertex_colors", "<i4"), ("reserved", "<i4", 2),
 ])
+# GPUTriangle (96 B): v0..v2 = (vertex, 10-10-10-2 normal bits), d0 = (half2 uv
+# bits x3, material index bits), d1 = (tangent bits, bitangent bits, end of
+# leaf +-1, 0), d2 = (RGBA8 colour bits x3, opacity)
+GPU_TRIANGLE_DTYPE = np.dtype([("v0", "<f4", 4), ("v1", "<f4", 4), ("v2", "<f4", 4), ("d0", "<f4", 4),
+                               ("d1", "<f4", 4), ("d2", "<f4", 4)])
+assert GPU_TRIANGLE_DTYPE.itemsize == 96
 TEXTURE_FIELDS = ("base_color_texture", "opacity_texture", "roughness_texture", "metalness_texture", "ao_texture",
                   "normal_texture")
 LIGHT_DTYPE = np.dtype([("direction", "<f4", 3), ("type", "<u4"), ("radiance", "<f4", 3), ("reserved", "<f4")])
@@ -193,6 +199,7 @@ PROTOTYPES = {
     "atlas_rt_batch_graph_capture": (status, [vp, vp, vp, vp, vp]),
     "atlas_rt_batch_graph_launch": (status, [vp, vp]),
     "atlas_rt_aabb_tree_build": (status, [vp, u32, u32, vp, C.POINTER(u32), vp, vp, C.POINTER(u32)]),
+    "atlas_rt_blas_set_triangles": (status, [vp, vp, u32]),
     "atlas_rt_blas_set_shading": (status, [vp, vp, vp]),
     "atlas_rt_blas_set_opacity": (status, [vp, vp, vp]),
     "atlas_rt_blas_set_tangents": (status, [vp, vp]),

@@ -8,6 +8,7 @@
 
 #include "builder.h"
 #include "rt_hostmath.h"
+#include "packing.h"
 #include "rt_internal.h"
 #include "wide.h"
 
@@ -279,6 +280,7 @@ atlas_rt_status atlas_rt_blas_destroy(atlas_rt_blas* b) {
     if (b->d_shade) hipFree(b->d_shade);
     if (b->d_uv) hipFree(b->d_uv);
     if (b->d_tangent) hipFree(b->d_tangent);
+    if (b->d_color) hipFree(b->d_color);
     b->spill.release();
     b->flags.release();
     b->timing.release();
@@ -323,9 +325,77 @@ atlas_rt_status atlas_rt_blas_set_shading(atlas_rt_blas* b, const float* normals
     }
     if (b->d_shade) hipFree(b->d_shade);
     if (b->d_uv) hipFree(b->d_uv);
+    if (b->d_color) hipFree(b->d_color);
     b->d_shade = reinterpret_cast<float4*>(d);
     b->d_uv = nullptr;
+    b->d_color = nullptr;
     b->has_alpha = false;
+    b->shade_normals.swap(host_n);
+    b->shade_mat.swap(host_m);
+    return ATLAS_RT_OK;
+}
+
+atlas_rt_status atlas_rt_blas_set_triangles(atlas_rt_blas* b, const atlas_rt_gpu_triangle* tris, uint32_t count) {
+    if (!b || !tris) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_blas_set_triangles: NULL argument");
+    if (!b->on_device) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_blas_set_triangles: BLAS was built host-only");
+    const size_t n = b->tris.size();
+    if (count != n)
+        return fail(ATLAS_RT_E_SIZE_MISMATCH, "atlas_rt_blas_set_triangles: " + std::to_string(count) +
+                                                  " records for " + std::to_string(n) + " BVH slots");
+    // Per slot: shading (n0 + material bits, n1 + opacity, n2), texture
+    // coordinates (uv0 uv1, uv2), tangent frame (t, bt) and vertex colours
+    // (three RGBA), the layouts the kernels read.
+    std::vector<float> shade(12 * n), uv(8 * n, 0.0f), tan(8 * n, 0.0f), col(12 * n);
+    std::vector<float> host_n(9 * n);
+    std::vector<int32_t> host_m(n);
+    bool alpha = false;
+    for (size_t i = 0; i < n; i++) {
+        const atlas_rt_gpu_triangle& r = tris[i];
+        if ((r.d1[2] > 0.0f) != (b->tris[i].v0[3] > 0.0f))
+            return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_blas_set_triangles: record " + std::to_string(i) +
+                                                    "'s end-of-leaf flag is not its BVH slot's (records must be in "
+                                                    "BVH slot order)");
+        const float* v[3] = {r.v0, r.v1, r.v2};
+        const float* bv[3] = {b->tris[i].v0, b->tris[i].v1, b->tris[i].v2};
+        for (int k = 0; k < 3; k++)
+            if (std::memcmp(v[k], bv[k], 12) != 0)
+                return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_blas_set_triangles: record " + std::to_string(i) +
+                                                        "'s vertices are not its BVH slot's triangle");
+        float* o = &shade[12 * i];
+        for (int k = 0; k < 3; k++) {
+            float nk[3];
+            unpack_unit_vector(float_bits(v[k][3]), nk);
+            std::memcpy(&o[4 * k], nk, 12);
+            std::memcpy(&host_n[9 * i + 3 * k], nk, 12);
+        }
+        const int32_t mat = int32_t(float_bits(r.d0[3]));
+        std::memcpy(&o[3], &mat, 4);
+        o[7] = r.d2[3];
+        o[11] = 0.0f;
+        host_m[i] = mat;
+        alpha = alpha || r.d2[3] != 1.0f;
+        for (int k = 0; k < 3; k++) unpack_half2(float_bits(r.d0[k]), &uv[8 * i + 2 * k]);
+        unpack_unit_vector(float_bits(r.d1[0]), &tan[8 * i]);
+        unpack_unit_vector(float_bits(r.d1[1]), &tan[8 * i + 4]);
+        for (int k = 0; k < 3; k++) unpack_unorm4x8(float_bits(r.d2[k]), &col[12 * i + 4 * k]);
+    }
+    float4* dev[4] = {nullptr, nullptr, nullptr, nullptr};
+    const std::vector<float>* src[4] = {&shade, &uv, &tan, &col};
+    for (int a = 0; a < 4; a++) {
+        hipError_t e = hipMalloc(reinterpret_cast<void**>(&dev[a]), src[a]->size() * sizeof(float));
+        if (e == hipSuccess) e = hipMemcpy(dev[a], src[a]->data(), src[a]->size() * sizeof(float), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            for (float4* d : dev)
+                if (d) hipFree(d);
+            return hip_fail(e, "atlas_rt_blas_set_triangles");
+        }
+    }
+    float4** mine[4] = {&b->d_shade, &b->d_uv, &b->d_tangent, &b->d_color};
+    for (int a = 0; a < 4; a++) {
+        if (*mine[a]) hipFree(*mine[a]);
+        *mine[a] = dev[a];
+    }
+    b->has_alpha = alpha;
     b->shade_normals.swap(host_n);
     b->shade_mat.swap(host_m);
     return ATLAS_RT_OK;

@@ -24,7 +24,7 @@ atlas_rt_status dev_alloc(T*& p, size_t count) {
 // Slot k's buffers sized by the paths per frame: both path queues, hits,
 // shadow rays, radiance per ray ID, the traversal stack spill and the
 // binning buffers.
-atlas_rt_status alloc_slot_rays(PathTracer& P, uint32_t k, uint32_t rays) {
+atlas_rt_status alloc_slot_rays(PathTracer& P, uint32_t k, uint32_t rays, uint32_t frames = 1) {
     FrameSlot& S = P.slot[k];
     void* old[] = {S.paths[0], S.paths[1], S.hits, S.shadow, S.Lpix, S.binned};
     for (void* p : old)
@@ -36,7 +36,7 @@ atlas_rt_status alloc_slot_rays(PathTracer& P, uint32_t k, uint32_t rays) {
     if (st == ATLAS_RT_OK) st = dev_alloc(S.paths[1], 3 * size_t(rays));
     if (st == ATLAS_RT_OK) st = dev_alloc(S.hits, rays);
     if (st == ATLAS_RT_OK) st = dev_alloc(S.shadow, 3 * size_t(rays));
-    if (st == ATLAS_RT_OK) st = dev_alloc(S.Lpix, rays);
+    if (st == ATLAS_RT_OK) st = dev_alloc(S.Lpix, size_t(rays) * frames);
     if (st == ATLAS_RT_OK) st = S.spill.ensure(slot_spill_bytes(rays, P.cap));
     if (st == ATLAS_RT_OK && P.binning) st = dev_alloc(S.binned, 3 * size_t(rays));
     if (st == ATLAS_RT_OK && P.binning) st = S.bin_ws.ensure(bin_sort_workspace(rays));
@@ -46,11 +46,11 @@ atlas_rt_status alloc_slot_rays(PathTracer& P, uint32_t k, uint32_t rays) {
 // A slot's frame block, flag words, stream and events.
 atlas_rt_status alloc_slot(PathTracer& P, uint32_t k, uint32_t rays) {
     FrameSlot& S = P.slot[k];
-    atlas_rt_status st = dev_alloc(S.ctl, 1);
+    atlas_rt_status st = dev_alloc(S.ctl, kBatchFrames);
     if (st == ATLAS_RT_OK) st = S.flags.ensure(256);
-    if (st == ATLAS_RT_OK) st = alloc_slot_rays(P, k, rays);
+    if (st == ATLAS_RT_OK) st = alloc_slot_rays(P, k, rays, P.batch);
     if (st != ATLAS_RT_OK) return st;
-    ATLAS_HIP_TRY(hipMemset(S.ctl, 0, sizeof(FrameCtl)));
+    ATLAS_HIP_TRY(hipMemset(S.ctl, 0, kBatchFrames * sizeof(FrameCtl)));
     ATLAS_HIP_TRY(hipMemset(S.flags.ptr, 0, 256));
     ATLAS_HIP_TRY(hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking));
     ATLAS_HIP_TRY(hipEventCreateWithFlags(&S.ev_begin, hipEventDisableTiming));
@@ -99,7 +99,8 @@ void free_realtime(PathTracer& P) {
 
 void free_all(PathTracer& P) {
     void* ptrs[] = {P.ctl, P.accum, const_cast<float4**>(P.shade_table), const_cast<float4**>(P.uv_table),
-                    const_cast<float4**>(P.tangent_table), P.materials, P.lights, P.tex, P.texels};
+                    const_cast<float4**>(P.tangent_table), const_cast<float4**>(P.color_table), P.materials,
+                    P.lights, P.tex, P.texels};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     free_cache_buffers(P);
@@ -287,29 +288,33 @@ atlas_rt_status atlas_rt::pt::bind_world(PathTracer& P) {
     ATLAS_HIP_TRY(hipDeviceSynchronize());
     const size_t nb = W.blas_handles.size();
     if (nb != P.table_count) {
-        const void* old[] = {P.shade_table, P.uv_table, P.tangent_table};
+        const void* old[] = {P.shade_table, P.uv_table, P.tangent_table, P.color_table};
         for (const void* p : old)
             if (p) (void)hipFree(const_cast<void*>(p));
-        P.shade_table = P.uv_table = P.tangent_table = nullptr;
+        P.shade_table = P.uv_table = P.tangent_table = P.color_table = nullptr;
         P.table_count = 0;
         atlas_rt_status st = dev_alloc(P.shade_table, nb);
         if (st == ATLAS_RT_OK) st = dev_alloc(P.uv_table, nb);
         if (st == ATLAS_RT_OK) st = dev_alloc(P.tangent_table, nb);
+        if (st == ATLAS_RT_OK) st = dev_alloc(P.color_table, nb);
         if (st != ATLAS_RT_OK) return st;
         P.table_count = nb;
     }
-    std::vector<const float4*> shade(nb), uv(nb), tangent(nb);
+    std::vector<const float4*> shade(nb), uv(nb), tangent(nb), color(nb);
     bool alpha = false;
     for (size_t k = 0; k < nb; k++) {
         shade[k] = W.blas_handles[k]->d_shade;
         uv[k] = W.blas_handles[k]->d_uv;
         tangent[k] = W.blas_handles[k]->d_tangent;
+        color[k] = W.blas_handles[k]->d_color;
         alpha = alpha || W.blas_handles[k]->has_alpha;
     }
     ATLAS_HIP_TRY(hipMemcpy(const_cast<float4**>(P.shade_table), shade.data(), nb * sizeof(float4*),
                             hipMemcpyHostToDevice));
     ATLAS_HIP_TRY(hipMemcpy(const_cast<float4**>(P.uv_table), uv.data(), nb * sizeof(float4*), hipMemcpyHostToDevice));
     ATLAS_HIP_TRY(hipMemcpy(const_cast<float4**>(P.tangent_table), tangent.data(), nb * sizeof(float4*),
+                            hipMemcpyHostToDevice));
+    ATLAS_HIP_TRY(hipMemcpy(const_cast<float4**>(P.color_table), color.data(), nb * sizeof(float4*),
                             hipMemcpyHostToDevice));
     P.alpha = alpha;
     P.world_dev = W.dev();
@@ -323,6 +328,7 @@ atlas_rt_status atlas_rt::pt::bind_world(PathTracer& P) {
     A.xform = P.xform;
     A.shade = P.shade_table;
     A.tangent = P.tangent_table;
+    A.color = P.color_table;
     A.alpha.shade = P.shade_table;
     A.alpha.uv = P.uv_table;
     if (P.realtime && P.rt_last_xform) {
@@ -400,6 +406,7 @@ atlas_rt_status atlas_rt_pathtracer_create(const atlas_rt_world* world, const at
     PathTracer& P = pt->P;
     P.binning = (params->flags & ATLAS_RT_PT_RAY_BINNING) != 0;
     P.wavefront = (params->flags & ATLAS_RT_PT_WAVEFRONT) != 0;
+    P.batch = (P.binning || P.wavefront) ? 1u : kBatchFrames;
     P.slots = slots ? slots : kDefaultFrameSlots;
     P.width = params->width;
     P.height = params->height;
@@ -486,14 +493,17 @@ atlas_rt_status atlas_rt_pathtracer_render(atlas_rt_pathtracer* pt, const atlas_
     }
     ATLAS_HIP_TRY(hipEventRecord(P.ev_start, s));
     uint32_t used = 0;
-    for (uint32_t f = 0; f < frames; f++) {
+    // resident frames go in batches of up to P.batch per launch
+    const uint32_t per = resident_frame(P) && !P.realtime ? P.batch : 1u;
+    for (uint32_t f = 0; f < frames;) {
         const uint32_t k = P.next_slot % n;
         if (!(used & (1u << k))) {
             ATLAS_HIP_TRY(hipStreamWaitEvent(P.slot[k].stream, P.ev_start, 0));
             used |= 1u << k;
         }
         order.index = f;
-        atlas_rt_status st = launch_frame(P, cam, k, order);
+        const uint32_t nb = std::min(per, frames - f);
+        atlas_rt_status st = launch_frame(P, cam, k, order, nb);
         if (st != ATLAS_RT_OK) {
             // leave `s` behind whatever was queued, then report
             for (uint32_t j = 0; j < n; j++)
@@ -506,7 +516,8 @@ atlas_rt_status atlas_rt_pathtracer_render(atlas_rt_pathtracer* pt, const atlas_
         P.hist_prev = order.prev;
         P.last_slot = k;
         P.next_slot = (k + 1) % n;
-        P.host_samples++;
+        P.host_samples += int32_t(nb);
+        f += nb;
     }
     for (uint32_t k = 0; k < n; k++)
         if (used & (1u << k)) ATLAS_HIP_TRY(hipStreamWaitEvent(s, P.slot[k].ev_acc, 0));
@@ -531,7 +542,8 @@ atlas_rt_status atlas_rt_pathtracer_set_realtime(atlas_rt_pathtracer* pt, const 
     P.rays = rays;
     P.next_slot = P.last_slot = 0;
     P.hist_prev = P.hist_prev2 = -1;
-    atlas_rt_status st = alloc_slot_rays(P, 0, rays);
+    // realtime frames run one at a time; progressive ones in batches
+    atlas_rt_status st = alloc_slot_rays(P, 0, rays, params ? 1u : P.batch);
     if (st != ATLAS_RT_OK || !params) return st;
     const size_t inst = P.world->instances.size();
     if (st == ATLAS_RT_OK) st = dev_alloc(P.rt_frame, pixels);

@@ -16,6 +16,7 @@ struct ShadeArgs {
     const float4* xform;          // object-to-world rows, 3 float4 per packed instance
     const float4* const* shade;   // per BLAS: 3 float4 per slot (n0 + material, n1 + opacity, n2)
     const float4* const* tangent; // per BLAS: 2 float4 per slot (tangent, bitangent), or null
+    const float4* const* color;   // per BLAS: 3 float4 per slot (vertex colours), or null
     const Material* materials;
     const Light* lights;
     int32_t light_count;
@@ -62,6 +63,9 @@ struct FrameSlot {
 
 constexpr uint32_t kMaxFrameSlots = 4;
 constexpr uint32_t kDefaultFrameSlots = 3;
+// Progressive frames one resident launch renders: the launch's tail (its
+// slowest paths) is paid once per batch, not once per frame.
+constexpr uint32_t kBatchFrames = 8;
 
 // The resident frame kernel's launch shape: 256-lane groups, four per CU
 // (four waves per SIMD), a 24-entry LDS stack per lane; deeper entries spill
@@ -105,6 +109,7 @@ struct PathTracer {
     const float4** shade_table = nullptr;
     const float4** uv_table = nullptr;
     const float4** tangent_table = nullptr;
+    const float4** color_table = nullptr;
     Material* materials = nullptr;
     uint32_t material_count = 0;
     Light* lights = nullptr;
@@ -122,6 +127,9 @@ struct PathTracer {
     // ATLAS_RT_PT_WAVEFRONT: bounce-by-bounce launches even where the
     // resident frame kernel could run
     bool wavefront = false;
+    // progressive frames per resident launch (each slot holds this many
+    // frames' control blocks and radiance buffers)
+    uint32_t batch = 1;
     ShadeArgs shade_args{};       // every field but the slot's buffers
     int32_t host_samples = 0;
     uint32_t rays = 0;            // paths per frame: pixels, times the samples per frame in realtime mode
@@ -154,8 +162,12 @@ struct FrameOrder {
     uint32_t index = 0;
 };
 
-// One progressive frame in slot `k` on that slot's stream.
-atlas_rt_status launch_frame(PathTracer& P, const Camera& cam, uint32_t k, const FrameOrder& order);
+// Progressive frames in slot `k` on that slot's stream: `frames` of them in
+// one resident launch when resident_frame(P) and frames <= P.batch, else one.
+atlas_rt_status launch_frame(PathTracer& P, const Camera& cam, uint32_t k, const FrameOrder& order, uint32_t frames);
+// Whether frames run as the resident frame kernel (plain path tracing: the
+// radiance cache and ray binning need the bounce-by-bounce queues).
+bool resident_frame(const PathTracer& P);
 // query_lag 1: both network snapshots from the live network, on `s`.
 atlas_rt_status refresh_snapshots(PathTracer& P, hipStream_t s);
 // The temporal resolve's inputs and output (pt_temporal.hip,

@@ -8,8 +8,12 @@
 //   sampling brdf/brdfSample.hsh:8-90 (cosine hemisphere, GGX VNDF)
 //   bounce   data/shader/pathtracer/rayHit.csh:160-325 (EvaluateBounce,
 //            EvaluateDirectLight, EvaluateIndirectLight)
-// Transcendentals come from the device math library, so results agree with
-// the CPU restatement to rounding, not bit for bit.
+// GLSL leaves the precision of sin, cos and pow to the implementation, so
+// the restatement fixes them: sin / cos of the sample angles by the Cephes
+// single-precision method (S. L. Moshier, Cephes Math Library: Cody-Waite
+// reduction by pi/4 in three parts, degree-7 / degree-8 polynomials on
+// [-pi/4, pi/4]) and pow(x, 5) as (x^2)^2 * x.  The CPU restatement uses the
+// same operations, so both round identically.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -73,17 +77,44 @@ __device__ __forceinline__ float rnd_pair(float a, float b) {
     return float_construct(jhash(__float_as_uint(a) ^ jhash(__float_as_uint(b))));
 }
 
+// ---- sin / cos / pow(x, 5) (see the header) --------------------------------
+// x >= 0 and moderate (the sample angles 2 pi u, u in [0, 1)).
+__device__ __forceinline__ void sin_cos(float x, float& sn, float& cs) {
+    int j = int(x * 1.27323954473516f);  // 4 / pi
+    float y = float(j);
+    if (j & 1) {
+        j += 1;
+        y += 1.0f;
+    }
+    j &= 7;
+    const bool swap = j == 2 || j == 6;
+    const bool sneg = j > 3;
+    const bool cneg = j == 2 || j == 4;
+    const float z = ((x - y * 0.78515625f) - y * 2.4187564849853515625e-4f) - y * 3.77489497744594108e-8f;
+    const float zz = z * z;
+    const float sp = ((-1.9515295891e-4f * zz + 8.3321608736e-3f) * zz - 1.6666654611e-1f) * zz * z + z;
+    const float cp = ((2.443315711809948e-5f * zz - 1.388731625493765e-3f) * zz + 4.166664568298827e-2f) * zz * zz -
+                     0.5f * zz + 1.0f;
+    const float s0 = swap ? cp : sp, c0 = swap ? sp : cp;
+    sn = sneg ? -s0 : s0;
+    cs = cneg ? -c0 : c0;
+}
+__device__ __forceinline__ float pow5(float x) {
+    const float x2 = x * x;
+    return (x2 * x2) * x;
+}
+
 // ---- BRDF (brdf.hsh) ------------------------------------------------------
 __device__ __forceinline__ V fresnel_schlick(V F0, float F90, float c) {
-    const float p = powf(1.0f - c, 5.0f);
+    const float p = pow5(1.0f - c);
     return F0 + (V{F90, F90, F90} - F0) * p;
 }
 __device__ __forceinline__ float disney_diffuse(float NdotV, float NdotL, float LdotH, float lr) {
     const float energyBias = mixf(0.0f, 0.5f, lr);
     const float energyFactor = mixf(1.0f, 1.0f / 1.51f, lr);
     const float FD90 = energyBias + 2.0f * LdotH * LdotH * lr;
-    const float lightScatter = 1.0f + (FD90 - 1.0f) * powf(1.0f - NdotL, 5.0f);
-    const float viewScatter = 1.0f + (FD90 - 1.0f) * powf(1.0f - NdotV, 5.0f);
+    const float lightScatter = 1.0f + (FD90 - 1.0f) * pow5(1.0f - NdotL);
+    const float viewScatter = 1.0f + (FD90 - 1.0f) * pow5(1.0f - NdotV);
     return lightScatter * viewScatter * energyFactor;
 }
 __device__ __forceinline__ float vis_smith_separable(float c, float alpha) {
@@ -158,7 +189,9 @@ __device__ __forceinline__ void tangent_frame(V N, V& t, V& b, bool normalize_b)
 __device__ __forceinline__ BrdfSample sample_diffuse(Surface& s, float u0, float u1) {
     const float theta = sqrtf(u0);
     const float phi = 2.0f * kPI * u1;
-    const V Ll = V{theta * cosf(phi), theta * sinf(phi), sqrtf(1.0f - u0)};
+    float sp, cp;
+    sin_cos(phi, sp, cp);
+    const V Ll = V{theta * cp, theta * sp, sqrtf(1.0f - u0)};
     V t, b;
     tangent_frame(s.N, t, b, false);
     s.L = normalize(t * Ll.x + b * Ll.y + s.N * Ll.z);
@@ -174,7 +207,8 @@ __device__ __forceinline__ BrdfSample sample_diffuse(Surface& s, float u0, float
 __device__ __forceinline__ V ggx_vndf(float x0, float x1, V Vt, float alpha) {
     const V Vh = normalize(V{Vt.x * alpha, Vt.y * alpha, Vt.z});
     const float phi = kPI * 2.0f * x0;
-    const float x = cosf(phi), y = sinf(phi);
+    float y, x;
+    sin_cos(phi, y, x);
     const float z = (1.0f - x1) * (1.0f + Vh.z) + -Vh.z;
     const float r = sqrtf(clampf(1.0f - z * z, 0.0f, 1.0f));
     const V c = V{x * r, y * r, z};

@@ -52,25 +52,30 @@ __device__ __forceinline__ uint32_t pcg32_next(uint64_t& state) {
 // seeds come from the master block in frame order; the slot's block holds
 // this frame's copy of the sample and frame counts.  Progressive frames
 // (advance = 1) count themselves; realtime frames do not.
-__global__ void k_frame_begin(FrameCtl* master, FrameCtl* ctl, int32_t bounces, uint32_t pixels, uint32_t advance) {
+// Frames f = 0 .. frames - 1 of one launch take ctl[f], in frame order.
+__global__ void k_frame_begin(FrameCtl* master, FrameCtl* ctl, int32_t bounces, uint32_t pixels, uint32_t advance,
+                              uint32_t frames) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     uint64_t state = master->pcg_state;
-    for (int32_t b = 0; b <= bounces && b < 16; b++)
-        ctl->seeds[b] = float(pcg32_next(state)) / float(0xFFFFFFFFu) * 0.999999f;
-    master->pcg_state = state;
-    ctl->sample_count = master->sample_count;
-    ctl->frame = master->frame;
-    if (advance) {
-        master->sample_count += 1;
-        master->frame += 1;
+    for (uint32_t f = 0; f < frames; f++) {
+        FrameCtl* c = ctl + f;
+        for (int32_t b = 0; b <= bounces && b < 16; b++)
+            c->seeds[b] = float(pcg32_next(state)) / float(0xFFFFFFFFu) * 0.999999f;
+        c->sample_count = master->sample_count;
+        c->frame = master->frame;
+        if (advance) {
+            master->sample_count += 1;
+            master->frame += 1;
+        }
+        c->paths[0] = pixels;
+        for (int i = 1; i < 17; i++) c->paths[i] = 0;
+        for (int i = 0; i < 16; i++) c->shadows[i] = 0;
+        c->train_paths = 0;
+        c->queries = 0;
+        c->train_samples = 0;
+        c->pull = 0;
     }
-    ctl->paths[0] = pixels;
-    for (int i = 1; i < 17; i++) ctl->paths[i] = 0;
-    for (int i = 0; i < 16; i++) ctl->shadows[i] = 0;
-    ctl->train_paths = 0;
-    ctl->queries = 0;
-    ctl->train_samples = 0;
-    ctl->pull = 0;
+    master->pcg_state = state;
 }
 
 // Stream compaction for a whole block: one LDS atomic per wave and one
@@ -325,9 +330,8 @@ struct ShadeOut {
 // EvaluateBounce of one path: p0 = (origin, ray ID bits), p1 = (direction,
 // training index bits), p2 = (throughput, 0); h = (t bits, hit slot or -1,
 // instance slot, 0).
-__device__ __forceinline__ void shade_path(const ShadeArgs& A, int32_t bounce, float4 p0, float4 p1, float4 p2, int4 h,
-                                           ShadeOut& out) {
-    FrameCtl* ctl = A.ctl;
+__device__ __forceinline__ void shade_path(const ShadeArgs& A, const FrameCtl* ctl, float4* Lpix, int32_t bounce,
+                                           float4 p0, float4 p1, float4 p2, int4 h, ShadeOut& out) {
     // ray ID: the pixel in progressive mode, pixel * samples per frame +
     // sample in realtime mode (ray.ID, rayGen.csh:45)
     const uint32_t id = __float_as_uint(p0.w);
@@ -337,7 +341,7 @@ __device__ __forceinline__ void shade_path(const ShadeArgs& A, int32_t bounce, f
     const V thr_in = mk(p2.x, p2.y, p2.z);
     V thr = thr_in;
     const V ro = mk(p0.x, p0.y, p0.z), rd = mk(p1.x, p1.y, p1.z);
-    float4 acc = A.Lpix[id];
+    float4 acc = Lpix[id];
     V add = mk(0.0f, 0.0f, 0.0f);
     V no = ro, nd = rd;
     const bool gbuffer = A.rt.enabled && bounce == 0 && id % A.rt.spf == 0;
@@ -375,6 +379,14 @@ __device__ __forceinline__ void shade_path(const ShadeArgs& A, int32_t bounce, f
         const float bs = dot(q, s) / den;
         const float bt = dot(p, rd) / den;
         const float br = 1.0f - bs - bt;
+        // vertex colours (surface.hsh:84, 98): r c0 + s c1 + t c2, rgb only
+        const float4* cp = A.color ? A.color[mesh] : nullptr;
+        if (m.use_vertex_colors > 0 && cp) {
+            const float4 c0 = cp[3 * size_t(slot)], c1 = cp[3 * size_t(slot) + 1], c2 = cp[3 * size_t(slot) + 2];
+            m.base[0] *= (br * c0.x + bs * c1.x) + bt * c2.x;
+            m.base[1] *= (br * c0.y + bs * c1.y) + bt * c2.y;
+            m.base[2] *= (br * c0.z + bs * c1.z) + bt * c2.z;
+        }
         // material maps (surface.hsh:86-124) at the texture coordinate of
         // these barycentrics
         const AlphaArgs& TX = A.alpha;
@@ -586,7 +598,7 @@ __device__ __forceinline__ void shade_path(const ShadeArgs& A, int32_t bounce, f
     acc.x += add.x;
     acc.y += add.y;
     acc.z += add.z;
-    A.Lpix[id] = acc;
+    Lpix[id] = acc;
     if (train) A.cache.rec_add[rec] = make_float4(add.x, add.y, add.z, 0.0f);
     const float energy = (thr.x + thr.y) + thr.z;
     if (energy == 0.0f || bounce == A.bounces) return;
@@ -597,8 +609,8 @@ __device__ __forceinline__ void shade_path(const ShadeArgs& A, int32_t bounce, f
 }
 
 __device__ __forceinline__ void shade_one(const ShadeArgs& A, int32_t bounce, uint32_t i, ShadeOut& out) {
-    shade_path(A, bounce, A.paths_in[3 * size_t(i)], A.paths_in[3 * size_t(i) + 1], A.paths_in[3 * size_t(i) + 2],
-               A.hits[i], out);
+    shade_path(A, A.ctl, A.Lpix, bounce, A.paths_in[3 * size_t(i)], A.paths_in[3 * size_t(i) + 1],
+               A.paths_in[3 * size_t(i) + 2], A.hits[i], out);
 }
 
 __global__ __launch_bounds__(kShadeBlock) void k_shade(ShadeArgs A, int32_t bounce) {
@@ -721,7 +733,8 @@ __device__ __forceinline__ uint32_t tile_pixel(uint32_t k, uint32_t width, uint3
 struct FrameGen {
     Camera cam;
     uint32_t width, height;
-    uint32_t rays;        // paths this frame
+    uint32_t rays;        // paths per frame
+    uint32_t frames;      // frames of this launch: frame f's paths follow frame f - 1's
     uint32_t spf;         // realtime: samples per pixel (ray ID = pixel * spf + sample); 0: progressive
     float jx, jy;         // realtime camera jitter
 };
@@ -733,12 +746,10 @@ __global__ __launch_bounds__(kFrameBlock, 4) void k_frame_persistent(ShadeArgs A
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     const LaneStack<kFrameBlock, kFrameLds> S{lds, spill, tid, blockIdx.x * kFrameBlock + tid,
                                               gridDim.x * kFrameBlock};
-    FrameCtl* ctl = A.ctl;
+    FrameCtl* const ctl0 = A.ctl;  // frame f's seeds and counts in ctl0[f]; ctl0[0].pull counts paths
     const WorldDev W = A.world;
     const AlphaArgs& AA = A.alpha;
-    // progressive jitter: the same for every pixel of the frame (rayGen.csh:33-34)
-    const float sc = float(ctl->sample_count);
-    const float pjx = rnd_pair(sc, 0.0f), pjy = rnd_pair(sc, 1.0f);
+    const uint32_t total = G.rays * G.frames;
     const V o = mk(G.cam.origin[0], G.cam.origin[1], G.cam.origin[2]);
     const V rgt = mk(G.cam.right[0], G.cam.right[1], G.cam.right[2]);
     const V btm = mk(G.cam.bottom[0], G.cam.bottom[1], G.cam.bottom[2]);
@@ -754,6 +765,8 @@ __global__ __launch_bounds__(kFrameBlock, 4) void k_frame_persistent(ShadeArgs A
     float4 shc = p0;     // the shadow ray's light before visibility
     bool pending = false;  // the path continues after its shadow ray (p0..p2 hold its next ray)
     bool exhausted = false;
+    const FrameCtl* ctl = ctl0;  // the frame of the lane's path
+    float4* Lpix = A.Lpix;       // that frame's radiance per ray ID
 
     for (;;) {
         if (!exhausted) {
@@ -761,19 +774,24 @@ __global__ __launch_bounds__(kFrameBlock, 4) void k_frame_persistent(ShadeArgs A
             if (idle) {
                 const uint32_t n = uint32_t(__popcll(idle));
                 uint32_t base = 0;
-                if (lane == 0) base = atomicAdd(&ctl->pull, n);
+                if (lane == 0) base = atomicAdd(&ctl0->pull, n);
                 base = __shfl(base, 0, 64);
                 if (phase == 0) {
-                    const uint32_t k = base + uint32_t(__popcll(idle & ((1ull << lane) - 1ull)));
-                    if (k < G.rays) {
+                    const uint32_t kk = base + uint32_t(__popcll(idle & ((1ull << lane) - 1ull)));
+                    if (kk < total) {
+                        const uint32_t f = kk / G.rays, k = kk - f * G.rays;
+                        ctl = ctl0 + f;
+                        Lpix = A.Lpix + size_t(f) * G.rays;
                         // rayGen.csh:25-90, as k_raygen / k_raygen_rt write slot k
                         uint32_t id;
                         V dir;
                         if (G.spf == 0) {
                             const uint32_t pixel = tile_pixel(k, G.width, G.height);
                             const uint32_t px = pixel % G.width, py = pixel / G.width;
-                            const float cx = (float(px) + pjx) / float(G.width);
-                            const float cy = (float(py) + pjy) / float(G.height);
+                            // the frame's jitter, the same for every pixel (rayGen.csh:33-34)
+                            const float sc = float(ctl->sample_count);
+                            const float cx = (float(px) + rnd_pair(sc, 0.0f)) / float(G.width);
+                            const float cy = (float(py) + rnd_pair(sc, 1.0f)) / float(G.height);
                             dir = normalize(((o + rgt * cx) + btm * cy) - loc);
                             id = pixel;
                         } else {
@@ -784,7 +802,7 @@ __global__ __launch_bounds__(kFrameBlock, 4) void k_frame_persistent(ShadeArgs A
                             dir = normalize(((o + rgt * cx) + btm * cy) - loc);
                             id = pixel * G.spf + k % G.spf;
                         }
-                        A.Lpix[id] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                        Lpix[id] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                         p0 = make_float4(loc.x, loc.y, loc.z, __uint_as_float(id));
                         p1 = make_float4(dir.x, dir.y, dir.z, __uint_as_float(0u));
                         p2 = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
@@ -795,7 +813,7 @@ __global__ __launch_bounds__(kFrameBlock, 4) void k_frame_persistent(ShadeArgs A
                         rstat = kDone;
                     }
                 }
-                if (base + n >= G.rays) exhausted = true;
+                if (base + n >= total) exhausted = true;
             }
         }
         if (!__any(phase != 0)) {
@@ -840,7 +858,7 @@ __global__ __launch_bounds__(kFrameBlock, 4) void k_frame_persistent(ShadeArgs A
                 inst = L.inst;
             }
             ShadeOut so;
-            shade_path(A, bounce, p0, p1, p2, make_int4(__float_as_int(L.hitT), slot, inst, 0), so);
+            shade_path(A, ctl, Lpix, bounce, p0, p1, p2, make_int4(__float_as_int(L.hitT), slot, inst, 0), so);
             pending = so.cont;
             if (so.cont) {
                 p0 = so.pa[0];
@@ -868,11 +886,11 @@ __global__ __launch_bounds__(kFrameBlock, 4) void k_frame_persistent(ShadeArgs A
                         cc = cc * (10.0f / rm);
                     }
                     const uint32_t id = __float_as_uint(p0.w);
-                    float4 acc = A.Lpix[id];
+                    float4 acc = Lpix[id];
                     acc.x += cc.x;
                     acc.y += cc.y;
                     acc.z += cc.z;
-                    A.Lpix[id] = acc;
+                    Lpix[id] = acc;
                 }
             }
         }
@@ -934,16 +952,20 @@ __global__ __launch_bounds__(kBlock) void k_build_targets(FrameCtl* __restrict__
     }
 }
 
+// Adds `frames` frames' radiance to the accumulation image, frame after
+// frame per pixel (the order of one frame per launch).
 __global__ __launch_bounds__(kBlock) void k_accumulate(const float4* __restrict__ Lpix, float4* __restrict__ accum,
-                                                       uint32_t pixels) {
+                                                       uint32_t pixels, uint32_t frames) {
     const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
     if (p < pixels) {
-        const float4 l = Lpix[p];
         float4 a = accum[p];
-        a.x += l.x;
-        a.y += l.y;
-        a.z += l.z;
-        a.w += 1.0f;
+        for (uint32_t f = 0; f < frames; f++) {
+            const float4 l = Lpix[size_t(f) * pixels + p];
+            a.x += l.x;
+            a.y += l.y;
+            a.z += l.z;
+            a.w += 1.0f;
+        }
         accum[p] = a;
     }
 }
@@ -967,10 +989,6 @@ ShadeArgs slot_args(const PathTracer& P, uint32_t k) {
     return A;
 }
 
-// Whether a frame runs as one resident kernel: plain path tracing (the
-// radiance cache and ray binning need the bounce-by-bounce queues).
-bool resident_frame(const PathTracer& P) { return !P.shade_args.cache.enabled && !P.binning && !P.wavefront; }
-
 atlas_rt_status launch_resident(PathTracer& P, uint32_t k, const ShadeArgs& A, const FrameGen& G, hipStream_t s) {
     FrameSlot& S = P.slot[k];
     const uint32_t groups = uint32_t(device_cu_count()) * kResidentGroupsPerCu;
@@ -985,6 +1003,8 @@ atlas_rt_status launch_resident(PathTracer& P, uint32_t k, const ShadeArgs& A, c
 }
 
 }  // namespace
+
+bool resident_frame(const PathTracer& P) { return !P.shade_args.cache.enabled && !P.binning && !P.wavefront; }
 
 atlas_rt_status launch_bounces(PathTracer& P, uint32_t k, hipStream_t s, hipEvent_t train_ready,
                                const nrc::NrcDev* net) {
@@ -1055,10 +1075,10 @@ atlas_rt_status launch_frame_realtime(PathTracer& P, const Camera& cam, const at
     A.rt.enabled = 1;
     // realtime frames run one at a time in slot 0 on the caller's stream
     FrameSlot& S = P.slot[0];
-    hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(64), 0, s, P.ctl, S.ctl, P.bounces, P.rays, 0u);
+    hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(64), 0, s, P.ctl, S.ctl, P.bounces, P.rays, 0u, 1u);
     atlas_rt_status st;
     if (resident_frame(P)) {
-        const FrameGen G{cam, P.width, P.height, P.rays, spf, M.jitter[0], M.jitter[1]};
+        const FrameGen G{cam, P.width, P.height, P.rays, 1u, spf, M.jitter[0], M.jitter[1]};
         st = launch_resident(P, 0, slot_args(P, 0), G, s);
     } else {
         hipLaunchKernelGGL(k_raygen_rt, dim3((P.rays + kShadeBlock - 1) / kShadeBlock), dim3(kShadeBlock), 0, s, cam,
@@ -1108,7 +1128,7 @@ atlas_rt_status launch_frame_realtime(PathTracer& P, const Camera& cam, const at
 //   frame f - 2's, through the snapshot that training left (query_lag 1)
 //   (ev_train).
 // Everything else a frame touches is its slot's own.
-atlas_rt_status launch_frame(PathTracer& P, const Camera& cam, uint32_t k, const FrameOrder& order) {
+atlas_rt_status launch_frame(PathTracer& P, const Camera& cam, uint32_t k, const FrameOrder& order, uint32_t frames) {
     FrameSlot& S = P.slot[k];
     hipStream_t s = S.stream;
     const uint32_t pixels = P.width * P.height;
@@ -1117,16 +1137,18 @@ atlas_rt_status launch_frame(PathTracer& P, const Camera& cam, uint32_t k, const
     const CacheArgs& C = A.cache;
     const FrameSlot* before = order.prev >= 0 ? &P.slot[order.prev] : nullptr;
     const FrameSlot* before2 = order.prev2 >= 0 ? &P.slot[order.prev2] : nullptr;
+    const bool resident = resident_frame(P);
+    if (!resident || frames > P.batch) frames = 1;
     if (before) ATLAS_HIP_TRY(hipStreamWaitEvent(s, before->ev_begin, 0));
-    hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(64), 0, s, P.ctl, S.ctl, P.bounces, pixels, 1u);
+    hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(64), 0, s, P.ctl, S.ctl, P.bounces, pixels, 1u, frames);
     ATLAS_HIP_TRY(hipEventRecord(S.ev_begin, s));
-    if (resident_frame(P)) {
-        const FrameGen G{cam, P.width, P.height, pixels, 0u, 0.0f, 0.0f};
+    if (resident) {
+        const FrameGen G{cam, P.width, P.height, pixels, frames, 0u, 0.0f, 0.0f};
         const atlas_rt_status rs = launch_resident(P, k, A, G, s);
         if (rs != ATLAS_RT_OK) return rs;
         ATLAS_HIP_TRY(hipEventRecord(S.ev_train, s));
         if (before) ATLAS_HIP_TRY(hipStreamWaitEvent(s, before->ev_acc, 0));
-        hipLaunchKernelGGL(k_accumulate, dim3(grid_all), dim3(kBlock), 0, s, S.Lpix, P.accum, pixels);
+        hipLaunchKernelGGL(k_accumulate, dim3(grid_all), dim3(kBlock), 0, s, S.Lpix, P.accum, pixels, frames);
         ATLAS_HIP_TRY(hipEventRecord(S.ev_acc, s));
         ATLAS_HIP_TRY(hipGetLastError());
         return ATLAS_RT_OK;
@@ -1165,7 +1187,7 @@ atlas_rt_status launch_frame(PathTracer& P, const Camera& cam, uint32_t k, const
     }
     ATLAS_HIP_TRY(hipEventRecord(S.ev_train, s));
     if (before) ATLAS_HIP_TRY(hipStreamWaitEvent(s, before->ev_acc, 0));
-    hipLaunchKernelGGL(k_accumulate, dim3(grid_all), dim3(kBlock), 0, s, S.Lpix, P.accum, pixels);
+    hipLaunchKernelGGL(k_accumulate, dim3(grid_all), dim3(kBlock), 0, s, S.Lpix, P.accum, pixels, 1u);
     ATLAS_HIP_TRY(hipEventRecord(S.ev_acc, s));
     ATLAS_HIP_TRY(hipGetLastError());
     return ATLAS_RT_OK;

@@ -27,7 +27,8 @@ struct Material {
     int32_t metalness_texture;
     int32_t ao_texture;
     int32_t normal_texture;
-    int32_t reserved[3];
+    int32_t use_vertex_colors;  // base colour times the interpolated vertex colour
+    int32_t reserved[2];
 };
 
 // atlas_rt_gpu_light / GPULight (64 B), read as UnpackLight does

@@ -83,6 +83,9 @@ struct atlas_rt_blas {
     float4* d_uv = nullptr;
     //  d_tangent: optional, 2 x float4 per slot: (tangent, 0), (bitangent, 0).
     float4* d_tangent = nullptr;
+    //  d_color: optional, 3 x float4 per slot: the vertex colours (RGBA)
+    //           unpacked from GPUTriangle records (atlas_rt_blas_set_triangles).
+    float4* d_color = nullptr;
     bool has_alpha = false;             // some baked opacity is not 1
     std::vector<float> shade_normals;   // host copy, 9 floats per slot
     std::vector<int32_t> shade_mat;     // host copy, 1 per slot

@@ -104,6 +104,23 @@ typedef struct atlas_rt_packed_ray {
     float hit[4];
 } atlas_rt_packed_ray; /* 48 B */
 
+/* GPUTriangle / PackedTriangle (RTStructures.h:14-21), packed per BVH slot by
+ * MeshData::BuildBVH (MeshData.cpp:172-240) and unpacked by UnpackTriangle
+ * (raytracer/common.hsh:17-44):
+ *   v0..v2 = (vertex xyz, bits of the vertex normal as 10-10-10-2,
+ *            Packing.cpp:24-35),
+ *   d0 = (bits of half2 uv0, uv1, uv2, bits of the material index),
+ *   d1 = (bits of the 10-10-10-2 tangent, of the bitangent, +1 end of leaf / -1, 0),
+ *   d2 = (bits of RGBA8 colour 0, 1, 2, opacity). */
+typedef struct atlas_rt_gpu_triangle {
+    float v0[4];
+    float v1[4];
+    float v2[4];
+    float d0[4];
+    float d1[4];
+    float d2[4];
+} atlas_rt_gpu_triangle; /* 96 B */
+
 /* Volume::BVHTriangle input (BVH.h:26-33): vertices + caller's triangle index. */
 typedef struct atlas_rt_bvh_triangle {
     float v0[3];
@@ -518,6 +535,20 @@ atlas_rt_status atlas_rt_world_trace_any_rays(const atlas_rt_world* world, const
  * original order; material: one index per triangle or NULL (all 0). */
 atlas_rt_status atlas_rt_blas_set_shading(atlas_rt_blas* blas, const float* normals, const int32_t* material);
 
+/* The engine's own shading records of a BLAS: count atlas_rt_gpu_triangle in
+ * BVH slot order, as MeshData::BuildBVH leaves them in gpuTriangles and
+ * Mesh::BuildBVH uploads them (MeshData.cpp:172-240, Mesh.cpp:97-108); count
+ * must be the BLAS's slot count and each record's end-of-leaf flag (d1.z)
+ * the slot's.  They are unpacked as UnpackTriangle does
+ * (raytracer/common.hsh:17-44: unpackUnitVector, unpackHalf2x16,
+ * unpackUnorm4x8; the material index and opacity as stored), and the path
+ * tracer shades from the unpacked values: vertex normals, material index,
+ * opacity and texture coordinates (the opacity-aware traversal reads these
+ * two), the tangent frame of normal maps, and vertex colours for materials
+ * with use_vertex_colors.  Replaces atlas_rt_blas_set_shading, _set_opacity
+ * and _set_tangents for the BLAS. */
+atlas_rt_status atlas_rt_blas_set_triangles(atlas_rt_blas* blas, const atlas_rt_gpu_triangle* tris, uint32_t count);
+
 /* Per-triangle opacity and texture coordinates of a BLAS, used by the path
  * tracer's opacity-aware traversal (HitClosestTransparency /
  * HitAnyTransparency, raytracer/bvh.hsh:106-170,275-357,443-524).
@@ -568,7 +599,9 @@ typedef struct atlas_rt_material {
     int32_t metalness_texture;
     int32_t ao_texture;
     int32_t normal_texture;
-    int32_t reserved[3];
+    int32_t use_vertex_colors;  /* GPUMaterial.useVertexColors (RTStructures.h:72): base colour times the
+                                   interpolated vertex colour (surface.hsh:98); needs atlas_rt_blas_set_triangles */
+    int32_t reserved[2];
 } atlas_rt_material; /* 96 B */
 
 /* A punctual light as the engine's LightComponent describes it

@@ -287,7 +287,7 @@ class _Surface(C.Structure):
     _fields_ = [("opacity", C.c_void_p), ("uvs", C.c_void_p), ("textures", C.c_void_p),
                 ("texture_count", C.c_uint32), ("emitters", C.c_void_p), ("emitter_count", C.c_uint32),
                 ("lights_out", C.c_void_p), ("lights_cap", C.c_uint32), ("lights_count", C.c_void_p),
-                ("tangents", C.c_void_p)]
+                ("tangents", C.c_void_p), ("colors", C.c_void_p)]
 
 
 GPU_LIGHT_DTYPE = np.dtype([("P", "<f4", 4), ("N", "<f4", 4), ("color", "<f4", 4), ("data", "<f4", 4)])
@@ -296,7 +296,8 @@ GPU_LIGHT_DTYPE = np.dtype([("P", "<f4", 4), ("N", "<f4", 4), ("color", "<f4", 4
 def _surface_arg(surface, n_blas, keep):
     """OracleSurface from a dict with optional keys: opacity (per BLAS, slot
     order, or None), uvs (per BLAS, (n, 6) slot order, or None), tangents
-    (per BLAS, (n, 6) slot order, or None), textures (list of (h, w) or
+    (per BLAS, (n, 6) slot order, or None), colors (per BLAS, (n, 12) vertex
+    colours RGBA in slot order, or None), textures (list of (h, w) or
     (h, w, channels) uint8), emitters ((m, 4) float32), lights_cap (int: the
     light list is returned in surface["lights"])."""
     if surface is None:
@@ -318,6 +319,7 @@ def _surface_arg(surface, n_blas, keep):
     S.opacity = per_blas("opacity", 1)
     S.uvs = per_blas("uvs", 6)
     S.tangents = per_blas("tangents", 6)
+    S.colors = per_blas("colors", 12)
     tex = surface.get("textures") or []
     if tex:
         arr = (_Texture * len(tex))()
@@ -445,6 +447,53 @@ def f2h(values):
         L.oracle_h2f.argtypes = [C.c_uint16]
         L._f2h_bound = True
     return np.array([L.oracle_f2h(float(v)) for v in np.asarray(values, np.float32).ravel()], np.uint16)
+
+
+GPU_TRIANGLE_DTYPE = np.dtype([("v0", "<f4", 4), ("v1", "<f4", 4), ("v2", "<f4", 4), ("d0", "<f4", 4),
+                               ("d1", "<f4", 4), ("d2", "<f4", 4)])
+
+
+def unpack_triangles(records):
+    """UnpackTriangle (raytracer/common.hsh:17-44) of GPUTriangle records:
+    dict of normals (n, 9), material (n,), opacity (n,), uvs (n, 6),
+    tangents (n, 6), colors (n, 12)."""
+    r = np.ascontiguousarray(records, dtype=GPU_TRIANGLE_DTYPE)
+    n = len(r)
+    out = {"normals": np.zeros((n, 9), np.float32), "material": np.zeros(n, np.int32),
+           "opacity": np.zeros(n, np.float32), "uvs": np.zeros((n, 6), np.float32),
+           "tangents": np.zeros((n, 6), np.float32), "colors": np.zeros((n, 12), np.float32)}
+    L = lib()
+    L.oracle_unpack_triangles.restype = None
+    L.oracle_unpack_triangles.argtypes = [C.c_void_p, C.c_uint32] + [C.c_void_p] * 6
+    L.oracle_unpack_triangles(_p(r), n, _p(out["normals"]), _p(out["material"]), _p(out["opacity"]), _p(out["uvs"]),
+                              _p(out["tangents"]), _p(out["colors"]))
+    return out
+
+
+def pathtrace_counts():
+    """Traversal work of the last pathtrace() call: rays traced (closest hits
+    and shadow rays), inner nodes fetched (TLAS and BLAS), triangles tested,
+    instances entered."""
+    out = (C.c_uint64 * 4)()
+    L = lib()
+    L.oracle_pathtrace_counts.restype = None
+    L.oracle_pathtrace_counts.argtypes = [C.c_void_p]
+    L.oracle_pathtrace_counts(C.cast(out, C.c_void_p))
+    return {"rays": out[0], "node_visits": out[1], "tri_tests": out[2], "instance_visits": out[3]}
+
+
+def sincos(x):
+    """The restatement's (sin, cos, x^5) of float32 values (the path tracer's
+    sample-angle sine / cosine and pow(x, 5))."""
+    x = np.ascontiguousarray(x, np.float32).ravel()
+    sn, cs, p5 = (np.zeros_like(x) for _ in range(3))
+    L = lib()
+    if not getattr(L, "_sincos_bound", False):
+        L.oracle_sincos.restype = None
+        L.oracle_sincos.argtypes = [C.c_void_p] * 4 + [C.c_uint32]
+        L._sincos_bound = True
+    L.oracle_sincos(_p(x), _p(sn), _p(cs), _p(p5), len(x))
+    return sn, cs, p5
 
 
 def h2f(bits):

@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstring>
 #include <thread>
+#include <mutex>
 #include <vector>
 
 #include "rt_oracle.h"
@@ -43,7 +44,10 @@ struct OracleSurface {
     uint32_t lights_cap;
     uint32_t* lights_count;
     const float* const* tangents;  // per BLAS: 6 floats per slot (tangent, bitangent), or NULL
+    const float* const* colors;    // per BLAS: 12 floats per slot (vertex colours RGBA), or NULL
 };
+
+extern "C" float oracle_h2f(uint16_t h);
 
 namespace oracle {
 namespace {
@@ -111,6 +115,7 @@ struct Mat {
     int32_t id;  // RaytraceMaterial.ID
     float normal_scale;
     int32_t base_color_texture, roughness_texture, metalness_texture, ao_texture, normal_texture;
+    int32_t use_vertex_colors;  // GPUMaterial.useVertexColors
 };
 
 struct Tex {
@@ -161,8 +166,39 @@ struct Surf {
     Mat m;
 };
 
+// GLSL's sin, cos and pow(x, 5) have implementation-defined precision; the
+// restatement fixes them (and the kernels use the same operations): the
+// sample angles' sin / cos by Cephes' single-precision method (Moshier,
+// Cephes Math Library sinf.c / cosf.c: x * 4/pi to an even octant, pi/4
+// subtracted in three parts, sine and cosine polynomials on [-pi/4, pi/4]),
+// pow(x, 5) as (x^2)^2 * x.
+void sincos_ph(float x, float& sn, float& cs) {
+    int oct = int(x * 1.27323954473516f);
+    float k = float(oct);
+    if (oct & 1) {
+        oct += 1;
+        k += 1.0f;
+    }
+    oct &= 7;
+    const float r = ((x - k * 0.78515625f) - k * 2.4187564849853515625e-4f) - k * 3.77489497744594108e-8f;
+    const float r2 = r * r;
+    const float ps = ((-1.9515295891e-4f * r2 + 8.3321608736e-3f) * r2 - 1.6666654611e-1f) * r2 * r + r;
+    const float pc = ((2.443315711809948e-5f * r2 - 1.388731625493765e-3f) * r2 + 4.166664568298827e-2f) * r2 * r2 -
+                     0.5f * r2 + 1.0f;
+    switch (oct) {  // octants 0, 2, 4, 6: (sin, cos) = (s, c), (c, -s), (-s, -c), (-c, s)
+        case 0: sn = ps; cs = pc; break;
+        case 2: sn = pc; cs = -ps; break;
+        case 4: sn = -ps; cs = -pc; break;
+        default: sn = -pc; cs = ps; break;
+    }
+}
+inline float pow5f(float x) {
+    float sq = x * x;
+    return (sq * sq) * x;
+}
+
 inline V3 schlick(V3 F0, float F90, float c) {
-    float p = std::pow(1.0f - c, 5.0f);
+    float p = pow5f(1.0f - c);
     return F0 + mul(V3{F90, F90, F90} - F0, p);
 }
 void update(Surf& s) {
@@ -182,8 +218,8 @@ V3 diffuse(const Surf& s) {
     float lr = std::fmax(s.m.roughness * s.m.roughness, 0.00001f);
     float bias = mixf(0.0f, 0.5f, lr), factor = mixf(1.0f, 1.0f / 1.51f, lr);
     float fd90 = bias + 2.0f * s.LdotH * s.LdotH * lr;
-    float ls = 1.0f + (fd90 - 1.0f) * std::pow(1.0f - s.NdotL, 5.0f);
-    float vs = 1.0f + (fd90 - 1.0f) * std::pow(1.0f - s.NdotV, 5.0f);
+    float ls = 1.0f + (fd90 - 1.0f) * pow5f(1.0f - s.NdotL);
+    float vs = 1.0f + (fd90 - 1.0f) * pow5f(1.0f - s.NdotV);
     float d = ls * vs * factor;
     return mul(mul(mul(s.m.base, 1.0f - s.m.metalness), d), kInvPI);
 }
@@ -221,7 +257,9 @@ struct Bs {
 };
 Bs sample_diffuse(Surf& s, float u0, float u1) {
     float th = std::sqrt(u0), ph = 2.0f * kPI * u1;
-    V3 l{th * std::cos(ph), th * std::sin(ph), std::sqrt(1.0f - u0)};
+    float sph, cph;
+    sincos_ph(ph, sph, cph);
+    V3 l{th * cph, th * sph, std::sqrt(1.0f - u0)};
     V3 t, b;
     frame_of(s.N, t, b, false);
     s.L = nrm((mul(t, l.x) + mul(b, l.y)) + mul(s.N, l.z));
@@ -238,7 +276,9 @@ Bs sample_specular(Surf& s, float u0, float u1) {
     float phi = kPI * 2.0f * u0;
     float z = (1.0f - u1) * (1.0f + vh.z) + -vh.z;
     float r = std::sqrt(clampf(1.0f - z * z, 0.0f, 1.0f));
-    V3 c{std::cos(phi) * r, std::sin(phi) * r, z};
+    float sph, cph;
+    sincos_ph(phi, sph, cph);
+    V3 c{cph * r, sph * r, z};
     V3 Hh = c + vh;
     V3 h{Hh.x * alpha, Hh.y * alpha, std::fmax(Hh.z, 0.0f)};
     V3 M = nrm((mul(t, h.x) + mul(b, h.y)) + mul(N, h.z));
@@ -262,6 +302,7 @@ struct Scene {
     std::vector<std::vector<float>> opacity;  // per BLAS, slot order (MeshData.cpp:141), empty = opaque
     std::vector<std::vector<float>> uv;       // per BLAS, slot order: uv0, uv1, uv2 (6 floats), may be empty
     std::vector<std::vector<float>> tangent;  // per BLAS, slot order: tangent, bitangent (6 floats), may be empty
+    std::vector<std::vector<float>> color;    // per BLAS, slot order: three RGBA vertex colours, may be empty
     bool alpha = false;                       // some triangle is not opaque
     const float* xform;                       // 12 floats per packed instance
     std::vector<Mat> mats;
@@ -326,8 +367,22 @@ struct Primary {
     int32_t instance = -1;
 };
 
+// Traversal work of a path tracer call: rays traced (closest hits and
+// shadow rays), inner nodes fetched, triangles tested, instances entered.
+struct PtCount {
+    uint64_t rays = 0, nodes = 0, tris = 0, insts = 0;
+    void add(const TraceStats& st) {
+        rays++;
+        nodes += st.node_visits;
+        tris += st.tri_tests;
+        insts += st.inst_visits;
+    }
+};
+std::mutex g_count_mu;
+PtCount g_count;  // of the last oracle_pathtrace call
+
 V3 trace_path(const Scene& S, V3 o, V3 d, float pixel_seed, const float* seeds, std::vector<int32_t>& stack,
-              const CacheOut& co = CacheOut{}, Primary* prim = nullptr) {
+              const CacheOut& co = CacheOut{}, Primary* prim = nullptr, PtCount* cnt = nullptr) {
     V3 thr{1, 1, 1}, Lr{0, 0, 0};
     for (int32_t bounce = 0; bounce <= S.bounces; bounce++) {
         const V3 thr_in = thr;
@@ -336,6 +391,7 @@ V3 trace_path(const Scene& S, V3 o, V3 d, float pixel_seed, const float* seeds, 
         TraceStats st;
         const Opacity op{&S, tri_opacity};
         closest_rgpu_world(S.w, o, d, 0.0f, kInf, ATLAS_RT_MASK_ALL, h, st, stack, S.alpha ? &op : nullptr);
+        if (cnt) cnt->add(st);
         if (h.slot < 0) {
             if (prim && bounce == 0) prim->hit = false;
             V3 e = S.env * thr;
@@ -365,6 +421,13 @@ V3 trace_path(const Scene& S, V3 o, V3 d, float pixel_seed, const float* seeds, 
         float den = dot(q, e0);
         float dist = dot(p, e1) / den, bs = dot(q, s) / den, bt = dot(p, d) / den;
         float br = 1.0f - bs - bt;
+        // vertex colours (surface.hsh:84, 98): r c0 + s c1 + t c2, rgb only
+        if (m.use_vertex_colors > 0 && !S.color[mesh].empty()) {
+            const float* c = &S.color[mesh][12 * size_t(h.slot)];
+            m.base = V3{m.base.x * ((br * c[0] + bs * c[4]) + bt * c[8]),
+                        m.base.y * ((br * c[1] + bs * c[5]) + bt * c[9]),
+                        m.base.z * ((br * c[2] + bs * c[6]) + bt * c[10])};
+        }
         // material maps (surface.hsh:86-124) at the texture coordinate of the
         // world-space barycentrics
         V3 tc{0, 0, 0};
@@ -525,6 +588,7 @@ V3 trace_path(const Scene& S, V3 o, V3 d, float pixel_seed, const float* seeds, 
                     bool occluded = any_rgpu_world(S.w, so, sf.L, 0.0f, tmax, ATLAS_RT_MASK_SHADOW, st2, stack, true);
                     vis = occluded ? 0.0f : 1.0f;
                 }
+                if (cnt) cnt->add(st2);
             }
             V3 direct = mul(refl * mul(rad, vis), sf.NdotL);
             direct = V3{direct.x / lpdf, direct.y / lpdf, direct.z / lpdf};
@@ -679,6 +743,7 @@ void make_scene(Scene& S, std::vector<BlasView>& views, const atlas_rt_gpu_bvh_n
     S.opacity.resize(blas_count);
     S.uv.resize(blas_count);
     S.tangent.resize(blas_count);
+    S.color.resize(blas_count);
     for (uint32_t k = 0; k < blas_count; k++) {
         S.shade[k].assign(shade_normals[k], shade_normals[k] + 9 * size_t(blas_tri_counts[k]));
         S.mat[k].assign(shade_mat[k], shade_mat[k] + blas_tri_counts[k]);
@@ -689,6 +754,8 @@ void make_scene(Scene& S, std::vector<BlasView>& views, const atlas_rt_gpu_bvh_n
         if (ext && ext->uvs && ext->uvs[k]) S.uv[k].assign(ext->uvs[k], ext->uvs[k] + 6 * size_t(blas_tri_counts[k]));
         if (ext && ext->tangents && ext->tangents[k])
             S.tangent[k].assign(ext->tangents[k], ext->tangents[k] + 6 * size_t(blas_tri_counts[k]));
+        if (ext && ext->colors && ext->colors[k])
+            S.color[k].assign(ext->colors[k], ext->colors[k] + 12 * size_t(blas_tri_counts[k]));
     }
     for (uint32_t i = 0; i < n_mats; i++) {
         const auto& m = mats[i];
@@ -696,7 +763,7 @@ void make_scene(Scene& S, std::vector<BlasView>& views, const atlas_rt_gpu_bvh_n
                              V3{m.emissive[0], m.emissive[1], m.emissive[2]}, m.opacity, m.roughness, m.metalness,
                              m.ao, m.reflectance, m.two_sided, m.opacity_texture, m.invert_uvs, m.id,
                              m.normal_scale, m.base_color_texture, m.roughness_texture, m.metalness_texture,
-                             m.ao_texture, m.normal_texture});
+                             m.ao_texture, m.normal_texture, m.use_vertex_colors});
     }
     if (ext)
         for (uint32_t t = 0; t < ext->texture_count; t++)
@@ -838,6 +905,7 @@ void oracle_pathtrace(const atlas_rt_gpu_bvh_node* tlas, uint32_t tlas_count, co
     pcg.next();  // pcg32_init's discarded draw
     for (uint64_t i = 0; i < pcg_draws; i++) pcg.next();
     const V3 loc{cam->location[0], cam->location[1], cam->location[2]};
+    g_count = PtCount{};
     for (uint32_t f = 0; f < frames; f++) {
         std::vector<float> seeds(size_t(bounces) + 1);
         for (int32_t b = 0; b <= bounces; b++) seeds[b] = pcg.fast_uniform();
@@ -845,13 +913,20 @@ void oracle_pathtrace(const atlas_rt_gpu_bvh_node* tlas, uint32_t tlas_count, co
         const float jx = rand2(sc, 0.0f), jy = rand2(sc, 1.0f);
         parallel_pixels(width * height, threads, [&](uint32_t lo, uint32_t hi) {
             std::vector<int32_t> stack;
+            PtCount cnt;
             for (uint32_t pix = lo; pix < hi; pix++) {
-                V3 L = trace_path(S, loc, camera_dir(cam, pix, width, height, jx, jy), float(pix), seeds.data(), stack);
+                V3 L = trace_path(S, loc, camera_dir(cam, pix, width, height, jx, jy), float(pix), seeds.data(), stack,
+                                  CacheOut{}, nullptr, &cnt);
                 accum[4 * size_t(pix)] += L.x;
                 accum[4 * size_t(pix) + 1] += L.y;
                 accum[4 * size_t(pix) + 2] += L.z;
                 accum[4 * size_t(pix) + 3] += 1.0f;
             }
+            std::lock_guard<std::mutex> lock(g_count_mu);
+            g_count.rays += cnt.rays;
+            g_count.nodes += cnt.nodes;
+            g_count.tris += cnt.tris;
+            g_count.insts += cnt.insts;
         });
     }
 }
@@ -1064,6 +1139,62 @@ void oracle_pathtrace_realtime(const atlas_rt_gpu_bvh_node* tlas, uint32_t tlas_
             normal[pix] = uint32_t(oracle_f2h(nx)) | (uint32_t(oracle_f2h(nz)) << 16);
         }
     });
+}
+
+// The restatement's sin / cos of sample angles and pow(x, 5), for tests.
+void oracle_sincos(const float* x, float* sn, float* cs, float* p5, uint32_t n) {
+    for (uint32_t i = 0; i < n; i++) {
+        oracle::sincos_ph(x[i], sn[i], cs[i]);
+        p5[i] = oracle::pow5f(x[i]);
+    }
+}
+
+// Traversal work of the last oracle_pathtrace call: rays traced (closest
+// hits and shadow rays), inner nodes fetched (TLAS and BLAS), triangles
+// tested, instances entered.
+void oracle_pathtrace_counts(uint64_t* out) {
+    out[0] = g_count.rays;
+    out[1] = g_count.nodes;
+    out[2] = g_count.tris;
+    out[3] = g_count.insts;
+}
+
+// UnpackTriangle (raytracer/common.hsh:17-44) of GPUTriangle records:
+// per record the vertex normals (9 floats, unpackUnitVector of v0..v2.w),
+// the material index (d0.w bits), the opacity (d2.w), the texture
+// coordinates (6 floats, unpackHalf2x16 of d0.xyz), the tangent and
+// bitangent (6 floats, unpackUnitVector of d1.xy) and the vertex colours
+// (12 floats, unpackUnorm4x8 of d2.xyz).
+void oracle_unpack_triangles(const atlas_rt_gpu_triangle* r, uint32_t n, float* normals, int32_t* material,
+                             float* opacity, float* uvs, float* tangents, float* colors) {
+    auto bits = [](float f) {
+        uint32_t u;
+        std::memcpy(&u, &f, 4);
+        return u;
+    };
+    // unpackUnitVector (common/packing.hsh:2-12): k / 1023 * 2 - 1 per 10-bit field
+    auto unit = [](uint32_t c, float* o) {
+        for (int k = 0; k < 3; k++) o[k] = float((c >> (10 * k)) & 1023u) / 1023.0f * 2.0f - 1.0f;
+    };
+    for (uint32_t i = 0; i < n; i++) {
+        const atlas_rt_gpu_triangle& t = r[i];
+        unit(bits(t.v0[3]), normals + 9 * size_t(i));
+        unit(bits(t.v1[3]), normals + 9 * size_t(i) + 3);
+        unit(bits(t.v2[3]), normals + 9 * size_t(i) + 6);
+        material[i] = int32_t(bits(t.d0[3]));
+        opacity[i] = t.d2[3];
+        for (int k = 0; k < 3; k++) {
+            const uint32_t h = bits(t.d0[k]);
+            uvs[6 * size_t(i) + 2 * k] = oracle_h2f(uint16_t(h & 0xFFFFu));
+            uvs[6 * size_t(i) + 2 * k + 1] = oracle_h2f(uint16_t(h >> 16));
+        }
+        unit(bits(t.d1[0]), tangents + 6 * size_t(i));
+        unit(bits(t.d1[1]), tangents + 6 * size_t(i) + 3);
+        for (int k = 0; k < 3; k++) {
+            const uint32_t c = bits(t.d2[k]);
+            for (int j = 0; j < 4; j++) colors[12 * size_t(i) + 4 * k + j] = float((c >> (8 * j)) & 0xFFu) / 255.0f;
+        }
+    }
 }
 
 }  // extern "C"

@@ -150,13 +150,22 @@ def test_imported_oracle_blas(torch_cuda):
 
 
 def test_host_buffer_api(torch_cuda):
+    """A device BLAS also answers the engine's CPU traversal on the host
+    (BVH::GetIntersection semantics, the oracle's R-CPU), and it agrees with
+    the GPU's HitClosest wherever the two semantics do (tmin = 0: same hit
+    distance bit for bit)."""
     verts, rays = rt_cases.soup(2048, 8192, 41)
+    rays["tmin"] = 0.0
     blas = atlas_rt.Blas.build(verts)
     nodes, tris, orig = blas.export()
     h = blas.trace_closest_host(rays)
-    hr, _ = oracle_rt.trace_closest(nodes, tris, orig, rays)
-    assert np.array_equal(rt_cases.hit_bits(h), rt_cases.hit_bits(hr))
-    assert np.array_equal(blas.trace_any_host(rays), oracle_rt.trace_any(nodes, tris, rays))
+    hr, _, raw_t = oracle_rt.trace_closest(nodes, tris, orig, rays, mode=oracle_rt.MODE_RCPU, raw_t=True)
+    assert np.array_equal(h["prim_id"], hr["prim_id"])
+    hit = h["prim_id"] >= 0
+    assert hit.mean() > 0.02 and np.array_equal(h["t"][hit].view(np.uint32), raw_t[hit].view(np.uint32))
+    assert np.array_equal(blas.trace_any_host(rays), oracle_rt.trace_any(nodes, tris, rays, mode=oracle_rt.MODE_RCPU))
+    _, _, gpu_t = oracle_rt.trace_closest(nodes, tris, orig, rays, raw_t=True)
+    assert np.array_equal(raw_t.view(np.uint32), gpu_t.view(np.uint32))
 
 
 def test_hits_match_brute_force_and_fp64(torch_cuda):

@@ -1,8 +1,8 @@
 """The resident frame kernel against the bounce-by-bounce launches.
 
-By default a frame runs as one resident kernel that walks every path from
-its camera ray to its last bounce in one lane (trace, shade, shadow ray, next
-bounce).  ATLAS_RT_PT_WAVEFRONT runs the engine's dispatch shape instead: one
+By default progressive frames run as resident kernels that walk every path
+from its camera ray to its last bounce in one lane (trace, shade, shadow ray,
+next bounce), several frames per launch.  ATLAS_RT_PT_WAVEFRONT runs the engine's dispatch shape instead: one
 closest-hit launch, one shade launch and one shadow-ray launch per bounce over
 compacted path queues.  Every path computes the same arithmetic in the same
 order either way and writes only its own pixel, so the accumulated image, the
@@ -60,6 +60,20 @@ def test_resident_equals_wavefront_opaque(torch_cuda, fif):
     assert np.all(a[..., 3] == 5) and a[..., :3].max() > 0
     diff = np.nonzero(np.any(a.view(np.uint32) != b.view(np.uint32), axis=-1))
     assert len(diff[0]) == 0, f"{len(diff[0])} pixels differ, first {list(zip(*diff))[:5]}"
+
+
+def test_resident_batches_equal_wavefront(torch_cuda):
+    """19 frames in one call run as resident launches of 8, 8 and 3 frames
+    (each frame its own seeds and radiance buffer, summed per pixel in frame
+    order): the image equals 19 bounce-by-bounce frames bit for bit."""
+    print("case resident batches", flush=True)
+    w_, blases, mats = _plain(salt=6)
+    W_, H_ = 64, 48
+    cam = atlas_rt.camera((0.0, 15.0, -70.0), 0.0, 12.0, 50.0, W_ / H_)
+    a, b = _both(lambda wf: atlas_rt.PathTracer(w_, W_, H_, 3, mats, W.sun_light(), env=ENV, wavefront=wf),
+                 lambda pt: pt.render(cam, 19), lambda pt: pt.read().cpu().numpy())
+    assert np.all(a[..., 3] == 19)
+    assert a.tobytes() == b.tobytes()
 
 
 def test_resident_equals_wavefront_ragged_and_deep(torch_cuda):
