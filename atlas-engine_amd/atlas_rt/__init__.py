@@ -152,6 +152,13 @@ class Blas:
         check(lib().atlas_rt_trace_closest_rays(self._h, _ptr(d_rays), _ptr(d_hits), _ptr(d_slots), n, kernel,
                                                 _stream_ptr(stream)))
 
+    def trace_closest_noalloc(self, d_rays, d_hits, d_slots=None, count=None, kernel=_abi.KERNEL_AUTO, stream=None):
+        """trace_closest through the entry point that allocates nothing
+        (binary persistent kernels, fixed device scratch)."""
+        n = count if count is not None else d_rays.numel() // 8
+        check(lib().atlas_rt_trace_closest_rays_noalloc(self._h, _ptr(d_rays), _ptr(d_hits), _ptr(d_slots), n, kernel,
+                                                        _stream_ptr(stream)))
+
     def trace_closest_stats(self, d_rays, d_hits, d_visits, d_slots=None, count=None, stream=None):
         n = count if count is not None else d_rays.numel() // 8
         check(lib().atlas_rt_trace_closest_stats(self._h, _ptr(d_rays), _ptr(d_hits), _ptr(d_slots),

@@ -167,6 +167,7 @@ PROTOTYPES = {
     "atlas_rt_blas_info": (status, [vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32)]),
     "atlas_rt_blas_export": (status, [vp, vp, vp, vp]),
     "atlas_rt_trace_closest_rays": (status, [vp, vp, vp, vp, u32, i32, vp]),
+    "atlas_rt_trace_closest_rays_noalloc": (status, [vp, vp, vp, vp, u32, i32, vp]),
     "atlas_rt_trace_closest_stats": (status, [vp, vp, vp, vp, vp, u32, vp]),
     "atlas_rt_sort_rays_workspace": (status, [u32, C.POINTER(C.c_size_t)]),
     "atlas_rt_sort_rays_by_direction": (status, [vp, u32, vp, vp, vp, vp, C.c_size_t, vp]),

@@ -76,6 +76,25 @@ atlas_rt_status atlas_rt_trace_closest_rays(const atlas_rt_blas* blas, const atl
     return launch_blas_closest(*const_cast<atlas_rt_blas*>(blas), L);
 }
 
+atlas_rt_status atlas_rt_trace_closest_rays_noalloc(const atlas_rt_blas* blas, const atlas_rt_ray* d_rays,
+                                                    atlas_rt_hit* d_hits, int32_t* d_slot, uint32_t count,
+                                                    int32_t kernel, void* stream) {
+    atlas_rt_status st = check_blas_ready(blas, "atlas_rt_trace_closest_rays_noalloc");
+    if (st != ATLAS_RT_OK) return st;
+    if (count && (!d_rays || !d_hits))
+        return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_trace_closest_rays_noalloc: NULL buffer");
+    st = check_kernel(kernel, "atlas_rt_trace_closest_rays_noalloc");
+    if (st != ATLAS_RT_OK) return st;
+    TraceLaunch L;
+    L.rays = d_rays;
+    L.hits = d_hits;
+    L.slots = d_slot;
+    L.count = count;
+    L.kernel = kernel;
+    L.stream = reinterpret_cast<hipStream_t>(stream);
+    return launch_blas_closest_noalloc(*blas, L);
+}
+
 atlas_rt_status atlas_rt_sort_rays_workspace(uint32_t count, size_t* bytes) {
     if (!bytes) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_sort_rays_workspace: NULL argument");
     *bytes = bin_sort_workspace(count);

@@ -135,6 +135,9 @@ atlas_rt_status launch_blas_closest(atlas_rt_blas& blas, const TraceLaunch& L);
 // copies (host_trace.cpp).
 void host_trace_closest(const atlas_rt_blas& b, const atlas_rt_ray* rays, atlas_rt_hit* hits, uint32_t count);
 void host_trace_any(const atlas_rt_blas& b, const atlas_rt_ray* rays, uint32_t* hit, uint32_t count);
+// The binary persistent kernels with their counters and stack spill in the
+// library's fixed device scratch (no allocation).
+atlas_rt_status launch_blas_closest_noalloc(const atlas_rt_blas& blas, const TraceLaunch& L);
 atlas_rt_status launch_blas_any(atlas_rt_blas& blas, const TraceLaunch& L);
 
 // Upload the engine-layout arrays of `blas` to the current device.

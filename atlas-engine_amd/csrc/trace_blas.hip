@@ -16,6 +16,9 @@
 // array sized from the tree depth.  Results equal the engine's bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <string>
+
 #include "rt_internal.h"
 #include "trace_core.h"
 
@@ -308,6 +311,94 @@ atlas_rt_status launch_blas_closest(atlas_rt_blas& blas, const TraceLaunch& L0) 
     }
     ATLAS_HIP_TRY(hipGetLastError());
     return blas.timing.end(L.stream);
+}
+
+// ---------------------------------------------------------------------------
+// Closest hit without allocation: the persistent kernels' counters, overflow
+// flag and stack spill live in one file-scope device array instead of the
+// BLAS's scratch, so a trace never calls hipMalloc.  Every call on a device
+// shares that device's copy; calls that overlap on one device race.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr size_t kNoAllocCtlBytes = 256;
+constexpr size_t kNoAllocSpillBytes = size_t(16) << 20;
+
+__device__ __align__(256) uint8_t scratch[kNoAllocCtlBytes + kNoAllocSpillBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of the current device's `scratch`, looked up once per
+// device with that device current.
+atlas_rt_status scratch_address(void** out) {
+    int dev = 0;
+    ATLAS_HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(ATLAS_RT_E_UNSUPPORTED, "no-alloc trace: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+        if (!p) return fail(ATLAS_RT_E_HIP, "hipGetSymbolAddress(scratch) returned NULL");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return ATLAS_RT_OK;
+}
+
+__global__ void k_zero_words(uint32_t* __restrict__ w, uint32_t n) {
+    if (threadIdx.x < n) w[threadIdx.x] = 0u;
+}
+
+}  // namespace
+
+atlas_rt_status launch_blas_closest_noalloc(const atlas_rt_blas& blas, const TraceLaunch& L0) {
+    TraceLaunch L = L0;
+    if (L.kernel == ATLAS_RT_KERNEL_AUTO) L.kernel = ATLAS_RT_KERNEL_BVH2_PERSISTENT | ATLAS_RT_KERNEL_LEAF_BATCH;
+    const int base = L.kernel & ATLAS_RT_KERNEL_VARIANT_MASK;
+    if (base != ATLAS_RT_KERNEL_BVH2_PERSISTENT)
+        return fail(ATLAS_RT_E_UNSUPPORTED, "atlas_rt_trace_closest_rays_noalloc: only the binary persistent kernels "
+                                            "(ATLAS_RT_KERNEL_BVH2_PERSISTENT with the register-stack, float-slab and "
+                                            "leaf-batch bits, or AUTO)");
+    void* sp = nullptr;
+    atlas_rt_status st = scratch_address(&sp);
+    if (st != ATLAS_RT_OK) return st;
+    if (L.count == 0) return ATLAS_RT_OK;
+    const uint32_t groups = uint32_t(device_cu_count()) * 4;
+    const size_t lanes = size_t(groups) * kPBlock;
+    const uint32_t cap = limit_cap(blas.max_depth + 1, blas.stack_limit);
+    const size_t max_cap = kPLdsDepth + kNoAllocSpillBytes / (lanes * sizeof(int32_t));
+    if (cap > max_cap)
+        return fail(ATLAS_RT_E_STACK_OVERFLOW, "atlas_rt_trace_closest_rays_noalloc: the tree needs " +
+                                                   std::to_string(cap) + " stack entries per ray; the fixed scratch "
+                                                   "holds " + std::to_string(max_cap) +
+                                                   " (use atlas_rt_trace_closest_rays)");
+    uint32_t* ctl = reinterpret_cast<uint32_t*>(sp);
+    int32_t* spill = reinterpret_cast<int32_t*>(static_cast<uint8_t*>(sp) + kNoAllocCtlBytes);
+    // the ray counter, the wave counter and the overflow flag start at zero
+    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, L.stream, ctl, 4u);
+    const BlasDev B = blas_dev(blas);
+    const bool reg = (L.kernel & ATLAS_RT_KERNEL_STACK_REGISTER) != 0;
+    const bool fslab = (L.kernel & ATLAS_RT_KERNEL_SLAB_FLOAT) != 0;
+    const bool leaf_batch = (L.kernel & ATLAS_RT_KERNEL_LEAF_BATCH) != 0;
+    if (leaf_batch) {
+        if (fslab)
+            reg ? launch_persistent<2, true, true, kLeafDiv>(B, L, groups, spill, cap, ctl)
+                : launch_persistent<2, false, true, kLeafDiv>(B, L, groups, spill, cap, ctl);
+        else
+            reg ? launch_persistent<2, true, false, kLeafDiv>(B, L, groups, spill, cap, ctl)
+                : launch_persistent<2, false, false, kLeafDiv>(B, L, groups, spill, cap, ctl);
+    } else if (fslab) {
+        reg ? launch_persistent<2, true, true>(B, L, groups, spill, cap, ctl)
+            : launch_persistent<2, false, true>(B, L, groups, spill, cap, ctl);
+    } else {
+        reg ? launch_persistent<2, true>(B, L, groups, spill, cap, ctl)
+            : launch_persistent<2, false>(B, L, groups, spill, cap, ctl);
+    }
+    ATLAS_HIP_TRY(hipGetLastError());
+    return ATLAS_RT_OK;
 }
 
 atlas_rt_status launch_blas_any(atlas_rt_blas& blas, const TraceLaunch& L) {

@@ -240,6 +240,26 @@ atlas_rt_status atlas_rt_trace_closest_rays(const atlas_rt_blas* blas, const atl
                                             atlas_rt_hit* d_hits, int32_t* d_slot, uint32_t count,
                                             int32_t kernel, void* stream);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The same closest-hit trace and results as
+ * atlas_rt_trace_closest_rays with the same kernel, restricted to the binary
+ * persistent kernels (ATLAS_RT_KERNEL_BVH2_PERSISTENT, optionally with the
+ * STACK_REGISTER, SLAB_FLOAT and LEAF_BATCH bits; AUTO = persistent with leaf
+ * batching).  The kernels' ray and wave counters and their traversal stack
+ * spill live in a fixed 16 MiB + 256 B device array that each GPU holds until
+ * the process ends; nothing is allocated or freed, and the counters are
+ * cleared by a small kernel, not a memset.  A BLAS whose stack bound (depth +
+ * 1, or its stack limit) exceeds what that array holds (24 + 16 MiB /
+ * (4 B x 1024 lanes per CU x CUs), i.e. 40 entries on 256 CUs) is rejected with
+ * ATLAS_RT_E_STACK_OVERFLOW.  A ray that would exceed a stack limit set with
+ * atlas_rt_blas_set_stack_limit ends as a miss, and no overflow status is
+ * kept.  Every call on one GPU shares that GPU's array: calls that overlap on
+ * one GPU (other streams, other threads) race. */
+atlas_rt_status atlas_rt_trace_closest_rays_noalloc(const atlas_rt_blas* blas, const atlas_rt_ray* d_rays,
+                                                    atlas_rt_hit* d_hits, int32_t* d_slot, uint32_t count,
+                                                    int32_t kernel, void* stream);
+
 /* Same trace, additionally writing per ray (inner nodes fetched, triangles
  * tested) to d_visits (2 x uint32 per ray).  The counts define the
  * algorithmic bytes of a trace: 64 B per node (GPUBVHNode) and 48 B per
