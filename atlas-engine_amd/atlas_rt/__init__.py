@@ -577,6 +577,14 @@ class PathTracer:
     def render(self, cam, frames=1, stream=None):
         check(lib().atlas_rt_pathtracer_render(self._h, C.byref(cam), frames, _stream_ptr(stream)))
 
+    def capture(self, cam, frames=1):
+        """Capture `frames` progressive frames of `cam` as a HIP graph."""
+        check(lib().atlas_rt_pathtracer_capture(self._h, C.byref(cam), frames))
+
+    def replay(self, stream=None):
+        """Render the captured frames again (render(cam, frames) as of now)."""
+        check(lib().atlas_rt_pathtracer_replay(self._h, _stream_ptr(stream)))
+
     def set_realtime(self, samples_per_frame=4, history_length_max=32, history_clip_max=0.2,
                      current_clip_factor=2.0, max_radiance=65535.0, enable=True):
         """Realtime mode (defaults of PathTracingRenderer.h:38-43), or off."""
@@ -622,6 +630,18 @@ class PathTracer:
         n = C.c_int32()
         check(lib().atlas_rt_pathtracer_sample_count(self._h, C.byref(n)))
         return n.value
+
+    WORK_FIELDS = ("closest_rays", "shadow_rays", "hits", "nodes", "triangles", "instances")
+
+    def count_work(self, enable=True):
+        """Start (from zero) or stop counting the frames' traversal work."""
+        check(lib().atlas_rt_pathtracer_count_work(self._h, 1 if enable else 0))
+
+    def work_counts(self):
+        """The traversal work since count_work(): a dict of WORK_FIELDS."""
+        c = np.zeros(len(self.WORK_FIELDS), np.uint64)
+        check(lib().atlas_rt_pathtracer_work_counts(self._h, _ptr(c)))
+        return {k: int(v) for k, v in zip(self.WORK_FIELDS, c)}
 
     def set_textures(self, textures):
         """Material textures: a list of (height, width) or (height, width,

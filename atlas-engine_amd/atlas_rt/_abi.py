@@ -201,6 +201,10 @@ PROTOTYPES = {
     "atlas_rt_batch_graph_launch": (status, [vp, vp]),
     "atlas_rt_aabb_tree_build": (status, [vp, u32, u32, vp, C.POINTER(u32), vp, vp, C.POINTER(u32)]),
     "atlas_rt_blas_set_triangles": (status, [vp, vp, u32]),
+    "atlas_rt_pathtracer_count_work": (status, [vp, i32]),
+    "atlas_rt_pathtracer_capture": (status, [vp, vp, u32]),
+    "atlas_rt_pathtracer_replay": (status, [vp, vp]),
+    "atlas_rt_pathtracer_work_counts": (status, [vp, vp]),
     "atlas_rt_blas_set_shading": (status, [vp, vp, vp]),
     "atlas_rt_blas_set_opacity": (status, [vp, vp, vp]),
     "atlas_rt_blas_set_tangents": (status, [vp, vp]),

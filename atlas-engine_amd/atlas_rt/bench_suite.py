@@ -119,20 +119,44 @@ def c2(torch, log, frames=16):
              "instance": d_pi.cpu().numpy().copy()}
     res = {}
     imgs = {}
-    # (ray binning, frames in flight): the library's default, binned, and one frame at a time
-    for binning, fif in ((False, None), (True, None), (False, 1)):
+    # (ray binning, frames in flight, bounce-by-bounce launches): the
+    # library's default (resident frame kernel), binned, one frame at a time,
+    # and the engine's per-bounce dispatch shape
+    shapes = ((False, None, False), (True, None, False), (False, 1, False), (False, None, True), (False, 1, True))
+    for binning, fif, wavefront in shapes:
         pt = PathTracer(w, cc["width"], cc["height"], 4, mats, W.sun_light(), env=W.C2_ENV, binning=binning,
-                        frames_in_flight=fif)
+                        frames_in_flight=fif, wavefront=wavefront)
         pt.render(cam, 2)
         pt.reset()
         t = _sync_time(torch, lambda: pt.render(cam, frames))
-        res[(binning, fif)] = t / frames
-        imgs[(binning, fif)] = pt.read().cpu().numpy()
+        res[(binning, fif, wavefront)] = t / frames
+        imgs[(binning, fif, wavefront)] = pt.read().cpu().numpy()
         del pt
-    same = bool(imgs[(False, None)].tobytes() == imgs[(True, None)].tobytes())
-    same_serial = bool(imgs[(False, None)].tobytes() == imgs[(False, 1)].tobytes())
-    res[False], res[True] = res[(False, None)], res[(True, None)]
+    base = imgs[(False, None, False)].tobytes()
+    same = bool(base == imgs[(True, None, False)].tobytes())
+    same_serial = bool(base == imgs[(False, 1, False)].tobytes())
+    same_wavefront = all(base == imgs[(False, f, True)].tobytes() for f in (None, 1))
+    res[False], res[True] = res[(False, None, False)], res[(True, None, False)]
     px = cc["width"] * cc["height"]
+    # traversal work of one frame (a counting run outside the timed ones):
+    # algorithmic bytes = 64 B per inner node (TLAS or BLAS), 48 B per
+    # triangle test, 64 B per instance record, plus per closest hit the
+    # 96 B shading record and 96 B material the shading reads
+    pt = PathTracer(w, cc["width"], cc["height"], 4, mats, W.sun_light(), env=W.C2_ENV)
+    pt.count_work()
+    pt.render(cam, 1)
+    work = pt.work_counts()
+    del pt
+    # a batch of 8 frames captured once as a HIP graph and replayed
+    pt = PathTracer(w, cc["width"], cc["height"], 4, mats, W.sun_light(), env=W.C2_ENV)
+    pt.render(cam, 2)
+    pt.capture(cam, 8)
+    pt.replay()
+    t_graph = _sync_time(torch, lambda: [pt.replay() for _ in range(frames // 8)]) / (8 * (frames // 8))
+    del pt
+    trav_bytes = 64 * work["nodes"] + 48 * work["triangles"] + 64 * work["instances"]
+    frame_bytes = trav_bytes + 192 * work["hits"]
+    achieved = frame_bytes / res[False] / 1e9
     # realtime mode at the engine's defaults (4 samples per pixel, temporal resolve)
     from . import frame_matrices, projection_matrix, view_matrix
     pt = PathTracer(w, cc["width"], cc["height"], 4, mats, W.sun_light(), env=W.C2_ENV)
@@ -146,7 +170,9 @@ def c2(torch, log, frames=16):
     del pt
     log(f"c2 realtime: {t_rt * 1e3:.3f} ms/frame at 4 samples per pixel with the temporal resolve")
     log(f"c2: {res[False] * 1e3:.3f} ms/frame plain, {res[True] * 1e3:.3f} ms/frame binned (same image: {same}), "
-        f"{res[(False, 1)] * 1e3:.3f} ms/frame one frame at a time (same image: {same_serial})")
+        f"{res[(False, 1, False)] * 1e3:.3f} ms/frame one frame at a time (same image: {same_serial}); "
+        f"bounce-by-bounce {res[(False, None, True)] * 1e3:.3f} / {res[(False, 1, True)] * 1e3:.3f} "
+        f"(same image: {same_wavefront}); {frame_bytes / 1e6:.0f} MB/frame algorithmic, {achieved:.0f} GB/s")
     return {
         "workload": "c2: 256 instances of 8 random-blob meshes, 1280x720, 4 bounces, sun + constant env, "
                     "one sample per pixel per frame",
@@ -155,7 +181,19 @@ def c2(torch, log, frames=16):
         "ms_per_frame": round(res[False] * 1e3, 3),
         "mpaths_per_s": round(px / res[False] / 1e6, 1),
         "frames_in_flight": 3,
-        "one_frame_at_a_time_ms_per_frame": round(res[(False, 1)] * 1e3, 3),
+        "launch_shape": "resident frame kernel, up to 8 frames per launch",
+        "graph_replay_ms_per_frame": round(t_graph * 1e3, 3),
+        "graph_replay_note": "8 frames captured once as a HIP graph (atlas_rt_pathtracer_capture), replayed",
+        "one_frame_at_a_time_ms_per_frame": round(res[(False, 1, False)] * 1e3, 3),
+        "bounce_by_bounce_ms_per_frame": round(res[(False, None, True)] * 1e3, 3),
+        "bounce_by_bounce_one_frame_at_a_time_ms_per_frame": round(res[(False, 1, True)] * 1e3, 3),
+        "bounce_by_bounce_image_identical": same_wavefront,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
+                     "frac": round(achieved / 8000.0, 4), "alg_bytes_per_frame": int(frame_bytes),
+                     "traversal_bytes_per_frame": int(trav_bytes), "work_per_frame": work,
+                     "note": "algorithmic bytes of one frame from the kernel's own work counters (64 B per node, "
+                             "48 B per triangle test, 64 B per instance, 192 B of shading reads per hit) over "
+                             "the measured ms/frame"},
         "in_flight_image_identical": same_serial,
         "binned_ms_per_frame": round(res[True] * 1e3, 3),
         "binning_speedup": round(res[False] / res[True], 3),
@@ -253,8 +291,23 @@ def c3(torch, log):
     check = {"batches": []}
     for b, o, c in table[:n_check]:
         check["batches"].append((blases[b].export(), rays[o:o + c], hits_all[o:o + c].copy()))
+    # statistics pass (outside the timed launches): every batch's inner
+    # nodes and triangle tests through the engine-order kernel, for the
+    # algorithmic bytes of a launch (48 B per ray, 64 B per node, 48 B per
+    # triangle test)
+    d_vis = torch.empty((n, 2), dtype=torch.int32, device="cuda")
+    d_sh = torch.empty((n, 4), dtype=torch.float32, device="cuda")
+    for b, o, c in table:
+        blases[b].trace_closest_stats(d_rays[o:o + c], d_sh[o:o + c], d_vis[o:o + c])
+    torch.cuda.synchronize()
+    vis = d_vis.cpu().numpy().astype(np.int64)
+    same = bool(torch.equal(d_sh, d_hits))
+    del d_vis, d_sh
+    nodes, tests = int(vis[:, 0].sum()), int(vis[:, 1].sum())
+    alg = 48 * n + 64 * nodes + 48 * tests
+    achieved = alg / tl / 1e9
     log(f"c3: {n / tl / 1e6:.0f} Mrays/s launch (engine-exact {best}), {n / tg / 1e6:.0f} Mrays/s graph; "
-        f"tuner's pick {tuned}: {n / tt / 1e6:.0f} Mrays/s")
+        f"tuner's pick {tuned}: {n / tt / 1e6:.0f} Mrays/s; {alg / 1e9:.2f} GB algorithmic, {achieved:.0f} GB/s")
     # the same meshes built on the GPU (linear BVH), traced by an autotuned launch
     del bs
     t = time.perf_counter()
@@ -277,6 +330,12 @@ def c3(torch, log):
         "tuner_pick": tuned, "tuner_pick_ms": round(tt * 1e3, 3),
         "trials": [{"config": c, "ms": round(ms, 3)} for c, ms in trials],
         "ms": round(tl * 1e3, 3), "graph_ms": round(tg * 1e3, 3),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
+                     "frac": round(achieved / 8000.0, 4), "alg_bytes_per_launch": alg,
+                     "node_visits_per_ray": round(nodes / n, 3), "tri_tests_per_ray": round(tests / n, 3),
+                     "stats_hits_equal_launch": same,
+                     "note": "48 B per ray + 64 B per inner node + 48 B per triangle test, counted per batch by "
+                             "the engine-order statistics kernel, over the engine-exact launch's measured time"},
         "mrays_per_s": round(n / tl / 1e6, 1), "graph_mrays_per_s": round(n / tg / 1e6, 1),
         "_check": check,
         "device_build": {

@@ -100,12 +100,14 @@ void free_realtime(PathTracer& P) {
 void free_all(PathTracer& P) {
     void* ptrs[] = {P.ctl, P.accum, const_cast<float4**>(P.shade_table), const_cast<float4**>(P.uv_table),
                     const_cast<float4**>(P.tangent_table), const_cast<float4**>(P.color_table), P.materials,
-                    P.lights, P.tex, P.texels};
+                    P.lights, P.tex, P.texels, P.stats};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     free_cache_buffers(P);
     free_realtime(P);
     for (FrameSlot& S : P.slot) free_slot(S);
+    if (P.graph) (void)hipGraphExecDestroy(P.graph);
+    P.graph = nullptr;
     if (P.ev_start) (void)hipEventDestroy(P.ev_start);
     P.ev_start = nullptr;
 }
@@ -524,6 +526,82 @@ atlas_rt_status atlas_rt_pathtracer_render(atlas_rt_pathtracer* pt, const atlas_
     return ATLAS_RT_OK;
 }
 
+atlas_rt_status atlas_rt_pathtracer_capture(atlas_rt_pathtracer* pt, const atlas_rt_camera* camera, uint32_t frames) {
+    if (!pt || !camera) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_capture: NULL argument");
+    PathTracer& P = pt->P;
+    if (P.realtime || !resident_frame(P))
+        return fail(ATLAS_RT_E_UNSUPPORTED, "atlas_rt_pathtracer_capture: progressive frames of the resident frame "
+                                            "kernel only (no realtime mode, radiance cache, binning or "
+                                            "ATLAS_RT_PT_WAVEFRONT)");
+    if (frames < 1 || frames > P.batch)
+        return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_capture: frames must be 1.." +
+                                                std::to_string(P.batch));
+    atlas_rt_status st = bind_world(P);
+    if (st == ATLAS_RT_OK) st = apply_stack_cap(P);
+    if (st != ATLAS_RT_OK) return st;
+    Camera cam;
+    std::memcpy(cam.loc, camera->location, 12);
+    std::memcpy(cam.origin, camera->origin, 12);
+    std::memcpy(cam.right, camera->right, 12);
+    std::memcpy(cam.bottom, camera->bottom, 12);
+    // slot 0 is idle from here on: no frame of an earlier call still runs
+    ATLAS_HIP_TRY(hipDeviceSynchronize());
+    if (P.graph) (void)hipGraphExecDestroy(P.graph);
+    P.graph = nullptr;
+    P.graph_frames = 0;
+    hipStream_t cs = P.slot[0].stream;
+    ATLAS_HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed));
+    st = record_frames(P, cam, frames, cs);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(cs, &g);  // ends the capture whatever happened
+    if (st != ATLAS_RT_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    hipGraphExec_t x = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) return hip_fail(ei, "hipGraphInstantiate");
+    P.graph = x;
+    P.graph_frames = frames;
+    P.graph_key = graph_key(P);
+    return ATLAS_RT_OK;
+}
+
+atlas_rt_status atlas_rt_pathtracer_replay(atlas_rt_pathtracer* pt, void* stream) {
+    if (!pt) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_replay: NULL path tracer");
+    PathTracer& P = pt->P;
+    if (!P.graph) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_replay: nothing captured");
+    atlas_rt_status st = bind_world(P);
+    if (st == ATLAS_RT_OK) st = apply_stack_cap(P);
+    if (st != ATLAS_RT_OK) return st;
+    const GraphKey now = graph_key(P);
+    if (P.realtime || !resident_frame(P) || std::memcmp(&now, &P.graph_key, sizeof(now)) != 0)
+        return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_replay: the path tracer changed since the capture "
+                                            "(world update, stack limit, materials, lights, textures, mode or work "
+                                            "counting); capture again");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    // after every frame of earlier calls, on whatever slot it ran
+    const uint32_t n = P.slots;
+    for (uint32_t k = 0; k < n; k++) {
+        ATLAS_HIP_TRY(hipStreamWaitEvent(s, P.slot[k].ev_begin, 0));
+        ATLAS_HIP_TRY(hipStreamWaitEvent(s, P.slot[k].ev_acc, 0));
+    }
+    ATLAS_HIP_TRY(hipGraphLaunch(P.graph, s));
+    // later frames order after these as after a frame of slot 0
+    FrameSlot& S = P.slot[0];
+    ATLAS_HIP_TRY(hipEventRecord(S.ev_begin, s));
+    ATLAS_HIP_TRY(hipEventRecord(S.ev_train, s));
+    ATLAS_HIP_TRY(hipEventRecord(S.ev_acc, s));
+    P.hist_prev2 = P.hist_prev;
+    P.hist_prev = 0;
+    P.last_slot = 0;
+    P.next_slot = 1 % n;
+    P.host_samples += int32_t(P.graph_frames);
+    return ATLAS_RT_OK;
+}
+
 atlas_rt_status atlas_rt_pathtracer_set_realtime(atlas_rt_pathtracer* pt, const atlas_rt_realtime_params* params) {
     if (!pt) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_set_realtime: NULL path tracer");
     PathTracer& P = pt->P;
@@ -736,6 +814,34 @@ atlas_rt_status atlas_rt_pathtracer_status(atlas_rt_pathtracer* pt, void* stream
 atlas_rt_status atlas_rt_pathtracer_sample_count(const atlas_rt_pathtracer* pt, int32_t* count) {
     if (!pt || !count) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_sample_count: NULL argument");
     *count = pt->P.host_samples;
+    return ATLAS_RT_OK;
+}
+
+atlas_rt_status atlas_rt_pathtracer_count_work(atlas_rt_pathtracer* pt, int32_t enable) {
+    if (!pt) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_count_work: NULL path tracer");
+    PathTracer& P = pt->P;
+    ATLAS_HIP_TRY(hipDeviceSynchronize());
+    if (P.stats) (void)hipFree(P.stats);
+    P.stats = nullptr;
+    P.stats_partial = false;
+    if (!enable) return ATLAS_RT_OK;
+    atlas_rt_status st = dev_alloc(P.stats, kWorkCounters);
+    if (st != ATLAS_RT_OK) return st;
+    ATLAS_HIP_TRY(hipMemset(P.stats, 0, kWorkCounters * sizeof(unsigned long long)));
+    return ATLAS_RT_OK;
+}
+
+atlas_rt_status atlas_rt_pathtracer_work_counts(atlas_rt_pathtracer* pt, uint64_t* counts) {
+    if (!pt || !counts) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_work_counts: NULL argument");
+    const PathTracer& P = pt->P;
+    if (!P.stats) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_work_counts: counting is off");
+    if (P.stats_partial)
+        return fail(ATLAS_RT_E_UNSUPPORTED, "atlas_rt_pathtracer_work_counts: some frames ran bounce-by-bounce "
+                                            "launches or transparency walks, which are not counted");
+    unsigned long long c[kWorkCounters];
+    ATLAS_HIP_TRY(hipDeviceSynchronize());
+    ATLAS_HIP_TRY(hipMemcpy(c, P.stats, sizeof(c), hipMemcpyDeviceToHost));
+    for (int i = 0; i < kWorkCounters; i++) counts[i] = uint64_t(c[i]);
     return ATLAS_RT_OK;
 }
 

@@ -66,6 +66,8 @@ constexpr uint32_t kDefaultFrameSlots = 3;
 // Progressive frames one resident launch renders: the launch's tail (its
 // slowest paths) is paid once per batch, not once per frame.
 constexpr uint32_t kBatchFrames = 8;
+// Counters of atlas_rt_pathtracer_work_counts.
+constexpr int kWorkCounters = 6;
 
 // The resident frame kernel's launch shape: 256-lane groups, four per CU
 // (four waves per SIMD), a 24-entry LDS stack per lane; deeper entries spill
@@ -79,6 +81,18 @@ constexpr int kGridLds = 32;
 // Spill bytes a slot needs for `rays` paths with `cap` stack entries, for
 // either launch shape.
 size_t slot_spill_bytes(uint32_t rays, uint32_t cap);
+
+// What the kernels of a captured frame batch were given: slot 0's argument
+// block and buffers, the stack bound and the work counters.
+struct GraphKey {
+    ShadeArgs args;
+    const void* spill;
+    const void* flags;
+    const void* accum;
+    const void* stats;
+    uint32_t cap;
+    uint64_t world_gen;
+};
 
 struct PathTracer {
     uint32_t width = 0, height = 0;
@@ -130,6 +144,17 @@ struct PathTracer {
     // progressive frames per resident launch (each slot holds this many
     // frames' control blocks and radiance buffers)
     uint32_t batch = 1;
+    // traversal work counters of the resident frame kernel (null: off):
+    // closest rays, shadow rays, hits, nodes, triangles, instances; partial
+    // once a frame ran a launch shape that does not count
+    unsigned long long* stats = nullptr;
+    bool stats_partial = false;
+    // a captured batch of progressive frames (atlas_rt_pathtracer_capture):
+    // the graph and what its kernels were given, to refuse a replay after
+    // the path tracer's buffers or bindings changed
+    hipGraphExec_t graph = nullptr;
+    uint32_t graph_frames = 0;
+    GraphKey graph_key{};
     ShadeArgs shade_args{};       // every field but the slot's buffers
     int32_t host_samples = 0;
     uint32_t rays = 0;            // paths per frame: pixels, times the samples per frame in realtime mode
@@ -168,6 +193,12 @@ atlas_rt_status launch_frame(PathTracer& P, const Camera& cam, uint32_t k, const
 // Whether frames run as the resident frame kernel (plain path tracing: the
 // radiance cache and ray binning need the bounce-by-bounce queues).
 bool resident_frame(const PathTracer& P);
+// The kernels of `frames` progressive frames in slot 0 on `s`, with no event
+// or wait (for graph capture): seeds, the resident frame kernel, the
+// accumulation.
+atlas_rt_status record_frames(PathTracer& P, const Camera& cam, uint32_t frames, hipStream_t s);
+// Slot 0's argument block and buffers as a captured batch sees them.
+GraphKey graph_key(const PathTracer& P);
 // query_lag 1: both network snapshots from the live network, on `s`.
 atlas_rt_status refresh_snapshots(PathTracer& P, hipStream_t s);
 // The temporal resolve's inputs and output (pt_temporal.hip,

@@ -707,7 +707,7 @@ __global__ __launch_bounds__(BLOCK) void k_shadow(WorldDev W, const FrameCtl* __
 // starts: the frame's only tail is its slowest whole path.  Lanes that trace
 // closest hits and lanes that trace shadow rays share one step body over
 // opaque worlds (world_step_mixed).  Shading is batched per wave: the wave
-// keeps stepping until a quarter of its busy lanes have finished a walk (or
+// keeps stepping until half of its busy lanes have finished a walk (or
 // all have), then those lanes shade or add their shadow ray's light
 // together.  Paths refill idle lanes between batches (one atomic per wave).
 //
@@ -718,7 +718,7 @@ __global__ __launch_bounds__(BLOCK) void k_shadow(WorldDev W, const FrameCtl* __
 // ---------------------------------------------------------------------------
 constexpr int kFrameBlock = kResidentBlock;
 constexpr int kFrameLds = kResidentLds;
-constexpr uint32_t kShadeDiv = 4;
+constexpr uint32_t kShadeDiv = 2;
 
 // The pixel of the k-th path slot of the raygen kernels' 8x8-tile order.
 __device__ __forceinline__ uint32_t tile_pixel(uint32_t k, uint32_t width, uint32_t height) {
@@ -739,9 +739,20 @@ struct FrameGen {
     float jx, jy;         // realtime camera jitter
 };
 
-template <bool ALPHA>
+// Sum of a lane value over the wave.
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// With STATS the kernel also counts its traversal work into stats[0..5]:
+// closest-hit rays, shadow rays, closest hits, inner nodes fetched (both
+// levels), triangles tested and instances entered; the walks themselves are
+// unchanged.
+template <bool ALPHA, bool STATS>
 __global__ __launch_bounds__(kFrameBlock, 4) void k_frame_persistent(ShadeArgs A, FrameGen G, int32_t* __restrict__ spill,
-                                                                  uint32_t cap, uint32_t* __restrict__ flags) {
+                                                                  uint32_t cap, uint32_t* __restrict__ flags,
+                                                                  unsigned long long* __restrict__ stats) {
     __shared__ int32_t lds[kFrameLds * kFrameBlock];
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     const LaneStack<kFrameBlock, kFrameLds> S{lds, spill, tid, blockIdx.x * kFrameBlock + tid,
@@ -767,6 +778,7 @@ __global__ __launch_bounds__(kFrameBlock, 4) void k_frame_persistent(ShadeArgs A
     bool exhausted = false;
     const FrameCtl* ctl = ctl0;  // the frame of the lane's path
     float4* Lpix = A.Lpix;       // that frame's radiance per ray ID
+    uint32_t c_closest = 0, c_shadow = 0, c_hits = 0, c_nodes = 0, c_tris = 0, c_insts = 0;
 
     for (;;) {
         if (!exhausted) {
@@ -811,6 +823,7 @@ __global__ __launch_bounds__(kFrameBlock, 4) void k_frame_persistent(ShadeArgs A
                         world_begin(L, make_float4(p0.x, p0.y, p0.z, 0.0f), make_float4(p1.x, p1.y, p1.z, kInf));
                         ready = any_nan3(p1.x, p1.y, p1.z);
                         rstat = kDone;
+                        if (STATS) c_closest++;
                     }
                 }
                 if (base + n >= total) exhausted = true;
@@ -834,6 +847,16 @@ __global__ __launch_bounds__(kFrameBlock, 4) void k_frame_persistent(ShadeArgs A
                                                                             cap, op);
                     }
                 } else {
+                    if (STATS) {
+                        // the step's kind, as world_step_mixed decides it
+                        const bool top = L.sp + 1 < L.tlas_index;
+                        if (top && L.node < 0)
+                            c_insts++;
+                        else if (top || L.node >= 0)
+                            c_nodes++;
+                        else
+                            c_tris++;
+                    }
                     r = world_step_mixed<kFrameBlock, kFrameLds>(L, W, phase == 1 ? ATLAS_RT_MASK_ALL
                                                                                    : ATLAS_RT_MASK_SHADOW,
                                                                  phase == 2, S, cap);
@@ -857,6 +880,7 @@ __global__ __launch_bounds__(kFrameBlock, 4) void k_frame_persistent(ShadeArgs A
                 slot = L.slot;
                 inst = L.inst;
             }
+            if (STATS) c_hits += slot >= 0;
             ShadeOut so;
             shade_path(A, ctl, Lpix, bounce, p0, p1, p2, make_int4(__float_as_int(L.hitT), slot, inst, 0), so);
             pending = so.cont;
@@ -871,6 +895,7 @@ __global__ __launch_bounds__(kFrameBlock, 4) void k_frame_persistent(ShadeArgs A
                             make_float4(so.sh[1].x, so.sh[1].y, so.sh[1].z, so.sh[1].w));
                 phase = 2;
                 ready = false;
+                if (STATS) c_shadow++;
                 continue;
             }
         } else {
@@ -900,10 +925,18 @@ __global__ __launch_bounds__(kFrameBlock, 4) void k_frame_persistent(ShadeArgs A
             phase = 1;
             ready = any_nan3(p1.x, p1.y, p1.z);
             rstat = kDone;
+            if (STATS) c_closest++;
         } else {
             phase = 0;
             ready = false;
         }
+    }
+    if (STATS) {
+        // every lane of the wave leaves the loop together
+        const uint32_t s[6] = {wave_sum(c_closest), wave_sum(c_shadow), wave_sum(c_hits),
+                               wave_sum(c_nodes),   wave_sum(c_tris),   wave_sum(c_insts)};
+        if (lane == 0)
+            for (int i = 0; i < 6; i++) atomicAdd(&stats[i], (unsigned long long)s[i]);
     }
 }
 
@@ -994,10 +1027,18 @@ atlas_rt_status launch_resident(PathTracer& P, uint32_t k, const ShadeArgs& A, c
     const uint32_t groups = uint32_t(device_cu_count()) * kResidentGroupsPerCu;
     int32_t* spill = reinterpret_cast<int32_t*>(S.spill.ptr);
     uint32_t* flags = reinterpret_cast<uint32_t*>(S.flags.ptr);
-    if (P.alpha)
-        hipLaunchKernelGGL(k_frame_persistent<true>, dim3(groups), dim3(kFrameBlock), 0, s, A, G, spill, P.cap, flags);
-    else
-        hipLaunchKernelGGL(k_frame_persistent<false>, dim3(groups), dim3(kFrameBlock), 0, s, A, G, spill, P.cap, flags);
+    if (P.alpha) {
+        // the transparency walks test a whole leaf per step: not counted
+        if (P.stats) P.stats_partial = true;
+        hipLaunchKernelGGL((k_frame_persistent<true, false>), dim3(groups), dim3(kFrameBlock), 0, s, A, G, spill, P.cap,
+                           flags, nullptr);
+    } else if (P.stats) {
+        hipLaunchKernelGGL((k_frame_persistent<false, true>), dim3(groups), dim3(kFrameBlock), 0, s, A, G, spill, P.cap,
+                           flags, P.stats);
+    } else {
+        hipLaunchKernelGGL((k_frame_persistent<false, false>), dim3(groups), dim3(kFrameBlock), 0, s, A, G, spill,
+                           P.cap, flags, nullptr);
+    }
     ATLAS_HIP_TRY(hipGetLastError());
     return ATLAS_RT_OK;
 }
@@ -1006,10 +1047,37 @@ atlas_rt_status launch_resident(PathTracer& P, uint32_t k, const ShadeArgs& A, c
 
 bool resident_frame(const PathTracer& P) { return !P.shade_args.cache.enabled && !P.binning && !P.wavefront; }
 
+GraphKey graph_key(const PathTracer& P) {
+    GraphKey key;
+    std::memset(&key, 0, sizeof(key));  // padding too: keys are compared bytewise
+    key.args = slot_args(P, 0);
+    key.spill = P.slot[0].spill.ptr;
+    key.flags = P.slot[0].flags.ptr;
+    key.accum = P.accum;
+    key.stats = P.stats;
+    key.cap = P.cap;
+    key.world_gen = P.world_gen;
+    return key;
+}
+
+atlas_rt_status record_frames(PathTracer& P, const Camera& cam, uint32_t frames, hipStream_t s) {
+    FrameSlot& S = P.slot[0];
+    const uint32_t pixels = P.width * P.height;
+    hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(64), 0, s, P.ctl, S.ctl, P.bounces, pixels, 1u, frames);
+    const FrameGen G{cam, P.width, P.height, pixels, frames, 0u, 0.0f, 0.0f};
+    const atlas_rt_status rs = launch_resident(P, 0, slot_args(P, 0), G, s);
+    if (rs != ATLAS_RT_OK) return rs;
+    hipLaunchKernelGGL(k_accumulate, dim3((pixels + kBlock - 1) / kBlock), dim3(kBlock), 0, s, S.Lpix, P.accum,
+                       pixels, frames);
+    ATLAS_HIP_TRY(hipGetLastError());
+    return ATLAS_RT_OK;
+}
+
 atlas_rt_status launch_bounces(PathTracer& P, uint32_t k, hipStream_t s, hipEvent_t train_ready,
                                const nrc::NrcDev* net) {
     FrameSlot& S = P.slot[k];
     const uint32_t rays = P.rays;
+    if (P.stats) P.stats_partial = true;  // only the resident frame kernel counts
     const uint32_t grid_all = (rays + kBlock - 1) / kBlock;
     uint32_t* flags = reinterpret_cast<uint32_t*>(S.flags.ptr);
     const ShadeArgs base = slot_args(P, k);

@@ -348,6 +348,17 @@ def probe_c2(binning, fif=None, wavefront=False):
     torch.cuda.synchronize()
     print(f"c2 binning={binning} fif={fif} wavefront={wavefront}: {(time.perf_counter() - t) / 16 * 1e3:.3f} "
           "ms/frame", flush=True)
+    if not (binning or wavefront):
+        # one batch of 8 frames through the counting kernel: the traversal
+        # work behind each frame (algorithmic bytes: 64 B per node, 48 B per
+        # triangle test, 64 B per instance, 192 B of shading reads per hit)
+        p.count_work()
+        p.render(cam, 8)
+        c = p.work_counts()
+        p.count_work(False)
+        b = 64 * c["nodes"] + 48 * c["triangles"] + 64 * c["instances"] + 192 * c["hits"]
+        print(f"c2 work per frame: " + ", ".join(f"{k} {v / 8:.0f}" for k, v in c.items()) +
+              f"; {b / 8 / 1e6:.1f} MB algorithmic", flush=True)
 
 
 def probe_realtime(wavefront=False):
@@ -592,6 +603,7 @@ def main():
     stream = torch.cuda.current_stream()
     kernel = args.kernel
     variants = {}
+    fastest = None
     if kernel == 0:
         # pick the fastest traversal variant (outside the timed region)
         for k in (1, 2, 3, 4, 1 | REG, 2 | REG, 3 | REG, 4 | REG, 1 | FSLAB, 2 | FSLAB, 1 | REG | FSLAB,
@@ -606,11 +618,10 @@ def main():
             e1.record(stream)
             torch.cuda.synchronize()
             variants[k] = e0.elapsed_time(e1) / 5
-        # the engine-exact kernels win unless another variant (same nearest
-        # hit, other traversal order) is more than 3% faster
-        exact = min((k for k in variants if k in EXACT_KERNELS), key=variants.get)
+        # the headline is the fastest engine-exact kernel (the engine's
+        # slots bit for bit); the fastest of all is reported beside it
+        kernel = min((k for k in variants if k in EXACT_KERNELS), key=variants.get)
         fastest = min(variants, key=variants.get)
-        kernel = fastest if variants[fastest] < 0.97 * variants[exact] else exact
     for _ in range(args.warmup):
         blas.trace_closest(d_rays, d_hits, kernel=kernel, stream=stream)
     torch.cuda.synchronize()
@@ -655,6 +666,9 @@ def main():
             "bvh_nodes": int(len(nodes)),
             "kernel": KERNEL_NAMES.get(kernel, str(kernel)),
             "variant_ms": {KERNEL_NAMES.get(k, str(k)): round(v, 4) for k, v in variants.items()},
+            "fastest_variant": KERNEL_NAMES.get(fastest, str(fastest)) if variants else None,
+            "kernel_note": "headline: the fastest engine-exact kernel; variants marked 'same nearest hit' may "
+                           "return another slot at equal-t ties" if variants else "kernel given by --kernel",
         },
         "roofline": {
             "bound": "hbm",

@@ -235,7 +235,12 @@ atlas_rt_status atlas_rt_blas_export(const atlas_rt_blas* blas, atlas_rt_gpu_bvh
 /* Closest hit, engine semantics of HitClosest (bvh.hsh:191-273, CheckLeafClosest
  * :44-72, IntersectAABB/IntersectTriangle intersections.hsh:18-58): accept
  * t > tmin, t < tmax, t < current t; first accepted wins equal t.  d_slot (may
- * be NULL) receives the BVH slot (the engine's hitID) per ray. */
+ * be NULL) receives the BVH slot (the engine's hitID) per ray.
+ * Ordering: the traces of one BLAS (closest, any, stats) keep their ray and
+ * wave counters, traversal stack spill and overflow flag in the BLAS, so they
+ * are stream-ordered: traces of the same BLAS on different streams must not
+ * overlap (make one stream wait on an event recorded after the other's
+ * trace).  Traces of different BLASes are independent. */
 atlas_rt_status atlas_rt_trace_closest_rays(const atlas_rt_blas* blas, const atlas_rt_ray* d_rays,
                                             atlas_rt_hit* d_hits, int32_t* d_slot, uint32_t count,
                                             int32_t kernel, void* stream);
@@ -537,7 +542,9 @@ atlas_rt_status atlas_rt_trace_any(const atlas_rt_world* world, const atlas_rt_p
 
 /* Compact two-level traces (per-ray tmin/tmax): hits carry the caller's
  * triangle index in the hit BLAS, d_instance (may be NULL) the caller's
- * instance index, -1 on a miss. */
+ * instance index, -1 on a miss.  Like BLAS traces, the traces of one world
+ * share the world's stack spill and overflow flag: traces of the same world
+ * on different streams must not overlap. */
 atlas_rt_status atlas_rt_world_trace_closest_rays(const atlas_rt_world* world, const atlas_rt_ray* d_rays,
                                                   atlas_rt_hit* d_hits, int32_t* d_instance, uint32_t count,
                                                   uint32_t mask, void* stream);
@@ -802,6 +809,33 @@ atlas_rt_status atlas_rt_pathtracer_reset(atlas_rt_pathtracer* pt);
  * width*height float4; d_out is a device buffer.  Synchronous on `stream`. */
 atlas_rt_status atlas_rt_pathtracer_read(atlas_rt_pathtracer* pt, float* d_out, void* stream);
 atlas_rt_status atlas_rt_pathtracer_sample_count(const atlas_rt_pathtracer* pt, int32_t* count);
+/* No engine counterpart: traversal work counters, the measure of a frame's
+ * algorithmic bytes (64 B per inner node, TLAS or BLAS, 48 B per triangle
+ * test, 64 B per instance record).  enable = 1 starts counting from zero,
+ * 0 stops; frames run as before (same walks, same image) through a counting
+ * instance of the resident frame kernel.  atlas_rt_pathtracer_work_counts
+ * waits for every frame and writes counts[0..5]: closest-hit rays, shadow
+ * rays, closest hits, inner nodes fetched, triangles tested and instances
+ * entered since counting started; ATLAS_RT_E_UNSUPPORTED if some frame ran
+ * bounce-by-bounce launches (ATLAS_RT_PT_WAVEFRONT, binning, the radiance
+ * cache) or transparency walks, which do not count. */
+atlas_rt_status atlas_rt_pathtracer_count_work(atlas_rt_pathtracer* pt, int32_t enable);
+/* No engine counterpart (the engine records its frame into a command list
+ * each frame, PathTracingRenderer.cpp:51-208): the kernels of `frames`
+ * progressive frames (1 to 8) of `camera` captured once as a HIP graph.
+ * atlas_rt_pathtracer_replay launches it on `stream`, after every frame of
+ * earlier calls; each replay renders what atlas_rt_pathtracer_render(camera,
+ * frames) would at that point (the seed stream and sample count continue),
+ * the same image bit for bit, and later render calls order after it.  Plain
+ * progressive path tracing only: realtime mode, the radiance cache, binning
+ * and ATLAS_RT_PT_WAVEFRONT are refused.  A replay after anything that
+ * changes what the kernels were given (a world update or stack limit,
+ * materials, lights, textures, work counting) returns
+ * ATLAS_RT_E_INVALID_ARG: capture again.  The graph clears its counters with
+ * kernels, not memsets. */
+atlas_rt_status atlas_rt_pathtracer_capture(atlas_rt_pathtracer* pt, const atlas_rt_camera* camera, uint32_t frames);
+atlas_rt_status atlas_rt_pathtracer_replay(atlas_rt_pathtracer* pt, void* stream);
+atlas_rt_status atlas_rt_pathtracer_work_counts(atlas_rt_pathtracer* pt, uint64_t* counts);
 /* Waits for `stream` and every frame in flight, then reads and clears the
  * path tracer's overflow flags: ATLAS_RT_E_STACK_OVERFLOW if a frame since
  * the last call stopped a path at its world's stack limit

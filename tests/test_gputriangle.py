@@ -207,7 +207,7 @@ def test_gpu_pt_from_packed_records_matches_oracle(torch_cuda):
     close = np.all(np.abs(a - r) <= 1e-4 + 1e-3 * np.abs(r), axis=-1)
     same = np.all(img.view(np.uint32) == ref.view(np.uint32), axis=-1)
     print(f"pt packed: {close.mean():.4f} of pixels within tolerance, {same.mean():.4f} bit for bit")
-    assert close.mean() > 0.99 and abs(a.mean() - r.mean()) <= 0.005 * r.mean()
+    assert close.all() and abs(a.mean() - r.mean()) <= 0.005 * r.mean()
     # vertex colours matter where the material asks for them
     mats2 = mats.copy()
     mats2["use_vertex_colors"] = 0

@@ -62,9 +62,10 @@ def test_oracle_pt_direct_light_head_on_closed_form():
 @pytest.mark.gpu
 def test_gpu_pt_matches_oracle_per_pixel(torch_cuda):
     """Same scene, camera, seeds and RNG streams: the GPU's wavefront path
-    tracer and the oracle's per-pixel restatement agree pixel by pixel.  A
-    pixel may differ where last-bit differences of the transcendental
-    functions flip a Russian-roulette or lobe decision; those must be rare."""
+    tracer and the oracle's per-pixel restatement agree pixel by pixel.  Both
+    sides compute sin, cos and the Schlick power with the same restatement,
+    so no sampling decision can flip: every pixel agrees to float rounding
+    (sums of the same terms), with no allowance for outliers."""
     print("case pt parity", flush=True)
     torch = torch_cuda
     meshes, normals, mats, xf, which = pt_cases.small_c2()
@@ -86,7 +87,7 @@ def test_gpu_pt_matches_oracle_per_pixel(torch_cuda):
     a, r = img[..., :3].astype(np.float64), ref[..., :3].astype(np.float64)
     close = np.all(np.abs(a - r) <= 1e-4 + 1e-3 * np.abs(r), axis=-1)
     print("pt parity: fraction of matching pixels", close.mean(), "mean", a.mean(), r.mean())
-    assert close.mean() > 0.97
+    assert close.all()
     assert abs(a.mean() - r.mean()) <= 0.01 * r.mean()
     # the scene is actually lit: some sun, some env, some dark
     assert (r > 0).any() and r.max() > 0.2
@@ -107,7 +108,7 @@ def test_gpu_pt_progressive_and_reset(torch_cuda):
     # the 5 frames continue the same seed stream as one call of 5 frames
     ref = _oracle_render(S, cam, 48, 32, 3, 5, W.sun_light())
     close = np.all(np.abs(img[..., :3] - ref[..., :3]) <= 1e-4 + 1e-3 * np.abs(ref[..., :3]), axis=-1)
-    assert close.mean() > 0.97
+    assert close.all()
     pt.reset()
     img2 = pt.read().cpu().numpy()
     assert np.all(img2 == 0) and pt.sample_count() == 0
@@ -117,7 +118,7 @@ def test_gpu_pt_progressive_and_reset(torch_cuda):
     ref1 = _oracle_render(S, cam, 48, 32, 3, 1, W.sun_light(), first_sample=0, pcg_draws=5 * 4)
     img3 = pt.read().cpu().numpy()
     close = np.all(np.abs(img3[..., :3] - ref1[..., :3]) <= 1e-4 + 1e-3 * np.abs(ref1[..., :3]), axis=-1)
-    assert close.mean() > 0.97 and np.all(img3[..., 3] == 1)
+    assert close.all() and np.all(img3[..., 3] == 1)
 
 
 @pytest.mark.gpu
@@ -152,6 +153,6 @@ def test_gpu_pt_renders_the_world_after_an_update(torch_cuda):
     a, r = img[..., :3].astype(np.float64), ref[..., :3].astype(np.float64)
     close = np.all(np.abs(a - r) <= 1e-4 + 1e-3 * np.abs(r), axis=-1)
     print("pt after update: fraction of matching pixels", close.mean())
-    assert close.mean() > 0.97 and abs(a.mean() - r.mean()) <= 0.01 * r.mean()
+    assert close.all() and abs(a.mean() - r.mean()) <= 0.01 * r.mean()
     assert not np.allclose(before[..., :3], img[..., :3])  # the scene did change
     pt.close()

@@ -249,7 +249,7 @@ def test_gpu_alpha_traces_match_oracle(torch_cuda):
     assert np.array_equal(g_inst[hit], inst[hit])
 
 
-def _pt_compare(img, ref, frac=0.97):
+def _pt_compare(img, ref):
     a, r = img[..., :3].astype(np.float64), ref[..., :3].astype(np.float64)
     close = np.all(np.abs(a - r) <= 1e-4 + 1e-3 * np.abs(r), axis=-1)
     return close.mean(), a.mean(), r.mean()
@@ -301,7 +301,7 @@ def test_gpu_pt_opacity_matches_oracle_per_pixel(torch_cuda):
                               np.array(W.C2_ENV, np.float32), cam, W_, H_, bounces, frames, surface=surf)
     frac, am, rm = _pt_compare(img, ref)
     print("pt opacity parity: fraction of matching pixels", frac, "mean", am, rm)
-    assert frac > 0.97 and abs(am - rm) <= 0.01 * rm
+    assert frac == 1.0 and abs(am - rm) <= 0.01 * rm
     # the cut-outs and the map matter: the same scene all opaque differs
     ref_opaque = oracle_rt.pathtrace(S["tlas"], S["inst"], S["xform"], S["blases"], S["shading"], mats, sun,
                                      np.array(W.C2_ENV, np.float32), cam, W_, H_, bounces, frames)
@@ -349,7 +349,7 @@ def test_gpu_pt_triangle_lights_match_oracle(torch_cuda):
     img = pt.read().cpu().numpy()
     frac, am, rm = _pt_compare(img, ref)
     print("pt triangle lights parity: fraction of matching pixels", frac, "mean", am, rm)
-    assert frac > 0.97 and abs(am - rm) <= 0.01 * rm
+    assert frac == 1.0 and abs(am - rm) <= 0.01 * rm
     # switching the lamps off darkens the image
     pt.set_emissive_lights(None)
     assert len(pt.lights()) == 1

@@ -103,17 +103,16 @@ def test_gpu_pt_cache_frame_matches_oracle(torch_cuda):
     print(f"pt cache parity: {close.mean():.4f} of pixels match; queries gpu {n_queries} oracle {has_q.sum()}; "
           f"training pixels {train.sum()}, cache term mean {cache_term.mean():.4f}, image mean {a.mean():.4f}")
     assert np.all(img[..., 3] == 1)
-    assert close.mean() > 0.97
+    assert close.all()
     assert abs(int(n_queries) - int(has_q.sum())) <= 0.02 * has_q.sum()
     assert cache_term.mean() > 0.01 * expect.mean()  # the cache term is really exercised
 
     oq, ot = _targets_from_records(train, rec)
     print(f"pt cache samples: gpu {len(gq)} (stats {n_samples}), oracle {len(oq)}")
     assert len(gq) == n_samples and abs(len(gq) - len(oq)) <= 0.03 * len(oq) and len(oq) > 200
-    # Sampled directions go through sin/cos/sqrt, whose last bits differ
-    # between the GPU and the host's libm, so vertices after the first match
-    # to rounding, not bitwise: pair each GPU sample with the nearest oracle
-    # sample in (position, direction, normal) and compare.
+    # Training samples are written in a different order on the two sides:
+    # pair each GPU sample with the nearest oracle sample in (position,
+    # direction, normal); every one must have a partner and match its target.
     from scipy.spatial import cKDTree
     key = lambda q: np.concatenate([q[:, 0:3], 10.0 * q[:, 4:7], 10.0 * q[:, 8:11]], axis=1)
     dist, j = cKDTree(key(oq)).query(key(gq))
@@ -122,7 +121,7 @@ def test_gpu_pt_cache_frame_matches_oracle(torch_cuda):
     tmatch = np.all(np.abs(gt[:, :3] - ot[j]) <= 1e-5 + 1e-3 * np.abs(ot[j]), axis=1) & found
     print(f"pt cache samples: {found.mean():.4f} of GPU samples have an oracle vertex within 1e-3 "
           f"({exact.mean():.4f} bit-exact), {tmatch.sum() / max(found.sum(), 1):.4f} of those match the target")
-    assert found.mean() >= 0.97 and tmatch.sum() >= 0.95 * found.sum()
+    assert found.all() and tmatch.sum() == found.sum()
     assert np.all(gt[:, 3] == 0) and np.all(np.isfinite(gt))
 
 

@@ -163,3 +163,108 @@ def test_resident_equals_wavefront_realtime(torch_cuda):
     for k in range(6):
         assert a[k].tobytes() == b[k].tobytes(), k
     assert a[_abi.RT_FRAME][..., :3].max() > 0
+
+
+def test_work_counts_equal_the_oracles(torch_cuda):
+    """The counting instance of the resident kernel: its closest-hit and
+    shadow rays, inner nodes, triangle tests and instance entries over three
+    frames equal the oracle's counts of the same paths (its traversal
+    restatement counts each fetch as it makes it), and the image equals the
+    one the non-counting kernel renders."""
+    import oracle_rt
+    print("case resident work counts", flush=True)
+    meshes, normals, mats, xf, which = pt_cases.small_c2(n_inst=48, salt=3)
+    S = pt_cases.oracle_scene(meshes, normals, mats, xf, which)
+    w_, blases = pt_cases.product_scene(meshes, normals, xf, which, S["boxes"])
+    W_, H_, bounces, frames = 64, 48, 4, 3
+    cam = atlas_rt.camera((0.0, 15.0, -70.0), 0.0, 12.0, 50.0, W_ / H_)
+    sun = W.sun_light()
+    imgs = []
+    for counting in (True, False):
+        pt = atlas_rt.PathTracer(w_, W_, H_, bounces, mats, sun, env=ENV)
+        if counting:
+            pt.count_work()
+            assert set(pt.work_counts().values()) == {0}
+        pt.render(cam, frames)
+        imgs.append(pt.read().cpu().numpy())
+        if counting:
+            g = pt.work_counts()
+            pt.count_work(False)
+        pt.close()
+    assert imgs[0].tobytes() == imgs[1].tobytes()
+    oracle_rt.pathtrace(S["tlas"], S["inst"], S["xform"], S["blases"], S["shading"], mats, sun,
+                        np.array(ENV, np.float32), cam, W_, H_, bounces, frames)
+    o = oracle_rt.pathtrace_counts()
+    print("work counts gpu", g, "oracle", o)
+    assert g["closest_rays"] >= frames * W_ * H_ and g["hits"] <= g["closest_rays"]
+    assert g["shadow_rays"] <= g["hits"]
+    assert g["closest_rays"] + g["shadow_rays"] == o["rays"]
+    assert (g["nodes"], g["triangles"], g["instances"]) == (o["node_visits"], o["tri_tests"], o["instance_visits"])
+
+
+def test_work_counts_refuse_uncounted_launches(torch_cuda):
+    """Bounce-by-bounce frames do not count: reading the counters after one
+    is an error, not a partial sum."""
+    w_, blases, mats = _plain(salt=5, n_inst=16)
+    W_, H_ = 32, 16
+    cam = atlas_rt.camera((0.0, 15.0, -70.0), 0.0, 12.0, 50.0, W_ / H_)
+    pt = atlas_rt.PathTracer(w_, W_, H_, 2, mats, W.sun_light(), env=ENV, wavefront=True)
+    pt.count_work()
+    pt.render(cam, 1)
+    with pytest.raises(atlas_rt.AtlasRTError):
+        pt.work_counts()
+    pt.close()
+
+
+def test_graph_replays_equal_eager_frames(torch_cuda):
+    """A captured batch of 3 progressive frames, replayed between eager
+    render calls, leaves after every step the image (and sample count) that
+    the same sequence of eager calls leaves; each replay is read back with
+    eager work and synchronisations in between.  A replay after the world's
+    stack limit changed is refused."""
+    print("case graph replay", flush=True)
+    torch = torch_cuda
+    w_, blases, mats = _plain(salt=8)
+    W_, H_ = 80, 48
+    cam = atlas_rt.camera((0.0, 15.0, -70.0), 0.0, 12.0, 50.0, W_ / H_)
+    g = atlas_rt.PathTracer(w_, W_, H_, 4, mats, W.sun_light(), env=ENV)
+    e = atlas_rt.PathTracer(w_, W_, H_, 4, mats, W.sun_light(), env=ENV)
+    g.render(cam, 1)
+    e.render(cam, 1)
+    g.capture(cam, 3)
+    assert g.sample_count() == 1
+    assert g.read().cpu().numpy().tobytes() == e.read().cpu().numpy().tobytes()
+    side = torch.cuda.Stream()
+    for step, (kind, n) in enumerate([("replay", 3), ("render", 2), ("replay", 3), ("replay", 3), ("render", 5),
+                                      ("replay", 3)]):
+        if kind == "replay":
+            g.replay(stream=side if step == 3 else None)
+        else:
+            g.render(cam, n)
+        e.render(cam, n)
+        torch.cuda.synchronize()
+        a, b = g.read().cpu().numpy(), e.read().cpu().numpy()
+        assert g.sample_count() == e.sample_count(), step
+        assert a.tobytes() == b.tobytes(), step
+    assert np.all(a[..., 3] == 20) and a[..., :3].max() > 0
+    w_.set_stack_limit(8)
+    with pytest.raises(atlas_rt.AtlasRTError):
+        g.replay()
+    w_.set_stack_limit(0)
+    g.capture(cam, 2)
+    g.replay()
+    e.render(cam, 2)
+    assert g.read().cpu().numpy().tobytes() == e.read().cpu().numpy().tobytes()
+    g.close()
+    e.close()
+
+
+def test_capture_refuses_bounce_by_bounce_frames(torch_cuda):
+    w_, blases, mats = _plain(salt=5, n_inst=16)
+    cam = atlas_rt.camera((0.0, 15.0, -70.0), 0.0, 12.0, 50.0, 2.0)
+    pt = atlas_rt.PathTracer(w_, 32, 16, 2, mats, W.sun_light(), env=ENV, wavefront=True)
+    with pytest.raises(atlas_rt.AtlasRTError):
+        pt.capture(cam, 1)
+    with pytest.raises(atlas_rt.AtlasRTError):
+        pt.replay()
+    pt.close()
