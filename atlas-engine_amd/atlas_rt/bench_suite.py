@@ -301,7 +301,8 @@ def c3(torch, log):
         blases[b].trace_closest_stats(d_rays[o:o + c], d_sh[o:o + c], d_vis[o:o + c])
     torch.cuda.synchronize()
     vis = d_vis.cpu().numpy().astype(np.int64)
-    same = bool(torch.equal(d_sh, d_hits))
+    # compare bits: a miss's prim_id -1 reads as a float NaN
+    same = bool(torch.equal(d_sh.view(torch.int32), d_hits.view(torch.int32)))
     del d_vis, d_sh
     nodes, tests = int(vis[:, 0].sum()), int(vis[:, 1].sum())
     alg = 48 * n + 64 * nodes + 48 * tests

@@ -60,10 +60,11 @@ atlas_rt_status alloc_slot(PathTracer& P, uint32_t k, uint32_t rays) {
 }
 
 void free_slot_cache(FrameSlot& S) {
-    void* ptrs[] = {S.rec, S.q_buf, S.q_meta, S.q_out, S.s_q, S.s_t};
+    void* ptrs[] = {S.rec, S.q_buf, S.q_meta, S.q_out, S.s_q, S.s_t, S.train_list};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     S.rec = S.q_buf = S.q_meta = S.q_out = S.s_q = S.s_t = nullptr;
+    S.train_list = nullptr;
 }
 
 void free_slot(FrameSlot& S) {
@@ -496,7 +497,7 @@ atlas_rt_status atlas_rt_pathtracer_render(atlas_rt_pathtracer* pt, const atlas_
     ATLAS_HIP_TRY(hipEventRecord(P.ev_start, s));
     uint32_t used = 0;
     // resident frames go in batches of up to P.batch per launch
-    const uint32_t per = resident_frame(P) && !P.realtime ? P.batch : 1u;
+    const uint32_t per = frame_batch(P);
     for (uint32_t f = 0; f < frames;) {
         const uint32_t k = P.next_slot % n;
         if (!(used & (1u << k))) {
@@ -529,7 +530,7 @@ atlas_rt_status atlas_rt_pathtracer_render(atlas_rt_pathtracer* pt, const atlas_
 atlas_rt_status atlas_rt_pathtracer_capture(atlas_rt_pathtracer* pt, const atlas_rt_camera* camera, uint32_t frames) {
     if (!pt || !camera) return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_capture: NULL argument");
     PathTracer& P = pt->P;
-    if (P.realtime || !resident_frame(P))
+    if (P.realtime || !resident_frame(P) || P.shade_args.cache.enabled)
         return fail(ATLAS_RT_E_UNSUPPORTED, "atlas_rt_pathtracer_capture: progressive frames of the resident frame "
                                             "kernel only (no realtime mode, radiance cache, binning or "
                                             "ATLAS_RT_PT_WAVEFRONT)");
@@ -577,7 +578,8 @@ atlas_rt_status atlas_rt_pathtracer_replay(atlas_rt_pathtracer* pt, void* stream
     if (st == ATLAS_RT_OK) st = apply_stack_cap(P);
     if (st != ATLAS_RT_OK) return st;
     const GraphKey now = graph_key(P);
-    if (P.realtime || !resident_frame(P) || std::memcmp(&now, &P.graph_key, sizeof(now)) != 0)
+    if (P.realtime || !resident_frame(P) || P.shade_args.cache.enabled ||
+        std::memcmp(&now, &P.graph_key, sizeof(now)) != 0)
         return fail(ATLAS_RT_E_INVALID_ARG, "atlas_rt_pathtracer_replay: the path tracer changed since the capture "
                                             "(world update, stack limit, materials, lights, textures, mode or work "
                                             "counting); capture again");
@@ -876,6 +878,7 @@ atlas_rt_status atlas_rt_pathtracer_set_cache(atlas_rt_pathtracer* pt, atlas_rt_
         if (st == ATLAS_RT_OK) st = dev_alloc(S.q_out, pixels);
         if (st == ATLAS_RT_OK) st = dev_alloc(S.s_q, 3 * recs);
         if (st == ATLAS_RT_OK) st = dev_alloc(S.s_t, recs);
+        if (st == ATLAS_RT_OK) st = dev_alloc(S.train_list, cap);
     }
     for (int j = 0; j < 2 && params->query_lag && st == ATLAS_RT_OK; j++) {
         st = dev_alloc(P.snap_table[j], cache->table_params);

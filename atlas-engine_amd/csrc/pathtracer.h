@@ -50,6 +50,7 @@ struct FrameSlot {
     Scratch bin_ws;
     // radiance cache
     float4* rec = nullptr;        // training records: 5 float4 each (query x3, beta, added radiance)
+    uint32_t* train_list = nullptr;  // resident frames: training pixels
     float4* q_buf = nullptr;      // render-path queries, 3 float4 each
     float4* q_meta = nullptr;
     float4* q_out = nullptr;
@@ -190,9 +191,12 @@ struct FrameOrder {
 // Progressive frames in slot `k` on that slot's stream: `frames` of them in
 // one resident launch when resident_frame(P) and frames <= P.batch, else one.
 atlas_rt_status launch_frame(PathTracer& P, const Camera& cam, uint32_t k, const FrameOrder& order, uint32_t frames);
-// Whether frames run as the resident frame kernel (plain path tracing: the
-// radiance cache and ray binning need the bounce-by-bounce queues).
+// Whether frames run as the resident frame kernel (ray binning sorts the
+// bounce-by-bounce queues; ATLAS_RT_PT_WAVEFRONT asks for them).
 bool resident_frame(const PathTracer& P);
+// Progressive frames per launch: P.batch for plain resident frames, else 1
+// (realtime mode, the radiance cache, bounce-by-bounce launches).
+uint32_t frame_batch(const PathTracer& P);
 // The kernels of `frames` progressive frames in slot 0 on `s`, with no event
 // or wait (for graph capture): seeds, the resident frame kernel, the
 // accumulation.

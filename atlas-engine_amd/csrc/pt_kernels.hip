@@ -117,6 +117,32 @@ __device__ __forceinline__ uint32_t mix32(uint32_t h) {
     return h;
 }
 
+// Resident frames with the cache: the frame's training pixels, picked as
+// k_raygen picks them, listed by training index so that the resident kernel
+// starts them first (they trace every bounce; started late, they would
+// hold the frame long after the short render paths are done).  Every
+// pixel's radiance starts at zero, with w = 1 marking a listed pixel.
+__global__ __launch_bounds__(kShadeBlock) void k_train_list(uint32_t pixels, FrameCtl* __restrict__ ctl,
+                                                            float4* __restrict__ Lpix, CacheArgs C, int32_t bounces) {
+    const uint32_t g0 = blockIdx.x * kShadeBlock + threadIdx.x;
+    const bool valid = g0 < pixels;
+    uint32_t* const ctr[1] = {&ctl->train_paths};
+    const bool pick[1] = {valid && mix32(g0 * 0x9e3779b1u + ctl->frame * 0x632be5abu) % C.train_stride == 0};
+    uint32_t t[1];
+    block_alloc<1>(ctr, pick, t);
+    if (!valid) return;
+    const bool listed = pick[0] && t[0] < C.train_cap;
+    if (listed) {
+        C.train_list[t[0]] = g0;
+        for (int32_t k = 0; k <= bounces; k++) {
+            const size_t r = size_t(t[0]) * (bounces + 1) + k;
+            C.rec_beta[r] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            C.rec_add[r] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+    }
+    Lpix[g0] = make_float4(0.0f, 0.0f, 0.0f, listed ? 1.0f : 0.0f);
+}
+
 // rayGen.csh:25-90 with one sample per frame (z = 0): jittered pinhole ray
 // through the near-plane corner frame; the write index groups 8x8 tiles.
 __global__ __launch_bounds__(kShadeBlock) void k_raygen(Camera cam, uint32_t width, uint32_t height,
@@ -748,8 +774,11 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 // With STATS the kernel also counts its traversal work into stats[0..5]:
 // closest-hit rays, shadow rays, closest hits, inner nodes fetched (both
 // levels), triangles tested and instances entered; the walks themselves are
-// unchanged.
-template <bool ALPHA, bool STATS>
+// unchanged.  With CACHE (radiance cache, one frame per launch) the
+// training paths listed by k_train_list come first, recording every vertex,
+// then every other pixel in tile order, ending at its second vertex with a
+// cache query as k_shade writes them.
+template <bool ALPHA, bool STATS, bool CACHE>
 __global__ __launch_bounds__(kFrameBlock, 4) void k_frame_persistent(ShadeArgs A, FrameGen G, int32_t* __restrict__ spill,
                                                                   uint32_t cap, uint32_t* __restrict__ flags,
                                                                   unsigned long long* __restrict__ stats) {
@@ -760,7 +789,9 @@ __global__ __launch_bounds__(kFrameBlock, 4) void k_frame_persistent(ShadeArgs A
     FrameCtl* const ctl0 = A.ctl;  // frame f's seeds and counts in ctl0[f]; ctl0[0].pull counts paths
     const WorldDev W = A.world;
     const AlphaArgs& AA = A.alpha;
-    const uint32_t total = G.rays * G.frames;
+    // with the cache, the listed training paths are pulled before the pixels
+    const uint32_t listed = CACHE ? min(ctl0->train_paths, A.cache.train_cap) : 0u;
+    const uint32_t total = G.rays * G.frames + listed;
     const V o = mk(G.cam.origin[0], G.cam.origin[1], G.cam.origin[2]);
     const V rgt = mk(G.cam.right[0], G.cam.right[1], G.cam.right[2]);
     const V btm = mk(G.cam.bottom[0], G.cam.bottom[1], G.cam.bottom[2]);
@@ -776,8 +807,9 @@ __global__ __launch_bounds__(kFrameBlock, 4) void k_frame_persistent(ShadeArgs A
     float4 shc = p0;     // the shadow ray's light before visibility
     bool pending = false;  // the path continues after its shadow ray (p0..p2 hold its next ray)
     bool exhausted = false;
-    const FrameCtl* ctl = ctl0;  // the frame of the lane's path
+    FrameCtl* ctl = ctl0;  // the frame of the lane's path
     float4* Lpix = A.Lpix;       // that frame's radiance per ray ID
+    const CacheArgs& C = A.cache;
     uint32_t c_closest = 0, c_shadow = 0, c_hits = 0, c_nodes = 0, c_tris = 0, c_insts = 0;
 
     for (;;) {
@@ -791,14 +823,26 @@ __global__ __launch_bounds__(kFrameBlock, 4) void k_frame_persistent(ShadeArgs A
                 if (phase == 0) {
                     const uint32_t kk = base + uint32_t(__popcll(idle & ((1ull << lane) - 1ull)));
                     if (kk < total) {
-                        const uint32_t f = kk / G.rays, k = kk - f * G.rays;
+                        // with the cache (one frame): listed training paths, then every pixel
+                        const uint32_t f = CACHE ? 0u : kk / G.rays;
+                        const uint32_t k = CACHE ? (kk < listed ? 0u : kk - listed) : kk - f * G.rays;
                         ctl = ctl0 + f;
                         Lpix = A.Lpix + size_t(f) * G.rays;
                         // rayGen.csh:25-90, as k_raygen / k_raygen_rt write slot k
                         uint32_t id;
+                        uint32_t train = 0;
+                        bool skip = false;
                         V dir;
                         if (G.spf == 0) {
-                            const uint32_t pixel = tile_pixel(k, G.width, G.height);
+                            uint32_t pixel;
+                            if (CACHE && kk < listed) {
+                                pixel = C.train_list[kk];
+                                train = kk + 1;
+                            } else {
+                                pixel = tile_pixel(k, G.width, G.height);
+                                // a listed pixel was traced first, as a training path
+                                skip = CACHE && Lpix[pixel].w != 0.0f;
+                            }
                             const uint32_t px = pixel % G.width, py = pixel / G.width;
                             // the frame's jitter, the same for every pixel (rayGen.csh:33-34)
                             const float sc = float(ctl->sample_count);
@@ -814,16 +858,20 @@ __global__ __launch_bounds__(kFrameBlock, 4) void k_frame_persistent(ShadeArgs A
                             dir = normalize(((o + rgt * cx) + btm * cy) - loc);
                             id = pixel * G.spf + k % G.spf;
                         }
-                        Lpix[id] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                        p0 = make_float4(loc.x, loc.y, loc.z, __uint_as_float(id));
-                        p1 = make_float4(dir.x, dir.y, dir.z, __uint_as_float(0u));
-                        p2 = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-                        bounce = 0;
-                        phase = 1;
-                        world_begin(L, make_float4(p0.x, p0.y, p0.z, 0.0f), make_float4(p1.x, p1.y, p1.z, kInf));
-                        ready = any_nan3(p1.x, p1.y, p1.z);
-                        rstat = kDone;
-                        if (STATS) c_closest++;
+                        if (!skip) {
+                            // k_train_list zeroed the cache mode's radiance (and its marks)
+                            if (!CACHE) Lpix[id] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                            p0 = make_float4(loc.x, loc.y, loc.z, __uint_as_float(id));
+                            p1 = make_float4(dir.x, dir.y, dir.z, __uint_as_float(train));
+                            p2 = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+                            bounce = 0;
+                            phase = 1;
+                            world_begin(L, make_float4(p0.x, p0.y, p0.z, 0.0f),
+                                        make_float4(p1.x, p1.y, p1.z, kInf));
+                            ready = any_nan3(p1.x, p1.y, p1.z);
+                            rstat = kDone;
+                            if (STATS) c_closest++;
+                        }
                     }
                 }
                 if (base + n >= total) exhausted = true;
@@ -883,6 +931,23 @@ __global__ __launch_bounds__(kFrameBlock, 4) void k_frame_persistent(ShadeArgs A
             if (STATS) c_hits += slot >= 0;
             ShadeOut so;
             shade_path(A, ctl, Lpix, bounce, p0, p1, p2, make_int4(__float_as_int(L.hitT), slot, inst, 0), so);
+            if (CACHE) {
+                // the path ends with a cache query: one counter update per wave
+                const uint64_t qm = __ballot(so.query);
+                if (qm) {
+                    const int leader = __ffsll((unsigned long long)qm) - 1;
+                    uint32_t qb = 0;
+                    if (int(lane) == leader) qb = atomicAdd(&ctl->queries, uint32_t(__popcll(qm)));
+                    qb = __shfl(qb, leader, 64);
+                    if (so.query) {
+                        const uint32_t qk = qb + uint32_t(__popcll(qm & ((1ull << lane) - 1ull)));
+                        C.q_buf[3 * size_t(qk)] = so.q[0];
+                        C.q_buf[3 * size_t(qk) + 1] = so.q[1];
+                        C.q_buf[3 * size_t(qk) + 2] = so.q[2];
+                        C.q_meta[qk] = so.q[3];
+                    }
+                }
+            }
             pending = so.cont;
             if (so.cont) {
                 p0 = so.pa[0];
@@ -916,6 +981,15 @@ __global__ __launch_bounds__(kFrameBlock, 4) void k_frame_persistent(ShadeArgs A
                     acc.y += cc.y;
                     acc.z += cc.z;
                     Lpix[id] = acc;
+                    const uint32_t rec = CACHE ? __float_as_uint(shc.w) : 0u;
+                    if (rec) {
+                        // a training path's vertex takes the light too
+                        float4 a = C.rec_add[rec - 1];
+                        a.x += cc.x;
+                        a.y += cc.y;
+                        a.z += cc.z;
+                        C.rec_add[rec - 1] = a;
+                    }
                 }
             }
         }
@@ -1018,6 +1092,7 @@ ShadeArgs slot_args(const PathTracer& P, uint32_t k) {
         A.cache.rec_add = S.rec + 4 * recs;
         A.cache.q_buf = S.q_buf;
         A.cache.q_meta = S.q_meta;
+        A.cache.train_list = S.train_list;
     }
     return A;
 }
@@ -1027,25 +1102,36 @@ atlas_rt_status launch_resident(PathTracer& P, uint32_t k, const ShadeArgs& A, c
     const uint32_t groups = uint32_t(device_cu_count()) * kResidentGroupsPerCu;
     int32_t* spill = reinterpret_cast<int32_t*>(S.spill.ptr);
     uint32_t* flags = reinterpret_cast<uint32_t*>(S.flags.ptr);
-    if (P.alpha) {
-        // the transparency walks test a whole leaf per step: not counted
-        if (P.stats) P.stats_partial = true;
-        hipLaunchKernelGGL((k_frame_persistent<true, false>), dim3(groups), dim3(kFrameBlock), 0, s, A, G, spill, P.cap,
-                           flags, nullptr);
-    } else if (P.stats) {
-        hipLaunchKernelGGL((k_frame_persistent<false, true>), dim3(groups), dim3(kFrameBlock), 0, s, A, G, spill, P.cap,
-                           flags, P.stats);
-    } else {
-        hipLaunchKernelGGL((k_frame_persistent<false, false>), dim3(groups), dim3(kFrameBlock), 0, s, A, G, spill,
+    const bool cache = A.cache.enabled;
+    // the transparency walks test a whole leaf per step, and the cache mode
+    // is not counted
+    if (P.stats && (P.alpha || cache)) P.stats_partial = true;
+    if (cache && P.alpha)
+        hipLaunchKernelGGL((k_frame_persistent<true, false, true>), dim3(groups), dim3(kFrameBlock), 0, s, A, G, spill,
                            P.cap, flags, nullptr);
-    }
+    else if (cache)
+        hipLaunchKernelGGL((k_frame_persistent<false, false, true>), dim3(groups), dim3(kFrameBlock), 0, s, A, G, spill,
+                           P.cap, flags, nullptr);
+    else if (P.alpha)
+        hipLaunchKernelGGL((k_frame_persistent<true, false, false>), dim3(groups), dim3(kFrameBlock), 0, s, A, G,
+                           spill, P.cap, flags, nullptr);
+    else if (P.stats)
+        hipLaunchKernelGGL((k_frame_persistent<false, true, false>), dim3(groups), dim3(kFrameBlock), 0, s, A, G,
+                           spill, P.cap, flags, P.stats);
+    else
+        hipLaunchKernelGGL((k_frame_persistent<false, false, false>), dim3(groups), dim3(kFrameBlock), 0, s, A, G,
+                           spill, P.cap, flags, nullptr);
     ATLAS_HIP_TRY(hipGetLastError());
     return ATLAS_RT_OK;
 }
 
 }  // namespace
 
-bool resident_frame(const PathTracer& P) { return !P.shade_args.cache.enabled && !P.binning && !P.wavefront; }
+bool resident_frame(const PathTracer& P) { return !P.binning && !P.wavefront; }
+
+uint32_t frame_batch(const PathTracer& P) {
+    return resident_frame(P) && !P.realtime && !P.shade_args.cache.enabled ? P.batch : 1u;
+}
 
 GraphKey graph_key(const PathTracer& P) {
     GraphKey key;
@@ -1206,23 +1292,10 @@ atlas_rt_status launch_frame(PathTracer& P, const Camera& cam, uint32_t k, const
     const FrameSlot* before = order.prev >= 0 ? &P.slot[order.prev] : nullptr;
     const FrameSlot* before2 = order.prev2 >= 0 ? &P.slot[order.prev2] : nullptr;
     const bool resident = resident_frame(P);
-    if (!resident || frames > P.batch) frames = 1;
+    if (frames > frame_batch(P)) frames = 1;
     if (before) ATLAS_HIP_TRY(hipStreamWaitEvent(s, before->ev_begin, 0));
     hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(64), 0, s, P.ctl, S.ctl, P.bounces, pixels, 1u, frames);
     ATLAS_HIP_TRY(hipEventRecord(S.ev_begin, s));
-    if (resident) {
-        const FrameGen G{cam, P.width, P.height, pixels, frames, 0u, 0.0f, 0.0f};
-        const atlas_rt_status rs = launch_resident(P, k, A, G, s);
-        if (rs != ATLAS_RT_OK) return rs;
-        ATLAS_HIP_TRY(hipEventRecord(S.ev_train, s));
-        if (before) ATLAS_HIP_TRY(hipStreamWaitEvent(s, before->ev_acc, 0));
-        hipLaunchKernelGGL(k_accumulate, dim3(grid_all), dim3(kBlock), 0, s, S.Lpix, P.accum, pixels, frames);
-        ATLAS_HIP_TRY(hipEventRecord(S.ev_acc, s));
-        ATLAS_HIP_TRY(hipGetLastError());
-        return ATLAS_RT_OK;
-    }
-    hipLaunchKernelGGL(k_raygen, dim3((pixels + kShadeBlock - 1) / kShadeBlock), dim3(kShadeBlock), 0, s, cam, P.width,
-                       P.height, S.ctl, S.paths[0], S.Lpix, C, P.bounces);
     const bool lag = C.enabled && P.query_lag;
     nrc::NrcDev snap{};
     if (lag) {
@@ -1231,8 +1304,29 @@ atlas_rt_status launch_frame(PathTracer& P, const Camera& cam, uint32_t k, const
         snap.wn = P.snap_wn[order.index & 1u];
     }
     const FrameSlot* ready = lag ? before2 : before;
-    const atlas_rt_status bs = launch_bounces(P, k, s, ready ? ready->ev_train : nullptr, lag ? &snap : nullptr);
-    if (bs != ATLAS_RT_OK) return bs;
+    if (resident) {
+        // With the cache, the resident grid (every CU) starts after the frame
+        // before has trained: launched beside that frame's network kernels it
+        // would hold the CUs they wait for.
+        if (C.enabled && before) ATLAS_HIP_TRY(hipStreamWaitEvent(s, before->ev_train, 0));
+        if (C.enabled)
+            hipLaunchKernelGGL(k_train_list, dim3((pixels + kShadeBlock - 1) / kShadeBlock), dim3(kShadeBlock), 0, s,
+                               pixels, S.ctl, S.Lpix, C, P.bounces);
+        const FrameGen G{cam, P.width, P.height, pixels, frames, 0u, 0.0f, 0.0f};
+        const atlas_rt_status rs = launch_resident(P, k, A, G, s);
+        if (rs != ATLAS_RT_OK) return rs;
+        if (C.enabled) {
+            // the render paths' cache queries, once the network they read is trained
+            if (ready) ATLAS_HIP_TRY(hipStreamWaitEvent(s, ready->ev_train, 0));
+            ATLAS_HIP_TRY(nrc::launch_infer(lag ? snap : P.nrc->dev, S.q_buf, S.q_out, &S.ctl->queries, P.rays, s));
+            hipLaunchKernelGGL(k_cache_apply, dim3(grid_all), dim3(kBlock), 0, s, S.ctl, S.q_meta, S.q_out, S.Lpix);
+        }
+    } else {
+        hipLaunchKernelGGL(k_raygen, dim3((pixels + kShadeBlock - 1) / kShadeBlock), dim3(kShadeBlock), 0, s, cam,
+                           P.width, P.height, S.ctl, S.paths[0], S.Lpix, C, P.bounces);
+        const atlas_rt_status bs = launch_bounces(P, k, s, ready ? ready->ev_train : nullptr, lag ? &snap : nullptr);
+        if (bs != ATLAS_RT_OK) return bs;
+    }
     if (C.enabled) {
         hipLaunchKernelGGL(k_build_targets, dim3((P.train_cap + kBlock - 1) / kBlock), dim3(kBlock), 0, s, S.ctl, C,
                            P.bounces, S.s_q, S.s_t);
@@ -1255,7 +1349,7 @@ atlas_rt_status launch_frame(PathTracer& P, const Camera& cam, uint32_t k, const
     }
     ATLAS_HIP_TRY(hipEventRecord(S.ev_train, s));
     if (before) ATLAS_HIP_TRY(hipStreamWaitEvent(s, before->ev_acc, 0));
-    hipLaunchKernelGGL(k_accumulate, dim3(grid_all), dim3(kBlock), 0, s, S.Lpix, P.accum, pixels, 1u);
+    hipLaunchKernelGGL(k_accumulate, dim3(grid_all), dim3(kBlock), 0, s, S.Lpix, P.accum, pixels, frames);
     ATLAS_HIP_TRY(hipEventRecord(S.ev_acc, s));
     ATLAS_HIP_TRY(hipGetLastError());
     return ATLAS_RT_OK;

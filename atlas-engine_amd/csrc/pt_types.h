@@ -115,6 +115,7 @@ struct CacheArgs {
     float4* rec_add;
     float4* q_buf;           // 3 float4 per render-path query
     float4* q_meta;          // (throughput xyz, pixel bits)
+    uint32_t* train_list;    // resident frames: the frame's training pixels, by training index
     uint32_t train_cap;      // maximum training paths per frame
     uint32_t train_stride;   // about one pixel in train_stride trains
     uint32_t enabled;

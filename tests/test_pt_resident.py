@@ -268,3 +268,46 @@ def test_capture_refuses_bounce_by_bounce_frames(torch_cuda):
     with pytest.raises(atlas_rt.AtlasRTError):
         pt.replay()
     pt.close()
+
+
+def test_resident_cache_frames_equal_wavefront(torch_cuda):
+    """With the radiance cache the resident kernel ends render paths at their
+    second vertex with a query and records the training paths' vertices, as
+    the bounce-by-bounce kernels do.  One frame: the image (direct part plus
+    the cache term) is the same bit for bit, and so are the numbers of
+    queries and training samples and the multiset of training samples (their
+    order in the buffer follows the lanes that wrote them).  Three more
+    frames with training in between: every frame trains on samples in
+    another order, so the images agree to float rounding."""
+    print("case resident cache", flush=True)
+    torch = torch_cuda
+    meshes, normals, mats, xf, which = pt_cases.small_c2(salt=0, extent=8.0, height=8.0)
+    S = pt_cases.oracle_scene(meshes, normals, mats, xf, which)
+    w_, blases = pt_cases.product_scene(meshes, normals, xf, which, S["boxes"])
+    W_, H_, bounces = 96, 64, 6
+    cam = atlas_rt.camera((0.0, 0.0, -8.8), 0.0, 0.0, 55.0, W_ / H_)
+    out = []
+    for wavefront in (False, True):
+        rc = atlas_rt.RadianceCache((-12.0,) * 3, (12.0,) * 3, levels=8, log2_table=14, base_resolution=4,
+                                    max_resolution=128, seed=5)
+        pt = atlas_rt.PathTracer(w_, W_, H_, bounces, mats, W.sun_light(), env=ENV, wavefront=wavefront)
+        pt.set_cache(rc, train_stride=4, train_steps=1)
+        pt.render(cam, 1)
+        img1 = pt.read().cpu().numpy()
+        stats = pt.cache_stats()
+        q, t = pt.cache_samples(W_ * H_ * (bounces + 1))
+        samples = np.concatenate([q.cpu().numpy(), t.cpu().numpy()], axis=1)
+        samples = samples[np.lexsort(samples.T[::-1])]
+        pt.render(cam, 3)
+        img4 = pt.read().cpu().numpy()
+        out.append((img1, stats, samples, img4))
+        pt.close()
+        rc.close()
+    (a1, sa, qa, a4), (b1, sb, qb, b4) = out
+    print("cache stats resident", sa, "bounce by bounce", sb)
+    assert sa[0] == sb[0] and sa[1] == sb[1] and sa[0] > 100 and sa[1] > 100
+    assert a1.tobytes() == b1.tobytes()
+    assert qa.tobytes() == qb.tobytes()
+    assert np.all(a4[..., 3] == 4) and np.all(b4[..., 3] == 4)
+    x, y = a4[..., :3].astype(np.float64), b4[..., :3].astype(np.float64)
+    assert np.all(np.abs(x - y) <= 1e-3 + 1e-3 * np.abs(y)), np.abs(x - y).max()
