@@ -192,7 +192,8 @@ struct FrameOrder {
 // one resident launch when resident_frame(P) and frames <= P.batch, else one.
 atlas_rt_status launch_frame(PathTracer& P, const Camera& cam, uint32_t k, const FrameOrder& order, uint32_t frames);
 // Whether frames run as the resident frame kernel (ray binning sorts the
-// bounce-by-bounce queues; ATLAS_RT_PT_WAVEFRONT asks for them).
+// bounce-by-bounce queues; ATLAS_RT_PT_WAVEFRONT asks for them; with the
+// radiance cache only one frame at a time).
 bool resident_frame(const PathTracer& P);
 // Progressive frames per launch: P.batch for plain resident frames, else 1
 // (realtime mode, the radiance cache, bounce-by-bounce launches).

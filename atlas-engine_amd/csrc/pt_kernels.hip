@@ -1127,7 +1127,11 @@ atlas_rt_status launch_resident(PathTracer& P, uint32_t k, const ShadeArgs& A, c
 
 }  // namespace
 
-bool resident_frame(const PathTracer& P) { return !P.binning && !P.wavefront; }
+bool resident_frame(const PathTracer& P) {
+    // With the cache, frames run resident one at a time; with several in
+    // flight the bounce-by-bounce launches overlap their training-path tails
+    return !P.binning && !P.wavefront && (!P.shade_args.cache.enabled || P.slots == 1);
+}
 
 uint32_t frame_batch(const PathTracer& P) {
     return resident_frame(P) && !P.realtime && !P.shade_args.cache.enabled ? P.batch : 1u;

@@ -271,9 +271,9 @@ def test_capture_refuses_bounce_by_bounce_frames(torch_cuda):
 
 
 def test_resident_cache_frames_equal_wavefront(torch_cuda):
-    """With the radiance cache the resident kernel ends render paths at their
-    second vertex with a query and records the training paths' vertices, as
-    the bounce-by-bounce kernels do.  One frame: the image (direct part plus
+    """With the radiance cache and one frame at a time the resident kernel
+    ends render paths at their second vertex with a query and records the
+    training paths' vertices, as the bounce-by-bounce kernels do.  One frame: the image (direct part plus
     the cache term) is the same bit for bit, and so are the numbers of
     queries and training samples and the multiset of training samples (their
     order in the buffer follows the lanes that wrote them).  Three more
@@ -290,7 +290,8 @@ def test_resident_cache_frames_equal_wavefront(torch_cuda):
     for wavefront in (False, True):
         rc = atlas_rt.RadianceCache((-12.0,) * 3, (12.0,) * 3, levels=8, log2_table=14, base_resolution=4,
                                     max_resolution=128, seed=5)
-        pt = atlas_rt.PathTracer(w_, W_, H_, bounces, mats, W.sun_light(), env=ENV, wavefront=wavefront)
+        pt = atlas_rt.PathTracer(w_, W_, H_, bounces, mats, W.sun_light(), env=ENV, wavefront=wavefront,
+                                 frames_in_flight=1)
         pt.set_cache(rc, train_stride=4, train_steps=1)
         pt.render(cam, 1)
         img1 = pt.read().cpu().numpy()
