@@ -38,8 +38,9 @@ static atlas_rt_status check_kernel(int32_t kernel, const char* who) {
     const int32_t extra = kernel & ~int32_t(ATLAS_RT_KERNEL_VARIANT_MASK | ATLAS_RT_KERNEL_STACK_REGISTER |
                                             ATLAS_RT_KERNEL_SLAB_FLOAT | ATLAS_RT_KERNEL_LEAF_BATCH);
     const bool fslab_ok = !(kernel & ATLAS_RT_KERNEL_SLAB_FLOAT) || base == ATLAS_RT_KERNEL_BVH2_EXACT ||
-                          base == ATLAS_RT_KERNEL_BVH2_PERSISTENT;
-    const bool batch_ok = !(kernel & ATLAS_RT_KERNEL_LEAF_BATCH) || base == ATLAS_RT_KERNEL_BVH2_PERSISTENT;
+                          base == ATLAS_RT_KERNEL_BVH2_PERSISTENT || base == ATLAS_RT_KERNEL_BVH4;
+    const bool batch_ok = !(kernel & ATLAS_RT_KERNEL_LEAF_BATCH) || base == ATLAS_RT_KERNEL_BVH2_PERSISTENT ||
+                          base == ATLAS_RT_KERNEL_BVH4;
     if (extra == 0 && fslab_ok && batch_ok && base >= ATLAS_RT_KERNEL_AUTO && base <= ATLAS_RT_KERNEL_BVH8 &&
         !(base == ATLAS_RT_KERNEL_AUTO && kernel != ATLAS_RT_KERNEL_AUTO))
         return ATLAS_RT_OK;

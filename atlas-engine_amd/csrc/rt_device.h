@@ -121,6 +121,25 @@ __device__ __forceinline__ void node_test_float(const DevRay& R, const float4& a
     left_first = (hl ? lo_l : tmax) <= (hr ? lo_r : tmax);
 }
 
+// The same widened float test for one box (the wide-node SLAB_FLOAT
+// variants): hit and entry distance as node_test_float decides them for a
+// child; the exact test where the direction has a component outside the
+// float reciprocal's accurate range.
+__device__ __forceinline__ bool slab_dist_float(const DevRay& R, float lx, float ly, float lz, float hx, float hy,
+                                                float hz, float tmax, float& dist) {
+    if (!R.fast) return slab_dist(R, lx, ly, lz, hx, hy, hz, tmax, dist);
+    const float t0x = (lx - R.ox) * R.fx, t0y = (ly - R.oy) * R.fy, t0z = (lz - R.oz) * R.fz;
+    const float t1x = (hx - R.ox) * R.fx, t1y = (hy - R.oy) * R.fy, t1z = (hz - R.oz) * R.fz;
+    const float lo = fmaxf(fmaxf(R.tmin, fminf(t0x, t1x)), fmaxf(fminf(t0y, t1y), fminf(t0z, t1z)));
+    const float hi = fminf(fminf(tmax, fmaxf(t0x, t1x)), fminf(fmaxf(t0y, t1y), fmaxf(t0z, t1z)));
+    constexpr float kWiden = 9.5367431640625e-07f;   // 2^-20
+    constexpr float kFloor = 7.52316384526264e-37f;   // 2^-120
+    constexpr float kBig = 3.4028234663852886e+38f;   // FLT_MAX
+    const bool hit = lo - hi <= kWiden * (fminf(fabsf(lo), kBig) + fminf(fabsf(hi), kBig)) + kFloor;
+    dist = hit ? lo : tmax;
+    return hit;
+}
+
 // Triangle test on the precomputed layout (v0, e0 = v1 - v0, e1 = v2 - v0; the
 // subtractions are the shader's own first two operations, done once on the
 // host with the same rounding).  Writes (t,u,v); returns the engine's

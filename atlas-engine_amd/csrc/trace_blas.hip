@@ -267,6 +267,15 @@ atlas_rt_status launch_blas_closest(atlas_rt_blas& blas, const TraceLaunch& L0) 
         else if (node_w == 2)
             reg ? launch_persistent<2, true>(B, L, groups, spill, cap, ctl)
                 : launch_persistent<2, false>(B, L, groups, spill, cap, ctl);
+        else if (node_w == 4 && leaf_batch && fslab)
+            reg ? launch_persistent<4, true, true, kLeafDiv>(B, L, groups, spill, cap, ctl)
+                : launch_persistent<4, false, true, kLeafDiv>(B, L, groups, spill, cap, ctl);
+        else if (node_w == 4 && leaf_batch)
+            reg ? launch_persistent<4, true, false, kLeafDiv>(B, L, groups, spill, cap, ctl)
+                : launch_persistent<4, false, false, kLeafDiv>(B, L, groups, spill, cap, ctl);
+        else if (node_w == 4 && fslab)
+            reg ? launch_persistent<4, true, true>(B, L, groups, spill, cap, ctl)
+                : launch_persistent<4, false, true>(B, L, groups, spill, cap, ctl);
         else if (node_w == 4)
             reg ? launch_persistent<4, true>(B, L, groups, spill, cap, ctl)
                 : launch_persistent<4, false>(B, L, groups, spill, cap, ctl);

@@ -295,7 +295,8 @@ __device__ __forceinline__ int wide_leaf_step_closest(const BlasDev& B, LaneStat
 
 // W-wide nodes (W = 4 or 8): test every child box, visit hit children
 // nearest first (equal entry distance: lower child index first).
-template <int W, bool STATS, class STK>
+// FLOAT_SLAB: the widened float box test (slab_dist_float).
+template <int W, bool STATS, class STK, bool FLOAT_SLAB = false>
 __device__ __forceinline__ int step_closest_w(const BlasDev& B, LaneState& L, const STK& S, uint32_t cap,
                                               uint32_t& n_nodes, uint32_t& n_tris) {
     if (L.node < 0) return wide_leaf_step_closest(B, L, S, n_tris, STATS);
@@ -315,9 +316,14 @@ __device__ __forceinline__ int step_closest_w(const BlasDev& B, LaneState& L, co
         const int q = c >> 2, k = c & 3;
         ptr[c] = __float_as_int(comp4(f[6 * G + q], k));
         d[c] = 0.0f;
-        h[c] = c < cnt && slab_dist(L.R, comp4(f[q], k), comp4(f[G + q], k), comp4(f[2 * G + q], k),
-                                    comp4(f[3 * G + q], k), comp4(f[4 * G + q], k), comp4(f[5 * G + q], k), L.hitT,
-                                    d[c]);
+        if (FLOAT_SLAB)
+            h[c] = c < cnt && slab_dist_float(L.R, comp4(f[q], k), comp4(f[G + q], k), comp4(f[2 * G + q], k),
+                                              comp4(f[3 * G + q], k), comp4(f[4 * G + q], k),
+                                              comp4(f[5 * G + q], k), L.hitT, d[c]);
+        else
+            h[c] = c < cnt && slab_dist(L.R, comp4(f[q], k), comp4(f[G + q], k), comp4(f[2 * G + q], k),
+                                        comp4(f[3 * G + q], k), comp4(f[4 * G + q], k), comp4(f[5 * G + q], k),
+                                        L.hitT, d[c]);
         hits += h[c] ? 1 : 0;
     }
     if (hits == 0) {
@@ -345,13 +351,11 @@ __device__ __forceinline__ int step_closest_w(const BlasDev& B, LaneState& L, co
 }
 
 // One closest-hit step of the variant (node width NODE_W in {2, 4, 8};
-// REG: register-topped stack; FLOAT_SLAB: widened float slab test, binary
-// nodes only).  NODE_W = 2 without REG or FLOAT_SLAB is the engine-order
-// binary step above.
+// REG: register-topped stack; FLOAT_SLAB: widened float slab test).
+// NODE_W = 2 without REG or FLOAT_SLAB is the engine-order binary step above.
 template <int NODE_W, bool REG, int BLOCK, int LDS_DEPTH, bool STATS, bool FLOAT_SLAB = false>
 __device__ __forceinline__ int step_variant(const BlasDev& B, LaneState& L, const LaneStack<BLOCK, LDS_DEPTH>& S,
                                             uint32_t cap, uint32_t& n_nodes, uint32_t& n_tris) {
-    static_assert(NODE_W == 2 || !FLOAT_SLAB, "the float slab test is for binary nodes");
     if constexpr (NODE_W == 2 && !REG && !FLOAT_SLAB)
         return step_closest<BLOCK, LDS_DEPTH, STATS>(B, L, S, cap, n_nodes, n_tris);
     else if constexpr (NODE_W == 2 && REG)
@@ -359,27 +363,36 @@ __device__ __forceinline__ int step_variant(const BlasDev& B, LaneState& L, cons
     else if constexpr (NODE_W == 2)
         return step_closest_s<STATS, FLOAT_SLAB>(B, L, MemStack<BLOCK, LDS_DEPTH>{S}, cap, n_nodes, n_tris);
     else if constexpr (REG)
-        return step_closest_w<NODE_W, STATS>(B, L, RegStack<BLOCK, LDS_DEPTH>{S}, cap, n_nodes, n_tris);
+        return step_closest_w<NODE_W, STATS, RegStack<BLOCK, LDS_DEPTH>, FLOAT_SLAB>(B, L, RegStack<BLOCK, LDS_DEPTH>{S},
+                                                                                  cap, n_nodes, n_tris);
     else
-        return step_closest_w<NODE_W, STATS>(B, L, MemStack<BLOCK, LDS_DEPTH>{S}, cap, n_nodes, n_tris);
+        return step_closest_w<NODE_W, STATS, MemStack<BLOCK, LDS_DEPTH>, FLOAT_SLAB>(B, L, MemStack<BLOCK, LDS_DEPTH>{S},
+                                                                                  cap, n_nodes, n_tris);
 }
 
-// The two halves of a binary step, for kernels that batch leaf work across a
-// wave: an inner-node step (L.node >= 0) and a triangle step (L.node < 0).
-template <bool REG, int BLOCK, int LDS_DEPTH, bool FLOAT_SLAB>
+// The two halves of a step, for kernels that batch leaf work across a wave:
+// an inner-node step (L.node >= 0) and a triangle step (L.node < 0), for
+// binary or wide nodes.
+template <int NODE_W, bool REG, int BLOCK, int LDS_DEPTH, bool FLOAT_SLAB>
 __device__ __forceinline__ int node_step_variant(const BlasDev& B, LaneState& L, const LaneStack<BLOCK, LDS_DEPTH>& S,
                                                  uint32_t cap) {
-    uint32_t n_nodes = 0;
-    if constexpr (REG)
+    uint32_t n_nodes = 0, n_tris = 0;
+    if constexpr (NODE_W > 2)
+        return step_variant<NODE_W, REG, BLOCK, LDS_DEPTH, false, FLOAT_SLAB>(B, L, S, cap, n_nodes, n_tris);
+    else if constexpr (REG)
         return node_step_closest<false, FLOAT_SLAB>(B, L, RegStack<BLOCK, LDS_DEPTH>{S}, cap, n_nodes);
     else
         return node_step_closest<false, FLOAT_SLAB>(B, L, MemStack<BLOCK, LDS_DEPTH>{S}, cap, n_nodes);
 }
 
-template <bool REG, int BLOCK, int LDS_DEPTH>
+template <int NODE_W, bool REG, int BLOCK, int LDS_DEPTH>
 __device__ __forceinline__ int leaf_step_variant(const BlasDev& B, LaneState& L, const LaneStack<BLOCK, LDS_DEPTH>& S) {
     uint32_t n_tris = 0;
-    if constexpr (REG)
+    if constexpr (NODE_W > 2 && REG)
+        return wide_leaf_step_closest(B, L, RegStack<BLOCK, LDS_DEPTH>{S}, n_tris, false);
+    else if constexpr (NODE_W > 2)
+        return wide_leaf_step_closest(B, L, MemStack<BLOCK, LDS_DEPTH>{S}, n_tris, false);
+    else if constexpr (REG)
         return leaf_step_closest(B, L, RegStack<BLOCK, LDS_DEPTH>{S}, n_tris, false);
     else
         return leaf_step_closest(B, L, MemStack<BLOCK, LDS_DEPTH>{S}, n_tris, false);
@@ -397,14 +410,13 @@ template <int NODE_W, bool REG, int BLOCK, int LDS_DEPTH, bool FLOAT_SLAB, int L
 __device__ __forceinline__ int wave_step(const BlasDev& B, LaneState& L, const LaneStack<BLOCK, LDS_DEPTH>& S,
                                          uint32_t cap, bool active) {
     if constexpr (LEAF_DIV > 0) {
-        static_assert(NODE_W == 2, "leaf batching is for binary nodes");
         const bool at_leaf = active && L.node < 0;
         const uint32_t nl = uint32_t(__popcll(__ballot(at_leaf)));
         const uint32_t na = uint32_t(__popcll(__ballot(active)));
         if (nl * LEAF_DIV >= na) {  // wave-uniform
-            if (at_leaf) return leaf_step_variant<REG, BLOCK, LDS_DEPTH>(B, L, S);
+            if (at_leaf) return leaf_step_variant<NODE_W, REG, BLOCK, LDS_DEPTH>(B, L, S);
         } else if (active && !at_leaf) {
-            return node_step_variant<REG, BLOCK, LDS_DEPTH, FLOAT_SLAB>(B, L, S, cap);
+            return node_step_variant<NODE_W, REG, BLOCK, LDS_DEPTH, FLOAT_SLAB>(B, L, S, cap);
         }
         return kContinue;
     } else {

@@ -46,6 +46,11 @@ KERNEL_NAMES = {1: "bvh2-exact grid (engine order, bit-exact)",
                 2 | REG | LEAFB: "bvh2-exact persistent, register stack top, leaf-batched waves",
                 2 | FSLAB | LEAFB: "bvh2 persistent, widened float slab, leaf-batched waves (same nearest hit)",
                 2 | REG | FSLAB | LEAFB: "bvh2 persistent, register stack top, widened float slab, leaf-batched "
+                                         "waves (same nearest hit)",
+                3 | LEAFB: "bvh4 persistent, leaf-batched waves (same nearest hit)",
+                3 | FSLAB: "bvh4 persistent, widened float slab (same nearest hit)",
+                3 | FSLAB | LEAFB: "bvh4 persistent, widened float slab, leaf-batched waves (same nearest hit)",
+                3 | REG | FSLAB | LEAFB: "bvh4 persistent, register stack top, widened float slab, leaf-batched "
                                          "waves (same nearest hit)"}
 METRIC = "Mrays/s incoherent closest-hit; path-trace time to equal rel-MSE, cache vs plain"
 
@@ -613,7 +618,8 @@ def main():
     if kernel == 0:
         # pick the fastest traversal variant (outside the timed region)
         for k in (1, 2, 3, 4, 1 | REG, 2 | REG, 3 | REG, 4 | REG, 1 | FSLAB, 2 | FSLAB, 1 | REG | FSLAB,
-                  2 | REG | FSLAB, 2 | LEAFB, 2 | REG | LEAFB, 2 | FSLAB | LEAFB, 2 | REG | FSLAB | LEAFB):
+                  2 | REG | FSLAB, 2 | LEAFB, 2 | REG | LEAFB, 2 | FSLAB | LEAFB, 2 | REG | FSLAB | LEAFB,
+                  3 | LEAFB, 3 | FSLAB, 3 | FSLAB | LEAFB, 3 | REG | FSLAB | LEAFB):
             for _ in range(3):
                 blas.trace_closest(d_rays, d_hits, kernel=k, stream=stream)
             torch.cuda.synchronize()

@@ -218,7 +218,8 @@ atlas_rt_status atlas_rt_blas_export(const atlas_rt_blas* blas, atlas_rt_gpu_bvh
 #define ATLAS_RT_KERNEL_BVH8 4
 #define ATLAS_RT_KERNEL_VARIANT_MASK 0xF
 #define ATLAS_RT_KERNEL_STACK_REGISTER 0x10
-/* With the binary kernels (1, 2): the widened float slab test.  Slab distances
+/* With the binary kernels (1, 2) and, for single-BLAS traces, the 4-wide
+ * kernel (3): the widened float slab test.  Slab distances
  * use the float reciprocal of the direction and a box counts as hit within
  * 2^-20 of its entry / exit distances, so every box the engine-exact test
  * enters is entered; triangles stay exact.  Same nearest hit as the engine
@@ -226,7 +227,8 @@ atlas_rt_status atlas_rt_blas_export(const atlas_rt_blas* blas, atlas_rt_gpu_bvh
  * the engine's slab test culls at a box face may be found. */
 #define ATLAS_RT_KERNEL_SLAB_FLOAT 0x20
 /* Or'ing ATLAS_RT_KERNEL_LEAF_BATCH into ATLAS_RT_KERNEL_BVH2_PERSISTENT
- * (with or without the two bits above) makes a wave take triangle steps only
+ * (with or without the two bits above), or, for single-BLAS traces, into
+ * ATLAS_RT_KERNEL_BVH4, makes a wave take triangle steps only
  * once a quarter of its live lanes wait at a leaf, inner-node steps
  * otherwise.  Every ray's own traversal is unchanged, so results are those of
  * the same kernel without the bit. */

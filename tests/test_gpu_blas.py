@@ -210,7 +210,9 @@ VARIANTS = [("bvh2 grid, register stack", 1 | 0x10), ("bvh2 persistent, register
             ("bvh4", 3), ("bvh4, register stack", 3 | 0x10), ("bvh8", 4), ("bvh8, register stack", 4 | 0x10),
             ("bvh2 grid, float slab", 1 | 0x20), ("bvh2 persistent, float slab", 2 | 0x20),
             ("bvh2 grid, register stack, float slab", 1 | 0x30),
-            ("bvh2 persistent, register stack, float slab", 2 | 0x30)]
+            ("bvh2 persistent, register stack, float slab", 2 | 0x30),
+            ("bvh4, leaf batch", 3 | 0x40), ("bvh4, float slab", 3 | 0x20), ("bvh4, float slab, leaf batch", 3 | 0x60),
+            ("bvh4, register stack, float slab, leaf batch", 3 | 0x70)]
 
 
 @pytest.mark.parametrize("name,kernel", VARIANTS)
@@ -250,7 +252,8 @@ def _fp32_t(rays, tris, i, slot):
     return sol[0] if hit else np.inf
 
 
-@pytest.mark.parametrize("kernel", [3, 4, 3 | 0x10, 4 | 0x10, 2 | 0x10, 2 | 0x20, 2 | 0x30, 1 | 0x30, 2 | 0x70])
+@pytest.mark.parametrize("kernel", [3, 4, 3 | 0x10, 4 | 0x10, 2 | 0x10, 2 | 0x20, 2 | 0x30, 1 | 0x30, 2 | 0x70,
+                                    3 | 0x60, 3 | 0x70])
 def test_variant_kernels_edge_rays_and_deep_chain(torch_cuda, kernel):
     """Vertex- and edge-aimed rays (non-watertight culling cases) and an
     80-deep tree whose stack spills past the register top and LDS rows.
