@@ -192,8 +192,8 @@ atlas_rt_status atlas_rt_batch_create(const atlas_rt_blas* const* blases, uint32
     atlas_rt_status st = upload(s->d_table, s->table);
     if (st == ATLAS_RT_OK) st = upload(s->d_batches, s->batches);
     if (st == ATLAS_RT_OK) st = upload(s->d_prefix, s->prefix);
-    if (st == ATLAS_RT_OK) st = s->flags.ensure(256);
-    if (st == ATLAS_RT_OK && hipMemset(s->flags.ptr, 0, 256) != hipSuccess) st = fail(ATLAS_RT_E_HIP, "hipMemset");
+    if (st == ATLAS_RT_OK) st = s->flags.ensure(kFlagBytes);
+    if (st == ATLAS_RT_OK && hipMemset(s->flags.ptr, 0, kFlagBytes) != hipSuccess) st = fail(ATLAS_RT_E_HIP, "hipMemset");
     if (st != ATLAS_RT_OK) {
         atlas_rt_batch_destroy(s);
         return st;

@@ -67,6 +67,9 @@ constexpr uint32_t kDefaultFrameSlots = 3;
 // Progressive frames one resident launch renders: the launch's tail (its
 // slowest paths) is paid once per batch, not once per frame.
 constexpr uint32_t kBatchFrames = 8;
+// The resident frame kernel's range counters (kPullRanges) live in the
+// slot's control blocks.
+static_assert(kPullRanges <= kBatchFrames, "the range counters live in the control blocks");
 // Counters of atlas_rt_pathtracer_work_counts.
 constexpr int kWorkCounters = 6;
 

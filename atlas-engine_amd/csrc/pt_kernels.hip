@@ -73,8 +73,9 @@ __global__ void k_frame_begin(FrameCtl* master, FrameCtl* ctl, int32_t bounces, 
         c->train_paths = 0;
         c->queries = 0;
         c->train_samples = 0;
-        c->pull = 0;
     }
+    // the resident kernel's range counters live in the first kPullRanges blocks
+    for (uint32_t r = 0; r < kPullRanges; r++) ctl[r].pull = 0;
     master->pcg_state = state;
 }
 
@@ -811,18 +812,26 @@ __global__ __launch_bounds__(kFrameBlock, 4) void k_frame_persistent(ShadeArgs A
     float4* Lpix = A.Lpix;       // that frame's radiance per ray ID
     const CacheArgs& C = A.cache;
     uint32_t c_closest = 0, c_shadow = 0, c_hits = 0, c_nodes = 0, c_tris = 0, c_insts = 0;
+    // paths from kPullRanges ranges of the path order, counters ctl0[r].pull
+    RangePull rp;
+    rp.init(total);
+    static_assert(sizeof(FrameCtl) % sizeof(uint32_t) == 0, "counters are word-strided");
 
     for (;;) {
         if (!exhausted) {
             const uint64_t idle = __ballot(phase == 0);
             if (idle) {
                 const uint32_t n = uint32_t(__popcll(idle));
-                uint32_t base = 0;
-                if (lane == 0) base = atomicAdd(&ctl0->pull, n);
+                uint32_t base = 0, end = 0;
+                if (lane == 0)
+                    rp.take(&ctl0[0].pull, uint32_t(sizeof(FrameCtl) / sizeof(uint32_t)), total, n, base, end);
                 base = __shfl(base, 0, 64);
+                end = __shfl(end, 0, 64);
+                rp.broadcast(0);
+                if (rp.dry >= kPullRanges) exhausted = true;
                 if (phase == 0) {
                     const uint32_t kk = base + uint32_t(__popcll(idle & ((1ull << lane) - 1ull)));
-                    if (kk < total) {
+                    if (kk < end) {
                         // with the cache (one frame): listed training paths, then every pixel
                         const uint32_t f = CACHE ? 0u : kk / G.rays;
                         const uint32_t k = CACHE ? (kk < listed ? 0u : kk - listed) : kk - f * G.rays;
@@ -874,7 +883,6 @@ __global__ __launch_bounds__(kFrameBlock, 4) void k_frame_persistent(ShadeArgs A
                         }
                     }
                 }
-                if (base + n >= total) exhausted = true;
             }
         }
         if (!__any(phase != 0)) {

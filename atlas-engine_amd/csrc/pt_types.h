@@ -84,7 +84,7 @@ struct FrameCtl {
     uint32_t queries;        // cache queries issued this frame
     uint32_t train_samples;  // training samples built this frame
     float train_loss;        // loss of the frame's last training step
-    uint32_t pull;           // paths taken so far by the resident frame kernel
+    uint32_t pull;           // the resident frame kernel's counter of path range r (ctl[r], r < kPullRanges)
     uint32_t pad2[2];
 };
 

@@ -108,6 +108,16 @@ struct atlas_rt_blas {
 
 namespace atlas_rt {
 
+// Resident (persistent) kernels hand out their work from kPullRanges
+// contiguous ranges, one counter each: a wave starts on the range of its XCD
+// and moves on when that runs dry, spreading the counter updates over eight
+// addresses (one per XCD of the MI355X) instead of one.
+constexpr uint32_t kPullRanges = 8;
+// BLAS flag block: [0] unused, [1] finished waves, [2] stack overflow flag,
+// and range r's counter at word kRangeWord * (r + 1) (128 B apart).
+constexpr uint32_t kRangeWord = 32;
+constexpr size_t kFlagBytes = size_t(kRangeWord) * (kPullRanges + 1) * sizeof(uint32_t);
+
 // Kernel launchers (trace_blas.hip).
 struct TraceLaunch {
     const void* rays = nullptr;  // atlas_rt_ray[count]

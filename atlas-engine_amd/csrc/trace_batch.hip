@@ -54,17 +54,22 @@ __global__ __launch_bounds__(BLOCK) void k_batch_persistent(BatchDev D, const ui
     bool exhausted = false;
     uint32_t ray = 0;
     float tmax = 0.0f;
+    RangePull rp;  // work items from kPullRanges ranges, counters ctl[kRangeWord * (r + 1)]
+    rp.init(count);
     for (;;) {
         if (!exhausted) {
             const uint64_t idle = __ballot(!active);
             if (idle) {
                 const uint32_t n = __popcll(idle);
-                uint32_t base = 0;
-                if (lane == 0) base = atomicAdd(&ctl[0], n);
+                uint32_t base = 0, end = 0;
+                if (lane == 0) rp.take(ctl + kRangeWord, kRangeWord, count, n, base, end);
                 base = __shfl(base, 0);
+                end = __shfl(end, 0);
+                rp.broadcast(0);
+                if (rp.dry >= kPullRanges) exhausted = true;
                 if (!active) {
                     const uint32_t w = base + __popcll(idle & ((1ull << lane) - 1));
-                    if (w < count) {
+                    if (w < end) {
                         const uint32_t b = find_batch(D.prefix, D.batch_count, w);
                         const uint4 desc = D.batches[b];
                         ray = desc.y + (w - D.prefix[b]);
@@ -80,7 +85,6 @@ __global__ __launch_bounds__(BLOCK) void k_batch_persistent(BatchDev D, const ui
                         }
                     }
                 }
-                if (base + n >= count) exhausted = true;
             }
         }
         if (!__any(active)) {
@@ -105,7 +109,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_persistent(BatchDev D, const ui
         __threadfence();
         const uint32_t waves = gridDim.x * (BLOCK / kWave);
         if (atomicAdd(&ctl[1], 1u) == waves - 1) {
-            atomicExch(&ctl[0], 0u);
+            for (uint32_t r = 0; r < kPullRanges; r++) atomicExch(&ctl[kRangeWord * (r + 1)], 0u);
             atomicExch(&ctl[1], 0u);
         }
     }

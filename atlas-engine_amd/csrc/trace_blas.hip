@@ -75,17 +75,22 @@ __global__ __launch_bounds__(BLOCK) void k_blas_closest_persistent(BlasDev B, co
     bool exhausted = false;
     uint32_t idx = 0;
     float tmax = 0.0f;
+    RangePull rp;  // rays from kPullRanges ranges, counters ctl[kRangeWord * (r + 1)]
+    rp.init(count);
     for (;;) {
         if (!exhausted) {
             const uint64_t idle = __ballot(!active);
             if (idle) {
                 const uint32_t n = __popcll(idle);
-                uint32_t base = 0;
-                if (lane == 0) base = atomicAdd(&ctl[0], n);
+                uint32_t base = 0, end = 0;
+                if (lane == 0) rp.take(ctl + kRangeWord, kRangeWord, count, n, base, end);
                 base = __shfl(base, 0);
+                end = __shfl(end, 0);
+                rp.broadcast(0);
+                if (rp.dry >= kPullRanges) exhausted = true;
                 if (!active) {
                     idx = base + __popcll(idle & ((1ull << lane) - 1));
-                    if (idx < count) {
+                    if (idx < end) {
                         const float4 r0 = rays[2 * size_t(idx)];
                         const float4 r1 = rays[2 * size_t(idx) + 1];
                         tmax = r1.w;
@@ -98,7 +103,6 @@ __global__ __launch_bounds__(BLOCK) void k_blas_closest_persistent(BlasDev B, co
                         }
                     }
                 }
-                if (base + n >= count) exhausted = true;
             }
         }
         if (!__any(active)) {
@@ -125,7 +129,7 @@ __global__ __launch_bounds__(BLOCK) void k_blas_closest_persistent(BlasDev B, co
         __threadfence();
         const uint32_t waves = gridDim.x * (BLOCK / kWave);
         if (atomicAdd(&ctl[1], 1u) == waves - 1) {
-            atomicExch(&ctl[0], 0u);
+            for (uint32_t r = 0; r < kPullRanges; r++) atomicExch(&ctl[kRangeWord * (r + 1)], 0u);
             atomicExch(&ctl[1], 0u);
         }
     }
@@ -202,9 +206,9 @@ static atlas_rt_status prepare(atlas_rt_blas& blas, uint32_t threads, uint32_t l
         if (st != ATLAS_RT_OK) return st;
     }
     if (!blas.flags.ptr) {
-        atlas_rt_status st = blas.flags.ensure(256);
+        atlas_rt_status st = blas.flags.ensure(kFlagBytes);
         if (st != ATLAS_RT_OK) return st;
-        ATLAS_HIP_TRY(hipMemset(blas.flags.ptr, 0, 256));
+        ATLAS_HIP_TRY(hipMemset(blas.flags.ptr, 0, kFlagBytes));
     }
     return ATLAS_RT_OK;
 }
@@ -330,7 +334,8 @@ atlas_rt_status launch_blas_closest(atlas_rt_blas& blas, const TraceLaunch& L0) 
 // ---------------------------------------------------------------------------
 namespace {
 
-constexpr size_t kNoAllocCtlBytes = 256;
+constexpr size_t kNoAllocCtlBytes = 2048;
+static_assert(kFlagBytes <= kNoAllocCtlBytes, "the flag block fits the control bytes");
 constexpr size_t kNoAllocSpillBytes = size_t(16) << 20;
 
 __device__ __align__(256) uint8_t scratch[kNoAllocCtlBytes + kNoAllocSpillBytes];
@@ -358,7 +363,7 @@ atlas_rt_status scratch_address(void** out) {
 }
 
 __global__ void k_zero_words(uint32_t* __restrict__ w, uint32_t n) {
-    if (threadIdx.x < n) w[threadIdx.x] = 0u;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) w[i] = 0u;
 }
 
 }  // namespace
@@ -386,8 +391,8 @@ atlas_rt_status launch_blas_closest_noalloc(const atlas_rt_blas& blas, const Tra
                                                    " (use atlas_rt_trace_closest_rays)");
     uint32_t* ctl = reinterpret_cast<uint32_t*>(sp);
     int32_t* spill = reinterpret_cast<int32_t*>(static_cast<uint8_t*>(sp) + kNoAllocCtlBytes);
-    // the ray counter, the wave counter and the overflow flag start at zero
-    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, L.stream, ctl, 4u);
+    // the range counters, the wave counter and the overflow flag start at zero
+    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(256), 0, L.stream, ctl, uint32_t(kNoAllocCtlBytes / 4));
     const BlasDev B = blas_dev(blas);
     const bool reg = (L.kernel & ATLAS_RT_KERNEL_STACK_REGISTER) != 0;
     const bool fslab = (L.kernel & ATLAS_RT_KERNEL_SLAB_FLOAT) != 0;

@@ -27,6 +27,47 @@ namespace atlas_rt {
 // Entries of the register-resident top of the stack (register stack kernels).
 constexpr int kRegTop = 6;
 
+// The XCD (accelerator complex) the wave runs on.
+__device__ __forceinline__ uint32_t xcc_id() {
+    uint32_t v;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
+    return v & 0xfu;
+}
+
+// A wave's position in the kPullRanges ranges of [0, total): counter r sits
+// at counters[r * stride] (32-bit words).  take() is called by one lane: it
+// reserves n items from the current range, moving to the next range while
+// the current one is dry, and returns the reserved [base, end) clipped to the
+// range (base == end when every range is dry: then dry == kPullRanges).
+struct RangePull {
+    uint32_t span, cur, dry;
+    __device__ __forceinline__ void init(uint32_t total) {
+        span = (total + kPullRanges - 1) / kPullRanges;
+        cur = xcc_id() % kPullRanges;
+        dry = 0;
+    }
+    __device__ __forceinline__ void take(uint32_t* counters, uint32_t stride, uint32_t total, uint32_t n,
+                                         uint32_t& base, uint32_t& end) {
+        base = end = 0;
+        while (dry < kPullRanges) {
+            const uint32_t lo = cur * span, hi = min(lo + span, total);
+            const uint32_t b = lo < hi ? lo + atomicAdd(&counters[size_t(cur) * stride], n) : hi;
+            if (b < hi) {
+                base = b;
+                end = hi;
+                return;
+            }
+            dry++;
+            cur = (cur + 1) % kPullRanges;
+        }
+    }
+    // after take() on one lane: every lane of the wave takes its state
+    __device__ __forceinline__ void broadcast(int from) {
+        cur = __shfl(cur, from, 64);
+        dry = __shfl(dry, from, 64);
+    }
+};
+
 struct LaneState {
     DevRay R;
     float hitT;

@@ -254,7 +254,7 @@ atlas_rt_status atlas_rt_trace_closest_rays(const atlas_rt_blas* blas, const atl
  * persistent kernels (ATLAS_RT_KERNEL_BVH2_PERSISTENT, optionally with the
  * STACK_REGISTER, SLAB_FLOAT and LEAF_BATCH bits; AUTO = persistent with leaf
  * batching).  The kernels' ray and wave counters and their traversal stack
- * spill live in a fixed 16 MiB + 256 B device array that each GPU holds until
+ * spill live in a fixed 16 MiB + 2 KiB device array that each GPU holds until
  * the process ends; nothing is allocated or freed, and the counters are
  * cleared by a small kernel, not a memset.  A BLAS whose stack bound (depth +
  * 1, or its stack limit) exceeds what that array holds (24 + 16 MiB /
