@@ -727,16 +727,18 @@ __global__ __launch_bounds__(BLOCK) void k_shadow(WorldDev W, const FrameCtl* __
 
 // ---------------------------------------------------------------------------
 // Resident frame kernel: one path per lane from camera ray to its last
-// bounce.  A resident grid of waves takes paths from the frame's `pull`
-// counter in the raygen kernels' tile order.  Each lane walks its path as
+// bounce.  A resident grid of waves takes paths in the raygen kernels' tile
+// order, from eight ranges of that order (one counter per XCD, RangePull).
+// Each lane walks its path as
 // trace (closest hit) -> shade -> shadow ray (any hit) -> next bounce, so no
 // launch waits for the slowest path of a bounce before the next bounce
 // starts: the frame's only tail is its slowest whole path.  Lanes that trace
 // closest hits and lanes that trace shadow rays share one step body over
 // opaque worlds (world_step_mixed).  Shading is batched per wave: the wave
-// keeps stepping until half of its busy lanes have finished a walk (or
-// all have), then those lanes shade or add their shadow ray's light
-// together.  Paths refill idle lanes between batches (one atomic per wave).
+// keeps stepping until its busy lanes have all finished a walk (kShadeDiv
+// = 1; 1 / kShadeDiv of them in general), then those lanes shade or add
+// their shadow ray's light together.  Paths refill idle lanes between
+// batches (one atomic per wave).
 //
 // Every path runs the same arithmetic in the same order as in the
 // bounce-by-bounce kernels (raygen, k_trace, shade_path, k_shadow), and
@@ -745,7 +747,7 @@ __global__ __launch_bounds__(BLOCK) void k_shadow(WorldDev W, const FrameCtl* __
 // ---------------------------------------------------------------------------
 constexpr int kFrameBlock = kResidentBlock;
 constexpr int kFrameLds = kResidentLds;
-constexpr uint32_t kShadeDiv = 2;
+constexpr uint32_t kShadeDiv = 1;
 
 // The pixel of the k-th path slot of the raygen kernels' 8x8-tile order.
 __device__ __forceinline__ uint32_t tile_pixel(uint32_t k, uint32_t width, uint32_t height) {

@@ -248,10 +248,14 @@ atlas_rt_status launch_blas_closest(atlas_rt_blas& blas, const TraceLaunch& L0) 
     atlas_rt_status st;
     uint32_t cap = 0;
     if (persistent) {
-        // Resident grid: four 256-lane groups per CU (measured on C0: 0.98 ms
-        // against 1.06 ms with eight; fewer resident waves keep more of the
-        // tree in each XCD's L2).
-        const uint32_t groups = uint32_t(device_cu_count()) * 4;
+        // Resident grid: 256-lane groups per CU.  Four in general (measured
+        // on C0: 0.98 ms against 1.06 ms with eight; fewer resident waves
+        // keep more of the tree in each XCD's L2); five for the binary exact
+        // leaf-batched kernel, whose 96 VGPRs leave room for a fifth wave per
+        // SIMD (C0: 0.824 against 0.852 ms, where the float-slab variants
+        // slow down with five).
+        const bool five = node_w == 2 && leaf_batch && !reg && !fslab;
+        const uint32_t groups = uint32_t(device_cu_count()) * (five ? 5u : 4u);
         st = prepare(blas, groups * kPBlock, kPLdsDepth, cap, node_w);
         if (st != ATLAS_RT_OK) return st;
         ctl = reinterpret_cast<uint32_t*>(blas.flags.ptr);
