@@ -23,10 +23,11 @@ struct LbvhMeshOut {
 };
 
 // Builds every mesh of the batch with one set of launches on `stream`.
-// verts: per mesh, 9 floats per triangle (v0, v1, v2) in host memory.
+// verts: device memory, 9 floats per triangle (v0, v1, v2), the meshes one
+// after another in batch order.
 // Per mesh at most 2^24 triangles; at most 1024 meshes per call.
 // depth_out: per mesh (host), filled after the stream has been synchronised.
-atlas_rt_status lbvh_build(const std::vector<const float*>& verts, const std::vector<uint32_t>& counts,
+atlas_rt_status lbvh_build(const float* verts, const std::vector<uint32_t>& counts,
                            const std::vector<LbvhMeshOut>& outs, std::vector<uint32_t>& depth_out,
                            hipStream_t stream, float* device_ms);
 

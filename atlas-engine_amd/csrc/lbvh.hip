@@ -513,11 +513,11 @@ struct DevBuf {
 
 }  // namespace
 
-atlas_rt_status lbvh_build(const std::vector<const float*>& verts, const std::vector<uint32_t>& counts,
+atlas_rt_status lbvh_build(const float* verts, const std::vector<uint32_t>& counts,
                            const std::vector<LbvhMeshOut>& outs, std::vector<uint32_t>& depth_out,
                            hipStream_t stream, float* device_ms) {
     const uint32_t k = uint32_t(counts.size());
-    if (k == 0 || k > kMaxMeshes || verts.size() != k || outs.size() != k)
+    if (k == 0 || k > kMaxMeshes || !verts || outs.size() != k)
         return fail(ATLAS_RT_E_INVALID_ARG, "lbvh_build: 1..1024 meshes per call");
     std::vector<uint32_t> leaf_off(k + 1, 0), inner_off(k + 1, 0);
     for (uint32_t m = 0; m < k; m++) {
@@ -528,7 +528,6 @@ atlas_rt_status lbvh_build(const std::vector<const float*>& verts, const std::ve
         if (leaf_off[m + 1] < leaf_off[m]) return fail(ATLAS_RT_E_INVALID_ARG, "lbvh_build: more than 2^32 triangles");
     }
     const uint32_t total = leaf_off[k], inner = inner_off[k], nodes = total + inner;
-    DevBuf<float> d_verts;
     DevBuf<uint32_t> d_off;
     DevBuf<LbvhMeshOut> d_outs;
     DevBuf<float4> cen, leaf_lo, leaf_hi, inner_lo, inner_hi;
@@ -538,8 +537,7 @@ atlas_rt_status lbvh_build(const std::vector<const float*>& verts, const std::ve
     DevBuf<uint32_t> cnt, depth, parts, maxd;
     DevBuf<int2> dep0, dep1;
     DevBuf<int4> st0, st1;
-    hipError_t e = d_verts.alloc(9 * size_t(total));
-    if (e == hipSuccess) e = d_off.alloc(2 * (k + 1));
+    hipError_t e = d_off.alloc(2 * (k + 1));
     if (e == hipSuccess) e = d_outs.alloc(k);
     if (e == hipSuccess) e = cen.alloc(total);
     if (e == hipSuccess) e = leaf_lo.alloc(total);
@@ -560,9 +558,6 @@ atlas_rt_status lbvh_build(const std::vector<const float*>& verts, const std::ve
     if (e == hipSuccess) e = st0.alloc(nodes);
     if (e == hipSuccess) e = st1.alloc(nodes);
     if (e != hipSuccess) return hip_fail(e, "lbvh_build (scratch)");
-    for (uint32_t m = 0; m < k && e == hipSuccess; m++)
-        e = hipMemcpyAsync(d_verts.p + 9 * size_t(leaf_off[m]), verts[m], 9 * size_t(counts[m]) * sizeof(float),
-                           hipMemcpyHostToDevice, stream);
     std::vector<uint32_t> offs(leaf_off);
     offs.insert(offs.end(), inner_off.begin(), inner_off.end());
     if (e == hipSuccess)
@@ -585,7 +580,7 @@ atlas_rt_status lbvh_build(const std::vector<const float*>& verts, const std::ve
         ATLAS_HIP_TRY(hipEventCreate(&ev.t1));
         ATLAS_HIP_TRY(hipEventRecord(ev.t0, stream));
     }
-    const Batch Bt{d_verts.p, d_off.p, d_off.p + (k + 1), k, total, inner};
+    const Batch Bt{verts, d_off.p, d_off.p + (k + 1), k, total, inner};
     // running minimum starts at +FLT_MAX, running maximum at -FLT_MAX (ordered-int form)
     auto ordered = [](float f) {
         int i;

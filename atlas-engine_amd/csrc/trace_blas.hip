@@ -253,7 +253,7 @@ atlas_rt_status launch_blas_closest(atlas_rt_blas& blas, const TraceLaunch& L0) 
         // keep more of the tree in each XCD's L2); five for the binary exact
         // leaf-batched kernel, whose 96 VGPRs leave room for a fifth wave per
         // SIMD (C0: 0.824 against 0.852 ms, where the float-slab variants
-        // slow down with five).
+        // slow down with five; six, built for 80 VGPRs, took 0.84 ms).
         const bool five = node_w == 2 && leaf_batch && !reg && !fslab;
         const uint32_t groups = uint32_t(device_cu_count()) * (five ? 5u : 4u);
         st = prepare(blas, groups * kPBlock, kPLdsDepth, cap, node_w);

@@ -26,6 +26,7 @@ void children_of(const atlas_rt_gpu_bvh_node& n, Child& l, Child& r) {
     std::memcpy(r.bmax, n.right_max, 12);
     l.ptr = n.left_ptr;
     r.ptr = n.right_ptr;
+    l.leaf = r.leaf = false;
 }
 
 }  // namespace
@@ -85,67 +86,66 @@ WideTree collapse_wide(const std::vector<atlas_rt_gpu_bvh_node>& nodes, const st
         ch.leaf = true;
         return true;
     };
-    struct Item {
-        int32_t bin;     // binary node
-        uint32_t wide;   // wide node it becomes
-    };
-    std::vector<Item> work{{0, 0}};
-    std::vector<std::vector<Child>> kids(1);
-    std::vector<std::vector<uint32_t>> inner_children(1);
-    for (size_t w = 0; w < work.size(); w++) {
-        std::vector<Child> c(2);
-        children_of(nodes[size_t(work[w].bin)], c[0], c[1]);
-        for (Child& ch : c) as_leaf(ch);
-        while (int(c.size()) < width) {
+    // wide nodes in breadth-first order: node w is written when it is taken
+    // from the queue, and its inner children get the next free numbers
+    const size_t per = size_t(F) * 4;
+    std::vector<int32_t> queue{0};  // binary node of every wide node
+    queue.reserve(nodes.size() / size_t(width - 1) + 1);
+    out.data.reserve(queue.capacity() * per);
+    Child c[8];
+    for (size_t w = 0; w < queue.size(); w++) {
+        int n = 2;
+        children_of(nodes[size_t(queue[w])], c[0], c[1]);
+        as_leaf(c[0]);
+        as_leaf(c[1]);
+        while (n < width) {
             int best = -1;
             float best_sa = -1.0f;
-            for (int i = 0; i < int(c.size()); i++)
-                if (!c[size_t(i)].leaf && c[size_t(i)].ptr >= 0 && area(c[size_t(i)]) > best_sa) {
-                    best_sa = area(c[size_t(i)]);
+            for (int i = 0; i < n; i++)
+                if (!c[i].leaf && c[i].ptr >= 0 && area(c[i]) > best_sa) {
+                    best_sa = area(c[i]);
                     best = i;
                 }
             if (best < 0) break;
             Child l, r;
-            children_of(nodes[size_t(c[size_t(best)].ptr)], l, r);
+            children_of(nodes[size_t(c[best].ptr)], l, r);
             as_leaf(l);
             as_leaf(r);
-            c[size_t(best)] = l;
-            c.push_back(r);
+            c[best] = l;
+            c[n++] = r;
         }
-        for (Child& ch : c)
-            if (!ch.leaf && ch.ptr >= 0) {
-                const uint32_t id = uint32_t(work.size());
-                work.push_back({ch.ptr, id});
-                kids.emplace_back();
-                inner_children.emplace_back();
-                inner_children[w].push_back(id);
-                ch.ptr = int32_t(id);
+        out.data.resize(out.data.size() + per, 0.0f);
+        float* nd = out.data.data() + w * per;
+        for (int i = 0; i < n; i++) {
+            if (!c[i].leaf && c[i].ptr >= 0) {
+                const int32_t id = int32_t(queue.size());
+                queue.push_back(c[i].ptr);
+                c[i].ptr = id;
             }
-        kids[w] = std::move(c);
-    }
-    out.node_count = uint32_t(work.size());
-    out.data.assign(size_t(out.node_count) * F * 4, 0.0f);
-    for (size_t w = 0; w < work.size(); w++) {
-        float* nd = out.data.data() + w * size_t(F) * 4;
-        const auto& c = kids[w];
-        for (size_t i = 0; i < c.size(); i++) {
-            const size_t q = i / 4, lane = i % 4;
-            for (int a = 0; a < 3; a++) {
-                nd[(size_t(a) * G + q) * 4 + lane] = c[i].bmin[a];
-                nd[(size_t(3 + a) * G + q) * 4 + lane] = c[i].bmax[a];
+            const size_t q = size_t(i) / 4, lane = size_t(i) % 4;
+            for (int ax = 0; ax < 3; ax++) {
+                nd[(size_t(ax) * G + q) * 4 + lane] = c[i].bmin[ax];
+                nd[(size_t(3 + ax) * G + q) * 4 + lane] = c[i].bmax[ax];
             }
             std::memcpy(&nd[(6 * size_t(G) + q) * 4 + lane], &c[i].ptr, 4);
         }
-        const int32_t n = int32_t(c.size());
         std::memcpy(&nd[7 * size_t(G) * 4], &n, 4);
     }
+    out.node_count = uint32_t(queue.size());
     // stack bound: along any root-to-leaf path, a node pushes at most
     // (children - 1) entries; children are numbered after their parents
-    std::vector<uint32_t> depth(work.size(), 0);
-    for (size_t w = work.size(); w-- > 0;) {
+    std::vector<uint32_t> depth(queue.size(), 0);
+    for (size_t w = queue.size(); w-- > 0;) {
+        const float* nd = out.data.data() + w * per;
+        int32_t n;
+        std::memcpy(&n, &nd[7 * size_t(G) * 4], 4);
         uint32_t below = 0;
-        for (uint32_t ch : inner_children[w]) below = std::max(below, depth[ch]);
-        depth[w] = uint32_t(kids[w].size()) - 1 + below;
+        for (int i = 0; i < n; i++) {
+            int32_t ptr;
+            std::memcpy(&ptr, &nd[(6 * size_t(G) + size_t(i) / 4) * 4 + size_t(i) % 4], 4);
+            if (ptr >= 0) below = std::max(below, depth[size_t(ptr)]);
+        }
+        depth[w] = uint32_t(n) - 1 + below;
     }
     out.max_stack = depth[0];
     return out;
