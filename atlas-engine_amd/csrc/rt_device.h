@@ -34,6 +34,17 @@
 
 namespace atlas_rt {
 
+// Loads of tree, triangle and instance records.  Their pointers are often
+// read from tables in memory (a world's BLAS table, a batch's BLAS list), so
+// the compiler cannot tell their address space and emits flat loads, whose
+// waits also cover every LDS access in flight (the traversal stack).  These
+// records always live in global memory: load them as such.
+__device__ __forceinline__ float4 ldg4(const float4* p) {
+    using V4 = float __attribute__((ext_vector_type(4)));
+    const V4 v = *(const __attribute__((address_space(1))) V4*)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
 struct DevRay {
     float ox, oy, oz;
     float dx, dy, dz;

@@ -73,7 +73,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_persistent(BatchDev D, const ui
                         const uint32_t b = find_batch(D.prefix, D.batch_count, w);
                         const uint4 desc = D.batches[b];
                         ray = desc.y + (w - D.prefix[b]);
-                        B = D.blases[desc.x];
+                        B = ld_blas(D.blases + desc.x);
                         const float4 r0 = rays[2 * size_t(ray)];
                         const float4 r1 = rays[2 * size_t(ray) + 1];
                         tmax = r1.w;
@@ -127,7 +127,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_grid(BatchDev D, const uint32_t
     const uint32_t b = find_batch(D.prefix, D.batch_count, w);
     const uint4 desc = D.batches[b];
     const uint32_t ray = desc.y + (w - D.prefix[b]);
-    const BlasDev B = D.blases[desc.x];
+    const BlasDev B = ld_blas(D.blases + desc.x);
     const float4 r0 = rays[2 * size_t(ray)];
     const float4 r1 = rays[2 * size_t(ray) + 1];
     LaneState L;

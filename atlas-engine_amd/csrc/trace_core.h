@@ -68,6 +68,20 @@ struct RangePull {
     }
 };
 
+// A BLAS record read from a table in global memory (see ldg4).
+__device__ __forceinline__ BlasDev ld_blas(const BlasDev* p) {
+    const __attribute__((address_space(1))) BlasDev* g = (const __attribute__((address_space(1))) BlasDev*)p;
+    BlasDev b;
+    b.nodes = g->nodes;
+    b.tris = g->tris;
+    b.vtx = g->vtx;
+    b.nodes4 = g->nodes4;
+    b.nodes8 = g->nodes8;
+    b.node_count = g->node_count;
+    b.tri_count = g->tri_count;
+    return b;
+}
+
 struct LaneState {
     DevRay R;
     float hitT;
@@ -88,14 +102,20 @@ struct LaneStack {
     uint32_t column;
     uint32_t stride;
 
+    // The LDS rows are addressed through an LDS (address space 3) pointer:
+    // with two generic pointers the compiler merges the two loads of get()
+    // into one flat load of a selected address, which waits for every
+    // outstanding vector memory access as well as the LDS.
+    using LdsPtr = __attribute__((address_space(3))) int32_t*;
+    __device__ __forceinline__ LdsPtr rows() const { return (LdsPtr)lds; }
     __device__ __forceinline__ void put(uint32_t sp, int32_t v) const {
         if (sp < LDS_DEPTH)
-            lds[sp * BLOCK + lane] = v;
+            rows()[sp * BLOCK + lane] = v;
         else
             spill[size_t(sp - LDS_DEPTH) * stride + column] = v;
     }
     __device__ __forceinline__ int32_t get(uint32_t sp) const {
-        if (sp < LDS_DEPTH) return lds[sp * BLOCK + lane];
+        if (sp < LDS_DEPTH) return rows()[sp * BLOCK + lane];
         return spill[size_t(sp - LDS_DEPTH) * stride + column];
     }
 };
@@ -144,7 +164,7 @@ __device__ __forceinline__ int step_closest(const BlasDev& B, LaneState& L, cons
                                             uint32_t cap, uint32_t& n_nodes, uint32_t& n_tris) {
     if (L.node >= 0) {
         const float4* np = B.nodes + 4 * size_t(L.node);
-        const float4 a = np[0], b = np[1], c = np[2], d = np[3];
+        const float4 a = ldg4(np), b = ldg4(np + 1), c = ldg4(np + 2), d = ldg4(np + 3);
         if (STATS) n_nodes++;
         float dl, dr;
         const bool hl = slab_dist(L.R, a.x, a.y, a.z, a.w, b.x, b.y, L.hitT, dl);
@@ -167,7 +187,7 @@ __device__ __forceinline__ int step_closest(const BlasDev& B, LaneState& L, cons
     }
     const int32_t tp = ~L.node;
     const float4* tr = B.tris + 3 * size_t(tp);
-    const float4 a = tr[0], b = tr[1], c = tr[2];
+    const float4 a = ldg4(tr), b = ldg4(tr + 1), c = ldg4(tr + 2);
     if (STATS) n_tris++;
     float t, u, v;
     const bool geo = tri_uvt(L.R, a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z, t, u, v);
@@ -192,7 +212,7 @@ __device__ __forceinline__ int step_any(const BlasDev& B, LaneState& L, const La
                                         uint32_t cap, float tmax) {
     if (L.node >= 0) {
         const float4* np = B.nodes + 4 * size_t(L.node);
-        const float4 a = np[0], b = np[1], c = np[2], d = np[3];
+        const float4 a = ldg4(np), b = ldg4(np + 1), c = ldg4(np + 2), d = ldg4(np + 3);
         float dl, dr;
         const bool hl = slab_dist(L.R, a.x, a.y, a.z, a.w, b.x, b.y, tmax, dl);
         const bool hr = slab_dist(L.R, b.z, b.w, c.x, c.y, c.z, c.w, tmax, dr);
@@ -213,7 +233,7 @@ __device__ __forceinline__ int step_any(const BlasDev& B, LaneState& L, const La
     }
     const int32_t tp = ~L.node;
     const float4* tr = B.tris + 3 * size_t(tp);
-    const float4 a = tr[0], b = tr[1], c = tr[2];
+    const float4 a = ldg4(tr), b = ldg4(tr + 1), c = ldg4(tr + 2);
     float t, u, v;
     const bool geo = tri_uvt(L.R, a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z, t, u, v);
     if (geo && t > L.R.tmin && t < tmax) {
@@ -245,7 +265,7 @@ __device__ __forceinline__ int leaf_step_closest(const BlasDev& B, LaneState& L,
                                                  bool stats) {
     const int32_t tp = ~L.node;
     const float4* tr = B.tris + 3 * size_t(tp);
-    const float4 a = tr[0], b = tr[1], c = tr[2];
+    const float4 a = ldg4(tr), b = ldg4(tr + 1), c = ldg4(tr + 2);
     if (stats) n_tris++;
     float t, u, v;
     const bool geo = tri_uvt(L.R, a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z, t, u, v);
@@ -270,7 +290,7 @@ template <bool STATS, bool FLOAT_SLAB, class STK>
 __device__ __forceinline__ int node_step_closest(const BlasDev& B, LaneState& L, const STK& S, uint32_t cap,
                                                  uint32_t& n_nodes) {
     const float4* np = B.nodes + 4 * size_t(L.node);
-    const float4 a = np[0], b = np[1], c = np[2], d = np[3];
+    const float4 a = ldg4(np), b = ldg4(np + 1), c = ldg4(np + 2), d = ldg4(np + 3);
     if (STATS) n_nodes++;
     bool hl, hr, leftFirst;
     if (FLOAT_SLAB) {
@@ -315,7 +335,7 @@ __device__ __forceinline__ int wide_leaf_step_closest(const BlasDev& B, LaneStat
     const uint32_t enc = uint32_t(~L.node);
     const uint32_t tp = enc >> 3, code = enc & 7u;
     const float4* tr = B.tris + 3 * size_t(tp);
-    const float4 a = tr[0], b = tr[1], c = tr[2];
+    const float4 a = ldg4(tr), b = ldg4(tr + 1), c = ldg4(tr + 2);
     if (stats) n_tris++;
     float t, u, v;
     const bool geo = tri_uvt(L.R, a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z, t, u, v);
@@ -345,7 +365,7 @@ __device__ __forceinline__ int step_closest_w(const BlasDev& B, LaneState& L, co
     const float4* np = (W == 4 ? B.nodes4 : B.nodes8) + F * size_t(L.node);
     float4 f[F];
 #pragma unroll
-    for (int i = 0; i < F; i++) f[i] = np[i];
+    for (int i = 0; i < F; i++) f[i] = ldg4(np + i);
     if (STATS) n_nodes++;
     const int cnt = __float_as_int(f[7 * G].x);
     float d[W];
@@ -488,7 +508,7 @@ __device__ __forceinline__ float4 lane_hit_record(const BlasDev& B, const LaneSt
     if (L.slot < 0) return make_float4(tmax, 0.0f, 0.0f, __int_as_float(-1));
     const float4* vp = B.vtx + 3 * size_t(L.slot);
     float t, u, v;
-    refine_tuv(L.R.ox, L.R.oy, L.R.oz, L.R.dx, L.R.dy, L.R.dz, vp[0], vp[1], vp[2], t, u, v);
+    refine_tuv(L.R.ox, L.R.oy, L.R.oz, L.R.dx, L.R.dy, L.R.dz, ldg4(vp), ldg4(vp + 1), ldg4(vp + 2), t, u, v);
     return make_float4(t, u, v, __int_as_float(L.prim));
 }
 

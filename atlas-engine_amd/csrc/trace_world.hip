@@ -50,7 +50,7 @@ __global__ __launch_bounds__(BLOCK) void k_world_closest_rays(WorldDev W, const 
     int32_t caller_inst = -1;
     if (L.slot >= 0) {
         const float4* im = W.instances + 4 * size_t(L.inst);
-        const BlasDev B = W.blases[__float_as_int(im[3].x)];
+        const BlasDev B = ld_blas(W.blases + __float_as_int(im[3].x));
         const float4* vp = B.vtx + 3 * size_t(L.slot);
         float t, u, v;
         refine_tuv(L.hox, L.hoy, L.hoz, L.hdx, L.hdy, L.hdz, vp[0], vp[1], vp[2], t, u, v);

@@ -83,7 +83,7 @@ __device__ __forceinline__ void world_begin(WorldLane& L, float4 r0, float4 r1) 
 __device__ __forceinline__ bool enter_instance(WorldLane& L, const WorldDev& W, uint32_t mask) {
     const int32_t ip = ~L.node;
     const float4* m = W.instances + 4 * size_t(ip);
-    const float4 c0 = m[0], c1 = m[1], c2 = m[2], meta = m[3];
+    const float4 c0 = ldg4(m), c1 = ldg4(m + 1), c2 = ldg4(m + 2), meta = ldg4(m + 3);
     const float ox = dot4(L.o0x, L.o0y, L.o0z, 1.0f, c0);
     const float oy = dot4(L.o0x, L.o0y, L.o0z, 1.0f, c1);
     const float oz = dot4(L.o0x, L.o0y, L.o0z, 1.0f, c2);
@@ -93,7 +93,7 @@ __device__ __forceinline__ bool enter_instance(WorldLane& L, const WorldDev& W, 
     make_dev_ray(L.R, ox, oy, oz, dx, dy, dz, L.R.tmin);
     L.cur_inst = ip;
     const int32_t mesh = __float_as_int(meta.x);
-    L.B = W.blases[mesh];
+    L.B = ld_blas(W.blases + mesh);
     L.node = L.B.node_count ? 0 : ~0;
     return (uint32_t(__float_as_int(meta.w)) & mask) > 0;
 }
@@ -120,7 +120,7 @@ __device__ __forceinline__ int world_step_closest(WorldLane& L, const WorldDev& 
             return kContinue;
         }
         const float4* np = W.tlas + 4 * size_t(L.node);
-        const float4 a = np[0], b = np[1], c = np[2], d = np[3];
+        const float4 a = ldg4(np), b = ldg4(np + 1), c = ldg4(np + 2), d = ldg4(np + 3);
         float dl, dr;
         const bool hl = slab_dist(L.R, a.x, a.y, a.z, a.w, b.x, b.y, L.hitT, dl);
         const bool hr = slab_dist(L.R, b.z, b.w, c.x, c.y, c.z, c.w, L.hitT, dr);
@@ -142,7 +142,7 @@ __device__ __forceinline__ int world_step_closest(WorldLane& L, const WorldDev& 
     }
     if (L.node >= 0) {
         const float4* np = L.B.nodes + 4 * size_t(L.node);
-        const float4 a = np[0], b = np[1], c = np[2], d = np[3];
+        const float4 a = ldg4(np), b = ldg4(np + 1), c = ldg4(np + 2), d = ldg4(np + 3);
         float dl, dr;
         const bool hl = slab_dist(L.R, a.x, a.y, a.z, a.w, b.x, b.y, L.hitT, dl);
         const bool hr = slab_dist(L.R, b.z, b.w, c.x, c.y, c.z, c.w, L.hitT, dr);
@@ -164,7 +164,7 @@ __device__ __forceinline__ int world_step_closest(WorldLane& L, const WorldDev& 
     }
     const int32_t tp = ~L.node;
     const float4* tr = L.B.tris + 3 * size_t(tp);
-    const float4 a = tr[0], b = tr[1], c = tr[2];
+    const float4 a = ldg4(tr), b = ldg4(tr + 1), c = ldg4(tr + 2);
     float t, u, v;
     const bool geo = tri_uvt(L.R, a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z, t, u, v);
     if (geo && t > L.R.tmin && t < L.hitT && accept(L.cur_inst, tp, u, v)) {
@@ -211,7 +211,7 @@ __device__ __forceinline__ int world_step_any(WorldLane& L, const WorldDev& W, u
             return kContinue;
         }
         const float4* np = W.tlas + 4 * size_t(L.node);
-        const float4 a = np[0], b = np[1], c = np[2], d = np[3];
+        const float4 a = ldg4(np), b = ldg4(np + 1), c = ldg4(np + 2), d = ldg4(np + 3);
         float dl, dr;
         const bool hl = slab_dist(L.R, a.x, a.y, a.z, a.w, b.x, b.y, tmax, dl);
         const bool hr = slab_dist(L.R, b.z, b.w, c.x, c.y, c.z, c.w, tmax, dr);
@@ -232,7 +232,7 @@ __device__ __forceinline__ int world_step_any(WorldLane& L, const WorldDev& W, u
     }
     if (L.node >= 0) {
         const float4* np = L.B.nodes + 4 * size_t(L.node);
-        const float4 a = np[0], b = np[1], c = np[2], d = np[3];
+        const float4 a = ldg4(np), b = ldg4(np + 1), c = ldg4(np + 2), d = ldg4(np + 3);
         float dl, dr;
         const bool hl = slab_dist(L.R, a.x, a.y, a.z, a.w, b.x, b.y, tmax, dl);
         const bool hr = slab_dist(L.R, b.z, b.w, c.x, c.y, c.z, c.w, tmax, dr);
@@ -253,7 +253,7 @@ __device__ __forceinline__ int world_step_any(WorldLane& L, const WorldDev& W, u
     }
     const int32_t tp = ~L.node;
     const float4* tr = L.B.tris + 3 * size_t(tp);
-    const float4 a = tr[0], b = tr[1], c = tr[2];
+    const float4 a = ldg4(tr), b = ldg4(tr + 1), c = ldg4(tr + 2);
     float t, u, v;
     const bool geo = tri_uvt(L.R, a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z, t, u, v);
     if (geo && t > L.R.tmin && t < tmax) {
@@ -304,7 +304,7 @@ __device__ __forceinline__ int world_step_mixed(WorldLane& L, const WorldDev& W,
     }
     if (top || L.node >= 0) {
         const float4* np = (top ? W.tlas : L.B.nodes) + 4 * size_t(L.node);
-        const float4 a = np[0], b = np[1], c = np[2], d = np[3];
+        const float4 a = ldg4(np), b = ldg4(np + 1), c = ldg4(np + 2), d = ldg4(np + 3);
         float dl, dr;
         const bool hl = slab_dist(L.R, a.x, a.y, a.z, a.w, b.x, b.y, L.hitT, dl);
         const bool hr = slab_dist(L.R, b.z, b.w, c.x, c.y, c.z, c.w, L.hitT, dr);
@@ -326,7 +326,7 @@ __device__ __forceinline__ int world_step_mixed(WorldLane& L, const WorldDev& W,
     }
     const int32_t tp = ~L.node;
     const float4* tr = L.B.tris + 3 * size_t(tp);
-    const float4 a = tr[0], b = tr[1], c = tr[2];
+    const float4 a = ldg4(tr), b = ldg4(tr + 1), c = ldg4(tr + 2);
     float t, u, v;
     const bool geo = tri_uvt(L.R, a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z, t, u, v);
     if (geo && t > L.R.tmin && t < L.hitT) {
@@ -378,7 +378,7 @@ __device__ __forceinline__ int world_step_transparency(WorldLane& L, const World
             return kContinue;
         }
         const float4* np = W.tlas + 4 * size_t(L.node);
-        const float4 a = np[0], b = np[1], c = np[2], d = np[3];
+        const float4 a = ldg4(np), b = ldg4(np + 1), c = ldg4(np + 2), d = ldg4(np + 3);
         float dl, dr;
         const bool hl = slab_dist(L.R, a.x, a.y, a.z, a.w, b.x, b.y, tmax, dl);
         const bool hr = slab_dist(L.R, b.z, b.w, c.x, c.y, c.z, c.w, tmax, dr);
@@ -399,7 +399,7 @@ __device__ __forceinline__ int world_step_transparency(WorldLane& L, const World
     }
     if (L.node >= 0) {
         const float4* np = L.B.nodes + 4 * size_t(L.node);
-        const float4 a = np[0], b = np[1], c = np[2], d = np[3];
+        const float4 a = ldg4(np), b = ldg4(np + 1), c = ldg4(np + 2), d = ldg4(np + 3);
         float dl, dr;
         const bool hl = slab_dist(L.R, a.x, a.y, a.z, a.w, b.x, b.y, tmax, dl);
         const bool hr = slab_dist(L.R, b.z, b.w, c.x, c.y, c.z, c.w, tmax, dr);
@@ -421,7 +421,7 @@ __device__ __forceinline__ int world_step_transparency(WorldLane& L, const World
     float leaf = L.trans;
     for (int32_t tp = ~L.node;; tp++) {
         const float4* tr = L.B.tris + 3 * size_t(tp);
-        const float4 a = tr[0], b = tr[1], c = tr[2];
+        const float4 a = ldg4(tr), b = ldg4(tr + 1), c = ldg4(tr + 2);
         float t, u, v;
         const bool geo = tri_uvt(L.R, a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z, t, u, v);
         if (geo && t > L.R.tmin && t < tmax) leaf *= (1.0f - opacity(L.cur_inst, tp, u, v));
