@@ -111,6 +111,10 @@ void free_all(PathTracer& P) {
     P.graph = nullptr;
     if (P.ev_start) (void)hipEventDestroy(P.ev_start);
     P.ev_start = nullptr;
+    for (hipEvent_t& e : P.occ_ev) {
+        if (e) (void)hipEventDestroy(e);
+        e = nullptr;
+    }
 }
 
 // Random::Init seeds PCG32 with uint64(SampleUniformFloat()) = 0

@@ -73,12 +73,19 @@ static_assert(kPullRanges <= kBatchFrames, "the range counters live in the contr
 // Counters of atlas_rt_pathtracer_work_counts.
 constexpr int kWorkCounters = 6;
 
-// The resident frame kernel's launch shape: 256-lane groups, four per CU
-// (four waves per SIMD), a 24-entry LDS stack per lane; deeper entries spill
-// to the slot's spill rows, one column per resident lane.
+// The resident frame kernel's launch shape: 256-lane groups, a 24-entry LDS
+// stack per lane (deeper entries spill to the slot's spill rows, one column
+// per resident lane), and three or four groups per CU, one wave per SIMD
+// each.  Three leave a lane 168 VGPRs (128 at four, with about twice the
+// register spills) and win where most paths end early; four hide more
+// memory latency where most paths are shaded at every bounce.  After one
+// warm-up launch, kOccTrials plain launches alternate the two, timed, and
+// the next one waits for them once and keeps the faster (resident_shape).
 constexpr int kResidentBlock = 256;
 constexpr int kResidentLds = 24;
-constexpr uint32_t kResidentGroupsPerCu = 4;
+constexpr uint32_t kResidentGroupsPerCu = 4;  // the most groups; spill rows are sized for it
+constexpr uint32_t kOccTrials = 4;  // two pairs
+static_assert(kOccTrials % 2 == 0, "trials run in pairs");
 // The grid-shaped kernels' stack: 128-lane groups, 32 LDS entries.
 constexpr int kGridBlock = 128;
 constexpr int kGridLds = 32;
@@ -159,6 +166,13 @@ struct PathTracer {
     hipGraphExec_t graph = nullptr;
     uint32_t graph_frames = 0;
     GraphKey graph_key{};
+    // the resident grid's groups per CU once both shapes have been timed
+    // (0 before), the plain launches seen while it was open, the end of the
+    // warm-up launch and each trial's start / end events, the trials' paths
+    uint32_t occ = 0;
+    uint32_t occ_launches = 0;
+    hipEvent_t occ_ev[1 + 2 * kOccTrials] = {};
+    uint64_t occ_paths[kOccTrials] = {};
     ShadeArgs shade_args{};       // every field but the slot's buffers
     int32_t host_samples = 0;
     uint32_t rays = 0;            // paths per frame: pixels, times the samples per frame in realtime mode

@@ -780,9 +780,10 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 // unchanged.  With CACHE (radiance cache, one frame per launch) the
 // training paths listed by k_train_list come first, recording every vertex,
 // then every other pixel in tile order, ending at its second vertex with a
-// cache query as k_shade writes them.
-template <bool ALPHA, bool STATS, bool CACHE>
-__global__ __launch_bounds__(kFrameBlock, 4) void k_frame_persistent(ShadeArgs A, FrameGen G, int32_t* __restrict__ spill,
+// cache query as k_shade writes them.  OCC: resident groups per CU (waves
+// per SIMD) the registers are allocated for.
+template <bool ALPHA, bool STATS, bool CACHE, int OCC>
+__global__ __launch_bounds__(kFrameBlock, OCC) void k_frame_persistent(ShadeArgs A, FrameGen G, int32_t* __restrict__ spill,
                                                                   uint32_t cap, uint32_t* __restrict__ flags,
                                                                   unsigned long long* __restrict__ stats) {
     __shared__ int32_t lds[kFrameLds * kFrameBlock];
@@ -1107,9 +1108,64 @@ ShadeArgs slot_args(const PathTracer& P, uint32_t k) {
     return A;
 }
 
+// Groups per CU for a plain resident launch.  While the shape is open, the
+// first launch runs four groups untimed (it warms the caches and code); the
+// next kOccTrials launches are trials in pairs, four groups then three, each
+// starting after the previous launch has ended (frames in flight would
+// otherwise overlap them) and timed by events t0 / t1 around the kernel.
+// The launch after the trials waits for the last one to end and keeps three
+// groups only if they took less time over the pairs whose two launches
+// traced the same number of paths (launch tails differ with launch size).
+// Launches inside a stream capture are never trials and take four groups
+// while the shape is open.
+uint32_t resident_shape(PathTracer& P, const FrameGen& G, hipStream_t s, hipEvent_t& t0, hipEvent_t& t1) {
+    t0 = t1 = nullptr;
+    if (P.occ) return P.occ;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return kResidentGroupsPerCu;
+    const uint32_t n = P.occ_launches;
+    if (n == 1 + kOccTrials) {
+        P.occ = kResidentGroupsPerCu;
+        if (hipEventSynchronize(P.occ_ev[2 * kOccTrials]) != hipSuccess) return P.occ;
+        double t4 = 0.0, t3 = 0.0;
+        for (uint32_t p = 0; p < kOccTrials / 2; p++) {
+            const uint32_t a = 2 * p, b = 2 * p + 1;  // four groups, three groups
+            float ma = 0.0f, mb = 0.0f;
+            if (hipEventElapsedTime(&ma, P.occ_ev[1 + 2 * a], P.occ_ev[2 + 2 * a]) != hipSuccess ||
+                hipEventElapsedTime(&mb, P.occ_ev[1 + 2 * b], P.occ_ev[2 + 2 * b]) != hipSuccess)
+                return P.occ;
+            if (P.occ_paths[a] != P.occ_paths[b]) continue;
+            t4 += ma;
+            t3 += mb;
+        }
+        if (t3 > 0.0 && t3 < t4) P.occ = 3;
+        return P.occ;
+    }
+    for (hipEvent_t& e : P.occ_ev)
+        if (!e && hipEventCreate(&e) != hipSuccess) {
+            P.occ = kResidentGroupsPerCu;
+            return P.occ;
+        }
+    P.occ_launches++;
+    if (n == 0) {  // the warm-up launch: its end starts the first trial
+        t1 = P.occ_ev[0];
+        return kResidentGroupsPerCu;
+    }
+    const uint32_t i = n - 1;
+    if (hipStreamWaitEvent(s, P.occ_ev[2 * i], 0) != hipSuccess) {  // the previous launch's end
+        P.occ = kResidentGroupsPerCu;
+        return P.occ;
+    }
+    t0 = P.occ_ev[1 + 2 * i];
+    t1 = P.occ_ev[2 + 2 * i];
+    P.occ_paths[i] = uint64_t(G.rays) * G.frames;
+    return (i & 1) ? 3u : 4u;
+}
+
 atlas_rt_status launch_resident(PathTracer& P, uint32_t k, const ShadeArgs& A, const FrameGen& G, hipStream_t s) {
     FrameSlot& S = P.slot[k];
-    const uint32_t groups = uint32_t(device_cu_count()) * kResidentGroupsPerCu;
+    const uint32_t cus = uint32_t(device_cu_count());
+    const uint32_t groups = cus * kResidentGroupsPerCu;
     int32_t* spill = reinterpret_cast<int32_t*>(S.spill.ptr);
     uint32_t* flags = reinterpret_cast<uint32_t*>(S.flags.ptr);
     const bool cache = A.cache.enabled;
@@ -1117,20 +1173,33 @@ atlas_rt_status launch_resident(PathTracer& P, uint32_t k, const ShadeArgs& A, c
     // is not counted
     if (P.stats && (P.alpha || cache)) P.stats_partial = true;
     if (cache && P.alpha)
-        hipLaunchKernelGGL((k_frame_persistent<true, false, true>), dim3(groups), dim3(kFrameBlock), 0, s, A, G, spill,
-                           P.cap, flags, nullptr);
+        hipLaunchKernelGGL((k_frame_persistent<true, false, true, 4>), dim3(groups), dim3(kFrameBlock), 0, s, A, G,
+                           spill, P.cap, flags, nullptr);
     else if (cache)
-        hipLaunchKernelGGL((k_frame_persistent<false, false, true>), dim3(groups), dim3(kFrameBlock), 0, s, A, G, spill,
-                           P.cap, flags, nullptr);
-    else if (P.alpha)
-        hipLaunchKernelGGL((k_frame_persistent<true, false, false>), dim3(groups), dim3(kFrameBlock), 0, s, A, G,
+        hipLaunchKernelGGL((k_frame_persistent<false, false, true, 4>), dim3(groups), dim3(kFrameBlock), 0, s, A, G,
                            spill, P.cap, flags, nullptr);
-    else if (P.stats)
-        hipLaunchKernelGGL((k_frame_persistent<false, true, false>), dim3(groups), dim3(kFrameBlock), 0, s, A, G,
+    else if (P.stats && !P.alpha)
+        hipLaunchKernelGGL((k_frame_persistent<false, true, false, 4>), dim3(groups), dim3(kFrameBlock), 0, s, A, G,
                            spill, P.cap, flags, P.stats);
-    else
-        hipLaunchKernelGGL((k_frame_persistent<false, false, false>), dim3(groups), dim3(kFrameBlock), 0, s, A, G,
-                           spill, P.cap, flags, nullptr);
+    else {
+        hipEvent_t t0, t1;
+        const uint32_t occ = resident_shape(P, G, s, t0, t1);
+        if (t0) ATLAS_HIP_TRY(hipEventRecord(t0, s));
+        const dim3 grid(cus * occ);
+        if (P.alpha && occ == 3)
+            hipLaunchKernelGGL((k_frame_persistent<true, false, false, 3>), grid, dim3(kFrameBlock), 0, s, A, G, spill,
+                               P.cap, flags, nullptr);
+        else if (P.alpha)
+            hipLaunchKernelGGL((k_frame_persistent<true, false, false, 4>), grid, dim3(kFrameBlock), 0, s, A, G, spill,
+                               P.cap, flags, nullptr);
+        else if (occ == 3)
+            hipLaunchKernelGGL((k_frame_persistent<false, false, false, 3>), grid, dim3(kFrameBlock), 0, s, A, G,
+                               spill, P.cap, flags, nullptr);
+        else
+            hipLaunchKernelGGL((k_frame_persistent<false, false, false, 4>), grid, dim3(kFrameBlock), 0, s, A, G,
+                               spill, P.cap, flags, nullptr);
+        if (t1) ATLAS_HIP_TRY(hipEventRecord(t1, s));
+    }
     ATLAS_HIP_TRY(hipGetLastError());
     return ATLAS_RT_OK;
 }

@@ -275,7 +275,7 @@ def cpu_configs(extra, threads):
 PROBES = """probes:
   pt plain|cache [stride steps [lag]]
                   the dense radiance-cache scene (256 blob instances in a
-                  30-unit cube) at 1280 x 720, 8 bounces: 4 untimed frames,
+                  30-unit cube) at 1280 x 720, 8 bounces: 6 untimed frames,
                   then 8 frames, plain or with the cache (1/32 training
                   pixels, 2 Adam steps per frame, query lag 0, unless given)
   nrc             cache training steps at 32 .. 262,144 samples and
@@ -288,13 +288,13 @@ PROBES = """probes:
   lbvh            three GPU linear-BVH builds of the c3 meshes (512 meshes,
                   ~7.9M triangles)
   c2 [binned]     the config-2 path trace (256 instances, 1280 x 720,
-                  4 bounces): 2 untimed frames, then 16, optionally with
+                  4 bounces): 6 untimed frames, then 16, optionally with
                   ray binning
   (pt and c2 take fif=N anywhere after the command: N progressive frames
   in flight instead of the library's default; pt, c2 and realtime take
   `wavefront`: bounce-by-bounce launches instead of the resident frame kernel)
   realtime        the same scene in realtime mode (4 samples per pixel,
-                  G-buffer, temporal resolve): 2 untimed frames, then 16
+                  G-buffer, temporal resolve): 6 untimed frames, then 16
   summarize-pmc DIR   per-dispatch counter averages of a --pmc run's CSVs"""
 
 COPY_BYTES = 256 << 20
@@ -323,7 +323,8 @@ def probe_pt(mode, stride=32, steps=2, lag=0, fif=None, wavefront=False):
         rc = RadianceCache((-20, -20, -20), (20, 20, 20), levels=12, log2_table=19, base_resolution=16,
                            max_resolution=2048)
         p.set_cache(rc, train_stride=stride, train_steps=steps, query_lag=lag)
-    p.render(cam, 4)
+    for _ in range(6):  # past the resident grid-shape trials
+        p.render(cam, 1)
     torch.cuda.synchronize()
     t = time.perf_counter()
     p.render(cam, 8)
@@ -346,7 +347,8 @@ def probe_c2(binning, fif=None, wavefront=False):
     cam = camera(cc["eye"], cc["yaw_deg"], cc["pitch_deg"], cc["fov_y_deg"], cc["width"] / cc["height"])
     p = PathTracer(w, cc["width"], cc["height"], 4, W.c2_materials(mats_d), W.sun_light(), env=W.C2_ENV,
                    binning=binning, frames_in_flight=fif, wavefront=wavefront)
-    p.render(cam, 2)
+    for _ in range(6):  # past the resident grid-shape trials
+        p.render(cam, 1)
     torch.cuda.synchronize()
     t = time.perf_counter()
     p.render(cam, 16)
@@ -383,7 +385,7 @@ def probe_realtime(wavefront=False):
     view = view_matrix(cc["eye"], cc["yaw_deg"], cc["pitch_deg"])
     proj = projection_matrix(cc["fov_y_deg"], cc["width"] / cc["height"])
     fm = frame_matrices(view, proj, proj @ view)
-    for _ in range(2):
+    for _ in range(6):
         p.render_realtime(cam, fm)
     torch.cuda.synchronize()
     t = time.perf_counter()

@@ -76,6 +76,28 @@ def test_resident_batches_equal_wavefront(torch_cuda):
     assert a.tobytes() == b.tobytes()
 
 
+@pytest.mark.parametrize("fif", [1, 3])
+def test_resident_grid_shape_trials_keep_the_image(torch_cuda, fif):
+    """After an untimed first launch, four plain launches alternate four and
+    three resident groups per CU (timed, each after the previous one has
+    ended); the sixth waits for them and every later launch takes the faster
+    shape: nine one-frame calls give the bounce-by-bounce image bit for bit."""
+    print(f"case resident grid shape trials fif={fif}", flush=True)
+    w_, blases, mats = _plain(salt=4)
+    W_, H_ = 96, 64
+    cam = atlas_rt.camera((0.0, 15.0, -70.0), 0.0, 12.0, 50.0, W_ / H_)
+
+    def render(pt):
+        for _ in range(9):
+            pt.render(cam, 1)
+
+    a, b = _both(lambda wf: atlas_rt.PathTracer(w_, W_, H_, 4, mats, W.sun_light(), env=ENV, frames_in_flight=fif,
+                                                wavefront=wf),
+                 render, lambda pt: pt.read().cpu().numpy())
+    assert np.all(a[..., 3] == 9) and a[..., :3].max() > 0
+    assert a.tobytes() == b.tobytes()
+
+
 def test_resident_equals_wavefront_ragged_and_deep(torch_cuda):
     """A frame size that is not a multiple of the 8x8 tiles (paths in row
     order) and eight bounces."""
