@@ -729,7 +729,11 @@ atlas_rt_status atlas_rt_pathtracer_destroy(atlas_rt_pathtracer* pt);
  * Frames run on the path tracer's own streams (ATLAS_RT_PT_FRAMES_IN_FLIGHT),
  * after the work already queued on `stream` and after the frames of earlier
  * calls, and `stream` waits for them: work queued on `stream` afterwards
- * sees the image. */
+ * sees the image.  The resident frame kernel's grid shape (three or four
+ * groups per CU) is chosen by timing: of a path tracer's first six plain
+ * resident launches, launches two to five alternate the shapes, each after
+ * the previous launch has ended, and the sixth waits for them once on the
+ * host; the image does not depend on the shape. */
 atlas_rt_status atlas_rt_pathtracer_render(atlas_rt_pathtracer* pt, const atlas_rt_camera* camera, uint32_t frames,
                                            void* stream);
 
