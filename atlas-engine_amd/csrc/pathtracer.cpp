@@ -26,17 +26,27 @@ atlas_rt_status dev_alloc(T*& p, size_t count) {
 // binning buffers.
 atlas_rt_status alloc_slot_rays(PathTracer& P, uint32_t k, uint32_t rays, uint32_t frames = 1) {
     FrameSlot& S = P.slot[k];
-    void* old[] = {S.paths[0], S.paths[1], S.hits, S.shadow, S.Lpix, S.binned};
+    void* old[] = {S.paths[0], S.paths[1], S.hits, S.shadow, S.Lpix, S.binned, S.order, S.order_cnt, S.order_cls};
     for (void* p : old)
         if (p) (void)hipFree(p);
     S.paths[0] = S.paths[1] = nullptr;
     S.hits = nullptr;
     S.shadow = S.Lpix = S.binned = nullptr;
+    S.order = S.order_cnt = nullptr;
+    S.order_cls = nullptr;
+    S.order_cap = 0;
+    const size_t paths = size_t(rays) * frames;
+    const size_t span = (paths + kPullRanges - 1) / kPullRanges;
+    const size_t order_blocks = (span + kOrderSpan - 1) / kOrderSpan;
     atlas_rt_status st = dev_alloc(S.paths[0], 3 * size_t(rays));
     if (st == ATLAS_RT_OK) st = dev_alloc(S.paths[1], 3 * size_t(rays));
     if (st == ATLAS_RT_OK) st = dev_alloc(S.hits, rays);
     if (st == ATLAS_RT_OK) st = dev_alloc(S.shadow, 3 * size_t(rays));
     if (st == ATLAS_RT_OK) st = dev_alloc(S.Lpix, size_t(rays) * frames);
+    if (st == ATLAS_RT_OK) st = dev_alloc(S.order, paths);
+    if (st == ATLAS_RT_OK) st = dev_alloc(S.order_cnt, kPullRanges * (order_blocks + 1));
+    if (st == ATLAS_RT_OK) st = dev_alloc(S.order_cls, paths);
+    if (st == ATLAS_RT_OK) S.order_cap = paths;
     if (st == ATLAS_RT_OK) st = S.spill.ensure(slot_spill_bytes(rays, P.cap));
     if (st == ATLAS_RT_OK && P.binning) st = dev_alloc(S.binned, 3 * size_t(rays));
     if (st == ATLAS_RT_OK && P.binning) st = S.bin_ws.ensure(bin_sort_workspace(rays));
@@ -68,7 +78,8 @@ void free_slot_cache(FrameSlot& S) {
 }
 
 void free_slot(FrameSlot& S) {
-    void* ptrs[] = {S.ctl, S.paths[0], S.paths[1], S.hits, S.shadow, S.Lpix, S.binned};
+    void* ptrs[] = {S.ctl, S.paths[0], S.paths[1], S.hits, S.shadow, S.Lpix, S.binned, S.order, S.order_cnt,
+                    S.order_cls};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     S.spill.release();
@@ -101,7 +112,7 @@ void free_realtime(PathTracer& P) {
 void free_all(PathTracer& P) {
     void* ptrs[] = {P.ctl, P.accum, const_cast<float4**>(P.shade_table), const_cast<float4**>(P.uv_table),
                     const_cast<float4**>(P.tangent_table), const_cast<float4**>(P.color_table), P.materials,
-                    P.lights, P.tex, P.texels, P.stats};
+                    P.lights, P.tex, P.texels, P.stats, P.hit0};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     free_cache_buffers(P);
@@ -431,6 +442,7 @@ atlas_rt_status atlas_rt_pathtracer_create(const atlas_rt_world* world, const at
     if (st == ATLAS_RT_OK && hipEventCreateWithFlags(&P.ev_start, hipEventDisableTiming) != hipSuccess)
         st = fail(ATLAS_RT_E_HIP, "atlas_rt_pathtracer_create: hipEventCreateWithFlags failed");
     if (st == ATLAS_RT_OK) st = dev_alloc(P.accum, pixels);
+    if (st == ATLAS_RT_OK) st = dev_alloc(P.hit0, pixels);
     if (st == ATLAS_RT_OK) st = dev_alloc(P.materials, material_count);
     if (st != ATLAS_RT_OK) {
         free_all(P);
@@ -442,6 +454,7 @@ atlas_rt_status atlas_rt_pathtracer_create(const atlas_rt_world* world, const at
     hipError_t e = hipSuccess;
     e = hipMemcpy(P.ctl, &ctl, sizeof(ctl), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(P.accum, 0, pixels * sizeof(float4));
+    if (e == hipSuccess) e = hipMemset(P.hit0, 0, pixels);
     if (e == hipSuccess) e = hipMemcpy(P.materials, materials, material_count * sizeof(Material), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         free_all(P);

@@ -48,6 +48,12 @@ struct FrameSlot {
     Scratch flags;
     float4* binned = nullptr;
     Scratch bin_ws;
+    // resident launches: path numbers in pull order and the order's block
+    // counts (k_order_* in pt_kernels.hip), sized for rays x frames paths
+    uint32_t* order = nullptr;
+    uint32_t* order_cnt = nullptr;
+    uint8_t* order_cls = nullptr;
+    size_t order_cap = 0;  // paths
     // radiance cache
     float4* rec = nullptr;        // training records: 5 float4 each (query x3, beta, added radiance)
     uint32_t* train_list = nullptr;  // resident frames: training pixels
@@ -85,6 +91,8 @@ constexpr int kResidentBlock = 256;
 constexpr int kResidentLds = 24;
 constexpr uint32_t kResidentGroupsPerCu = 4;  // the most groups; spill rows are sized for it
 constexpr uint32_t kOccTrials = 4;  // two pairs
+// Path numbers per block of the path-order kernels (k_order_*).
+constexpr uint32_t kOrderSpan = 2048;
 static_assert(kOccTrials % 2 == 0, "trials run in pairs");
 // The grid-shaped kernels' stack: 128-lane groups, 32 LDS entries.
 constexpr int kGridBlock = 128;
@@ -171,6 +179,9 @@ struct PathTracer {
     // warm-up launch and each trial's start / end events, the trials' paths
     uint32_t occ = 0;
     uint32_t occ_launches = 0;
+    // per pixel: whether the camera ray hit something, as the last resident
+    // launch saw it; orders the next launch's paths (width x height bytes)
+    uint8_t* hit0 = nullptr;
     hipEvent_t occ_ev[1 + 2 * kOccTrials] = {};
     uint64_t occ_paths[kOccTrials] = {};
     ShadeArgs shade_args{};       // every field but the slot's buffers

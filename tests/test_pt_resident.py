@@ -98,6 +98,28 @@ def test_resident_grid_shape_trials_keep_the_image(torch_cuda, fif):
     assert a.tobytes() == b.tobytes()
 
 
+def test_resident_path_order_with_frames_in_flight(torch_cuda):
+    """Batches of frames pull the paths of last launch's hit pixels first
+    (a per-range stable partition built before each launch).  Three slots in
+    flight write the shared hit map while another slot builds its order;
+    the order must stay a permutation, so six 4-frame calls give the
+    bounce-by-bounce image bit for bit (ragged size: paths in row order)."""
+    print("case resident path order fif=3", flush=True)
+    w_, blases, mats = _plain(salt=9)
+    W_, H_ = 90, 61
+    cam = atlas_rt.camera((0.0, 15.0, -70.0), 0.0, 12.0, 50.0, W_ / H_)
+
+    def render(pt):
+        for _ in range(6):
+            pt.render(cam, 4)
+
+    a, b = _both(lambda wf: atlas_rt.PathTracer(w_, W_, H_, 4, mats, W.sun_light(), env=ENV, frames_in_flight=3,
+                                                wavefront=wf),
+                 render, lambda pt: pt.read().cpu().numpy())
+    assert np.all(a[..., 3] == 24) and a[..., :3].max() > 0
+    assert a.tobytes() == b.tobytes()
+
+
 def test_resident_equals_wavefront_ragged_and_deep(torch_cuda):
     """A frame size that is not a multiple of the 8x8 tiles (paths in row
     order) and eight bounces."""
