@@ -14,8 +14,11 @@
 #include <atomic>
 #include <condition_variable>
 #include <cstring>
+#include <exception>
 #include <functional>
 #include <mutex>
+#include <new>
+#include <string>
 #include <thread>
 
 #include "lbvh.h"
@@ -43,6 +46,17 @@ struct DeviceArena {
     }
 };
 
+// A temporary device allocation of the build, freed on every way out
+// (hipFree waits for the work that uses it).
+struct DevTemp {
+    void* p = nullptr;
+    ~DevTemp() { reset(); }
+    void reset() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+    }
+};
+
 void free_partial(std::vector<atlas_rt_blas*>& made) {
     for (atlas_rt_blas* b : made) atlas_rt_blas_destroy(b);
     made.clear();
@@ -50,21 +64,20 @@ void free_partial(std::vector<atlas_rt_blas*>& made) {
 
 // A fixed team of host threads that runs one parallel loop after another:
 // run(n, fn) calls fn(i) for every i in [0, n), the calling thread taking
-// part, and returns when all calls have returned.  The team starts its
-// threads once, so the many short loops of a staged copy cost a wake-up each.
+// part, and returns when all calls have returned, rethrowing the first
+// exception a call threw.  The team starts its threads once, so the many
+// short loops of a staged copy cost a wake-up each.
 class Team {
   public:
     explicit Team(int threads) {
-        for (int t = 1; t < threads; t++) pool_.emplace_back([this] { work(); });
-    }
-    ~Team() {
-        {
-            std::lock_guard<std::mutex> g(mu_);
-            quit_ = true;
+        try {
+            for (int t = 1; t < threads; t++) pool_.emplace_back([this] { work(); });
+        } catch (...) {
+            stop();  // the threads already started
+            throw;
         }
-        wake_.notify_all();
-        for (auto& th : pool_) th.join();
     }
+    ~Team() { stop(); }
     template <class F>
     void run(uint32_t n, F&& fn) {
         if (n == 0) return;
@@ -82,13 +95,28 @@ class Team {
             gen_++;
         }
         wake_.notify_all();
-        for (uint32_t i; (i = next_.fetch_add(1)) < n;) job(i);
+        drain(job, n);
         std::unique_lock<std::mutex> g(mu_);
         idle_.wait(g, [this] { return busy_ == 0; });
         job_ = nullptr;
+        if (err_) {
+            std::exception_ptr e = err_;
+            err_ = nullptr;
+            g.unlock();
+            std::rethrow_exception(e);
+        }
     }
 
   private:
+    void stop() {
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            quit_ = true;
+        }
+        wake_.notify_all();
+        for (auto& th : pool_) th.join();
+        pool_.clear();
+    }
     void work() {
         uint64_t seen = 0;
         for (;;) {
@@ -102,9 +130,22 @@ class Team {
                 job = job_;
                 n = n_;
             }
-            for (uint32_t i; (i = next_.fetch_add(1)) < n;) (*job)(i);
+            drain(*job, n);
             std::lock_guard<std::mutex> g(mu_);
             if (--busy_ == 0) idle_.notify_one();
+        }
+    }
+    // Takes indices until none is left; after a throw, takes the rest
+    // without calling the job.
+    void drain(std::function<void(uint32_t)>& job, uint32_t n) {
+        for (uint32_t i; (i = next_.fetch_add(1)) < n;) {
+            try {
+                job(i);
+            } catch (...) {
+                std::lock_guard<std::mutex> g(mu_);
+                if (!err_) err_ = std::current_exception();
+                next_.store(n);
+            }
         }
     }
     std::vector<std::thread> pool_;
@@ -114,6 +155,7 @@ class Team {
     uint32_t n_ = 0, busy_ = 0;
     uint64_t gen_ = 0;
     std::atomic<uint32_t> next_{0};
+    std::exception_ptr err_;
     bool quit_ = false;
 };
 
@@ -263,12 +305,10 @@ atlas_rt_status build_batch(const float* const* verts, const uint32_t* counts, u
     }
     auto arena = std::make_shared<DeviceArena>();
     hipError_t e = hipMalloc(&arena->build, std::max<size_t>(at, 1));
-    uint32_t* orig = nullptr;
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&orig), total * sizeof(uint32_t));
-    if (e != hipSuccess) {
-        if (orig) (void)hipFree(orig);
-        return hip_fail(e, "atlas_rt_blas_build_device (outputs)");
-    }
+    DevTemp orig_mem;
+    if (e == hipSuccess) e = hipMalloc(&orig_mem.p, total * sizeof(uint32_t));
+    if (e != hipSuccess) return hip_fail(e, "atlas_rt_blas_build_device (outputs)");
+    uint32_t* const orig = static_cast<uint32_t*>(orig_mem.p);
     char* base = static_cast<char*>(arena->build);
     std::vector<LbvhMeshOut> o(n);
     const size_t first = made.size();
@@ -286,37 +326,42 @@ atlas_rt_status build_batch(const float* const* verts, const uint32_t* counts, u
     // the caller's triangles, meshes one after another, up through pinned staging
     Team team(default_threads());
     Staging stage(s);
-    float* d_verts = nullptr;
+    DevTemp verts_mem;
     std::vector<Piece> tri_in(n);
     for (uint32_t m = 0; m < n; m++)  // read only: up() copies from the pieces
         tri_in[m] = Piece{9 * off_o[m] * sizeof(float), 9 * size_t(c[m]) * sizeof(float),
                           reinterpret_cast<char*>(const_cast<float*>(v[m]))};
     e = stage.init();
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&d_verts), std::max<size_t>(9 * total, 1) * sizeof(float));
-    if (e == hipSuccess) e = stage.up(team, reinterpret_cast<char*>(d_verts), 9 * total * sizeof(float), tri_in);
-    if (e != hipSuccess) {
-        if (d_verts) (void)hipFree(d_verts);
-        (void)hipFree(orig);
-        return hip_fail(e, "atlas_rt_blas_build_device (triangle upload)");
-    }
+    if (e == hipSuccess) e = hipMalloc(&verts_mem.p, std::max<size_t>(9 * total, 1) * sizeof(float));
+    if (e == hipSuccess) e = stage.up(team, static_cast<char*>(verts_mem.p), 9 * total * sizeof(float), tri_in);
+    if (e != hipSuccess) return hip_fail(e, "atlas_rt_blas_build_device (triangle upload)");
     // the host mirrors are sized (and their pages touched) by the team while
     // this thread waits for the GPU build
+    std::exception_ptr sizing_err;
     std::thread sizing([&] {
-        team.run(n, [&](uint32_t m) {
-            atlas_rt_blas* b = made[first + m];
-            b->nodes.resize(size_t(c[m]) - 1);
-            b->tris.resize(c[m]);
-            b->orig_idx.resize(c[m]);
-        });
+        try {
+            team.run(n, [&](uint32_t m) {
+                atlas_rt_blas* b = made[first + m];
+                b->nodes.resize(size_t(c[m]) - 1);
+                b->tris.resize(c[m]);
+                b->orig_idx.resize(c[m]);
+            });
+        } catch (...) {
+            sizing_err = std::current_exception();
+        }
     });
     std::vector<uint32_t> depth;
-    atlas_rt_status st = lbvh_build(d_verts, c, o, depth, s, ms);
-    sizing.join();
-    (void)hipFree(d_verts);
-    if (st != ATLAS_RT_OK) {
-        (void)hipFree(orig);
-        return st;
+    atlas_rt_status st;
+    try {
+        st = lbvh_build(static_cast<const float*>(verts_mem.p), c, o, depth, s, ms);
+    } catch (...) {
+        sizing.join();
+        throw;
     }
+    sizing.join();
+    if (sizing_err) std::rethrow_exception(sizing_err);
+    verts_mem.reset();
+    if (st != ATLAS_RT_OK) return st;
     for (uint32_t m = 0; m < n; m++) made[first + m]->max_depth = depth[m];
     // host mirrors, read back through the staging straight into each BLAS's
     // vectors: node and vertex records from the build arena, the caller's
@@ -339,7 +384,7 @@ atlas_rt_status build_batch(const float* const* verts, const uint32_t* counts, u
     e = stage.down(team, base, tris_begin, recs);
     if (e == hipSuccess) e = stage.down(team, reinterpret_cast<const char*>(orig), total * sizeof(uint32_t), idx);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
-    (void)hipFree(orig);
+    orig_mem.reset();
     if (e != hipSuccess) return hip_fail(e, "atlas_rt_blas_build_device (host mirror)");
     // the wide collapse, one mesh per host thread
     std::vector<WideTree> w4(n), w8(n);
@@ -400,8 +445,14 @@ atlas_rt_status atlas_rt_blas_build_device(const float* const* verts, const uint
     for (uint32_t first = 0; first < count; first += 1024) {
         const uint32_t n = std::min<uint32_t>(1024, count - first);
         float ms = 0.0f;
-        const atlas_rt_status st = build_batch(verts + first, counts + first, n, dev, s, made,
-                                               device_ms ? &ms : nullptr);
+        atlas_rt_status st;
+        try {
+            st = build_batch(verts + first, counts + first, n, dev, s, made, device_ms ? &ms : nullptr);
+        } catch (const std::bad_alloc&) {
+            st = fail(ATLAS_RT_E_OUT_OF_MEMORY, "atlas_rt_blas_build_device: host allocation failed");
+        } catch (const std::exception& ex) {  // e.g. no host thread could be started
+            st = fail(ATLAS_RT_E_OUT_OF_MEMORY, std::string("atlas_rt_blas_build_device: ") + ex.what());
+        }
         if (st != ATLAS_RT_OK) {
             free_partial(made);
             return st;
