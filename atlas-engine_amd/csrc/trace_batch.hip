@@ -54,9 +54,16 @@ __global__ __launch_bounds__(BLOCK) void k_batch_persistent(BatchDev D, const ui
     bool exhausted = false;
     uint32_t ray = 0;
     float tmax = 0.0f;
+    bool pending = false;  // the lane's ray has ended; its hit record is not written yet
     RangePull rp;  // work items from kPullRanges ranges, counters ctl[kRangeWord * (r + 1)]
     rp.init(count);
     for (;;) {
+        // hit records of the rays ended since the last refill, all at once
+        // (the FP64 refinement runs once per refill round)
+        if (pending) {
+            hits[ray] = lane_hit_record(B, L, tmax);
+            pending = false;
+        }
         if (!exhausted) {
             const uint64_t idle = __ballot(!active);
             if (idle) {
@@ -99,7 +106,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_persistent(BatchDev D, const ui
                     atomicOr(&ctl[2], 1u);
                     L.slot = -1;
                 }
-                hits[ray] = lane_hit_record(B, L, tmax);
+                pending = true;
                 active = false;
             }
             if (uint32_t(__popcll(__ballot(active))) < threshold) break;

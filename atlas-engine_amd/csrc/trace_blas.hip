@@ -75,9 +75,19 @@ __global__ __launch_bounds__(BLOCK) void k_blas_closest_persistent(BlasDev B, co
     bool exhausted = false;
     uint32_t idx = 0;
     float tmax = 0.0f;
+    bool pending = false;  // the lane's ray has ended; its hit record is not written yet
     RangePull rp;  // rays from kPullRanges ranges, counters ctl[kRangeWord * (r + 1)]
     rp.init(count);
     for (;;) {
+        // The hit records of the rays that ended since the last refill, all
+        // at once: the FP64 refinement (refine_tuv) then runs once per refill
+        // round instead of in every traversal iteration in which some lane
+        // of the wave ended its ray.
+        if (pending) {
+            hits[idx] = lane_hit_record(B, L, tmax);
+            if (slots) slots[idx] = L.slot;
+            pending = false;
+        }
         if (!exhausted) {
             const uint64_t idle = __ballot(!active);
             if (idle) {
@@ -117,8 +127,7 @@ __global__ __launch_bounds__(BLOCK) void k_blas_closest_persistent(BlasDev B, co
                     atomicOr(&ctl[2], 1u);
                     L.slot = -1;
                 }
-                hits[idx] = lane_hit_record(B, L, tmax);
-                if (slots) slots[idx] = L.slot;
+                pending = true;
                 active = false;
             }
             if (uint32_t(__popcll(__ballot(active))) < threshold) break;
