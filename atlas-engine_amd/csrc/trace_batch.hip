@@ -55,6 +55,12 @@ __global__ __launch_bounds__(BLOCK) void k_batch_persistent(BatchDev D, const ui
     uint32_t ray = 0;
     float tmax = 0.0f;
     bool pending = false;  // the lane's ray has ended; its hit record is not written yet
+    // The lane's current batch: work items [cur_lo, cur_hi) are its rays from
+    // cur_ray0, traced against B.  A wave pulls consecutive work items, so a
+    // new ray is nearly always in the batch of the lane's last one, and the
+    // binary search over the batch table (a chain of dependent loads) and the
+    // BLAS record load are skipped.
+    uint32_t cur_lo = 1, cur_hi = 0, cur_ray0 = 0;
     RangePull rp;  // work items from kPullRanges ranges, counters ctl[kRangeWord * (r + 1)]
     rp.init(count);
     for (;;) {
@@ -77,10 +83,15 @@ __global__ __launch_bounds__(BLOCK) void k_batch_persistent(BatchDev D, const ui
                 if (!active) {
                     const uint32_t w = base + __popcll(idle & ((1ull << lane) - 1));
                     if (w < end) {
-                        const uint32_t b = find_batch(D.prefix, D.batch_count, w);
-                        const uint4 desc = D.batches[b];
-                        ray = desc.y + (w - D.prefix[b]);
-                        B = ld_blas(D.blases + desc.x);
+                        if (w < cur_lo || w >= cur_hi) {
+                            const uint32_t b = find_batch(D.prefix, D.batch_count, w);
+                            const uint4 desc = D.batches[b];
+                            cur_lo = D.prefix[b];
+                            cur_hi = D.prefix[b + 1];
+                            cur_ray0 = desc.y;
+                            B = ld_blas(D.blases + desc.x);
+                        }
+                        ray = cur_ray0 + (w - cur_lo);
                         const float4 r0 = rays[2 * size_t(ray)];
                         const float4 r1 = rays[2 * size_t(ray) + 1];
                         tmax = r1.w;
