@@ -486,7 +486,7 @@ __global__ __launch_bounds__(kB) void k_sort_scatter(const uint64_t* __restrict_
             out[before + rank] = key;
         }
         __syncthreads();
-        // advance the running offsets by this round's counts
+        // advance the running offsets by this pass's counts
         uint32_t add = 0;
         for (int w = 0; w < kWaves; w++) add += wave_cnt[w][threadIdx.x];
         running[threadIdx.x] += add;

@@ -585,7 +585,7 @@ def main():
     if args.probe:
         return probe(args.probe)
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; multi-GPU sharding is design-only "
+        print(json.dumps({"error": "the benchmark runs on one GPU; multi-GPU sharding is design-only "
                                    "(rays and BLAS batches are independent), run with --gpus 1"}))
         sys.exit(2)
     import torch
