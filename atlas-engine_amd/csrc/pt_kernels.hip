@@ -386,10 +386,10 @@ __device__ __forceinline__ void shade_path(const ShadeArgs& A, const FrameCtl* c
         const int32_t mesh = __float_as_int(meta.x);
         const int32_t mat_off = __float_as_int(meta.y);
         const float4* M = A.xform + 3 * size_t(inst);
-        const float4* vp = A.world.blases[mesh].vtx + 3 * size_t(slot);
+        const float4* vp = ld_blas(A.world.blases + mesh).vtx + 3 * size_t(slot);
         const float4* sp = A.shade[mesh] + 3 * size_t(slot);
-        const float4 a0 = vp[0], a1 = vp[1], a2 = vp[2];
-        const float4 s0 = sp[0], s1 = sp[1], s2 = sp[2];
+        const float4 a0 = ldg4(vp), a1 = ldg4(vp + 1), a2 = ldg4(vp + 2);
+        const float4 s0 = ldg4(sp), s1 = ldg4(sp + 1), s2 = ldg4(sp + 2);
         // TransformTriangle (raytracer/surface.hsh:46-61)
         const V v0 = row_xform(M, mk(a0.x, a0.y, a0.z), 1.0f);
         const V v1 = row_xform(M, mk(a1.x, a1.y, a1.z), 1.0f);
@@ -533,10 +533,10 @@ __device__ __forceinline__ void shade_path(const ShadeArgs& A, const FrameCtl* c
                 const int32_t lslot = int32_t(__float_as_uint(Lt.data[0]) & 0x0FFFFFFFu);
                 const int32_t lmesh = __float_as_int(A.world.instances[4 * size_t(li) + 3].x);
                 const float4* LM = A.xform + 3 * size_t(li);
-                const float4* lv = A.world.blases[lmesh].vtx + 3 * size_t(lslot);
+                const float4* lv = ld_blas(A.world.blases + lmesh).vtx + 3 * size_t(lslot);
                 const float4* ls = A.shade[lmesh] + 3 * size_t(lslot);
-                const float4 b0 = lv[0], b1 = lv[1], b2 = lv[2];
-                const float4 c0 = ls[0], c1 = ls[1], c2 = ls[2];
+                const float4 b0 = ldg4(lv), b1 = ldg4(lv + 1), b2 = ldg4(lv + 2);
+                const float4 c0 = ldg4(ls), c1 = ldg4(ls + 1), c2 = ldg4(ls + 2);
                 // TransformTriangle + SampleTriangleLight (raytracer/lights.hsh:11-31)
                 const V a0 = row_xform(LM, mk(b0.x, b0.y, b0.z), 1.0f);
                 const V a1 = row_xform(LM, mk(b1.x, b1.y, b1.z), 1.0f);
