@@ -360,7 +360,10 @@ atlas_rt_status atlas_rt_blas_build_many(const atlas_rt_bvh_triangle* const* tri
  * tree differs from the SAH/SBVH one (nearest hits are the same, slots and
  * traversal cost differ).  Host copies are mirrored for export.  device_ms
  * (may be NULL) receives the GPU time of the build kernels, uploads excluded.
- * Synchronous on `stream`. */
+ * Synchronous on `stream`.  During the call the library uses up to 16 host
+ * threads and two 16 MiB pinned host buffers (the triangle upload, the host
+ * mirrors' readback and the wide nodes go through them), released on return;
+ * ATLAS_RT_E_OUT_OF_MEMORY if host memory or threads run out. */
 atlas_rt_status atlas_rt_blas_build_device(const float* const* verts, const uint32_t* counts, uint32_t count,
                                            uint32_t flags, void* stream, atlas_rt_blas** out, float* device_ms);
 
